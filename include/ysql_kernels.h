@@ -1,0 +1,330 @@
+/*
+ * ysql_kernels.h — C ABI of the YallaSQL operator hot path for AMD MI355X (gfx950).
+ *
+ * Every entry point replaces one host launcher of the reference's
+ * `YallaSQL::Kernel` namespace (reference `include/kernels/*.hpp`), which the
+ * reference's operators (reference `include/engine/operators/*.hpp`) call with
+ * device pointers and a stream.  The reference launchers are C++ templates over
+ * element type and functor; here they are type-tagged C functions.
+ *
+ * Conventions (all entry points):
+ *   - Data pointers are device pointers unless the parameter says "host".
+ *   - Scalars of the vector ops are 1-element device buffers, as in the
+ *     reference (`bound_value.hpp:21-48`).  Null masks are 1 byte per row
+ *     (1 = NULL), as the reference's NullBitSet (`null_bit_set.hpp:5-31`).
+ *   - Outputs and workspaces are owned by the caller.  Nothing allocates
+ *     device memory inside a call and nothing leaks (the reference's
+ *     `launch_radix_sort` allocated and leaked, `radix_sort_kernel.cu:139-158`).
+ *   - Everything is asynchronous on `stream` (a hipStream_t; NULL = the null
+ *     stream).  Device-side counts (filter count, pair count, consumed_a) are
+ *     written to device memory; the caller decides when to synchronise.
+ *   - Every entry point returns an int status (YSQL_OK = 0, negative = error)
+ *     and never exits the process (the reference's CUDA_CHECK called exit(),
+ *     `utils/macros.hpp:6-22`).
+ *   - Entry points keep no global mutable state: they are reentrant across
+ *     streams.  Calls that share one workspace on two streams at once race.
+ */
+#ifndef YSQL_KERNELS_H
+#define YSQL_KERNELS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void *ysql_stream_t; /* hipStream_t */
+
+/* ---- status codes ---------------------------------------------------- */
+enum {
+  YSQL_OK = 0,
+  YSQL_EINVAL = -1,      /* bad argument (null pointer, unsupported dtype/op, n too large) */
+  YSQL_EWORKSPACE = -2,  /* workspace smaller than the matching *_ws_bytes() */
+  YSQL_EOVERFLOW = -3,   /* capacity exceeded (reported by host-side wrappers) */
+  YSQL_EHIP = -4,        /* a HIP runtime call failed */
+  YSQL_EUNSUPPORTED = -5 /* combination not implemented */
+};
+
+/* ---- data types: DataType of the reference (enums/data_type.hpp:2-7) -- */
+enum {
+  YSQL_INT = 0,      /* int32                                   */
+  YSQL_FLOAT = 1,    /* float32                                 */
+  YSQL_DATETIME = 2, /* int64 (epoch seconds)                   */
+  YSQL_STRING = 3,   /* 256-byte NUL-padded (string_kernel.hpp) */
+  YSQL_BOOL = 4,     /* uint8 mask                              */
+  YSQL_UKEY32 = 16,  /* uint32 order-preserving sort key        */
+  YSQL_UKEY64 = 17   /* uint64 order-preserving sort key        */
+};
+#define YSQL_STRING_BYTES 256
+
+/* ---- comparison ops: comparison_operators_kernel.hpp:25-104 ----------
+ * Note the reference names: LEOperator is '<', GEOperator is '>'.         */
+enum { YSQL_LT = 0, YSQL_LE = 1, YSQL_GT = 2, YSQL_GE = 3, YSQL_EQ = 4, YSQL_NE = 5 };
+/* ---- mask logic: ANDOperator / OROperator ---------------------------- */
+enum { YSQL_AND = 0, YSQL_OR = 1 };
+/* ---- arithmetic: binary_operators_kernel.hpp:27-51 ------------------- */
+enum { YSQL_ADD = 0, YSQL_SUB = 1, YSQL_MUL = 2, YSQL_DIV = 3, YSQL_REM = 4 };
+/* ---- aggregates: AggType, aggregate.hpp:7-9 --------------------------- */
+enum {
+  YSQL_AGG_MAX = 0,
+  YSQL_AGG_MIN = 1,
+  YSQL_AGG_SUM = 2,
+  YSQL_AGG_AVG = 3,
+  YSQL_AGG_COUNT = 4,
+  YSQL_AGG_COUNT_STAR = 5
+};
+
+/* Accumulator of one aggregate, resident in device memory and folded into
+ * across batches (the reference keeps one device accumulator per
+ * AggregateExpression, aggregate.hpp:21-30,201-205).
+ *   INT   MIN/MAX -> i (int32 range), SUM/AVG -> i (int64; the reference's
+ *                    int32 sum wraps, aggregate.hpp:122-124 — divergent on purpose)
+ *   FLOAT MIN/MAX -> f, SUM/AVG -> d (double, starting at exactly 0)
+ *   DATETIME      -> i
+ *   count         -> rows folded that were not NULL (AVG's divisor, COUNT). */
+typedef struct ysql_acc {
+  int64_t i;
+  double d;
+  float f;
+  uint32_t pad;
+  int64_t count;
+} ysql_acc;
+
+/* Column descriptor for the fused multi-column ops (host struct). */
+typedef struct ysql_col {
+  const void *src;
+  const uint8_t *src_null; /* may be NULL: no NULLs (output nulls zeroed) */
+  void *dst;
+  uint8_t *dst_null; /* may be NULL: output null bytes not written */
+  uint32_t elem_bytes; /* 1, 4, 8 or 256 */
+  uint32_t pad;
+} ysql_col;
+#define YSQL_MAX_COLS 8
+
+/* ======================================================================
+ * Sort-key normalisation (A1)
+ * Replaces launch_int_to_uint32 / launch_float_to_uint32
+ * (radix_sort_kernel.hpp:83-89 -> radix_sort_utils.cu:39-53), called from
+ * BoundRefSortExpression::evaluate (bound_ref_sort.hpp:46,50).
+ *   INT      -> uint32  x ^ 0x80000000
+ *   FLOAT    -> uint32  sign ? ~bits : bits ^ 0x80000000   (IEEE total order)
+ *   DATETIME -> uint64  x ^ 2^63  (the reference used raw bits,
+ *                       bound_ref_sort.hpp:53-55: divergent for negatives)
+ * `descending` != 0 additionally complements the key, so an ascending stable
+ * sort of the output yields the descending order with ties in input order. */
+int ysql_sortkey_encode(int dtype, int descending, const void *in, void *out, uint64_t n,
+                        ysql_stream_t stream);
+/* Inverse of ysql_sortkey_encode (uint32_to_int / uint32_to_float,
+ * radix_sort_utils.cu:12-37). */
+int ysql_sortkey_decode(int dtype, int descending, const void *in, void *out, uint64_t n,
+                        ysql_stream_t stream);
+
+/* ======================================================================
+ * LSD radix sort, 8-bit digits (A2-A6)
+ * Replaces launch_radix_sort<OP,T,BlockDim>(T* d_in, uint32_t*& d_idx, N, stream)
+ * (radix_sort_kernel.hpp:22-23 -> radix_sort_kernel.cu:139-178), called from
+ * OrderOperator::sortLocallyBatch (order_operator.hpp:196-265).
+ * Keys are order-preserving unsigned keys (YSQL_UKEY32 / YSQL_UKEY64, the
+ * output of ysql_sortkey_encode).  Stable.  On return keys_inout holds the
+ * keys in sorted order (descending if `descending`, like DescOp) and
+ * perm_out[sorted_pos] = original row.  Only bits [begin_bit, end_bit) are
+ * ranked (pass 0, key_bits for the full key).  n < 2^30.               */
+int ysql_radix_sort_ws_bytes(int key_dtype, uint64_t n, size_t *bytes);
+int ysql_radix_argsort(int key_dtype, int descending, void *keys_inout, uint32_t *perm_out,
+                       uint64_t n, int begin_bit, int end_bit, void *ws, size_t ws_bytes,
+                       ysql_stream_t stream);
+/* Same sort with a caller-supplied uint32 payload (vals_inout) carried along
+ * in place of the row index (used for runs whose payload is a global row id). */
+int ysql_radix_sort_pairs(int key_dtype, int descending, void *keys_inout, uint32_t *vals_inout,
+                          uint64_t n, int begin_bit, int end_bit, void *ws, size_t ws_bytes,
+                          ysql_stream_t stream);
+
+/* ======================================================================
+ * Inclusive scans (A4, A13)
+ * Replace launch_prefix_sum<uint32_t> (prefix_sum.hpp:28-29 -> .cu:158-178)
+ * and launch_prefix_sum_mask (prefix_sum.hpp:26 -> .cu:140-156): out[i] =
+ * in[0] + ... + in[i]; for the mask form out[i] = #nonzero bytes in [0..i]. */
+int ysql_scan_ws_bytes(uint64_t n, size_t *bytes);
+int ysql_scan_u32_inclusive(const uint32_t *in, uint32_t *out, uint64_t n, void *ws,
+                            size_t ws_bytes, ysql_stream_t stream);
+int ysql_scan_mask_inclusive(const uint8_t *mask, uint32_t *out, uint64_t n, void *ws,
+                             size_t ws_bytes, ysql_stream_t stream);
+
+/* ======================================================================
+ * Merge path (A8, A9)
+ * Replaces launch_merge_sorted_array_kernel<T,OP>(A,B,C,new_idx,table_idx,
+ * lasti,k_mx,m,n,stream) (merge_batchs.hpp:50-51 -> merge_batchs.cu:143-158),
+ * called from OrderOperator::merge_two_runs (order_operator.cpp:116-220).
+ * Emits the first k_max outputs of the stable merge of sorted A (m) and B (n):
+ * src_is_a[k] = 1 if output k comes from A, src_idx[k] = its index in its
+ * source, C_opt[k] (optional) the key.  Ties take A first (the rule of
+ * merge_batchs.cu:61; the reference's co-rank used a strict compare,
+ * merge_batchs.cu:22, which duplicates and drops rows on ties — fixed here).
+ * *consumed_a_out (device int64) = number of A elements among the outputs.
+ * dtype: INT/FLOAT/DATETIME compare typed values (FLOAT with IEEE '<', as the
+ * reference), UKEY32/UKEY64 compare encoded keys.  `descending` merges
+ * descending sequences (DescSortMergeOp).                                 */
+int ysql_merge_ws_bytes(uint64_t m, uint64_t n, uint64_t k_max, size_t *bytes);
+int ysql_merge_path(int dtype, int descending, const void *A, uint64_t m, const void *B,
+                    uint64_t n, uint64_t k_max, void *C_opt, uint32_t *src_idx,
+                    uint8_t *src_is_a, int64_t *consumed_a_out, void *ws, size_t ws_bytes,
+                    ysql_stream_t stream);
+/* Fused merge of (key, uint32 payload) runs: out = stable merge, ties from A
+ * first.  This is merge_path + the two gathers of move_rows_batch_merge
+ * (move_rows_kernel.hpp:85-143) in one pass, for encoded keys only.        */
+int ysql_merge_pairs(int key_dtype, int descending, const void *a_keys, const uint32_t *a_vals,
+                     uint64_t m, const void *b_keys, const uint32_t *b_vals, uint64_t n,
+                     void *out_keys, uint32_t *out_vals, void *ws, size_t ws_bytes,
+                     ysql_stream_t stream);
+
+/* ======================================================================
+ * Row movement (A7, A10, A14, A20)
+ * launch_move_one_col (move_rows_kernel.cu:51-70,136-143):
+ *     dst[i] = src[idx[i]], dst_null[i] = src_null[idx[i]]                 */
+int ysql_gather(uint32_t elem_bytes, const void *src, const uint8_t *src_null,
+                const uint32_t *idx, uint64_t n, void *dst, uint8_t *dst_null,
+                ysql_stream_t stream);
+/* launch_move_one_col_merge (move_rows_kernel.cu:72-105,146-156):
+ *     dst[d] = (is_a[i] ? a : b)[idx[i]], d = reverse ? n-1-i : i           */
+int ysql_gather2(uint32_t elem_bytes, const void *a, const uint8_t *a_null, const void *b,
+                 const uint8_t *b_null, const uint32_t *idx, const uint8_t *is_a, uint64_t n,
+                 int reverse, void *dst, uint8_t *dst_null, ysql_stream_t stream);
+/* lanch_move_rows_join_kernel (move_rows_kernel.cu:7-30,170-180):
+ *     dst[i] = src[pairs[2i + side]]  (side 0 = left, 1 = right)             */
+int ysql_gather_pairs(uint32_t elem_bytes, const void *src, const uint8_t *src_null,
+                      const uint32_t *pairs, int side, uint64_t n, void *dst, uint8_t *dst_null,
+                      ysql_stream_t stream);
+/* launch_move_rows_filter_kernel (move_rows_kernel.cu:32-49,158-167):
+ *     if mask[i]: dst[map[i]-1] = src[i]   (map = inclusive mask scan).
+ * NULL rows are written with their null byte (the reference skipped them and
+ * left holes, move_rows_kernel.cu:45 — divergent on purpose).              */
+int ysql_compact(uint32_t elem_bytes, const void *src, const uint8_t *src_null,
+                 const uint8_t *mask, const uint32_t *map, uint64_t n, void *dst,
+                 uint8_t *dst_null, ysql_stream_t stream);
+/* Fused FilterOperator::next (filter_operator.hpp:31-98): mask -> scan ->
+ * compaction of every column in one pass with no host-visible intermediate.
+ * Stable.  *count_out (device uint64) = number of selected rows.           */
+int ysql_compact_ws_bytes(uint64_t n, size_t *bytes);
+int ysql_compact_cols(const uint8_t *mask, const ysql_col *cols, int ncols, uint64_t n,
+                      uint64_t *count_out, void *ws, size_t ws_bytes, ysql_stream_t stream);
+/* Same, with the predicate `pred_col <cmp> scalar` evaluated in the same
+ * pass (no mask in HBM).  `scalar` is a 1-element device buffer of dtype. */
+int ysql_filter_compact(int dtype, int cmp, const void *pred_col, const void *scalar,
+                        const ysql_col *cols, int ncols, uint64_t n, uint64_t *count_out,
+                        void *ws, size_t ws_bytes, ysql_stream_t stream);
+
+/* ======================================================================
+ * Predicates (A12)
+ * Replaces launch_conditional_operators<T,Op>(d_rhs, d_lhs, t_rhs, t_lhs,
+ * d_res, sz, stream, isneg) (comparison_operators_kernel.hpp:7-8 ->
+ * comparison_operators_kernel.cu:114-199):
+ *     mask[i] = op(lhs[i or 0], rhs[i or 0]) XOR negate   (operand order op(lhs, rhs)) */
+int ysql_compare(int dtype, int cmp, const void *lhs, int lhs_is_scalar, const void *rhs,
+                 int rhs_is_scalar, int negate, uint8_t *mask_out, uint64_t n,
+                 ysql_stream_t stream);
+/* AND / OR of masks (launch_conditional_operators<bool, ANDOperator/OROperator>). */
+int ysql_mask_logic(int op, const uint8_t *a, int a_is_scalar, const uint8_t *b,
+                    int b_is_scalar, uint8_t *out, uint64_t n, ysql_stream_t stream);
+
+/* ======================================================================
+ * Projection arithmetic and casts (A15)
+ * launch_binary_operators<T,Op> (binary_operators_kernel.hpp:15-16 -> .cu:58-75):
+ *     out[i] = op(lhs[i or 0], rhs[i or 0]); DIV/REM by 0 -> 0; REM int only.  */
+int ysql_arith(int dtype, int op, const void *lhs, int lhs_is_scalar, const void *rhs,
+               int rhs_is_scalar, void *out, uint64_t n, ysql_stream_t stream);
+/* launch_numerical_cast<S,D> (cast_operators_kernel.hpp:8-9 -> .cu:24-43):
+ *     out[i] = (D) in[i]  for INT/FLOAT/DATETIME/BOOL sources.              */
+int ysql_cast(int src_dtype, int dst_dtype, const void *in, void *out, uint64_t n,
+              ysql_stream_t stream);
+
+/* ======================================================================
+ * Reductions (A16, A17)
+ * Replace launch_reduction_operators<T,Tres,Op> / launch_reduction_count_notnull
+ * (reduction_kernel.hpp:7-15 -> reduction_kernel.cu:78-205): fold n values
+ * (NULL rows skipped) into the caller-owned accumulator *acc_inout, which
+ * persists across batches as in the reference.  Deterministic: block partials
+ * are combined in block order, so a float SUM depends only on the input.
+ * COUNT_STAR counts all rows.                                              */
+int ysql_acc_init(int dtype, int agg, ysql_acc *acc, int nacc, ysql_stream_t stream);
+int ysql_reduce_ws_bytes(uint64_t n, size_t *bytes);
+int ysql_reduce(int dtype, int agg, const void *in, const uint8_t *null_opt, uint64_t n,
+                ysql_acc *acc_inout, void *ws, size_t ws_bytes, ysql_stream_t stream);
+/* AggregateExpression::getAggregate (aggregate.hpp:149-192): writes the
+ * final value: MIN/MAX -> dtype; SUM -> int64 (INT, DATETIME) or float
+ * (FLOAT, from the double sum); AVG -> float = (double)sum / (double)count;
+ * COUNT / COUNT_STAR -> int64.  Replaces div_avg_kernel and
+ * convert_double_to_float_kernel (reduction_kernel.cu:5-25).             */
+int ysql_agg_finalize(int dtype, int agg, const ysql_acc *acc, void *out, ysql_stream_t stream);
+
+/* GROUP BY (no reference function: the reference's AggregateOperator never
+ * reads `groups`, aggregate_operator.hpp:92-103).  acc_inout[g] folds the
+ * rows with group_id == g; group_id >= n_groups skips the row.           */
+int ysql_group_reduce(int dtype, int agg, const uint32_t *group_id, uint32_t n_groups,
+                      const void *in, const uint8_t *null_opt, uint64_t n, ysql_acc *acc_inout,
+                      ysql_stream_t stream);
+
+/* ======================================================================
+ * Joins (A18-A21)
+ * Replaces launch_outer_join_operators<T,Op> (comparison_operators_kernel.hpp
+ * :10-16 -> .cu:135-150), called from ComparisonJoinExpression::evaluate
+ * (comparison_join.hpp:70-236): every (l, r) with op(left[l], right[r])
+ * is a pair.  Pairs are written as 2 x uint32 (l, r), ordered by l then r
+ * (the reference's order was nondeterministic; the pair SET is the same).
+ * At most `cap` pairs are written; *count_out (device uint64) is the full
+ * count, so count > cap means overflow (the reference wrote past its
+ * 8 MiB buffer, comparison_join.hpp:87).  EQ uses a sort/hash build over
+ * `right`; the other ops are a tiled nested loop.                         */
+int ysql_join_ws_bytes(int dtype, int cmp, uint64_t L, uint64_t R, size_t *bytes);
+int ysql_join_pairs(int dtype, int cmp, const void *left, uint64_t L, const void *right,
+                    uint64_t R, uint32_t *pairs_out, uint64_t cap, uint64_t *count_out,
+                    void *ws, size_t ws_bytes, ysql_stream_t stream);
+/* launch_and_join_operators (comparison_operators_kernel.hpp:17-23 ->
+ * .cu:24-54,152-167): pairs of p1 that also occur in p2, each p1 entry at
+ * most once, in p1 order.                                                   */
+int ysql_pairs_intersect_ws_bytes(uint64_t n1, uint64_t n2, size_t *bytes);
+int ysql_pairs_intersect(const uint32_t *p1, uint64_t n1, const uint32_t *p2, uint64_t n2,
+                         uint32_t *pairs_out, uint64_t cap, uint64_t *count_out, void *ws,
+                         size_t ws_bytes, ysql_stream_t stream);
+/* launch_cross_product_col<T> (cross_product_kernel.hpp:10-11 ->
+ * cross_product_kernel.cu:29-40): for l < L, r < R, row = l*R + r:
+ *     dst[row] = is_left ? src[l] : src[r]                                  */
+int ysql_cross_expand(uint32_t elem_bytes, const void *src, const uint8_t *src_null, uint64_t L,
+                      uint64_t R, int is_left, void *dst, uint8_t *dst_null,
+                      ysql_stream_t stream);
+
+/* Fused star join + grouped SUM (no reference function; composition of
+ * join_pairs on dense primary keys + gather + arith + group_reduce).
+ * For each fact row i: every dimension d must match (fk_d[i] in [0, rows_d)
+ * and, if pass_d != NULL, pass_d[fk_d[i]] != 0); the group is
+ *     g = sum_d (group_d[fk_d[i]] - group_base_d) * stride_d
+ * over the dimensions with group_d != NULL (strides in dimension order,
+ * stride of the last grouped dimension = 1), and the row adds
+ * (int64)m1[i] * m2[i] to sums[g] and 1 to counts[g] (sums/counts are
+ * folded into, not overwritten).                                          */
+typedef struct ysql_star_dim {
+  const int32_t *fk;      /* fact foreign-key column (n rows)              */
+  uint32_t rows;          /* dimension rows; primary key = row index       */
+  uint32_t group_card;    /* number of group values (0 = not grouped)      */
+  const uint8_t *pass;    /* dimension filter result (rows bytes) or NULL  */
+  const int32_t *group;   /* dimension group attribute (rows) or NULL      */
+  int32_t group_base;     /* subtracted from group values                  */
+  uint32_t pad;
+} ysql_star_dim;
+#define YSQL_MAX_DIMS 4
+#define YSQL_MAX_GROUPS 16384
+int ysql_star_join_group_sum(const ysql_star_dim *dims, int ndims, const int32_t *m1,
+                             const int32_t *m2, uint64_t n, int64_t *sums, uint64_t *counts,
+                             uint32_t n_groups, ysql_stream_t stream);
+
+/* ======================================================================
+ * Utilities                                                              */
+/* Library version string (static storage). */
+const char *ysql_version(void);
+/* Last HIP error text for this thread (static thread-local storage). */
+const char *ysql_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* YSQL_KERNELS_H */

@@ -1,0 +1,444 @@
+"""Parity of every C-ABI kernel against the CPU oracle on seeded inputs.
+
+Bit-exact for integer, index and mask results; float SUM/AVG within 1e-6
+relative.  Sizes cover empty, single-row, wave (63/64/65) and tile (4095,
+4096, 4097) boundaries and a large case.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [0, 1, 63, 64, 65, 4095, 4096, 4097, 100_003]
+
+
+@pytest.fixture(scope="module")
+def ops(dev):
+    from yallasql_amd import ops as _ops
+
+    return _ops
+
+
+def T(a, dev):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+
+def N(t):
+    return t.cpu().numpy()
+
+
+def u32(t):
+    return N(t).view(np.uint32)
+
+
+def rng(seed):
+    return np.random.default_rng(seed)
+
+
+# ---------------------------------------------------------------- A1
+@pytest.mark.parametrize("dtype", [np.int32, np.float32, np.int64])
+@pytest.mark.parametrize("desc", [False, True])
+def test_sortkey_encode(ops, dev, dtype, desc):
+    r = rng(1)
+    if dtype == np.float32:
+        x = (r.standard_normal(5000) * 1e3).astype(np.float32)
+        x[:6] = [0.0, -0.0, np.inf, -np.inf, 1e-40, -1e-40]
+    elif dtype == np.int32:
+        x = r.integers(-2**31, 2**31, 5000, dtype=np.int64).astype(np.int32)
+        x[:2] = [-2**31, 2**31 - 1]
+    else:
+        x = r.integers(-2**63, 2**63 - 1, 5000, dtype=np.int64)
+    got = N(ops.sortkey_encode(T(x, dev), desc))
+    exp = O.sortkey_encode(x, desc)
+    assert np.array_equal(got.view(exp.dtype), exp)
+    back = N(ops.sortkey_decode(T(got, dev), torch.from_numpy(x[:1]).dtype, desc))
+    assert np.array_equal(back.view(x.dtype), x)
+
+
+# ---------------------------------------------------------------- A2-A6
+@pytest.mark.parametrize("n", SIZES + [1 << 20])
+@pytest.mark.parametrize("bits", [32, 64])
+@pytest.mark.parametrize("desc", [False, True])
+def test_radix_argsort(ops, dev, n, bits, desc):
+    r = rng(n + bits)
+    if bits == 32:
+        keys = r.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32)
+        if n > 10:
+            keys[: n // 3] = keys[0]  # heavy ties: stability
+        kt = T(keys.view(np.int32), dev)
+    else:
+        keys = r.integers(0, 2**63, n, dtype=np.uint64)
+        if n > 10:
+            keys[n // 2:] = r.integers(0, 50, n - n // 2)
+        kt = T(keys.view(np.int64), dev)
+    perm = ops.radix_argsort(kt, descending=desc)
+    exp = O.stable_argsort(keys, desc)
+    assert np.array_equal(u32(perm), exp)
+    assert np.array_equal(N(kt).view(keys.dtype), keys[exp])
+
+
+def test_radix_sort_pairs_bits(ops, dev):
+    r = rng(7)
+    n = 70_001
+    keys = r.integers(0, 2**20, n, dtype=np.uint64).astype(np.uint32)
+    vals = r.integers(0, 2**31, n, dtype=np.int64).astype(np.int32)
+    kt, vt = T(keys.view(np.int32), dev), T(vals, dev)
+    ops.radix_sort_pairs(kt, vt, begin_bit=0, end_bit=20)
+    exp = O.stable_argsort(keys)
+    assert np.array_equal(N(vt), vals[exp])
+    # 3 passes (odd) lands back in the caller's buffers
+    assert np.array_equal(N(kt).view(np.uint32), keys[exp])
+
+
+def test_radix_all_equal_and_sorted(ops, dev):
+    n = 50_000
+    for keys in (np.full(n, 7, np.uint32), np.arange(n, dtype=np.uint32), np.arange(n, dtype=np.uint32)[::-1].copy()):
+        kt = T(keys.view(np.int32), dev)
+        perm = ops.radix_argsort(kt)
+        assert np.array_equal(u32(perm), O.stable_argsort(keys))
+
+
+# ---------------------------------------------------------------- A8-A10
+@pytest.mark.parametrize("m,n", [(0, 0), (0, 5), (5, 0), (7, 1), (1, 7), (2048, 2048), (3000, 5000), (100_000, 77_777)])
+@pytest.mark.parametrize("desc", [False, True])
+def test_merge_path(ops, dev, m, n, desc):
+    r = rng(m * 31 + n)
+    A = np.sort(r.integers(0, 20, m).astype(np.int32))
+    B = np.sort(r.integers(0, 20, n).astype(np.int32))
+    if desc:
+        A, B = A[::-1].copy(), B[::-1].copy()
+    if (m, n) in ((7, 1), (1, 7)):  # the co-rank tie counterexample
+        A[:] = 5
+        B[:] = 5
+    for kmax in (m + n, (m + n) // 2):
+        c, idx, isa, cons = ops.merge_path(O.INT, desc, T(A, dev), T(B, dev), k_max=kmax)
+        ec, eidx, eisa, econs = O.merge(O.INT, desc, A, B, k_max=kmax)
+        assert np.array_equal(N(c), ec)
+        assert np.array_equal(u32(idx), eidx)
+        assert np.array_equal(N(isa), eisa)
+        assert int(cons.item()) == econs
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.int64])
+def test_merge_path_typed(ops, dev, dtype):
+    r = rng(3)
+    A = np.sort(r.standard_normal(5000).astype(dtype) if dtype == np.float32 else r.integers(-99, 99, 5000))
+    B = np.sort(r.standard_normal(3000).astype(dtype) if dtype == np.float32 else r.integers(-99, 99, 3000))
+    dc = O.FLOAT if dtype == np.float32 else O.DATETIME
+    c, idx, isa, cons = ops.merge_path(dc, False, T(A, dev), T(B, dev))
+    ec, eidx, eisa, econs = O.merge(dc, False, A, B)
+    assert np.array_equal(u32(idx), eidx) and np.array_equal(N(isa), eisa) and int(cons.item()) == econs
+
+
+@pytest.mark.parametrize("m,n", [(1, 0), (4097, 1), (100_000, 123_457)])
+def test_merge_pairs(ops, dev, m, n):
+    r = rng(m + n)
+    ak = np.sort(r.integers(0, 1000, m).astype(np.uint64))
+    bk = np.sort(r.integers(0, 1000, n).astype(np.uint64))
+    av = np.arange(m, dtype=np.int32)
+    bv = np.arange(m, m + n, dtype=np.int32)
+    ok, ov = ops.merge_pairs(T(ak.view(np.int64), dev), T(av, dev), T(bk.view(np.int64), dev), T(bv, dev))
+    allk = np.concatenate([ak, bk])
+    exp = O.stable_argsort(allk)
+    assert np.array_equal(N(ok).view(np.uint64), allk[exp])
+    assert np.array_equal(N(ov), np.concatenate([av, bv])[exp])
+
+
+# ---------------------------------------------------------------- A4/A13
+@pytest.mark.parametrize("n", SIZES)
+def test_scans(ops, dev, n):
+    r = rng(n)
+    mask = (r.random(n) < 0.4).astype(np.uint8) * r.integers(1, 255, n).astype(np.uint8)
+    got = ops.scan_mask_inclusive(T(mask, dev))
+    assert np.array_equal(u32(got), O.scan_mask(mask))
+    x = r.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32)
+    got = ops.scan_u32_inclusive(T(x.view(np.int32), dev))
+    assert np.array_equal(u32(got), O.scan_u32(x))
+
+
+# ---------------------------------------------------------------- A14 / fused filter
+@pytest.mark.parametrize("n", SIZES + [1 << 20])
+def test_compact_cols(ops, dev, n):
+    r = rng(n + 5)
+    mask = (r.random(n) < 0.5).astype(np.uint8)
+    a = r.integers(-1000, 1000, n).astype(np.int32)
+    b = r.standard_normal(n).astype(np.float32)
+    c = r.integers(0, 2**62, n, dtype=np.int64)
+    nul = (r.random(n) < 0.1).astype(np.uint8)
+    outs, cnt = ops.compact_cols(T(mask, dev), [T(a, dev), T(b, dev), T(c, dev)], [T(nul, dev), None, None])
+    assert cnt == int(mask.sum())
+    for (d, dn), src, sn in zip(outs, [a, b, c], [nul, None, None]):
+        ed, edn = O.compact(src, mask, sn)
+        assert np.array_equal(N(d), ed)
+        assert np.array_equal(N(dn), edn)
+
+
+@pytest.mark.parametrize("n", [0, 1, 4097, 1 << 20])
+@pytest.mark.parametrize("cmp", [0, 1, 2, 3, 4, 5])
+def test_filter_compact(ops, dev, n, cmp):
+    r = rng(n + cmp)
+    price = (1 + 999 * r.random(n)).astype(np.float32)
+    ids = r.permutation(n).astype(np.int32)
+    thr = np.array([500.0], dtype=np.float32)
+    if n > 10:
+        price[:5] = 500.0
+    outs, cnt = ops.filter_compact(cmp, T(price, dev), T(thr, dev), [T(ids, dev), T(price, dev)])
+    mask = O.compare(cmp, price, thr, rhs_scalar=True)
+    assert cnt == int(mask.sum())
+    assert np.array_equal(N(outs[0][0]), O.compact(ids, mask)[0])
+    assert np.array_equal(N(outs[1][0]), O.compact(price, mask)[0])
+
+
+def test_compact_strings(ops, dev):
+    r = rng(9)
+    n = 5000
+    s = r.integers(0, 128, (n, 256)).astype(np.uint8)
+    mask = (r.random(n) < 0.3).astype(np.uint8)
+    outs, cnt = ops.compact_cols(T(mask, dev), [T(s, dev)])
+    assert np.array_equal(N(outs[0][0]), s[mask.astype(bool)])
+
+
+def test_compact_map(ops, dev):
+    r = rng(11)
+    n = 10_000
+    mask = (r.random(n) < 0.5).astype(np.uint8)
+    x = r.integers(0, 100, n).astype(np.int32)
+    mt = T(mask, dev)
+    mp = ops.scan_mask_inclusive(mt)
+    cnt = int(u32(mp)[-1])
+    d, dn = ops.compact(T(x, dev), mt, mp, cnt)
+    assert np.array_equal(N(d), O.compact(x, mask)[0])
+
+
+# ---------------------------------------------------------------- A12 / A15
+@pytest.mark.parametrize("dtype", [np.int32, np.float32, np.int64])
+@pytest.mark.parametrize("cmp", range(6))
+def test_compare(ops, dev, dtype, cmp):
+    r = rng(cmp)
+    n = 9999
+    a = r.integers(-5, 5, n).astype(dtype)
+    b = r.integers(-5, 5, n).astype(dtype)
+    s = np.array([1], dtype=dtype)
+    for neg in (False, True):
+        assert np.array_equal(N(ops.compare(cmp, T(a, dev), T(b, dev), negate=neg)), O.compare(cmp, a, b, negate=neg))
+        assert np.array_equal(N(ops.compare(cmp, T(a, dev), T(s, dev), rhs_scalar=True)),
+                              O.compare(cmp, a, s, rhs_scalar=True))
+        assert np.array_equal(N(ops.compare(cmp, T(s, dev), T(b, dev), lhs_scalar=True)),
+                              O.compare(cmp, s, b, lhs_scalar=True))
+
+
+def test_compare_strings(ops, dev):
+    r = rng(2)
+    n = 3000
+    a = np.zeros((n, 256), np.uint8)
+    b = np.zeros((n, 256), np.uint8)
+    a[:, :3] = r.integers(97, 100, (n, 3))
+    b[:, :3] = r.integers(97, 100, (n, 3))
+    for cmp in range(6):
+        assert np.array_equal(N(ops.compare(cmp, T(a, dev), T(b, dev))), O.compare(cmp, a, b))
+
+
+def test_mask_logic(ops, dev):
+    r = rng(4)
+    a = (r.random(5000) < 0.5).astype(np.uint8)
+    b = (r.random(5000) < 0.5).astype(np.uint8)
+    for op in (0, 1):
+        assert np.array_equal(N(ops.mask_logic(op, T(a, dev), T(b, dev))), O.mask_logic(op, a, b))
+
+
+@pytest.mark.parametrize("dtype", [np.int32, np.int64, np.float32])
+@pytest.mark.parametrize("op", range(5))
+def test_arith(ops, dev, dtype, op):
+    if dtype == np.float32 and op == 4:
+        return
+    r = rng(op)
+    n = 7777
+    if dtype == np.float32:
+        a = r.standard_normal(n).astype(dtype)
+        b = r.standard_normal(n).astype(dtype)
+        b[::7] = 0
+    else:
+        info = np.iinfo(dtype)
+        a = r.integers(info.min, info.max, n, dtype=dtype)
+        b = r.integers(-3, 3, n).astype(dtype)
+        a[:3] = [info.min, info.min, info.max]
+        b[:3] = [-1, 0, -1]
+    s = np.array([3], dtype=dtype)
+    assert np.array_equal(N(ops.arith(op, T(a, dev), T(b, dev))), O.arith(op, a, b))
+    assert np.array_equal(N(ops.arith(op, T(s, dev), T(b, dev), lhs_scalar=True)), O.arith(op, s, b, lhs_scalar=True))
+
+
+def test_cast(ops, dev):
+    r = rng(5)
+    f = (r.standard_normal(5000) * 1e10).astype(np.float32)
+    f[:4] = [np.nan, np.inf, -np.inf, 3e9]
+    i = r.integers(-2**31, 2**31, 5000, dtype=np.int64).astype(np.int32)
+    l = r.integers(-2**62, 2**62, 5000, dtype=np.int64)
+    for x, dst in [(f, np.int32), (f, np.int64), (i, np.float32), (i, np.int64), (l, np.int32), (l, np.float32)]:
+        got = N(ops.cast(T(x, dev), torch.from_numpy(np.zeros(1, dst)).dtype))
+        assert np.array_equal(got, O.cast(x, dst)), (x.dtype, dst)
+
+
+# ---------------------------------------------------------------- gathers / cross
+def test_gathers(ops, dev):
+    r = rng(6)
+    n = 20_000
+    src = r.integers(0, 1 << 40, n, dtype=np.int64)
+    sn = (r.random(n) < 0.2).astype(np.uint8)
+    idx = r.integers(0, n, n).astype(np.uint32)
+    d, dn = ops.gather(T(src, dev), T(idx.view(np.int32), dev), T(sn, dev))
+    ed, edn = O.gather(src, idx, sn)
+    assert np.array_equal(N(d), ed) and np.array_equal(N(dn), edn)
+    b = r.integers(0, 1 << 40, n, dtype=np.int64)
+    isa = (r.random(n) < 0.5).astype(np.uint8)
+    for rev in (False, True):
+        d, dn = ops.gather2(T(src, dev), T(b, dev), T(idx.view(np.int32), dev), T(isa, dev), reverse=rev)
+        ed, _ = O.gather2(src, b, idx, isa, reverse=rev)
+        assert np.array_equal(N(d), ed)
+    pairs = r.integers(0, n, (n, 2)).astype(np.uint32)
+    for side in (0, 1):
+        d, _ = ops.gather_pairs(T(src, dev), T(pairs.view(np.int32), dev), side, n)
+        assert np.array_equal(N(d), O.gather_pairs(src, pairs, side)[0])
+
+
+def test_cross_expand(ops, dev):
+    r = rng(8)
+    x = r.integers(0, 100, 37).astype(np.int32)
+    for is_left in (True, False):
+        d, _ = ops.cross_expand(T(x, dev), 37, 256 if not is_left else 37, is_left)
+        e, _ = O.cross_expand(x, 37, 256 if not is_left else 37, is_left)
+        assert np.array_equal(N(d), e)
+
+
+# ---------------------------------------------------------------- A16/A17
+@pytest.mark.parametrize("n", [1, 65, 4097, 65536, 65537, 1 << 20, 3_000_001])
+@pytest.mark.parametrize("dtype", [np.int32, np.float32, np.int64])
+def test_reduce(ops, dev, n, dtype):
+    r = rng(n)
+    if dtype == np.float32:
+        x = r.random(n).astype(np.float32)
+    else:
+        x = r.integers(-10**6, 10**6, n).astype(dtype)
+    nul = (r.random(n) < 0.05).astype(np.uint8)
+    dc = O._DT[np.dtype(dtype)]
+    xt, nt = T(x, dev), T(nul, dev)
+    for agg in range(6):
+        acc = ops.acc_new(dc, agg, device=dev)
+        # two batches folded into the same accumulator
+        h = n // 2
+        ops.reduce(dc, agg, xt[:h], acc, null=nt[:h])
+        ops.reduce(dc, agg, xt[h:], acc, null=nt[h:])
+        got = ops.acc_read(acc)[0]
+        eacc = O.acc_new(dc, agg)
+        O.reduce(dc, agg, x, eacc, null=nul if agg != 5 else None)
+        e = eacc[0]
+        assert got["count"] == e.count
+        if agg in (0, 1):
+            if dtype == np.float32:
+                assert got["f"] == e.f
+            else:
+                assert got["i"] == e.i
+        if agg in (2, 3):
+            if dtype == np.float32:
+                assert abs(got["d"] - e.d) <= 1e-6 * max(1.0, abs(e.d))
+            else:
+                assert got["i"] == e.i
+        fin = N(ops.agg_finalize(dc, agg, acc))
+        efin = O.agg_finalize(dc, agg, eacc)
+        if dtype == np.float32 and agg in (2, 3):
+            g32, e32 = fin[:4].view(np.float32)[0], efin[:4].view(np.float32)[0]
+            assert abs(g32 - e32) <= 1e-6 * max(1.0, abs(e32))
+        elif agg == 3:
+            assert abs(fin[:4].view(np.float32)[0] - efin[:4].view(np.float32)[0]) <= 1e-6 * abs(efin[:4].view(np.float32)[0]) + 1e-30
+        else:
+            assert np.array_equal(fin, efin)
+
+
+def test_reduce_deterministic(ops, dev):
+    r = rng(12)
+    x = T(r.random(5_000_000).astype(np.float32), dev)
+    outs = set()
+    for _ in range(3):
+        acc = ops.acc_new(O.FLOAT, 2, device=dev)
+        ops.reduce(O.FLOAT, 2, x, acc)
+        outs.add(ops.acc_read(acc)[0]["d"])
+    assert len(outs) == 1
+
+
+@pytest.mark.parametrize("ng", [7, 4096, 7000])
+def test_group_reduce(ops, dev, ng):
+    r = rng(ng)
+    n = 300_000
+    gid = r.integers(0, ng + 3, n).astype(np.uint32)  # some out-of-range ids skip
+    xi = r.integers(-10**6, 10**6, n).astype(np.int64)
+    for agg in (0, 1, 2, 4, 5):
+        acc = ops.acc_new(O.DATETIME, agg, ng, device=dev)
+        ops.group_reduce(O.DATETIME, agg, T(gid.view(np.int32), dev), ng, T(xi, dev), acc)
+        got = ops.acc_read(acc)
+        eacc = O.acc_new(O.DATETIME, agg, ng)
+        O.group_reduce(O.DATETIME, agg, gid, ng, xi, eacc)
+        for g in range(ng):
+            assert got[g]["count"] == eacc[g].count
+            if agg in (0, 1, 2):
+                assert got[g]["i"] == eacc[g].i
+
+
+# ---------------------------------------------------------------- joins
+@pytest.mark.parametrize("dtype", [np.int32, np.float32, np.int64])
+@pytest.mark.parametrize("cmp", range(6))
+def test_join_pairs(ops, dev, dtype, cmp):
+    r = rng(cmp + 10)
+    L_, R_ = 1500, 700
+    left = r.integers(0, 300, L_).astype(dtype)
+    right = r.integers(0, 300, R_).astype(dtype)
+    pairs, c = ops.join_pairs(cmp, T(left, dev), T(right, dev))
+    exp = O.join_pairs(cmp, left, right)
+    assert c == exp.shape[0]
+    assert np.array_equal(N(pairs).view(np.uint32), exp)
+
+
+def test_join_overflow_reports_count(ops, dev):
+    left = np.zeros(100, np.int32)
+    right = np.zeros(50, np.int32)
+    pairs, c = ops.join_pairs(4, T(left, dev), T(right, dev), cap=10)
+    assert c == 5000 and pairs.shape[0] == 5000  # retried with the exact capacity
+
+
+def test_pairs_intersect(ops, dev):
+    r = rng(13)
+    p1 = r.integers(0, 40, (3000, 2)).astype(np.uint32)
+    p2 = r.integers(0, 40, (2000, 2)).astype(np.uint32)
+    got, c = ops.pairs_intersect(T(p1.view(np.int32), dev), T(p2.view(np.int32), dev))
+    exp = O.pairs_intersect(p1, p2)
+    assert np.array_equal(N(got).view(np.uint32), exp)
+
+
+def test_star_join_group_sum(ops, dev):
+    r = rng(14)
+    n = 2_000_000
+    dims_np = []
+    rows = [2556, 2000, 30000, 200000]
+    fks = [r.integers(0, R, n).astype(np.int32) for R in rows]
+    year = (1992 + (7 * np.arange(2556)) // 2556).astype(np.int32)
+    region = r.integers(0, 7, 30000).astype(np.int32)
+    cat = r.integers(0, 25, 200000).astype(np.int32)
+    brand = r.integers(0, 1000, 200000).astype(np.int32)
+    dims_np = [
+        {"fk": fks[0], "rows": 2556, "group": year, "group_base": 1992, "group_card": 7},
+        {"fk": fks[1], "rows": 2000},
+        {"fk": fks[2], "rows": 30000, "pass": (region == 0).astype(np.uint8)},
+        {"fk": fks[3], "rows": 200000, "pass": (cat == 0).astype(np.uint8), "group": brand, "group_card": 1000},
+    ]
+    m1 = r.integers(90000, 10000001, n).astype(np.int32)
+    m2 = r.integers(0, 11, n).astype(np.int32)
+    dims_t = []
+    for d in dims_np:
+        dt_ = {k: (T(v, dev) if isinstance(v, np.ndarray) else v) for k, v in d.items()}
+        dims_t.append(dt_)
+    sums, counts = ops.star_join_group_sum(dims_t, T(m1, dev), T(m2, dev), 7000)
+    es, ec = O.star_join_group_sum(dims_np, m1, m2, 7000)
+    assert np.array_equal(N(sums), es)
+    assert np.array_equal(N(counts).astype(np.uint64), ec)

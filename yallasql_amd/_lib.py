@@ -1,0 +1,127 @@
+"""ctypes binding of libyallasql_amd.so (the C ABI in include/ysql_kernels.h).
+
+The library is the product: there is no Python or CPU fallback.  Importing
+this module raises if the built library is missing.
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libyallasql_amd.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(
+        f"{LIB_PATH} is not built; run `python -m yallasql_amd.build` "
+        "(or __graft_entry__.build()) first"
+    )
+
+lib = C.CDLL(LIB_PATH)
+
+# ---- constants (mirror include/ysql_kernels.h) ---------------------------
+OK, EINVAL, EWORKSPACE, EOVERFLOW, EHIP, EUNSUPPORTED = 0, -1, -2, -3, -4, -5
+INT, FLOAT, DATETIME, STRING, BOOL, UKEY32, UKEY64 = 0, 1, 2, 3, 4, 16, 17
+STRING_BYTES = 256
+LT, LE, GT, GE, EQ, NE = 0, 1, 2, 3, 4, 5
+AND, OR = 0, 1
+ADD, SUB, MUL, DIV, REM = 0, 1, 2, 3, 4
+AGG_MAX, AGG_MIN, AGG_SUM, AGG_AVG, AGG_COUNT, AGG_COUNT_STAR = 0, 1, 2, 3, 4, 5
+MAX_COLS = 8
+MAX_DIMS = 4
+MAX_GROUPS = 16384
+
+
+class Acc(C.Structure):
+    _fields_ = [("i", C.c_int64), ("d", C.c_double), ("f", C.c_float), ("pad", C.c_uint32), ("count", C.c_int64)]
+
+
+class Col(C.Structure):
+    _fields_ = [
+        ("src", C.c_void_p), ("src_null", C.c_void_p), ("dst", C.c_void_p), ("dst_null", C.c_void_p),
+        ("elem_bytes", C.c_uint32), ("pad", C.c_uint32),
+    ]
+
+
+class StarDim(C.Structure):
+    _fields_ = [
+        ("fk", C.c_void_p), ("rows", C.c_uint32), ("group_card", C.c_uint32), ("pass_", C.c_void_p),
+        ("group", C.c_void_p), ("group_base", C.c_int32), ("pad", C.c_uint32),
+    ]
+
+
+assert C.sizeof(Acc) == 32 and C.sizeof(Col) == 40 and C.sizeof(StarDim) == 40
+
+P = C.c_void_p
+U64 = C.c_uint64
+I32 = C.c_int
+U32 = C.c_uint32
+SZP = C.POINTER(C.c_size_t)
+
+_SIGS = {
+    "ysql_sortkey_encode": [I32, I32, P, P, U64, P],
+    "ysql_sortkey_decode": [I32, I32, P, P, U64, P],
+    "ysql_radix_sort_ws_bytes": [I32, U64, SZP],
+    "ysql_radix_argsort": [I32, I32, P, P, U64, I32, I32, P, C.c_size_t, P],
+    "ysql_radix_sort_pairs": [I32, I32, P, P, U64, I32, I32, P, C.c_size_t, P],
+    "ysql_scan_ws_bytes": [U64, SZP],
+    "ysql_scan_u32_inclusive": [P, P, U64, P, C.c_size_t, P],
+    "ysql_scan_mask_inclusive": [P, P, U64, P, C.c_size_t, P],
+    "ysql_merge_ws_bytes": [U64, U64, U64, SZP],
+    "ysql_merge_path": [I32, I32, P, U64, P, U64, U64, P, P, P, P, P, C.c_size_t, P],
+    "ysql_merge_pairs": [I32, I32, P, P, U64, P, P, U64, P, P, P, C.c_size_t, P],
+    "ysql_gather": [U32, P, P, P, U64, P, P, P],
+    "ysql_gather2": [U32, P, P, P, P, P, P, U64, I32, P, P, P],
+    "ysql_gather_pairs": [U32, P, P, P, I32, U64, P, P, P],
+    "ysql_compact": [U32, P, P, P, P, U64, P, P, P],
+    "ysql_compact_ws_bytes": [U64, SZP],
+    "ysql_compact_cols": [P, C.POINTER(Col), I32, U64, P, P, C.c_size_t, P],
+    "ysql_filter_compact": [I32, I32, P, P, C.POINTER(Col), I32, U64, P, P, C.c_size_t, P],
+    "ysql_compare": [I32, I32, P, I32, P, I32, I32, P, U64, P],
+    "ysql_mask_logic": [I32, P, I32, P, I32, P, U64, P],
+    "ysql_arith": [I32, I32, P, I32, P, I32, P, U64, P],
+    "ysql_cast": [I32, I32, P, P, U64, P],
+    "ysql_acc_init": [I32, I32, P, I32, P],
+    "ysql_reduce_ws_bytes": [U64, SZP],
+    "ysql_reduce": [I32, I32, P, P, U64, P, P, C.c_size_t, P],
+    "ysql_agg_finalize": [I32, I32, P, P, P],
+    "ysql_group_reduce": [I32, I32, P, U32, P, P, U64, P, P],
+    "ysql_join_ws_bytes": [I32, I32, U64, U64, SZP],
+    "ysql_join_pairs": [I32, I32, P, U64, P, U64, P, U64, P, P, C.c_size_t, P],
+    "ysql_pairs_intersect_ws_bytes": [U64, U64, SZP],
+    "ysql_pairs_intersect": [P, U64, P, U64, P, U64, P, P, C.c_size_t, P],
+    "ysql_cross_expand": [U32, P, P, U64, U64, I32, P, P, P],
+    "ysql_star_join_group_sum": [C.POINTER(StarDim), I32, P, P, U64, P, P, U32, P],
+}
+
+for _name, _args in _SIGS.items():
+    _f = getattr(lib, _name)
+    _f.argtypes = _args
+    _f.restype = C.c_int
+
+lib.ysql_version.restype = C.c_char_p
+lib.ysql_version.argtypes = []
+lib.ysql_last_error.restype = C.c_char_p
+lib.ysql_last_error.argtypes = []
+
+
+class YsqlError(RuntimeError):
+    def __init__(self, fn, rc):
+        msg = lib.ysql_last_error().decode(errors="replace")
+        super().__init__(f"{fn} returned {rc}" + (f" ({msg})" if msg else ""))
+        self.rc = rc
+
+
+def check(fn, rc):
+    if rc != OK:
+        raise YsqlError(fn, rc)
+    return rc
+
+
+def call(name, *args):
+    """Call a C-ABI entry point and raise YsqlError on a nonzero status."""
+    return check(name, getattr(lib, name)(*args))
+
+
+def ws_bytes(name, *args):
+    out = C.c_size_t(0)
+    call(name, *args, C.byref(out))
+    return out.value

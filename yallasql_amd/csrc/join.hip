@@ -1,0 +1,523 @@
+// join.hip — inner joins producing pair lists, and pair-list intersection.
+//
+// Reference: outer_join_batches (comparison_operators_kernel.cu:5-22) tests
+// every (l, r) of a 32x32-thread 2-D grid and appends matches at an atomic
+// slot of an 8 MiB buffer with no bound check; and_join_batches (:24-54)
+// intersects two pair lists in an O(P1*P2) grid.
+//
+// Here every join writes its pairs ordered by (l, r) through a decoupled
+// look-back scan of per-row match counts, never past `cap`, and reports the
+// full count:
+//   EQ on INT/FLOAT/DATETIME: the right side is sorted by key with the radix
+//     sort (stable, so equal keys stay in r order), each distinct key's run
+//     (start, count) goes into an open-addressing hash table, and each left
+//     row probes it once.
+//   other ops and strings: a nested loop over right chunks staged in LDS,
+//     counted once and then written.
+#include <math.h>
+
+#include "common.hpp"
+
+namespace ysql {
+
+constexpr int kJoinBlock = 256;
+
+// ---- key canonicalisation for hashing: equal under '==' <=> equal bits
+template <typename T>
+struct JoinKey;
+template <>
+struct JoinKey<int32_t> {
+  typedef uint32_t U;
+  static __device__ __forceinline__ bool ok(int32_t) { return true; }
+  static __device__ __forceinline__ U bits(int32_t v) { return (uint32_t)v; }
+};
+template <>
+struct JoinKey<float> {
+  typedef uint32_t U;
+  static __device__ __forceinline__ bool ok(float v) { return !isnan(v); }
+  static __device__ __forceinline__ U bits(float v) { return v == 0.0f ? 0u : __float_as_uint(v); }
+};
+template <>
+struct JoinKey<int64_t> {
+  typedef uint64_t U;
+  static __device__ __forceinline__ bool ok(int64_t) { return true; }
+  static __device__ __forceinline__ U bits(int64_t v) { return (uint64_t)v; }
+};
+
+__device__ __forceinline__ uint32_t hash_u64(uint64_t x) {
+  x ^= x >> 33;
+  x *= 0xff51afd7ed558ccdull;
+  x ^= x >> 33;
+  x *= 0xc4ceb9fe1a85ec53ull;
+  x ^= x >> 33;
+  return (uint32_t)x;
+}
+
+struct HashSlot {
+  uint64_t key;
+  uint32_t start;
+  uint32_t count;
+};
+
+template <typename T>
+__global__ void k_join_keys(const T *right, uint64_t R, typename JoinKey<T>::U *keys, uint32_t *nan_count) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < R; i += (uint64_t)gridDim.x * blockDim.x) {
+    const T v = right[i];
+    // NaN keys never match: map them to the maximum key so they sort last
+    // and are excluded from the table.
+    keys[i] = JoinKey<T>::ok(v) ? JoinKey<T>::bits(v) : (typename JoinKey<T>::U)(~(typename JoinKey<T>::U)0);
+    if (!JoinKey<T>::ok(v)) atomicAdd(nan_count, 1u);
+  }
+}
+
+template <typename U>
+__device__ __forceinline__ uint64_t upper_bound_u(const U *a, uint64_t n, U v, uint64_t lo) {
+  uint64_t hi = n;
+  while (lo < hi) {
+    uint64_t mid = (lo + hi) >> 1;
+    if (a[mid] <= v) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// One thread per run start of the sorted keys inserts (key, start, count).
+// Keys whose rows are NaN (counted in nan_count, sorted last) are skipped.
+template <typename U>
+__global__ void k_join_build(const U *skeys, uint64_t R, const uint32_t *nan_count, HashSlot *table,
+                             uint32_t *used, uint32_t mask) {
+  const uint64_t valid = R - *nan_count;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < valid; i += (uint64_t)gridDim.x * blockDim.x) {
+    const U k = skeys[i];
+    if (i > 0 && skeys[i - 1] == k) continue;
+    const uint64_t end = upper_bound_u<U>(skeys, valid, k, i + 1);
+    uint32_t h = hash_u64((uint64_t)k) & mask;
+    while (atomicCAS(&used[h], 0u, 1u) != 0u) h = (h + 1) & mask;
+    table[h].key = (uint64_t)k;
+    table[h].start = (uint32_t)i;
+    table[h].count = (uint32_t)(end - i);
+  }
+}
+
+__device__ __forceinline__ uint32_t take_tile_j(uint32_t *ctr, uint32_t *s_tile) {
+  if (threadIdx.x == 0) *s_tile = atomicAdd(ctr, 1u);
+  __syncthreads();
+  return *s_tile;
+}
+
+// Probe: each thread owns 4 consecutive left rows of a 1024-row tile.
+template <typename T>
+__global__ __launch_bounds__(kJoinBlock) void k_join_probe(const T *left, uint64_t L, const HashSlot *table,
+                                                           const uint32_t *used, uint32_t mask, const uint32_t *perm,
+                                                           uint32_t *pairs, uint64_t cap, uint64_t *count_out,
+                                                           uint32_t *ctr, uint64_t *status, uint32_t ntiles) {
+  __shared__ uint64_t s_red[kJoinBlock / kWave + 1];
+  __shared__ uint32_t s_tile;
+  __shared__ uint64_t s_prefix;
+  const uint32_t tile = take_tile_j(ctr, &s_tile);
+  const uint64_t row0 = (uint64_t)tile * (kJoinBlock * 4) + threadIdx.x * 4;
+  uint32_t st[4], cn[4];
+  uint64_t mine = 0;
+#pragma unroll
+  for (int e = 0; e < 4; e++) {
+    st[e] = 0;
+    cn[e] = 0;
+    const uint64_t l = row0 + e;
+    if (l < L) {
+      const T v = left[l];
+      if (JoinKey<T>::ok(v)) {
+        const uint64_t k = (uint64_t)JoinKey<T>::bits(v);
+        uint32_t h = hash_u64(k) & mask;
+        while (used[h]) {
+          if (table[h].key == k) {
+            st[e] = table[h].start;
+            cn[e] = table[h].count;
+            break;
+          }
+          h = (h + 1) & mask;
+        }
+      }
+    }
+    mine += cn[e];
+  }
+  uint64_t total;
+  const uint64_t excl = block_exclusive_sum<kJoinBlock, uint64_t>(mine, s_red, &total);
+  if (threadIdx.x < kWave) {
+    const uint64_t p = lookback_exclusive(status, tile, total);
+    if (threadIdx.x == 0) s_prefix = p;
+  }
+  __syncthreads();
+  uint64_t pos = s_prefix + excl;
+#pragma unroll
+  for (int e = 0; e < 4; e++) {
+    for (uint32_t q = 0; q < cn[e]; q++, pos++) {
+      if (pos < cap) {
+        pairs[2 * pos] = (uint32_t)(row0 + e);
+        pairs[2 * pos + 1] = perm[st[e] + q];
+      }
+    }
+  }
+  if (tile == ntiles - 1 && threadIdx.x == 0) *count_out = s_prefix + total;
+}
+
+// ---- nested loop (θ-joins, strings): one left row per thread, right rows
+// staged in LDS in chunks; PASS 0 counts, PASS 1 writes.
+constexpr int kNlChunk = 1024;
+
+template <typename T, int CMP>
+__device__ __forceinline__ bool nl_cmp(const T &a, const T &b) {
+  return cmp_static<CMP>(a, b);
+}
+
+template <typename T, int CMP>
+__global__ __launch_bounds__(kJoinBlock) void k_join_nl(const T *left, uint64_t L, const T *right, uint64_t R,
+                                                        uint32_t *pairs, uint64_t cap, uint64_t *count_out,
+                                                        uint32_t *ctr, uint64_t *status, uint32_t ntiles) {
+  __shared__ T s_r[kNlChunk];
+  __shared__ uint64_t s_red[kJoinBlock / kWave + 1];
+  __shared__ uint32_t s_tile;
+  __shared__ uint64_t s_prefix;
+  const uint32_t tile = take_tile_j(ctr, &s_tile);
+  const uint64_t l = (uint64_t)tile * kJoinBlock + threadIdx.x;
+  const bool lv = l < L;
+  const T a = lv ? left[l] : T();
+  uint64_t mine = 0;
+  for (uint64_t r0 = 0; r0 < R; r0 += kNlChunk) {
+    const int cnt = (int)((R - r0) < (uint64_t)kNlChunk ? (R - r0) : (uint64_t)kNlChunk);
+    __syncthreads();
+    for (int x = threadIdx.x; x < cnt; x += kJoinBlock) s_r[x] = right[r0 + x];
+    __syncthreads();
+    if (lv)
+      for (int x = 0; x < cnt; x++) mine += nl_cmp<T, CMP>(a, s_r[x]);
+  }
+  uint64_t total;
+  const uint64_t excl = block_exclusive_sum<kJoinBlock, uint64_t>(mine, s_red, &total);
+  if (threadIdx.x < kWave) {
+    const uint64_t p = lookback_exclusive(status, tile, total);
+    if (threadIdx.x == 0) s_prefix = p;
+  }
+  __syncthreads();
+  uint64_t pos = s_prefix + excl;
+  for (uint64_t r0 = 0; r0 < R; r0 += kNlChunk) {
+    const int cnt = (int)((R - r0) < (uint64_t)kNlChunk ? (R - r0) : (uint64_t)kNlChunk);
+    __syncthreads();
+    for (int x = threadIdx.x; x < cnt; x += kJoinBlock) s_r[x] = right[r0 + x];
+    __syncthreads();
+    if (lv && mine)
+      for (int x = 0; x < cnt; x++)
+        if (nl_cmp<T, CMP>(a, s_r[x])) {
+          if (pos < cap) {
+            pairs[2 * pos] = (uint32_t)l;
+            pairs[2 * pos + 1] = (uint32_t)(r0 + x);
+          }
+          pos++;
+        }
+  }
+  if (tile == ntiles - 1 && threadIdx.x == 0) *count_out = s_prefix + total;
+}
+
+// Strings: global reads, no staging (256-byte rows).
+__global__ __launch_bounds__(kJoinBlock) void k_join_nl_str(const char *left, uint64_t L, const char *right,
+                                                            uint64_t R, int cmp, uint32_t *pairs, uint64_t cap,
+                                                            uint64_t *count_out, uint32_t *ctr, uint64_t *status,
+                                                            uint32_t ntiles) {
+  __shared__ uint64_t s_red[kJoinBlock / kWave + 1];
+  __shared__ uint32_t s_tile;
+  __shared__ uint64_t s_prefix;
+  const uint32_t tile = take_tile_j(ctr, &s_tile);
+  const uint64_t l = (uint64_t)tile * kJoinBlock + threadIdx.x;
+  const bool lv = l < L;
+  const char *a = left + (lv ? l : 0) * YSQL_STRING_BYTES;
+  uint64_t mine = 0;
+  if (lv)
+    for (uint64_t r = 0; r < R; r++) mine += cmp_apply(cmp, str_cmp(a, right + r * YSQL_STRING_BYTES), 0);
+  uint64_t total;
+  const uint64_t excl = block_exclusive_sum<kJoinBlock, uint64_t>(mine, s_red, &total);
+  if (threadIdx.x < kWave) {
+    const uint64_t p = lookback_exclusive(status, tile, total);
+    if (threadIdx.x == 0) s_prefix = p;
+  }
+  __syncthreads();
+  uint64_t pos = s_prefix + excl;
+  if (lv && mine)
+    for (uint64_t r = 0; r < R; r++)
+      if (cmp_apply(cmp, str_cmp(a, right + r * YSQL_STRING_BYTES), 0)) {
+        if (pos < cap) {
+          pairs[2 * pos] = (uint32_t)l;
+          pairs[2 * pos + 1] = (uint32_t)r;
+        }
+        pos++;
+      }
+  if (tile == ntiles - 1 && threadIdx.x == 0) *count_out = s_prefix + total;
+}
+
+// ---- pair-list intersection: p2 sorted as (l << 32 | r), p1 entries binary
+// searched, kept entries compacted in p1 order.
+__global__ void k_pairs_key(const uint32_t *p, uint64_t n, uint64_t *keys) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    keys[i] = ((uint64_t)p[2 * i] << 32) | p[2 * i + 1];
+}
+
+__global__ __launch_bounds__(kJoinBlock) void k_pairs_probe(const uint32_t *p1, uint64_t n1, const uint64_t *skeys,
+                                                            uint64_t n2, uint32_t *out, uint64_t cap,
+                                                            uint64_t *count_out, uint32_t *ctr, uint64_t *status,
+                                                            uint32_t ntiles) {
+  __shared__ uint64_t s_red[kJoinBlock / kWave + 1];
+  __shared__ uint32_t s_tile;
+  __shared__ uint64_t s_prefix;
+  const uint32_t tile = take_tile_j(ctr, &s_tile);
+  const uint64_t i = (uint64_t)tile * kJoinBlock + threadIdx.x;
+  bool hit = false;
+  uint64_t k = 0;
+  if (i < n1) {
+    k = ((uint64_t)p1[2 * i] << 32) | p1[2 * i + 1];
+    uint64_t lo = 0, hi = n2;
+    while (lo < hi) {
+      uint64_t mid = (lo + hi) >> 1;
+      if (skeys[mid] < k) lo = mid + 1;
+      else hi = mid;
+    }
+    hit = lo < n2 && skeys[lo] == k;
+  }
+  uint64_t total;
+  const uint64_t excl = block_exclusive_sum<kJoinBlock, uint64_t>(hit ? 1 : 0, s_red, &total);
+  if (threadIdx.x < kWave) {
+    const uint64_t p = lookback_exclusive(status, tile, total);
+    if (threadIdx.x == 0) s_prefix = p;
+  }
+  __syncthreads();
+  const uint64_t pos = s_prefix + excl;
+  if (hit && pos < cap) {
+    out[2 * pos] = (uint32_t)(k >> 32);
+    out[2 * pos + 1] = (uint32_t)k;
+  }
+  if (tile == ntiles - 1 && threadIdx.x == 0) *count_out = s_prefix + total;
+}
+
+__global__ void k_join_reset(uint32_t *ctr, uint64_t *status, uint32_t ntiles, uint32_t *used, uint32_t nused) {
+  const uint32_t i0 = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  if (i0 < 64) ctr[i0] = 0;
+  for (uint32_t i = i0; i < ntiles; i += stride) status[i] = 0;
+  for (uint32_t i = i0; i < nused; i += stride) used[i] = 0;
+}
+
+__global__ void k_store_zero_u64(uint64_t *p) { *p = 0; }
+
+// ---- workspace layout
+struct JoinWs {
+  uint32_t *ctr;      // [0] tile counter, [1] NaN count
+  uint64_t *status;   // ntiles + 1
+  void *keys;         // R keys (u32 or u64)
+  uint32_t *perm;     // R
+  HashSlot *table;    // cap
+  uint32_t *used;     // cap
+  void *sort_ws;
+  size_t sort_ws_bytes;
+};
+
+static uint32_t hash_cap(uint64_t R) {
+  uint64_t c = 64;
+  while (c < 2 * R) c <<= 1;
+  return (uint32_t)c;
+}
+
+static bool is_hash_eq(int dtype, int cmp) {
+  return cmp == YSQL_EQ && (dtype == YSQL_INT || dtype == YSQL_FLOAT || dtype == YSQL_DATETIME);
+}
+
+static size_t join_ws_layout(int dtype, int cmp, uint64_t L, uint64_t R, void *ws, size_t bytes, JoinWs *out) {
+  const bool hash = is_hash_eq(dtype, cmp);
+  const uint64_t tile_rows = hash ? kJoinBlock * 4 : kJoinBlock;
+  const uint64_t ntiles = ceil_div(L, tile_rows) + 1;
+  const int kdt = dtype == YSQL_DATETIME ? YSQL_UKEY64 : YSQL_UKEY32;
+  size_t sort_bytes = 0;
+  if (hash) ysql_radix_sort_ws_bytes(kdt, R, &sort_bytes);
+  const size_t kb = dtype == YSQL_DATETIME ? 8 : 4;
+  const uint32_t cap = hash ? hash_cap(R) : 1;
+  if (!out) {
+    WsCounter c;
+    c.take<uint32_t>(64);
+    c.take<uint64_t>(ntiles);
+    if (hash) {
+      c.take<char>(R * kb + 8);
+      c.take<uint32_t>(R + 1);
+      c.take<HashSlot>(cap);
+      c.take<uint32_t>(cap);
+      c.take<char>(sort_bytes);
+    }
+    return c.off;
+  }
+  WsCarver c(ws, bytes);
+  out->ctr = c.take<uint32_t>(64);
+  out->status = c.take<uint64_t>(ntiles);
+  bool ok = out->ctr && out->status;
+  if (hash) {
+    out->keys = c.take<char>(R * kb + 8);
+    out->perm = c.take<uint32_t>(R + 1);
+    out->table = c.take<HashSlot>(cap);
+    out->used = c.take<uint32_t>(cap);
+    out->sort_ws = c.take<char>(sort_bytes);
+    out->sort_ws_bytes = sort_bytes;
+    ok = ok && out->keys && out->perm && out->table && out->used && out->sort_ws;
+  }
+  return ok ? c.off : 0;
+}
+
+template <typename T>
+static int join_hash(const T *left, uint64_t L, const T *right, uint64_t R, uint32_t *pairs, uint64_t cap,
+                     uint64_t *count_out, const JoinWs &w, hipStream_t s) {
+  typedef typename JoinKey<T>::U U;
+  const uint32_t tcap = hash_cap(R);
+  const uint32_t ntiles = (uint32_t)ceil_div(L, kJoinBlock * 4);
+  k_join_reset<<<256, 256, 0, s>>>(w.ctr, w.status, ntiles + 1, w.used, tcap);
+  YSQL_LAUNCH_CHECK();
+  uint64_t g = ceil_div(R, 256);
+  if (g > 4096) g = 4096;
+  if (g < 1) g = 1;
+  if (R > 0) {
+    k_join_keys<T><<<(unsigned)g, 256, 0, s>>>(right, R, (U *)w.keys, w.ctr + 1);
+    YSQL_LAUNCH_CHECK();
+    const int kdt = sizeof(U) == 8 ? YSQL_UKEY64 : YSQL_UKEY32;
+    int rc = ysql_radix_argsort(kdt, 0, w.keys, w.perm, R, 0, 8 * sizeof(U), w.sort_ws, w.sort_ws_bytes, s);
+    if (rc) return rc;
+    k_join_build<U><<<(unsigned)g, 256, 0, s>>>((const U *)w.keys, R, w.ctr + 1, w.table, w.used, tcap - 1);
+    YSQL_LAUNCH_CHECK();
+  }
+  k_join_probe<T><<<ntiles, kJoinBlock, 0, s>>>(left, L, w.table, w.used, tcap - 1, w.perm, pairs, cap, count_out,
+                                                w.ctr, w.status, ntiles);
+  YSQL_LAUNCH_CHECK();
+  return YSQL_OK;
+}
+
+template <typename T>
+static int join_nl(int cmp, const T *left, uint64_t L, const T *right, uint64_t R, uint32_t *pairs, uint64_t cap,
+                   uint64_t *count_out, const JoinWs &w, hipStream_t s) {
+  const uint32_t ntiles = (uint32_t)ceil_div(L, kJoinBlock);
+  k_join_reset<<<64, 256, 0, s>>>(w.ctr, w.status, ntiles + 1, nullptr, 0);
+  YSQL_LAUNCH_CHECK();
+#define NL_CASE(C)                                                                                                \
+  case C:                                                                                                         \
+    k_join_nl<T, C><<<ntiles, kJoinBlock, 0, s>>>(left, L, right, R, pairs, cap, count_out, w.ctr, w.status,     \
+                                                  ntiles);                                                        \
+    break;
+  switch (cmp) {
+    NL_CASE(YSQL_LT)
+    NL_CASE(YSQL_LE)
+    NL_CASE(YSQL_GT)
+    NL_CASE(YSQL_GE)
+    NL_CASE(YSQL_EQ)
+    NL_CASE(YSQL_NE)
+    default: return YSQL_EINVAL;
+  }
+#undef NL_CASE
+  YSQL_LAUNCH_CHECK();
+  return YSQL_OK;
+}
+
+}  // namespace ysql
+
+using namespace ysql;
+
+extern "C" {
+
+int ysql_join_ws_bytes(int dtype, int cmp, uint64_t L, uint64_t R, size_t *bytes) {
+  if (!bytes) return YSQL_EINVAL;
+  *bytes = join_ws_layout(dtype, cmp, L, R, nullptr, 0, nullptr);
+  return YSQL_OK;
+}
+
+int ysql_join_pairs(int dtype, int cmp, const void *left, uint64_t L, const void *right, uint64_t R,
+                    uint32_t *pairs_out, uint64_t cap, uint64_t *count_out, void *ws, size_t ws_bytes,
+                    ysql_stream_t stream) {
+  hipStream_t s = as_stream(stream);
+  if (!count_out || cmp < YSQL_LT || cmp > YSQL_NE) return YSQL_EINVAL;
+  if (L >= (1ull << 32) || R >= (1ull << 30)) return YSQL_EINVAL;
+  if (cap > 0 && !pairs_out) return YSQL_EINVAL;
+  if (L == 0 || R == 0) {
+    k_store_zero_u64<<<1, 1, 0, s>>>(count_out);
+    YSQL_LAUNCH_CHECK();
+    return YSQL_OK;
+  }
+  if (!left || !right) return YSQL_EINVAL;
+  JoinWs w;
+  memset(&w, 0, sizeof(w));
+  if (!ws || ws_bytes < join_ws_layout(dtype, cmp, L, R, nullptr, 0, nullptr) ||
+      !join_ws_layout(dtype, cmp, L, R, ws, ws_bytes, &w))
+    return YSQL_EWORKSPACE;
+  if (is_hash_eq(dtype, cmp)) {
+    switch (dtype) {
+      case YSQL_INT: return join_hash<int32_t>((const int32_t *)left, L, (const int32_t *)right, R, pairs_out, cap, count_out, w, s);
+      case YSQL_FLOAT: return join_hash<float>((const float *)left, L, (const float *)right, R, pairs_out, cap, count_out, w, s);
+      case YSQL_DATETIME: return join_hash<int64_t>((const int64_t *)left, L, (const int64_t *)right, R, pairs_out, cap, count_out, w, s);
+    }
+    return YSQL_EINVAL;
+  }
+  switch (dtype) {
+    case YSQL_INT: return join_nl<int32_t>(cmp, (const int32_t *)left, L, (const int32_t *)right, R, pairs_out, cap, count_out, w, s);
+    case YSQL_FLOAT: return join_nl<float>(cmp, (const float *)left, L, (const float *)right, R, pairs_out, cap, count_out, w, s);
+    case YSQL_DATETIME: return join_nl<int64_t>(cmp, (const int64_t *)left, L, (const int64_t *)right, R, pairs_out, cap, count_out, w, s);
+    case YSQL_STRING: {
+      const uint32_t ntiles = (uint32_t)ceil_div(L, kJoinBlock);
+      k_join_reset<<<64, 256, 0, s>>>(w.ctr, w.status, ntiles + 1, nullptr, 0);
+      YSQL_LAUNCH_CHECK();
+      k_join_nl_str<<<ntiles, kJoinBlock, 0, s>>>((const char *)left, L, (const char *)right, R, cmp, pairs_out, cap,
+                                                   count_out, w.ctr, w.status, ntiles);
+      YSQL_LAUNCH_CHECK();
+      return YSQL_OK;
+    }
+  }
+  return YSQL_EINVAL;
+}
+
+int ysql_pairs_intersect_ws_bytes(uint64_t n1, uint64_t n2, size_t *bytes) {
+  if (!bytes) return YSQL_EINVAL;
+  size_t sb = 0;
+  ysql_radix_sort_ws_bytes(YSQL_UKEY64, n2, &sb);
+  WsCounter c;
+  c.take<uint32_t>(64);
+  c.take<uint64_t>(ceil_div(n1, kJoinBlock) + 1);
+  c.take<uint64_t>(n2 + 1);
+  c.take<uint32_t>(n2 + 1);
+  c.take<char>(sb);
+  *bytes = c.off;
+  return YSQL_OK;
+}
+
+int ysql_pairs_intersect(const uint32_t *p1, uint64_t n1, const uint32_t *p2, uint64_t n2, uint32_t *pairs_out,
+                         uint64_t cap, uint64_t *count_out, void *ws, size_t ws_bytes, ysql_stream_t stream) {
+  hipStream_t s = as_stream(stream);
+  if (!count_out) return YSQL_EINVAL;
+  if (n1 >= (1ull << 32) || n2 >= (1ull << 30)) return YSQL_EINVAL;
+  if (n1 == 0 || n2 == 0) {
+    k_store_zero_u64<<<1, 1, 0, s>>>(count_out);
+    YSQL_LAUNCH_CHECK();
+    return YSQL_OK;
+  }
+  if (!p1 || !p2 || (cap > 0 && !pairs_out)) return YSQL_EINVAL;
+  size_t need = 0, sb = 0;
+  ysql_pairs_intersect_ws_bytes(n1, n2, &need);
+  ysql_radix_sort_ws_bytes(YSQL_UKEY64, n2, &sb);
+  if (!ws || ws_bytes < need) return YSQL_EWORKSPACE;
+  WsCarver c(ws, ws_bytes);
+  uint32_t *ctr = c.take<uint32_t>(64);
+  const uint32_t ntiles = (uint32_t)ceil_div(n1, kJoinBlock);
+  uint64_t *status = c.take<uint64_t>(ntiles + 1);
+  uint64_t *keys = c.take<uint64_t>(n2 + 1);
+  uint32_t *perm = c.take<uint32_t>(n2 + 1);
+  void *sws = c.take<char>(sb);
+  if (!ctr || !status || !keys || !perm || !sws) return YSQL_EWORKSPACE;
+  k_join_reset<<<64, 256, 0, s>>>(ctr, status, ntiles + 1, nullptr, 0);
+  YSQL_LAUNCH_CHECK();
+  uint64_t g = ceil_div(n2, 256);
+  if (g > 4096) g = 4096;
+  k_pairs_key<<<(unsigned)g, 256, 0, s>>>(p2, n2, keys);
+  YSQL_LAUNCH_CHECK();
+  int rc = ysql_radix_argsort(YSQL_UKEY64, 0, keys, perm, n2, 0, 64, sws, sb, s);
+  if (rc) return rc;
+  k_pairs_probe<<<ntiles, kJoinBlock, 0, s>>>(p1, n1, keys, n2, pairs_out, cap, count_out, ctr, status, ntiles);
+  YSQL_LAUNCH_CHECK();
+  return YSQL_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,607 @@
+// reduce.hip — MIN/MAX/SUM/AVG/COUNT reductions, grouped aggregation and
+// the fused star join + grouped SUM.
+//
+// Reference: reduction_sum_wrap (reduction_kernel.cu:78-121) reduces 512
+// values per block with a 32-lane shuffle tree and folds each block into the
+// accumulator with one global atomic (float SUM through a double atomicAdd,
+// so the result depends on block timing).
+//
+// Here the input is cut into canonical tiles of 65536 rows.  A tile's partial
+// is computed in a fixed order (each thread folds its rows in order, a wave64
+// butterfly, then the 4 waves in order), tile partials are stored, and the
+// last workgroup to finish folds them in a fixed order and then folds the
+// total into the caller's accumulator.  The result is a function of the input
+// alone: the same float SUM comes out of every run and every launch shape.
+#include <float.h>
+#include <math.h>
+
+#include "common.hpp"
+
+namespace ysql {
+
+constexpr int kRedBlock = 256;
+constexpr uint64_t kRedTile = 65536;
+
+// Accumulator kinds
+//   KI: int64 in .i (INT, DATETIME: MIN/MAX/SUM/AVG)
+//   KF: float in .f (FLOAT MIN/MAX)
+//   KD: double in .d (FLOAT SUM/AVG)
+//   KC: count only (COUNT, COUNT_STAR)
+enum { KI = 0, KF = 1, KD = 2, KC = 3 };
+
+__host__ __device__ static inline int acc_kind(int dtype, int agg) {
+  if (agg == YSQL_AGG_COUNT || agg == YSQL_AGG_COUNT_STAR) return KC;
+  if (dtype == YSQL_FLOAT) return (agg == YSQL_AGG_SUM || agg == YSQL_AGG_AVG) ? KD : KF;
+  return KI;
+}
+
+// AGG reduced to its combine operator: 0 = sum, 1 = min, 2 = max
+__host__ __device__ static inline int agg_op(int agg) {
+  return agg == YSQL_AGG_MIN ? 1 : (agg == YSQL_AGG_MAX ? 2 : 0);
+}
+
+template <int KIND, int OP>
+__device__ __forceinline__ void acc_identity(ysql_acc &a, int dtype) {
+  a.i = 0;
+  a.d = 0.0;
+  a.f = 0.f;
+  a.pad = 0;
+  a.count = 0;
+  if (KIND == KI) {
+    if (OP == 1) a.i = dtype == YSQL_INT ? (int64_t)INT32_MAX : INT64_MAX;
+    if (OP == 2) a.i = dtype == YSQL_INT ? (int64_t)INT32_MIN : INT64_MIN;
+  } else if (KIND == KF) {
+    if (OP == 1) a.f = __builtin_inff();
+    if (OP == 2) a.f = -__builtin_inff();
+  }
+}
+
+template <int KIND, int OP>
+__device__ __forceinline__ void acc_combine(ysql_acc &a, const ysql_acc &b) {
+  a.count += b.count;
+  if (KIND == KI) {
+    if (OP == 0) a.i = (int64_t)((uint64_t)a.i + (uint64_t)b.i);
+    else if (OP == 1) a.i = b.i < a.i ? b.i : a.i;
+    else a.i = b.i > a.i ? b.i : a.i;
+  } else if (KIND == KF) {
+    if (OP == 1) a.f = fminf(a.f, b.f);
+    else a.f = fmaxf(a.f, b.f);
+  } else if (KIND == KD) {
+    a.d += b.d;
+  }
+}
+
+template <typename T, int KIND, int OP>
+__device__ __forceinline__ void acc_fold(ysql_acc &a, T v) {
+  a.count += 1;
+  if (KIND == KI) {
+    const int64_t x = (int64_t)v;
+    if (OP == 0) a.i = (int64_t)((uint64_t)a.i + (uint64_t)x);
+    else if (OP == 1) a.i = x < a.i ? x : a.i;
+    else a.i = x > a.i ? x : a.i;
+  } else if (KIND == KF) {
+    if (OP == 1) a.f = fminf(a.f, (float)v);
+    else a.f = fmaxf(a.f, (float)v);
+  } else if (KIND == KD) {
+    a.d += (double)v;
+  }
+}
+
+template <int KIND, int OP>
+__device__ __forceinline__ ysql_acc acc_shfl_xor(const ysql_acc &a, int m) {
+  ysql_acc b;
+  b.count = shfl_xor_t(a.count, m);
+  b.i = (KIND == KI) ? shfl_xor_t(a.i, m) : 0;
+  b.d = (KIND == KD) ? shfl_xor_t(a.d, m) : 0.0;
+  b.f = (KIND == KF) ? shfl_xor_t(a.f, m) : 0.f;
+  b.pad = 0;
+  return b;
+}
+
+// Fixed-shape block reduction; result valid in thread 0.
+template <int KIND, int OP>
+__device__ __forceinline__ ysql_acc block_reduce_acc(ysql_acc v, ysql_acc *s_w) {
+  constexpr int W = kRedBlock / kWave;
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    ysql_acc o = acc_shfl_xor<KIND, OP>(v, m);
+    acc_combine<KIND, OP>(v, o);
+  }
+  const int wave = threadIdx.x / kWave;
+  if (lane_id() == 0) s_w[wave] = v;
+  __syncthreads();
+  ysql_acc r = s_w[0];
+#pragma unroll
+  for (int w = 1; w < W; w++) acc_combine<KIND, OP>(r, s_w[w]);
+  __syncthreads();
+  return r;
+}
+
+// Folds the rows [base, end) of one canonical tile, in a fixed order:
+// rounds of 1024 rows, thread t owning rows t*4 .. t*4+3 of each round.
+template <typename T, int KIND, int OP, bool STAR>
+__device__ __forceinline__ void fold_tile(ysql_acc &p, const T *in, const uint8_t *nul, uint64_t base, uint64_t end) {
+  const int t = threadIdx.x;
+  for (uint64_t r = base; r < end; r += 1024) {
+    const uint64_t i0 = r + (uint64_t)t * 4;
+    if (STAR || (KIND == KC && !nul)) {  // every row counts; values unread
+      uint64_t c = 0;
+      for (int e = 0; e < 4; e++) c += (i0 + e < end);
+      p.count += c;
+      continue;
+    }
+    if (KIND == KC) {  // COUNT: rows whose null byte is 0; values unread
+      uint64_t c = 0;
+      for (int e = 0; e < 4; e++) c += (i0 + e < end) && !nul[i0 + e];
+      p.count += c;
+      continue;
+    }
+    T v[4];
+    uint8_t nn[4] = {0, 0, 0, 0};
+    if (i0 + 4 <= end && ((uintptr_t)(in + i0) & 15) == 0) {
+      if (sizeof(T) == 4) {
+        uint4 q = *(const uint4 *)(in + i0);
+        memcpy(v, &q, 16);
+      } else {
+        uint4 q[2];
+        q[0] = ((const uint4 *)(in + i0))[0];
+        q[1] = ((const uint4 *)(in + i0))[1];
+        memcpy(v, q, 32);
+      }
+      if (nul) {
+        if (((uintptr_t)(nul + i0) & 3) == 0) {
+          uint32_t w = *(const uint32_t *)(nul + i0);
+          memcpy(nn, &w, 4);
+        } else {
+          for (int e = 0; e < 4; e++) nn[e] = nul[i0 + e];
+        }
+      }
+#pragma unroll
+      for (int e = 0; e < 4; e++)
+        if (!nn[e]) acc_fold<T, KIND, OP>(p, v[e]);
+    } else {
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        const uint64_t i = i0 + e;
+        if (i < end && !(nul && nul[i])) acc_fold<T, KIND, OP>(p, in[i]);
+      }
+    }
+  }
+}
+
+template <typename T, int KIND, int OP, bool STAR>
+__global__ __launch_bounds__(kRedBlock) void k_reduce(const T *in, const uint8_t *nul, uint64_t n, int dtype,
+                                                      ysql_acc *acc, ysql_acc *partials, uint32_t *ticket,
+                                                      uint32_t ntiles) {
+  __shared__ ysql_acc s_w[kRedBlock / kWave];
+  __shared__ int s_last;
+  for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    ysql_acc p;
+    acc_identity<KIND, OP>(p, dtype);
+    const uint64_t base = (uint64_t)tile * kRedTile;
+    const uint64_t end = base + kRedTile < n ? base + kRedTile : n;
+    fold_tile<T, KIND, OP, STAR>(p, in, nul, base, end);
+    ysql_acc tot = block_reduce_acc<KIND, OP>(p, s_w);
+    if (threadIdx.x == 0) partials[tile] = tot;
+  }
+  __threadfence();
+  if (threadIdx.x == 0) s_last = (atomicAdd(ticket, 1u) == gridDim.x - 1);
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();
+  ysql_acc p;
+  acc_identity<KIND, OP>(p, dtype);
+  for (uint32_t x = threadIdx.x; x < ntiles; x += kRedBlock) {
+    ysql_acc q = partials[x];
+    acc_combine<KIND, OP>(p, q);
+  }
+  ysql_acc tot = block_reduce_acc<KIND, OP>(p, s_w);
+  if (threadIdx.x == 0) {
+    ysql_acc a = *acc;
+    acc_combine<KIND, OP>(a, tot);
+    *acc = a;
+    *ticket = 0;
+  }
+}
+
+__global__ void k_reduce_reset(uint32_t *ticket) { *ticket = 0; }
+
+__global__ void k_acc_init(ysql_acc *acc, int nacc, int dtype, int agg) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nacc) return;
+  ysql_acc a;
+  const int kind = acc_kind(dtype, agg), op = agg_op(agg);
+  a.i = 0;
+  a.d = 0.0;
+  a.f = 0.f;
+  a.pad = 0;
+  a.count = 0;
+  if (kind == KI && op == 1) a.i = dtype == YSQL_INT ? (int64_t)INT32_MAX : INT64_MAX;
+  if (kind == KI && op == 2) a.i = dtype == YSQL_INT ? (int64_t)INT32_MIN : INT64_MIN;
+  if (kind == KF && op == 1) a.f = __builtin_inff();
+  if (kind == KF && op == 2) a.f = -__builtin_inff();
+  acc[i] = a;
+}
+
+__global__ void k_finalize(int dtype, int agg, const ysql_acc *acc, void *out) {
+  const ysql_acc a = *acc;
+  switch (agg) {
+    case YSQL_AGG_MIN:
+    case YSQL_AGG_MAX:
+      if (dtype == YSQL_INT) *(int32_t *)out = (int32_t)a.i;
+      else if (dtype == YSQL_DATETIME) *(int64_t *)out = a.i;
+      else *(float *)out = a.f;
+      break;
+    case YSQL_AGG_SUM:
+      if (dtype == YSQL_FLOAT) *(float *)out = (float)a.d;
+      else *(int64_t *)out = a.i;
+      break;
+    case YSQL_AGG_AVG: {
+      const double s = dtype == YSQL_FLOAT ? a.d : (double)a.i;
+      *(float *)out = (float)(s / (double)a.count);
+      break;
+    }
+    default: *(int64_t *)out = a.count; break;
+  }
+}
+
+static size_t reduce_ws_size(uint64_t n) {
+  WsCounter c;
+  c.take<uint32_t>(64);
+  c.take<ysql_acc>(ceil_div(n, kRedTile) + 1);
+  return c.off;
+}
+
+template <typename T, int KIND, int OP, bool STAR>
+static int reduce_launch(const void *in, const uint8_t *nul, uint64_t n, int dtype, ysql_acc *acc, uint32_t *ticket,
+                         ysql_acc *partials, hipStream_t s) {
+  const uint32_t ntiles = (uint32_t)ceil_div(n, kRedTile);
+  uint32_t g = ntiles;
+  const uint32_t cap = (uint32_t)grid_cap() * 4;
+  if (g > cap) g = cap;
+  k_reduce<T, KIND, OP, STAR><<<g, kRedBlock, 0, s>>>((const T *)in, nul, n, dtype, acc, partials, ticket, ntiles);
+  YSQL_LAUNCH_CHECK();
+  return YSQL_OK;
+}
+
+template <typename T>
+static int reduce_typed(int dtype, int agg, const void *in, const uint8_t *nul, uint64_t n, ysql_acc *acc,
+                        uint32_t *ticket, ysql_acc *partials, hipStream_t s) {
+  const int kind = acc_kind(dtype, agg), op = agg_op(agg);
+  const bool star = agg == YSQL_AGG_COUNT_STAR;
+  if (star) return reduce_launch<T, KC, 0, true>(in, nullptr, n, dtype, acc, ticket, partials, s);
+  if (kind == KC) return reduce_launch<T, KC, 0, false>(in, nul, n, dtype, acc, ticket, partials, s);
+  if (kind == KI) {
+    if (op == 0) return reduce_launch<T, KI, 0, false>(in, nul, n, dtype, acc, ticket, partials, s);
+    if (op == 1) return reduce_launch<T, KI, 1, false>(in, nul, n, dtype, acc, ticket, partials, s);
+    return reduce_launch<T, KI, 2, false>(in, nul, n, dtype, acc, ticket, partials, s);
+  }
+  if constexpr (std::is_same<T, float>::value) {
+    if (kind == KD) return reduce_launch<T, KD, 0, false>(in, nul, n, dtype, acc, ticket, partials, s);
+    if (op == 1) return reduce_launch<T, KF, 1, false>(in, nul, n, dtype, acc, ticket, partials, s);
+    return reduce_launch<T, KF, 2, false>(in, nul, n, dtype, acc, ticket, partials, s);
+  }
+  return YSQL_EINVAL;
+}
+
+// ---------------------------------------------------------------------------
+// GROUP BY: LDS-privatised per-group accumulators (8-byte value + 32-bit
+// count per group) when they fit, folded into the caller's accumulators with
+// one global atomic per touched group per workgroup.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kGroupLdsMax = 4096;
+
+// float <-> order-preserving uint32 (so float MIN/MAX become integer atomics)
+__device__ __forceinline__ uint32_t f2ord(float f) {
+  uint32_t b = __float_as_uint(f);
+  return (b & 0x80000000u) ? ~b : (b ^ 0x80000000u);
+}
+__device__ __forceinline__ float ord2f(uint32_t k) {
+  uint32_t b = (k & 0x80000000u) ? (k ^ 0x80000000u) : ~k;
+  return __uint_as_float(b);
+}
+
+__device__ __forceinline__ void atomic_fmin_global(float *p, float v) {
+  unsigned int *a = (unsigned int *)p;
+  unsigned int old = __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  while (true) {
+    float cur = __uint_as_float(old);
+    float nv = fminf(cur, v);
+    if (__float_as_uint(nv) == old) break;
+    unsigned int prev = atomicCAS(a, old, __float_as_uint(nv));
+    if (prev == old) break;
+    old = prev;
+  }
+}
+__device__ __forceinline__ void atomic_fmax_global(float *p, float v) {
+  unsigned int *a = (unsigned int *)p;
+  unsigned int old = __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  while (true) {
+    float cur = __uint_as_float(old);
+    float nv = fmaxf(cur, v);
+    if (__float_as_uint(nv) == old) break;
+    unsigned int prev = atomicCAS(a, old, __float_as_uint(nv));
+    if (prev == old) break;
+    old = prev;
+  }
+}
+
+template <int KIND, int OP>
+__device__ __forceinline__ void global_fold(ysql_acc *a, uint64_t val_bits, uint64_t cnt) {
+  if (cnt == 0) return;
+  atomicAdd((unsigned long long *)&a->count, (unsigned long long)cnt);
+  if (KIND == KI) {
+    const long long v = (long long)val_bits;
+    if (OP == 0) atomicAdd((unsigned long long *)&a->i, (unsigned long long)v);
+    else if (OP == 1) atomicMin((long long *)&a->i, v);
+    else atomicMax((long long *)&a->i, v);
+  } else if (KIND == KD) {
+    atomicAdd(&a->d, __longlong_as_double((long long)val_bits));
+  } else if (KIND == KF) {
+    const float f = ord2f((uint32_t)val_bits);
+    if (OP == 1) atomic_fmin_global(&a->f, f);
+    else atomic_fmax_global(&a->f, f);
+  }
+}
+
+template <typename T, int KIND, int OP>
+__device__ __forceinline__ uint64_t slot_identity(int dtype) {
+  if (KIND == KI) {
+    if (OP == 1) return (uint64_t)(dtype == YSQL_INT ? (int64_t)INT32_MAX : INT64_MAX);
+    if (OP == 2) return (uint64_t)(dtype == YSQL_INT ? (int64_t)INT32_MIN : INT64_MIN);
+    return 0;
+  }
+  if (KIND == KF) return OP == 1 ? (uint64_t)f2ord(__builtin_inff()) : (uint64_t)f2ord(-__builtin_inff());
+  return 0;  // KD: bits of +0.0
+}
+
+template <typename T, int KIND, int OP>
+__device__ __forceinline__ void lds_fold(unsigned long long *slot, T v) {
+  if (KIND == KI) {
+    const long long x = (long long)v;
+    if (OP == 0) atomicAdd(slot, (unsigned long long)x);
+    else if (OP == 1) atomicMin((long long *)slot, x);
+    else atomicMax((long long *)slot, x);
+  } else if (KIND == KD) {
+    atomicAdd((double *)slot, (double)v);
+  } else if (KIND == KF) {
+    const unsigned long long k = f2ord((float)v);
+    if (OP == 1) atomicMin(slot, k);
+    else atomicMax(slot, k);
+  }
+}
+
+template <typename T, int KIND, int OP>
+__global__ __launch_bounds__(512) void k_group_lds(const uint32_t *gid, uint32_t ng, const T *in, const uint8_t *nul,
+                                                   uint64_t n, int dtype, bool star, ysql_acc *acc) {
+  __shared__ unsigned long long s_val[kGroupLdsMax];
+  __shared__ uint32_t s_cnt[kGroupLdsMax];
+  const uint64_t ident = slot_identity<T, KIND, OP>(dtype);
+  for (uint32_t g = threadIdx.x; g < ng; g += blockDim.x) {
+    s_val[g] = ident;
+    s_cnt[g] = 0;
+  }
+  __syncthreads();
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t g = gid[i];
+    if (g >= ng) continue;
+    if (!star && nul && nul[i]) continue;
+    atomicAdd(&s_cnt[g], 1u);
+    if (KIND != KC) lds_fold<T, KIND, OP>(&s_val[g], in[i]);
+  }
+  __syncthreads();
+  for (uint32_t g = threadIdx.x; g < ng; g += blockDim.x) global_fold<KIND, OP>(&acc[g], s_val[g], s_cnt[g]);
+}
+
+template <typename T, int KIND, int OP>
+__global__ __launch_bounds__(256) void k_group_global(const uint32_t *gid, uint32_t ng, const T *in,
+                                                      const uint8_t *nul, uint64_t n, bool star, ysql_acc *acc) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t g = gid[i];
+    if (g >= ng) continue;
+    if (!star && nul && nul[i]) continue;
+    uint64_t bits = 0;
+    if (KIND == KI) bits = (uint64_t)(int64_t)in[i];
+    else if (KIND == KD) bits = (uint64_t)__double_as_longlong((double)in[i]);
+    else if (KIND == KF) bits = f2ord((float)in[i]);
+    global_fold<KIND, OP>(&acc[g], bits, 1);
+  }
+}
+
+template <typename T, int KIND, int OP>
+static int group_launch(const uint32_t *gid, uint32_t ng, const void *in, const uint8_t *nul, uint64_t n, int dtype,
+                        bool star, ysql_acc *acc, hipStream_t s) {
+  uint64_t g = ceil_div(n, 512 * 8);
+  const uint64_t cap = (uint64_t)grid_cap() * 2;
+  if (g > cap) g = cap;
+  if (g < 1) g = 1;
+  if (ng <= kGroupLdsMax)
+    k_group_lds<T, KIND, OP><<<(unsigned)g, 512, 0, s>>>(gid, ng, (const T *)in, nul, n, dtype, star, acc);
+  else
+    k_group_global<T, KIND, OP><<<(unsigned)(g * 2), 256, 0, s>>>(gid, ng, (const T *)in, nul, n, star, acc);
+  YSQL_LAUNCH_CHECK();
+  return YSQL_OK;
+}
+
+template <typename T>
+static int group_typed(int dtype, int agg, const uint32_t *gid, uint32_t ng, const void *in, const uint8_t *nul,
+                       uint64_t n, ysql_acc *acc, hipStream_t s) {
+  const int kind = acc_kind(dtype, agg), op = agg_op(agg);
+  const bool star = agg == YSQL_AGG_COUNT_STAR;
+  if (kind == KC) return group_launch<T, KC, 0>(gid, ng, in, nul, n, dtype, star, acc, s);
+  if (kind == KI) {
+    if (op == 0) return group_launch<T, KI, 0>(gid, ng, in, nul, n, dtype, star, acc, s);
+    if (op == 1) return group_launch<T, KI, 1>(gid, ng, in, nul, n, dtype, star, acc, s);
+    return group_launch<T, KI, 2>(gid, ng, in, nul, n, dtype, star, acc, s);
+  }
+  if constexpr (std::is_same<T, float>::value) {
+    if (kind == KD) return group_launch<T, KD, 0>(gid, ng, in, nul, n, dtype, star, acc, s);
+    if (op == 1) return group_launch<T, KF, 1>(gid, ng, in, nul, n, dtype, star, acc, s);
+    return group_launch<T, KF, 2>(gid, ng, in, nul, n, dtype, star, acc, s);
+  }
+  return YSQL_EINVAL;
+}
+
+// ---------------------------------------------------------------------------
+// Star join + grouped SUM(m1 * m2): one pass over the fact table.  The
+// dimension columns (pass flags and group attributes) are small and stay in
+// L2; the fact foreign keys are read in dimension order and a row stops at
+// its first failing dimension, so the measures are read only for rows that
+// qualify.
+// ---------------------------------------------------------------------------
+struct StarArgs {
+  ysql_star_dim d[YSQL_MAX_DIMS];
+  uint32_t stride[YSQL_MAX_DIMS];
+};
+
+constexpr uint32_t kStarLdsMax = 8192;
+
+template <bool LDS>
+__global__ __launch_bounds__(1024) void k_star(StarArgs a, int nd, const int32_t *__restrict__ m1,
+                                               const int32_t *__restrict__ m2, uint64_t n, int64_t *sums,
+                                               uint64_t *counts, uint32_t ng) {
+  __shared__ unsigned long long s_sum[LDS ? kStarLdsMax : 1];
+  __shared__ uint32_t s_cnt[LDS ? kStarLdsMax : 1];
+  if (LDS) {
+    for (uint32_t g = threadIdx.x; g < ng; g += blockDim.x) {
+      s_sum[g] = 0;
+      s_cnt[g] = 0;
+    }
+    __syncthreads();
+  }
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t g = 0;
+    bool ok = true;
+#pragma unroll
+    for (int d = 0; d < YSQL_MAX_DIMS; d++) {
+      if (d < nd && ok) {
+        const int32_t f = a.d[d].fk[i];
+        if (f < 0 || (uint32_t)f >= a.d[d].rows) ok = false;
+        else if (a.d[d].pass && !a.d[d].pass[f]) ok = false;
+        else if (a.d[d].group) {
+          const int64_t v = (int64_t)a.d[d].group[f] - a.d[d].group_base;
+          if (v < 0 || v >= (int64_t)a.d[d].group_card) ok = false;
+          else g += (uint32_t)v * a.stride[d];
+        }
+      }
+    }
+    if (!ok) continue;
+    const long long prod = (long long)m1[i] * (long long)m2[i];
+    if (LDS) {
+      atomicAdd(&s_sum[g], (unsigned long long)prod);
+      atomicAdd(&s_cnt[g], 1u);
+    } else {
+      atomicAdd((unsigned long long *)&sums[g], (unsigned long long)prod);
+      atomicAdd((unsigned long long *)&counts[g], 1ull);
+    }
+  }
+  if (LDS) {
+    __syncthreads();
+    for (uint32_t g = threadIdx.x; g < ng; g += blockDim.x) {
+      if (s_cnt[g]) {
+        atomicAdd((unsigned long long *)&sums[g], s_sum[g]);
+        atomicAdd((unsigned long long *)&counts[g], (unsigned long long)s_cnt[g]);
+      }
+    }
+  }
+}
+
+}  // namespace ysql
+
+using namespace ysql;
+
+extern "C" {
+
+int ysql_acc_init(int dtype, int agg, ysql_acc *acc, int nacc, ysql_stream_t stream) {
+  if (nacc <= 0) return YSQL_OK;
+  if (!acc || agg < YSQL_AGG_MAX || agg > YSQL_AGG_COUNT_STAR) return YSQL_EINVAL;
+  k_acc_init<<<(nacc + 255) / 256, 256, 0, as_stream(stream)>>>(acc, nacc, dtype, agg);
+  YSQL_LAUNCH_CHECK();
+  return YSQL_OK;
+}
+
+int ysql_reduce_ws_bytes(uint64_t n, size_t *bytes) {
+  if (!bytes) return YSQL_EINVAL;
+  *bytes = reduce_ws_size(n);
+  return YSQL_OK;
+}
+
+int ysql_reduce(int dtype, int agg, const void *in, const uint8_t *null_opt, uint64_t n, ysql_acc *acc_inout, void *ws,
+                size_t ws_bytes, ysql_stream_t stream) {
+  if (!acc_inout || agg < YSQL_AGG_MAX || agg > YSQL_AGG_COUNT_STAR) return YSQL_EINVAL;
+  if (n == 0) return YSQL_OK;
+  if (!in && agg != YSQL_AGG_COUNT_STAR && agg != YSQL_AGG_COUNT) return YSQL_EINVAL;
+  if (n >= (1ull << 40)) return YSQL_EINVAL;
+  WsCarver c(ws, ws_bytes);
+  uint32_t *ticket = c.take<uint32_t>(64);
+  ysql_acc *partials = c.take<ysql_acc>(ceil_div(n, kRedTile) + 1);
+  if (!ws || !ticket || !partials) return YSQL_EWORKSPACE;
+  hipStream_t s = as_stream(stream);
+  k_reduce_reset<<<1, 1, 0, s>>>(ticket);
+  YSQL_LAUNCH_CHECK();
+  if (agg == YSQL_AGG_COUNT || agg == YSQL_AGG_COUNT_STAR) {
+    // counts read only the null mask (reduction_count_notnull reads only isnull)
+    return reduce_typed<int32_t>(YSQL_INT, agg, nullptr, null_opt, n, acc_inout, ticket, partials, s);
+  }
+  switch (dtype) {
+    case YSQL_INT: return reduce_typed<int32_t>(dtype, agg, in, null_opt, n, acc_inout, ticket, partials, s);
+    case YSQL_FLOAT: return reduce_typed<float>(dtype, agg, in, null_opt, n, acc_inout, ticket, partials, s);
+    case YSQL_DATETIME: return reduce_typed<int64_t>(dtype, agg, in, null_opt, n, acc_inout, ticket, partials, s);
+  }
+  return YSQL_EINVAL;
+}
+
+int ysql_agg_finalize(int dtype, int agg, const ysql_acc *acc, void *out, ysql_stream_t stream) {
+  if (!acc || !out || agg < YSQL_AGG_MAX || agg > YSQL_AGG_COUNT_STAR) return YSQL_EINVAL;
+  k_finalize<<<1, 1, 0, as_stream(stream)>>>(dtype, agg, acc, out);
+  YSQL_LAUNCH_CHECK();
+  return YSQL_OK;
+}
+
+int ysql_group_reduce(int dtype, int agg, const uint32_t *group_id, uint32_t n_groups, const void *in,
+                      const uint8_t *null_opt, uint64_t n, ysql_acc *acc_inout, ysql_stream_t stream) {
+  if (n == 0 || n_groups == 0) return YSQL_OK;
+  if (!group_id || !acc_inout || agg < YSQL_AGG_MAX || agg > YSQL_AGG_COUNT_STAR) return YSQL_EINVAL;
+  if (!in && agg != YSQL_AGG_COUNT_STAR) return YSQL_EINVAL;
+  hipStream_t s = as_stream(stream);
+  const void *src = in ? in : (const void *)group_id;
+  switch (dtype) {
+    case YSQL_INT: return group_typed<int32_t>(dtype, agg, group_id, n_groups, src, null_opt, n, acc_inout, s);
+    case YSQL_FLOAT: return group_typed<float>(dtype, agg, group_id, n_groups, src, null_opt, n, acc_inout, s);
+    case YSQL_DATETIME: return group_typed<int64_t>(dtype, agg, group_id, n_groups, src, null_opt, n, acc_inout, s);
+  }
+  return YSQL_EINVAL;
+}
+
+int ysql_star_join_group_sum(const ysql_star_dim *dims, int ndims, const int32_t *m1, const int32_t *m2, uint64_t n,
+                             int64_t *sums, uint64_t *counts, uint32_t n_groups, ysql_stream_t stream) {
+  if (ndims < 1 || ndims > YSQL_MAX_DIMS || !dims || !m1 || !m2 || !sums || !counts) return YSQL_EINVAL;
+  if (n_groups == 0 || n_groups > YSQL_MAX_GROUPS) return YSQL_EINVAL;
+  StarArgs a;
+  memset(&a, 0, sizeof(a));
+  uint64_t stride = 1;
+  for (int d = ndims - 1; d >= 0; d--) {
+    a.d[d] = dims[d];
+    if (!dims[d].fk) return YSQL_EINVAL;
+    a.stride[d] = (uint32_t)stride;
+    if (dims[d].group) {
+      if (dims[d].group_card == 0) return YSQL_EINVAL;
+      stride *= dims[d].group_card;
+    }
+  }
+  if (stride > n_groups) return YSQL_EINVAL;
+  if (n == 0) return YSQL_OK;
+  hipStream_t s = as_stream(stream);
+  uint64_t g = ceil_div(n, 1024 * 16);
+  const uint64_t cap = (uint64_t)grid_cap();
+  if (g > cap) g = cap;
+  if (g < 1) g = 1;
+  if (n_groups <= kStarLdsMax)
+    k_star<true><<<(unsigned)g, 1024, 0, s>>>(a, ndims, m1, m2, n, sums, counts, n_groups);
+  else
+    k_star<false><<<(unsigned)g, 1024, 0, s>>>(a, ndims, m1, m2, n, sums, counts, n_groups);
+  YSQL_LAUNCH_CHECK();
+  return YSQL_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,325 @@
+"""Python entry points over the C ABI, on torch device tensors.
+
+Device memory and streams come from torch (plumbing only); every op runs the
+HIP kernels of libyallasql_amd.so on torch's current stream.  Sort keys and
+unsigned values are carried in int32/int64 tensors holding the unsigned bits.
+"""
+import ctypes as C
+
+import torch
+
+from . import _lib as L
+from ._lib import call, ws_bytes
+
+_DT = {torch.int32: L.INT, torch.float32: L.FLOAT, torch.int64: L.DATETIME, torch.uint8: L.BOOL}
+
+
+def dtype_code(t):
+    if t.dtype == torch.uint8 and t.dim() == 2 and t.shape[1] == L.STRING_BYTES:
+        return L.STRING
+    return _DT[t.dtype]
+
+
+def elem_bytes(t):
+    if t.dim() == 2:
+        return t.shape[1] * t.element_size()
+    return t.element_size()
+
+
+def _p(t):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+def _s():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _dev(t):
+    return t.device
+
+
+def workspace(nbytes, device):
+    return torch.empty(max(int(nbytes), 256), dtype=torch.uint8, device=device)
+
+
+def _rows(t):
+    return t.shape[0]
+
+
+# ---- A1 -----------------------------------------------------------------
+def sortkey_encode(x, descending=False):
+    out = torch.empty_like(x) if x.dtype != torch.float32 else torch.empty(x.shape, dtype=torch.int32, device=x.device)
+    call("ysql_sortkey_encode", dtype_code(x), int(descending), _p(x), _p(out), x.numel(), _s())
+    return out
+
+
+def sortkey_decode(k, dtype, descending=False):
+    out = torch.empty(k.shape, dtype=dtype, device=k.device)
+    call("ysql_sortkey_decode", _DT[dtype], int(descending), _p(k), _p(out), k.numel(), _s())
+    return out
+
+
+# ---- A2-A6 --------------------------------------------------------------
+def _kdt(keys):
+    return L.UKEY64 if keys.element_size() == 8 else L.UKEY32
+
+
+def radix_argsort(keys, descending=False, begin_bit=0, end_bit=None):
+    """Sorts `keys` (unsigned bits) in place; returns perm (int32 bits of u32)."""
+    n = keys.numel()
+    kdt = _kdt(keys)
+    end_bit = 8 * keys.element_size() if end_bit is None else end_bit
+    perm = torch.empty(n, dtype=torch.int32, device=keys.device)
+    ws = workspace(ws_bytes("ysql_radix_sort_ws_bytes", kdt, n), keys.device)
+    call("ysql_radix_argsort", kdt, int(descending), _p(keys), _p(perm), n, begin_bit, end_bit, _p(ws), ws.numel(), _s())
+    return perm
+
+
+def radix_sort_pairs(keys, vals, descending=False, begin_bit=0, end_bit=None, ws=None):
+    n = keys.numel()
+    kdt = _kdt(keys)
+    end_bit = 8 * keys.element_size() if end_bit is None else end_bit
+    if ws is None:
+        ws = workspace(ws_bytes("ysql_radix_sort_ws_bytes", kdt, n), keys.device)
+    call("ysql_radix_sort_pairs", kdt, int(descending), _p(keys), _p(vals), n, begin_bit, end_bit, _p(ws), ws.numel(),
+         _s())
+    return keys, vals
+
+
+# ---- A4 / A13 -----------------------------------------------------------
+def scan_u32_inclusive(x):
+    n = x.numel()
+    out = torch.empty(n, dtype=torch.int32, device=x.device)
+    ws = workspace(ws_bytes("ysql_scan_ws_bytes", n), x.device)
+    call("ysql_scan_u32_inclusive", _p(x), _p(out), n, _p(ws), ws.numel(), _s())
+    return out
+
+
+def scan_mask_inclusive(mask):
+    n = mask.numel()
+    out = torch.empty(n, dtype=torch.int32, device=mask.device)
+    ws = workspace(ws_bytes("ysql_scan_ws_bytes", n), mask.device)
+    call("ysql_scan_mask_inclusive", _p(mask), _p(out), n, _p(ws), ws.numel(), _s())
+    return out
+
+
+# ---- A8-A10 -------------------------------------------------------------
+def merge_path(dtype, descending, A, B, k_max=None, want_keys=True):
+    m, n = A.numel(), B.numel()
+    kmax = m + n if k_max is None else min(k_max, m + n)
+    dev = A.device
+    C_ = torch.empty(kmax, dtype=A.dtype, device=dev) if want_keys else None
+    idx = torch.empty(kmax, dtype=torch.int32, device=dev)
+    isa = torch.empty(kmax, dtype=torch.uint8, device=dev)
+    cons = torch.zeros(1, dtype=torch.int64, device=dev)
+    ws = workspace(ws_bytes("ysql_merge_ws_bytes", m, n, kmax), dev)
+    call("ysql_merge_path", dtype, int(descending), _p(A), m, _p(B), n, kmax, _p(C_), _p(idx), _p(isa), _p(cons),
+         _p(ws), ws.numel(), _s())
+    return C_, idx, isa, cons
+
+
+def merge_pairs(a_keys, a_vals, b_keys, b_vals, descending=False, out_keys=None, out_vals=None, ws=None):
+    m, n = a_keys.numel(), b_keys.numel()
+    dev = a_keys.device
+    if out_keys is None:
+        out_keys = torch.empty(m + n, dtype=a_keys.dtype, device=dev)
+    if out_vals is None:
+        out_vals = torch.empty(m + n, dtype=torch.int32, device=dev)
+    if ws is None:
+        ws = workspace(ws_bytes("ysql_merge_ws_bytes", m, n, m + n), dev)
+    call("ysql_merge_pairs", _kdt(a_keys), int(descending), _p(a_keys), _p(a_vals), m, _p(b_keys), _p(b_vals), n,
+         _p(out_keys), _p(out_vals), _p(ws), ws.numel(), _s())
+    return out_keys, out_vals
+
+
+# ---- row movement --------------------------------------------------------
+def _null_out(src_null, n, dev):
+    return torch.empty(n, dtype=torch.uint8, device=dev)
+
+
+def gather(src, idx, src_null=None):
+    n = idx.numel()
+    dst = torch.empty((n,) + tuple(src.shape[1:]), dtype=src.dtype, device=src.device)
+    dn = _null_out(src_null, n, src.device)
+    call("ysql_gather", elem_bytes(src), _p(src), _p(src_null), _p(idx), n, _p(dst), _p(dn), _s())
+    return dst, dn
+
+
+def gather2(a, b, idx, is_a, reverse=False, a_null=None, b_null=None):
+    n = idx.numel()
+    dst = torch.empty((n,) + tuple(a.shape[1:]), dtype=a.dtype, device=a.device)
+    dn = _null_out(None, n, a.device)
+    call("ysql_gather2", elem_bytes(a), _p(a), _p(a_null), _p(b), _p(b_null), _p(idx), _p(is_a), n, int(reverse),
+         _p(dst), _p(dn), _s())
+    return dst, dn
+
+
+def gather_pairs(src, pairs, side, n, src_null=None):
+    dst = torch.empty((n,) + tuple(src.shape[1:]), dtype=src.dtype, device=src.device)
+    dn = _null_out(src_null, n, src.device)
+    call("ysql_gather_pairs", elem_bytes(src), _p(src), _p(src_null), _p(pairs), side, n, _p(dst), _p(dn), _s())
+    return dst, dn
+
+
+def compact(src, mask, mapping, count, src_null=None):
+    n = mask.numel()
+    dst = torch.empty((count,) + tuple(src.shape[1:]), dtype=src.dtype, device=src.device)
+    dn = _null_out(src_null, count, src.device)
+    call("ysql_compact", elem_bytes(src), _p(src), _p(src_null), _p(mask), _p(mapping), n, _p(dst), _p(dn), _s())
+    return dst, dn
+
+
+def _cols(srcs, nulls, n):
+    outs, cols = [], (L.Col * max(1, len(srcs)))()
+    for i, s in enumerate(srcs):
+        d = torch.empty_like(s)
+        dn = torch.empty(n, dtype=torch.uint8, device=s.device)
+        sn = nulls[i] if nulls else None
+        cols[i] = L.Col(s.data_ptr(), sn.data_ptr() if sn is not None else None, d.data_ptr(), dn.data_ptr(),
+                        elem_bytes(s), 0)
+        outs.append((d, dn))
+    return cols, outs
+
+
+def compact_cols(mask, srcs, nulls=None):
+    """Fused mask -> scan -> compaction of every column.  Returns ([(dst, dst_null)], count)."""
+    n = mask.numel()
+    cols, outs = _cols(srcs, nulls, n)
+    cnt = torch.zeros(1, dtype=torch.int64, device=mask.device)
+    ws = workspace(ws_bytes("ysql_compact_ws_bytes", n), mask.device)
+    call("ysql_compact_cols", _p(mask), cols, len(srcs), n, _p(cnt), _p(ws), ws.numel(), _s())
+    c = int(cnt.item())
+    return [(d[:c], dn[:c]) for d, dn in outs], c
+
+
+def filter_compact(cmp, pred, scalar, srcs, nulls=None):
+    n = pred.numel()
+    cols, outs = _cols(srcs, nulls, n)
+    cnt = torch.zeros(1, dtype=torch.int64, device=pred.device)
+    ws = workspace(ws_bytes("ysql_compact_ws_bytes", n), pred.device)
+    call("ysql_filter_compact", dtype_code(pred), cmp, _p(pred), _p(scalar), cols, len(srcs), n, _p(cnt), _p(ws),
+         ws.numel(), _s())
+    c = int(cnt.item())
+    return [(d[:c], dn[:c]) for d, dn in outs], c
+
+
+# ---- A12 / A15 ---------------------------------------------------------
+def compare(cmp, lhs, rhs, lhs_scalar=False, rhs_scalar=False, negate=False, n=None):
+    n = n if n is not None else (rhs.shape[0] if lhs_scalar else lhs.shape[0])
+    dev = lhs.device
+    out = torch.empty(n, dtype=torch.uint8, device=dev)
+    dt = dtype_code(lhs if not lhs_scalar else rhs) if lhs.dim() == 2 or rhs.dim() == 2 else _DT[lhs.dtype]
+    call("ysql_compare", dt, cmp, _p(lhs), int(lhs_scalar), _p(rhs), int(rhs_scalar), int(negate), _p(out), n, _s())
+    return out
+
+
+def mask_logic(op, a, b, a_scalar=False, b_scalar=False, n=None):
+    n = n if n is not None else (b.numel() if a_scalar else a.numel())
+    out = torch.empty(n, dtype=torch.uint8, device=a.device)
+    call("ysql_mask_logic", op, _p(a), int(a_scalar), _p(b), int(b_scalar), _p(out), n, _s())
+    return out
+
+
+def arith(op, lhs, rhs, lhs_scalar=False, rhs_scalar=False, n=None):
+    n = n if n is not None else (rhs.numel() if lhs_scalar else lhs.numel())
+    out = torch.empty(n, dtype=lhs.dtype, device=lhs.device)
+    call("ysql_arith", _DT[lhs.dtype], op, _p(lhs), int(lhs_scalar), _p(rhs), int(rhs_scalar), _p(out), n, _s())
+    return out
+
+
+def cast(x, dst_dtype):
+    out = torch.empty(x.shape, dtype=dst_dtype, device=x.device)
+    call("ysql_cast", _DT[x.dtype], _DT[dst_dtype], _p(x), _p(out), x.numel(), _s())
+    return out
+
+
+# ---- A16 / A17 ---------------------------------------------------------
+ACC_DTYPE = torch.uint8  # accumulators are raw 32-byte structs
+
+
+def acc_new(dtype_c, agg, nacc=1, device="cuda"):
+    acc = torch.empty(32 * nacc, dtype=torch.uint8, device=device)
+    call("ysql_acc_init", dtype_c, agg, _p(acc), nacc, _s())
+    return acc
+
+
+def acc_read(acc):
+    """Host view of accumulators: list of dicts."""
+    raw = acc.cpu().numpy().tobytes()
+    out = []
+    for k in range(len(raw) // 32):
+        a = L.Acc.from_buffer_copy(raw[32 * k:32 * k + 32])
+        out.append({"i": a.i, "d": a.d, "f": a.f, "count": a.count})
+    return out
+
+
+def reduce(dtype_c, agg, x, acc, null=None, n=None):
+    n = x.numel() if n is None else n
+    ws = workspace(ws_bytes("ysql_reduce_ws_bytes", n), acc.device)
+    call("ysql_reduce", dtype_c, agg, _p(x), _p(null), n, _p(acc), _p(ws), ws.numel(), _s())
+    return acc
+
+
+def agg_finalize(dtype_c, agg, acc):
+    out = torch.zeros(8, dtype=torch.uint8, device=acc.device)
+    call("ysql_agg_finalize", dtype_c, agg, _p(acc), _p(out), _s())
+    return out
+
+
+def group_reduce(dtype_c, agg, gid, n_groups, x, acc, null=None):
+    call("ysql_group_reduce", dtype_c, agg, _p(gid), n_groups, _p(x), _p(null), gid.numel(), _p(acc), _s())
+    return acc
+
+
+# ---- A18-A21 -------------------------------------------------------------
+def join_pairs(cmp, left, right, cap=None):
+    """Returns (pairs[count, 2] int32, count).  Retries once with the exact capacity on overflow."""
+    Ln, Rn = left.shape[0], right.shape[0]
+    dt = dtype_code(left)
+    dev = left.device
+    ws = workspace(ws_bytes("ysql_join_ws_bytes", dt, cmp, Ln, Rn), dev)
+    cnt = torch.zeros(1, dtype=torch.int64, device=dev)
+    cap = max(Ln, 1024) if cap is None else cap
+    pairs = torch.empty((max(cap, 1), 2), dtype=torch.int32, device=dev)
+    call("ysql_join_pairs", dt, cmp, _p(left), Ln, _p(right), Rn, _p(pairs), cap, _p(cnt), _p(ws), ws.numel(), _s())
+    c = int(cnt.item())
+    if c > cap:
+        return join_pairs(cmp, left, right, cap=c)
+    return pairs[:c], c
+
+
+def pairs_intersect(p1, p2, cap=None):
+    n1, n2 = p1.shape[0], p2.shape[0]
+    dev = p1.device
+    cap = max(n1, 1) if cap is None else cap
+    ws = workspace(ws_bytes("ysql_pairs_intersect_ws_bytes", n1, n2), dev)
+    cnt = torch.zeros(1, dtype=torch.int64, device=dev)
+    out = torch.empty((max(cap, 1), 2), dtype=torch.int32, device=dev)
+    call("ysql_pairs_intersect", _p(p1), n1, _p(p2), n2, _p(out), cap, _p(cnt), _p(ws), ws.numel(), _s())
+    c = int(cnt.item())
+    return out[:c], c
+
+
+def cross_expand(src, Ln, Rn, is_left, src_null=None):
+    total = Ln * Rn
+    dst = torch.empty((total,) + tuple(src.shape[1:]), dtype=src.dtype, device=src.device)
+    dn = torch.empty(total, dtype=torch.uint8, device=src.device)
+    call("ysql_cross_expand", elem_bytes(src), _p(src), _p(src_null), Ln, Rn, int(is_left), _p(dst), _p(dn), _s())
+    return dst, dn
+
+
+def star_join_group_sum(dims, m1, m2, n_groups, sums=None, counts=None):
+    """dims: list of dicts {fk, rows, pass (uint8 or None), group (int32 or None), group_base, group_card}."""
+    dev = m1.device
+    arr = (L.StarDim * len(dims))()
+    for i, d in enumerate(dims):
+        arr[i] = L.StarDim(d["fk"].data_ptr(), d["rows"], d.get("group_card", 0),
+                           d["pass"].data_ptr() if d.get("pass") is not None else None,
+                           d["group"].data_ptr() if d.get("group") is not None else None,
+                           d.get("group_base", 0), 0)
+    if sums is None:
+        sums = torch.zeros(n_groups, dtype=torch.int64, device=dev)
+    if counts is None:
+        counts = torch.zeros(n_groups, dtype=torch.int64, device=dev)
+    call("ysql_star_join_group_sum", arr, len(dims), _p(m1), _p(m2), m1.numel(), _p(sums), _p(counts), n_groups, _s())
+    return sums, counts
