@@ -139,6 +139,27 @@ int ysql_radix_argsort(int key_dtype, int descending, void *keys_inout, uint32_t
 int ysql_radix_sort_pairs(int key_dtype, int descending, void *keys_inout, uint32_t *vals_inout,
                           uint64_t n, int begin_bit, int end_bit, void *ws, size_t ws_bytes,
                           ysql_stream_t stream);
+/* Out-of-place form: (keys_in, vals_in) -> (keys_out, vals_out).  keys_in
+ * may equal keys_out (in place) or be disjoint from it.  vals_in == NULL
+ * gives payload val_base + row (a run of a larger table carries its global
+ * row ids this way).  Saves the D2D copy that BoundRefSortExpression made
+ * before sorting (bound_ref_sort.hpp:35-44).                               */
+int ysql_radix_sort_ex(int key_dtype, int descending, const void *keys_in, const uint32_t *vals_in,
+                       uint32_t val_base, void *keys_out, uint32_t *vals_out, uint64_t n,
+                       int begin_bit, int end_bit, void *ws, size_t ws_bytes, ysql_stream_t stream);
+/* Stepwise form of the same sort, for callers that schedule or time the
+ * passes themselves (the reference's launcher loops over digit passes,
+ * radix_sort_kernel.cu:161-167): ysql_radix_begin builds the digit
+ * histograms of keys_in; then ysql_radix_pass for pass = 0 .. P-1,
+ * P = ceil((end_bit - begin_bit) / 8), each reading the previous pass's
+ * output (pass 0: keys_in, vals_in or val_base + row) into a distinct
+ * buffer.  All calls of one sort share `ws` and the stream.               */
+int ysql_radix_begin(int key_dtype, int descending, const void *keys_in, uint64_t n, int begin_bit,
+                     int end_bit, void *ws, size_t ws_bytes, ysql_stream_t stream);
+int ysql_radix_pass(int key_dtype, int descending, int pass, const void *keys_in,
+                    const uint32_t *vals_in, uint32_t val_base, void *keys_out, uint32_t *vals_out,
+                    uint64_t n, int begin_bit, int end_bit, void *ws, size_t ws_bytes,
+                    ysql_stream_t stream);
 
 /* ======================================================================
  * Inclusive scans (A4, A13)

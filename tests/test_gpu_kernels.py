@@ -93,6 +93,25 @@ def test_radix_sort_pairs_bits(ops, dev):
     assert np.array_equal(N(kt).view(np.uint32), keys[exp])
 
 
+@pytest.mark.parametrize("end_bit", [64, 63, 24, 20])
+def test_radix_sort_ex_out_of_place(ops, dev, end_bit):
+    r = rng(end_bit)
+    n = 123_457
+    keys = r.integers(0, 2**min(end_bit, 63), n, dtype=np.uint64)
+    kin = T(keys.view(np.int64), dev)
+    kout = torch.empty_like(kin)
+    vout = torch.empty(n, dtype=torch.int32, device=dev)
+    ops.radix_sort_ex(kin, kout, vout, val_base=1000, end_bit=end_bit)
+    exp = O.stable_argsort(keys)
+    assert np.array_equal(N(vout), (exp + 1000).astype(np.int32))
+    assert np.array_equal(N(kout).view(np.uint64), keys[exp])
+    assert np.array_equal(N(kin).view(np.uint64), keys)  # input untouched
+    # in place with the same pass count (odd counts go through the workspace)
+    ops.radix_sort_ex(kin, kin, vout, end_bit=end_bit)
+    assert np.array_equal(N(vout), exp.astype(np.int32))
+    assert np.array_equal(N(kin).view(np.uint64), keys[exp])
+
+
 def test_radix_all_equal_and_sorted(ops, dev):
     n = 50_000
     for keys in (np.full(n, 7, np.uint32), np.arange(n, dtype=np.uint32), np.arange(n, dtype=np.uint32)[::-1].copy()):
@@ -306,11 +325,17 @@ def test_gathers(ops, dev):
 
 def test_cross_expand(ops, dev):
     r = rng(8)
-    x = r.integers(0, 100, 37).astype(np.int32)
-    for is_left in (True, False):
-        d, _ = ops.cross_expand(T(x, dev), 37, 256 if not is_left else 37, is_left)
-        e, _ = O.cross_expand(x, 37, 256 if not is_left else 37, is_left)
-        assert np.array_equal(N(d), e)
+    left = r.integers(0, 100, 37).astype(np.int32)
+    right = r.integers(0, 100, 256).astype(np.int32)
+    nul = (r.random(256) < 0.3).astype(np.uint8)
+    d, _ = ops.cross_expand(T(left, dev), 37, 256, True)
+    assert np.array_equal(N(d), O.cross_expand(left, 37, 256, True)[0])
+    d, dn = ops.cross_expand(T(right, dev), 37, 256, False, src_null=T(nul, dev))
+    e, en = O.cross_expand(right, 37, 256, False, src_null=nul)
+    assert np.array_equal(N(d), e) and np.array_equal(N(dn), en)
+    s = r.integers(0, 128, (5, 256)).astype(np.uint8)
+    d, _ = ops.cross_expand(T(s, dev), 5, 3, True)
+    assert np.array_equal(N(d), np.repeat(s, 3, axis=0))
 
 
 # ---------------------------------------------------------------- A16/A17

@@ -62,6 +62,9 @@ _SIGS = {
     "ysql_radix_sort_ws_bytes": [I32, U64, SZP],
     "ysql_radix_argsort": [I32, I32, P, P, U64, I32, I32, P, C.c_size_t, P],
     "ysql_radix_sort_pairs": [I32, I32, P, P, U64, I32, I32, P, C.c_size_t, P],
+    "ysql_radix_sort_ex": [I32, I32, P, P, U32, P, P, U64, I32, I32, P, C.c_size_t, P],
+    "ysql_radix_begin": [I32, I32, P, U64, I32, I32, P, C.c_size_t, P],
+    "ysql_radix_pass": [I32, I32, I32, P, P, U32, P, P, U64, I32, I32, P, C.c_size_t, P],
     "ysql_scan_ws_bytes": [U64, SZP],
     "ysql_scan_u32_inclusive": [P, P, U64, P, C.c_size_t, P],
     "ysql_scan_mask_inclusive": [P, P, U64, P, C.c_size_t, P],

@@ -76,7 +76,7 @@ __global__ void k_copy_pairs(const K *ks, const uint32_t *vs, K *kd, uint32_t *v
 
 // ---------------------------------------------------------------------------
 // One digit pass.
-//   kin/vin -> kout/vout; vin == nullptr means the payload is the row index.
+//   kin/vin -> kout/vout; vin == nullptr means the payload is val_base + row.
 //   hist: the 256 counts of this digit over all keys.
 //   status: ntiles * 256 words of this pass, zeroed; status_next: the same
 //   area for the following pass, zeroed here tile by tile (it was last used
@@ -84,8 +84,9 @@ __global__ void k_copy_pairs(const K *ks, const uint32_t *vs, K *kd, uint32_t *v
 // ---------------------------------------------------------------------------
 template <typename K, int BLOCK, int KPT>
 __global__ __launch_bounds__(BLOCK) void k_onesweep(const K *__restrict__ kin, const uint32_t *__restrict__ vin,
-                                                    K *__restrict__ kout, uint32_t *__restrict__ vout, uint32_t n,
-                                                    int shift, uint32_t desc, const uint32_t *__restrict__ hist,
+                                                    uint32_t val_base, K *__restrict__ kout,
+                                                    uint32_t *__restrict__ vout, uint32_t n, int shift, uint32_t desc,
+                                                    const uint32_t *__restrict__ hist,
                                                     uint32_t *status, uint32_t *status_next, uint32_t *ctr) {
   static_assert(BLOCK >= kBins, "one thread per digit");
   constexpr int W = BLOCK / kWave;
@@ -117,7 +118,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const K *__restrict__ kin, c
     const uint32_t idx = wbase + i * kWave + lane;
     const bool valid = idx < n;
     key[i] = valid ? kin[idx] : (K)0;
-    val[i] = vin ? (valid ? vin[idx] : 0u) : idx;
+    val[i] = vin ? (valid ? vin[idx] : 0u) : val_base + idx;
   }
 
   // ---- rank inside the wave, in (item, lane) order
@@ -251,66 +252,95 @@ static size_t radix_ws_layout(uint64_t n, void *ws, size_t bytes, RadixWs *out) 
 }
 
 template <typename K>
-static int radix_sort_impl(int desc, K *keys, uint32_t *vals, bool iota, uint64_t n, int begin_bit, int end_bit,
-                           void *ws, size_t ws_bytes, hipStream_t s) {
-  constexpr int BLOCK = RadixTile<K>::BLOCK, KPT = RadixTile<K>::KPT, TILE = BLOCK * KPT;
+static int radix_check(const void *keys_in, uint64_t n, int begin_bit, int end_bit) {
   constexpr int KBITS = 8 * sizeof(K);
-  if (n == 0) return YSQL_OK;
-  if (!keys || !vals) return YSQL_EINVAL;
+  if (!keys_in) return YSQL_EINVAL;
   if (n >= (1ull << 30)) return YSQL_EINVAL;
   if (begin_bit < 0 || end_bit > KBITS || begin_bit >= end_bit) return YSQL_EINVAL;
-  RadixWs w;
-  if (!ws || ws_bytes < radix_ws_layout<K>(n, nullptr, 0, nullptr) || !radix_ws_layout<K>(n, ws, ws_bytes, &w))
-    return YSQL_EWORKSPACE;
-  const int passes = (end_bit - begin_bit + 7) / 8;
-  const uint32_t ntiles = (uint32_t)ceil_div(n, TILE);
-  const uint32_t dmask = desc ? 0xFFu : 0u;
-
-  // zero hist + counters + the first pass's status words
-  {
-    uint64_t words = 8 * kBins + 64;
-    k_zero_u32<<<1024, 256, 0, s>>>(w.hist, words);
-    YSQL_LAUNCH_CHECK();
-    k_zero_u32<<<1024, 256, 0, s>>>(w.st0, (uint64_t)ntiles * kBins);
-    YSQL_LAUNCH_CHECK();
-  }
-  {
-    uint64_t g = ceil_div(n, 256 * 16);
-    uint64_t cap = (uint64_t)grid_cap() * 2;
-    if (g > cap) g = cap;
-    if (g < 1) g = 1;
-    k_radix_hist<K><<<(unsigned)g, 256, 0, s>>>(keys, n, begin_bit, passes, dmask, w.hist);
-    YSQL_LAUNCH_CHECK();
-  }
-  K *kbuf[2] = {keys, (K *)w.kalt};
-  uint32_t *vbuf[2] = {vals, w.valt};
-  uint32_t *st[2] = {w.st0, w.st1};
-  for (int p = 0; p < passes; p++) {
-    const int src = p & 1, dst = src ^ 1;
-    const uint32_t *vin = (p == 0 && iota) ? nullptr : vbuf[src];
-    uint32_t *snext = (p + 1 < passes) ? st[dst] : nullptr;
-    k_onesweep<K, BLOCK, KPT><<<ntiles, BLOCK, 0, s>>>(kbuf[src], vin, kbuf[dst], vbuf[dst], (uint32_t)n,
-                                                       begin_bit + 8 * p, dmask, w.hist + p * kBins, st[src], snext,
-                                                       w.ctr + p);
-    YSQL_LAUNCH_CHECK();
-  }
-  if (passes & 1) {
-    uint64_t g = ceil_div(n, 256 * 4);
-    uint64_t cap = (uint64_t)grid_cap() * 4;
-    if (g > cap) g = cap;
-    k_copy_pairs<K><<<(unsigned)g, 256, 0, s>>>(kbuf[1], vbuf[1], kbuf[0], vbuf[0], n);
-    YSQL_LAUNCH_CHECK();
-  }
   return YSQL_OK;
 }
 
-static int radix_dispatch(int key_dtype, int desc, void *keys, uint32_t *vals, bool iota, uint64_t n, int begin_bit,
-                          int end_bit, void *ws, size_t ws_bytes, hipStream_t s) {
-  if (key_dtype == YSQL_UKEY32)
-    return radix_sort_impl<uint32_t>(desc, (uint32_t *)keys, vals, iota, n, begin_bit, end_bit, ws, ws_bytes, s);
-  if (key_dtype == YSQL_UKEY64)
-    return radix_sort_impl<uint64_t>(desc, (uint64_t *)keys, vals, iota, n, begin_bit, end_bit, ws, ws_bytes, s);
-  return YSQL_EINVAL;
+static inline int radix_passes(int begin_bit, int end_bit) { return (end_bit - begin_bit + 7) / 8; }
+
+// Zeroes the histograms, tile counters and first status area, then builds
+// the histograms of every digit from keys_in.
+template <typename K>
+static int radix_begin_t(int desc, const K *keys_in, uint64_t n, int begin_bit, int end_bit, const RadixWs &w,
+                         hipStream_t s) {
+  constexpr int TILE = RadixTile<K>::BLOCK * RadixTile<K>::KPT;
+  const uint32_t ntiles = (uint32_t)ceil_div(n, TILE);
+  k_zero_u32<<<64, 256, 0, s>>>(w.hist, 8 * kBins + 64);
+  YSQL_LAUNCH_CHECK();
+  uint64_t zg = ceil_div((uint64_t)ntiles * kBins, 256 * 8);
+  if (zg > 2048) zg = 2048;
+  if (zg < 1) zg = 1;
+  k_zero_u32<<<(unsigned)zg, 256, 0, s>>>(w.st0, (uint64_t)ntiles * kBins);
+  YSQL_LAUNCH_CHECK();
+  uint64_t g = ceil_div(n, 256 * 16);
+  const uint64_t cap = (uint64_t)grid_cap() * 2;
+  if (g > cap) g = cap;
+  if (g < 1) g = 1;
+  k_radix_hist<K><<<(unsigned)g, 256, 0, s>>>(keys_in, n, begin_bit, radix_passes(begin_bit, end_bit),
+                                              desc ? 0xFFu : 0u, w.hist);
+  YSQL_LAUNCH_CHECK();
+  return YSQL_OK;
+}
+
+// One digit pass p of `passes`: (kin, vin or val_base + row) -> (kout, vout).
+template <typename K>
+static int radix_pass_t(int desc, int p, int passes, const K *kin, const uint32_t *vin, uint32_t val_base, K *kout,
+                        uint32_t *vout, uint64_t n, int begin_bit, const RadixWs &w, hipStream_t s) {
+  constexpr int BLOCK = RadixTile<K>::BLOCK, KPT = RadixTile<K>::KPT, TILE = BLOCK * KPT;
+  const uint32_t ntiles = (uint32_t)ceil_div(n, TILE);
+  uint32_t *st = (p & 1) ? w.st1 : w.st0;
+  uint32_t *snext = (p + 1 < passes) ? ((p & 1) ? w.st0 : w.st1) : nullptr;
+  k_onesweep<K, BLOCK, KPT><<<ntiles, BLOCK, 0, s>>>(kin, vin, val_base, kout, vout, (uint32_t)n, begin_bit + 8 * p,
+                                                     desc ? 0xFFu : 0u, w.hist + p * kBins, st, snext, w.ctr + p);
+  YSQL_LAUNCH_CHECK();
+  return YSQL_OK;
+}
+
+// Full sort keys_in/vals_in -> keys_out/vals_out.  keys_in may equal
+// keys_out (in place); vals_in == nullptr gives payload val_base + row.
+template <typename K>
+static int radix_sort_t(int desc, const K *kin, const uint32_t *vin, uint32_t vbase, K *kout, uint32_t *vout,
+                        uint64_t n, int begin_bit, int end_bit, void *ws, size_t ws_bytes, hipStream_t s) {
+  if (n == 0) return YSQL_OK;
+  int rc = radix_check<K>(kin, n, begin_bit, end_bit);
+  if (rc) return rc;
+  if (!kout || !vout) return YSQL_EINVAL;
+  RadixWs w;
+  if (!ws || ws_bytes < radix_ws_layout<K>(n, nullptr, 0, nullptr) || !radix_ws_layout<K>(n, ws, ws_bytes, &w))
+    return YSQL_EWORKSPACE;
+  const int P = radix_passes(begin_bit, end_bit);
+  rc = radix_begin_t<K>(desc, kin, n, begin_bit, end_bit, w, s);
+  if (rc) return rc;
+  K *kalt = (K *)w.kalt;
+  uint32_t *valt = w.valt;
+  // With an even pass count, or distinct in/out buffers, the passes can
+  // alternate so that the last one lands in the output.  An odd count in
+  // place runs through the workspace and copies back at the end.
+  const bool aliased = ((const void *)kin == (const void *)kout) || (vin && vin == vout);
+  const bool copy_back = (P & 1) && aliased;
+  const K *ksrc = kin;
+  const uint32_t *vsrc = vin;
+  for (int p = 0; p < P; p++) {
+    const bool to_out = copy_back ? (p & 1) : (((P - 1 - p) & 1) == 0);
+    K *kd = to_out ? kout : kalt;
+    uint32_t *vd = to_out ? vout : valt;
+    rc = radix_pass_t<K>(desc, p, P, ksrc, vsrc, vbase, kd, vd, n, begin_bit, w, s);
+    if (rc) return rc;
+    ksrc = kd;
+    vsrc = vd;
+  }
+  if (copy_back) {
+    uint64_t g = ceil_div(n, 256 * 4);
+    const uint64_t cap = (uint64_t)grid_cap() * 4;
+    if (g > cap) g = cap;
+    k_copy_pairs<K><<<(unsigned)g, 256, 0, s>>>(kalt, valt, kout, vout, n);
+    YSQL_LAUNCH_CHECK();
+  }
+  return YSQL_OK;
 }
 
 }  // namespace ysql
@@ -327,16 +357,85 @@ int ysql_radix_sort_ws_bytes(int key_dtype, uint64_t n, size_t *bytes) {
   return YSQL_OK;
 }
 
+int ysql_radix_sort_ex(int key_dtype, int descending, const void *keys_in, const uint32_t *vals_in, uint32_t val_base,
+                       void *keys_out, uint32_t *vals_out, uint64_t n, int begin_bit, int end_bit, void *ws,
+                       size_t ws_bytes, ysql_stream_t stream) {
+  hipStream_t s = as_stream(stream);
+  if (key_dtype == YSQL_UKEY32)
+    return radix_sort_t<uint32_t>(descending, (const uint32_t *)keys_in, vals_in, val_base, (uint32_t *)keys_out,
+                                  vals_out, n, begin_bit, end_bit, ws, ws_bytes, s);
+  if (key_dtype == YSQL_UKEY64)
+    return radix_sort_t<uint64_t>(descending, (const uint64_t *)keys_in, vals_in, val_base, (uint64_t *)keys_out,
+                                  vals_out, n, begin_bit, end_bit, ws, ws_bytes, s);
+  return YSQL_EINVAL;
+}
+
 int ysql_radix_argsort(int key_dtype, int descending, void *keys_inout, uint32_t *perm_out, uint64_t n, int begin_bit,
                        int end_bit, void *ws, size_t ws_bytes, ysql_stream_t stream) {
-  return radix_dispatch(key_dtype, descending, keys_inout, perm_out, true, n, begin_bit, end_bit, ws, ws_bytes,
-                        as_stream(stream));
+  if (n > 0 && !perm_out) return YSQL_EINVAL;
+  return ysql_radix_sort_ex(key_dtype, descending, keys_inout, nullptr, 0, keys_inout, perm_out, n, begin_bit, end_bit,
+                            ws, ws_bytes, stream);
 }
 
 int ysql_radix_sort_pairs(int key_dtype, int descending, void *keys_inout, uint32_t *vals_inout, uint64_t n,
                           int begin_bit, int end_bit, void *ws, size_t ws_bytes, ysql_stream_t stream) {
-  return radix_dispatch(key_dtype, descending, keys_inout, vals_inout, false, n, begin_bit, end_bit, ws, ws_bytes,
-                        as_stream(stream));
+  if (n > 0 && !vals_inout) return YSQL_EINVAL;
+  return ysql_radix_sort_ex(key_dtype, descending, keys_inout, vals_inout, 0, keys_inout, vals_inout, n, begin_bit,
+                            end_bit, ws, ws_bytes, stream);
+}
+
+int ysql_radix_begin(int key_dtype, int descending, const void *keys_in, uint64_t n, int begin_bit, int end_bit,
+                     void *ws, size_t ws_bytes, ysql_stream_t stream) {
+  if (n == 0) return YSQL_OK;
+  hipStream_t s = as_stream(stream);
+  RadixWs w;
+  if (key_dtype == YSQL_UKEY32) {
+    int rc = radix_check<uint32_t>(keys_in, n, begin_bit, end_bit);
+    if (rc) return rc;
+    if (!ws || ws_bytes < radix_ws_layout<uint32_t>(n, nullptr, 0, nullptr) ||
+        !radix_ws_layout<uint32_t>(n, ws, ws_bytes, &w))
+      return YSQL_EWORKSPACE;
+    return radix_begin_t<uint32_t>(descending, (const uint32_t *)keys_in, n, begin_bit, end_bit, w, s);
+  }
+  if (key_dtype == YSQL_UKEY64) {
+    int rc = radix_check<uint64_t>(keys_in, n, begin_bit, end_bit);
+    if (rc) return rc;
+    if (!ws || ws_bytes < radix_ws_layout<uint64_t>(n, nullptr, 0, nullptr) ||
+        !radix_ws_layout<uint64_t>(n, ws, ws_bytes, &w))
+      return YSQL_EWORKSPACE;
+    return radix_begin_t<uint64_t>(descending, (const uint64_t *)keys_in, n, begin_bit, end_bit, w, s);
+  }
+  return YSQL_EINVAL;
+}
+
+int ysql_radix_pass(int key_dtype, int descending, int pass, const void *keys_in, const uint32_t *vals_in,
+                    uint32_t val_base, void *keys_out, uint32_t *vals_out, uint64_t n, int begin_bit, int end_bit,
+                    void *ws, size_t ws_bytes, ysql_stream_t stream) {
+  if (n == 0) return YSQL_OK;
+  hipStream_t s = as_stream(stream);
+  if (!keys_out || !vals_out || keys_in == keys_out) return YSQL_EINVAL;
+  const int P = radix_passes(begin_bit, end_bit);
+  if (pass < 0 || pass >= P) return YSQL_EINVAL;
+  RadixWs w;
+  if (key_dtype == YSQL_UKEY32) {
+    int rc = radix_check<uint32_t>(keys_in, n, begin_bit, end_bit);
+    if (rc) return rc;
+    if (!ws || ws_bytes < radix_ws_layout<uint32_t>(n, nullptr, 0, nullptr) ||
+        !radix_ws_layout<uint32_t>(n, ws, ws_bytes, &w))
+      return YSQL_EWORKSPACE;
+    return radix_pass_t<uint32_t>(descending, pass, P, (const uint32_t *)keys_in, vals_in, val_base,
+                                  (uint32_t *)keys_out, vals_out, n, begin_bit, w, s);
+  }
+  if (key_dtype == YSQL_UKEY64) {
+    int rc = radix_check<uint64_t>(keys_in, n, begin_bit, end_bit);
+    if (rc) return rc;
+    if (!ws || ws_bytes < radix_ws_layout<uint64_t>(n, nullptr, 0, nullptr) ||
+        !radix_ws_layout<uint64_t>(n, ws, ws_bytes, &w))
+      return YSQL_EWORKSPACE;
+    return radix_pass_t<uint64_t>(descending, pass, P, (const uint64_t *)keys_in, vals_in, val_base,
+                                  (uint64_t *)keys_out, vals_out, n, begin_bit, w, s);
+  }
+  return YSQL_EINVAL;
 }
 
 }  // extern "C"

@@ -287,12 +287,12 @@ __global__ __launch_bounds__(kScanBlock) void k_compact(const uint8_t *mask, con
   if (tile == ntiles - 1 && t == 0 && count_out) *count_out = prefix + tot;
 }
 
-static bool cols_ok(const ysql_col *cols, int ncols) {
+static bool cols_ok(const ysql_col *cols, int ncols, bool need_ptrs) {
   if (ncols < 0 || ncols > YSQL_MAX_COLS || (ncols > 0 && !cols)) return false;
   for (int i = 0; i < ncols; i++) {
     uint32_t eb = cols[i].elem_bytes;
     if (!(eb == 1 || eb == 2 || eb == 4 || eb == 8 || eb == YSQL_STRING_BYTES)) return false;
-    if (!cols[i].src || !cols[i].dst) return false;
+    if (need_ptrs && (!cols[i].src || !cols[i].dst)) return false;
   }
   return true;
 }
@@ -322,7 +322,7 @@ static void launch_pred(int cmp, dim3 g, hipStream_t s, const PT *pred, const PT
 static int compact_impl(int mode, int dtype, int cmp, const uint8_t *mask, const void *pred, const void *scalar,
                         const ysql_col *cols, int ncols, uint64_t n, uint64_t *count_out, void *ws, size_t ws_bytes,
                         hipStream_t s) {
-  if (!cols_ok(cols, ncols)) return YSQL_EINVAL;
+  if (!cols_ok(cols, ncols, n > 0)) return YSQL_EINVAL;
   if (n >= (1ull << 40)) return YSQL_EINVAL;
   if (n == 0) {
     if (count_out) {

@@ -86,6 +86,18 @@ def radix_sort_pairs(keys, vals, descending=False, begin_bit=0, end_bit=None, ws
     return keys, vals
 
 
+def radix_sort_ex(keys_in, keys_out, vals_out, vals_in=None, val_base=0, descending=False, begin_bit=0,
+                  end_bit=None, ws=None):
+    n = keys_in.numel()
+    kdt = _kdt(keys_in)
+    end_bit = 8 * keys_in.element_size() if end_bit is None else end_bit
+    if ws is None:
+        ws = workspace(ws_bytes("ysql_radix_sort_ws_bytes", kdt, n), keys_in.device)
+    call("ysql_radix_sort_ex", kdt, int(descending), _p(keys_in), _p(vals_in), val_base, _p(keys_out), _p(vals_out), n,
+         begin_bit, end_bit, _p(ws), ws.numel(), _s())
+    return keys_out, vals_out
+
+
 # ---- A4 / A13 -----------------------------------------------------------
 def scan_u32_inclusive(x):
     n = x.numel()
