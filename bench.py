@@ -39,6 +39,8 @@ def parse():
     ap.add_argument("--rows", type=int, default=N_ROWS)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-verify", action="store_true")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="run one plain 1.6 GB device copy first (calibrates HBM byte counters in a profile)")
     return ap.parse_args()
 
 
@@ -207,6 +209,8 @@ def main():
 
     dev = torch.device("cuda:0")
     keys = gen_keys(args.rows, dev)
+    if args.calib_copy:
+        torch.empty_like(keys).copy_(keys)
     torch.cuda.synchronize()
     job = MultiRunSort(keys, RUN_ROWS)
     for _ in range(args.warmup):
@@ -233,6 +237,19 @@ def main():
     if not args.no_verify:
         ko, vo = job.result()
         verify(keys, ko, vo)
+
+    # attainable HBM bandwidth: a plain 1.6 GB device copy (read + write)
+    dst = torch.empty_like(keys)
+    copy_ms = []
+    for _ in range(3):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        dst.copy_(keys)
+        b.record()
+        torch.cuda.synchronize()
+        copy_ms.append(a.elapsed_time(b))
+    copy_gbs = 2 * keys.numel() * 8 / (min(copy_ms) * 1e-3) / 1e9
+    del dst
 
     traffic = None
     tf = os.path.join(ROOT, "profiles", "traffic.json")
@@ -281,6 +298,8 @@ def main():
                                                            "avg_ms": sum(merge_ms) / len(merge_ms)},
             "whole_job": {"algorithmic_GB": alg / 1e9, "GB_s": alg / (ms * 1e-3) / 1e9,
                           "frac": alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
+            "device_copy": {"GB_s": copy_gbs, "frac": copy_gbs / HBM_PEAK_GBS,
+                            "what": "torch copy_ of 1.6 GB (read + write), best of 3: attainable HBM rate"},
         },
         "cpu_baseline": cpu,
         "verified": not args.no_verify,
