@@ -287,7 +287,7 @@ def main():
             "unit": "GB/s",
             "frac": pass_gbs / HBM_PEAK_GBS,
             "traffic": traffic,
-            "kernel": "k_onesweep<uint64,256,16> (one 8-bit digit pass of a 2^25-row run)",
+            "kernel": "k_onesweep<uint64,512,16> (one 8-bit digit pass of a 2^25-row run)",
             "bytes_per_launch": 2 * RUN_ROWS * 12,
             "avg_launch_ms": sum(pass_ms) / len(pass_ms),
         },
