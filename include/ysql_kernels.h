@@ -119,6 +119,13 @@ int ysql_sortkey_encode(int dtype, int descending, const void *in, void *out, ui
  * radix_sort_utils.cu:12-37). */
 int ysql_sortkey_decode(int dtype, int descending, const void *in, void *out, uint64_t n,
                         ysql_stream_t stream);
+/* Appends one more ORDER BY column to a composite 64-bit key:
+ *     key[i] = (key[i] << w) | encode(in[i])   (w = 32 for INT/FLOAT, 64 for
+ * DATETIME), so an ascending stable sort of the composite orders rows by
+ * the first column, then the second, ... (the reference sorted on orders[0]
+ * only, order_operator.hpp:277).                                           */
+int ysql_sortkey_pack(int dtype, int descending, const void *in, uint64_t n, uint64_t *key_inout,
+                      ysql_stream_t stream);
 
 /* ======================================================================
  * LSD radix sort, 8-bit digits (A2-A6)
@@ -284,6 +291,14 @@ int ysql_agg_finalize(int dtype, int agg, const ysql_acc *acc, void *out, ysql_s
 int ysql_group_reduce(int dtype, int agg, const uint32_t *group_id, uint32_t n_groups,
                       const void *in, const uint8_t *null_opt, uint64_t n, ysql_acc *acc_inout,
                       ysql_stream_t stream);
+/* Dense group ids over up to 4 int32 columns (host array of device
+ * pointers; no reference function):
+ *     gid[i] = sum_c (cols[c][i] - base[c]) * stride_c,
+ * strides in column order (last column stride 1); a value outside
+ * [base[c], base[c] + card[c]) gives UINT32_MAX, which ysql_group_reduce
+ * skips.                                                                   */
+int ysql_group_ids(const int32_t *const *cols, const int32_t *base, const uint32_t *card, int ncols,
+                   uint64_t n, uint32_t *gid_out, ysql_stream_t stream);
 
 /* ======================================================================
  * Joins (A18-A21)

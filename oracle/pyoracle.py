@@ -280,3 +280,5 @@ def cpu_multirun_sort_u64(keys, run_rows, threads=1):
     ov = np.empty(keys.size, dtype=np.uint32)
     _chk(lib.orc_cpu_multirun_sort_u64(_p(keys), keys.size, run_rows, threads, _p(ok), _p(ov)))
     return ok, ov
+
+LT, LE, GT, GE, EQ, NE = 0, 1, 2, 3, 4, 5

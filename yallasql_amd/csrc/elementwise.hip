@@ -64,6 +64,14 @@ __global__ void k_xor64(const uint64_t *in, uint64_t *out, uint64_t n, uint64_t 
   GRID_STRIDE(i, n) out[i] = in[i] ^ x;
 }
 
+__global__ void k_pack32(const uint32_t *in, uint64_t *key, uint64_t n, int is_float, uint32_t flip) {
+  GRID_STRIDE(i, n) {
+    uint32_t b = in[i];
+    uint32_t k = is_float ? ((b & 0x80000000u) ? ~b : (b ^ 0x80000000u)) : (b ^ 0x80000000u);
+    key[i] = (key[i] << 32) | (uint64_t)(k ^ flip);
+  }
+}
+
 static int encode_impl(int dtype, int desc, const void *in, void *out, uint64_t n, hipStream_t s, bool decode) {
   if (n == 0) return YSQL_OK;
   if (!in || !out) return YSQL_EINVAL;
@@ -296,6 +304,23 @@ int ysql_sortkey_encode(int dtype, int descending, const void *in, void *out, ui
 }
 int ysql_sortkey_decode(int dtype, int descending, const void *in, void *out, uint64_t n, ysql_stream_t stream) {
   return encode_impl(dtype, descending, in, out, n, as_stream(stream), true);
+}
+int ysql_sortkey_pack(int dtype, int descending, const void *in, uint64_t n, uint64_t *key_inout,
+                      ysql_stream_t stream) {
+  if (n == 0) return YSQL_OK;
+  if (!in || !key_inout) return YSQL_EINVAL;
+  hipStream_t s = as_stream(stream);
+  if (dtype == YSQL_INT || dtype == YSQL_FLOAT) {
+    k_pack32<<<ew_grid(n), kEwBlock, 0, s>>>((const uint32_t *)in, key_inout, n, dtype == YSQL_FLOAT,
+                                             descending ? 0xFFFFFFFFu : 0u);
+  } else if (dtype == YSQL_DATETIME) {
+    k_xor64<<<ew_grid(n), kEwBlock, 0, s>>>((const uint64_t *)in, key_inout, n,
+                                            0x8000000000000000ull ^ (descending ? ~0ull : 0ull));
+  } else {
+    return YSQL_EINVAL;
+  }
+  YSQL_LAUNCH_CHECK();
+  return YSQL_OK;
 }
 
 int ysql_compare(int dtype, int cmp, const void *lhs, int lhs_is_scalar, const void *rhs, int rhs_is_scalar,

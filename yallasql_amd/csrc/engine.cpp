@@ -1,0 +1,831 @@
+// engine.cpp — operators and expressions of the host engine.
+#include "engine.hpp"
+
+#include <algorithm>
+#include <cstring>
+
+namespace YallaSQL {
+
+void hipCheck(hipError_t e, const char *what) {
+  if (e != hipSuccess) throw EngineError(std::string(what) + ": " + hipGetErrorString(e), YSQL_EHIP);
+}
+void abiCheck(int rc, const char *what) {
+  if (rc != YSQL_OK) throw EngineError(std::string(what) + " returned " + std::to_string(rc) + " (" + ysql_last_error() + ")", rc);
+}
+
+DeviceBuffer::DeviceBuffer(size_t n, hipStream_t s) : bytes(n), stream(s) {
+  hipCheck(hipMallocAsync(&ptr, n ? n : 1, s), "hipMallocAsync");
+}
+DeviceBuffer::~DeviceBuffer() {
+  if (owned && ptr) (void)hipFreeAsync(ptr, stream);
+}
+
+static BufPtr newBuf(size_t n, hipStream_t s) { return std::make_shared<DeviceBuffer>(n, s); }
+
+size_t Batch::totalBytes() const {
+  size_t b = 0;
+  for (size_t c = 0; c < columns.size(); c++) b += batchSize * (columns[c]->bytes + (nulls[c] ? 1 : 0));
+  return b;
+}
+
+std::unique_ptr<Batch> Batch::slice(size_t start, size_t rows) const {
+  auto v = std::make_unique<Batch>(*this);
+  for (auto &o : v->offset) o += start;
+  v->batchSize = rows;
+  return v;
+}
+
+// ---------------------------------------------------------------- cache
+BatchID CacheManager::putBatch(std::unique_ptr<Batch> b) {
+  if (!b) throw EngineError("Invalid batch provided.");
+  const size_t sz = b->totalBytes();
+  if (budget_ && bytes_ + sz > budget_)
+    throw EngineError("HBM budget exceeded: " + std::to_string(bytes_ + sz) + " > " + std::to_string(budget_),
+                      YSQL_EOVERFLOW);
+  bytes_ += sz;
+  peak_ = std::max(peak_, bytes_);
+  const BatchID id = next_++;
+  cache_[id] = std::move(b);
+  return id;
+}
+std::unique_ptr<Batch> CacheManager::getBatch(BatchID id) {
+  auto it = cache_.find(id);
+  if (it == cache_.end()) throw EngineError("unknown batch id " + std::to_string(id));
+  auto b = std::move(it->second);
+  cache_.erase(it);
+  bytes_ -= b->totalBytes();
+  return b;
+}
+Batch &CacheManager::refBatch(BatchID id) {
+  auto it = cache_.find(id);
+  if (it == cache_.end()) throw EngineError("unknown batch id " + std::to_string(id));
+  return *it->second;
+}
+
+// ---------------------------------------------------------------- context
+ExecContext::ExecContext(size_t budget, hipStream_t stream) : cache_(budget) {
+  if (stream) {
+    stream_ = stream;
+  } else {
+    hipCheck(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+    own_stream_ = true;
+  }
+  hipCheck(hipMalloc(&dcount_, 64), "hipMalloc");
+  hipCheck(hipHostMalloc(&hcount_, 64, hipHostMallocDefault), "hipHostMalloc");
+}
+ExecContext::~ExecContext() {
+  (void)hipStreamSynchronize(stream_);
+  if (ws_) (void)hipFreeAsync(ws_, stream_);
+  (void)hipStreamSynchronize(stream_);
+  (void)hipFree(dcount_);
+  (void)hipHostFree(hcount_);
+  if (own_stream_) (void)hipStreamDestroy(stream_);
+}
+void *ExecContext::workspace(size_t bytes) {
+  if (bytes > ws_bytes_) {
+    if (ws_) hipCheck(hipFreeAsync(ws_, stream_), "hipFreeAsync");
+    ws_bytes_ = std::max(bytes, ws_bytes_ + ws_bytes_ / 2) + 256;
+    hipCheck(hipMallocAsync(&ws_, ws_bytes_, stream_), "hipMallocAsync");
+  }
+  return ws_;
+}
+uint64_t *ExecContext::deviceCounter() { return dcount_; }
+uint64_t ExecContext::readCounter() {
+  hipCheck(hipMemcpyAsync(hcount_, dcount_, 8, hipMemcpyDeviceToHost, stream_), "hipMemcpyAsync");
+  hipCheck(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  return *hcount_;
+}
+void ExecContext::sync() { hipCheck(hipStreamSynchronize(stream_), "hipStreamSynchronize"); }
+
+// ---------------------------------------------------------------- expressions
+DataType BoundRefExpression::returnType(const std::vector<ColumnPtr> &in) const {
+  if (idx < 0 || (size_t)idx >= in.size()) throw EngineError("column index out of range: " + std::to_string(idx));
+  return in[idx]->type;
+}
+ExprValue BoundRefExpression::evaluate(const Batch &b, ExecContext &) {
+  if (idx < 0 || (size_t)idx >= b.numCols()) throw EngineError("column index out of range: " + std::to_string(idx));
+  ExprValue v;
+  v.buf = b.data[idx];
+  v.offset = b.offset[idx];
+  v.nullbuf = b.nulls[idx];
+  v.rows = b.batchSize;
+  v.type = b.columns[idx]->type;
+  return v;
+}
+
+ExprValue BoundValueExpression::evaluate(const Batch &, ExecContext &ctx) {
+  if (!dev) {
+    dev = newBuf(8, ctx.stream());
+    const void *src = type == DataType::FLOAT ? (const void *)&fval : (const void *)&ival;
+    size_t n = type == DataType::FLOAT || type == DataType::INT ? 4 : 8;
+    int32_t i32 = (int32_t)ival;
+    if (type == DataType::INT) src = &i32;
+    hipCheck(hipMemcpyAsync(dev->ptr, src, n, hipMemcpyHostToDevice, ctx.stream()), "hipMemcpyAsync");
+    ctx.sync();  // the host source is a local
+  }
+  ExprValue v;
+  v.buf = dev;
+  v.rows = 1;
+  v.type = type;
+  v.scalar = true;
+  return v;
+}
+
+// A constant compared or combined with a column of another numeric type is
+// converted on the host (the reference relied on DuckDB's literal casts).
+static void coerce(Expression *e, DataType want) {
+  auto *bv = dynamic_cast<BoundValueExpression *>(e);
+  if (!bv || bv->type == want) return;
+  if (want == DataType::FLOAT) bv->fval = bv->type == DataType::FLOAT ? bv->fval : (float)bv->ival;
+  else bv->ival = bv->type == DataType::FLOAT ? (int64_t)bv->fval : bv->ival;
+  bv->type = want;
+  bv->dev.reset();
+}
+
+// Derived values get their own copy of the source's null bytes, so data and
+// nulls share one offset (0) like every other batch column.
+static BufPtr copyNulls(const ExprValue &v, ExecContext &ctx) {
+  if (!v.nullbuf || v.scalar) return nullptr;
+  auto nb = newBuf(v.rows, ctx.stream());
+  hipCheck(hipMemcpyAsync(nb->ptr, v.nul(), v.rows, hipMemcpyDeviceToDevice, ctx.stream()), "hipMemcpyAsync");
+  return nb;
+}
+
+static size_t outRows(const ExprValue &a, const ExprValue &b) {
+  if (a.scalar && b.scalar) return 1;
+  return a.scalar ? b.rows : a.rows;
+}
+
+ExprValue ComparisonExpression::evaluate(const Batch &b, ExecContext &ctx) {
+  std::vector<ColumnPtr> schema = b.columns;
+  if (dynamic_cast<BoundValueExpression *>(rhs.get())) coerce(rhs.get(), lhs->returnType(schema));
+  if (dynamic_cast<BoundValueExpression *>(lhs.get())) coerce(lhs.get(), rhs->returnType(schema));
+  ExprValue l = lhs->evaluate(b, ctx), r = rhs->evaluate(b, ctx);
+  if (l.type != r.type) throw EngineError("comparison of different types");
+  ExprValue out;
+  out.rows = outRows(l, r);
+  out.mask = true;
+  out.scalar = l.scalar && r.scalar;
+  out.buf = newBuf(out.rows, ctx.stream());
+  out.nullbuf = copyNulls(l.scalar ? r : l, ctx);
+  abiCheck(ysql_compare(abiType(l.type), cmp, l.ptr(), l.scalar, r.ptr(), r.scalar, 0, (uint8_t *)out.buf->ptr,
+                        out.rows, ctx.stream()),
+           "ysql_compare");
+  return out;
+}
+
+ExprValue ConjunctionExpression::evaluate(const Batch &b, ExecContext &ctx) {
+  ExprValue l = lhs->evaluate(b, ctx), r = rhs->evaluate(b, ctx);
+  if (!l.mask || !r.mask) throw EngineError("AND/OR of non-boolean operands");
+  ExprValue out;
+  out.rows = outRows(l, r);
+  out.mask = true;
+  out.scalar = l.scalar && r.scalar;
+  out.buf = newBuf(out.rows, ctx.stream());
+  abiCheck(ysql_mask_logic(op, (const uint8_t *)l.ptr(), l.scalar, (const uint8_t *)r.ptr(), r.scalar,
+                           (uint8_t *)out.buf->ptr, out.rows, ctx.stream()),
+           "ysql_mask_logic");
+  return out;
+}
+
+ExprValue BoundFuncExpression::evaluate(const Batch &b, ExecContext &ctx) {
+  std::vector<ColumnPtr> schema = b.columns;
+  if (dynamic_cast<BoundValueExpression *>(rhs.get())) coerce(rhs.get(), lhs->returnType(schema));
+  if (dynamic_cast<BoundValueExpression *>(lhs.get())) coerce(lhs.get(), rhs->returnType(schema));
+  ExprValue l = lhs->evaluate(b, ctx), r = rhs->evaluate(b, ctx);
+  if (l.type != r.type) throw EngineError("arithmetic on different types");
+  ExprValue out;
+  out.rows = outRows(l, r);
+  out.type = l.type;
+  out.scalar = l.scalar && r.scalar;
+  out.buf = newBuf(out.rows * getDataTypeNumBytes(l.type), ctx.stream());
+  out.nullbuf = copyNulls(l.scalar ? r : l, ctx);  // bound_func.hpp: nulls of the non-constant operand
+  abiCheck(ysql_arith(abiType(l.type), op, l.ptr(), l.scalar, r.ptr(), r.scalar, out.buf->ptr, out.rows,
+                      ctx.stream()),
+           "ysql_arith");
+  return out;
+}
+
+ExprValue CastExpression::evaluate(const Batch &b, ExecContext &ctx) {
+  ExprValue c = child->evaluate(b, ctx);
+  ExprValue out;
+  out.rows = c.rows;
+  out.type = type;
+  out.scalar = c.scalar;
+  out.nullbuf = copyNulls(c, ctx);
+  out.buf = newBuf(c.rows * getDataTypeNumBytes(type), ctx.stream());
+  abiCheck(ysql_cast(c.mask ? YSQL_BOOL : abiType(c.type), abiType(type), c.ptr(), out.buf->ptr, c.rows, ctx.stream()),
+           "ysql_cast");
+  return out;
+}
+
+// ---------------------------------------------------------------- scan
+TableScanOperator::TableScanOperator(ExecContext &ctx, std::shared_ptr<Table> t, size_t batch_rows)
+    : Operator(ctx), table_(std::move(t)), batch_rows_(batch_rows ? batch_rows : (size_t)1 << 22) {
+  columns = table_->columns;
+}
+BatchID TableScanOperator::next(CacheManager &cache) {
+  if (isFinished || pos_ >= table_->rows) {
+    isFinished = true;
+    return 0;
+  }
+  const size_t n = std::min(batch_rows_, table_->rows - pos_);
+  auto b = std::make_unique<Batch>();
+  b->columns = table_->columns;
+  b->data = table_->data;
+  b->nulls = table_->nulls;
+  b->offset.assign(b->columns.size(), pos_);
+  b->batchSize = n;
+  b->stream = ctx.stream();
+  pos_ += n;
+  return cache.putBatch(std::move(b));
+}
+
+// ---------------------------------------------------------------- filter
+FilterOperator::FilterOperator(ExecContext &ctx, OpPtr child, ExprPtr pred) : Operator(ctx), pred_(std::move(pred)) {
+  columns = child->columns;
+  children.push_back(std::move(child));
+}
+
+static int flipCmp(int c) {
+  switch (c) {
+    case YSQL_LT: return YSQL_GT;
+    case YSQL_LE: return YSQL_GE;
+    case YSQL_GT: return YSQL_LT;
+    case YSQL_GE: return YSQL_LE;
+  }
+  return c;
+}
+
+BatchID FilterOperator::next(CacheManager &cache) {
+  while (true) {
+    if (isFinished) return 0;
+    const BatchID cid = children[0]->next(cache);
+    if (cid == 0) {
+      isFinished = true;
+      return 0;
+    }
+    auto in = cache.getBatch(cid);
+    const size_t n = in->batchSize;
+    if (n == 0) continue;  // the reference's Aggregate spun here on empty batches
+    hipStream_t s = ctx.stream();
+    // outputs sized for every row; the batch reports the selected count
+    auto out = std::make_unique<Batch>();
+    out->columns = in->columns;
+    out->stream = s;
+    std::vector<ysql_col> cols(in->numCols());
+    for (size_t c = 0; c < in->numCols(); c++) {
+      out->data.push_back(newBuf(n * in->columns[c]->bytes, s));
+      out->nulls.push_back(in->nulls[c] ? newBuf(n, s) : nullptr);
+      out->offset.push_back(0);
+      cols[c] = ysql_col{in->col(c), in->nul(c), out->data[c]->ptr,
+                         out->nulls[c] ? (uint8_t *)out->nulls[c]->ptr : nullptr, in->columns[c]->bytes, 0};
+    }
+    size_t wsb = 0;
+    abiCheck(ysql_compact_ws_bytes(n, &wsb), "ysql_compact_ws_bytes");
+    // fused path: <column> <cmp> <constant> (either side)
+    auto *cmp = dynamic_cast<ComparisonExpression *>(pred_.get());
+    BoundRefExpression *ref = nullptr;
+    BoundValueExpression *val = nullptr;
+    int op = -1;
+    if (cmp) {
+      if ((ref = dynamic_cast<BoundRefExpression *>(cmp->lhs.get())) &&
+          (val = dynamic_cast<BoundValueExpression *>(cmp->rhs.get())))
+        op = cmp->cmp;
+      else if ((ref = dynamic_cast<BoundRefExpression *>(cmp->rhs.get())) &&
+               (val = dynamic_cast<BoundValueExpression *>(cmp->lhs.get())))
+        op = flipCmp(cmp->cmp);
+    }
+    if (op >= 0 && ref && ref->idx >= 0 && (size_t)ref->idx < in->numCols() &&
+        in->columns[ref->idx]->type != DataType::STRING) {
+      coerce(val, in->columns[ref->idx]->type);
+      ExprValue sv = val->evaluate(*in, ctx);
+      for (size_t c0 = 0; c0 < cols.size(); c0 += YSQL_MAX_COLS) {
+        const int nc = (int)std::min<size_t>(YSQL_MAX_COLS, cols.size() - c0);
+        abiCheck(ysql_filter_compact(abiType(in->columns[ref->idx]->type), op, in->col(ref->idx), sv.ptr(),
+                                     cols.data() + c0, nc, n, ctx.deviceCounter(), ctx.workspace(wsb), wsb, s),
+                 "ysql_filter_compact");
+      }
+    } else {
+      ExprValue m = pred_->evaluate(*in, ctx);
+      if (!m.mask) throw EngineError("filter predicate is not boolean");
+      if (m.scalar) throw EngineError("constant filter predicate");
+      for (size_t c0 = 0; c0 < cols.size(); c0 += YSQL_MAX_COLS) {
+        const int nc = (int)std::min<size_t>(YSQL_MAX_COLS, cols.size() - c0);
+        abiCheck(ysql_compact_cols((const uint8_t *)m.ptr(), cols.data() + c0, nc, n, ctx.deviceCounter(),
+                                   ctx.workspace(wsb), wsb, s),
+                 "ysql_compact_cols");
+      }
+    }
+    const uint64_t cnt = ctx.readCounter();
+    if (cnt == 0) continue;
+    out->batchSize = cnt;
+    return cache.putBatch(std::move(out));
+  }
+}
+
+// ---------------------------------------------------------------- projection
+ProjectionOperator::ProjectionOperator(ExecContext &ctx, OpPtr child, std::vector<ExprPtr> exprs)
+    : Operator(ctx), exprs_(std::move(exprs)) {
+  for (size_t i = 0; i < exprs_.size(); i++) {
+    DataType t = exprs_[i]->isMask() ? DataType::INT : exprs_[i]->returnType(child->columns);
+    columns.push_back(std::make_shared<Column>(exprs_[i]->name(), t));
+  }
+  children.push_back(std::move(child));
+}
+BatchID ProjectionOperator::next(CacheManager &cache) {
+  if (isFinished) return 0;
+  const BatchID cid = children[0]->next(cache);
+  if (cid == 0) {
+    isFinished = true;
+    return 0;
+  }
+  auto in = cache.getBatch(cid);
+  auto out = std::make_unique<Batch>();
+  out->stream = ctx.stream();
+  out->batchSize = in->batchSize;
+  for (size_t i = 0; i < exprs_.size(); i++) {
+    ExprValue v = exprs_[i]->evaluate(*in, ctx);
+    if (v.scalar && in->batchSize != 1) throw EngineError("constant projection is not supported");
+    if (v.mask) {  // boolean -> INT column
+      auto buf = newBuf(v.rows * 4, ctx.stream());
+      abiCheck(ysql_cast(YSQL_BOOL, YSQL_INT, v.ptr(), buf->ptr, v.rows, ctx.stream()), "ysql_cast");
+      v.buf = buf;
+      v.offset = 0;
+      v.mask = false;
+      v.type = DataType::INT;
+    }
+    out->columns.push_back(columns[i]);
+    out->data.push_back(v.buf);
+    out->offset.push_back(v.offset);
+    out->nulls.push_back(v.nullbuf);
+  }
+  return cache.putBatch(std::move(out));
+}
+
+// ---------------------------------------------------------------- order
+OrderOperator::OrderOperator(ExecContext &ctx, OpPtr child, std::vector<OrderKey> keys)
+    : Operator(ctx), keys_(std::move(keys)) {
+  columns = child->columns;
+  if (keys_.empty()) throw EngineError("ORDER BY without keys");
+  for (auto &k : keys_) {
+    if (k.col < 0 || (size_t)k.col >= columns.size()) throw EngineError("ORDER BY column out of range");
+    const DataType t = columns[k.col]->type;
+    if (t == DataType::STRING) throw EngineError("ORDER BY on strings is not supported");
+    key_bits_ += t == DataType::DATETIME ? 64 : 32;
+  }
+  if (key_bits_ > 64) throw EngineError("ORDER BY keys wider than 64 bits");
+  wide_ = key_bits_ > 32;
+  children.push_back(std::move(child));
+}
+
+BufPtr OrderOperator::compositeKeys(const Batch &b) {
+  const size_t n = b.batchSize;
+  hipStream_t s = ctx.stream();
+  if (!wide_) {
+    auto k = newBuf(n * 4, s);
+    const auto &key = keys_[0];
+    abiCheck(ysql_sortkey_encode(abiType(columns[key.col]->type), key.desc, b.col(key.col), k->ptr, n, s),
+             "ysql_sortkey_encode");
+    return k;
+  }
+  auto k = newBuf(n * 8, s);
+  for (auto &key : keys_)
+    abiCheck(ysql_sortkey_pack(abiType(columns[key.col]->type), key.desc, b.col(key.col), n, (uint64_t *)k->ptr, s),
+             "ysql_sortkey_pack");
+  return k;
+}
+
+OrderOperator::Run OrderOperator::sortBatch(const Batch &b) {
+  const size_t n = b.batchSize;
+  hipStream_t s = ctx.stream();
+  Run r;
+  r.keys = compositeKeys(b);
+  auto perm = newBuf(n * 4, s);
+  const int kdt = wide_ ? YSQL_UKEY64 : YSQL_UKEY32;
+  size_t wsb = 0;
+  abiCheck(ysql_radix_sort_ws_bytes(kdt, n, &wsb), "ysql_radix_sort_ws_bytes");
+  // the packed key occupies its low key_bits_ bits
+  abiCheck(ysql_radix_argsort(kdt, 0, r.keys->ptr, (uint32_t *)perm->ptr, n, 0, key_bits_, ctx.workspace(wsb), wsb, s),
+           "ysql_radix_argsort");
+  auto out = std::make_shared<Batch>();
+  out->columns = b.columns;
+  out->stream = s;
+  out->batchSize = n;
+  for (size_t c = 0; c < b.numCols(); c++) {
+    out->data.push_back(newBuf(n * b.columns[c]->bytes, s));
+    out->nulls.push_back(b.nulls[c] ? newBuf(n, s) : nullptr);
+    out->offset.push_back(0);
+    abiCheck(ysql_gather(b.columns[c]->bytes, b.col(c), b.nul(c), (const uint32_t *)perm->ptr, n, out->data[c]->ptr,
+                         out->nulls[c] ? (uint8_t *)out->nulls[c]->ptr : nullptr, s),
+             "ysql_gather");
+  }
+  r.batch = out;
+  return r;
+}
+
+OrderOperator::Run OrderOperator::mergeRuns(const Run &a, const Run &b) {
+  hipStream_t s = ctx.stream();
+  const size_t m = a.batch->batchSize, n = b.batch->batchSize, k = m + n;
+  const int kdt = wide_ ? YSQL_UKEY64 : YSQL_UKEY32;
+  Run r;
+  r.keys = newBuf(k * (wide_ ? 8 : 4), s);
+  auto idx = newBuf(k * 4, s), isa = newBuf(k, s);
+  size_t wsb = 0;
+  abiCheck(ysql_merge_ws_bytes(m, n, k, &wsb), "ysql_merge_ws_bytes");
+  abiCheck(ysql_merge_path(kdt, 0, a.keys->ptr, m, b.keys->ptr, n, k, r.keys->ptr, (uint32_t *)idx->ptr,
+                           (uint8_t *)isa->ptr, (int64_t *)ctx.deviceCounter(), ctx.workspace(wsb), wsb, s),
+           "ysql_merge_path");
+  auto out = std::make_shared<Batch>();
+  out->columns = a.batch->columns;
+  out->stream = s;
+  out->batchSize = k;
+  for (size_t c = 0; c < out->columns.size(); c++) {
+    const bool hasn = a.batch->nulls[c] || b.batch->nulls[c];
+    out->data.push_back(newBuf(k * out->columns[c]->bytes, s));
+    out->nulls.push_back(hasn ? newBuf(k, s) : nullptr);
+    out->offset.push_back(0);
+    abiCheck(ysql_gather2(out->columns[c]->bytes, a.batch->col(c), a.batch->nul(c), b.batch->col(c), b.batch->nul(c),
+                          (const uint32_t *)idx->ptr, (const uint8_t *)isa->ptr, k, 0, out->data[c]->ptr,
+                          hasn ? (uint8_t *)out->nulls[c]->ptr : nullptr, s),
+             "ysql_gather2");
+  }
+  r.batch = out;
+  return r;
+}
+
+void OrderOperator::firstPass(CacheManager &cache) {
+  // pass 1: every child batch becomes one sorted run (order_operator.hpp:89-121)
+  while (true) {
+    const BatchID id = children[0]->next(cache);
+    if (!id) break;
+    auto b = cache.getBatch(id);
+    if (b->batchSize == 0) continue;
+    max_batch_ = std::max(max_batch_, b->batchSize);
+    runs_.push_back(sortBatch(*b));
+  }
+  // pass 2: pairwise merges, level by level, carrying an odd run
+  // (order_operator.hpp:160-193)
+  while (runs_.size() > 1) {
+    std::vector<Run> nxt;
+    for (size_t i = 0; i + 1 < runs_.size(); i += 2) nxt.push_back(mergeRuns(runs_[i], runs_[i + 1]));
+    if (runs_.size() % 2) nxt.push_back(runs_.back());
+    runs_ = std::move(nxt);
+  }
+  done_first_ = true;
+}
+
+BatchID OrderOperator::next(CacheManager &cache) {
+  if (isFinished) return 0;
+  if (!done_first_) firstPass(cache);
+  if (runs_.empty() || emit_pos_ >= runs_[0].batch->batchSize) {
+    isFinished = true;
+    runs_.clear();
+    return 0;
+  }
+  const size_t n = std::min(max_batch_, runs_[0].batch->batchSize - emit_pos_);
+  auto v = runs_[0].batch->slice(emit_pos_, n);
+  emit_pos_ += n;
+  return cache.putBatch(std::move(v));
+}
+
+// ---------------------------------------------------------------- aggregate
+AggregateOperator::AggregateOperator(ExecContext &ctx, OpPtr child, std::vector<AggSpec> aggs,
+                                     std::vector<GroupSpec> groups)
+    : Operator(ctx), aggs_(std::move(aggs)), groups_(std::move(groups)) {
+  if (groups_.size() > 4) throw EngineError("at most 4 GROUP BY columns");
+  uint64_t ng = 1;
+  for (auto &g : groups_) {
+    if (g.col < 0 || (size_t)g.col >= child->columns.size() || child->columns[g.col]->type != DataType::INT)
+      throw EngineError("GROUP BY needs INT columns");
+    if (g.card == 0) throw EngineError("GROUP BY cardinality 0");
+    ng *= g.card;
+  }
+  if (ng > YSQL_MAX_GROUPS) throw EngineError("too many groups");
+  ngroups_ = (uint32_t)ng;
+  for (auto &g : groups_) columns.push_back(std::make_shared<Column>(child->columns[g.col]->name, DataType::INT));
+  for (auto &a : aggs_) {
+    if (a.agg < YSQL_AGG_MAX || a.agg > YSQL_AGG_COUNT_STAR) throw EngineError("bad aggregate");
+    DataType src = DataType::INT;
+    if (a.agg != YSQL_AGG_COUNT_STAR) {
+      if (a.col < 0 || (size_t)a.col >= child->columns.size()) throw EngineError("aggregate column out of range");
+      src = child->columns[a.col]->type;
+      if (src == DataType::STRING) throw EngineError("aggregate over strings");
+    }
+    DataType out;
+    switch (a.agg) {
+      case YSQL_AGG_MIN: case YSQL_AGG_MAX: out = src; break;
+      case YSQL_AGG_SUM: out = src == DataType::FLOAT ? DataType::FLOAT : DataType::DATETIME; break;
+      case YSQL_AGG_AVG: out = DataType::FLOAT; break;
+      default: out = DataType::DATETIME; break;
+    }
+    columns.push_back(std::make_shared<Column>("agg", out));
+  }
+  children.push_back(std::move(child));
+}
+
+BatchID AggregateOperator::next(CacheManager &cache) {
+  if (isFinished) return 0;
+  hipStream_t s = ctx.stream();
+  const size_t na = aggs_.size();
+  const uint32_t G = ngroups_;
+  // accumulators: one per aggregate per group, plus a row count per group
+  auto accs = newBuf((na + 1) * G * sizeof(ysql_acc), s);
+  ysql_acc *A = (ysql_acc *)accs->ptr;
+  const auto &in_cols = children[0]->columns;
+  for (size_t i = 0; i < na; i++) {
+    const int t = aggs_[i].agg == YSQL_AGG_COUNT_STAR ? YSQL_INT : abiType(in_cols[aggs_[i].col]->type);
+    abiCheck(ysql_acc_init(t, aggs_[i].agg, A + i * G, (int)G, s), "ysql_acc_init");
+  }
+  abiCheck(ysql_acc_init(YSQL_INT, YSQL_AGG_COUNT_STAR, A + na * G, (int)G, s), "ysql_acc_init");
+  while (true) {
+    const BatchID id = children[0]->next(cache);
+    if (!id) break;
+    auto b = cache.getBatch(id);
+    const size_t n = b->batchSize;
+    if (n == 0) continue;  // reference spun forever here (aggregate_operator.hpp:46-48)
+    if (groups_.empty()) {
+      size_t wsb = 0;
+      abiCheck(ysql_reduce_ws_bytes(n, &wsb), "ysql_reduce_ws_bytes");
+      for (size_t i = 0; i < na; i++) {
+        const int agg = aggs_[i].agg;
+        const bool star = agg == YSQL_AGG_COUNT_STAR;
+        abiCheck(ysql_reduce(star ? YSQL_INT : abiType(b->columns[aggs_[i].col]->type), agg,
+                             star ? nullptr : b->col(aggs_[i].col), star ? nullptr : b->nul(aggs_[i].col), n, A + i,
+                             ctx.workspace(wsb), wsb, s),
+                 "ysql_reduce");
+      }
+      abiCheck(ysql_reduce(YSQL_INT, YSQL_AGG_COUNT_STAR, nullptr, nullptr, n, A + na, ctx.workspace(wsb), wsb, s),
+               "ysql_reduce");
+    } else {
+      auto gid = newBuf(n * 4, s);
+      const int32_t *gcols[4];
+      int32_t base[4];
+      uint32_t card[4];
+      for (size_t g = 0; g < groups_.size(); g++) {
+        gcols[g] = (const int32_t *)b->col(groups_[g].col);
+        base[g] = groups_[g].base;
+        card[g] = groups_[g].card;
+      }
+      abiCheck(ysql_group_ids(gcols, base, card, (int)groups_.size(), n, (uint32_t *)gid->ptr, s), "ysql_group_ids");
+      for (size_t i = 0; i < na; i++) {
+        const int agg = aggs_[i].agg;
+        const bool star = agg == YSQL_AGG_COUNT_STAR;
+        abiCheck(ysql_group_reduce(star ? YSQL_INT : abiType(b->columns[aggs_[i].col]->type), agg,
+                                   (const uint32_t *)gid->ptr, G, star ? gid->ptr : b->col(aggs_[i].col),
+                                   star ? nullptr : b->nul(aggs_[i].col), n, A + i * G, s),
+                 "ysql_group_reduce");
+      }
+      abiCheck(ysql_group_reduce(YSQL_INT, YSQL_AGG_COUNT_STAR, (const uint32_t *)gid->ptr, G, gid->ptr, nullptr, n,
+                                 A + na * G, s),
+               "ysql_group_reduce");
+    }
+  }
+  // finalise on the host (getAggregate, aggregate.hpp:149-192)
+  std::vector<ysql_acc> h((na + 1) * G);
+  hipCheck(hipMemcpyAsync(h.data(), A, h.size() * sizeof(ysql_acc), hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+  ctx.sync();
+  std::vector<uint32_t> live;
+  for (uint32_t g = 0; g < G; g++)
+    if (groups_.empty() || h[na * G + g].count > 0) live.push_back(g);
+  const size_t rows = live.size();
+  auto out = std::make_unique<Batch>();
+  out->stream = s;
+  out->batchSize = rows;
+  out->columns = columns;
+  std::vector<std::vector<uint8_t>> host(columns.size());
+  std::vector<std::vector<uint8_t>> hnul(columns.size(), std::vector<uint8_t>(rows, 0));
+  for (size_t c = 0; c < columns.size(); c++) host[c].resize(rows * columns[c]->bytes);
+  // group key columns from the dense id
+  std::vector<uint64_t> stride(groups_.size());
+  uint64_t st = 1;
+  for (int g = (int)groups_.size() - 1; g >= 0; g--) {
+    stride[g] = st;
+    st *= groups_[g].card;
+  }
+  for (size_t r = 0; r < rows; r++) {
+    const uint32_t g = live[r];
+    for (size_t k = 0; k < groups_.size(); k++) {
+      int32_t v = (int32_t)((g / stride[k]) % groups_[k].card) + groups_[k].base;
+      memcpy(&host[k][r * 4], &v, 4);
+    }
+    for (size_t i = 0; i < na; i++) {
+      const ysql_acc &a = h[i * G + g];
+      const size_t c = groups_.size() + i;
+      const DataType src = aggs_[i].agg == YSQL_AGG_COUNT_STAR ? DataType::INT : in_cols[aggs_[i].col]->type;
+      uint8_t *dst = &host[c][r * columns[c]->bytes];
+      const int agg = aggs_[i].agg;
+      if (agg == YSQL_AGG_COUNT || agg == YSQL_AGG_COUNT_STAR) {
+        int64_t v = a.count;
+        memcpy(dst, &v, 8);
+        continue;
+      }
+      if (a.count == 0) hnul[c][r] = 1;  // MIN/MAX/SUM/AVG of no rows is NULL
+      if (agg == YSQL_AGG_MIN || agg == YSQL_AGG_MAX) {
+        if (src == DataType::INT) { int32_t v = (int32_t)a.i; memcpy(dst, &v, 4); }
+        else if (src == DataType::DATETIME) memcpy(dst, &a.i, 8);
+        else memcpy(dst, &a.f, 4);
+      } else if (agg == YSQL_AGG_SUM) {
+        if (src == DataType::FLOAT) { float v = (float)a.d; memcpy(dst, &v, 4); }
+        else memcpy(dst, &a.i, 8);
+      } else {  // AVG: (double)sum / (double)count as float (div_avg_kernel)
+        const double sum = src == DataType::FLOAT ? a.d : (double)a.i;
+        float v = a.count ? (float)(sum / (double)a.count) : 0.f;
+        memcpy(dst, &v, 4);
+      }
+    }
+  }
+  for (size_t c = 0; c < columns.size(); c++) {
+    auto d = newBuf(host[c].size(), s);
+    auto nb = newBuf(rows, s);
+    if (rows) {
+      hipCheck(hipMemcpyAsync(d->ptr, host[c].data(), host[c].size(), hipMemcpyHostToDevice, s), "hipMemcpyAsync");
+      hipCheck(hipMemcpyAsync(nb->ptr, hnul[c].data(), rows, hipMemcpyHostToDevice, s), "hipMemcpyAsync");
+    }
+    out->data.push_back(d);
+    out->nulls.push_back(nb);
+    out->offset.push_back(0);
+  }
+  ctx.sync();  // host staging goes out of scope
+  isFinished = true;
+  if (rows == 0) return 0;
+  return cache.putBatch(std::move(out));
+}
+
+// ---------------------------------------------------------------- join
+JoinOperator::JoinOperator(ExecContext &ctx, OpPtr left, OpPtr right, int lcol, int rcol, int cmp)
+    : Operator(ctx), lcol_(lcol), rcol_(rcol), cmp_(cmp) {
+  if (lcol < 0 || (size_t)lcol >= left->columns.size() || rcol < 0 || (size_t)rcol >= right->columns.size())
+    throw EngineError("join column out of range");
+  if (left->columns[lcol]->type != right->columns[rcol]->type) throw EngineError("join on different types");
+  columns = left->columns;
+  columns.insert(columns.end(), right->columns.begin(), right->columns.end());
+  children.push_back(std::move(left));
+  children.push_back(std::move(right));
+}
+
+void JoinOperator::cacheLeftChild(CacheManager &cache) {
+  while (true) {
+    const BatchID id = children[0]->next(cache);
+    if (!id) break;
+    auto b = cache.getBatch(id);
+    if (b->batchSize) left_.push_back(std::shared_ptr<Batch>(std::move(b)));
+  }
+  left_cached_ = true;
+}
+
+BatchID JoinOperator::next(CacheManager &cache) {
+  if (isFinished) return 0;
+  if (!left_cached_) cacheLeftChild(cache);
+  hipStream_t s = ctx.stream();
+  while (true) {
+    if (!right_ || li_ >= left_.size()) {
+      const BatchID id = children[1]->next(cache);
+      if (!id || left_.empty()) {
+        isFinished = true;
+        return 0;
+      }
+      right_ = std::shared_ptr<Batch>(cache.getBatch(id));
+      li_ = 0;
+      if (right_->batchSize == 0) continue;
+    }
+    const Batch &L = *left_[li_++];
+    const Batch &R = *right_;
+    const int dt = abiType(L.columns[lcol_]->type);
+    size_t wsb = 0;
+    abiCheck(ysql_join_ws_bytes(dt, cmp_, L.batchSize, R.batchSize, &wsb), "ysql_join_ws_bytes");
+    BufPtr pairs;
+    uint64_t cnt = 0;
+    for (int attempt = 0; attempt < 2; attempt++) {
+      pairs = newBuf(cap_ * 8, s);
+      abiCheck(ysql_join_pairs(dt, cmp_, L.col(lcol_), L.batchSize, R.col(rcol_), R.batchSize, (uint32_t *)pairs->ptr,
+                               cap_, ctx.deviceCounter(), ctx.workspace(wsb), wsb, s),
+               "ysql_join_pairs");
+      cnt = ctx.readCounter();
+      if (cnt <= cap_) break;
+      cap_ = cnt;  // capacity grows to the exact count and the pair set is rebuilt
+    }
+    if (cnt == 0) continue;
+    auto out = std::make_unique<Batch>();
+    out->columns = columns;
+    out->stream = s;
+    out->batchSize = cnt;
+    auto emit = [&](const Batch &src, int side) {
+      for (size_t c = 0; c < src.numCols(); c++) {
+        auto d = newBuf(cnt * src.columns[c]->bytes, s);
+        auto nb = src.nulls[c] ? newBuf(cnt, s) : nullptr;
+        abiCheck(ysql_gather_pairs(src.columns[c]->bytes, src.col(c), src.nul(c), (const uint32_t *)pairs->ptr, side,
+                                   cnt, d->ptr, nb ? (uint8_t *)nb->ptr : nullptr, s),
+                 "ysql_gather_pairs");
+        out->data.push_back(d);
+        out->nulls.push_back(nb);
+        out->offset.push_back(0);
+      }
+    };
+    emit(L, 0);
+    emit(R, 1);
+    return cache.putBatch(std::move(out));
+  }
+}
+
+// ---------------------------------------------------------------- cross product
+CrossProductOperator::CrossProductOperator(ExecContext &ctx, OpPtr left, OpPtr right) : Operator(ctx) {
+  columns = left->columns;
+  columns.insert(columns.end(), right->columns.begin(), right->columns.end());
+  children.push_back(std::move(left));
+  children.push_back(std::move(right));
+}
+
+BatchID CrossProductOperator::next(CacheManager &cache) {
+  if (isFinished) return 0;
+  hipStream_t s = ctx.stream();
+  if (!left_cached_) {
+    while (true) {
+      const BatchID id = children[0]->next(cache);
+      if (!id) break;
+      auto b = cache.getBatch(id);
+      if (b->batchSize) left_.push_back(std::shared_ptr<Batch>(std::move(b)));
+    }
+    left_cached_ = true;
+  }
+  constexpr size_t kMaxOut = (size_t)1 << 24;  // rows per output batch
+  while (true) {
+    if (!right_ || roff_ >= right_->batchSize) {
+      const BatchID id = children[1]->next(cache);
+      if (!id || left_.empty()) {
+        isFinished = true;
+        return 0;
+      }
+      right_ = std::shared_ptr<Batch>(cache.getBatch(id));
+      roff_ = 0;
+      li_ = 0;
+      size_t lmax = 1;
+      for (auto &l : left_) lmax = std::max(lmax, l->batchSize);
+      rchunk_ = std::max<size_t>(1, kMaxOut / lmax);
+      if (right_->batchSize == 0) continue;
+    }
+    const Batch &L = *left_[li_];
+    const size_t rlen = std::min(rchunk_, right_->batchSize - roff_);
+    auto Rv = right_->slice(roff_, rlen);
+    const size_t rows = L.batchSize * rlen;
+    auto out = std::make_unique<Batch>();
+    out->columns = columns;
+    out->stream = s;
+    out->batchSize = rows;
+    auto emit = [&](const Batch &src, int is_left) {
+      for (size_t c = 0; c < src.numCols(); c++) {
+        auto d = newBuf(rows * src.columns[c]->bytes, s);
+        auto nb = src.nulls[c] ? newBuf(rows, s) : nullptr;
+        abiCheck(ysql_cross_expand(src.columns[c]->bytes, src.col(c), src.nul(c), L.batchSize, rlen, is_left, d->ptr,
+                                   nb ? (uint8_t *)nb->ptr : nullptr, s),
+                 "ysql_cross_expand");
+        out->data.push_back(d);
+        out->nulls.push_back(nb);
+        out->offset.push_back(0);
+      }
+    };
+    emit(L, 1);
+    emit(*Rv, 0);
+    if (++li_ >= left_.size()) {
+      li_ = 0;
+      roff_ += rlen;
+    }
+    return cache.putBatch(std::move(out));
+  }
+}
+
+// ---------------------------------------------------------------- execution
+HostResult executeToHost(Operator &root, ExecContext &ctx) {
+  HostResult r;
+  r.columns = root.columns;
+  r.data.resize(r.columns.size());
+  r.nulls.resize(r.columns.size());
+  root.init();
+  CacheManager &cache = ctx.cache();
+  while (true) {
+    const BatchID id = root.next(cache);
+    if (!id) break;
+    auto b = cache.getBatch(id);
+    const size_t n = b->batchSize;
+    for (size_t c = 0; c < r.columns.size(); c++) {
+      const size_t eb = r.columns[c]->bytes;
+      auto &d = r.data[c];
+      auto &nl = r.nulls[c];
+      d.resize((r.rows + n) * eb);
+      nl.resize(r.rows + n, 0);
+      if (n) {
+        hipCheck(hipMemcpyAsync(d.data() + r.rows * eb, b->col(c), n * eb, hipMemcpyDeviceToHost, ctx.stream()),
+                 "hipMemcpyAsync");
+        if (b->nul(c))
+          hipCheck(hipMemcpyAsync(nl.data() + r.rows, b->nul(c), n, hipMemcpyDeviceToHost, ctx.stream()),
+                   "hipMemcpyAsync");
+      }
+    }
+    ctx.sync();
+    r.rows += n;
+    r.batches++;
+  }
+  return r;
+}
+
+}  // namespace YallaSQL

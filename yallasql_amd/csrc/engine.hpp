@@ -1,0 +1,366 @@
+// engine.hpp — host engine above the kernel ABI.
+//
+// Mirrors the reference's data contract and operator interface:
+//   DataType / Column / Batch  (include/enums/data_type.hpp:2-7,
+//                               include/db/column.hpp:7-25, include/batch.hpp:21-145)
+//   CacheManager               (include/engine/cache_manager.hpp:11-89), here the
+//                               HBM residency manager with a byte budget
+//   Operator::init / next(CacheManager&) -> BatchID, 0 = exhausted
+//                              (include/engine/operators/operator.hpp:44-49)
+//   Expression::evaluate       (include/engine/operators/expressions/expression.hpp:31-67)
+// All device work of one context runs on its HIP stream; host
+// synchronisation happens only where a row count must reach the host (filter
+// and join output sizes), as in the reference.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "ysql_kernels.h"
+
+namespace YallaSQL {
+
+enum class DataType : uint8_t { INT = 0, FLOAT, DATETIME, STRING };
+
+inline unsigned getDataTypeNumBytes(DataType t) {
+  switch (t) {
+    case DataType::INT: return 4;
+    case DataType::FLOAT: return 4;
+    case DataType::DATETIME: return 8;
+    case DataType::STRING: return YSQL_STRING_BYTES;
+  }
+  return 0;
+}
+inline int abiType(DataType t) { return (int)t; }  // YSQL_INT.. share the numbering
+
+struct EngineError : std::runtime_error {
+  int code;
+  EngineError(const std::string &m, int c = YSQL_EINVAL) : std::runtime_error(m), code(c) {}
+};
+
+void hipCheck(hipError_t e, const char *what);
+void abiCheck(int rc, const char *what);
+
+struct Column {
+  std::string name;
+  DataType type;
+  unsigned bytes;
+  Column(std::string n, DataType t) : name(std::move(n)), type(t), bytes(getDataTypeNumBytes(t)) {}
+};
+using ColumnPtr = std::shared_ptr<Column>;
+
+// Device allocation on a stream (stream-ordered allocator), freed on the
+// same stream when the last owner drops it.
+struct DeviceBuffer {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+  hipStream_t stream = nullptr;
+  bool owned = true;
+  DeviceBuffer(size_t n, hipStream_t s);
+  DeviceBuffer(void *borrowed, size_t n) : ptr(borrowed), bytes(n), owned(false) {}
+  ~DeviceBuffer();
+  DeviceBuffer(const DeviceBuffer &) = delete;
+  DeviceBuffer &operator=(const DeviceBuffer &) = delete;
+};
+using BufPtr = std::shared_ptr<DeviceBuffer>;
+
+// A batch: one device buffer per column (batchSize x bytes) and one null
+// byte per row per column.  Buffers are shared, so a batch can be a zero-copy
+// view of a table or of another batch (the reference copied instead,
+// expressions/bound_ref.hpp:30-37).
+class Batch {
+ public:
+  std::vector<ColumnPtr> columns;
+  std::vector<BufPtr> data;     // data[c]->ptr + offset[c]
+  std::vector<BufPtr> nulls;    // may hold nullptr (no NULLs in that column)
+  std::vector<size_t> offset;   // row offset into the shared buffers
+  size_t batchSize = 0;
+  hipStream_t stream = nullptr;
+
+  size_t numCols() const { return columns.size(); }
+  void *col(size_t c) const {
+    return static_cast<char *>(data[c]->ptr) + offset[c] * columns[c]->bytes;
+  }
+  uint8_t *nul(size_t c) const {
+    return nulls[c] ? static_cast<uint8_t *>(nulls[c]->ptr) + offset[c] : nullptr;
+  }
+  size_t totalBytes() const;
+  // Rows [start, start + rows) as a view sharing this batch's buffers.
+  std::unique_ptr<Batch> slice(size_t start, size_t rows) const;
+};
+
+using BatchID = unsigned;
+
+// BatchID -> Batch, with byte accounting against an HBM budget.  Exceeding
+// the budget raises EngineError (the serving loop admits queries so that it
+// does not happen; the reference spilled to host memory and files instead,
+// cache_manager.hpp:67-88).
+class CacheManager {
+ public:
+  explicit CacheManager(size_t budget_bytes) : budget_(budget_bytes) {}
+  BatchID putBatch(std::unique_ptr<Batch> b);
+  std::unique_ptr<Batch> getBatch(BatchID id);
+  Batch &refBatch(BatchID id);
+  size_t bytesInUse() const { return bytes_; }
+  size_t peakBytes() const { return peak_; }
+  size_t budget() const { return budget_; }
+
+ private:
+  std::unordered_map<BatchID, std::unique_ptr<Batch>> cache_;
+  BatchID next_ = 1;
+  size_t bytes_ = 0, peak_ = 0, budget_ = 0;
+};
+
+// Per-context execution state: the stream, a growable scratch workspace for
+// kernel calls, and small pinned host slots for device-to-host counts.
+class ExecContext {
+ public:
+  ExecContext(size_t budget, hipStream_t stream);
+  ~ExecContext();
+  hipStream_t stream() const { return stream_; }
+  CacheManager &cache() { return cache_; }
+  void *workspace(size_t bytes);   // valid until the next call
+  uint64_t *deviceCounter();        // one device uint64
+  uint64_t readCounter();           // synchronises the stream
+  void sync();
+
+ private:
+  hipStream_t stream_ = nullptr;
+  bool own_stream_ = false;
+  CacheManager cache_;
+  void *ws_ = nullptr;
+  size_t ws_bytes_ = 0;
+  uint64_t *dcount_ = nullptr;
+  uint64_t *hcount_ = nullptr;
+};
+
+// In-HBM table.
+class Table {
+ public:
+  std::string name;
+  std::vector<ColumnPtr> columns;
+  std::vector<BufPtr> data, nulls;
+  size_t rows = 0;
+};
+
+// ---------------------------------------------------------------- expressions
+struct ExprValue {
+  BufPtr buf;          // device data
+  size_t offset = 0;   // rows into buf (views of batch columns)
+  BufPtr nullbuf;      // device null bytes or nullptr
+  size_t rows = 0;     // 1 for a scalar
+  DataType type = DataType::INT;
+  bool scalar = false;
+  bool mask = false;   // uint8 predicate result
+  void *ptr() const { return static_cast<char *>(buf->ptr) + offset * (mask ? 1 : getDataTypeNumBytes(type)); }
+  uint8_t *nul() const { return nullbuf ? static_cast<uint8_t *>(nullbuf->ptr) + offset : nullptr; }
+};
+
+class Expression {
+ public:
+  virtual ~Expression() = default;
+  // Result type given the input schema.
+  virtual DataType returnType(const std::vector<ColumnPtr> &in) const = 0;
+  virtual bool isMask() const { return false; }
+  virtual ExprValue evaluate(const Batch &b, ExecContext &ctx) = 0;
+  virtual std::string name() const = 0;
+};
+using ExprPtr = std::unique_ptr<Expression>;
+
+class BoundRefExpression : public Expression {
+ public:
+  explicit BoundRefExpression(int idx) : idx(idx) {}
+  int idx;
+  DataType returnType(const std::vector<ColumnPtr> &in) const override;
+  ExprValue evaluate(const Batch &b, ExecContext &ctx) override;
+  std::string name() const override { return "#" + std::to_string(idx); }
+};
+
+class BoundValueExpression : public Expression {
+ public:
+  BoundValueExpression(DataType t, int64_t iv, float fv) : type(t), ival(iv), fval(fv) {}
+  DataType type;
+  int64_t ival;
+  float fval;
+  BufPtr dev;  // 1-element device buffer, created on first use
+  DataType returnType(const std::vector<ColumnPtr> &) const override { return type; }
+  ExprValue evaluate(const Batch &b, ExecContext &ctx) override;
+  std::string name() const override { return "const"; }
+};
+
+class ComparisonExpression : public Expression {
+ public:
+  ComparisonExpression(int cmp, ExprPtr l, ExprPtr r) : cmp(cmp), lhs(std::move(l)), rhs(std::move(r)) {}
+  int cmp;
+  ExprPtr lhs, rhs;
+  DataType returnType(const std::vector<ColumnPtr> &) const override { return DataType::INT; }
+  bool isMask() const override { return true; }
+  ExprValue evaluate(const Batch &b, ExecContext &ctx) override;
+  std::string name() const override { return "cmp"; }
+};
+
+class ConjunctionExpression : public Expression {
+ public:
+  ConjunctionExpression(int op, ExprPtr l, ExprPtr r) : op(op), lhs(std::move(l)), rhs(std::move(r)) {}
+  int op;
+  ExprPtr lhs, rhs;
+  DataType returnType(const std::vector<ColumnPtr> &) const override { return DataType::INT; }
+  bool isMask() const override { return true; }
+  ExprValue evaluate(const Batch &b, ExecContext &ctx) override;
+  std::string name() const override { return "logic"; }
+};
+
+class BoundFuncExpression : public Expression {
+ public:
+  BoundFuncExpression(int op, ExprPtr l, ExprPtr r) : op(op), lhs(std::move(l)), rhs(std::move(r)) {}
+  int op;
+  ExprPtr lhs, rhs;
+  DataType returnType(const std::vector<ColumnPtr> &in) const override { return lhs->returnType(in); }
+  ExprValue evaluate(const Batch &b, ExecContext &ctx) override;
+  std::string name() const override { return "func"; }
+};
+
+class CastExpression : public Expression {
+ public:
+  CastExpression(DataType t, ExprPtr c) : type(t), child(std::move(c)) {}
+  DataType type;
+  ExprPtr child;
+  DataType returnType(const std::vector<ColumnPtr> &) const override { return type; }
+  ExprValue evaluate(const Batch &b, ExecContext &ctx) override;
+  std::string name() const override { return "cast"; }
+};
+
+// ---------------------------------------------------------------- operators
+class Operator {
+ public:
+  std::vector<ColumnPtr> columns;  // output schema
+  bool isFinished = false;
+  explicit Operator(ExecContext &ctx) : ctx(ctx) {}
+  virtual ~Operator() = default;
+  virtual void init() {
+    for (auto &c : children) c->init();
+  }
+  virtual BatchID next(CacheManager &cache) = 0;
+  std::vector<std::unique_ptr<Operator>> children;
+  ExecContext &ctx;
+};
+using OpPtr = std::unique_ptr<Operator>;
+
+class TableScanOperator : public Operator {
+ public:
+  TableScanOperator(ExecContext &ctx, std::shared_ptr<Table> t, size_t batch_rows);
+  BatchID next(CacheManager &cache) override;
+
+ private:
+  std::shared_ptr<Table> table_;
+  size_t batch_rows_, pos_ = 0;
+};
+
+class FilterOperator : public Operator {
+ public:
+  FilterOperator(ExecContext &ctx, OpPtr child, ExprPtr pred);
+  BatchID next(CacheManager &cache) override;
+
+ private:
+  ExprPtr pred_;
+};
+
+class ProjectionOperator : public Operator {
+ public:
+  ProjectionOperator(ExecContext &ctx, OpPtr child, std::vector<ExprPtr> exprs);
+  BatchID next(CacheManager &cache) override;
+
+ private:
+  std::vector<ExprPtr> exprs_;
+};
+
+struct OrderKey {
+  int col;
+  bool desc;
+};
+
+class OrderOperator : public Operator {
+ public:
+  OrderOperator(ExecContext &ctx, OpPtr child, std::vector<OrderKey> keys);
+  BatchID next(CacheManager &cache) override;
+
+ private:
+  struct Run {
+    std::shared_ptr<Batch> batch;  // sorted rows
+    BufPtr keys;                   // sorted composite keys (u32 or u64)
+  };
+  void firstPass(CacheManager &cache);
+  Run sortBatch(const Batch &b);
+  Run mergeRuns(const Run &a, const Run &b);
+  BufPtr compositeKeys(const Batch &b);
+  std::vector<OrderKey> keys_;
+  int key_bits_ = 0;
+  bool wide_ = false;  // composite key is 64 bits
+  bool done_first_ = false;
+  std::vector<Run> runs_;
+  size_t max_batch_ = 0, emit_pos_ = 0;
+};
+
+struct AggSpec {
+  int agg;
+  int col;
+};
+struct GroupSpec {
+  int col;
+  int32_t base;
+  uint32_t card;
+};
+
+class AggregateOperator : public Operator {
+ public:
+  AggregateOperator(ExecContext &ctx, OpPtr child, std::vector<AggSpec> aggs, std::vector<GroupSpec> groups);
+  BatchID next(CacheManager &cache) override;
+
+ private:
+  std::vector<AggSpec> aggs_;
+  std::vector<GroupSpec> groups_;
+  uint32_t ngroups_ = 1;
+};
+
+class JoinOperator : public Operator {
+ public:
+  JoinOperator(ExecContext &ctx, OpPtr left, OpPtr right, int lcol, int rcol, int cmp);
+  BatchID next(CacheManager &cache) override;
+
+ private:
+  void cacheLeftChild(CacheManager &cache);
+  int lcol_, rcol_, cmp_;
+  bool left_cached_ = false;
+  std::vector<std::shared_ptr<Batch>> left_;
+  std::shared_ptr<Batch> right_;
+  size_t li_ = 0;
+  size_t cap_ = 1 << 20;
+};
+
+class CrossProductOperator : public Operator {
+ public:
+  CrossProductOperator(ExecContext &ctx, OpPtr left, OpPtr right);
+  BatchID next(CacheManager &cache) override;
+
+ private:
+  bool left_cached_ = false;
+  std::vector<std::shared_ptr<Batch>> left_;
+  std::shared_ptr<Batch> right_;
+  size_t li_ = 0, roff_ = 0, rchunk_ = 0;
+};
+
+// Copies every batch the root produces to host memory.
+struct HostResult {
+  std::vector<ColumnPtr> columns;
+  std::vector<std::vector<uint8_t>> data, nulls;
+  size_t rows = 0, batches = 0;
+};
+HostResult executeToHost(Operator &root, ExecContext &ctx);
+
+}  // namespace YallaSQL

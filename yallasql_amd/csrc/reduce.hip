@@ -506,11 +506,59 @@ __global__ __launch_bounds__(1024) void k_star(StarArgs a, int nd, const int32_t
   }
 }
 
+struct GidArgs {
+  const int32_t *col[4];
+  int32_t base[4];
+  uint32_t card[4];
+  uint32_t stride[4];
+};
+
+__global__ void k_group_ids(GidArgs a, int nc, uint64_t n, uint32_t *gid) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t g = 0;
+    bool ok = true;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      if (c < nc) {
+        const int64_t v = (int64_t)a.col[c][i] - a.base[c];
+        if (v < 0 || v >= (int64_t)a.card[c]) ok = false;
+        g += (uint32_t)v * a.stride[c];
+      }
+    }
+    gid[i] = ok ? g : 0xFFFFFFFFu;
+  }
+}
+
 }  // namespace ysql
 
 using namespace ysql;
 
 extern "C" {
+
+int ysql_group_ids(const int32_t *const *cols, const int32_t *base, const uint32_t *card, int ncols, uint64_t n,
+                   uint32_t *gid_out, ysql_stream_t stream) {
+  if (ncols < 1 || ncols > 4 || !cols || !base || !card) return YSQL_EINVAL;
+  if (n == 0) return YSQL_OK;
+  if (!gid_out) return YSQL_EINVAL;
+  GidArgs a;
+  memset(&a, 0, sizeof(a));
+  uint64_t stride = 1;
+  for (int c = ncols - 1; c >= 0; c--) {
+    if (!cols[c] || card[c] == 0) return YSQL_EINVAL;
+    a.col[c] = cols[c];
+    a.base[c] = base[c];
+    a.card[c] = card[c];
+    a.stride[c] = (uint32_t)stride;
+    stride *= card[c];
+    if (stride >= 0xFFFFFFFFull) return YSQL_EINVAL;
+  }
+  uint64_t g = ceil_div(n, 256 * 4);
+  const uint64_t cap = (uint64_t)grid_cap() * 4;
+  if (g > cap) g = cap;
+  k_group_ids<<<(unsigned)g, 256, 0, as_stream(stream)>>>(a, ncols, n, gid_out);
+  YSQL_LAUNCH_CHECK();
+  return YSQL_OK;
+}
 
 int ysql_acc_init(int dtype, int agg, ysql_acc *acc, int nacc, ysql_stream_t stream) {
   if (nacc <= 0) return YSQL_OK;

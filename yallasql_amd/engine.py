@@ -1,0 +1,256 @@
+"""Python handle on the host engine (include/ysql_engine.h).
+
+    ctx = Context()
+    t = ctx.table("t", id=ids, price=prices)          # numpy columns -> HBM
+    p = ctx.scan(t, batch_rows=1 << 18)
+    p = p.filter(cmp(GT, ref(1), f32(500.0))).order([1, 0], [True, False])
+    cols, nulls = p.execute()                          # numpy columns
+
+Expression helpers build ysql_expr trees; plan methods consume their inputs.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+
+lib = L.lib
+P = C.c_void_p
+
+_sig = {
+    "ysql_ctx_create": ([C.c_uint64, P], P),
+    "ysql_ctx_destroy": ([P], None),
+    "ysql_ctx_peak_bytes": ([P], C.c_uint64),
+    "ysql_table_create": ([P, C.c_char_p], P),
+    "ysql_table_add_column": ([P, C.c_char_p, C.c_int, P, P, C.c_uint64], C.c_int),
+    "ysql_table_add_device_column": ([P, C.c_char_p, C.c_int, P, P, C.c_uint64], C.c_int),
+    "ysql_table_rows": ([P], C.c_uint64),
+    "ysql_table_destroy": ([P], None),
+    "ysql_expr_ref": ([C.c_int], P),
+    "ysql_expr_i32": ([C.c_int32], P),
+    "ysql_expr_f32": ([C.c_float], P),
+    "ysql_expr_i64": ([C.c_int64], P),
+    "ysql_expr_cmp": ([C.c_int, P, P], P),
+    "ysql_expr_logic": ([C.c_int, P, P], P),
+    "ysql_expr_arith": ([C.c_int, P, P], P),
+    "ysql_expr_cast": ([C.c_int, P], P),
+    "ysql_expr_destroy": ([P], None),
+    "ysql_plan_scan": ([P, P, C.c_uint64], P),
+    "ysql_plan_filter": ([P, P], P),
+    "ysql_plan_project": ([P, C.POINTER(P), C.c_int], P),
+    "ysql_plan_order": ([P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int], P),
+    "ysql_plan_aggregate": ([P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int),
+                             C.POINTER(C.c_int32), C.POINTER(C.c_uint32), C.c_int], P),
+    "ysql_plan_join": ([P, P, C.c_int, C.c_int, C.c_int], P),
+    "ysql_plan_cross": ([P, P], P),
+    "ysql_plan_destroy": ([P], None),
+    "ysql_plan_execute": ([P, C.POINTER(P)], C.c_int),
+    "ysql_result_ncols": ([P], C.c_int),
+    "ysql_result_rows": ([P], C.c_uint64),
+    "ysql_result_batches": ([P], C.c_uint64),
+    "ysql_result_dtype": ([P, C.c_int], C.c_int),
+    "ysql_result_data": ([P, C.c_int], P),
+    "ysql_result_nulls": ([P, C.c_int], P),
+    "ysql_result_destroy": ([P], None),
+    "ysql_engine_last_error": ([], C.c_char_p),
+}
+for _n, (_a, _r) in _sig.items():
+    _f = getattr(lib, _n)
+    _f.argtypes = _a
+    _f.restype = _r
+
+NP_DT = {L.INT: np.int32, L.FLOAT: np.float32, L.DATETIME: np.int64}
+DT_OF = {np.dtype(np.int32): L.INT, np.dtype(np.float32): L.FLOAT, np.dtype(np.int64): L.DATETIME}
+
+
+class EngineError(RuntimeError):
+    pass
+
+
+def _chk_ptr(p, what):
+    if not p:
+        raise EngineError(f"{what}: {lib.ysql_engine_last_error().decode()}")
+    return p
+
+
+class Expr:
+    def __init__(self, h):
+        self.h = _chk_ptr(h, "expression")
+
+    def _take(self):
+        h, self.h = self.h, None
+        if h is None:
+            raise EngineError("expression used twice")
+        return h
+
+
+def ref(i):
+    return Expr(lib.ysql_expr_ref(i))
+
+
+def i32(v):
+    return Expr(lib.ysql_expr_i32(int(v)))
+
+
+def f32(v):
+    return Expr(lib.ysql_expr_f32(float(v)))
+
+
+def i64(v):
+    return Expr(lib.ysql_expr_i64(int(v)))
+
+
+def cmp(op, a, b):
+    return Expr(lib.ysql_expr_cmp(op, a._take(), b._take()))
+
+
+def logic(op, a, b):
+    return Expr(lib.ysql_expr_logic(op, a._take(), b._take()))
+
+
+def arith(op, a, b):
+    return Expr(lib.ysql_expr_arith(op, a._take(), b._take()))
+
+
+def cast(dtype, e):
+    return Expr(lib.ysql_expr_cast(dtype, e._take()))
+
+
+class Plan:
+    def __init__(self, ctx, h):
+        self.ctx = ctx
+        self.h = _chk_ptr(h, "plan")
+
+    def _take(self):
+        h, self.h = self.h, None
+        if h is None:
+            raise EngineError("plan node used twice")
+        return h
+
+    def filter(self, pred):
+        return Plan(self.ctx, lib.ysql_plan_filter(self._take(), pred._take()))
+
+    def project(self, exprs):
+        arr = (P * len(exprs))(*[e._take() for e in exprs])
+        return Plan(self.ctx, lib.ysql_plan_project(self._take(), arr, len(exprs)))
+
+    def order(self, cols, desc=None):
+        desc = desc or [0] * len(cols)
+        return Plan(self.ctx, lib.ysql_plan_order(self._take(), (C.c_int * len(cols))(*cols),
+                                                  (C.c_int * len(cols))(*[int(d) for d in desc]), len(cols)))
+
+    def aggregate(self, aggs, groups=()):
+        """aggs: [(agg, col)]; groups: [(col, base, card)]."""
+        a = (C.c_int * len(aggs))(*[x[0] for x in aggs])
+        c = (C.c_int * len(aggs))(*[x[1] for x in aggs])
+        ng = len(groups)
+        gc = (C.c_int * max(ng, 1))(*[g[0] for g in groups])
+        gb = (C.c_int32 * max(ng, 1))(*[g[1] for g in groups])
+        gk = (C.c_uint32 * max(ng, 1))(*[g[2] for g in groups])
+        return Plan(self.ctx, lib.ysql_plan_aggregate(self._take(), a, c, len(aggs), gc, gb, gk, ng))
+
+    def join(self, right, lcol, rcol, op=L.EQ):
+        return Plan(self.ctx, lib.ysql_plan_join(self._take(), right._take(), lcol, rcol, op))
+
+    def cross(self, right):
+        return Plan(self.ctx, lib.ysql_plan_cross(self._take(), right._take()))
+
+    def execute(self):
+        """Runs the plan to exhaustion; returns (columns, nulls, batches) as numpy arrays."""
+        out = P()
+        rc = lib.ysql_plan_execute(self.h, C.byref(out))
+        if rc != 0:
+            raise EngineError(f"execute returned {rc}: {lib.ysql_engine_last_error().decode()}")
+        try:
+            n = lib.ysql_result_rows(out)
+            cols, nulls = [], []
+            for c in range(lib.ysql_result_ncols(out)):
+                dt = lib.ysql_result_dtype(out, c)
+                if dt == L.STRING:
+                    nb = n * 256
+                    arr = np.ctypeslib.as_array((C.c_uint8 * max(nb, 1)).from_address(lib.ysql_result_data(out, c) or 0)) if n else np.zeros(0, np.uint8)
+                    cols.append(np.array(arr[:nb]).reshape(n, 256))
+                else:
+                    npdt = np.dtype(NP_DT[dt])
+                    if n:
+                        buf = (C.c_uint8 * (n * npdt.itemsize)).from_address(lib.ysql_result_data(out, c))
+                        cols.append(np.frombuffer(bytes(buf), dtype=npdt).copy())
+                    else:
+                        cols.append(np.zeros(0, npdt))
+                if n:
+                    nbuf = (C.c_uint8 * n).from_address(lib.ysql_result_nulls(out, c))
+                    nulls.append(np.frombuffer(bytes(nbuf), dtype=np.uint8).copy())
+                else:
+                    nulls.append(np.zeros(0, np.uint8))
+            batches = lib.ysql_result_batches(out)
+        finally:
+            lib.ysql_result_destroy(out)
+        return cols, nulls, batches
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.ysql_plan_destroy(self.h)
+            self.h = None
+
+
+class Table:
+    def __init__(self, ctx, h, keep):
+        self.ctx, self.h, self._keep = ctx, h, keep
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.ysql_table_destroy(self.h)
+            self.h = None
+
+
+class Context:
+    def __init__(self, hbm_budget=0, stream=None):
+        self.h = _chk_ptr(lib.ysql_ctx_create(hbm_budget, stream), "context")
+        self._tables = []
+
+    def table(self, name, columns, nulls=None):
+        """columns: list of (name, numpy array | (n, 256) uint8 strings)."""
+        h = _chk_ptr(lib.ysql_table_create(self.h, name.encode()), "table")
+        keep = []
+        for i, (cn, arr) in enumerate(columns):
+            arr = np.ascontiguousarray(arr)
+            dt = L.STRING if (arr.dtype == np.uint8 and arr.ndim == 2) else DT_OF[arr.dtype]
+            nb = None if nulls is None or nulls[i] is None else np.ascontiguousarray(nulls[i], dtype=np.uint8)
+            keep += [arr, nb]
+            rc = lib.ysql_table_add_column(h, cn.encode(), dt, arr.ctypes.data_as(P),
+                                           None if nb is None else nb.ctypes.data_as(P), arr.shape[0])
+            if rc != 0:
+                raise EngineError(lib.ysql_engine_last_error().decode())
+        t = Table(self, h, keep)
+        self._tables.append(t)
+        return t
+
+    def device_table(self, name, columns, nulls=None):
+        """columns: list of (name, torch device tensor) borrowed by the table."""
+        h = _chk_ptr(lib.ysql_table_create(self.h, name.encode()), "table")
+        keep = []
+        for i, (cn, t) in enumerate(columns):
+            dt = L.STRING if t.dim() == 2 else {4: (L.FLOAT if t.is_floating_point() else L.INT), 8: L.DATETIME}[t.element_size()]
+            nb = None if nulls is None else nulls[i]
+            keep += [t, nb]
+            rc = lib.ysql_table_add_device_column(h, cn.encode(), dt, P(t.data_ptr()),
+                                                  None if nb is None else P(nb.data_ptr()), t.shape[0])
+            if rc != 0:
+                raise EngineError(lib.ysql_engine_last_error().decode())
+        t = Table(self, h, keep)
+        self._tables.append(t)
+        return t
+
+    def scan(self, table, batch_rows=0):
+        return Plan(self, lib.ysql_plan_scan(self.h, table.h, batch_rows))
+
+    def peak_bytes(self):
+        return lib.ysql_ctx_peak_bytes(self.h)
+
+    def close(self):
+        for t in self._tables:
+            t.__del__()
+        self._tables = []
+        if self.h:
+            lib.ysql_ctx_destroy(self.h)
+            self.h = None
