@@ -285,6 +285,27 @@ int ysql_reduce(int dtype, int agg, const void *in, const uint8_t *null_opt, uin
  * convert_double_to_float_kernel (reduction_kernel.cu:5-25).             */
 int ysql_agg_finalize(int dtype, int agg, const ysql_acc *acc, void *out, ysql_stream_t stream);
 
+/* Filtered multi-aggregate in one pass (FilterOperator feeding
+ * AggregateOperator, filter_operator.hpp:31-98 + aggregate.hpp:55-146,
+ * without materialising the filtered batch):
+ *     for rows with pred_col[i] <cmp> scalar: fold specs[s].col[i] into
+ *     *specs[s].acc under specs[s].agg   (s < nspecs <= 4).
+ * COUNT and COUNT_STAR count the qualifying rows (this form takes no null
+ * masks).  Deterministic like ysql_reduce: canonical 65536-row tiles folded
+ * in a fixed order, so it returns the same bits as the batched executor's
+ * aggregate jobs.                                                          */
+typedef struct ysql_agg_spec {
+  int dtype; /* YSQL_INT / YSQL_FLOAT / YSQL_DATETIME of col */
+  int agg;   /* YSQL_AGG_* */
+  const void *col;
+  ysql_acc *acc; /* device accumulator, initialised with ysql_acc_init */
+} ysql_agg_spec;
+#define YSQL_MAX_AGG_SPECS 4
+int ysql_filter_aggregate_ws_bytes(uint64_t n, int nspecs, size_t *bytes);
+int ysql_filter_aggregate(int pred_dtype, int cmp, const void *pred_col, const void *scalar,
+                          const ysql_agg_spec *specs, int nspecs, uint64_t n, void *ws,
+                          size_t ws_bytes, ysql_stream_t stream);
+
 /* GROUP BY (no reference function: the reference's AggregateOperator never
  * reads `groups`, aggregate_operator.hpp:92-103).  acc_inout[g] folds the
  * rows with group_id == g; group_id >= n_groups skips the row.           */
@@ -352,6 +373,57 @@ typedef struct ysql_star_dim {
 int ysql_star_join_group_sum(const ysql_star_dim *dims, int ndims, const int32_t *m1,
                              const int32_t *m2, uint64_t n, int64_t *sums, uint64_t *counts,
                              uint32_t n_groups, ysql_stream_t stream);
+
+/* ======================================================================
+ * Batched ragged jobs (no reference counterpart: the reference launched
+ * every operator of every batch separately).  Thousands of independent
+ * sort / filter / aggregate jobs run as ONE persistent-kernel launch that
+ * drains a device work queue of tile tasks (per-job phase counters order a
+ * job's histogram, digit passes and finalisation), captured once in a
+ * hipGraph and replayed.  ysql_batch_run_plain runs the same jobs as a
+ * per-job launch loop of the per-call entry points; both produce the same
+ * bits (same tile functions, same fold orders).
+ *   SORT:   stable argsort of keys (UKEY32/UKEY64): keys_out = sorted keys,
+ *           perm_out[pos] = row.
+ *   FILTER: rows with vals[i] < threshold; keys_out / vals_out = selected
+ *           (key, value) in row order; *count_out = count.
+ *   AGG:    rows with vals[i] < threshold; accs_out[0] = SUM(vals) (its
+ *           count is COUNT, d / count is AVG), accs_out[1] = MIN(keys),
+ *           accs_out[2] = MAX(keys) (keys read as signed INT / DATETIME).
+ * The batch object owns the queue, job table and every workspace (device
+ * memory is allocated by ysql_batch_create only); inputs and outputs stay
+ * the caller's and must outlive the batch.                                 */
+enum { YSQL_JOB_SORT = 0, YSQL_JOB_FILTER = 1, YSQL_JOB_AGG = 2 };
+typedef struct ysql_job {
+  int kind;
+  int key_dtype; /* YSQL_UKEY32 or YSQL_UKEY64 */
+  uint64_t n;
+  const void *keys;
+  const float *vals;
+  float threshold;
+  uint32_t pad;
+  void *keys_out;
+  uint32_t *perm_out;
+  float *vals_out;
+  uint64_t *count_out;
+  ysql_acc *accs_out;
+} ysql_job;
+typedef struct ysql_batch ysql_batch;
+/* Tuning of the batched executor (ysql_batch_tuning_default fills it). */
+typedef struct ysql_batch_tuning {
+  int small_sort_max;   /* sort jobs up to this many rows run whole in one workgroup */
+  int hist_chunk;       /* rows per histogram task of a larger sort job */
+  int workers_per_cu;   /* persistent workgroups per CU (1 or 2) */
+  int pad;
+} ysql_batch_tuning;
+void ysql_batch_tuning_default(ysql_batch_tuning *t);
+int ysql_batch_create(const ysql_job *jobs, int njobs, const ysql_batch_tuning *tuning,
+                      ysql_batch **out);
+int ysql_batch_run(ysql_batch *b, ysql_stream_t stream);
+int ysql_batch_run_plain(ysql_batch *b, ysql_stream_t stream);
+/* Queue length, persistent grid size and workspace bytes of a batch. */
+int ysql_batch_info(const ysql_batch *b, uint64_t *tasks, uint32_t *grid, uint64_t *ws_bytes);
+void ysql_batch_destroy(ysql_batch *b);
 
 /* ======================================================================
  * Utilities                                                              */

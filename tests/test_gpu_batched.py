@@ -1,0 +1,106 @@
+"""Batched persistent-kernel executor: the same ragged jobs run as one
+captured launch and as the per-job launch loop give identical bits, and
+every job matches the oracle."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+persistent_off = pytest.mark.skip(
+    reason="persistent batched launch hung on MI355X (cause not identified); "
+    "ysql_batch_run is disabled and returns YSQL_EUNSUPPORTED")
+
+
+def make_jobs(seed=0, extra=60, hi=300_000):
+    r = np.random.default_rng(seed)
+    edge = [0, 1, 100, 4095, 4096, 4097, 6144, 8193, 65535, 65536, 65537, 200_000]
+    rand = list(np.floor(np.exp(r.uniform(0, math.log(hi), extra))).astype(int))
+    # every edge size once per kind (sort, filter, aggregate), then random sizes
+    sizes = [s for s in edge for _ in range(3)] + rand
+    kinds = [j % 3 for j in range(len(sizes))]
+    key64 = [(j // 3) % 2 == 1 for j in range(len(sizes))]
+    thr = r.random(len(sizes)).astype(np.float32)
+    return sizes, kinds, key64, thr
+
+
+@pytest.fixture(scope="module")
+def jobset(dev):
+    from yallasql_amd.batched import JobSet
+
+    sizes, kinds, key64, thr = make_jobs()
+    js = JobSet(sizes, kinds, key64, thr, dev, seed=1).create()
+    yield js
+    js.close()
+
+
+@persistent_off
+def test_batched_equals_plain_bitwise(jobset):
+    jobset.clear_outputs()
+    jobset.run_plain()
+    plain = jobset.snapshot()
+    jobset.clear_outputs()
+    jobset.run_batched()
+    batched = jobset.snapshot()
+    for a, b in zip(plain, batched):
+        assert np.array_equal(a, b)
+    # the captured graph replays to the same bits
+    jobset.clear_outputs()
+    jobset.run_batched()
+    again = jobset.snapshot()
+    for a, b in zip(batched, again):
+        assert np.array_equal(a, b)
+
+
+def check_jobs_against_oracle(jobset):
+    from yallasql_amd.batched import AGG, FILTER, SORT
+    for j in range(len(jobset.sizes)):
+        keys, vals = jobset.job_inputs(j)
+        kind, thr = jobset.kinds[j], np.float32(jobset.thr[j])
+        ukeys = keys.view(np.uint64 if keys.itemsize == 8 else np.uint32)
+        if kind == SORT:
+            ko, perm = jobset.job_outputs(j)
+            exp = O.stable_argsort(ukeys)
+            assert np.array_equal(perm.view(np.uint32), exp), j
+            assert np.array_equal(ko.view(ukeys.dtype), ukeys[exp]), j
+        elif kind == FILTER:
+            ko, vo, c = jobset.job_outputs(j)
+            m = vals < thr
+            assert c == int(m.sum()), j
+            assert np.array_equal(ko, keys[m]) and np.array_equal(vo, vals[m]), j
+        else:
+            accs = jobset.job_outputs(j)
+            m = vals < thr
+            assert accs[0].count == int(m.sum()), j
+            s = math.fsum(vals[m].astype(np.float64))
+            assert abs(accs[0].d - s) <= 1e-6 * max(1.0, abs(s)), j
+            if m.any():
+                assert accs[1].i == int(keys[m].min()) and accs[2].i == int(keys[m].max()), j
+
+
+@persistent_off
+def test_batched_jobs_match_oracle(jobset):
+    jobset.clear_outputs()
+    jobset.run_batched()
+    check_jobs_against_oracle(jobset)
+
+
+def test_plain_loop_jobs_match_oracle(jobset):
+    jobset.clear_outputs()
+    jobset.run_plain()
+    check_jobs_against_oracle(jobset)
+
+
+def test_batch_run_disabled(jobset):
+    from yallasql_amd._lib import YsqlError
+
+    with pytest.raises(YsqlError):
+        jobset.run_batched()
+
+
+def test_batch_info(jobset):
+    info = jobset.info()
+    assert info["tasks"] > 0 and info["grid"] >= 256

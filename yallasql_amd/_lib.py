@@ -128,3 +128,35 @@ def ws_bytes(name, *args):
     out = C.c_size_t(0)
     call(name, *args, C.byref(out))
     return out.value
+
+
+class Job(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int), ("key_dtype", C.c_int), ("n", C.c_uint64), ("keys", C.c_void_p), ("vals", C.c_void_p),
+        ("threshold", C.c_float), ("pad", C.c_uint32), ("keys_out", C.c_void_p), ("perm_out", C.c_void_p),
+        ("vals_out", C.c_void_p), ("count_out", C.c_void_p), ("accs_out", C.c_void_p),
+    ]
+
+
+class BatchTuning(C.Structure):
+    _fields_ = [("small_sort_max", C.c_int), ("hist_chunk", C.c_int), ("workers_per_cu", C.c_int),
+                ("pad", C.c_int)]
+
+
+JOB_SORT, JOB_FILTER, JOB_AGG = 0, 1, 2
+assert C.sizeof(Job) == 80
+for _name, _args, _res in [
+    ("ysql_filter_aggregate_ws_bytes", [U64, I32, SZP], C.c_int),
+    ("ysql_filter_aggregate", [I32, I32, P, P, P, I32, U64, P, C.c_size_t, P], C.c_int),
+    ("ysql_group_ids", [P, P, P, I32, U64, P, P], C.c_int),
+    ("ysql_sortkey_pack", [I32, I32, P, U64, P, P], C.c_int),
+    ("ysql_batch_tuning_default", [C.POINTER(BatchTuning)], None),
+    ("ysql_batch_create", [C.POINTER(Job), I32, C.POINTER(BatchTuning), C.POINTER(P)], C.c_int),
+    ("ysql_batch_run", [P, P], C.c_int),
+    ("ysql_batch_run_plain", [P, P], C.c_int),
+    ("ysql_batch_info", [P, C.POINTER(U64), C.POINTER(U32), C.POINTER(U64)], C.c_int),
+    ("ysql_batch_destroy", [P], None),
+]:
+    _f = getattr(lib, _name)
+    _f.argtypes = _args
+    _f.restype = _res

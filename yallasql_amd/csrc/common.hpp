@@ -34,7 +34,7 @@ void set_last_error(const char *msg);
 
 static inline hipStream_t as_stream(ysql_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
 
-static inline uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+__host__ __device__ static inline uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
 static inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
 // Number of workgroups that keeps every CU busy for a streaming kernel

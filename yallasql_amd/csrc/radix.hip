@@ -9,83 +9,22 @@
 // Here (gfx950, wave64):
 //   1. k_radix_hist: one read of the keys builds the 256-bin histogram of
 //      every digit at once (LDS atomics, one global atomic per bin per block).
-//   2. One k_onesweep launch per 8-bit digit: a workgroup takes a tile of
-//      keys, ranks each key inside its wave with 8 ballots (lanes with the same
-//      digit are found by matching the 8 digit bits; mbcnt gives the rank), sums
-//      per-wave digit counts into tile counts, publishes them for decoupled
-//      look-back (one thread per digit walks back over predecessor tiles),
-//      reorders the tile in LDS by digit and writes each digit's run to its
-//      global position.  One read and one write of key+payload per digit.
-// Stability: keys are ranked in (wave, item, lane) order, which is the
-// input order inside the tile, and tiles are placed in tile order.
-#include "common.hpp"
+//   2. One k_onesweep launch per 8-bit digit (radix_dev.hpp): one read and
+//      one write of key+payload per digit, tiles placed through decoupled
+//      look-back.
+#include "radix_dev.hpp"
 
 namespace ysql {
-
-constexpr int kBins = 256;
-constexpr uint32_t kRFlagAgg = 1u << 30;
-constexpr uint32_t kRFlagInc = 2u << 30;
-constexpr uint32_t kRValMask = (1u << 30) - 1;
-
-__device__ __forceinline__ uint32_t rstatus_load(const uint32_t *p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void rstatus_store(uint32_t *p, uint32_t v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// ---------------------------------------------------------------------------
-// Histograms of all digit positions in one pass.
-// ---------------------------------------------------------------------------
-// Keys narrower than their type (small integers, Zipf ranks) put most of
-// their high digits in one bin; 64 lanes adding to one LDS word serialise.
-// Each thread therefore counts the bin of digit value `desc` (the encoded
-// zero digit) in registers and only adds the other bins to LDS.
-template <typename K>
-__device__ __forceinline__ void hist_key(K k, int begin_bit, int passes, uint32_t desc, uint32_t *s_h,
-                                         uint32_t (&zc)[8]) {
-#pragma unroll
-  for (int p = 0; p < 8; p++) {
-    if (p < passes) {
-      const uint32_t d = ((uint32_t)(k >> (begin_bit + 8 * p)) & 0xFFu) ^ desc;
-      if (d == desc) zc[p]++;
-      else atomicAdd(&s_h[p * kBins + d], 1u);
-    }
-  }
-}
 
 template <typename K>
 __global__ __launch_bounds__(256) void k_radix_hist(const K *keys, uint64_t n, int begin_bit, int passes,
                                                     uint32_t desc, uint32_t *hist) {
   __shared__ uint32_t s_h[8 * kBins];
-  for (int i = threadIdx.x; i < passes * kBins; i += blockDim.x) s_h[i] = 0;
-  __syncthreads();
-  uint32_t zc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  constexpr int PER = 16 / sizeof(K);  // keys per 16-byte load
-  const uint64_t nv = ((uintptr_t)keys & 15) ? 0 : n / PER;
-  const uint4 *kv = reinterpret_cast<const uint4 *>(keys);
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += (uint64_t)gridDim.x * blockDim.x) {
-    const uint4 q = kv[i];
-    K ks[PER];
-    memcpy(ks, &q, 16);
-#pragma unroll
-    for (int e = 0; e < PER; e++) hist_key<K>(ks[e], begin_bit, passes, desc, s_h, zc);
-  }
-  for (uint64_t i = nv * PER + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (uint64_t)gridDim.x * blockDim.x)
-    hist_key<K>(keys[i], begin_bit, passes, desc, s_h, zc);
-#pragma unroll
-  for (int p = 0; p < 8; p++) {
-    if (p < passes) {
-      const uint32_t z = wave_sum(zc[p]);
-      if (lane_id() == 0 && z) atomicAdd(&s_h[p * kBins + desc], z);
-    }
-  }
-  __syncthreads();
-  for (int i = threadIdx.x; i < passes * kBins; i += blockDim.x) {
-    uint32_t v = s_h[i];
-    if (v) atomicAdd(&hist[i], v);
-  }
+  const uint64_t chunk = ceil_div(ceil_div(n, gridDim.x), 64) * 64;
+  const uint64_t begin = (uint64_t)blockIdx.x * chunk;
+  const uint64_t end = begin + chunk < n ? begin + chunk : n;
+  if (begin >= end) return;  // whole blocks leave together
+  hist_block<K>(keys, begin, end, begin_bit, passes, desc, s_h, hist);
 }
 
 // Zeroes [p, p + words) of 32-bit words.
@@ -102,159 +41,22 @@ __global__ void k_copy_pairs(const K *ks, const uint32_t *vs, K *kd, uint32_t *v
   }
 }
 
-// ---------------------------------------------------------------------------
-// One digit pass.
-//   kin/vin -> kout/vout; vin == nullptr means the payload is val_base + row.
-//   hist: the 256 counts of this digit over all keys.
-//   status: ntiles * 256 words of this pass, zeroed; status_next: the same
-//   area for the following pass, zeroed here tile by tile (it was last used
-//   two passes ago, whose kernel has finished).
-// ---------------------------------------------------------------------------
+// One digit pass; tiles are taken in launch order from *ctr.
 template <typename K, int BLOCK, int KPT, bool EARLY>
 __global__ __launch_bounds__(BLOCK) void k_onesweep(const K *__restrict__ kin, const uint32_t *__restrict__ vin,
                                                     uint32_t val_base, K *__restrict__ kout,
                                                     uint32_t *__restrict__ vout, uint32_t n, int shift, uint32_t desc,
-                                                    const uint32_t *__restrict__ hist,
-                                                    uint32_t *status, uint32_t *status_next, uint32_t *ctr) {
-  static_assert(BLOCK >= kBins, "one thread per digit");
-  constexpr int W = BLOCK / kWave;
-  constexpr int TILE = BLOCK * KPT;
-  __shared__ K s_k[TILE];
-  __shared__ uint32_t s_wh[W][kBins];
-  __shared__ uint32_t s_dstart[kBins];
-  __shared__ uint32_t s_glob[kBins];
-  __shared__ uint32_t s_red[W + 1];
+                                                    const uint32_t *__restrict__ hist, uint32_t *status,
+                                                    uint32_t *status_next, uint32_t *ctr) {
+  __shared__ OnesweepSmem<K, BLOCK, KPT> sm;
   __shared__ uint32_t s_tile;
-
-  const int t = threadIdx.x;
-  const int lane = lane_id();
-  const int wave = t / kWave;
-  if (t == 0) s_tile = atomicAdd(ctr, 1u);
-  for (int i = t; i < W * kBins; i += BLOCK) (&s_wh[0][0])[i] = 0;
+  if (threadIdx.x == 0) s_tile = atomicAdd(ctr, 1u);
   __syncthreads();
-  const uint32_t tile = s_tile;
-  const uint32_t base = tile * (uint32_t)TILE;
-  if (status_next && t < kBins) status_next[tile * kBins + t] = 0;
-
-  // ---- load: wave w owns rows [base + w*KPT*64, ...), item i lane l -> row
-  K key[KPT];
-  uint32_t val[KPT];
-  const uint32_t wbase = base + (uint32_t)wave * KPT * kWave;
-#pragma unroll
-  for (int i = 0; i < KPT; i++) {
-    const uint32_t idx = wbase + i * kWave + lane;
-    const bool valid = idx < n;
-    key[i] = valid ? kin[idx] : (K)0;
-    val[i] = vin ? (valid ? vin[idx] : 0u) : val_base + idx;
-  }
-
-  // ---- rank inside the wave, in (item, lane) order
-  uint32_t rank[KPT];
-#pragma unroll
-  for (int i = 0; i < KPT; i++) {
-    const uint32_t idx = wbase + i * kWave + lane;
-    const bool valid = idx < n;
-    const uint32_t d = ((uint32_t)(key[i] >> shift) & 0xFFu) ^ desc;
-    uint64_t m = ballot(valid);
-#pragma unroll
-    for (int b = 0; b < 8; b++) {
-      const bool bit = (d >> b) & 1u;
-      const uint64_t bal = ballot(bit);
-      m &= bit ? bal : ~bal;
-    }
-    const uint32_t lower = popc_below(m);
-    uint32_t pre = 0;
-    if (valid) pre = s_wh[wave][d];
-    rank[i] = pre + lower;
-    if (valid && lower == 0) s_wh[wave][d] = pre + (uint32_t)__popcll(m);
-  }
-  __syncthreads();
-
-  // ---- per-digit tile counts, wave offsets, publication
-  uint32_t cnt = 0;
-  if (t < kBins) {
-    uint32_t run = 0;
-#pragma unroll
-    for (int w = 0; w < W; w++) {
-      uint32_t c = s_wh[w][t];
-      s_wh[w][t] = run;
-      run += c;
-    }
-    cnt = run;
-    rstatus_store(&status[tile * kBins + t], (tile == 0 ? kRFlagInc : kRFlagAgg) | cnt);
-  }
-  // Decoupled look-back, one thread per digit.  EARLY walks back right
-  // after publishing the aggregate, so this tile's inclusive prefix (which
-  // later tiles wait for) appears as soon as possible.
-  uint32_t excl = 0;
-  auto lookback = [&]() {
-    if (t < kBins && tile > 0) {
-      int64_t k = (int64_t)tile - 1;
-      while (true) {
-        const uint32_t s = rstatus_load(&status[(uint32_t)k * kBins + t]);
-        const uint32_t f = s & ~kRValMask;
-        if (f == 0) continue;
-        excl += s & kRValMask;
-        if (f == kRFlagInc) break;
-        k--;
-      }
-      rstatus_store(&status[tile * kBins + t], kRFlagInc | (excl + cnt));
-    }
-  };
-  if constexpr (EARLY) lookback();
-  uint32_t tot;
-  const uint32_t dstart = block_exclusive_sum<BLOCK, uint32_t>(t < kBins ? cnt : 0u, s_red, &tot);
-  const uint32_t h = t < kBins ? hist[t] : 0u;
-  const uint32_t gstart = block_exclusive_sum<BLOCK, uint32_t>(h, s_red, &tot);
-  if (t < kBins) s_dstart[t] = dstart;
-  __syncthreads();
-
-  // ---- reorder the tile's keys in LDS by digit (payloads follow later
-  // through the same LDS area, which keeps the block at 36 KB of LDS)
-  uint32_t pos[KPT];
-#pragma unroll
-  for (int i = 0; i < KPT; i++) {
-    const uint32_t idx = wbase + i * kWave + lane;
-    pos[i] = 0xFFFFFFFFu;
-    if (idx < n) {
-      const uint32_t d = ((uint32_t)(key[i] >> shift) & 0xFFu) ^ desc;
-      pos[i] = s_dstart[d] + s_wh[wave][d] + rank[i];
-      s_k[pos[i]] = key[i];
-    }
-  }
-
-  if constexpr (!EARLY) lookback();
-  if (t < kBins) s_glob[t] = gstart + excl - dstart;
-  __syncthreads();
-
-  // ---- write each digit's run of keys to its global position
-  const uint32_t valid_cnt = (n - base) < (uint32_t)TILE ? (n - base) : (uint32_t)TILE;
-  uint32_t opos[KPT];
-#pragma unroll
-  for (int q = 0; q < KPT; q++) {
-    const uint32_t j = t + q * BLOCK;
-    opos[q] = 0;
-    if (j < valid_cnt) {
-      const K kk = s_k[j];
-      const uint32_t d = ((uint32_t)(kk >> shift) & 0xFFu) ^ desc;
-      opos[q] = s_glob[d] + j;
-      kout[opos[q]] = kk;
-    }
-  }
-  __syncthreads();
-  // ---- then the payloads, through the same LDS area
-  uint32_t *s_v = reinterpret_cast<uint32_t *>(s_k);
-#pragma unroll
-  for (int i = 0; i < KPT; i++)
-    if (pos[i] != 0xFFFFFFFFu) s_v[pos[i]] = val[i];
-  __syncthreads();
-#pragma unroll
-  for (int q = 0; q < KPT; q++) {
-    const uint32_t j = t + q * BLOCK;
-    if (j < valid_cnt) vout[opos[q]] = s_v[j];
-  }
+  onesweep_tile<K, BLOCK, KPT, EARLY>(sm, s_tile, kin, vin, val_base, kout, vout, n, shift, desc, hist, status,
+                                      status_next);
 }
 
+// ---------------------------------------------------------------------------
 // ---------------------------------------------------------------------------
 // Host driver
 // ---------------------------------------------------------------------------

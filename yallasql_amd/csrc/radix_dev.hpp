@@ -1,0 +1,248 @@
+// radix_dev.hpp — device-side building blocks of the 8-bit LSD radix sort,
+// shared by the per-call kernels (radix.hip) and the persistent batched
+// executor (batched.hip).
+//
+//   hist_block:    one workgroup adds the 256-bin histograms of every digit
+//                  position of keys[begin, end) into a global histogram.
+//   onesweep_tile: one workgroup performs one digit pass over one tile:
+//                  wave-level ranking with 8 ballots (lanes with the same
+//                  digit match on all 8 digit bits; mbcnt gives the rank), per
+//                  digit tile counts published for decoupled look-back (one
+//                  thread per digit walks back over predecessor tiles), the
+//                  tile reordered by digit in LDS, then each digit's run of
+//                  keys and payloads written to its global position.
+// Stability: keys are ranked in (wave, item, lane) order, which is the input
+// order inside the tile, and tiles are placed in tile order.
+#pragma once
+
+#include "common.hpp"
+
+namespace ysql {
+
+constexpr int kBins = 256;
+constexpr uint32_t kRFlagAgg = 1u << 30;
+constexpr uint32_t kRFlagInc = 2u << 30;
+constexpr uint32_t kRValMask = (1u << 30) - 1;
+
+__device__ __forceinline__ uint32_t rstatus_load(const uint32_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void rstatus_store(uint32_t *p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Keys narrower than their type (small integers, Zipf ranks) put most of
+// their high digits in one bin; 64 lanes adding to one LDS word serialise.
+// Each thread therefore counts the bin of digit value `desc` (the encoded
+// zero digit) in registers and only adds the other bins to LDS.
+template <typename K>
+__device__ __forceinline__ void hist_key(K k, int begin_bit, int passes, uint32_t desc, uint32_t *s_h,
+                                         uint32_t (&zc)[8]) {
+#pragma unroll
+  for (int p = 0; p < 8; p++) {
+    if (p < passes) {
+      const uint32_t d = ((uint32_t)(k >> (begin_bit + 8 * p)) & 0xFFu) ^ desc;
+      if (d == desc) zc[p]++;
+      else atomicAdd(&s_h[p * kBins + d], 1u);
+    }
+  }
+}
+
+// All threads of the block: histograms of keys[begin, end) (s_h: 8 * 256
+// words of LDS) added into hist (passes * 256 words).  Contains barriers.
+template <typename K>
+__device__ __forceinline__ void hist_block(const K *keys, uint64_t begin, uint64_t end, int begin_bit, int passes,
+                                           uint32_t desc, uint32_t *s_h, uint32_t *hist) {
+  for (int i = threadIdx.x; i < passes * kBins; i += blockDim.x) s_h[i] = 0;
+  __syncthreads();
+  uint32_t zc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  constexpr int PER = 16 / sizeof(K);  // keys per 16-byte load
+  uint64_t i = begin;
+  // head until 16-byte alignment, then vector loads, then the tail
+  while (i < end && ((uintptr_t)(keys + i) & 15)) {
+    if (i - begin == threadIdx.x) hist_key<K>(keys[i], begin_bit, passes, desc, s_h, zc);
+    i++;
+  }
+  const uint64_t vstart = i;
+  const uint64_t nv = (end - vstart) / PER;
+  const uint4 *kv = reinterpret_cast<const uint4 *>(keys + vstart);
+  for (uint64_t v = threadIdx.x; v < nv; v += blockDim.x) {
+    const uint4 q = kv[v];
+    K ks[PER];
+    memcpy(ks, &q, 16);
+#pragma unroll
+    for (int e = 0; e < PER; e++) hist_key<K>(ks[e], begin_bit, passes, desc, s_h, zc);
+  }
+  for (uint64_t r = vstart + nv * PER + threadIdx.x; r < end; r += blockDim.x)
+    hist_key<K>(keys[r], begin_bit, passes, desc, s_h, zc);
+#pragma unroll
+  for (int p = 0; p < 8; p++) {
+    if (p < passes) {
+      const uint32_t z = wave_sum(zc[p]);
+      if (lane_id() == 0 && z) atomicAdd(&s_h[p * kBins + desc], z);
+    }
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < passes * kBins; b += blockDim.x) {
+    const uint32_t v = s_h[b];
+    if (v) atomicAdd(&hist[b], v);
+  }
+  __syncthreads();
+}
+
+template <typename K, int BLOCK, int KPT>
+struct OnesweepSmem {
+  static constexpr int W = BLOCK / kWave;
+  static constexpr int TILE = BLOCK * KPT;
+  K k[TILE];  // keys, then payloads, of the tile in digit order
+  uint32_t wh[W][kBins];
+  uint32_t dstart[kBins];
+  uint32_t glob[kBins];
+  uint32_t red[W + 1];
+};
+
+// One digit pass over tile `tile` of a sequence of n keys.
+//   kin/vin -> kout/vout; vin == nullptr means the payload is val_base + row.
+//   hist: the 256 counts of this digit over all n keys.
+//   status: ntiles * 256 words of this pass (zeroed before the pass);
+//   status_next: the same area for the following pass, zeroed here tile by
+//   tile (it was last read two passes ago, which has completed).
+// Every thread of the block calls it; contains barriers.
+template <typename K, int BLOCK, int KPT, bool EARLY>
+__device__ __forceinline__ void onesweep_tile(OnesweepSmem<K, BLOCK, KPT> &sm, uint32_t tile,
+                                              const K *__restrict__ kin, const uint32_t *__restrict__ vin,
+                                              uint32_t val_base, K *__restrict__ kout, uint32_t *__restrict__ vout,
+                                              uint32_t n, int shift, uint32_t desc,
+                                              const uint32_t *__restrict__ hist, uint32_t *status,
+                                              uint32_t *status_next) {
+  static_assert(BLOCK >= kBins, "one thread per digit");
+  constexpr int W = BLOCK / kWave;
+  constexpr int TILE = BLOCK * KPT;
+  const int t = threadIdx.x;
+  const int lane = lane_id();
+  const int wave = t / kWave;
+  for (int i = t; i < W * kBins; i += BLOCK) (&sm.wh[0][0])[i] = 0;
+  __syncthreads();
+  const uint32_t base = tile * (uint32_t)TILE;
+  if (status_next && t < kBins) status_next[tile * kBins + t] = 0;
+
+  // ---- load: wave w owns rows [base + w*KPT*64, ...), item i lane l -> row
+  K key[KPT];
+  uint32_t val[KPT];
+  const uint32_t wbase = base + (uint32_t)wave * KPT * kWave;
+#pragma unroll
+  for (int i = 0; i < KPT; i++) {
+    const uint32_t idx = wbase + i * kWave + lane;
+    const bool valid = idx < n;
+    key[i] = valid ? kin[idx] : (K)0;
+    val[i] = vin ? (valid ? vin[idx] : 0u) : val_base + idx;
+  }
+
+  // ---- rank inside the wave, in (item, lane) order
+  uint32_t rank[KPT];
+#pragma unroll
+  for (int i = 0; i < KPT; i++) {
+    const uint32_t idx = wbase + i * kWave + lane;
+    const bool valid = idx < n;
+    const uint32_t d = ((uint32_t)(key[i] >> shift) & 0xFFu) ^ desc;
+    uint64_t m = ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+      const bool bit = (d >> b) & 1u;
+      const uint64_t bal = ballot(bit);
+      m &= bit ? bal : ~bal;
+    }
+    const uint32_t lower = popc_below(m);
+    uint32_t pre = 0;
+    if (valid) pre = sm.wh[wave][d];
+    rank[i] = pre + lower;
+    if (valid && lower == 0) sm.wh[wave][d] = pre + (uint32_t)__popcll(m);
+  }
+  __syncthreads();
+
+  // ---- per-digit tile counts, wave offsets, publication
+  uint32_t cnt = 0;
+  if (t < kBins) {
+    uint32_t run = 0;
+#pragma unroll
+    for (int w = 0; w < W; w++) {
+      const uint32_t c = sm.wh[w][t];
+      sm.wh[w][t] = run;
+      run += c;
+    }
+    cnt = run;
+    rstatus_store(&status[tile * kBins + t], (tile == 0 ? kRFlagInc : kRFlagAgg) | cnt);
+  }
+  // Decoupled look-back, one thread per digit.  EARLY walks back right
+  // after publishing the aggregate; the default walks back after the LDS
+  // reorder (measured faster on MI355X).
+  uint32_t excl = 0;
+  auto lookback = [&]() {
+    if (t < kBins && tile > 0) {
+      int64_t k = (int64_t)tile - 1;
+      while (true) {
+        const uint32_t s = rstatus_load(&status[(uint32_t)k * kBins + t]);
+        const uint32_t f = s & ~kRValMask;
+        if (f == 0) continue;
+        excl += s & kRValMask;
+        if (f == kRFlagInc) break;
+        k--;
+      }
+      rstatus_store(&status[tile * kBins + t], kRFlagInc | (excl + cnt));
+    }
+  };
+  if constexpr (EARLY) lookback();
+  uint32_t tot;
+  const uint32_t dstart = block_exclusive_sum<BLOCK, uint32_t>(t < kBins ? cnt : 0u, sm.red, &tot);
+  const uint32_t h = t < kBins ? hist[t] : 0u;
+  const uint32_t gstart = block_exclusive_sum<BLOCK, uint32_t>(h, sm.red, &tot);
+  if (t < kBins) sm.dstart[t] = dstart;
+  __syncthreads();
+
+  // ---- reorder the tile's keys in LDS by digit (payloads follow later
+  // through the same LDS area)
+  uint32_t pos[KPT];
+#pragma unroll
+  for (int i = 0; i < KPT; i++) {
+    const uint32_t idx = wbase + i * kWave + lane;
+    pos[i] = 0xFFFFFFFFu;
+    if (idx < n) {
+      const uint32_t d = ((uint32_t)(key[i] >> shift) & 0xFFu) ^ desc;
+      pos[i] = sm.dstart[d] + sm.wh[wave][d] + rank[i];
+      sm.k[pos[i]] = key[i];
+    }
+  }
+  if constexpr (!EARLY) lookback();
+  if (t < kBins) sm.glob[t] = gstart + excl - dstart;
+  __syncthreads();
+
+  // ---- write each digit's run of keys to its global position
+  const uint32_t valid_cnt = (n - base) < (uint32_t)TILE ? (n - base) : (uint32_t)TILE;
+  uint32_t opos[KPT];
+#pragma unroll
+  for (int q = 0; q < KPT; q++) {
+    const uint32_t j = t + q * BLOCK;
+    opos[q] = 0;
+    if (j < valid_cnt) {
+      const K kk = sm.k[j];
+      const uint32_t d = ((uint32_t)(kk >> shift) & 0xFFu) ^ desc;
+      opos[q] = sm.glob[d] + j;
+      kout[opos[q]] = kk;
+    }
+  }
+  __syncthreads();
+  // ---- then the payloads, through the same LDS area
+  uint32_t *s_v = reinterpret_cast<uint32_t *>(sm.k);
+#pragma unroll
+  for (int i = 0; i < KPT; i++)
+    if (pos[i] != 0xFFFFFFFFu) s_v[pos[i]] = val[i];
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < KPT; q++) {
+    const uint32_t j = t + q * BLOCK;
+    if (j < valid_cnt) vout[opos[q]] = s_v[j];
+  }
+  __syncthreads();  // LDS is reused by the caller's next tile
+}
+
+}  // namespace ysql

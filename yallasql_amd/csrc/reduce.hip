@@ -653,3 +653,111 @@ int ysql_star_join_group_sum(const ysql_star_dim *dims, int ndims, const int32_t
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Filtered multi-aggregate (agg_dev.hpp), per-call form.
+// ---------------------------------------------------------------------------
+#include "agg_dev.hpp"
+
+namespace ysql {
+
+struct FaArgs {
+  FaSpec spec[kFaMaxSpecs];
+  ysql_acc *out[kFaMaxSpecs];
+};
+
+template <typename PT, int CMP>
+__global__ __launch_bounds__(kFaBlock) void k_filter_agg(const PT *pred, const PT *scalar, FaArgs a, int ns, uint64_t n,
+                                                         ysql_acc *partials, uint32_t *ticket, uint32_t ntiles) {
+  __shared__ ysql_acc s_w[kFaBlock / kWave];
+  __shared__ int s_last;
+  const PT sc = scalar[0];
+  for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x)
+    filter_agg_tile<PT, CMP, kFaBlock>(tile, pred, sc, a.spec, ns, n, partials, s_w);
+  __threadfence();
+  if (threadIdx.x == 0) s_last = (atomicAdd(ticket, 1u) == gridDim.x - 1);
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();
+  filter_agg_finish<kFaBlock>(a.spec, ns, ntiles, partials, a.out, s_w);
+  if (threadIdx.x == 0) *ticket = 0;
+}
+
+template <typename PT>
+static int filter_agg_launch(int cmp, const void *pred, const void *scalar, const FaArgs &a, int ns, uint64_t n,
+                             ysql_acc *partials, uint32_t *ticket, hipStream_t s) {
+  const uint32_t ntiles = (uint32_t)ceil_div(n, kFaTile);
+  uint32_t g = ntiles;
+  const uint32_t cap = (uint32_t)grid_cap() * 2;
+  if (g > cap) g = cap;
+  const PT *p = (const PT *)pred;
+  const PT *sc = (const PT *)scalar;
+#define FA_CASE(C) \
+  case C: k_filter_agg<PT, C><<<g, kFaBlock, 0, s>>>(p, sc, a, ns, n, partials, ticket, ntiles); break;
+  switch (cmp) {
+    FA_CASE(YSQL_LT)
+    FA_CASE(YSQL_LE)
+    FA_CASE(YSQL_GT)
+    FA_CASE(YSQL_GE)
+    FA_CASE(YSQL_EQ)
+    FA_CASE(YSQL_NE)
+    default: return YSQL_EINVAL;
+  }
+#undef FA_CASE
+  YSQL_LAUNCH_CHECK();
+  return YSQL_OK;
+}
+
+static size_t filter_agg_ws_size(uint64_t n, int ns) {
+  WsCounter c;
+  c.take<uint32_t>(64);
+  c.take<ysql_acc>(ceil_div(n, kFaTile) * (uint64_t)ns + 1);
+  return c.off;
+}
+
+}  // namespace ysql
+
+extern "C" {
+
+int ysql_filter_aggregate_ws_bytes(uint64_t n, int nspecs, size_t *bytes) {
+  if (!bytes || nspecs < 1 || nspecs > kFaMaxSpecs) return YSQL_EINVAL;
+  *bytes = filter_agg_ws_size(n, nspecs);
+  return YSQL_OK;
+}
+
+int ysql_filter_aggregate(int pred_dtype, int cmp, const void *pred_col, const void *scalar, const ysql_agg_spec *specs,
+                          int nspecs, uint64_t n, void *ws, size_t ws_bytes, ysql_stream_t stream) {
+  if (nspecs < 1 || nspecs > kFaMaxSpecs || !specs || cmp < YSQL_LT || cmp > YSQL_NE) return YSQL_EINVAL;
+  FaArgs a;
+  memset(&a, 0, sizeof(a));
+  for (int s = 0; s < nspecs; s++) {
+    const ysql_agg_spec &x = specs[s];
+    if (!x.acc || x.agg < YSQL_AGG_MAX || x.agg > YSQL_AGG_COUNT_STAR) return YSQL_EINVAL;
+    const bool counting = x.agg == YSQL_AGG_COUNT || x.agg == YSQL_AGG_COUNT_STAR;
+    if (!counting && (!x.col || (x.dtype != YSQL_INT && x.dtype != YSQL_FLOAT && x.dtype != YSQL_DATETIME)))
+      return YSQL_EINVAL;
+    a.spec[s].kind = fa_kind(x.dtype, x.agg);
+    a.spec[s].op = fa_op(x.agg);
+    a.spec[s].dtype = x.dtype;
+    a.spec[s].col = x.col;
+    a.out[s] = x.acc;
+  }
+  if (n == 0) return YSQL_OK;
+  if (!pred_col || !scalar) return YSQL_EINVAL;
+  if (n >= (1ull << 40)) return YSQL_EINVAL;
+  WsCarver c(ws, ws_bytes);
+  uint32_t *ticket = c.take<uint32_t>(64);
+  ysql_acc *partials = c.take<ysql_acc>(ceil_div(n, kFaTile) * (uint64_t)nspecs + 1);
+  if (!ws || !ticket || !partials) return YSQL_EWORKSPACE;
+  hipStream_t s = as_stream(stream);
+  k_reduce_reset<<<1, 1, 0, s>>>(ticket);
+  YSQL_LAUNCH_CHECK();
+  switch (pred_dtype) {
+    case YSQL_INT: return filter_agg_launch<int32_t>(cmp, pred_col, scalar, a, nspecs, n, partials, ticket, s);
+    case YSQL_FLOAT: return filter_agg_launch<float>(cmp, pred_col, scalar, a, nspecs, n, partials, ticket, s);
+    case YSQL_DATETIME: return filter_agg_launch<int64_t>(cmp, pred_col, scalar, a, nspecs, n, partials, ticket, s);
+  }
+  return YSQL_EINVAL;
+}
+
+}  // extern "C"

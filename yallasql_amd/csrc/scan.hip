@@ -8,7 +8,7 @@
 // Brent-Kung scan with a serial carry (prefix_sum_kernel.cu:7-103), its
 // separate mask, map and per-column scatter passes (filter_operator.hpp:31-98)
 // become one pass.
-#include "common.hpp"
+#include "scan_dev.hpp"
 
 namespace ysql {
 
@@ -177,114 +177,15 @@ struct ColArgs {
   ysql_col c[YSQL_MAX_COLS];
 };
 
-template <typename PT>
-__device__ __forceinline__ void load16(const PT *p, uint64_t base, uint64_t rows, int t, PT (&v)[16]) {
-  const uint64_t r0 = (uint64_t)t * 16;
-  if (r0 + 16 <= rows && ((uintptr_t)(p + base + r0) & 15) == 0) {
-    const uint4 *q = (const uint4 *)(p + base + r0);
-    constexpr int NQ = 16 * sizeof(PT) / 16;
-    uint4 tmp[NQ];
-#pragma unroll
-    for (int i = 0; i < NQ; i++) tmp[i] = q[i];
-    memcpy(&v[0], &tmp[0], sizeof(tmp));
-  } else {
-#pragma unroll
-    for (int e = 0; e < 16; e++) v[e] = (r0 + e < rows) ? p[base + r0 + e] : PT();
-  }
-}
-
 template <int MODE, typename PT, int CMP>
 __global__ __launch_bounds__(kScanBlock) void k_compact(const uint8_t *mask, const PT *pred, const PT *scalar, ColArgs cols,
                                                         int ncols, uint64_t n, uint64_t *count_out, uint32_t *ctr,
                                                         uint64_t *status, uint32_t ntiles) {
-  __shared__ uint16_t s_src[kScanTile];
-  __shared__ uint64_t s_red[kScanBlock / kWave + 1];
+  __shared__ CompactSmem<kScanBlock> sm;
   __shared__ uint32_t s_tile;
-  __shared__ uint64_t s_prefix;
   const uint32_t tile = take_tile(ctr, &s_tile);
-  const uint64_t base = (uint64_t)tile * kScanTile;
-  const int t = threadIdx.x;
-  const uint64_t rows = (n - base) < (uint64_t)kScanTile ? (n - base) : (uint64_t)kScanTile;
-
-  uint32_t flags = 0;
-  if constexpr (MODE == 0) {
-    uint8_t m[16];
-    load16<uint8_t>(mask, base, rows, t, m);
-#pragma unroll
-    for (int e = 0; e < 16; e++) flags |= (uint32_t)(m[e] != 0) << e;
-  } else {
-    PT v[16];
-    load16<PT>(pred, base, rows, t, v);
-    const PT s = scalar[0];
-#pragma unroll
-    for (int e = 0; e < 16; e++) flags |= (uint32_t)cmp_static<CMP>(v[e], s) << e;
-  }
-  {
-    const uint64_t r0 = (uint64_t)t * 16;
-    if (r0 + 16 > rows) flags &= (r0 >= rows) ? 0u : ((1u << (rows - r0)) - 1u);
-  }
-  const uint32_t cnt = __builtin_popcount(flags);
-  uint64_t total;
-  const uint32_t excl = (uint32_t)block_exclusive_sum<kScanBlock, uint64_t>(cnt, s_red, &total);
-  if (t < kWave) {
-    uint64_t p = lookback_exclusive(status, tile, total);
-    if (t == 0) s_prefix = p;
-  }
-  {
-    uint32_t k = excl;
-    uint32_t f = flags;
-    while (f) {
-      int e = __builtin_ctz(f);
-      f &= f - 1;
-      s_src[k++] = (uint16_t)(t * 16 + e);
-    }
-  }
-  __syncthreads();
-  const uint64_t prefix = s_prefix;
-  const uint32_t tot = (uint32_t)total;
-  for (int c = 0; c < ncols; c++) {
-    const ysql_col col = cols.c[c];
-    switch (col.elem_bytes) {
-      case 4: {
-        const uint32_t *src = (const uint32_t *)col.src;
-        uint32_t *dst = (uint32_t *)col.dst;
-        for (uint32_t r = t; r < tot; r += kScanBlock) dst[prefix + r] = src[base + s_src[r]];
-        break;
-      }
-      case 8: {
-        const uint64_t *src = (const uint64_t *)col.src;
-        uint64_t *dst = (uint64_t *)col.dst;
-        for (uint32_t r = t; r < tot; r += kScanBlock) dst[prefix + r] = src[base + s_src[r]];
-        break;
-      }
-      case 1: {
-        const uint8_t *src = (const uint8_t *)col.src;
-        uint8_t *dst = (uint8_t *)col.dst;
-        for (uint32_t r = t; r < tot; r += kScanBlock) dst[prefix + r] = src[base + s_src[r]];
-        break;
-      }
-      case 2: {
-        const uint16_t *src = (const uint16_t *)col.src;
-        uint16_t *dst = (uint16_t *)col.dst;
-        for (uint32_t r = t; r < tot; r += kScanBlock) dst[prefix + r] = src[base + s_src[r]];
-        break;
-      }
-      default: {  // 256-byte strings: 16 chunks of 16 bytes per row
-        const uint4 *src = (const uint4 *)col.src;
-        uint4 *dst = (uint4 *)col.dst;
-        for (uint32_t x = t; x < tot * 16; x += kScanBlock) {
-          uint32_t r = x >> 4, ch = x & 15;
-          dst[(prefix + r) * 16 + ch] = src[(base + s_src[r]) * 16 + ch];
-        }
-        break;
-      }
-    }
-    if (col.dst_null) {
-      for (uint32_t r = t; r < tot; r += kScanBlock)
-        col.dst_null[prefix + r] = col.src_null ? col.src_null[base + s_src[r]] : 0;
-    }
-  }
-  if (tile == ntiles - 1 && t == 0 && count_out) *count_out = prefix + tot;
+  const PT s = MODE == 1 ? scalar[0] : PT();
+  compact_tile<kScanBlock, MODE, PT, CMP>(sm, tile, mask, pred, s, cols.c, ncols, n, count_out, status, ntiles);
 }
 
 static bool cols_ok(const ysql_col *cols, int ncols, bool need_ptrs) {

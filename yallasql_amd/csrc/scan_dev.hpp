@@ -1,0 +1,132 @@
+// scan_dev.hpp — device-side stream compaction of one tile, shared by the
+// per-call compaction kernels (scan.hip) and the batched executor.
+//
+// Each thread owns 16 consecutive rows of a tile of BLOCK * 16 rows: it
+// computes their selection flags (from a byte mask or a fused predicate),
+// the block scans the per-thread counts, wave 0 finds the tile's exclusive
+// prefix by decoupled look-back over 64-bit status words, the selected rows'
+// tile offsets are listed in LDS in order, and every column is copied with
+// one output row per thread per step (coalesced stores, loads inside the
+// tile's cache lines).  Stable: selected rows keep their order.
+#pragma once
+
+#include "common.hpp"
+
+namespace ysql {
+
+template <int BLOCK>
+struct CompactSmem {
+  uint16_t src[BLOCK * 16];
+  uint64_t red[BLOCK / kWave + 1];
+  uint64_t prefix;
+};
+
+template <typename PT>
+__device__ __forceinline__ void load16(const PT *p, uint64_t base, uint64_t rows, int t, PT (&v)[16]) {
+  const uint64_t r0 = (uint64_t)t * 16;
+  if (r0 + 16 <= rows && ((uintptr_t)(p + base + r0) & 15) == 0) {
+    const uint4 *q = (const uint4 *)(p + base + r0);
+    constexpr int NQ = 16 * sizeof(PT) / 16;
+    uint4 tmp[NQ];
+#pragma unroll
+    for (int i = 0; i < NQ; i++) tmp[i] = q[i];
+    memcpy(&v[0], &tmp[0], sizeof(tmp));
+  } else {
+#pragma unroll
+    for (int e = 0; e < 16; e++) v[e] = (r0 + e < rows) ? p[base + r0 + e] : PT();
+  }
+}
+
+// MODE 0: selection from mask bytes; MODE 1: pred[i] <CMP> scalar.
+// Every thread of the block calls it; contains barriers.
+template <int BLOCK, int MODE, typename PT, int CMP>
+__device__ __forceinline__ void compact_tile(CompactSmem<BLOCK> &sm, uint32_t tile, const uint8_t *mask,
+                                             const PT *pred, PT scalar, const ysql_col *cols, int ncols, uint64_t n,
+                                             uint64_t *count_out, uint64_t *status, uint32_t ntiles) {
+  constexpr int TILE = BLOCK * 16;
+  const uint64_t base = (uint64_t)tile * TILE;
+  const int t = threadIdx.x;
+  const uint64_t rows = (n - base) < (uint64_t)TILE ? (n - base) : (uint64_t)TILE;
+
+  uint32_t flags = 0;
+  if constexpr (MODE == 0) {
+    uint8_t m[16];
+    load16<uint8_t>(mask, base, rows, t, m);
+#pragma unroll
+    for (int e = 0; e < 16; e++) flags |= (uint32_t)(m[e] != 0) << e;
+  } else {
+    PT v[16];
+    load16<PT>(pred, base, rows, t, v);
+#pragma unroll
+    for (int e = 0; e < 16; e++) flags |= (uint32_t)cmp_static<CMP>(v[e], scalar) << e;
+  }
+  {
+    const uint64_t r0 = (uint64_t)t * 16;
+    if (r0 + 16 > rows) flags &= (r0 >= rows) ? 0u : ((1u << (rows - r0)) - 1u);
+  }
+  const uint32_t cnt = __builtin_popcount(flags);
+  uint64_t total;
+  const uint32_t excl = (uint32_t)block_exclusive_sum<BLOCK, uint64_t>(cnt, sm.red, &total);
+  if (t < kWave) {
+    const uint64_t p = lookback_exclusive(status, tile, total);
+    if (t == 0) sm.prefix = p;
+  }
+  {
+    uint32_t k = excl;
+    uint32_t f = flags;
+    while (f) {
+      const int e = __builtin_ctz(f);
+      f &= f - 1;
+      sm.src[k++] = (uint16_t)(t * 16 + e);
+    }
+  }
+  __syncthreads();
+  const uint64_t prefix = sm.prefix;
+  const uint32_t tot = (uint32_t)total;
+  for (int c = 0; c < ncols; c++) {
+    const ysql_col col = cols[c];
+    switch (col.elem_bytes) {
+      case 4: {
+        const uint32_t *src = (const uint32_t *)col.src;
+        uint32_t *dst = (uint32_t *)col.dst;
+        for (uint32_t r = t; r < tot; r += BLOCK) dst[prefix + r] = src[base + sm.src[r]];
+        break;
+      }
+      case 8: {
+        const uint64_t *src = (const uint64_t *)col.src;
+        uint64_t *dst = (uint64_t *)col.dst;
+        for (uint32_t r = t; r < tot; r += BLOCK) dst[prefix + r] = src[base + sm.src[r]];
+        break;
+      }
+      case 1: {
+        const uint8_t *src = (const uint8_t *)col.src;
+        uint8_t *dst = (uint8_t *)col.dst;
+        for (uint32_t r = t; r < tot; r += BLOCK) dst[prefix + r] = src[base + sm.src[r]];
+        break;
+      }
+      case 2: {
+        const uint16_t *src = (const uint16_t *)col.src;
+        uint16_t *dst = (uint16_t *)col.dst;
+        for (uint32_t r = t; r < tot; r += BLOCK) dst[prefix + r] = src[base + sm.src[r]];
+        break;
+      }
+      default: {  // 256-byte strings: 16 chunks of 16 bytes per row
+        const uint4 *src = (const uint4 *)col.src;
+        uint4 *dst = (uint4 *)col.dst;
+        for (uint32_t x = t; x < tot * 16; x += BLOCK) {
+          const uint32_t r = x >> 4, ch = x & 15;
+          dst[(prefix + r) * 16 + ch] = src[(base + sm.src[r]) * 16 + ch];
+        }
+        break;
+      }
+    }
+    if (col.dst_null) {
+      for (uint32_t r = t; r < tot; r += BLOCK)
+        col.dst_null[prefix + r] = col.src_null ? col.src_null[base + sm.src[r]] : 0;
+    }
+  }
+  if (tile == ntiles - 1 && t == 0 && count_out) *count_out = prefix + tot;
+  __syncthreads();  // LDS is reused by the caller's next tile
+}
+
+}  // namespace ysql
