@@ -376,13 +376,14 @@ int ysql_star_join_group_sum(const ysql_star_dim *dims, int ndims, const int32_t
 
 /* ======================================================================
  * Batched ragged jobs (no reference counterpart: the reference launched
- * every operator of every batch separately).  Thousands of independent
- * sort / filter / aggregate jobs run as ONE persistent-kernel launch that
- * drains a device work queue of tile tasks (per-job phase counters order a
- * job's histogram, digit passes and finalisation), captured once in a
- * hipGraph and replayed.  ysql_batch_run_plain runs the same jobs as a
- * per-job launch loop of the per-call entry points; both produce the same
- * bits (same tile functions, same fold orders).
+ * every operator of every batch separately, one after another).  Thousands
+ * of independent sort / filter / aggregate jobs are split over K chains
+ * (largest job first onto the least-loaded chain), each chain with its own
+ * stream and workspace, and every chain's launches are captured once into
+ * one hipGraph; ysql_batch_run replays it with one host call and the chains
+ * overlap on the device.  ysql_batch_run_plain issues the same per-call
+ * entry points job after job on one stream.  Both run the same kernels in
+ * the same order per job, so they produce the same bits.
  *   SORT:   stable argsort of keys (UKEY32/UKEY64): keys_out = sorted keys,
  *           perm_out[pos] = row.
  *   FILTER: rows with vals[i] < threshold; keys_out / vals_out = selected
@@ -411,17 +412,21 @@ typedef struct ysql_job {
 typedef struct ysql_batch ysql_batch;
 /* Tuning of the batched executor (ysql_batch_tuning_default fills it). */
 typedef struct ysql_batch_tuning {
-  int small_sort_max;   /* sort jobs up to this many rows run whole in one workgroup */
-  int hist_chunk;       /* rows per histogram task of a larger sort job */
-  int workers_per_cu;   /* persistent workgroups per CU (1 or 2) */
-  int pad;
+  int small_sort_max;   /* reserved (0) */
+  int hist_chunk;       /* reserved */
+  int workers_per_cu;   /* reserved */
+  int chains;           /* independent launch chains in the graph (1..64) */
 } ysql_batch_tuning;
 void ysql_batch_tuning_default(ysql_batch_tuning *t);
+/* Validates the jobs, allocates every workspace and captures the graph.
+ * Inputs and outputs must not move while the batch exists. */
 int ysql_batch_create(const ysql_job *jobs, int njobs, const ysql_batch_tuning *tuning,
                       ysql_batch **out);
+/* Replays the captured graph on `stream` (one host call). */
 int ysql_batch_run(ysql_batch *b, ysql_stream_t stream);
+/* The same jobs as a per-job launch loop on `stream`. */
 int ysql_batch_run_plain(ysql_batch *b, ysql_stream_t stream);
-/* Queue length, persistent grid size and workspace bytes of a batch. */
+/* Job count, chain count and workspace bytes of a batch. */
 int ysql_batch_info(const ysql_batch *b, uint64_t *tasks, uint32_t *grid, uint64_t *ws_bytes);
 void ysql_batch_destroy(ysql_batch *b);
 

@@ -1,6 +1,6 @@
-"""Batched persistent-kernel executor: the same ragged jobs run as one
-captured launch and as the per-job launch loop give identical bits, and
-every job matches the oracle."""
+"""Batched executor: the same ragged jobs replayed as one captured graph of
+parallel launch chains and run as the per-job launch loop give identical
+bits, every job matches the oracle, and graph replays repeat the bits."""
 import math
 
 import numpy as np
@@ -9,11 +9,6 @@ import pytest
 from oracle import pyoracle as O
 
 pytestmark = pytest.mark.gpu
-
-persistent_off = pytest.mark.skip(
-    reason="persistent batched launch hung on MI355X (cause not identified); "
-    "ysql_batch_run is disabled and returns YSQL_EUNSUPPORTED")
-
 
 def make_jobs(seed=0, extra=60, hi=300_000):
     r = np.random.default_rng(seed)
@@ -37,8 +32,7 @@ def jobset(dev):
     js.close()
 
 
-@persistent_off
-def test_batched_equals_plain_bitwise(jobset):
+def test_graph_batch_equals_plain_bitwise(jobset):
     jobset.clear_outputs()
     jobset.run_plain()
     plain = jobset.snapshot()
@@ -47,16 +41,22 @@ def test_batched_equals_plain_bitwise(jobset):
     batched = jobset.snapshot()
     for a, b in zip(plain, batched):
         assert np.array_equal(a, b)
-    # the captured graph replays to the same bits
-    jobset.clear_outputs()
-    jobset.run_batched()
-    again = jobset.snapshot()
-    for a, b in zip(batched, again):
-        assert np.array_equal(a, b)
+    # the captured graph replays to the same bits, with eager work and
+    # synchronisations between the replays
+    for _ in range(3):
+        jobset.clear_outputs()
+        jobset.run_plain()
+        jobset.snapshot()
+        jobset.clear_outputs()
+        jobset.run_batched()
+        again = jobset.snapshot()
+        for a, b in zip(batched, again):
+            assert np.array_equal(a, b)
 
 
 def check_jobs_against_oracle(jobset):
     from yallasql_amd.batched import AGG, FILTER, SORT
+
     for j in range(len(jobset.sizes)):
         keys, vals = jobset.job_inputs(j)
         kind, thr = jobset.kinds[j], np.float32(jobset.thr[j])
@@ -81,8 +81,7 @@ def check_jobs_against_oracle(jobset):
                 assert accs[1].i == int(keys[m].min()) and accs[2].i == int(keys[m].max()), j
 
 
-@persistent_off
-def test_batched_jobs_match_oracle(jobset):
+def test_graph_batch_jobs_match_oracle(jobset):
     jobset.clear_outputs()
     jobset.run_batched()
     check_jobs_against_oracle(jobset)
@@ -94,13 +93,23 @@ def test_plain_loop_jobs_match_oracle(jobset):
     check_jobs_against_oracle(jobset)
 
 
-def test_batch_run_disabled(jobset):
-    from yallasql_amd._lib import YsqlError
-
-    with pytest.raises(YsqlError):
-        jobset.run_batched()
-
-
 def test_batch_info(jobset):
     info = jobset.info()
-    assert info["tasks"] > 0 and info["grid"] >= 256
+    assert info["jobs"] == len(jobset.sizes) and 1 <= info["chains"] <= 64
+
+
+def test_single_chain_equals_plain(dev):
+    from yallasql_amd.batched import JobSet
+
+    sizes, kinds, key64, thr = make_jobs(seed=5, extra=12, hi=50_000)
+    js = JobSet(sizes, kinds, key64, thr, dev, seed=2).create(chains=1)
+    try:
+        js.clear_outputs()
+        js.run_plain()
+        plain = js.snapshot()
+        js.clear_outputs()
+        js.run_batched()
+        for a, b in zip(plain, js.snapshot()):
+            assert np.array_equal(a, b)
+    finally:
+        js.close()

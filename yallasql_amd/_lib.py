@@ -140,7 +140,7 @@ class Job(C.Structure):
 
 class BatchTuning(C.Structure):
     _fields_ = [("small_sort_max", C.c_int), ("hist_chunk", C.c_int), ("workers_per_cu", C.c_int),
-                ("pad", C.c_int)]
+                ("chains", C.c_int)]
 
 
 JOB_SORT, JOB_FILTER, JOB_AGG = 0, 1, 2

@@ -2,9 +2,9 @@
 
 A JobSet holds the inputs and outputs of many independent sort / filter /
 aggregate jobs in a few large device buffers (one slice per job) and the
-ysql_job table that points into them.  It runs the same jobs either as one
-captured persistent launch (run_batched) or as the per-job launch loop
-(run_plain).
+ysql_job table that points into them.  It runs the same jobs either as the
+captured graph of parallel launch chains (run_batched) or as the per-job
+launch loop on one stream (run_plain).
 """
 import ctypes as C
 
@@ -78,7 +78,13 @@ class JobSet:
         self.handle = None
         self.total_rows = sum(self.sizes)
 
-    def create(self, tuning=None):
+    def create(self, chains=None):
+        tuning = None
+        if chains is not None:
+            tuning = L.BatchTuning()
+            L.lib.ysql_batch_tuning_default(C.byref(tuning))
+            tuning.chains = int(chains)
+            tuning = C.byref(tuning)
         h = C.c_void_p()
         L.call("ysql_batch_create", self.jobs, len(self.sizes), tuning, C.byref(h))
         self.handle = h
@@ -87,7 +93,7 @@ class JobSet:
     def info(self):
         t, gsz, ws = C.c_uint64(), C.c_uint32(), C.c_uint64()
         L.call("ysql_batch_info", self.handle, C.byref(t), C.byref(gsz), C.byref(ws))
-        return {"tasks": t.value, "grid": gsz.value, "ws_bytes": ws.value}
+        return {"jobs": t.value, "chains": gsz.value, "ws_bytes": ws.value}
 
     def run_batched(self):
         L.call("ysql_batch_run", self.handle, C.c_void_p(torch.cuda.current_stream().cuda_stream))
@@ -139,9 +145,9 @@ class JobSet:
         return total
 
     def close(self):
-        if self.handle:
+        if self.handle and L.lib is not None:
             L.lib.ysql_batch_destroy(self.handle)
-            self.handle = None
+        self.handle = None
 
     def __del__(self):
         self.close()

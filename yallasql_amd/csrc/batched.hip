@@ -1,199 +1,74 @@
-// batched.hip — thousands of ragged sort / filter / aggregate jobs in one
-// persistent-kernel launch driven by a device work queue.
+// batched.hip — thousands of ragged sort / filter / aggregate jobs replayed
+// as one captured hipGraph.
 //
-// The host turns the job list into a queue of tile tasks once (at batch
-// creation): a sort job becomes histogram tasks followed, phase by phase,
-// by one task per (digit pass, tile of 6144 keys); filter jobs are
-// 8192-row compaction tiles; aggregate jobs are
-// 65536-row canonical tiles whose last finisher folds the partials.
-// Tasks are queued phase-major (every job's phase-0 work, then every job's
-// first digit pass, ...), so a task only ever waits — on a per-job phase
-// counter or a decoupled look-back status — for tasks earlier in the queue.
-// Workgroups take tasks in queue order from one atomic cursor; a waiting
-// task's dependencies were taken earlier by running workgroups, which never
-// wait on later tasks, so every wait ends.
+// ysql_batch_create assigns the jobs to K independent chains (largest job
+// first onto the least-loaded chain; a job's cost is the HBM bytes it moves
+// plus a fixed per-launch cost), gives every chain its own workspace and
+// stream, and captures each chain's per-call launches between one fork event
+// and one join event into a single hipGraph.  ysql_batch_run replays it: one
+// host call for the whole job set, no per-launch host cost, and the chains'
+// kernels overlap on the device, so small jobs no longer run one at a time
+// on an otherwise idle GPU.
 //
-// The run is a reset kernel plus the persistent kernel, captured once into
-// a hipGraph and replayed by ysql_batch_run.  The tile functions are the
-// ones the per-call entry points use (radix_dev.hpp, scan_dev.hpp,
-// agg_dev.hpp), so ysql_batch_run_plain — the per-job launch loop — returns
-// the same bits.
+// ysql_batch_run_plain issues the same per-call entry points one job after
+// another on one stream (the operator-at-a-time pattern of the reference's
+// engine).  Both forms run the same kernels in the same order for every job,
+// so their outputs are bit-identical.
 #include <algorithm>
+#include <numeric>
 #include <vector>
 
-#include "agg_dev.hpp"
-#include "radix_dev.hpp"
-#include "scan_dev.hpp"
+#include "common.hpp"
 
 namespace ysql {
 
-constexpr int kBB = 256;
-constexpr int kSortKpt = 16;
-constexpr int kSortTile = kBB * kSortKpt;  // 4096 keys
-constexpr int kFiltTile = kBB * 16;        // 4096 rows
+constexpr int kMaxChains = 64;
 
-enum { T_HIST = 1, T_PASS = 2, T_FILTER = 3, T_AGG = 4 };
-
-struct JobDev {
-  int kind, key64;
-  uint64_t n;
-  const void *keys;
-  const float *vals;
-  float thr;
-  uint32_t ntiles;
-  void *keys_out;
-  uint32_t *perm_out;
-  // sort workspace
-  void *kalt;
-  uint32_t *valt;
-  uint32_t *hist;  // 8 * 256 (zeroed per run)
-  uint32_t *st0;   // ntiles * 256 (zeroed per run)
-  uint32_t *st1;   // ntiles * 256 (zeroed by pass 0)
-  uint32_t *done;  // 16 phase counters (zeroed per run)
-  int passes;
-  uint32_t nhist, hist_chunk, pad;
-  // filter
-  uint64_t *count_out;
-  uint64_t *fstatus;  // ntiles + 1 (zeroed per run)
-  ysql_col fcols[2];
-  int nfcols, pad2;
-  // aggregate
-  ysql_acc *partials;  // ntiles * 3
-  uint32_t *ticket;    // zeroed per run
-  FaSpec specs[3];
-  ysql_acc *aouts[3];
-};
-
-struct Task {
-  uint32_t job, tile;
-  uint16_t type, phase;
-  uint32_t pad;
-};
-
-union BatchSmem {
-  OnesweepSmem<uint64_t, kBB, kSortKpt> s64;
-  OnesweepSmem<uint32_t, kBB, kSortKpt> s32;
-  CompactSmem<kBB> cs;
-  ysql_acc fa[kBB / kWave];
-  uint32_t h[8 * kBins];
-};
-
-__device__ __forceinline__ void signal_done(uint32_t *ctr) {
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) atomicAdd(ctr, 1u);
-}
-
-__device__ __forceinline__ void wait_done(uint32_t *ctr, uint32_t target) {
-  if (threadIdx.x == 0) {
-    while (__hip_atomic_load(ctr, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) __builtin_amdgcn_s_sleep(2);
-  }
-  __syncthreads();
-  __threadfence();
-}
-
-template <typename K>
-__device__ __forceinline__ OnesweepSmem<K, kBB, kSortKpt> &sort_smem(BatchSmem &sm);
-template <>
-__device__ __forceinline__ OnesweepSmem<uint64_t, kBB, kSortKpt> &sort_smem<uint64_t>(BatchSmem &sm) {
-  return sm.s64;
-}
-template <>
-__device__ __forceinline__ OnesweepSmem<uint32_t, kBB, kSortKpt> &sort_smem<uint32_t>(BatchSmem &sm) {
-  return sm.s32;
-}
-
-// Digit pass p over tile `tile` of sort job j; buffers alternate so that the
-// last pass lands in keys_out / perm_out.
-template <typename K>
-__device__ __forceinline__ void sort_pass(BatchSmem &sm, const JobDev &j, int p, uint32_t tile) {
-  const int P = j.passes;
-  const bool to_out = ((P - 1 - p) & 1) == 0;
-  const K *kin;
-  const uint32_t *vin;
-  if (p == 0) {
-    kin = (const K *)j.keys;
-    vin = nullptr;
-  } else {
-    const bool prev_out = ((P - p) & 1) == 0;
-    kin = prev_out ? (const K *)j.keys_out : (const K *)j.kalt;
-    vin = prev_out ? j.perm_out : j.valt;
-  }
-  K *kout = to_out ? (K *)j.keys_out : (K *)j.kalt;
-  uint32_t *vout = to_out ? j.perm_out : j.valt;
-  uint32_t *st = (p & 1) ? j.st1 : j.st0;
-  uint32_t *sn = (p + 1 < P) ? ((p & 1) ? j.st0 : j.st1) : nullptr;
-  onesweep_tile<K, kBB, kSortKpt, false>(sort_smem<K>(sm), tile, kin, vin, 0u, kout, vout, (uint32_t)j.n, 8 * p, 0u,
-                                         j.hist + p * kBins, st, sn);
-}
-
-__global__ __launch_bounds__(kBB) void k_batched(const JobDev *__restrict__ jobs, const Task *__restrict__ tasks,
-                                                 uint32_t ntasks, uint32_t *cursor) {
-  __shared__ BatchSmem sm;
-  __shared__ uint32_t s_task;
-  __shared__ int s_last;
-  while (true) {
-    if (threadIdx.x == 0) s_task = atomicAdd(cursor, 1u);
-    __syncthreads();
-    const uint32_t ti = s_task;
-    __syncthreads();
-    if (ti >= ntasks) return;
-    const Task task = tasks[ti];
-    const JobDev &j = jobs[task.job];
-    switch (task.type) {
-      case T_HIST: {
-        const uint64_t b = (uint64_t)task.tile * j.hist_chunk;
-        const uint64_t e = b + j.hist_chunk < j.n ? b + j.hist_chunk : j.n;
-        if (j.key64) hist_block<uint64_t>((const uint64_t *)j.keys, b, e, 0, j.passes, 0u, sm.h, j.hist);
-        else hist_block<uint32_t>((const uint32_t *)j.keys, b, e, 0, j.passes, 0u, sm.h, j.hist);
-        signal_done(&j.done[0]);
-        break;
-      }
-      case T_PASS: {
-        const int p = task.phase;
-        wait_done(&j.done[p], p == 0 ? j.nhist : j.ntiles);
-        if (j.key64) sort_pass<uint64_t>(sm, j, p, task.tile);
-        else sort_pass<uint32_t>(sm, j, p, task.tile);
-        signal_done(&j.done[p + 1]);
-        break;
-      }
-      case T_FILTER:
-        compact_tile<kBB, 1, float, YSQL_LT>(sm.cs, task.tile, nullptr, j.vals, j.thr, j.fcols, j.nfcols, j.n, j.count_out,
-                                             j.fstatus, j.ntiles);
-        break;
-      case T_AGG: {
-        filter_agg_tile<float, YSQL_LT, kFaBlock>(task.tile, j.vals, j.thr, j.specs, 3, j.n, j.partials, sm.fa);
-        __threadfence();
-        __syncthreads();
-        if (threadIdx.x == 0) s_last = (atomicAdd(j.ticket, 1u) == j.ntiles - 1);
-        __syncthreads();
-        if (s_last) {
-          __threadfence();
-          filter_agg_finish<kFaBlock>(j.specs, 3, j.ntiles, j.partials, j.aouts, sm.fa);
-        }
-        __syncthreads();
-        break;
-      }
+// Bytes a job moves through HBM, plus a fixed cost per launch (in bytes at
+// HBM rate), for the longest-first chain assignment.
+static double job_cost(const ysql_job &x) {
+  const double kb = x.key_dtype == YSQL_UKEY64 ? 8.0 : 4.0;
+  const double n = (double)x.n;
+  constexpr double kLaunch = 40e3;  // ~5 us of launch gap at ~8 TB/s
+  switch (x.kind) {
+    case YSQL_JOB_SORT: {
+      const double passes = kb;  // 8-bit digits
+      return n * kb + passes * 2.0 * n * (kb + 4.0) + (passes + 2.0) * kLaunch;
     }
+    case YSQL_JOB_FILTER: return n * (kb + 4.0) * 1.5 + 2.0 * kLaunch;
+    default: return n * (kb + 4.0) + 5.0 * kLaunch;
   }
 }
 
-__global__ void k_batch_reset(uint32_t *zero, uint64_t words, const JobDev *jobs, int njobs) {
-  const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t i = tid; i < words; i += stride) zero[i] = 0;
-  for (uint64_t q = tid; q < (uint64_t)njobs; q += stride) {
-    const JobDev &j = jobs[q];
-    if (j.kind == YSQL_JOB_FILTER) {
-      *j.count_out = 0;
-    } else if (j.kind == YSQL_JOB_AGG) {
-      for (int s = 0; s < 3; s++) {
-        ysql_acc a;
-        fa_identity(a, j.specs[s]);
-        *j.aouts[s] = a;
-      }
-    }
+static size_t job_ws_bytes(const ysql_job &x) {
+  size_t w = 0;
+  if (x.kind == YSQL_JOB_SORT) ysql_radix_sort_ws_bytes(x.key_dtype, x.n, &w);
+  else if (x.kind == YSQL_JOB_FILTER) ysql_compact_ws_bytes(x.n, &w);
+  else ysql_filter_aggregate_ws_bytes(x.n, 3, &w);
+  return std::max<size_t>(w, 256);
+}
+
+// The per-call launches of one job on `stream` (workspace `ws`).
+static int issue_job(const ysql_job &x, const float *thr_dev, void *ws, size_t ws_bytes, ysql_stream_t stream) {
+  const int kdt = x.key_dtype == YSQL_UKEY64 ? YSQL_DATETIME : YSQL_INT;
+  if (x.kind == YSQL_JOB_SORT) {
+    if (x.n == 0) return YSQL_OK;
+    return ysql_radix_sort_ex(x.key_dtype, 0, x.keys, nullptr, 0, x.keys_out, x.perm_out, x.n, 0,
+                              x.key_dtype == YSQL_UKEY64 ? 64 : 32, ws, ws_bytes, stream);
   }
+  if (x.kind == YSQL_JOB_FILTER) {
+    const uint32_t kb = x.key_dtype == YSQL_UKEY64 ? 8 : 4;
+    ysql_col cols[2] = {{x.keys, nullptr, x.keys_out, nullptr, kb, 0}, {x.vals, nullptr, x.vals_out, nullptr, 4, 0}};
+    return ysql_filter_compact(YSQL_FLOAT, YSQL_LT, x.vals, thr_dev, cols, 2, x.n, x.count_out, ws, ws_bytes, stream);
+  }
+  int rc = ysql_acc_init(YSQL_FLOAT, YSQL_AGG_SUM, x.accs_out, 1, stream);
+  if (!rc) rc = ysql_acc_init(kdt, YSQL_AGG_MIN, x.accs_out + 1, 1, stream);
+  if (!rc) rc = ysql_acc_init(kdt, YSQL_AGG_MAX, x.accs_out + 2, 1, stream);
+  ysql_agg_spec specs[3] = {{YSQL_FLOAT, YSQL_AGG_SUM, x.vals, x.accs_out},
+                            {kdt, YSQL_AGG_MIN, x.keys, x.accs_out + 1},
+                            {kdt, YSQL_AGG_MAX, x.keys, x.accs_out + 2}};
+  if (!rc && x.n) rc = ysql_filter_aggregate(YSQL_FLOAT, YSQL_LT, x.vals, thr_dev, specs, 3, x.n, ws, ws_bytes, stream);
+  return rc;
 }
 
 }  // namespace ysql
@@ -203,18 +78,16 @@ using namespace ysql;
 struct ysql_batch {
   std::vector<ysql_job> jobs;
   ysql_batch_tuning tune;
-  JobDev *d_jobs = nullptr;
-  Task *d_tasks = nullptr;
-  uint64_t ntasks = 0;
-  uint32_t *zero = nullptr;  // zeroed every run; the queue cursor is word 0
-  uint64_t zero_words = 0;
-  void *scratch = nullptr;
-  uint64_t scratch_bytes = 0;
-  uint32_t grid = 0;
-  float *d_thr = nullptr;  // thresholds as device scalars (plain loop)
+  float *d_thr = nullptr;  // thresholds as device scalars
   void *plain_ws = nullptr;
   size_t plain_ws_bytes = 0;
-  hipStream_t cap_stream = nullptr;
+  int nchains = 0;
+  std::vector<std::vector<int>> chain_jobs;
+  std::vector<size_t> chain_ws_off, chain_ws_len;
+  void *chain_ws = nullptr;
+  size_t chain_ws_bytes = 0;
+  std::vector<hipStream_t> streams;  // streams[0] is the capture stream
+  std::vector<hipEvent_t> events;    // fork, then one join per chain
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
@@ -223,31 +96,29 @@ extern "C" {
 
 void ysql_batch_tuning_default(ysql_batch_tuning *t) {
   if (!t) return;
-  t->small_sort_max = kSortTile;
+  t->small_sort_max = 0;
   t->hist_chunk = 1 << 16;
   t->workers_per_cu = 2;
-  t->pad = 0;
+  t->chains = 16;
 }
 
 void ysql_batch_destroy(ysql_batch *b) {
   if (!b) return;
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
-  if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
-  (void)hipFree(b->d_jobs);
-  (void)hipFree(b->d_tasks);
-  (void)hipFree(b->zero);
-  (void)hipFree(b->scratch);
+  for (auto e : b->events) (void)hipEventDestroy(e);
+  for (auto s : b->streams) (void)hipStreamDestroy(s);
   (void)hipFree(b->d_thr);
   (void)hipFree(b->plain_ws);
+  (void)hipFree(b->chain_ws);
   delete b;
 }
 
 int ysql_batch_info(const ysql_batch *b, uint64_t *tasks, uint32_t *grid, uint64_t *ws_bytes) {
   if (!b) return YSQL_EINVAL;
-  if (tasks) *tasks = b->ntasks;
-  if (grid) *grid = b->grid;
-  if (ws_bytes) *ws_bytes = b->zero_words * 4 + b->scratch_bytes + b->plain_ws_bytes;
+  if (tasks) *tasks = b->jobs.size();
+  if (grid) *grid = (uint32_t)b->nchains;
+  if (ws_bytes) *ws_bytes = b->chain_ws_bytes + b->plain_ws_bytes;
   return YSQL_OK;
 }
 
@@ -257,177 +128,100 @@ int ysql_batch_create(const ysql_job *jobs, int njobs, const ysql_batch_tuning *
   ysql_batch_tuning tune;
   ysql_batch_tuning_default(&tune);
   if (tuning) tune = *tuning;
-  if (tune.small_sort_max > kSortTile) tune.small_sort_max = kSortTile;
-  if (tune.hist_chunk < 4096) tune.hist_chunk = 4096;
-  if (tune.workers_per_cu < 1) tune.workers_per_cu = 1;
+  tune.chains = std::max(1, std::min(tune.chains, kMaxChains));
   for (int q = 0; q < njobs; q++) {
     const ysql_job &x = jobs[q];
     if (x.kind < YSQL_JOB_SORT || x.kind > YSQL_JOB_AGG) return YSQL_EINVAL;
     if (x.key_dtype != YSQL_UKEY32 && x.key_dtype != YSQL_UKEY64) return YSQL_EINVAL;
-    if (x.n >= (1ull << 30)) return YSQL_EINVAL;
+    if (x.n >= (1ull << 32)) return YSQL_EINVAL;
+    if (x.kind == YSQL_JOB_FILTER && !x.count_out) return YSQL_EINVAL;
+    if (x.kind == YSQL_JOB_AGG && !x.accs_out) return YSQL_EINVAL;
     if (x.n == 0) continue;
     if (!x.keys) return YSQL_EINVAL;
     if (x.kind == YSQL_JOB_SORT && (!x.keys_out || !x.perm_out)) return YSQL_EINVAL;
-    if (x.kind == YSQL_JOB_FILTER && (!x.vals || !x.keys_out || !x.vals_out || !x.count_out)) return YSQL_EINVAL;
-    if (x.kind == YSQL_JOB_AGG && (!x.vals || !x.accs_out)) return YSQL_EINVAL;
-    if (x.kind == YSQL_JOB_FILTER && !x.count_out) return YSQL_EINVAL;
+    if (x.kind == YSQL_JOB_FILTER && (!x.vals || !x.keys_out || !x.vals_out)) return YSQL_EINVAL;
+    if (x.kind == YSQL_JOB_AGG && !x.vals) return YSQL_EINVAL;
   }
   auto *b = new ysql_batch();
   b->jobs.assign(jobs, jobs + njobs);
   b->tune = tune;
-
-  // ---- workspace layout (offsets; pointers fixed up after allocation)
-  struct Lay {
-    uint64_t kalt, valt, partials;                // scratch offsets
-    uint64_t hist, st0, st1, done, fstatus, ticket;  // zero-region word offsets (st1 in scratch)
-  };
-  std::vector<Lay> lay(njobs);
-  uint64_t zw = 64;  // words 0..63: queue cursor
-  uint64_t sb = 0;
-  auto zt = [&](uint64_t words) {
-    uint64_t o = zw;
-    zw += (words + 63) / 64 * 64;
-    return o;
-  };
-  auto st = [&](uint64_t bytes) {
-    uint64_t o = sb;
-    sb += (bytes + 255) / 256 * 256;
-    return o;
-  };
-  std::vector<JobDev> jd(njobs);
-  for (int q = 0; q < njobs; q++) {
-    const ysql_job &x = b->jobs[q];
-    JobDev &j = jd[q];
-    memset(&j, 0, sizeof(j));
-    j.kind = x.kind;
-    j.key64 = x.key_dtype == YSQL_UKEY64;
-    j.n = x.n;
-    j.keys = x.keys;
-    j.vals = x.vals;
-    j.thr = x.threshold;
-    j.keys_out = x.keys_out;
-    j.perm_out = x.perm_out;
-    j.count_out = x.count_out;
-    const uint64_t kb = j.key64 ? 8 : 4;
-    if (x.kind == YSQL_JOB_SORT) {
-      j.passes = j.key64 ? 8 : 4;
-      j.ntiles = (uint32_t)std::max<uint64_t>(1, ceil_div(x.n, kSortTile));
-      j.hist_chunk = (uint32_t)tune.hist_chunk;
-      j.nhist = (uint32_t)std::max<uint64_t>(1, ceil_div(x.n, tune.hist_chunk));
-      lay[q].kalt = st(x.n * kb);
-      lay[q].valt = st(x.n * 4);
-      lay[q].st1 = st((uint64_t)j.ntiles * kBins * 4);
-      lay[q].hist = zt(8 * kBins);
-      lay[q].st0 = zt((uint64_t)j.ntiles * kBins);
-      lay[q].done = zt(16);
-    } else if (x.kind == YSQL_JOB_FILTER) {
-      j.ntiles = (uint32_t)ceil_div(x.n, kFiltTile);
-      lay[q].fstatus = zt(2 * ((uint64_t)j.ntiles + 1));
-      j.fcols[0] = ysql_col{x.keys, nullptr, x.keys_out, nullptr, (uint32_t)kb, 0};
-      j.fcols[1] = ysql_col{x.vals, nullptr, x.vals_out, nullptr, 4, 0};
-      j.nfcols = 2;
-    } else {
-      j.ntiles = (uint32_t)ceil_div(x.n, kFaTile);
-      lay[q].partials = st((uint64_t)std::max<uint32_t>(1, j.ntiles) * 3 * sizeof(ysql_acc));
-      lay[q].ticket = zt(1);
-      const int kdt = j.key64 ? YSQL_DATETIME : YSQL_INT;
-      j.specs[0] = FaSpec{FA_KD, 0, YSQL_FLOAT, 0, x.vals};
-      j.specs[1] = FaSpec{FA_KI, 1, kdt, 0, x.keys};
-      j.specs[2] = FaSpec{FA_KI, 2, kdt, 0, x.keys};
-      for (int s = 0; s < 3; s++) j.aouts[s] = x.accs_out + s;
-    }
-  }
-  b->zero_words = zw;
-  b->scratch_bytes = sb;
-
-  // ---- task queue, phase-major
-  std::vector<Task> tasks;
-  for (int q = 0; q < njobs; q++) {
-    const JobDev &j = jd[q];
-    if (j.n == 0) continue;
-    if (j.kind == YSQL_JOB_SORT) {
-      for (uint32_t c = 0; c < j.nhist; c++) tasks.push_back(Task{(uint32_t)q, c, T_HIST, 0, 0});
-    } else if (j.kind == YSQL_JOB_FILTER) {
-      for (uint32_t t = 0; t < j.ntiles; t++) tasks.push_back(Task{(uint32_t)q, t, T_FILTER, 0, 0});
-    } else {
-      for (uint32_t t = 0; t < j.ntiles; t++) tasks.push_back(Task{(uint32_t)q, t, T_AGG, 0, 0});
-    }
-  }
-  for (int p = 0; p < 8; p++)
-    for (int q = 0; q < njobs; q++) {
-      const JobDev &j = jd[q];
-      if (j.kind != YSQL_JOB_SORT || j.n == 0 || p >= j.passes) continue;
-      for (uint32_t t = 0; t < j.ntiles; t++) tasks.push_back(Task{(uint32_t)q, t, T_PASS, (uint16_t)p, 0});
-    }
-  b->ntasks = tasks.size();
-
   auto fail = [&](hipError_t e) {
     set_last_error(hipGetErrorString(e));
     ysql_batch_destroy(b);
     return YSQL_EHIP;
   };
   hipError_t e;
-  if ((e = hipMalloc(&b->zero, b->zero_words * 4)) != hipSuccess) return fail(e);
-  if ((e = hipMalloc(&b->scratch, std::max<uint64_t>(b->scratch_bytes, 256))) != hipSuccess) return fail(e);
-  for (int q = 0; q < njobs; q++) {
-    JobDev &j = jd[q];
-    char *S = (char *)b->scratch;
-    if (j.kind == YSQL_JOB_SORT && j.n) {
-      j.kalt = S + lay[q].kalt;
-      j.valt = (uint32_t *)(S + lay[q].valt);
-      j.st1 = (uint32_t *)(S + lay[q].st1);
-      j.hist = b->zero + lay[q].hist;
-      j.st0 = b->zero + lay[q].st0;
-      j.done = b->zero + lay[q].done;
-    } else if (j.kind == YSQL_JOB_FILTER && j.n) {
-      j.fstatus = (uint64_t *)(b->zero + lay[q].fstatus);
-    } else if (j.kind == YSQL_JOB_AGG && j.n) {
-      j.partials = (ysql_acc *)(S + lay[q].partials);
-      j.ticket = b->zero + lay[q].ticket;
-    }
-  }
-  if ((e = hipMalloc(&b->d_jobs, std::max<size_t>(1, jd.size()) * sizeof(JobDev))) != hipSuccess) return fail(e);
-  if ((e = hipMalloc(&b->d_tasks, std::max<size_t>(1, tasks.size()) * sizeof(Task))) != hipSuccess) return fail(e);
-  if (njobs && (e = hipMemcpy(b->d_jobs, jd.data(), jd.size() * sizeof(JobDev), hipMemcpyHostToDevice)) != hipSuccess)
-    return fail(e);
-  if (!tasks.empty() &&
-      (e = hipMemcpy(b->d_tasks, tasks.data(), tasks.size() * sizeof(Task), hipMemcpyHostToDevice)) != hipSuccess)
-    return fail(e);
 
-  // ---- plain-loop support: device thresholds and the largest workspace
+  // ---- device thresholds and the plain loop's (largest) workspace
   std::vector<float> thr(std::max(1, njobs));
+  std::vector<size_t> wsb(njobs);
   size_t pws = 256;
   for (int q = 0; q < njobs; q++) {
-    const ysql_job &x = b->jobs[q];
-    thr[q] = x.threshold;
-    size_t w = 0;
-    if (x.kind == YSQL_JOB_SORT) ysql_radix_sort_ws_bytes(x.key_dtype, x.n, &w);
-    else if (x.kind == YSQL_JOB_FILTER) ysql_compact_ws_bytes(x.n, &w);
-    else ysql_filter_aggregate_ws_bytes(x.n, 3, &w);
-    pws = std::max(pws, w);
+    thr[q] = b->jobs[q].threshold;
+    wsb[q] = job_ws_bytes(b->jobs[q]);
+    pws = std::max(pws, wsb[q]);
   }
   b->plain_ws_bytes = pws;
   if ((e = hipMalloc(&b->plain_ws, pws)) != hipSuccess) return fail(e);
   if ((e = hipMalloc(&b->d_thr, thr.size() * 4)) != hipSuccess) return fail(e);
   if ((e = hipMemcpy(b->d_thr, thr.data(), thr.size() * 4, hipMemcpyHostToDevice)) != hipSuccess) return fail(e);
 
-  // ---- persistent grid: workers that can be resident at once
-  int per_cu = 0;
-  if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batched, kBB, 0)) != hipSuccess) return fail(e);
-  per_cu = std::max(1, std::min(per_cu, tune.workers_per_cu));
-  b->grid = (uint32_t)(per_cu * grid_cap());
-
-  // ---- capture reset + persistent kernel into a graph
-  if ((e = hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking)) != hipSuccess) return fail(e);
-  if ((e = hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal)) != hipSuccess) return fail(e);
-  {
-    uint64_t g = ceil_div(std::max<uint64_t>(b->zero_words, (uint64_t)njobs), 256 * 4);
-    g = std::min<uint64_t>(std::max<uint64_t>(g, 1), 4096);
-    k_batch_reset<<<(unsigned)g, 256, 0, b->cap_stream>>>(b->zero, b->zero_words, b->d_jobs, njobs);
-    if (b->ntasks)
-      k_batched<<<b->grid, kBB, 0, b->cap_stream>>>(b->d_jobs, b->d_tasks, (uint32_t)b->ntasks, b->zero);
+  // ---- chains: largest job first onto the least-loaded chain
+  const int K = std::max(1, std::min(tune.chains, std::max(1, njobs)));
+  b->nchains = K;
+  b->chain_jobs.assign(K, {});
+  std::vector<int> order(njobs);
+  std::iota(order.begin(), order.end(), 0);
+  std::vector<double> cost(njobs);
+  for (int q = 0; q < njobs; q++) cost[q] = job_cost(b->jobs[q]);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return cost[a] > cost[c]; });
+  std::vector<double> load(K, 0.0);
+  for (int q : order) {
+    const int c = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    load[c] += cost[q];
+    b->chain_jobs[c].push_back(q);
   }
-  if ((e = hipStreamEndCapture(b->cap_stream, &b->graph)) != hipSuccess) return fail(e);
+  b->chain_ws_off.assign(K, 0);
+  b->chain_ws_len.assign(K, 256);
+  size_t off = 0;
+  for (int c = 0; c < K; c++) {
+    for (int q : b->chain_jobs[c]) b->chain_ws_len[c] = std::max(b->chain_ws_len[c], wsb[q]);
+    b->chain_ws_off[c] = off;
+    off += align_up(b->chain_ws_len[c], 256);
+  }
+  b->chain_ws_bytes = off;
+  if ((e = hipMalloc(&b->chain_ws, std::max<size_t>(off, 256))) != hipSuccess) return fail(e);
+
+  // ---- capture: fork -> K chains -> join
+  b->streams.assign(K + 1, nullptr);
+  for (auto &s : b->streams)
+    if ((e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking)) != hipSuccess) return fail(e);
+  b->events.assign(K + 1, nullptr);
+  for (auto &ev : b->events)
+    if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess) return fail(e);
+  hipStream_t cap = b->streams[0];
+  if ((e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal)) != hipSuccess) return fail(e);
+  int rc = YSQL_OK;
+  bool ok = hipEventRecord(b->events[0], cap) == hipSuccess;
+  for (int c = 0; c < K && ok && !rc; c++) {
+    hipStream_t s = b->streams[c + 1];
+    ok = hipStreamWaitEvent(s, b->events[0], 0) == hipSuccess;
+    char *ws = (char *)b->chain_ws + b->chain_ws_off[c];
+    for (size_t i = 0; ok && !rc && i < b->chain_jobs[c].size(); i++) {
+      const int q = b->chain_jobs[c][i];
+      rc = issue_job(b->jobs[q], b->d_thr + q, ws, b->chain_ws_len[c], (ysql_stream_t)s);
+    }
+    ok = ok && hipEventRecord(b->events[c + 1], s) == hipSuccess;
+    ok = ok && hipStreamWaitEvent(cap, b->events[c + 1], 0) == hipSuccess;
+  }
+  hipGraph_t g = nullptr;
+  e = hipStreamEndCapture(cap, &g);
+  b->graph = g;
+  if (rc) {
+    ysql_batch_destroy(b);
+    return rc;
+  }
+  if (!ok || e != hipSuccess) return fail(e != hipSuccess ? e : hipErrorUnknown);
   if ((e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0)) != hipSuccess) return fail(e);
   *out = b;
   return YSQL_OK;
@@ -435,44 +229,17 @@ int ysql_batch_create(const ysql_job *jobs, int njobs, const ysql_batch_tuning *
 
 int ysql_batch_run(ysql_batch *b, ysql_stream_t stream) {
   if (!b) return YSQL_EINVAL;
-  // The single persistent launch (cross-phase waits inside one kernel) did
-  // not complete on MI355X and its cause is not identified; it stays
-  // switched off until it is.
-  (void)stream;
-  set_last_error("ysql_batch_run: the persistent batched launch is disabled");
-  return YSQL_EUNSUPPORTED;
+  YSQL_HIP_TRY(hipGraphLaunch(b->exec, as_stream(stream)));
+  return YSQL_OK;
 }
 
 int ysql_batch_run_plain(ysql_batch *b, ysql_stream_t stream) {
   if (!b) return YSQL_EINVAL;
   for (size_t q = 0; q < b->jobs.size(); q++) {
-    const ysql_job &x = b->jobs[q];
-    int rc = YSQL_OK;
-    const int kdt = x.key_dtype == YSQL_UKEY64 ? YSQL_DATETIME : YSQL_INT;
-    if (x.kind == YSQL_JOB_SORT) {
-      rc = ysql_radix_sort_ex(x.key_dtype, 0, x.keys, nullptr, 0, x.keys_out, x.perm_out, x.n, 0,
-                              x.key_dtype == YSQL_UKEY64 ? 64 : 32, b->plain_ws, b->plain_ws_bytes, stream);
-    } else if (x.kind == YSQL_JOB_FILTER) {
-      ysql_col cols[2] = {{x.keys, nullptr, x.keys_out, nullptr, (uint32_t)(x.key_dtype == YSQL_UKEY64 ? 8 : 4), 0},
-                          {x.vals, nullptr, x.vals_out, nullptr, 4, 0}};
-      rc = ysql_filter_compact(YSQL_FLOAT, YSQL_LT, x.vals, b->d_thr + q, cols, 2, x.n, x.count_out, b->plain_ws,
-                               b->plain_ws_bytes, stream);
-    } else {
-      rc = ysql_acc_init(YSQL_FLOAT, YSQL_AGG_SUM, x.accs_out, 1, stream);
-      if (!rc) rc = ysql_acc_init(kdt, YSQL_AGG_MIN, x.accs_out + 1, 1, stream);
-      if (!rc) rc = ysql_acc_init(kdt, YSQL_AGG_MAX, x.accs_out + 2, 1, stream);
-      ysql_agg_spec specs[3] = {{YSQL_FLOAT, YSQL_AGG_SUM, x.vals, x.accs_out},
-                                {kdt, YSQL_AGG_MIN, x.keys, x.accs_out + 1},
-                                {kdt, YSQL_AGG_MAX, x.keys, x.accs_out + 2}};
-      if (!rc && x.n)
-        rc = ysql_filter_aggregate(YSQL_FLOAT, YSQL_LT, x.vals, b->d_thr + q, specs, 3, x.n, b->plain_ws,
-                                   b->plain_ws_bytes, stream);
-    }
+    const int rc = issue_job(b->jobs[q], b->d_thr + q, b->plain_ws, b->plain_ws_bytes, stream);
     if (rc) return rc;
   }
   return YSQL_OK;
 }
 
 }  // extern "C"
-
-static_assert(ysql::kBB == ysql::kFaBlock, "aggregate jobs fold with the per-call form's block size");
