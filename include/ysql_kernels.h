@@ -291,9 +291,9 @@ int ysql_agg_finalize(int dtype, int agg, const ysql_acc *acc, void *out, ysql_s
  *     for rows with pred_col[i] <cmp> scalar: fold specs[s].col[i] into
  *     *specs[s].acc under specs[s].agg   (s < nspecs <= 4).
  * COUNT and COUNT_STAR count the qualifying rows (this form takes no null
- * masks).  Deterministic like ysql_reduce: canonical 65536-row tiles folded
- * in a fixed order, so it returns the same bits as the batched executor's
- * aggregate jobs.                                                          */
+ * masks).  Deterministic: canonical 32768-row tiles folded in a fixed order,
+ * so the float SUM is a function of the input alone (the same bits from
+ * every launch shape and from the batched executor's aggregate jobs).      */
 typedef struct ysql_agg_spec {
   int dtype; /* YSQL_INT / YSQL_FLOAT / YSQL_DATETIME of col */
   int agg;   /* YSQL_AGG_* */
@@ -305,6 +305,13 @@ int ysql_filter_aggregate_ws_bytes(uint64_t n, int nspecs, size_t *bytes);
 int ysql_filter_aggregate(int pred_dtype, int cmp, const void *pred_col, const void *scalar,
                           const ysql_agg_spec *specs, int nspecs, uint64_t n, void *ws,
                           size_t ws_bytes, ysql_stream_t stream);
+/* Same, with flags: YSQL_FA_OVERWRITE stores this call's result into the
+ * accumulators instead of folding it in (no ysql_acc_init needed; an empty
+ * input stores the identities). */
+#define YSQL_FA_OVERWRITE 1
+int ysql_filter_aggregate_ex(int pred_dtype, int cmp, const void *pred_col, const void *scalar,
+                             const ysql_agg_spec *specs, int nspecs, uint64_t n, int flags,
+                             void *ws, size_t ws_bytes, ysql_stream_t stream);
 
 /* GROUP BY (no reference function: the reference's AggregateOperator never
  * reads `groups`, aggregate_operator.hpp:92-103).  acc_inout[g] folds the

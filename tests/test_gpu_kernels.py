@@ -393,6 +393,73 @@ def test_reduce_deterministic(ops, dev):
     assert len(outs) == 1
 
 
+FA_SIZES = [0, 1, 7, 8, 9, 32767, 32768, 32769, 1_000_003]
+
+
+@pytest.mark.parametrize("n", FA_SIZES)
+@pytest.mark.parametrize("pdt", [np.int32, np.float32, np.int64])
+def test_filter_aggregate(ops, dev, n, pdt):
+    """WHERE pred <cmp> c with SUM(float), MIN(int32), MAX(int64), COUNT,
+    SUM(int32) in one pass; against the oracle's compare + reduce."""
+    r = rng(n + 7)
+    if pdt == np.float32:
+        pred = r.random(n + 1).astype(np.float32)
+        c = np.float32(0.37)
+    else:
+        pred = r.integers(-1000, 1000, n + 1).astype(pdt)
+        c = pdt(-250)
+    f = r.random(n + 1).astype(np.float32)
+    i32 = r.integers(-10**9, 10**9, n + 1).astype(np.int32)
+    i64 = r.integers(-2**62, 2**62, n + 1).astype(np.int64)
+    full = [T(x, dev) for x in (pred, f, i32, i64)]
+    # device views at an odd offset (elements 1..n) take the unaligned path
+    for off in (0, 1):
+        P, F, I, J = (x[off:off + n] for x in (pred, f, i32, i64))
+        tp, tf, ti, tj = (x[off:off + n] for x in full)
+        for cmp in range(6):
+            m = O.compare(cmp, P, np.array([c], dtype=pdt), rhs_scalar=True, n=n).astype(bool)
+            specs = [(O.FLOAT, 2, tf), (O.INT, 1, ti), (O.DATETIME, 0, tj), (O.INT, 4, None), (O.INT, 2, ti)]
+            pdc = O._DT[np.dtype(pdt)]
+            # specs over the predicate column and two specs over one column
+            # (the kernel loads such a column once)
+            shared = [(pdc, 2, tp), (pdc, 1, tp), (O.DATETIME, 0, tj), (O.DATETIME, 1, tj)]
+            for group in (specs[:4], specs[4:], shared):
+                accs = [ops.acc_new(dc, agg, device=dev) for dc, agg, _ in group]
+                ops.filter_aggregate(tp, cmp, c.item(), [(dc, agg, col, a) for (dc, agg, col), a in zip(group, accs)],
+                                     overwrite=(off == 1))
+                src = {id(tf): F, id(ti): I, id(tj): J, id(tp): P}
+                for (dc, agg, col), a in zip(group, accs):
+                    got = ops.acc_read(a)[0]
+                    e = O.acc_new(dc, agg)
+                    x = src[id(col)][m] if col is not None else np.zeros(int(m.sum()), np.int32)
+                    O.reduce(dc, agg, x, e)
+                    e = e[0]
+                    assert got["count"] == e.count, (cmp, dc, agg)
+                    if dc == O.FLOAT and agg == 1:
+                        assert got["f"] == e.f, (cmp, agg)
+                    elif dc == O.FLOAT and agg == 2:
+                        assert abs(got["d"] - e.d) <= 1e-6 * max(1.0, abs(e.d)), (cmp, agg)
+                    elif agg != 4:
+                        assert got["i"] == e.i, (cmp, dc, agg)
+
+
+def test_filter_aggregate_fold_vs_overwrite(ops, dev):
+    r = rng(3)
+    x = r.random(100_000).astype(np.float32)
+    tx = T(x, dev)
+    acc = ops.acc_new(O.FLOAT, 2, device=dev)
+    ops.filter_aggregate(tx, 0, 0.5, [(O.FLOAT, 2, tx, acc)])
+    ops.filter_aggregate(tx, 0, 0.5, [(O.FLOAT, 2, tx, acc)])  # folds: twice the sum
+    twice = ops.acc_read(acc)[0]
+    ops.filter_aggregate(tx, 0, 0.5, [(O.FLOAT, 2, tx, acc)], overwrite=True)
+    once = ops.acc_read(acc)[0]
+    assert twice["count"] == 2 * once["count"] == 2 * int((x < 0.5).sum())
+    assert abs(twice["d"] - 2 * once["d"]) <= 1e-9 * abs(twice["d"])
+    # empty input with overwrite stores the identity
+    ops.filter_aggregate(tx[:0], 0, 0.5, [(O.FLOAT, 2, tx, acc)], overwrite=True)
+    assert ops.acc_read(acc)[0]["count"] == 0 and ops.acc_read(acc)[0]["d"] == 0.0
+
+
 @pytest.mark.parametrize("ng", [7, 4096, 7000])
 def test_group_reduce(ops, dev, ng):
     r = rng(ng)

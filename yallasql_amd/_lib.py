@@ -138,6 +138,13 @@ class Job(C.Structure):
     ]
 
 
+class AggSpec(C.Structure):
+    _fields_ = [("dtype", C.c_int), ("agg", C.c_int), ("col", C.c_void_p), ("acc", C.c_void_p)]
+
+
+FA_OVERWRITE = 1
+
+
 class BatchTuning(C.Structure):
     _fields_ = [("small_sort_max", C.c_int), ("hist_chunk", C.c_int), ("workers_per_cu", C.c_int),
                 ("chains", C.c_int)]
@@ -148,6 +155,7 @@ assert C.sizeof(Job) == 80
 for _name, _args, _res in [
     ("ysql_filter_aggregate_ws_bytes", [U64, I32, SZP], C.c_int),
     ("ysql_filter_aggregate", [I32, I32, P, P, P, I32, U64, P, C.c_size_t, P], C.c_int),
+    ("ysql_filter_aggregate_ex", [I32, I32, P, P, P, I32, U64, I32, P, C.c_size_t, P], C.c_int),
     ("ysql_group_ids", [P, P, P, I32, U64, P, P], C.c_int),
     ("ysql_sortkey_pack", [I32, I32, P, U64, P, P], C.c_int),
     ("ysql_batch_tuning_default", [C.POINTER(BatchTuning)], None),

@@ -3,30 +3,34 @@
 // executor.
 //
 // SELECT agg_1(c_1), ..., agg_k(c_k) WHERE pred <cmp> scalar in one read of
-// the columns.  The input is cut into canonical tiles of 65536 rows; a
-// tile's partial is folded in a fixed order (thread t folds rows t, t+256,
-// ... of the tile, then a fixed wave64 butterfly, then the 4 waves in order)
-// and the partials are combined in a fixed order, so float sums are a
-// function of the input alone — the plain per-call form and the batched form
-// return the same bits.
+// the columns.  The input is cut into canonical tiles of 32768 rows; a
+// tile's partial is folded in a fixed order (thread t folds its runs of
+// kFaRpt rows in row order, then a fixed wave64 butterfly, then the 4 waves
+// in order) and the partials are combined in a fixed order, so float sums are a
+// function of the input alone, whatever the grid and whichever caller
+// launches the tiles.
 #pragma once
 
 #include "common.hpp"
 
 namespace ysql {
 
-constexpr uint64_t kFaTile = 65536;
+constexpr uint64_t kFaTile = 32768;
 constexpr int kFaBlock = 256;
 constexpr int kFaMaxSpecs = 4;
+constexpr int kFaRpt = 8;  // rows per thread per step
 
 // Accumulator kinds: int64 in .i, float in .f (MIN/MAX), double in .d (SUM),
 // count only.
 enum { FA_KI = 0, FA_KF = 1, FA_KD = 2, FA_KC = 3 };
 
 struct FaSpec {
-  int kind, op, dtype, pad;  // op: 0 sum, 1 min, 2 max
+  int kind, op, dtype;  // op: 0 sum, 1 min, 2 max
+  int src;              // where the values come from: FA_OWN (load col), FA_PRED (the
+                        // predicate column), or FA_SPEC0 + t (the column of spec t < this one)
   const void *col;
 };
+enum { FA_OWN = 0, FA_PRED = 1, FA_SPEC0 = 2 };
 
 __host__ __device__ inline int fa_kind(int dtype, int agg) {
   if (agg == YSQL_AGG_COUNT || agg == YSQL_AGG_COUNT_STAR) return FA_KC;
@@ -62,33 +66,18 @@ __device__ __forceinline__ void fa_combine(ysql_acc &a, const ysql_acc &b, const
   }
 }
 
-__device__ __forceinline__ void fa_fold_row(ysql_acc &a, const FaSpec &s, uint64_t row) {
-  a.count += 1;
-  if (s.kind == FA_KI) {
-    const int64_t x = s.dtype == YSQL_INT ? (int64_t)((const int32_t *)s.col)[row] : ((const int64_t *)s.col)[row];
-    if (s.op == 0) a.i = (int64_t)((uint64_t)a.i + (uint64_t)x);
-    else if (s.op == 1) a.i = x < a.i ? x : a.i;
-    else a.i = x > a.i ? x : a.i;
-  } else if (s.kind == FA_KF) {
-    const float x = ((const float *)s.col)[row];
-    a.f = s.op == 1 ? fminf(a.f, x) : fmaxf(a.f, x);
-  } else if (s.kind == FA_KD) {
-    a.d += (double)((const float *)s.col)[row];
-  }
-}
-
 // Fixed-shape block reduction (BLOCK threads); result in every thread.
+// Only the count and the field of the spec's kind travel.
 template <int BLOCK>
 __device__ __forceinline__ ysql_acc fa_block_reduce(ysql_acc v, const FaSpec &s, ysql_acc *s_w) {
   constexpr int W = BLOCK / kWave;
 #pragma unroll
   for (int m = 32; m >= 1; m >>= 1) {
-    ysql_acc o;
+    ysql_acc o = v;
     o.count = shfl_xor_t(v.count, m);
-    o.i = shfl_xor_t(v.i, m);
-    o.d = shfl_xor_t(v.d, m);
-    o.f = shfl_xor_t(v.f, m);
-    o.pad = 0;
+    if (s.kind == FA_KI) o.i = shfl_xor_t(v.i, m);
+    else if (s.kind == FA_KD) o.d = shfl_xor_t(v.d, m);
+    else if (s.kind == FA_KF) o.f = shfl_xor_t(v.f, m);
     fa_combine(v, o, s);
   }
   if (lane_id() == 0) s_w[threadIdx.x / kWave] = v;
@@ -100,7 +89,119 @@ __device__ __forceinline__ ysql_acc fa_block_reduce(ysql_acc v, const FaSpec &s,
   return r;
 }
 
-// Partials of canonical tile `tile`: partials[tile * ns + s].
+// kFaRpt consecutive values from row r0 (rows >= end read as T()): 128-bit
+// loads when the run is whole and 16-byte aligned.
+template <typename T>
+__device__ __forceinline__ void fa_load_rows(const T *p, uint64_t r0, uint64_t end, T (&v)[kFaRpt]) {
+  if (r0 + kFaRpt <= end && ((uintptr_t)(p + r0) & 15) == 0) {
+    constexpr int NQ = kFaRpt * sizeof(T) / 16;
+    uint4 q[NQ];
+    const uint4 *src = reinterpret_cast<const uint4 *>(p + r0);
+#pragma unroll
+    for (int i = 0; i < NQ; i++) q[i] = src[i];
+    memcpy(&v[0], &q[0], sizeof(q));
+  } else {
+#pragma unroll
+    for (int e = 0; e < kFaRpt; e++) v[e] = (r0 + e < end) ? p[r0 + e] : T();
+  }
+}
+
+// Element width of a spec's column: 8 bytes for int64, else 4.
+__host__ __device__ inline bool fa_wide(const FaSpec &s) { return s.kind == FA_KI && s.dtype != YSQL_INT; }
+
+// The kFaRpt values of a spec's column at rows r0.. as 32-bit words (element e
+// in w[e], or in w[2e], w[2e+1] when 8 bytes wide); rows >= end read 0.
+__device__ __forceinline__ void fa_load_words(const FaSpec &s, uint64_t r0, uint64_t end, uint32_t (&w)[2 * kFaRpt]) {
+  const bool full = r0 + kFaRpt <= end;
+  if (fa_wide(s)) {
+    const uint64_t *p = (const uint64_t *)s.col + r0;
+    if (full && ((uintptr_t)p & 15) == 0) {
+      const uint4 *q = reinterpret_cast<const uint4 *>(p);
+#pragma unroll
+      for (int i = 0; i < kFaRpt / 2; i++) {
+        const uint4 x = q[i];
+        w[4 * i] = x.x;
+        w[4 * i + 1] = x.y;
+        w[4 * i + 2] = x.z;
+        w[4 * i + 3] = x.w;
+      }
+    } else {
+#pragma unroll
+      for (int e = 0; e < kFaRpt; e++) {
+        const uint64_t x = r0 + e < end ? p[e] : 0ull;
+        w[2 * e] = (uint32_t)x;
+        w[2 * e + 1] = (uint32_t)(x >> 32);
+      }
+    }
+  } else {
+    const uint32_t *p = (const uint32_t *)s.col + r0;
+    if (full && ((uintptr_t)p & 15) == 0) {
+      const uint4 *q = reinterpret_cast<const uint4 *>(p);
+#pragma unroll
+      for (int i = 0; i < kFaRpt / 4; i++) {
+        const uint4 x = q[i];
+        w[4 * i] = x.x;
+        w[4 * i + 1] = x.y;
+        w[4 * i + 2] = x.z;
+        w[4 * i + 3] = x.w;
+      }
+    } else {
+#pragma unroll
+      for (int e = 0; e < kFaRpt; e++) w[e] = r0 + e < end ? p[e] : 0u;
+    }
+  }
+}
+
+// Folds the selected rows (bit e of `flags` = row r0 + e) of one spec, in
+// row order, from its loaded words.  Unselected rows fold the operator's
+// identity (0 for SUM, the current value for MIN/MAX), which leaves the
+// accumulator bit-for-bit as if they were skipped.
+__device__ __forceinline__ void fa_fold_words(ysql_acc &a, const FaSpec &s, const uint32_t (&w)[2 * kFaRpt],
+                                              uint32_t flags) {
+  a.count += __builtin_popcount(flags);
+  if (s.kind == FA_KC) return;
+  if (s.kind == FA_KI) {
+    const bool wide = s.dtype != YSQL_INT;
+    if (s.op == 0) {
+      uint64_t sum = (uint64_t)a.i;
+#pragma unroll
+      for (int e = 0; e < kFaRpt; e++) {
+        const int64_t x = wide ? (int64_t)((uint64_t)w[2 * e] | ((uint64_t)w[2 * e + 1] << 32)) : (int64_t)(int32_t)w[e];
+        sum += ((flags >> e) & 1u) ? (uint64_t)x : 0ull;
+      }
+      a.i = (int64_t)sum;
+    } else {
+      int64_t m = a.i;
+#pragma unroll
+      for (int e = 0; e < kFaRpt; e++) {
+        const int64_t x = wide ? (int64_t)((uint64_t)w[2 * e] | ((uint64_t)w[2 * e + 1] << 32)) : (int64_t)(int32_t)w[e];
+        const bool take = ((flags >> e) & 1u) && (s.op == 1 ? x < m : x > m);
+        m = take ? x : m;
+      }
+      a.i = m;
+    }
+  } else if (s.kind == FA_KF) {
+    float m = a.f;
+#pragma unroll
+    for (int e = 0; e < kFaRpt; e++) {
+      const float x = __uint_as_float(w[e]);
+      const float r = s.op == 1 ? fminf(m, x) : fmaxf(m, x);
+      m = ((flags >> e) & 1u) ? r : m;
+    }
+    a.f = m;
+  } else {
+    double d = a.d;
+#pragma unroll
+    for (int e = 0; e < kFaRpt; e++) d += ((flags >> e) & 1u) ? (double)__uint_as_float(w[e]) : 0.0;
+    a.d = d;
+  }
+}
+
+// Partials of canonical tile `tile`: partials[tile * ns + s].  Thread t owns
+// rows base + t*kFaRpt .. of every step of BLOCK*kFaRpt rows and folds them
+// in row order.  Each step first issues every load (the predicate and each
+// spec column that is not the predicate column, 128-bit loads), then
+// evaluates the predicate and folds.
 template <typename PT, int CMP, int BLOCK>
 __device__ __forceinline__ void filter_agg_tile(uint32_t tile, const PT *pred, PT scalar, const FaSpec *specs, int ns,
                                                 uint64_t n, ysql_acc *partials, ysql_acc *s_w) {
@@ -115,11 +216,33 @@ __device__ __forceinline__ void filter_agg_tile(uint32_t tile, const PT *pred, P
       fa_identity(acc[s], sp[s]);
     }
   }
-  for (uint64_t row = base + threadIdx.x; row < end; row += BLOCK) {
-    if (!cmp_static<CMP>(pred[row], scalar)) continue;
+  for (uint64_t r0 = base + (uint64_t)threadIdx.x * kFaRpt; r0 < end; r0 += (uint64_t)BLOCK * kFaRpt) {
+    PT pv[kFaRpt];
+    fa_load_rows<PT>(pred, r0, end, pv);
+    uint32_t w[kFaMaxSpecs][2 * kFaRpt];
 #pragma unroll
     for (int s = 0; s < kFaMaxSpecs; s++)
-      if (s < ns) fa_fold_row(acc[s], sp[s], row);
+      if (s < ns && sp[s].kind != FA_KC && sp[s].src == FA_OWN) fa_load_words(sp[s], r0, end, w[s]);
+    uint32_t flags = 0;
+#pragma unroll
+    for (int e = 0; e < kFaRpt; e++) flags |= (uint32_t)(r0 + e < end && cmp_static<CMP>(pv[e], scalar)) << e;
+#pragma unroll
+    for (int s = 0; s < kFaMaxSpecs; s++) {
+      if (s < ns) {
+        const int src = sp[s].src;
+        if (src == FA_PRED) {
+          uint32_t pw[2 * kFaRpt];
+          memcpy(pw, pv, sizeof(pv));
+          fa_fold_words(acc[s], sp[s], pw, flags);
+        } else if (src == FA_OWN || sp[s].kind == FA_KC) {
+          fa_fold_words(acc[s], sp[s], w[s], flags);
+        } else {  // the words an earlier spec loaded (static indices only)
+#pragma unroll
+          for (int t = 0; t < s; t++)
+            if (src == FA_SPEC0 + t) fa_fold_words(acc[s], sp[s], w[t], flags);
+        }
+      }
+    }
   }
 #pragma unroll
   for (int s = 0; s < kFaMaxSpecs; s++) {
@@ -130,11 +253,13 @@ __device__ __forceinline__ void filter_agg_tile(uint32_t tile, const PT *pred, P
   }
 }
 
-// Combines the ntiles partials in a fixed order and folds the totals into
-// out[s] (each an accumulator that persists across calls).
+// Combines the ntiles partials in a fixed order; folds the totals into
+// out[s] (accumulators that persist across calls), or with `overwrite`
+// stores them there.
 template <int BLOCK>
 __device__ __forceinline__ void filter_agg_finish(const FaSpec *specs, int ns, uint32_t ntiles,
-                                                  const ysql_acc *partials, ysql_acc *const *out, ysql_acc *s_w) {
+                                                  const ysql_acc *partials, ysql_acc *const *out, bool overwrite,
+                                                  ysql_acc *s_w) {
   for (int s = 0; s < ns; s++) {
     const FaSpec sp = specs[s];
     ysql_acc p;
@@ -145,7 +270,9 @@ __device__ __forceinline__ void filter_agg_finish(const FaSpec *specs, int ns, u
     }
     const ysql_acc tot = fa_block_reduce<BLOCK>(p, sp, s_w);
     if (threadIdx.x == 0) {
-      ysql_acc a = *out[s];
+      ysql_acc a;
+      if (overwrite) fa_identity(a, sp);
+      else a = *out[s];
       fa_combine(a, tot, sp);
       *out[s] = a;
     }

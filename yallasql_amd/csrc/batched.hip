@@ -36,7 +36,7 @@ static double job_cost(const ysql_job &x) {
       return n * kb + passes * 2.0 * n * (kb + 4.0) + (passes + 2.0) * kLaunch;
     }
     case YSQL_JOB_FILTER: return n * (kb + 4.0) * 1.5 + 2.0 * kLaunch;
-    default: return n * (kb + 4.0) + 5.0 * kLaunch;
+    default: return n * (kb + 4.0) + 2.0 * kLaunch;
   }
 }
 
@@ -61,14 +61,11 @@ static int issue_job(const ysql_job &x, const float *thr_dev, void *ws, size_t w
     ysql_col cols[2] = {{x.keys, nullptr, x.keys_out, nullptr, kb, 0}, {x.vals, nullptr, x.vals_out, nullptr, 4, 0}};
     return ysql_filter_compact(YSQL_FLOAT, YSQL_LT, x.vals, thr_dev, cols, 2, x.n, x.count_out, ws, ws_bytes, stream);
   }
-  int rc = ysql_acc_init(YSQL_FLOAT, YSQL_AGG_SUM, x.accs_out, 1, stream);
-  if (!rc) rc = ysql_acc_init(kdt, YSQL_AGG_MIN, x.accs_out + 1, 1, stream);
-  if (!rc) rc = ysql_acc_init(kdt, YSQL_AGG_MAX, x.accs_out + 2, 1, stream);
   ysql_agg_spec specs[3] = {{YSQL_FLOAT, YSQL_AGG_SUM, x.vals, x.accs_out},
                             {kdt, YSQL_AGG_MIN, x.keys, x.accs_out + 1},
                             {kdt, YSQL_AGG_MAX, x.keys, x.accs_out + 2}};
-  if (!rc && x.n) rc = ysql_filter_aggregate(YSQL_FLOAT, YSQL_LT, x.vals, thr_dev, specs, 3, x.n, ws, ws_bytes, stream);
-  return rc;
+  return ysql_filter_aggregate_ex(YSQL_FLOAT, YSQL_LT, x.vals, thr_dev, specs, 3, x.n, YSQL_FA_OVERWRITE, ws, ws_bytes,
+                                  stream);
 }
 
 }  // namespace ysql
@@ -133,7 +130,7 @@ int ysql_batch_create(const ysql_job *jobs, int njobs, const ysql_batch_tuning *
     const ysql_job &x = jobs[q];
     if (x.kind < YSQL_JOB_SORT || x.kind > YSQL_JOB_AGG) return YSQL_EINVAL;
     if (x.key_dtype != YSQL_UKEY32 && x.key_dtype != YSQL_UKEY64) return YSQL_EINVAL;
-    if (x.n >= (1ull << 32)) return YSQL_EINVAL;
+    if (x.n >= (1ull << 30)) return YSQL_EINVAL;
     if (x.kind == YSQL_JOB_FILTER && !x.count_out) return YSQL_EINVAL;
     if (x.kind == YSQL_JOB_AGG && !x.accs_out) return YSQL_EINVAL;
     if (x.n == 0) continue;

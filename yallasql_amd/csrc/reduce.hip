@@ -13,6 +13,7 @@
 // total into the caller's accumulator.  The result is a function of the input
 // alone: the same float SUM comes out of every run and every launch shape.
 #include <float.h>
+#include <algorithm>
 #include <math.h>
 
 #include "common.hpp"
@@ -668,7 +669,8 @@ struct FaArgs {
 
 template <typename PT, int CMP>
 __global__ __launch_bounds__(kFaBlock) void k_filter_agg(const PT *pred, const PT *scalar, FaArgs a, int ns, uint64_t n,
-                                                         ysql_acc *partials, uint32_t *ticket, uint32_t ntiles) {
+                                                         ysql_acc *partials, uint32_t *ticket, uint32_t ntiles,
+                                                         int overwrite) {
   __shared__ ysql_acc s_w[kFaBlock / kWave];
   __shared__ int s_last;
   const PT sc = scalar[0];
@@ -679,21 +681,41 @@ __global__ __launch_bounds__(kFaBlock) void k_filter_agg(const PT *pred, const P
   __syncthreads();
   if (!s_last) return;
   __threadfence();
-  filter_agg_finish<kFaBlock>(a.spec, ns, ntiles, partials, a.out, s_w);
+  filter_agg_finish<kFaBlock>(a.spec, ns, ntiles, partials, a.out, overwrite != 0, s_w);
   if (threadIdx.x == 0) *ticket = 0;
+}
+
+// Identities into the outputs (an empty input with overwrite).
+__global__ void k_fa_identity(FaArgs a, int ns) {
+  const int s = threadIdx.x;
+  if (s < ns) {
+    ysql_acc v;
+    fa_identity(v, a.spec[s]);
+    *a.out[s] = v;
+  }
+}
+
+template <typename K>
+static uint32_t fa_blocks_per_cu(K kernel) {
+  int b = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, kernel, kFaBlock, 0) != hipSuccess || b < 1) b = 2;
+  return (uint32_t)b;
 }
 
 template <typename PT>
 static int filter_agg_launch(int cmp, const void *pred, const void *scalar, const FaArgs &a, int ns, uint64_t n,
-                             ysql_acc *partials, uint32_t *ticket, hipStream_t s) {
+                             ysql_acc *partials, uint32_t *ticket, int overwrite, hipStream_t s) {
   const uint32_t ntiles = (uint32_t)ceil_div(n, kFaTile);
-  uint32_t g = ntiles;
-  const uint32_t cap = (uint32_t)grid_cap() * 2;
-  if (g > cap) g = cap;
   const PT *p = (const PT *)pred;
   const PT *sc = (const PT *)scalar;
-#define FA_CASE(C) \
-  case C: k_filter_agg<PT, C><<<g, kFaBlock, 0, s>>>(p, sc, a, ns, n, partials, ticket, ntiles); break;
+  // grid: the workgroups that fit at once (tiles are taken grid-stride)
+#define FA_CASE(C)                                                                          \
+  case C: {                                                                                 \
+    static const uint32_t per_cu = fa_blocks_per_cu(k_filter_agg<PT, C>);                  \
+    const uint32_t g = std::min<uint32_t>(ntiles, per_cu * (uint32_t)grid_cap());           \
+    k_filter_agg<PT, C><<<g, kFaBlock, 0, s>>>(p, sc, a, ns, n, partials, ticket, ntiles, overwrite); \
+    break;                                                                                  \
+  }
   switch (cmp) {
     FA_CASE(YSQL_LT)
     FA_CASE(YSQL_LE)
@@ -725,39 +747,62 @@ int ysql_filter_aggregate_ws_bytes(uint64_t n, int nspecs, size_t *bytes) {
   return YSQL_OK;
 }
 
-int ysql_filter_aggregate(int pred_dtype, int cmp, const void *pred_col, const void *scalar, const ysql_agg_spec *specs,
-                          int nspecs, uint64_t n, void *ws, size_t ws_bytes, ysql_stream_t stream) {
+int ysql_filter_aggregate_ex(int pred_dtype, int cmp, const void *pred_col, const void *scalar,
+                             const ysql_agg_spec *specs, int nspecs, uint64_t n, int flags, void *ws, size_t ws_bytes,
+                             ysql_stream_t stream) {
   if (nspecs < 1 || nspecs > kFaMaxSpecs || !specs || cmp < YSQL_LT || cmp > YSQL_NE) return YSQL_EINVAL;
+  if (flags & ~YSQL_FA_OVERWRITE) return YSQL_EINVAL;
+  const int overwrite = (flags & YSQL_FA_OVERWRITE) ? 1 : 0;
   FaArgs a;
   memset(&a, 0, sizeof(a));
   for (int s = 0; s < nspecs; s++) {
     const ysql_agg_spec &x = specs[s];
     if (!x.acc || x.agg < YSQL_AGG_MAX || x.agg > YSQL_AGG_COUNT_STAR) return YSQL_EINVAL;
     const bool counting = x.agg == YSQL_AGG_COUNT || x.agg == YSQL_AGG_COUNT_STAR;
-    if (!counting && (!x.col || (x.dtype != YSQL_INT && x.dtype != YSQL_FLOAT && x.dtype != YSQL_DATETIME)))
-      return YSQL_EINVAL;
+    if (!counting && x.dtype != YSQL_INT && x.dtype != YSQL_FLOAT && x.dtype != YSQL_DATETIME) return YSQL_EINVAL;
+    if (!counting && !x.col && n > 0) return YSQL_EINVAL;  // an empty input reads no column
     a.spec[s].kind = fa_kind(x.dtype, x.agg);
     a.spec[s].op = fa_op(x.agg);
     a.spec[s].dtype = x.dtype;
     a.spec[s].col = x.col;
+    // a spec over the predicate column, or over the column of an earlier
+    // spec of the same width, reuses those loaded values
+    a.spec[s].src = FA_OWN;
+    if (x.col && x.col == pred_col && x.dtype == pred_dtype) a.spec[s].src = FA_PRED;
+    for (int t = 0; t < s && a.spec[s].src == FA_OWN && !counting; t++)
+      if (a.spec[t].src == FA_OWN && a.spec[t].kind != FA_KC && a.spec[t].col == x.col &&
+          fa_wide(a.spec[t]) == fa_wide(a.spec[s]))
+        a.spec[s].src = FA_SPEC0 + t;  // same column, same width: same words
     a.out[s] = x.acc;
   }
-  if (n == 0) return YSQL_OK;
+  hipStream_t s = as_stream(stream);
+  if (n == 0) {
+    if (overwrite) {
+      k_fa_identity<<<1, kWave, 0, s>>>(a, nspecs);
+      YSQL_LAUNCH_CHECK();
+    }
+    return YSQL_OK;
+  }
   if (!pred_col || !scalar) return YSQL_EINVAL;
   if (n >= (1ull << 40)) return YSQL_EINVAL;
   WsCarver c(ws, ws_bytes);
   uint32_t *ticket = c.take<uint32_t>(64);
   ysql_acc *partials = c.take<ysql_acc>(ceil_div(n, kFaTile) * (uint64_t)nspecs + 1);
   if (!ws || !ticket || !partials) return YSQL_EWORKSPACE;
-  hipStream_t s = as_stream(stream);
   k_reduce_reset<<<1, 1, 0, s>>>(ticket);
   YSQL_LAUNCH_CHECK();
   switch (pred_dtype) {
-    case YSQL_INT: return filter_agg_launch<int32_t>(cmp, pred_col, scalar, a, nspecs, n, partials, ticket, s);
-    case YSQL_FLOAT: return filter_agg_launch<float>(cmp, pred_col, scalar, a, nspecs, n, partials, ticket, s);
-    case YSQL_DATETIME: return filter_agg_launch<int64_t>(cmp, pred_col, scalar, a, nspecs, n, partials, ticket, s);
+    case YSQL_INT: return filter_agg_launch<int32_t>(cmp, pred_col, scalar, a, nspecs, n, partials, ticket, overwrite, s);
+    case YSQL_FLOAT: return filter_agg_launch<float>(cmp, pred_col, scalar, a, nspecs, n, partials, ticket, overwrite, s);
+    case YSQL_DATETIME:
+      return filter_agg_launch<int64_t>(cmp, pred_col, scalar, a, nspecs, n, partials, ticket, overwrite, s);
   }
   return YSQL_EINVAL;
+}
+
+int ysql_filter_aggregate(int pred_dtype, int cmp, const void *pred_col, const void *scalar, const ysql_agg_spec *specs,
+                          int nspecs, uint64_t n, void *ws, size_t ws_bytes, ysql_stream_t stream) {
+  return ysql_filter_aggregate_ex(pred_dtype, cmp, pred_col, scalar, specs, nspecs, n, 0, ws, ws_bytes, stream);
 }
 
 }  // extern "C"

@@ -272,6 +272,22 @@ def reduce(dtype_c, agg, x, acc, null=None, n=None):
     return acc
 
 
+def filter_aggregate(pred, cmp, scalar, specs, overwrite=False, n=None):
+    """Fused WHERE pred <cmp> scalar + aggregates.  specs: list of
+    (dtype_c, agg, column tensor or None, acc tensor).  With overwrite the
+    accumulators receive this call's result; otherwise it is folded in."""
+    n = pred.numel() if n is None else n
+    dev = pred.device
+    sc = torch.tensor([scalar], dtype=pred.dtype, device=dev)
+    arr = (L.AggSpec * len(specs))()
+    for k, (dc, agg, col, acc) in enumerate(specs):
+        arr[k] = L.AggSpec(dc, agg, _p(col) if col is not None else None, _p(acc))
+    ws = workspace(ws_bytes("ysql_filter_aggregate_ws_bytes", n, len(specs)), dev)
+    call("ysql_filter_aggregate_ex", _DT[pred.dtype], cmp, _p(pred), _p(sc), arr, len(specs), n,
+         L.FA_OVERWRITE if overwrite else 0, _p(ws), ws.numel(), _s())
+    return [acc for (_, _, _, acc) in specs]
+
+
 def agg_finalize(dtype_c, agg, acc):
     out = torch.zeros(8, dtype=torch.uint8, device=acc.device)
     call("ysql_agg_finalize", dtype_c, agg, _p(acc), _p(out), _s())
