@@ -46,25 +46,9 @@ def parse():
 
 def gen_keys(n, dev):
     """Config-2 keys on the GPU from a seeded torch generator."""
-    import torch
+    from yallasql_amd.datagen import multirun_keys
 
-    g = torch.Generator(device=dev)
-    g.manual_seed(SEED)
-    coin = torch.randint(0, 2, (n,), generator=g, device=dev, dtype=torch.int32).bool()
-    hi = torch.randint(0, 2**31, (n,), generator=g, device=dev, dtype=torch.int64)
-    lo = torch.randint(0, 2**32, (n,), generator=g, device=dev, dtype=torch.int64)
-    uni = (hi << 32) | lo
-    del hi, lo
-    ranks = torch.arange(1, ZIPF_N + 1, device=dev, dtype=torch.float64)
-    cdf = torch.cumsum(ranks.pow(-ZIPF_S), 0)
-    cdf /= cdf[-1].clone()
-    del ranks
-    u = torch.rand(n, generator=g, device=dev, dtype=torch.float64)
-    zipf = torch.searchsorted(cdf, u).clamp_(max=ZIPF_N - 1)
-    del u, cdf
-    keys = torch.where(coin, uni, zipf)
-    del uni, zipf, coin
-    return keys.contiguous()
+    return multirun_keys(n, dev, seed=SEED)
 
 
 class MultiRunSort:

@@ -241,6 +241,14 @@ int ysql_compact_cols(const uint8_t *mask, const ysql_col *cols, int ncols, uint
 int ysql_filter_compact(int dtype, int cmp, const void *pred_col, const void *scalar,
                         const ysql_col *cols, int ncols, uint64_t n, uint64_t *count_out,
                         void *ws, size_t ws_bytes, ysql_stream_t stream);
+/* Same with a range predicate bounds[0] <= pred_col[i] < bounds[1] (bounds:
+ * 2-element device array; the reference expresses it as two comparisons
+ * ANDed, conjunction.hpp:76-113, and compacts from the mask) and an output
+ * capacity: rows past `cap` are not written, *count_out is the full count
+ * (the caller retries with a larger capacity, like ysql_join_pairs).      */
+int ysql_filter_compact_range(int dtype, const void *pred_col, const void *bounds,
+                              const ysql_col *cols, int ncols, uint64_t n, uint64_t cap,
+                              uint64_t *count_out, void *ws, size_t ws_bytes, ysql_stream_t stream);
 
 /* ======================================================================
  * Predicates (A12)
@@ -312,6 +320,17 @@ int ysql_filter_aggregate(int pred_dtype, int cmp, const void *pred_col, const v
 int ysql_filter_aggregate_ex(int pred_dtype, int cmp, const void *pred_col, const void *scalar,
                              const ysql_agg_spec *specs, int nspecs, uint64_t n, int flags,
                              void *ws, size_t ws_bytes, ysql_stream_t stream);
+/* Shared scan for up to 4 queries (no reference counterpart: operator
+ * batching across in-flight queries).  Query q is `WHERE pred_col <cmp>
+ * scalars[q]` with the aggregates specs[q * nspecs + s]; every query must
+ * name the same columns, dtypes and aggregates (only the accumulators
+ * differ).  The columns are read once for all queries, and each query's
+ * result is bit-identical to running it alone through ysql_filter_aggregate_ex. */
+#define YSQL_MAX_SHARED_QUERIES 4
+int ysql_filter_aggregate_multi_ws_bytes(uint64_t n, int nqueries, int nspecs, size_t *bytes);
+int ysql_filter_aggregate_multi(int pred_dtype, int cmp, const void *pred_col, const void *scalars,
+                                int nqueries, const ysql_agg_spec *specs, int nspecs, uint64_t n,
+                                int flags, void *ws, size_t ws_bytes, ysql_stream_t stream);
 
 /* GROUP BY (no reference function: the reference's AggregateOperator never
  * reads `groups`, aggregate_operator.hpp:92-103).  acc_inout[g] folds the

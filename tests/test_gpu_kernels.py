@@ -393,6 +393,31 @@ def test_reduce_deterministic(ops, dev):
     assert len(outs) == 1
 
 
+@pytest.mark.parametrize("n", [0, 1, 4095, 4096, 4097, 300_001])
+@pytest.mark.parametrize("pdt", [np.int32, np.float32, np.int64])
+def test_filter_compact_range(ops, dev, n, pdt):
+    """lo <= pred < hi with a capacity smaller than the count (the wrapper
+    retries with the reported count), against the oracle's two comparisons
+    ANDed and its compaction."""
+    r = rng(n + 3)
+    if pdt == np.float32:
+        pred = r.random(n).astype(np.float32)
+        lo, hi = 0.25, 0.5
+    else:
+        pred = r.integers(-1000, 1000, n).astype(pdt)
+        lo, hi = -100, 300
+    pay = np.arange(n, dtype=np.int32)
+    m1 = O.compare(3, pred, np.array([lo], dtype=pdt), rhs_scalar=True, n=n)  # GE
+    m2 = O.compare(0, pred, np.array([hi], dtype=pdt), rhs_scalar=True, n=n)  # LT
+    m = O.mask_logic(0, m1, m2, n=n)
+    exp_k, _ = O.compact(pred, m)
+    exp_p, _ = O.compact(pay, m)
+    for cap in (None, 7):
+        (k, p), c = ops.filter_compact_range(T(pred, dev), lo, hi, [T(pred, dev), T(pay, dev)], cap=cap)
+        assert c == len(exp_k)
+        assert np.array_equal(N(k), exp_k) and np.array_equal(N(p), exp_p)
+
+
 FA_SIZES = [0, 1, 7, 8, 9, 32767, 32768, 32769, 1_000_003]
 
 
@@ -441,6 +466,32 @@ def test_filter_aggregate(ops, dev, n, pdt):
                         assert abs(got["d"] - e.d) <= 1e-6 * max(1.0, abs(e.d)), (cmp, agg)
                     elif agg != 4:
                         assert got["i"] == e.i, (cmp, dc, agg)
+
+
+@pytest.mark.parametrize("n", [0, 9, 32769, 2_000_003])
+@pytest.mark.parametrize("pdt", [np.int32, np.float32, np.int64])
+def test_filter_aggregate_shared_scan_same_bits(ops, dev, n, pdt):
+    """Each query of a shared scan returns the bytes it returns alone."""
+    r = rng(n + 11)
+    if pdt == np.float32:
+        pred = r.random(n).astype(np.float32)
+        consts = [float(c) for c in r.random(4)]
+    else:
+        pred = r.integers(-1000, 1000, n).astype(pdt)
+        consts = [int(c) for c in r.integers(-1000, 1000, 4)]
+    f, i64 = T(r.random(n).astype(np.float32), dev), T(r.integers(-2**62, 2**62, n).astype(np.int64), dev)
+    tp = T(pred, dev)
+    cols = [(O.FLOAT, 2, f), (O.DATETIME, 1, i64), (O.DATETIME, 0, i64), (O.INT, 5, None)]
+    for cmp in (0, 3, 5):
+        for nq in (1, 2, 3, 4):
+            accs = [[ops.acc_new(dc, agg, device=dev) for dc, agg, _ in cols] for _ in range(nq)]
+            ops.filter_aggregate_multi(tp, cmp, consts[:nq], cols, accs)
+            for q in range(nq):
+                alone = [ops.acc_new(dc, agg, device=dev) for dc, agg, _ in cols]
+                ops.filter_aggregate(tp, cmp, consts[q], [(dc, agg, col, a) for (dc, agg, col), a in zip(cols, alone)],
+                                     overwrite=True)
+                for a, b in zip(accs[q], alone):
+                    assert np.array_equal(N(a), N(b)), (cmp, nq, q)
 
 
 def test_filter_aggregate_fold_vs_overwrite(ops, dev):

@@ -78,6 +78,7 @@ _SIGS = {
     "ysql_compact_ws_bytes": [U64, SZP],
     "ysql_compact_cols": [P, C.POINTER(Col), I32, U64, P, P, C.c_size_t, P],
     "ysql_filter_compact": [I32, I32, P, P, C.POINTER(Col), I32, U64, P, P, C.c_size_t, P],
+    "ysql_filter_compact_range": [I32, P, P, C.POINTER(Col), I32, U64, U64, P, P, C.c_size_t, P],
     "ysql_compare": [I32, I32, P, I32, P, I32, I32, P, U64, P],
     "ysql_mask_logic": [I32, P, I32, P, I32, P, U64, P],
     "ysql_arith": [I32, I32, P, I32, P, I32, P, U64, P],
@@ -156,6 +157,8 @@ for _name, _args, _res in [
     ("ysql_filter_aggregate_ws_bytes", [U64, I32, SZP], C.c_int),
     ("ysql_filter_aggregate", [I32, I32, P, P, P, I32, U64, P, C.c_size_t, P], C.c_int),
     ("ysql_filter_aggregate_ex", [I32, I32, P, P, P, I32, U64, I32, P, C.c_size_t, P], C.c_int),
+    ("ysql_filter_aggregate_multi_ws_bytes", [U64, I32, I32, SZP], C.c_int),
+    ("ysql_filter_aggregate_multi", [I32, I32, P, P, I32, P, I32, U64, I32, P, C.c_size_t, P], C.c_int),
     ("ysql_group_ids", [P, P, P, I32, U64, P, P], C.c_int),
     ("ysql_sortkey_pack", [I32, I32, P, U64, P, P], C.c_int),
     ("ysql_batch_tuning_default", [C.POINTER(BatchTuning)], None),

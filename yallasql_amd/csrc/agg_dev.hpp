@@ -18,6 +18,7 @@ namespace ysql {
 constexpr uint64_t kFaTile = 32768;
 constexpr int kFaBlock = 256;
 constexpr int kFaMaxSpecs = 4;
+constexpr int kFaMaxQueries = 4;  // queries sharing one scan
 constexpr int kFaRpt = 8;  // rows per thread per step
 
 // Accumulator kinds: int64 in .i, float in .f (MIN/MAX), double in .d (SUM),
@@ -197,23 +198,29 @@ __device__ __forceinline__ void fa_fold_words(ysql_acc &a, const FaSpec &s, cons
   }
 }
 
-// Partials of canonical tile `tile`: partials[tile * ns + s].  Thread t owns
-// rows base + t*kFaRpt .. of every step of BLOCK*kFaRpt rows and folds them
-// in row order.  Each step first issues every load (the predicate and each
-// spec column that is not the predicate column, 128-bit loads), then
-// evaluates the predicate and folds.
-template <typename PT, int CMP, int BLOCK>
-__device__ __forceinline__ void filter_agg_tile(uint32_t tile, const PT *pred, PT scalar, const FaSpec *specs, int ns,
-                                                uint64_t n, ysql_acc *partials, ysql_acc *s_w) {
+// Partials of canonical tile `tile` for nq <= QMAX queries that share the
+// predicate column, the comparison and the spec columns and differ in the
+// constant: partials[(tile * nq + q) * ns + s].  Thread t owns rows
+// base + t*kFaRpt .. of every step of BLOCK*kFaRpt rows.  Each step first
+// issues every load (the predicate and each spec column that is not the
+// predicate column, 128-bit loads), then evaluates each query's predicate
+// and folds its rows in row order.  A query's fold sequence does not depend
+// on the other queries of the launch, so its result is the same alone or
+// batched.
+template <typename PT, int CMP, int BLOCK, int QMAX>
+__device__ __forceinline__ void filter_agg_tile(uint32_t tile, const PT *pred, const PT (&scalar)[QMAX], int nq,
+                                                const FaSpec *specs, int ns, uint64_t n, ysql_acc *partials,
+                                                ysql_acc *s_w) {
   const uint64_t base = (uint64_t)tile * kFaTile;
   const uint64_t end = base + kFaTile < n ? base + kFaTile : n;
-  ysql_acc acc[kFaMaxSpecs];
+  ysql_acc acc[QMAX][kFaMaxSpecs];
   FaSpec sp[kFaMaxSpecs];
 #pragma unroll
   for (int s = 0; s < kFaMaxSpecs; s++) {
     if (s < ns) {
       sp[s] = specs[s];
-      fa_identity(acc[s], sp[s]);
+#pragma unroll
+      for (int q = 0; q < QMAX; q++) fa_identity(acc[q][s], sp[s]);
     }
   }
   for (uint64_t r0 = base + (uint64_t)threadIdx.x * kFaRpt; r0 < end; r0 += (uint64_t)BLOCK * kFaRpt) {
@@ -223,58 +230,68 @@ __device__ __forceinline__ void filter_agg_tile(uint32_t tile, const PT *pred, P
 #pragma unroll
     for (int s = 0; s < kFaMaxSpecs; s++)
       if (s < ns && sp[s].kind != FA_KC && sp[s].src == FA_OWN) fa_load_words(sp[s], r0, end, w[s]);
-    uint32_t flags = 0;
 #pragma unroll
-    for (int e = 0; e < kFaRpt; e++) flags |= (uint32_t)(r0 + e < end && cmp_static<CMP>(pv[e], scalar)) << e;
+    for (int q = 0; q < QMAX; q++) {
+      if (q >= nq) break;
+      uint32_t flags = 0;
 #pragma unroll
-    for (int s = 0; s < kFaMaxSpecs; s++) {
-      if (s < ns) {
-        const int src = sp[s].src;
-        if (src == FA_PRED) {
-          uint32_t pw[2 * kFaRpt];
-          memcpy(pw, pv, sizeof(pv));
-          fa_fold_words(acc[s], sp[s], pw, flags);
-        } else if (src == FA_OWN || sp[s].kind == FA_KC) {
-          fa_fold_words(acc[s], sp[s], w[s], flags);
-        } else {  // the words an earlier spec loaded (static indices only)
+      for (int e = 0; e < kFaRpt; e++) flags |= (uint32_t)(r0 + e < end && cmp_static<CMP>(pv[e], scalar[q])) << e;
 #pragma unroll
-          for (int t = 0; t < s; t++)
-            if (src == FA_SPEC0 + t) fa_fold_words(acc[s], sp[s], w[t], flags);
+      for (int s = 0; s < kFaMaxSpecs; s++) {
+        if (s < ns) {
+          const int src = sp[s].src;
+          if (src == FA_PRED) {
+            uint32_t pw[2 * kFaRpt];
+            memcpy(pw, pv, sizeof(pv));
+            fa_fold_words(acc[q][s], sp[s], pw, flags);
+          } else if (src == FA_OWN || sp[s].kind == FA_KC) {
+            fa_fold_words(acc[q][s], sp[s], w[s], flags);
+          } else {  // the words an earlier spec loaded (static indices only)
+#pragma unroll
+            for (int t = 0; t < s; t++)
+              if (src == FA_SPEC0 + t) fa_fold_words(acc[q][s], sp[s], w[t], flags);
+          }
         }
       }
     }
   }
 #pragma unroll
-  for (int s = 0; s < kFaMaxSpecs; s++) {
-    if (s < ns) {
-      const ysql_acc tot = fa_block_reduce<BLOCK>(acc[s], sp[s], s_w);
-      if (threadIdx.x == 0) partials[(uint64_t)tile * ns + s] = tot;
+  for (int q = 0; q < QMAX; q++) {
+    if (q >= nq) break;
+#pragma unroll
+    for (int s = 0; s < kFaMaxSpecs; s++) {
+      if (s < ns) {
+        const ysql_acc tot = fa_block_reduce<BLOCK>(acc[q][s], sp[s], s_w);
+        if (threadIdx.x == 0) partials[((uint64_t)tile * nq + q) * ns + s] = tot;
+      }
     }
   }
 }
 
-// Combines the ntiles partials in a fixed order; folds the totals into
-// out[s] (accumulators that persist across calls), or with `overwrite`
-// stores them there.
+// Combines the ntiles partials of every (query, spec) in a fixed order; folds
+// the totals into out[q * ns + s] (accumulators that persist across calls),
+// or with `overwrite` stores them there.
 template <int BLOCK>
-__device__ __forceinline__ void filter_agg_finish(const FaSpec *specs, int ns, uint32_t ntiles,
+__device__ __forceinline__ void filter_agg_finish(const FaSpec *specs, int ns, int nq, uint32_t ntiles,
                                                   const ysql_acc *partials, ysql_acc *const *out, bool overwrite,
                                                   ysql_acc *s_w) {
-  for (int s = 0; s < ns; s++) {
-    const FaSpec sp = specs[s];
-    ysql_acc p;
-    fa_identity(p, sp);
-    for (uint32_t x = threadIdx.x; x < ntiles; x += BLOCK) {
-      const ysql_acc q = partials[(uint64_t)x * ns + s];
-      fa_combine(p, q, sp);
-    }
-    const ysql_acc tot = fa_block_reduce<BLOCK>(p, sp, s_w);
-    if (threadIdx.x == 0) {
-      ysql_acc a;
-      if (overwrite) fa_identity(a, sp);
-      else a = *out[s];
-      fa_combine(a, tot, sp);
-      *out[s] = a;
+  for (int q = 0; q < nq; q++) {
+    for (int s = 0; s < ns; s++) {
+      const FaSpec sp = specs[s];
+      ysql_acc p;
+      fa_identity(p, sp);
+      for (uint32_t x = threadIdx.x; x < ntiles; x += BLOCK) {
+        const ysql_acc v = partials[((uint64_t)x * nq + q) * ns + s];
+        fa_combine(p, v, sp);
+      }
+      const ysql_acc tot = fa_block_reduce<BLOCK>(p, sp, s_w);
+      if (threadIdx.x == 0) {
+        ysql_acc a;
+        if (overwrite) fa_identity(a, sp);
+        else a = *out[q * ns + s];
+        fa_combine(a, tot, sp);
+        *out[q * ns + s] = a;
+      }
     }
   }
 }

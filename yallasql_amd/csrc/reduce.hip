@@ -664,34 +664,38 @@ namespace ysql {
 
 struct FaArgs {
   FaSpec spec[kFaMaxSpecs];
-  ysql_acc *out[kFaMaxSpecs];
+  ysql_acc *out[kFaMaxQueries * kFaMaxSpecs];  // [query][spec]
 };
 
-template <typename PT, int CMP>
-__global__ __launch_bounds__(kFaBlock) void k_filter_agg(const PT *pred, const PT *scalar, FaArgs a, int ns, uint64_t n,
-                                                         ysql_acc *partials, uint32_t *ticket, uint32_t ntiles,
-                                                         int overwrite) {
+// QMAX = 1 for single queries (fewer registers, higher occupancy), 4 for
+// shared scans; a query's fold sequence is the same in both.
+template <typename PT, int CMP, int QMAX>
+__global__ __launch_bounds__(kFaBlock) void k_filter_agg(const PT *pred, const PT *scalars, int nq, FaArgs a, int ns,
+                                                         uint64_t n, ysql_acc *partials, uint32_t *ticket,
+                                                         uint32_t ntiles, int overwrite) {
   __shared__ ysql_acc s_w[kFaBlock / kWave];
   __shared__ int s_last;
-  const PT sc = scalar[0];
+  PT sc[QMAX];
+#pragma unroll
+  for (int q = 0; q < QMAX; q++) sc[q] = q < nq ? scalars[q] : PT();
   for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x)
-    filter_agg_tile<PT, CMP, kFaBlock>(tile, pred, sc, a.spec, ns, n, partials, s_w);
+    filter_agg_tile<PT, CMP, kFaBlock, QMAX>(tile, pred, sc, nq, a.spec, ns, n, partials, s_w);
   __threadfence();
   if (threadIdx.x == 0) s_last = (atomicAdd(ticket, 1u) == gridDim.x - 1);
   __syncthreads();
   if (!s_last) return;
   __threadfence();
-  filter_agg_finish<kFaBlock>(a.spec, ns, ntiles, partials, a.out, overwrite != 0, s_w);
+  filter_agg_finish<kFaBlock>(a.spec, ns, nq, ntiles, partials, a.out, overwrite != 0, s_w);
   if (threadIdx.x == 0) *ticket = 0;
 }
 
 // Identities into the outputs (an empty input with overwrite).
-__global__ void k_fa_identity(FaArgs a, int ns) {
-  const int s = threadIdx.x;
-  if (s < ns) {
+__global__ void k_fa_identity(FaArgs a, int ns, int nq) {
+  const int k = threadIdx.x;
+  if (k < ns * nq) {
     ysql_acc v;
-    fa_identity(v, a.spec[s]);
-    *a.out[s] = v;
+    fa_identity(v, a.spec[k % ns]);
+    *a.out[k] = v;
   }
 }
 
@@ -702,19 +706,19 @@ static uint32_t fa_blocks_per_cu(K kernel) {
   return (uint32_t)b;
 }
 
-template <typename PT>
-static int filter_agg_launch(int cmp, const void *pred, const void *scalar, const FaArgs &a, int ns, uint64_t n,
-                             ysql_acc *partials, uint32_t *ticket, int overwrite, hipStream_t s) {
+template <typename PT, int QMAX>
+static int filter_agg_launch(int cmp, const void *pred, const void *scalars, int nq, const FaArgs &a, int ns,
+                             uint64_t n, ysql_acc *partials, uint32_t *ticket, int overwrite, hipStream_t s) {
   const uint32_t ntiles = (uint32_t)ceil_div(n, kFaTile);
   const PT *p = (const PT *)pred;
-  const PT *sc = (const PT *)scalar;
+  const PT *sc = (const PT *)scalars;
   // grid: the workgroups that fit at once (tiles are taken grid-stride)
-#define FA_CASE(C)                                                                          \
-  case C: {                                                                                 \
-    static const uint32_t per_cu = fa_blocks_per_cu(k_filter_agg<PT, C>);                  \
-    const uint32_t g = std::min<uint32_t>(ntiles, per_cu * (uint32_t)grid_cap());           \
-    k_filter_agg<PT, C><<<g, kFaBlock, 0, s>>>(p, sc, a, ns, n, partials, ticket, ntiles, overwrite); \
-    break;                                                                                  \
+#define FA_CASE(C)                                                                                         \
+  case C: {                                                                                                \
+    static const uint32_t per_cu = fa_blocks_per_cu(k_filter_agg<PT, C, QMAX>);                           \
+    const uint32_t g = std::min<uint32_t>(ntiles, per_cu * (uint32_t)grid_cap());                          \
+    k_filter_agg<PT, C, QMAX><<<g, kFaBlock, 0, s>>>(p, sc, nq, a, ns, n, partials, ticket, ntiles, overwrite); \
+    break;                                                                                                 \
   }
   switch (cmp) {
     FA_CASE(YSQL_LT)
@@ -730,11 +734,78 @@ static int filter_agg_launch(int cmp, const void *pred, const void *scalar, cons
   return YSQL_OK;
 }
 
-static size_t filter_agg_ws_size(uint64_t n, int ns) {
+static size_t filter_agg_ws_size(uint64_t n, int nq, int ns) {
   WsCounter c;
   c.take<uint32_t>(64);
-  c.take<ysql_acc>(ceil_div(n, kFaTile) * (uint64_t)ns + 1);
+  c.take<ysql_acc>(ceil_div(n, kFaTile) * (uint64_t)nq * ns + 1);
   return c.off;
+}
+
+// nq queries; specs[q * nspecs + s]: every query has the same columns,
+// dtypes and aggregates (only the accumulators differ).
+static int filter_agg_impl(int pred_dtype, int cmp, const void *pred_col, const void *scalars, int nq,
+                           const ysql_agg_spec *specs, int nspecs, uint64_t n, int flags, void *ws, size_t ws_bytes,
+                           hipStream_t s) {
+  if (nspecs < 1 || nspecs > kFaMaxSpecs || nq < 1 || nq > kFaMaxQueries || !specs || cmp < YSQL_LT ||
+      cmp > YSQL_NE)
+    return YSQL_EINVAL;
+  if (flags & ~YSQL_FA_OVERWRITE) return YSQL_EINVAL;
+  const int overwrite = (flags & YSQL_FA_OVERWRITE) ? 1 : 0;
+  FaArgs a;
+  memset(&a, 0, sizeof(a));
+  for (int k = 0; k < nspecs; k++) {
+    const ysql_agg_spec &x = specs[k];
+    if (x.agg < YSQL_AGG_MAX || x.agg > YSQL_AGG_COUNT_STAR) return YSQL_EINVAL;
+    const bool counting = x.agg == YSQL_AGG_COUNT || x.agg == YSQL_AGG_COUNT_STAR;
+    if (!counting && x.dtype != YSQL_INT && x.dtype != YSQL_FLOAT && x.dtype != YSQL_DATETIME) return YSQL_EINVAL;
+    if (!counting && !x.col && n > 0) return YSQL_EINVAL;  // an empty input reads no column
+    a.spec[k].kind = fa_kind(x.dtype, x.agg);
+    a.spec[k].op = fa_op(x.agg);
+    a.spec[k].dtype = x.dtype;
+    a.spec[k].col = x.col;
+    // a spec over the predicate column, or over the column of an earlier
+    // spec of the same width, reuses those loaded values
+    a.spec[k].src = FA_OWN;
+    if (x.col && x.col == pred_col && x.dtype == pred_dtype) a.spec[k].src = FA_PRED;
+    for (int t = 0; t < k && a.spec[k].src == FA_OWN && !counting; t++)
+      if (a.spec[t].src == FA_OWN && a.spec[t].kind != FA_KC && a.spec[t].col == x.col &&
+          fa_wide(a.spec[t]) == fa_wide(a.spec[k]))
+        a.spec[k].src = FA_SPEC0 + t;  // same column, same width: same words
+  }
+  for (int q = 0; q < nq; q++) {
+    for (int k = 0; k < nspecs; k++) {
+      const ysql_agg_spec &x = specs[q * nspecs + k];
+      const ysql_agg_spec &x0 = specs[k];
+      if (!x.acc || x.dtype != x0.dtype || x.agg != x0.agg || x.col != x0.col) return YSQL_EINVAL;
+      a.out[q * nspecs + k] = x.acc;
+    }
+  }
+  if (n == 0) {
+    if (overwrite) {
+      k_fa_identity<<<1, kWave, 0, s>>>(a, nspecs, nq);
+      YSQL_LAUNCH_CHECK();
+    }
+    return YSQL_OK;
+  }
+  if (!pred_col || !scalars) return YSQL_EINVAL;
+  if (n >= (1ull << 40)) return YSQL_EINVAL;
+  WsCarver c(ws, ws_bytes);
+  uint32_t *ticket = c.take<uint32_t>(64);
+  ysql_acc *partials = c.take<ysql_acc>(ceil_div(n, kFaTile) * (uint64_t)nq * nspecs + 1);
+  if (!ws || !ticket || !partials) return YSQL_EWORKSPACE;
+  k_reduce_reset<<<1, 1, 0, s>>>(ticket);
+  YSQL_LAUNCH_CHECK();
+#define FA_DT(PT)                                                                                              \
+  return nq == 1 ? filter_agg_launch<PT, 1>(cmp, pred_col, scalars, nq, a, nspecs, n, partials, ticket, overwrite, s) \
+                 : filter_agg_launch<PT, kFaMaxQueries>(cmp, pred_col, scalars, nq, a, nspecs, n, partials, ticket,    \
+                                                        overwrite, s);
+  switch (pred_dtype) {
+    case YSQL_INT: FA_DT(int32_t)
+    case YSQL_FLOAT: FA_DT(float)
+    case YSQL_DATETIME: FA_DT(int64_t)
+  }
+#undef FA_DT
+  return YSQL_EINVAL;
 }
 
 }  // namespace ysql
@@ -743,66 +814,33 @@ extern "C" {
 
 int ysql_filter_aggregate_ws_bytes(uint64_t n, int nspecs, size_t *bytes) {
   if (!bytes || nspecs < 1 || nspecs > kFaMaxSpecs) return YSQL_EINVAL;
-  *bytes = filter_agg_ws_size(n, nspecs);
+  *bytes = filter_agg_ws_size(n, 1, nspecs);
   return YSQL_OK;
 }
 
 int ysql_filter_aggregate_ex(int pred_dtype, int cmp, const void *pred_col, const void *scalar,
                              const ysql_agg_spec *specs, int nspecs, uint64_t n, int flags, void *ws, size_t ws_bytes,
                              ysql_stream_t stream) {
-  if (nspecs < 1 || nspecs > kFaMaxSpecs || !specs || cmp < YSQL_LT || cmp > YSQL_NE) return YSQL_EINVAL;
-  if (flags & ~YSQL_FA_OVERWRITE) return YSQL_EINVAL;
-  const int overwrite = (flags & YSQL_FA_OVERWRITE) ? 1 : 0;
-  FaArgs a;
-  memset(&a, 0, sizeof(a));
-  for (int s = 0; s < nspecs; s++) {
-    const ysql_agg_spec &x = specs[s];
-    if (!x.acc || x.agg < YSQL_AGG_MAX || x.agg > YSQL_AGG_COUNT_STAR) return YSQL_EINVAL;
-    const bool counting = x.agg == YSQL_AGG_COUNT || x.agg == YSQL_AGG_COUNT_STAR;
-    if (!counting && x.dtype != YSQL_INT && x.dtype != YSQL_FLOAT && x.dtype != YSQL_DATETIME) return YSQL_EINVAL;
-    if (!counting && !x.col && n > 0) return YSQL_EINVAL;  // an empty input reads no column
-    a.spec[s].kind = fa_kind(x.dtype, x.agg);
-    a.spec[s].op = fa_op(x.agg);
-    a.spec[s].dtype = x.dtype;
-    a.spec[s].col = x.col;
-    // a spec over the predicate column, or over the column of an earlier
-    // spec of the same width, reuses those loaded values
-    a.spec[s].src = FA_OWN;
-    if (x.col && x.col == pred_col && x.dtype == pred_dtype) a.spec[s].src = FA_PRED;
-    for (int t = 0; t < s && a.spec[s].src == FA_OWN && !counting; t++)
-      if (a.spec[t].src == FA_OWN && a.spec[t].kind != FA_KC && a.spec[t].col == x.col &&
-          fa_wide(a.spec[t]) == fa_wide(a.spec[s]))
-        a.spec[s].src = FA_SPEC0 + t;  // same column, same width: same words
-    a.out[s] = x.acc;
-  }
-  hipStream_t s = as_stream(stream);
-  if (n == 0) {
-    if (overwrite) {
-      k_fa_identity<<<1, kWave, 0, s>>>(a, nspecs);
-      YSQL_LAUNCH_CHECK();
-    }
-    return YSQL_OK;
-  }
-  if (!pred_col || !scalar) return YSQL_EINVAL;
-  if (n >= (1ull << 40)) return YSQL_EINVAL;
-  WsCarver c(ws, ws_bytes);
-  uint32_t *ticket = c.take<uint32_t>(64);
-  ysql_acc *partials = c.take<ysql_acc>(ceil_div(n, kFaTile) * (uint64_t)nspecs + 1);
-  if (!ws || !ticket || !partials) return YSQL_EWORKSPACE;
-  k_reduce_reset<<<1, 1, 0, s>>>(ticket);
-  YSQL_LAUNCH_CHECK();
-  switch (pred_dtype) {
-    case YSQL_INT: return filter_agg_launch<int32_t>(cmp, pred_col, scalar, a, nspecs, n, partials, ticket, overwrite, s);
-    case YSQL_FLOAT: return filter_agg_launch<float>(cmp, pred_col, scalar, a, nspecs, n, partials, ticket, overwrite, s);
-    case YSQL_DATETIME:
-      return filter_agg_launch<int64_t>(cmp, pred_col, scalar, a, nspecs, n, partials, ticket, overwrite, s);
-  }
-  return YSQL_EINVAL;
+  return filter_agg_impl(pred_dtype, cmp, pred_col, scalar, 1, specs, nspecs, n, flags, ws, ws_bytes,
+                         as_stream(stream));
 }
 
 int ysql_filter_aggregate(int pred_dtype, int cmp, const void *pred_col, const void *scalar, const ysql_agg_spec *specs,
                           int nspecs, uint64_t n, void *ws, size_t ws_bytes, ysql_stream_t stream) {
-  return ysql_filter_aggregate_ex(pred_dtype, cmp, pred_col, scalar, specs, nspecs, n, 0, ws, ws_bytes, stream);
+  return filter_agg_impl(pred_dtype, cmp, pred_col, scalar, 1, specs, nspecs, n, 0, ws, ws_bytes, as_stream(stream));
+}
+
+int ysql_filter_aggregate_multi_ws_bytes(uint64_t n, int nqueries, int nspecs, size_t *bytes) {
+  if (!bytes || nspecs < 1 || nspecs > kFaMaxSpecs || nqueries < 1 || nqueries > kFaMaxQueries) return YSQL_EINVAL;
+  *bytes = filter_agg_ws_size(n, nqueries, nspecs);
+  return YSQL_OK;
+}
+
+int ysql_filter_aggregate_multi(int pred_dtype, int cmp, const void *pred_col, const void *scalars, int nqueries,
+                                const ysql_agg_spec *specs, int nspecs, uint64_t n, int flags, void *ws,
+                                size_t ws_bytes, ysql_stream_t stream) {
+  return filter_agg_impl(pred_dtype, cmp, pred_col, scalars, nqueries, specs, nspecs, n, flags, ws, ws_bytes,
+                         as_stream(stream));
 }
 
 }  // extern "C"

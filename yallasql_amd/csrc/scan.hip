@@ -177,15 +177,19 @@ struct ColArgs {
   ysql_col c[YSQL_MAX_COLS];
 };
 
+constexpr int kCmpRange = 100;  // internal: lo <= pred < hi
+
 template <int MODE, typename PT, int CMP>
 __global__ __launch_bounds__(kScanBlock) void k_compact(const uint8_t *mask, const PT *pred, const PT *scalar, ColArgs cols,
-                                                        int ncols, uint64_t n, uint64_t *count_out, uint32_t *ctr,
-                                                        uint64_t *status, uint32_t ntiles) {
+                                                        int ncols, uint64_t n, uint64_t cap, uint64_t *count_out,
+                                                        uint32_t *ctr, uint64_t *status, uint32_t ntiles) {
   __shared__ CompactSmem<kScanBlock> sm;
   __shared__ uint32_t s_tile;
   const uint32_t tile = take_tile(ctr, &s_tile);
-  const PT s = MODE == 1 ? scalar[0] : PT();
-  compact_tile<kScanBlock, MODE, PT, CMP>(sm, tile, mask, pred, s, cols.c, ncols, n, count_out, status, ntiles);
+  const PT s = MODE >= 1 ? scalar[0] : PT();
+  const PT hi = MODE == 2 ? scalar[1] : PT();
+  compact_tile<kScanBlock, MODE, PT, CMP>(sm, tile, mask, pred, s, hi, cols.c, ncols, n, cap, count_out, status,
+                                          ntiles);
 }
 
 static bool cols_ok(const ysql_col *cols, int ncols, bool need_ptrs) {
@@ -203,11 +207,16 @@ __global__ void k_store_zero(uint64_t *p) { *p = 0; }
 
 template <typename PT>
 static void launch_pred(int cmp, dim3 g, hipStream_t s, const PT *pred, const PT *scalar, const ColArgs &ca, int ncols,
-                        uint64_t n, uint64_t *count_out, const ScanWs &w, uint32_t ntiles) {
+                        uint64_t n, uint64_t cap, uint64_t *count_out, const ScanWs &w, uint32_t ntiles) {
+  if (cmp == kCmpRange) {
+    k_compact<2, PT, YSQL_LT><<<g, kScanBlock, 0, s>>>(nullptr, pred, scalar, ca, ncols, n, cap, count_out, w.ctr,
+                                                       w.status, ntiles);
+    return;
+  }
 #define PRED_CASE(C)                                                                                              \
   case C:                                                                                                         \
-    k_compact<1, PT, C><<<g, kScanBlock, 0, s>>>(nullptr, pred, scalar, ca, ncols, n, count_out, w.ctr, w.status, \
-                                                 ntiles);                                                         \
+    k_compact<1, PT, C><<<g, kScanBlock, 0, s>>>(nullptr, pred, scalar, ca, ncols, n, cap, count_out, w.ctr,      \
+                                                 w.status, ntiles);                                               \
     break;
   switch (cmp) {
     PRED_CASE(YSQL_LT)
@@ -221,8 +230,8 @@ static void launch_pred(int cmp, dim3 g, hipStream_t s, const PT *pred, const PT
 }
 
 static int compact_impl(int mode, int dtype, int cmp, const uint8_t *mask, const void *pred, const void *scalar,
-                        const ysql_col *cols, int ncols, uint64_t n, uint64_t *count_out, void *ws, size_t ws_bytes,
-                        hipStream_t s) {
+                        const ysql_col *cols, int ncols, uint64_t n, uint64_t cap, uint64_t *count_out, void *ws,
+                        size_t ws_bytes, hipStream_t s) {
   if (!cols_ok(cols, ncols, n > 0)) return YSQL_EINVAL;
   if (n >= (1ull << 40)) return YSQL_EINVAL;
   if (n == 0) {
@@ -233,7 +242,7 @@ static int compact_impl(int mode, int dtype, int cmp, const uint8_t *mask, const
     return YSQL_OK;
   }
   if (mode == 0 && !mask) return YSQL_EINVAL;
-  if (mode == 1 && (!pred || !scalar || cmp < YSQL_LT || cmp > YSQL_NE)) return YSQL_EINVAL;
+  if (mode == 1 && (!pred || !scalar || ((cmp < YSQL_LT || cmp > YSQL_NE) && cmp != kCmpRange))) return YSQL_EINVAL;
   ScanWs w;
   if (!ws || ws_bytes < scan_ws_size(n) || !scan_ws_carve(ws, ws_bytes, n, &w)) return YSQL_EWORKSPACE;
   const uint32_t ntiles = (uint32_t)ceil_div(n, kScanTile);
@@ -244,13 +253,13 @@ static int compact_impl(int mode, int dtype, int cmp, const uint8_t *mask, const
   for (int i = 0; i < ncols; i++) ca.c[i] = cols[i];
   dim3 g(ntiles);
   if (mode == 0) {
-    k_compact<0, uint8_t, 0><<<g, kScanBlock, 0, s>>>(mask, nullptr, nullptr, ca, ncols, n, count_out, w.ctr, w.status,
-                                                      ntiles);
+    k_compact<0, uint8_t, 0><<<g, kScanBlock, 0, s>>>(mask, nullptr, nullptr, ca, ncols, n, cap, count_out, w.ctr,
+                                                      w.status, ntiles);
   } else {
     switch (dtype) {
-      case YSQL_INT: launch_pred<int32_t>(cmp, g, s, (const int32_t *)pred, (const int32_t *)scalar, ca, ncols, n, count_out, w, ntiles); break;
-      case YSQL_FLOAT: launch_pred<float>(cmp, g, s, (const float *)pred, (const float *)scalar, ca, ncols, n, count_out, w, ntiles); break;
-      case YSQL_DATETIME: launch_pred<int64_t>(cmp, g, s, (const int64_t *)pred, (const int64_t *)scalar, ca, ncols, n, count_out, w, ntiles); break;
+      case YSQL_INT: launch_pred<int32_t>(cmp, g, s, (const int32_t *)pred, (const int32_t *)scalar, ca, ncols, n, cap, count_out, w, ntiles); break;
+      case YSQL_FLOAT: launch_pred<float>(cmp, g, s, (const float *)pred, (const float *)scalar, ca, ncols, n, cap, count_out, w, ntiles); break;
+      case YSQL_DATETIME: launch_pred<int64_t>(cmp, g, s, (const int64_t *)pred, (const int64_t *)scalar, ca, ncols, n, cap, count_out, w, ntiles); break;
       default: return YSQL_EINVAL;
     }
   }
@@ -306,12 +315,21 @@ int ysql_compact_ws_bytes(uint64_t n, size_t *bytes) {
 
 int ysql_compact_cols(const uint8_t *mask, const ysql_col *cols, int ncols, uint64_t n, uint64_t *count_out, void *ws,
                       size_t ws_bytes, ysql_stream_t stream) {
-  return compact_impl(0, 0, 0, mask, nullptr, nullptr, cols, ncols, n, count_out, ws, ws_bytes, as_stream(stream));
+  return compact_impl(0, 0, 0, mask, nullptr, nullptr, cols, ncols, n, ~0ull, count_out, ws, ws_bytes,
+                      as_stream(stream));
 }
 
 int ysql_filter_compact(int dtype, int cmp, const void *pred_col, const void *scalar, const ysql_col *cols, int ncols,
                         uint64_t n, uint64_t *count_out, void *ws, size_t ws_bytes, ysql_stream_t stream) {
-  return compact_impl(1, dtype, cmp, nullptr, pred_col, scalar, cols, ncols, n, count_out, ws, ws_bytes,
+  if (cmp < YSQL_LT || cmp > YSQL_NE) return YSQL_EINVAL;
+  return compact_impl(1, dtype, cmp, nullptr, pred_col, scalar, cols, ncols, n, ~0ull, count_out, ws, ws_bytes,
+                      as_stream(stream));
+}
+
+int ysql_filter_compact_range(int dtype, const void *pred_col, const void *bounds, const ysql_col *cols, int ncols,
+                              uint64_t n, uint64_t cap, uint64_t *count_out, void *ws, size_t ws_bytes,
+                              ysql_stream_t stream) {
+  return compact_impl(1, dtype, kCmpRange, nullptr, pred_col, bounds, cols, ncols, n, cap, count_out, ws, ws_bytes,
                       as_stream(stream));
 }
 

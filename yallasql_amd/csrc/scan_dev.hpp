@@ -37,11 +37,14 @@ __device__ __forceinline__ void load16(const PT *p, uint64_t base, uint64_t rows
   }
 }
 
-// MODE 0: selection from mask bytes; MODE 1: pred[i] <CMP> scalar.
+// MODE 0: selection from mask bytes; MODE 1: pred[i] <CMP> scalar;
+// MODE 2: scalar <= pred[i] < hi.  Output rows at index >= cap are not
+// written; *count_out receives the full count.
 // Every thread of the block calls it; contains barriers.
 template <int BLOCK, int MODE, typename PT, int CMP>
 __device__ __forceinline__ void compact_tile(CompactSmem<BLOCK> &sm, uint32_t tile, const uint8_t *mask,
-                                             const PT *pred, PT scalar, const ysql_col *cols, int ncols, uint64_t n,
+                                             const PT *pred, PT scalar, PT hi, const ysql_col *cols, int ncols,
+                                             uint64_t n, uint64_t cap,
                                              uint64_t *count_out, uint64_t *status, uint32_t ntiles) {
   constexpr int TILE = BLOCK * 16;
   const uint64_t base = (uint64_t)tile * TILE;
@@ -58,7 +61,10 @@ __device__ __forceinline__ void compact_tile(CompactSmem<BLOCK> &sm, uint32_t ti
     PT v[16];
     load16<PT>(pred, base, rows, t, v);
 #pragma unroll
-    for (int e = 0; e < 16; e++) flags |= (uint32_t)cmp_static<CMP>(v[e], scalar) << e;
+    for (int e = 0; e < 16; e++) {
+      const bool sel = MODE == 1 ? cmp_static<CMP>(v[e], scalar) : (!(v[e] < scalar) && v[e] < hi);
+      flags |= (uint32_t)sel << e;
+    }
   }
   {
     const uint64_t r0 = (uint64_t)t * 16;
@@ -82,7 +88,8 @@ __device__ __forceinline__ void compact_tile(CompactSmem<BLOCK> &sm, uint32_t ti
   }
   __syncthreads();
   const uint64_t prefix = sm.prefix;
-  const uint32_t tot = (uint32_t)total;
+  // rows whose output index reaches `cap` are not written (the count still is)
+  const uint32_t tot = prefix >= cap ? 0u : (uint32_t)(cap - prefix < total ? cap - prefix : total);
   for (int c = 0; c < ncols; c++) {
     const ysql_col col = cols[c];
     switch (col.elem_bytes) {
@@ -125,7 +132,7 @@ __device__ __forceinline__ void compact_tile(CompactSmem<BLOCK> &sm, uint32_t ti
         col.dst_null[prefix + r] = col.src_null ? col.src_null[base + sm.src[r]] : 0;
     }
   }
-  if (tile == ntiles - 1 && t == 0 && count_out) *count_out = prefix + tot;
+  if (tile == ntiles - 1 && t == 0 && count_out) *count_out = prefix + total;
   __syncthreads();  // LDS is reused by the caller's next tile
 }
 

@@ -215,6 +215,28 @@ def filter_compact(cmp, pred, scalar, srcs, nulls=None):
     return [(d[:c], dn[:c]) for d, dn in outs], c
 
 
+def filter_compact_range(pred, lo, hi, srcs, cap=None):
+    """Rows with lo <= pred < hi, every column of srcs compacted (no nulls).
+    Output buffers hold `cap` rows (default n); on overflow the call is
+    repeated with the exact count.  Returns ([dst], count)."""
+    n = pred.numel()
+    dev = pred.device
+    cap = n if cap is None else max(int(cap), 1)
+    bounds = torch.tensor([lo, hi], dtype=pred.dtype, device=dev)
+    outs = [torch.empty((cap,) + tuple(s.shape[1:]), dtype=s.dtype, device=dev) for s in srcs]
+    cols = (L.Col * max(1, len(srcs)))()
+    for i, (s, d) in enumerate(zip(srcs, outs)):
+        cols[i] = L.Col(s.data_ptr(), None, d.data_ptr(), None, elem_bytes(s), 0)
+    cnt = torch.zeros(1, dtype=torch.int64, device=dev)
+    ws = workspace(ws_bytes("ysql_compact_ws_bytes", n), dev)
+    call("ysql_filter_compact_range", dtype_code(pred), _p(pred), _p(bounds), cols, len(srcs), n, cap, _p(cnt),
+         _p(ws), ws.numel(), _s())
+    c = int(cnt.item())
+    if c > cap:
+        return filter_compact_range(pred, lo, hi, srcs, cap=c)
+    return [d[:c] for d in outs], c
+
+
 # ---- A12 / A15 ---------------------------------------------------------
 def compare(cmp, lhs, rhs, lhs_scalar=False, rhs_scalar=False, negate=False, n=None):
     n = n if n is not None else (rhs.shape[0] if lhs_scalar else lhs.shape[0])
@@ -286,6 +308,23 @@ def filter_aggregate(pred, cmp, scalar, specs, overwrite=False, n=None):
     call("ysql_filter_aggregate_ex", _DT[pred.dtype], cmp, _p(pred), _p(sc), arr, len(specs), n,
          L.FA_OVERWRITE if overwrite else 0, _p(ws), ws.numel(), _s())
     return [acc for (_, _, _, acc) in specs]
+
+
+def filter_aggregate_multi(pred, cmp, scalars, columns, accs, overwrite=True, n=None):
+    """Shared scan: query q is WHERE pred <cmp> scalars[q] with aggregates
+    columns[s] = (dtype_c, agg, column or None) into accs[q][s]."""
+    n = pred.numel() if n is None else n
+    dev = pred.device
+    nq, ns = len(scalars), len(columns)
+    sc = torch.tensor(list(scalars), dtype=pred.dtype, device=dev)
+    arr = (L.AggSpec * (nq * ns))()
+    for q in range(nq):
+        for k, (dc, agg, col) in enumerate(columns):
+            arr[q * ns + k] = L.AggSpec(dc, agg, _p(col) if col is not None else None, _p(accs[q][k]))
+    ws = workspace(ws_bytes("ysql_filter_aggregate_multi_ws_bytes", n, nq, ns), dev)
+    call("ysql_filter_aggregate_multi", _DT[pred.dtype], cmp, _p(pred), _p(sc), nq, arr, ns, n,
+         L.FA_OVERWRITE if overwrite else 0, _p(ws), ws.numel(), _s())
+    return accs
 
 
 def agg_finalize(dtype_c, agg, acc):
