@@ -543,6 +543,28 @@ def test_join_pairs(ops, dev, dtype, cmp):
     assert np.array_equal(N(pairs).view(np.uint32), exp)
 
 
+@pytest.mark.parametrize("R_", [2000, 5000])
+@pytest.mark.parametrize("L_", [4095, 4096, 4097, 300_001])
+@pytest.mark.parametrize("dtype", [np.int32, np.float32, np.int64])
+def test_join_eq_hash_probe(ops, dev, L_, dtype, R_):
+    """EQ hash join at probe-tile boundaries, on an unaligned left view,
+    with duplicate right keys, and (float) NaN and signed zeros; R_ = 2000
+    probes a table staged in LDS, R_ = 5000 the table in global memory."""
+    r = rng(L_ + 5)
+    left = r.integers(0, 5000, L_ + 1).astype(dtype)
+    right = r.integers(0, 5000, R_).astype(dtype)
+    if dtype == np.float32:
+        left[::97] = np.nan
+        right[::53] = np.nan
+        left[1::89] = -0.0
+        right[5] = 0.0
+    lt = T(left, dev)[1:]  # unaligned device view of rows 1..L_
+    pairs, c = ops.join_pairs(4, lt, T(right, dev), cap=1024)
+    exp = O.join_pairs(4, left[1:], right)
+    assert c == exp.shape[0]
+    assert np.array_equal(N(pairs).view(np.uint32), exp)
+
+
 def test_join_overflow_reports_count(ops, dev):
     left = np.zeros(100, np.int32)
     right = np.zeros(50, np.int32)

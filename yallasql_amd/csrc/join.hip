@@ -16,6 +16,8 @@
 //     counted once and then written.
 #include <math.h>
 
+#include <algorithm>
+
 #include "common.hpp"
 
 namespace ysql {
@@ -105,7 +107,14 @@ __device__ __forceinline__ uint32_t take_tile_j(uint32_t *ctr, uint32_t *s_tile)
   return *s_tile;
 }
 
-// Probe: each thread owns 4 consecutive left rows of a 1024-row tile.
+// Probe: each thread owns kProbeRpt consecutive left rows of a tile.  The
+// keys come in with 128-bit loads; every row's first slot (its used flag
+// and its 16-byte entry) is loaded before any is examined, so the lookups
+// of a thread overlap; rows whose first slot is taken by another key walk
+// the probe sequence one slot at a time.
+constexpr int kProbeRpt = 16;
+constexpr uint64_t kProbeTile = (uint64_t)kJoinBlock * kProbeRpt;
+
 template <typename T>
 __global__ __launch_bounds__(kJoinBlock) void k_join_probe(const T *left, uint64_t L, const HashSlot *table,
                                                            const uint32_t *used, uint32_t mask, const uint32_t *perm,
@@ -115,26 +124,48 @@ __global__ __launch_bounds__(kJoinBlock) void k_join_probe(const T *left, uint64
   __shared__ uint32_t s_tile;
   __shared__ uint64_t s_prefix;
   const uint32_t tile = take_tile_j(ctr, &s_tile);
-  const uint64_t row0 = (uint64_t)tile * (kJoinBlock * 4) + threadIdx.x * 4;
-  uint32_t st[4], cn[4];
+  const uint64_t row0 = (uint64_t)tile * kProbeTile + (uint64_t)threadIdx.x * kProbeRpt;
+  T v[kProbeRpt];
+  if (row0 + kProbeRpt <= L && ((uintptr_t)(left + row0) & 15) == 0) {
+    constexpr int NQ = kProbeRpt * sizeof(T) / 16;
+    uint4 q[NQ];
+    const uint4 *src = reinterpret_cast<const uint4 *>(left + row0);
+#pragma unroll
+    for (int i = 0; i < NQ; i++) q[i] = src[i];
+    memcpy(&v[0], &q[0], sizeof(q));
+  } else {
+#pragma unroll
+    for (int e = 0; e < kProbeRpt; e++) v[e] = row0 + e < L ? left[row0 + e] : T();
+  }
+  uint32_t h[kProbeRpt], u[kProbeRpt];
+  HashSlot slot[kProbeRpt];
+#pragma unroll
+  for (int e = 0; e < kProbeRpt; e++) {
+    const bool live = row0 + e < L && JoinKey<T>::ok(v[e]);
+    h[e] = hash_u64((uint64_t)JoinKey<T>::bits(v[e])) & mask;
+    u[e] = live ? used[h[e]] : 0u;
+    slot[e] = live ? table[h[e]] : HashSlot{0, 0, 0};
+  }
+  uint32_t st[kProbeRpt], cn[kProbeRpt];
   uint64_t mine = 0;
 #pragma unroll
-  for (int e = 0; e < 4; e++) {
+  for (int e = 0; e < kProbeRpt; e++) {
     st[e] = 0;
     cn[e] = 0;
-    const uint64_t l = row0 + e;
-    if (l < L) {
-      const T v = left[l];
-      if (JoinKey<T>::ok(v)) {
-        const uint64_t k = (uint64_t)JoinKey<T>::bits(v);
-        uint32_t h = hash_u64(k) & mask;
-        while (used[h]) {
-          if (table[h].key == k) {
-            st[e] = table[h].start;
-            cn[e] = table[h].count;
+    const uint64_t k = (uint64_t)JoinKey<T>::bits(v[e]);
+    if (u[e]) {
+      if (slot[e].key == k) {
+        st[e] = slot[e].start;
+        cn[e] = slot[e].count;
+      } else {
+        uint32_t hh = (h[e] + 1) & mask;
+        while (used[hh]) {
+          if (table[hh].key == k) {
+            st[e] = table[hh].start;
+            cn[e] = table[hh].count;
             break;
           }
-          h = (h + 1) & mask;
+          hh = (hh + 1) & mask;
         }
       }
     }
@@ -149,7 +180,7 @@ __global__ __launch_bounds__(kJoinBlock) void k_join_probe(const T *left, uint64
   __syncthreads();
   uint64_t pos = s_prefix + excl;
 #pragma unroll
-  for (int e = 0; e < 4; e++) {
+  for (int e = 0; e < kProbeRpt; e++) {
     for (uint32_t q = 0; q < cn[e]; q++, pos++) {
       if (pos < cap) {
         pairs[2 * pos] = (uint32_t)(row0 + e);
@@ -158,6 +189,91 @@ __global__ __launch_bounds__(kJoinBlock) void k_join_probe(const T *left, uint64
     }
   }
   if (tile == ntiles - 1 && threadIdx.x == 0) *count_out = s_prefix + total;
+}
+
+// Same probe with the hash table staged in LDS, for tables of up to
+// kLdsSlots slots (right sides of up to kLdsSlots / 2 distinct keys).  The
+// workgroups are persistent: each copies the table into LDS once (a slot
+// with count 0 is empty) and then takes tiles in order from the counter
+// until none are left, so every look-back still waits only on tiles taken
+// earlier by running workgroups.
+constexpr uint32_t kLdsSlots = 4096;
+
+template <typename T>
+__global__ __launch_bounds__(kJoinBlock) void k_join_probe_lds(const T *left, uint64_t L, const HashSlot *table,
+                                                               const uint32_t *used, uint32_t mask,
+                                                               const uint32_t *perm, uint32_t *pairs, uint64_t cap,
+                                                               uint64_t *count_out, uint32_t *ctr, uint64_t *status,
+                                                               uint32_t ntiles) {
+  __shared__ uint64_t s_key[kLdsSlots];
+  __shared__ uint32_t s_start[kLdsSlots];
+  __shared__ uint32_t s_cnt[kLdsSlots];
+  __shared__ uint64_t s_red[kJoinBlock / kWave + 1];
+  __shared__ uint32_t s_tile;
+  __shared__ uint64_t s_prefix;
+  for (uint32_t i = threadIdx.x; i <= mask; i += kJoinBlock) {
+    const bool u = used[i] != 0;
+    const HashSlot e = u ? table[i] : HashSlot{0, 0, 0};
+    s_key[i] = e.key;
+    s_start[i] = e.start;
+    s_cnt[i] = u ? e.count : 0u;
+  }
+  while (true) {
+    const uint32_t tile = take_tile_j(ctr, &s_tile);  // (its barrier also publishes the table)
+    if (tile >= ntiles) return;
+    const uint64_t row0 = (uint64_t)tile * kProbeTile + (uint64_t)threadIdx.x * kProbeRpt;
+    T v[kProbeRpt];
+    if (row0 + kProbeRpt <= L && ((uintptr_t)(left + row0) & 15) == 0) {
+      constexpr int NQ = kProbeRpt * sizeof(T) / 16;
+      uint4 q[NQ];
+      const uint4 *src = reinterpret_cast<const uint4 *>(left + row0);
+#pragma unroll
+      for (int i = 0; i < NQ; i++) q[i] = src[i];
+      memcpy(&v[0], &q[0], sizeof(q));
+    } else {
+#pragma unroll
+      for (int e = 0; e < kProbeRpt; e++) v[e] = row0 + e < L ? left[row0 + e] : T();
+    }
+    uint32_t st[kProbeRpt], cn[kProbeRpt];
+    uint64_t mine = 0;
+#pragma unroll
+    for (int e = 0; e < kProbeRpt; e++) {
+      st[e] = 0;
+      cn[e] = 0;
+      if (row0 + e < L && JoinKey<T>::ok(v[e])) {
+        const uint64_t k = (uint64_t)JoinKey<T>::bits(v[e]);
+        uint32_t h = hash_u64(k) & mask;
+        while (s_cnt[h]) {
+          if (s_key[h] == k) {
+            st[e] = s_start[h];
+            cn[e] = s_cnt[h];
+            break;
+          }
+          h = (h + 1) & mask;
+        }
+      }
+      mine += cn[e];
+    }
+    uint64_t total;
+    const uint64_t excl = block_exclusive_sum<kJoinBlock, uint64_t>(mine, s_red, &total);
+    if (threadIdx.x < kWave) {
+      const uint64_t p = lookback_exclusive(status, tile, total);
+      if (threadIdx.x == 0) s_prefix = p;
+    }
+    __syncthreads();
+    uint64_t pos = s_prefix + excl;
+#pragma unroll
+    for (int e = 0; e < kProbeRpt; e++) {
+      for (uint32_t q = 0; q < cn[e]; q++, pos++) {
+        if (pos < cap) {
+          pairs[2 * pos] = (uint32_t)(row0 + e);
+          pairs[2 * pos + 1] = perm[st[e] + q];
+        }
+      }
+    }
+    if (tile == ntiles - 1 && threadIdx.x == 0) *count_out = s_prefix + total;
+    __syncthreads();  // s_prefix and s_red are reused by the next tile
+  }
 }
 
 // ---- nested loop (θ-joins, strings): one left row per thread, right rows
@@ -328,7 +444,7 @@ static bool is_hash_eq(int dtype, int cmp) {
 
 static size_t join_ws_layout(int dtype, int cmp, uint64_t L, uint64_t R, void *ws, size_t bytes, JoinWs *out) {
   const bool hash = is_hash_eq(dtype, cmp);
-  const uint64_t tile_rows = hash ? kJoinBlock * 4 : kJoinBlock;
+  const uint64_t tile_rows = hash ? kProbeTile : kJoinBlock;
   const uint64_t ntiles = ceil_div(L, tile_rows) + 1;
   const int kdt = dtype == YSQL_DATETIME ? YSQL_UKEY64 : YSQL_UKEY32;
   size_t sort_bytes = 0;
@@ -369,7 +485,7 @@ static int join_hash(const T *left, uint64_t L, const T *right, uint64_t R, uint
                      uint64_t *count_out, const JoinWs &w, hipStream_t s) {
   typedef typename JoinKey<T>::U U;
   const uint32_t tcap = hash_cap(R);
-  const uint32_t ntiles = (uint32_t)ceil_div(L, kJoinBlock * 4);
+  const uint32_t ntiles = (uint32_t)ceil_div(L, kProbeTile);
   k_join_reset<<<256, 256, 0, s>>>(w.ctr, w.status, ntiles + 1, w.used, tcap);
   YSQL_LAUNCH_CHECK();
   uint64_t g = ceil_div(R, 256);
@@ -384,8 +500,20 @@ static int join_hash(const T *left, uint64_t L, const T *right, uint64_t R, uint
     k_join_build<U><<<(unsigned)g, 256, 0, s>>>((const U *)w.keys, R, w.ctr + 1, w.table, w.used, tcap - 1);
     YSQL_LAUNCH_CHECK();
   }
-  k_join_probe<T><<<ntiles, kJoinBlock, 0, s>>>(left, L, w.table, w.used, tcap - 1, w.perm, pairs, cap, count_out,
-                                                w.ctr, w.status, ntiles);
+  if (tcap <= kLdsSlots) {
+    static const uint32_t per_cu = [] {
+      int b = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_join_probe_lds<T>, kJoinBlock, 0) != hipSuccess || b < 1)
+        b = 1;
+      return (uint32_t)b;
+    }();
+    const uint32_t g = std::min<uint32_t>(ntiles, per_cu * (uint32_t)grid_cap());
+    k_join_probe_lds<T><<<std::max<uint32_t>(g, 1), kJoinBlock, 0, s>>>(left, L, w.table, w.used, tcap - 1, w.perm,
+                                                                        pairs, cap, count_out, w.ctr, w.status, ntiles);
+  } else {
+    k_join_probe<T><<<ntiles, kJoinBlock, 0, s>>>(left, L, w.table, w.used, tcap - 1, w.perm, pairs, cap, count_out,
+                                                  w.ctr, w.status, ntiles);
+  }
   YSQL_LAUNCH_CHECK();
   return YSQL_OK;
 }
