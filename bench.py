@@ -41,6 +41,10 @@ def parse():
     ap.add_argument("--no-verify", action="store_true")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run one plain 1.6 GB device copy first (calibrates HBM byte counters in a profile)")
+    ap.add_argument("--no-ops", action="store_true", help="skip the per-op roofline lines (filter, filter+aggregate)")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 4,096-job batched vs plain comparison")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving loop")
+    ap.add_argument("--serving-seconds", type=float, default=2.5, help="arrival window per rate")
     return ap.parse_args()
 
 
@@ -183,6 +187,148 @@ def cpu_baseline(keys_host, run_rows):
     }
 
 
+def _timed_ms(fn, reps=10):
+    """Median of `reps` runs of fn() on the current stream, HIP events."""
+    import torch
+
+    ts = []
+    for _ in range(reps):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        fn()
+        b.record()
+        b.synchronize()
+        ts.append(a.elapsed_time(b))
+    ts.sort()
+    return ts[len(ts) // 2]
+
+
+def op_lines(keys):
+    """Filter (compaction) and fused filter+aggregate over the 200M-row table:
+    rows/s and fraction of HBM peak by their algorithmic bytes."""
+    import torch
+
+    from yallasql_amd import _lib as L
+    from yallasql_amd import ops
+
+    n = keys.numel()
+    dev = keys.device
+    g = torch.Generator(device=dev)
+    g.manual_seed(SEED + 7)
+    vals = torch.rand(n, generator=g, device=dev)
+    out = {}
+    # WHERE vals < 0.5: COUNT, SUM(vals) (double), MIN(key), MAX(key) -- reads 12 B/row
+    cols = [(L.INT, L.AGG_COUNT_STAR, None), (L.FLOAT, L.AGG_SUM, vals), (L.DATETIME, L.AGG_MIN, keys),
+            (L.DATETIME, L.AGG_MAX, keys)]
+    accs = [[torch.empty(32, dtype=torch.uint8, device=dev) for _ in cols]]
+    run = lambda: ops.filter_aggregate_multi(vals, L.LT, [0.5], cols, accs)  # noqa: E731
+    run()
+    ms = _timed_ms(run)
+    b = n * 12
+    out["filter_aggregate"] = {"rows_per_s": n / (ms * 1e-3), "ms": ms, "algorithmic_GB": b / 1e9,
+                               "GB_s": b / (ms * 1e-3) / 1e9, "frac": b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                               "what": "WHERE f32 < c -> COUNT, SUM(f32), MIN/MAX(int64) over 200M rows"}
+    # four queries sharing one scan (the serving loop's batching)
+    accs4 = [[torch.empty(32, dtype=torch.uint8, device=dev) for _ in cols] for _ in range(4)]
+    run4 = lambda: ops.filter_aggregate_multi(vals, L.LT, [0.2, 0.4, 0.6, 0.8], cols, accs4)  # noqa: E731
+    run4()
+    ms4 = _timed_ms(run4)
+    out["filter_aggregate_shared_x4"] = {"queries_rows_per_s": 4 * n / (ms4 * 1e-3), "ms": ms4,
+                                         "GB_s": b / (ms4 * 1e-3) / 1e9, "frac": b / (ms4 * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    # WHERE vals < 0.5 compacting (key, vals): N*(4+8) read + M*(8+4) written
+    cnt = torch.zeros(1, dtype=torch.int64, device=dev)
+    kout, vout = torch.empty_like(keys), torch.empty_like(vals)
+    col = (L.Col * 2)()
+    col[0] = L.Col(keys.data_ptr(), None, kout.data_ptr(), None, 8, 0)
+    col[1] = L.Col(vals.data_ptr(), None, vout.data_ptr(), None, 4, 0)
+    sc = torch.tensor([0.5], device=dev)
+    ws = ops.workspace(L.ws_bytes("ysql_compact_ws_bytes", n), dev)
+    runf = lambda: L.call("ysql_filter_compact", L.FLOAT, L.LT, ops._p(vals), ops._p(sc), col, 2, n,  # noqa: E731
+                          ops._p(cnt), ops._p(ws), ws.numel(), ops._s())
+    runf()
+    msf = _timed_ms(runf)
+    m = int(cnt.item())
+    bf = n * 12 + m * 12
+    out["filter_compact"] = {"rows_per_s": n / (msf * 1e-3), "ms": msf, "selected": m,
+                             "GB_s": bf / (msf * 1e-3) / 1e9, "frac": bf / (msf * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                             "what": "WHERE f32 < 0.5 compacting (int64 key, f32) of 200M rows"}
+    del vals, kout, vout, ws
+    return out
+
+
+def batched_lines():
+    """4,096 ragged sort/filter/aggregate jobs (log-uniform 1,024..4,194,304
+    rows): the per-job launch loop against the captured graph replay."""
+    import numpy as np
+    import torch
+
+    from yallasql_amd.batched import JobSet, config4_jobs
+
+    dev = torch.device("cuda:0")
+    sizes, kinds, key64, thr = config4_jobs()
+    js = JobSet(sizes, kinds, key64, thr, dev, seed=4).create(chains=32)
+    js.run_batched()
+    js.run_plain()
+    torch.cuda.synchronize()
+
+    def wall(fn, reps=3):
+        best = None
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            best = dt if best is None else min(best, dt)
+        return best
+
+    t_plain = wall(js.run_plain)
+    t_graph = wall(js.run_batched)
+    js.clear_outputs()
+    js.run_plain()
+    a = js.snapshot()
+    js.clear_outputs()
+    js.run_batched()
+    b = js.snapshot()
+    same = all(np.array_equal(x, y) for x, y in zip(a, b))
+    alg = js.algorithmic_bytes()
+    out = {"jobs": len(sizes), "rows": js.total_rows, "plain_s": t_plain, "batched_s": t_graph,
+           "speedup": t_plain / t_graph, "identical_outputs": bool(same),
+           "batched_GB_s": alg / t_graph / 1e9, "batched_frac": alg / t_graph / 1e9 / HBM_PEAK_GBS,
+           "chains": js.info()["chains"],
+           "what": "plain: per-job launches on one stream; batched: one hipGraph replay of 32 launch chains"}
+    js.close()
+    del js
+    torch.cuda.empty_cache()
+    return out
+
+
+def serving_lines(seconds):
+    """Serving loop over the 200M-row table at 50/200/800 queries/s."""
+    import torch
+
+    from yallasql_amd.serving import Server, Table, latency_report, make_queries
+
+    dev = torch.device("cuda:0")
+    table = Table(N_ROWS, dev, seed=SEED)
+    srv = Server(table, slots=8, hbm_budget=200 << 30)
+    out = {}
+    try:
+        srv.serve(make_queries(table, 24, rate_qps=None, seed=1), realtime=False)  # warm-up
+        for rate in (50, 200, 800):
+            qs = make_queries(table, int(rate * seconds), rate_qps=rate, seed=rate)
+            wall = srv.serve(qs)
+            out[f"{rate}qps"] = latency_report(qs, wall, srv)
+    finally:
+        srv.close()
+    out["what"] = ("Poisson arrivals; 50% filter+aggregate, 30% ORDER BY key LIMIT 1000 above a random key, "
+                   "20% join with a 2,000-row dimension + GROUP BY (64 groups); 8 slots, 200 GB HBM budget; "
+                   "latency = arrival to result on the host")
+    del table
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -244,6 +390,14 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(keys[: 2 * RUN_ROWS].cpu().numpy().view("uint64"), RUN_ROWS)
 
+    extra_ops = None if args.no_ops else op_lines(keys)
+    nruns = len(job.runs)
+    del job
+    del keys
+    torch.cuda.empty_cache()
+    batched = None if args.no_batched else batched_lines()
+    serving = None if args.no_serving else serving_lines(args.serving_seconds)
+
     out = {
         "metric": "rows/s & %HBM roofline per op; batched vs plain speedup; p99 query latency",
         "value": rows_per_s,
@@ -262,7 +416,7 @@ def main():
                         "radix-sorted with 8-bit digits then merge-path merged pairwise",
             "rows": args.rows,
             "run_rows": RUN_ROWS,
-            "runs": len(job.runs),
+            "runs": nruns,
         },
         "roofline": {
             "bound": "hbm",
@@ -284,7 +438,10 @@ def main():
                           "frac": alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
             "device_copy": {"GB_s": copy_gbs, "frac": copy_gbs / HBM_PEAK_GBS,
                             "what": "torch copy_ of 1.6 GB (read + write), best of 3: attainable HBM rate"},
+            **(extra_ops or {}),
         },
+        "batched": batched,
+        "serving": serving,
         "cpu_baseline": cpu,
         "verified": not args.no_verify,
     }

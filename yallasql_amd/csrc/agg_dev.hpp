@@ -156,41 +156,43 @@ __device__ __forceinline__ void fa_load_words(const FaSpec &s, uint64_t r0, uint
 // Folds the selected rows (bit e of `flags` = row r0 + e) of one spec, in
 // row order, from its loaded words.  Unselected rows fold the operator's
 // identity (0 for SUM, the current value for MIN/MAX), which leaves the
-// accumulator bit-for-bit as if they were skipped.
+// accumulator bit-for-bit as if they were skipped.  One short body per
+// accumulator kind keeps the kernel small enough for the instruction cache.
 __device__ __forceinline__ void fa_fold_words(ysql_acc &a, const FaSpec &s, const uint32_t (&w)[2 * kFaRpt],
                                               uint32_t flags) {
   a.count += __builtin_popcount(flags);
-  if (s.kind == FA_KC) return;
   if (s.kind == FA_KI) {
     const bool wide = s.dtype != YSQL_INT;
+    int64_t x[kFaRpt];
+#pragma unroll
+    for (int e = 0; e < kFaRpt; e++)
+      x[e] = wide ? (int64_t)((uint64_t)w[2 * e] | ((uint64_t)w[2 * e + 1] << 32)) : (int64_t)(int32_t)w[e];
     if (s.op == 0) {
       uint64_t sum = (uint64_t)a.i;
 #pragma unroll
-      for (int e = 0; e < kFaRpt; e++) {
-        const int64_t x = wide ? (int64_t)((uint64_t)w[2 * e] | ((uint64_t)w[2 * e + 1] << 32)) : (int64_t)(int32_t)w[e];
-        sum += ((flags >> e) & 1u) ? (uint64_t)x : 0ull;
-      }
+      for (int e = 0; e < kFaRpt; e++) sum += ((flags >> e) & 1u) ? (uint64_t)x[e] : 0ull;
       a.i = (int64_t)sum;
     } else {
+      const bool mn = s.op == 1;
       int64_t m = a.i;
 #pragma unroll
       for (int e = 0; e < kFaRpt; e++) {
-        const int64_t x = wide ? (int64_t)((uint64_t)w[2 * e] | ((uint64_t)w[2 * e + 1] << 32)) : (int64_t)(int32_t)w[e];
-        const bool take = ((flags >> e) & 1u) && (s.op == 1 ? x < m : x > m);
-        m = take ? x : m;
+        const bool better = mn ? x[e] < m : x[e] > m;
+        m = (((flags >> e) & 1u) && better) ? x[e] : m;
       }
       a.i = m;
     }
   } else if (s.kind == FA_KF) {
+    const bool mn = s.op == 1;
     float m = a.f;
 #pragma unroll
     for (int e = 0; e < kFaRpt; e++) {
       const float x = __uint_as_float(w[e]);
-      const float r = s.op == 1 ? fminf(m, x) : fmaxf(m, x);
+      const float r = mn ? fminf(m, x) : fmaxf(m, x);
       m = ((flags >> e) & 1u) ? r : m;
     }
     a.f = m;
-  } else {
+  } else if (s.kind == FA_KD) {
     double d = a.d;
 #pragma unroll
     for (int e = 0; e < kFaRpt; e++) d += ((flags >> e) & 1u) ? (double)__uint_as_float(w[e]) : 0.0;
@@ -230,6 +232,27 @@ __device__ __forceinline__ void filter_agg_tile(uint32_t tile, const PT *pred, c
 #pragma unroll
     for (int s = 0; s < kFaMaxSpecs; s++)
       if (s < ns && sp[s].kind != FA_KC && sp[s].src == FA_OWN) fa_load_words(sp[s], r0, end, w[s]);
+    // specs over the predicate column or an earlier spec's column take a
+    // copy of those words (register moves, no second load)
+    {
+      uint32_t pw[2 * kFaRpt];
+      memcpy(pw, pv, sizeof(pv));
+#pragma unroll
+      for (int s = 0; s < kFaMaxSpecs; s++) {
+        if (s >= ns) continue;
+        const int src = sp[s].src;
+        if (src == FA_PRED) {
+#pragma unroll
+          for (int i = 0; i < 2 * kFaRpt; i++) w[s][i] = i < (int)(sizeof(pv) / 4) ? pw[i] : 0u;
+        }
+#pragma unroll
+        for (int t = 0; t < s; t++)
+          if (src == FA_SPEC0 + t) {
+#pragma unroll
+            for (int i = 0; i < 2 * kFaRpt; i++) w[s][i] = w[t][i];
+          }
+      }
+    }
 #pragma unroll
     for (int q = 0; q < QMAX; q++) {
       if (q >= nq) break;
@@ -237,22 +260,8 @@ __device__ __forceinline__ void filter_agg_tile(uint32_t tile, const PT *pred, c
 #pragma unroll
       for (int e = 0; e < kFaRpt; e++) flags |= (uint32_t)(r0 + e < end && cmp_static<CMP>(pv[e], scalar[q])) << e;
 #pragma unroll
-      for (int s = 0; s < kFaMaxSpecs; s++) {
-        if (s < ns) {
-          const int src = sp[s].src;
-          if (src == FA_PRED) {
-            uint32_t pw[2 * kFaRpt];
-            memcpy(pw, pv, sizeof(pv));
-            fa_fold_words(acc[q][s], sp[s], pw, flags);
-          } else if (src == FA_OWN || sp[s].kind == FA_KC) {
-            fa_fold_words(acc[q][s], sp[s], w[s], flags);
-          } else {  // the words an earlier spec loaded (static indices only)
-#pragma unroll
-            for (int t = 0; t < s; t++)
-              if (src == FA_SPEC0 + t) fa_fold_words(acc[q][s], sp[s], w[t], flags);
-          }
-        }
-      }
+      for (int s = 0; s < kFaMaxSpecs; s++)
+        if (s < ns) fa_fold_words(acc[q][s], sp[s], w[s], flags);
     }
   }
 #pragma unroll
