@@ -153,14 +153,35 @@ __device__ __forceinline__ void fa_load_words(const FaSpec &s, uint64_t r0, uint
   }
 }
 
+// Per-thread running value of one spec, one 64-bit word: the int64 (KI),
+// the double's bits (KD) or the float's bits (KF).  The row count is kept
+// once per query: without null masks every spec of a query folds the same
+// rows.
+__device__ __forceinline__ uint64_t fa_val_identity(const FaSpec &s) {
+  ysql_acc a;
+  fa_identity(a, s);
+  if (s.kind == FA_KI) return (uint64_t)a.i;
+  if (s.kind == FA_KF) return (uint64_t)__float_as_uint(a.f);
+  return (uint64_t)__double_as_longlong(a.d);
+}
+
+__device__ __forceinline__ ysql_acc fa_val_to_acc(uint64_t v, uint32_t count, const FaSpec &s) {
+  ysql_acc a;
+  fa_identity(a, s);
+  if (s.kind == FA_KI) a.i = (int64_t)v;
+  else if (s.kind == FA_KF) a.f = __uint_as_float((uint32_t)v);
+  else if (s.kind == FA_KD) a.d = __longlong_as_double((long long)v);
+  a.count = count;
+  return a;
+}
+
 // Folds the selected rows (bit e of `flags` = row r0 + e) of one spec, in
 // row order, from its loaded words.  Unselected rows fold the operator's
 // identity (0 for SUM, the current value for MIN/MAX), which leaves the
-// accumulator bit-for-bit as if they were skipped.  One short body per
+// value bit-for-bit as if they were skipped.  One short body per
 // accumulator kind keeps the kernel small enough for the instruction cache.
-__device__ __forceinline__ void fa_fold_words(ysql_acc &a, const FaSpec &s, const uint32_t (&w)[2 * kFaRpt],
+__device__ __forceinline__ void fa_fold_words(uint64_t &val, const FaSpec &s, const uint32_t (&w)[2 * kFaRpt],
                                               uint32_t flags) {
-  a.count += __builtin_popcount(flags);
   if (s.kind == FA_KI) {
     const bool wide = s.dtype != YSQL_INT;
     int64_t x[kFaRpt];
@@ -168,35 +189,35 @@ __device__ __forceinline__ void fa_fold_words(ysql_acc &a, const FaSpec &s, cons
     for (int e = 0; e < kFaRpt; e++)
       x[e] = wide ? (int64_t)((uint64_t)w[2 * e] | ((uint64_t)w[2 * e + 1] << 32)) : (int64_t)(int32_t)w[e];
     if (s.op == 0) {
-      uint64_t sum = (uint64_t)a.i;
+      uint64_t sum = val;
 #pragma unroll
       for (int e = 0; e < kFaRpt; e++) sum += ((flags >> e) & 1u) ? (uint64_t)x[e] : 0ull;
-      a.i = (int64_t)sum;
+      val = sum;
     } else {
       const bool mn = s.op == 1;
-      int64_t m = a.i;
+      int64_t m = (int64_t)val;
 #pragma unroll
       for (int e = 0; e < kFaRpt; e++) {
         const bool better = mn ? x[e] < m : x[e] > m;
         m = (((flags >> e) & 1u) && better) ? x[e] : m;
       }
-      a.i = m;
+      val = (uint64_t)m;
     }
   } else if (s.kind == FA_KF) {
     const bool mn = s.op == 1;
-    float m = a.f;
+    float m = __uint_as_float((uint32_t)val);
 #pragma unroll
     for (int e = 0; e < kFaRpt; e++) {
       const float x = __uint_as_float(w[e]);
       const float r = mn ? fminf(m, x) : fmaxf(m, x);
       m = ((flags >> e) & 1u) ? r : m;
     }
-    a.f = m;
+    val = (uint64_t)__float_as_uint(m);
   } else if (s.kind == FA_KD) {
-    double d = a.d;
+    double d = __longlong_as_double((long long)val);
 #pragma unroll
     for (int e = 0; e < kFaRpt; e++) d += ((flags >> e) & 1u) ? (double)__uint_as_float(w[e]) : 0.0;
-    a.d = d;
+    val = (uint64_t)__double_as_longlong(d);
   }
 }
 
@@ -215,16 +236,19 @@ __device__ __forceinline__ void filter_agg_tile(uint32_t tile, const PT *pred, c
                                                 ysql_acc *s_w) {
   const uint64_t base = (uint64_t)tile * kFaTile;
   const uint64_t end = base + kFaTile < n ? base + kFaTile : n;
-  ysql_acc acc[QMAX][kFaMaxSpecs];
+  uint64_t val[QMAX][kFaMaxSpecs];
+  uint32_t cnt[QMAX];
   FaSpec sp[kFaMaxSpecs];
 #pragma unroll
   for (int s = 0; s < kFaMaxSpecs; s++) {
     if (s < ns) {
       sp[s] = specs[s];
 #pragma unroll
-      for (int q = 0; q < QMAX; q++) fa_identity(acc[q][s], sp[s]);
+      for (int q = 0; q < QMAX; q++) val[q][s] = fa_val_identity(sp[s]);
     }
   }
+#pragma unroll
+  for (int q = 0; q < QMAX; q++) cnt[q] = 0;
   for (uint64_t r0 = base + (uint64_t)threadIdx.x * kFaRpt; r0 < end; r0 += (uint64_t)BLOCK * kFaRpt) {
     PT pv[kFaRpt];
     fa_load_rows<PT>(pred, r0, end, pv);
@@ -259,9 +283,10 @@ __device__ __forceinline__ void filter_agg_tile(uint32_t tile, const PT *pred, c
       uint32_t flags = 0;
 #pragma unroll
       for (int e = 0; e < kFaRpt; e++) flags |= (uint32_t)(r0 + e < end && cmp_static<CMP>(pv[e], scalar[q])) << e;
+      cnt[q] += __builtin_popcount(flags);
 #pragma unroll
       for (int s = 0; s < kFaMaxSpecs; s++)
-        if (s < ns) fa_fold_words(acc[q][s], sp[s], w[s], flags);
+        if (s < ns) fa_fold_words(val[q][s], sp[s], w[s], flags);
     }
   }
 #pragma unroll
@@ -270,7 +295,7 @@ __device__ __forceinline__ void filter_agg_tile(uint32_t tile, const PT *pred, c
 #pragma unroll
     for (int s = 0; s < kFaMaxSpecs; s++) {
       if (s < ns) {
-        const ysql_acc tot = fa_block_reduce<BLOCK>(acc[q][s], sp[s], s_w);
+        const ysql_acc tot = fa_block_reduce<BLOCK>(fa_val_to_acc(val[q][s], cnt[q], sp[s]), sp[s], s_w);
         if (threadIdx.x == 0) partials[((uint64_t)tile * nq + q) * ns + s] = tot;
       }
     }
