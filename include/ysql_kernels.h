@@ -154,6 +154,17 @@ int ysql_radix_sort_pairs(int key_dtype, int descending, void *keys_inout, uint3
 int ysql_radix_sort_ex(int key_dtype, int descending, const void *keys_in, const uint32_t *vals_in,
                        uint32_t val_base, void *keys_out, uint32_t *vals_out, uint64_t n,
                        int begin_bit, int end_bit, void *ws, size_t ws_bytes, ysql_stream_t stream);
+/* Same sort with an explicit tile shape of the digit passes (variant 0:
+ * 512 threads x 16 keys, 1: 256 x 16, 2: 256 x 8; -1 = the default for n,
+ * which ysql_radix_sort_ex uses).  Every variant returns the same stable
+ * permutation; they differ in speed by size, which an autotuner measures
+ * per size bucket.  Sorts of n <= 8192 keys run in one workgroup whatever
+ * the variant.                                                            */
+int ysql_radix_variant_count(void);
+int ysql_radix_sort_variant(int key_dtype, int variant, int descending, const void *keys_in,
+                            const uint32_t *vals_in, uint32_t val_base, void *keys_out,
+                            uint32_t *vals_out, uint64_t n, int begin_bit, int end_bit, void *ws,
+                            size_t ws_bytes, ysql_stream_t stream);
 /* Stepwise form of the same sort, for callers that schedule or time the
  * passes themselves (the reference's launcher loops over digit passes,
  * radix_sort_kernel.cu:161-167): ysql_radix_begin builds the digit
@@ -299,7 +310,8 @@ int ysql_agg_finalize(int dtype, int agg, const ysql_acc *acc, void *out, ysql_s
  *     for rows with pred_col[i] <cmp> scalar: fold specs[s].col[i] into
  *     *specs[s].acc under specs[s].agg   (s < nspecs <= 4).
  * COUNT and COUNT_STAR count the qualifying rows (this form takes no null
- * masks).  Deterministic: canonical 32768-row tiles folded in a fixed order,
+ * masks).  Deterministic: canonical tiles (4096..32768 rows, a function of n)
+ * folded in a fixed order,
  * so the float SUM is a function of the input alone (the same bits from
  * every launch shape and from the batched executor's aggregate jobs).      */
 typedef struct ysql_agg_spec {
@@ -438,10 +450,12 @@ typedef struct ysql_job {
 typedef struct ysql_batch ysql_batch;
 /* Tuning of the batched executor (ysql_batch_tuning_default fills it). */
 typedef struct ysql_batch_tuning {
-  int small_sort_max;   /* reserved (0) */
-  int hist_chunk;       /* reserved */
-  int workers_per_cu;   /* reserved */
-  int chains;           /* independent launch chains in the graph (1..64) */
+  int chains;      /* independent launch chains in the graph (1..64) */
+  int reserved[3]; /* 0 */
+  /* digit-pass tile variant of a sort job (see ysql_radix_sort_variant) by
+   * key width (0: UKEY32, 1: UKEY64) and size bucket floor(log2(n));
+   * -1 = the library default.  Filled by an autotuner. */
+  signed char sort_variant[2][32];
 } ysql_batch_tuning;
 void ysql_batch_tuning_default(ysql_batch_tuning *t);
 /* Validates the jobs, allocates every workspace and captures the graph.

@@ -59,7 +59,7 @@ def test_sortkey_encode(ops, dev, dtype, desc):
 
 
 # ---------------------------------------------------------------- A2-A6
-@pytest.mark.parametrize("n", SIZES + [1 << 20])
+@pytest.mark.parametrize("n", SIZES + [8191, 8192, 8193, 1 << 20])
 @pytest.mark.parametrize("bits", [32, 64])
 @pytest.mark.parametrize("desc", [False, True])
 def test_radix_argsort(ops, dev, n, bits, desc):
@@ -80,9 +80,32 @@ def test_radix_argsort(ops, dev, n, bits, desc):
     assert np.array_equal(N(kt).view(keys.dtype), keys[exp])
 
 
-def test_radix_sort_pairs_bits(ops, dev):
+@pytest.mark.parametrize("n", [8193, 100_003, (1 << 20) + 7])
+@pytest.mark.parametrize("bits", [32, 64])
+def test_radix_sort_variants_same_permutation(ops, dev, n, bits):
+    """Every digit-pass tile shape returns the stable permutation."""
+    from yallasql_amd import _lib as L
+
+    r = rng(n + bits + 1)
+    keys = r.integers(0, 2**bits - 1, n, dtype=np.uint64)
+    keys[: n // 4] = keys[7]  # ties
+    kin = T((keys.astype(np.uint32).view(np.int32) if bits == 32 else keys.view(np.int64)), dev)
+    kdt = L.UKEY32 if bits == 32 else L.UKEY64
+    exp = O.stable_argsort(keys.astype(np.uint32) if bits == 32 else keys)
+    ws = ops.workspace(L.ws_bytes("ysql_radix_sort_ws_bytes", kdt, n), dev)
+    for v in range(L.lib.ysql_radix_variant_count()):
+        kout, vout = torch.empty_like(kin), torch.empty(n, dtype=torch.int32, device=dev)
+        L.call("ysql_radix_sort_variant", kdt, v, 0, ops._p(kin), None, 0, ops._p(kout), ops._p(vout), n, 0, bits,
+               ops._p(ws), ws.numel(), ops._s())
+        assert np.array_equal(N(vout).view(np.uint32), exp), v
+    with pytest.raises(L.YsqlError):
+        L.call("ysql_radix_sort_variant", kdt, 99, 0, ops._p(kin), None, 0, ops._p(kout), ops._p(vout), n, 0, bits,
+               ops._p(ws), ws.numel(), ops._s())
+
+
+@pytest.mark.parametrize("n", [5_000, 70_001])  # one-workgroup sort, multi-tile sort
+def test_radix_sort_pairs_bits(ops, dev, n):
     r = rng(7)
-    n = 70_001
     keys = r.integers(0, 2**20, n, dtype=np.uint64).astype(np.uint32)
     vals = r.integers(0, 2**31, n, dtype=np.int64).astype(np.int32)
     kt, vt = T(keys.view(np.int32), dev), T(vals, dev)
@@ -93,10 +116,10 @@ def test_radix_sort_pairs_bits(ops, dev):
     assert np.array_equal(N(kt).view(np.uint32), keys[exp])
 
 
+@pytest.mark.parametrize("n", [8_000, 123_457])
 @pytest.mark.parametrize("end_bit", [64, 63, 24, 20])
-def test_radix_sort_ex_out_of_place(ops, dev, end_bit):
+def test_radix_sort_ex_out_of_place(ops, dev, end_bit, n):
     r = rng(end_bit)
-    n = 123_457
     keys = r.integers(0, 2**min(end_bit, 63), n, dtype=np.uint64)
     kin = T(keys.view(np.int64), dev)
     kout = torch.empty_like(kin)

@@ -63,6 +63,7 @@ _SIGS = {
     "ysql_radix_argsort": [I32, I32, P, P, U64, I32, I32, P, C.c_size_t, P],
     "ysql_radix_sort_pairs": [I32, I32, P, P, U64, I32, I32, P, C.c_size_t, P],
     "ysql_radix_sort_ex": [I32, I32, P, P, U32, P, P, U64, I32, I32, P, C.c_size_t, P],
+    "ysql_radix_sort_variant": [I32, I32, I32, P, P, U32, P, P, U64, I32, I32, P, C.c_size_t, P],
     "ysql_radix_begin": [I32, I32, P, U64, I32, I32, P, C.c_size_t, P],
     "ysql_radix_pass": [I32, I32, I32, P, P, U32, P, P, U64, I32, I32, P, C.c_size_t, P],
     "ysql_scan_ws_bytes": [U64, SZP],
@@ -147,8 +148,7 @@ FA_OVERWRITE = 1
 
 
 class BatchTuning(C.Structure):
-    _fields_ = [("small_sort_max", C.c_int), ("hist_chunk", C.c_int), ("workers_per_cu", C.c_int),
-                ("chains", C.c_int)]
+    _fields_ = [("chains", C.c_int), ("reserved", C.c_int * 3), ("sort_variant", (C.c_int8 * 32) * 2)]
 
 
 JOB_SORT, JOB_FILTER, JOB_AGG = 0, 1, 2

@@ -78,12 +78,19 @@ class JobSet:
         self.handle = None
         self.total_rows = sum(self.sizes)
 
-    def create(self, chains=None):
+    def create(self, chains=None, sort_variants=None):
+        """sort_variants: {32: [variant per log2 bucket], 64: [...]} (an
+        autotuner's table); None keeps the library defaults."""
         tuning = None
-        if chains is not None:
+        if chains is not None or sort_variants is not None:
             tuning = L.BatchTuning()
             L.lib.ysql_batch_tuning_default(C.byref(tuning))
-            tuning.chains = int(chains)
+            if chains is not None:
+                tuning.chains = int(chains)
+            for w, table in (sort_variants or {}).items():
+                row = tuning.sort_variant[0 if int(w) == 32 else 1]
+                for b, v in enumerate(table[:32]):
+                    row[b] = int(v)
             tuning = C.byref(tuning)
         h = C.c_void_p()
         L.call("ysql_batch_create", self.jobs, len(self.sizes), tuning, C.byref(h))

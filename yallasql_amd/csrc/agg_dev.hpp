@@ -1,9 +1,8 @@
-// agg_dev.hpp — filtered multi-aggregate over canonical tiles, shared by
-// the per-call kernel (reduce.hip, ysql_filter_aggregate) and the batched
-// executor.
+// agg_dev.hpp — filtered multi-aggregate over canonical tiles (the
+// kernels behind ysql_filter_aggregate*, reduce.hip).
 //
 // SELECT agg_1(c_1), ..., agg_k(c_k) WHERE pred <cmp> scalar in one read of
-// the columns.  The input is cut into canonical tiles of 32768 rows; a
+// the columns.  The input is cut into canonical tiles (fa_tile_rows(n) rows); a
 // tile's partial is folded in a fixed order (thread t folds its runs of
 // kFaRpt rows in row order, then a fixed wave64 butterfly, then the 4 waves
 // in order) and the partials are combined in a fixed order, so float sums are a
@@ -15,7 +14,16 @@
 
 namespace ysql {
 
-constexpr uint64_t kFaTile = 32768;
+// Canonical tile rows for an input of n rows: a power of two in
+// [4096, 32768] aiming at about 1024 tiles, so a small input still spreads
+// over many workgroups and a large one amortises each tile's reductions.
+// It depends on n alone, so the fold order (and a float SUM's bits) is a
+// function of the input.
+__host__ __device__ inline uint64_t fa_tile_rows(uint64_t n) {
+  uint64_t t = 4096;
+  while (t < 32768 && t * 1024 < n) t <<= 1;
+  return t;
+}
 constexpr int kFaBlock = 256;
 constexpr int kFaMaxSpecs = 4;
 constexpr int kFaMaxQueries = 4;  // queries sharing one scan
@@ -231,11 +239,11 @@ __device__ __forceinline__ void fa_fold_words(uint64_t &val, const FaSpec &s, co
 // on the other queries of the launch, so its result is the same alone or
 // batched.
 template <typename PT, int CMP, int BLOCK, int QMAX>
-__device__ __forceinline__ void filter_agg_tile(uint32_t tile, const PT *pred, const PT (&scalar)[QMAX], int nq,
-                                                const FaSpec *specs, int ns, uint64_t n, ysql_acc *partials,
-                                                ysql_acc *s_w) {
-  const uint64_t base = (uint64_t)tile * kFaTile;
-  const uint64_t end = base + kFaTile < n ? base + kFaTile : n;
+__device__ __forceinline__ void filter_agg_tile(uint32_t tile, uint64_t tile_rows, const PT *pred,
+                                                const PT (&scalar)[QMAX], int nq, const FaSpec *specs, int ns,
+                                                uint64_t n, ysql_acc *partials, ysql_acc *s_w) {
+  const uint64_t base = (uint64_t)tile * tile_rows;
+  const uint64_t end = base + tile_rows < n ? base + tile_rows : n;
   uint64_t val[QMAX][kFaMaxSpecs];
   uint32_t cnt[QMAX];
   FaSpec sp[kFaMaxSpecs];

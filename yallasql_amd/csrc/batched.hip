@@ -33,7 +33,8 @@ static double job_cost(const ysql_job &x) {
   switch (x.kind) {
     case YSQL_JOB_SORT: {
       const double passes = kb;  // 8-bit digits
-      return n * kb + passes * 2.0 * n * (kb + 4.0) + (passes + 2.0) * kLaunch;
+      const double launches = x.n <= 8192 ? 1.0 : passes + 2.0;  // one tile sorts in one launch
+      return n * kb + passes * 2.0 * n * (kb + 4.0) + launches * kLaunch;
     }
     case YSQL_JOB_FILTER: return n * (kb + 4.0) * 1.5 + 2.0 * kLaunch;
     default: return n * (kb + 4.0) + 2.0 * kLaunch;
@@ -48,13 +49,20 @@ static size_t job_ws_bytes(const ysql_job &x) {
   return std::max<size_t>(w, 256);
 }
 
+static int sort_variant(const ysql_batch_tuning &t, const ysql_job &x) {
+  int b = 0;
+  while (b < 31 && (2ull << b) <= x.n) b++;  // floor(log2(n))
+  return t.sort_variant[x.key_dtype == YSQL_UKEY64 ? 1 : 0][b];
+}
+
 // The per-call launches of one job on `stream` (workspace `ws`).
-static int issue_job(const ysql_job &x, const float *thr_dev, void *ws, size_t ws_bytes, ysql_stream_t stream) {
+static int issue_job(const ysql_job &x, const ysql_batch_tuning &t, const float *thr_dev, void *ws, size_t ws_bytes,
+                     ysql_stream_t stream) {
   const int kdt = x.key_dtype == YSQL_UKEY64 ? YSQL_DATETIME : YSQL_INT;
   if (x.kind == YSQL_JOB_SORT) {
     if (x.n == 0) return YSQL_OK;
-    return ysql_radix_sort_ex(x.key_dtype, 0, x.keys, nullptr, 0, x.keys_out, x.perm_out, x.n, 0,
-                              x.key_dtype == YSQL_UKEY64 ? 64 : 32, ws, ws_bytes, stream);
+    return ysql_radix_sort_variant(x.key_dtype, sort_variant(t, x), 0, x.keys, nullptr, 0, x.keys_out, x.perm_out,
+                                   x.n, 0, x.key_dtype == YSQL_UKEY64 ? 64 : 32, ws, ws_bytes, stream);
   }
   if (x.kind == YSQL_JOB_FILTER) {
     const uint32_t kb = x.key_dtype == YSQL_UKEY64 ? 8 : 4;
@@ -93,10 +101,9 @@ extern "C" {
 
 void ysql_batch_tuning_default(ysql_batch_tuning *t) {
   if (!t) return;
-  t->small_sort_max = 0;
-  t->hist_chunk = 1 << 16;
-  t->workers_per_cu = 2;
-  t->chains = 16;
+  memset(t, 0, sizeof(*t));
+  t->chains = 32;
+  memset(t->sort_variant, -1, sizeof(t->sort_variant));
 }
 
 void ysql_batch_destroy(ysql_batch *b) {
@@ -126,6 +133,9 @@ int ysql_batch_create(const ysql_job *jobs, int njobs, const ysql_batch_tuning *
   ysql_batch_tuning_default(&tune);
   if (tuning) tune = *tuning;
   tune.chains = std::max(1, std::min(tune.chains, kMaxChains));
+  for (auto &row : tune.sort_variant)
+    for (auto &v : row)
+      if (v >= ysql_radix_variant_count()) return YSQL_EINVAL;
   for (int q = 0; q < njobs; q++) {
     const ysql_job &x = jobs[q];
     if (x.kind < YSQL_JOB_SORT || x.kind > YSQL_JOB_AGG) return YSQL_EINVAL;
@@ -206,7 +216,7 @@ int ysql_batch_create(const ysql_job *jobs, int njobs, const ysql_batch_tuning *
     char *ws = (char *)b->chain_ws + b->chain_ws_off[c];
     for (size_t i = 0; ok && !rc && i < b->chain_jobs[c].size(); i++) {
       const int q = b->chain_jobs[c][i];
-      rc = issue_job(b->jobs[q], b->d_thr + q, ws, b->chain_ws_len[c], (ysql_stream_t)s);
+      rc = issue_job(b->jobs[q], b->tune, b->d_thr + q, ws, b->chain_ws_len[c], (ysql_stream_t)s);
     }
     ok = ok && hipEventRecord(b->events[c + 1], s) == hipSuccess;
     ok = ok && hipStreamWaitEvent(cap, b->events[c + 1], 0) == hipSuccess;
@@ -233,7 +243,7 @@ int ysql_batch_run(ysql_batch *b, ysql_stream_t stream) {
 int ysql_batch_run_plain(ysql_batch *b, ysql_stream_t stream) {
   if (!b) return YSQL_EINVAL;
   for (size_t q = 0; q < b->jobs.size(); q++) {
-    const int rc = issue_job(b->jobs[q], b->d_thr + q, b->plain_ws, b->plain_ws_bytes, stream);
+    const int rc = issue_job(b->jobs[q], b->tune, b->d_thr + q, b->plain_ws, b->plain_ws_bytes, stream);
     if (rc) return rc;
   }
   return YSQL_OK;

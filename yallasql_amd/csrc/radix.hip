@@ -33,6 +33,15 @@ __global__ void k_zero_u32(uint32_t *p, uint64_t words) {
     p[i] = 0;
 }
 
+// A whole sort of n <= one tile in one workgroup.
+template <typename K, int BLOCK, int KPT>
+__global__ __launch_bounds__(BLOCK) void k_sort_small(const K *kin, const uint32_t *vin, uint32_t val_base, K *kout,
+                                                      uint32_t *vout, uint32_t n, int begin_bit, int passes,
+                                                      uint32_t desc) {
+  __shared__ OnesweepSmem<K, BLOCK, KPT> sm;
+  small_sort_block<K, BLOCK, KPT>(sm, kin, vin, val_base, kout, vout, n, begin_bit, passes, desc);
+}
+
 template <typename K>
 __global__ void k_copy_pairs(const K *ks, const uint32_t *vs, K *kd, uint32_t *vd, uint64_t n) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
@@ -60,34 +69,51 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const K *__restrict__ kin, c
 // ---------------------------------------------------------------------------
 // Host driver
 // ---------------------------------------------------------------------------
-template <typename K>
+// Tile shapes of the digit pass (threads x keys per thread).  Variant 0
+// (8192-key tiles, look-back after the LDS reorder) measured best on 2^25
+// keys; the smaller tiles give mid-size sorts more workgroups.  A sort uses
+// one variant for all its passes (the status areas are laid out per tile).
+constexpr int kRadixVariants = 3;
+template <int V>
 struct RadixTile;
 template <>
-struct RadixTile<uint32_t> {
+struct RadixTile<0> {
   static constexpr int BLOCK = 512, KPT = 16;
-  static constexpr bool EARLY = false;
 };
 template <>
-struct RadixTile<uint64_t> {
-  // 8192-key tiles, look-back after the LDS reorder: measured best of
-  // 256..1024 threads x 8..32 keys on 2^25 keys (216 us per pass on MI355X).
-  static constexpr int BLOCK = 512, KPT = 16;
-  static constexpr bool EARLY = false;
+struct RadixTile<1> {
+  static constexpr int BLOCK = 256, KPT = 16;
 };
+template <>
+struct RadixTile<2> {
+  static constexpr int BLOCK = 256, KPT = 8;
+};
+constexpr int kSmallSortBlock = 512, kSmallSortKpt = 16;  // one-workgroup sorts up to 8192 keys
+constexpr uint64_t kMinTile = 256 * 8;                     // smallest variant tile: sizes the status areas
+
+// Default variant for an n-key sort (both key widths).
+static inline int radix_default_variant(uint64_t n) {
+  if (n <= (1ull << 19)) return 2;
+  if (n <= (1ull << 21)) return 1;
+  return 0;
+}
+
+static inline uint64_t variant_tile(int v) {
+  return v == 0 ? 512 * 16 : (v == 1 ? 256 * 16 : 256 * 8);
+}
 
 struct RadixWs {
   void *kalt;
   uint32_t *valt;
   uint32_t *hist;  // 8 * 256
   uint32_t *ctr;   // 8, padded
-  uint32_t *st0;   // ntiles * 256
+  uint32_t *st0;   // ntiles * 256 (ntiles of the smallest tile)
   uint32_t *st1;   // ntiles * 256
 };
 
 template <typename K>
 static size_t radix_ws_layout(uint64_t n, void *ws, size_t bytes, RadixWs *out) {
-  constexpr int TILE = RadixTile<K>::BLOCK * RadixTile<K>::KPT;
-  const uint64_t ntiles = ceil_div(n, TILE);
+  const uint64_t ntiles = ceil_div(n, kMinTile);
   if (out) {
     WsCarver c(ws, bytes);
     out->kalt = c.take<K>(n);
@@ -121,16 +147,16 @@ static inline int radix_passes(int begin_bit, int end_bit) { return (end_bit - b
 // Zeroes the histograms, tile counters and first status area, then builds
 // the histograms of every digit from keys_in.
 template <typename K>
-static int radix_begin_t(int desc, const K *keys_in, uint64_t n, int begin_bit, int end_bit, const RadixWs &w,
-                         hipStream_t s) {
-  constexpr int TILE = RadixTile<K>::BLOCK * RadixTile<K>::KPT;
-  const uint32_t ntiles = (uint32_t)ceil_div(n, TILE);
-  k_zero_u32<<<64, 256, 0, s>>>(w.hist, 8 * kBins + 64);
-  YSQL_LAUNCH_CHECK();
-  uint64_t zg = ceil_div((uint64_t)ntiles * kBins, 256 * 8);
+static int radix_begin_t(int desc, const K *keys_in, uint64_t n, int begin_bit, int end_bit, int variant,
+                         const RadixWs &w, hipStream_t s) {
+  const uint32_t ntiles = (uint32_t)ceil_div(n, variant_tile(variant));
+  // the histograms, pass counters and first status area are adjacent in
+  // the workspace: one launch clears them
+  const uint64_t zwords = (uint64_t)(w.st0 - w.hist) + (uint64_t)ntiles * kBins;
+  uint64_t zg = ceil_div(zwords, 256 * 8);
   if (zg > 2048) zg = 2048;
   if (zg < 1) zg = 1;
-  k_zero_u32<<<(unsigned)zg, 256, 0, s>>>(w.st0, (uint64_t)ntiles * kBins);
+  k_zero_u32<<<(unsigned)zg, 256, 0, s>>>(w.hist, zwords);
   YSQL_LAUNCH_CHECK();
   uint64_t g = ceil_div(n, 256 * 16);
   const uint64_t cap = (uint64_t)grid_cap() * 4;
@@ -143,15 +169,27 @@ static int radix_begin_t(int desc, const K *keys_in, uint64_t n, int begin_bit, 
 }
 
 // One digit pass p of `passes`: (kin, vin or val_base + row) -> (kout, vout).
-template <typename K>
-static int radix_pass_t(int desc, int p, int passes, const K *kin, const uint32_t *vin, uint32_t val_base, K *kout,
-                        uint32_t *vout, uint64_t n, int begin_bit, const RadixWs &w, hipStream_t s) {
-  constexpr int BLOCK = RadixTile<K>::BLOCK, KPT = RadixTile<K>::KPT, TILE = BLOCK * KPT;
+template <typename K, int V>
+static void radix_pass_v(int desc, int p, int passes, const K *kin, const uint32_t *vin, uint32_t val_base, K *kout,
+                         uint32_t *vout, uint64_t n, int begin_bit, const RadixWs &w, hipStream_t s) {
+  constexpr int BLOCK = RadixTile<V>::BLOCK, KPT = RadixTile<V>::KPT, TILE = BLOCK * KPT;
   const uint32_t ntiles = (uint32_t)ceil_div(n, TILE);
   uint32_t *st = (p & 1) ? w.st1 : w.st0;
   uint32_t *snext = (p + 1 < passes) ? ((p & 1) ? w.st0 : w.st1) : nullptr;
-  k_onesweep<K, BLOCK, KPT, RadixTile<K>::EARLY><<<ntiles, BLOCK, 0, s>>>(kin, vin, val_base, kout, vout, (uint32_t)n, begin_bit + 8 * p,
-                                                     desc ? 0xFFu : 0u, w.hist + p * kBins, st, snext, w.ctr + p);
+  k_onesweep<K, BLOCK, KPT, false><<<ntiles, BLOCK, 0, s>>>(kin, vin, val_base, kout, vout, (uint32_t)n,
+                                                            begin_bit + 8 * p, desc ? 0xFFu : 0u, w.hist + p * kBins,
+                                                            st, snext, w.ctr + p);
+}
+
+template <typename K>
+static int radix_pass_t(int desc, int p, int passes, const K *kin, const uint32_t *vin, uint32_t val_base, K *kout,
+                        uint32_t *vout, uint64_t n, int begin_bit, int variant, const RadixWs &w, hipStream_t s) {
+  switch (variant) {
+    case 0: radix_pass_v<K, 0>(desc, p, passes, kin, vin, val_base, kout, vout, n, begin_bit, w, s); break;
+    case 1: radix_pass_v<K, 1>(desc, p, passes, kin, vin, val_base, kout, vout, n, begin_bit, w, s); break;
+    case 2: radix_pass_v<K, 2>(desc, p, passes, kin, vin, val_base, kout, vout, n, begin_bit, w, s); break;
+    default: return YSQL_EINVAL;
+  }
   YSQL_LAUNCH_CHECK();
   return YSQL_OK;
 }
@@ -160,16 +198,25 @@ static int radix_pass_t(int desc, int p, int passes, const K *kin, const uint32_
 // keys_out (in place); vals_in == nullptr gives payload val_base + row.
 template <typename K>
 static int radix_sort_t(int desc, const K *kin, const uint32_t *vin, uint32_t vbase, K *kout, uint32_t *vout,
-                        uint64_t n, int begin_bit, int end_bit, void *ws, size_t ws_bytes, hipStream_t s) {
+                        uint64_t n, int begin_bit, int end_bit, int variant, void *ws, size_t ws_bytes,
+                        hipStream_t s) {
   if (n == 0) return YSQL_OK;
   int rc = radix_check<K>(kin, n, begin_bit, end_bit);
   if (rc) return rc;
   if (!kout || !vout) return YSQL_EINVAL;
+  if (variant < 0 || variant >= kRadixVariants) return YSQL_EINVAL;
   RadixWs w;
   if (!ws || ws_bytes < radix_ws_layout<K>(n, nullptr, 0, nullptr) || !radix_ws_layout<K>(n, ws, ws_bytes, &w))
     return YSQL_EWORKSPACE;
   const int P = radix_passes(begin_bit, end_bit);
-  rc = radix_begin_t<K>(desc, kin, n, begin_bit, end_bit, w, s);
+  if (n <= (uint64_t)kSmallSortBlock * kSmallSortKpt) {  // one tile: the whole sort in one workgroup
+    k_sort_small<K, kSmallSortBlock, kSmallSortKpt><<<1, kSmallSortBlock, 0, s>>>(kin, vin, vbase, kout, vout,
+                                                                                (uint32_t)n, begin_bit, P,
+                                                                                desc ? 0xFFu : 0u);
+    YSQL_LAUNCH_CHECK();
+    return YSQL_OK;
+  }
+  rc = radix_begin_t<K>(desc, kin, n, begin_bit, end_bit, variant, w, s);
   if (rc) return rc;
   K *kalt = (K *)w.kalt;
   uint32_t *valt = w.valt;
@@ -184,7 +231,7 @@ static int radix_sort_t(int desc, const K *kin, const uint32_t *vin, uint32_t vb
     const bool to_out = copy_back ? (p & 1) : (((P - 1 - p) & 1) == 0);
     K *kd = to_out ? kout : kalt;
     uint32_t *vd = to_out ? vout : valt;
-    rc = radix_pass_t<K>(desc, p, P, ksrc, vsrc, vbase, kd, vd, n, begin_bit, w, s);
+    rc = radix_pass_t<K>(desc, p, P, ksrc, vsrc, vbase, kd, vd, n, begin_bit, variant, w, s);
     if (rc) return rc;
     ksrc = kd;
     vsrc = vd;
@@ -213,17 +260,27 @@ int ysql_radix_sort_ws_bytes(int key_dtype, uint64_t n, size_t *bytes) {
   return YSQL_OK;
 }
 
+int ysql_radix_sort_variant(int key_dtype, int variant, int descending, const void *keys_in, const uint32_t *vals_in,
+                            uint32_t val_base, void *keys_out, uint32_t *vals_out, uint64_t n, int begin_bit,
+                            int end_bit, void *ws, size_t ws_bytes, ysql_stream_t stream) {
+  hipStream_t s = as_stream(stream);
+  if (variant < 0) variant = radix_default_variant(n);
+  if (key_dtype == YSQL_UKEY32)
+    return radix_sort_t<uint32_t>(descending, (const uint32_t *)keys_in, vals_in, val_base, (uint32_t *)keys_out,
+                                  vals_out, n, begin_bit, end_bit, variant, ws, ws_bytes, s);
+  if (key_dtype == YSQL_UKEY64)
+    return radix_sort_t<uint64_t>(descending, (const uint64_t *)keys_in, vals_in, val_base, (uint64_t *)keys_out,
+                                  vals_out, n, begin_bit, end_bit, variant, ws, ws_bytes, s);
+  return YSQL_EINVAL;
+}
+
+int ysql_radix_variant_count(void) { return kRadixVariants; }
+
 int ysql_radix_sort_ex(int key_dtype, int descending, const void *keys_in, const uint32_t *vals_in, uint32_t val_base,
                        void *keys_out, uint32_t *vals_out, uint64_t n, int begin_bit, int end_bit, void *ws,
                        size_t ws_bytes, ysql_stream_t stream) {
-  hipStream_t s = as_stream(stream);
-  if (key_dtype == YSQL_UKEY32)
-    return radix_sort_t<uint32_t>(descending, (const uint32_t *)keys_in, vals_in, val_base, (uint32_t *)keys_out,
-                                  vals_out, n, begin_bit, end_bit, ws, ws_bytes, s);
-  if (key_dtype == YSQL_UKEY64)
-    return radix_sort_t<uint64_t>(descending, (const uint64_t *)keys_in, vals_in, val_base, (uint64_t *)keys_out,
-                                  vals_out, n, begin_bit, end_bit, ws, ws_bytes, s);
-  return YSQL_EINVAL;
+  return ysql_radix_sort_variant(key_dtype, -1, descending, keys_in, vals_in, val_base, keys_out, vals_out, n,
+                                 begin_bit, end_bit, ws, ws_bytes, stream);
 }
 
 int ysql_radix_argsort(int key_dtype, int descending, void *keys_inout, uint32_t *perm_out, uint64_t n, int begin_bit,
@@ -251,7 +308,8 @@ int ysql_radix_begin(int key_dtype, int descending, const void *keys_in, uint64_
     if (!ws || ws_bytes < radix_ws_layout<uint32_t>(n, nullptr, 0, nullptr) ||
         !radix_ws_layout<uint32_t>(n, ws, ws_bytes, &w))
       return YSQL_EWORKSPACE;
-    return radix_begin_t<uint32_t>(descending, (const uint32_t *)keys_in, n, begin_bit, end_bit, w, s);
+    return radix_begin_t<uint32_t>(descending, (const uint32_t *)keys_in, n, begin_bit, end_bit,
+                                   radix_default_variant(n), w, s);
   }
   if (key_dtype == YSQL_UKEY64) {
     int rc = radix_check<uint64_t>(keys_in, n, begin_bit, end_bit);
@@ -259,7 +317,8 @@ int ysql_radix_begin(int key_dtype, int descending, const void *keys_in, uint64_
     if (!ws || ws_bytes < radix_ws_layout<uint64_t>(n, nullptr, 0, nullptr) ||
         !radix_ws_layout<uint64_t>(n, ws, ws_bytes, &w))
       return YSQL_EWORKSPACE;
-    return radix_begin_t<uint64_t>(descending, (const uint64_t *)keys_in, n, begin_bit, end_bit, w, s);
+    return radix_begin_t<uint64_t>(descending, (const uint64_t *)keys_in, n, begin_bit, end_bit,
+                                   radix_default_variant(n), w, s);
   }
   return YSQL_EINVAL;
 }
@@ -280,7 +339,7 @@ int ysql_radix_pass(int key_dtype, int descending, int pass, const void *keys_in
         !radix_ws_layout<uint32_t>(n, ws, ws_bytes, &w))
       return YSQL_EWORKSPACE;
     return radix_pass_t<uint32_t>(descending, pass, P, (const uint32_t *)keys_in, vals_in, val_base,
-                                  (uint32_t *)keys_out, vals_out, n, begin_bit, w, s);
+                                  (uint32_t *)keys_out, vals_out, n, begin_bit, radix_default_variant(n), w, s);
   }
   if (key_dtype == YSQL_UKEY64) {
     int rc = radix_check<uint64_t>(keys_in, n, begin_bit, end_bit);
@@ -289,7 +348,7 @@ int ysql_radix_pass(int key_dtype, int descending, int pass, const void *keys_in
         !radix_ws_layout<uint64_t>(n, ws, ws_bytes, &w))
       return YSQL_EWORKSPACE;
     return radix_pass_t<uint64_t>(descending, pass, P, (const uint64_t *)keys_in, vals_in, val_base,
-                                  (uint64_t *)keys_out, vals_out, n, begin_bit, w, s);
+                                  (uint64_t *)keys_out, vals_out, n, begin_bit, radix_default_variant(n), w, s);
   }
   return YSQL_EINVAL;
 }

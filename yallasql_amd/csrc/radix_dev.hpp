@@ -1,6 +1,5 @@
-// radix_dev.hpp — device-side building blocks of the 8-bit LSD radix sort,
-// shared by the per-call kernels (radix.hip) and the persistent batched
-// executor (batched.hip).
+// radix_dev.hpp — device-side building blocks of the 8-bit LSD radix sort
+// (used by the kernels of radix.hip).
 //
 //   hist_block:    one workgroup adds the 256-bin histograms of every digit
 //                  position of keys[begin, end) into a global histogram.
@@ -11,6 +10,8 @@
 //                  thread per digit walks back over predecessor tiles), the
 //                  tile reordered by digit in LDS, then each digit's run of
 //                  keys and payloads written to its global position.
+//   small_sort_block: a whole sort of one tile's worth of keys in one
+//                  workgroup (every pass through LDS).
 // Stability: keys are ranked in (wave, item, lane) order, which is the input
 // order inside the tile, and tiles are placed in tile order.
 #pragma once
@@ -243,6 +244,111 @@ __device__ __forceinline__ void onesweep_tile(OnesweepSmem<K, BLOCK, KPT> &sm, u
     if (j < valid_cnt) vout[opos[q]] = s_v[j];
   }
   __syncthreads();  // LDS is reused by the caller's next tile
+}
+
+// Whole sort of n <= BLOCK * KPT keys in one workgroup: every digit pass
+// ranks the keys held in registers (the onesweep wave ranking), offsets each
+// digit by the block's exclusive digit prefix (no histogram, no look-back),
+// scatters keys and then payloads through LDS and reloads them in the same
+// per-thread order.  Stable, so the permutation equals the multi-pass one.
+// kin may equal kout and vin vout (everything is read before the first
+// store).  Every thread of the block calls it; contains barriers.
+template <typename K, int BLOCK, int KPT>
+__device__ __forceinline__ void small_sort_block(OnesweepSmem<K, BLOCK, KPT> &sm, const K *kin, const uint32_t *vin,
+                                                 uint32_t val_base, K *kout, uint32_t *vout, uint32_t n,
+                                                 int begin_bit, int passes, uint32_t desc) {
+  static_assert(BLOCK >= kBins, "one thread per digit");
+  constexpr int W = BLOCK / kWave;
+  const int t = threadIdx.x;
+  const int lane = lane_id();
+  const int wave = t / kWave;
+  const uint32_t wbase = (uint32_t)wave * KPT * kWave;
+  K key[KPT];
+  uint32_t val[KPT];
+#pragma unroll
+  for (int i = 0; i < KPT; i++) {
+    const uint32_t idx = wbase + i * kWave + lane;
+    const bool valid = idx < n;
+    key[i] = valid ? kin[idx] : (K)0;
+    val[i] = vin ? (valid ? vin[idx] : 0u) : val_base + idx;
+  }
+  uint32_t *s_v = reinterpret_cast<uint32_t *>(sm.k);
+  for (int p = 0; p < passes; p++) {
+    const int shift = begin_bit + 8 * p;
+    for (int i = t; i < W * kBins; i += BLOCK) (&sm.wh[0][0])[i] = 0;
+    __syncthreads();
+    uint32_t rank[KPT];
+#pragma unroll
+    for (int i = 0; i < KPT; i++) {
+      const uint32_t idx = wbase + i * kWave + lane;
+      const bool valid = idx < n;
+      const uint32_t d = ((uint32_t)(key[i] >> shift) & 0xFFu) ^ desc;
+      uint64_t m = ballot(valid);
+#pragma unroll
+      for (int b = 0; b < 8; b++) {
+        const bool bit = (d >> b) & 1u;
+        const uint64_t bal = ballot(bit);
+        m &= bit ? bal : ~bal;
+      }
+      const uint32_t lower = popc_below(m);
+      uint32_t pre = 0;
+      if (valid) pre = sm.wh[wave][d];
+      rank[i] = pre + lower;
+      if (valid && lower == 0) sm.wh[wave][d] = pre + (uint32_t)__popcll(m);
+    }
+    __syncthreads();
+    uint32_t cnt = 0;
+    if (t < kBins) {
+      uint32_t run = 0;
+#pragma unroll
+      for (int w = 0; w < W; w++) {
+        const uint32_t c = sm.wh[w][t];
+        sm.wh[w][t] = run;
+        run += c;
+      }
+      cnt = run;
+    }
+    uint32_t tot;
+    const uint32_t dstart = block_exclusive_sum<BLOCK, uint32_t>(t < kBins ? cnt : 0u, sm.red, &tot);
+    if (t < kBins) sm.dstart[t] = dstart;
+    __syncthreads();
+    uint32_t pos[KPT];
+#pragma unroll
+    for (int i = 0; i < KPT; i++) {
+      const uint32_t idx = wbase + i * kWave + lane;
+      pos[i] = 0xFFFFFFFFu;
+      if (idx < n) {
+        const uint32_t d = ((uint32_t)(key[i] >> shift) & 0xFFu) ^ desc;
+        pos[i] = sm.dstart[d] + sm.wh[wave][d] + rank[i];
+        sm.k[pos[i]] = key[i];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < KPT; i++) {
+      const uint32_t idx = wbase + i * kWave + lane;
+      if (idx < n) key[i] = sm.k[idx];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < KPT; i++)
+      if (pos[i] != 0xFFFFFFFFu) s_v[pos[i]] = val[i];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < KPT; i++) {
+      const uint32_t idx = wbase + i * kWave + lane;
+      if (idx < n) val[i] = s_v[idx];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < KPT; i++) {
+    const uint32_t idx = wbase + i * kWave + lane;
+    if (idx < n) {
+      kout[idx] = key[i];
+      vout[idx] = val[i];
+    }
+  }
 }
 
 }  // namespace ysql

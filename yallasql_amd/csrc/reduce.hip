@@ -671,15 +671,15 @@ struct FaArgs {
 // shared scans; a query's fold sequence is the same in both.
 template <typename PT, int CMP, int QMAX>
 __global__ __launch_bounds__(kFaBlock) void k_filter_agg(const PT *pred, const PT *scalars, int nq, FaArgs a, int ns,
-                                                         uint64_t n, ysql_acc *partials, uint32_t *ticket,
-                                                         uint32_t ntiles, int overwrite) {
+                                                         uint64_t n, uint64_t tile_rows, ysql_acc *partials,
+                                                         uint32_t *ticket, uint32_t ntiles, int overwrite) {
   __shared__ ysql_acc s_w[kFaBlock / kWave];
   __shared__ int s_last;
   PT sc[QMAX];
 #pragma unroll
   for (int q = 0; q < QMAX; q++) sc[q] = q < nq ? scalars[q] : PT();
   for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x)
-    filter_agg_tile<PT, CMP, kFaBlock, QMAX>(tile, pred, sc, nq, a.spec, ns, n, partials, s_w);
+    filter_agg_tile<PT, CMP, kFaBlock, QMAX>(tile, tile_rows, pred, sc, nq, a.spec, ns, n, partials, s_w);
   __threadfence();
   if (threadIdx.x == 0) s_last = (atomicAdd(ticket, 1u) == gridDim.x - 1);
   __syncthreads();
@@ -709,7 +709,8 @@ static uint32_t fa_blocks_per_cu(K kernel) {
 template <typename PT, int QMAX>
 static int filter_agg_launch(int cmp, const void *pred, const void *scalars, int nq, const FaArgs &a, int ns,
                              uint64_t n, ysql_acc *partials, uint32_t *ticket, int overwrite, hipStream_t s) {
-  const uint32_t ntiles = (uint32_t)ceil_div(n, kFaTile);
+  const uint64_t tile_rows = fa_tile_rows(n);
+  const uint32_t ntiles = (uint32_t)ceil_div(n, tile_rows);
   const PT *p = (const PT *)pred;
   const PT *sc = (const PT *)scalars;
   // grid: the workgroups that fit at once (tiles are taken grid-stride)
@@ -717,7 +718,8 @@ static int filter_agg_launch(int cmp, const void *pred, const void *scalars, int
   case C: {                                                                                                \
     static const uint32_t per_cu = fa_blocks_per_cu(k_filter_agg<PT, C, QMAX>);                           \
     const uint32_t g = std::min<uint32_t>(ntiles, per_cu * (uint32_t)grid_cap());                          \
-    k_filter_agg<PT, C, QMAX><<<g, kFaBlock, 0, s>>>(p, sc, nq, a, ns, n, partials, ticket, ntiles, overwrite); \
+    k_filter_agg<PT, C, QMAX><<<g, kFaBlock, 0, s>>>(p, sc, nq, a, ns, n, tile_rows, partials, ticket, ntiles,      \
+                                                     overwrite);                                               \
     break;                                                                                                 \
   }
   switch (cmp) {
@@ -737,7 +739,7 @@ static int filter_agg_launch(int cmp, const void *pred, const void *scalars, int
 static size_t filter_agg_ws_size(uint64_t n, int nq, int ns) {
   WsCounter c;
   c.take<uint32_t>(64);
-  c.take<ysql_acc>(ceil_div(n, kFaTile) * (uint64_t)nq * ns + 1);
+  c.take<ysql_acc>(ceil_div(n, fa_tile_rows(n)) * (uint64_t)nq * ns + 1);
   return c.off;
 }
 
@@ -791,7 +793,7 @@ static int filter_agg_impl(int pred_dtype, int cmp, const void *pred_col, const 
   if (n >= (1ull << 40)) return YSQL_EINVAL;
   WsCarver c(ws, ws_bytes);
   uint32_t *ticket = c.take<uint32_t>(64);
-  ysql_acc *partials = c.take<ysql_acc>(ceil_div(n, kFaTile) * (uint64_t)nq * nspecs + 1);
+  ysql_acc *partials = c.take<ysql_acc>(ceil_div(n, fa_tile_rows(n)) * (uint64_t)nq * nspecs + 1);
   if (!ws || !ticket || !partials) return YSQL_EWORKSPACE;
   k_reduce_reset<<<1, 1, 0, s>>>(ticket);
   YSQL_LAUNCH_CHECK();

@@ -1,5 +1,5 @@
-// scan_dev.hpp — device-side stream compaction of one tile, shared by the
-// per-call compaction kernels (scan.hip) and the batched executor.
+// scan_dev.hpp — device-side stream compaction of one tile (used by the
+// compaction kernels of scan.hip).
 //
 // Each thread owns 16 consecutive rows of a tile of BLOCK * 16 rows: it
 // computes their selection flags (from a byte mask or a fused predicate),
