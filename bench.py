@@ -44,6 +44,7 @@ def parse():
     ap.add_argument("--no-ops", action="store_true", help="skip the per-op roofline lines (filter, filter+aggregate)")
     ap.add_argument("--no-batched", action="store_true", help="skip the 4,096-job batched vs plain comparison")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving loop")
+    ap.add_argument("--no-configs", action="store_true", help="skip the config-1 and config-3 lines")
     ap.add_argument("--serving-seconds", type=float, default=2.5, help="arrival window per rate")
     return ap.parse_args()
 
@@ -256,17 +257,95 @@ def op_lines(keys):
     return out
 
 
+def config1_line():
+    """Config 1: WHERE price > 500 ORDER BY price DESC, id ASC; SELECT id, price,
+    2*id over 1,048,576 rows in one batch, through the engine (result copied
+    to the host), median wall time of 10 queries."""
+    import numpy as np
+
+    from yallasql_amd import engine as E
+
+    n = 1 << 20
+    r = np.random.default_rng(0)
+    ids = r.permutation(n).astype(np.int32)
+    price = np.minimum((1 + 999 * r.random(n)).astype(np.float32), np.float32(999.9999))
+    cat = r.integers(0, 64, n).astype(np.int32)
+    ctx = E.Context()
+    t = ctx.table("t", [("id", ids), ("price", price), ("category", cat)])
+
+    def query():
+        p = ctx.scan(t, n).filter(E.cmp(2, E.ref(1), E.f32(500.0))).order([1, 0], [1, 0])
+        return p.project([E.ref(0), E.ref(1), E.arith(2, E.i32(2), E.ref(0))]).execute()
+
+    cols, _, _ = query()
+    ts = []
+    for _ in range(10):
+        t0 = time.perf_counter()
+        query()
+        ts.append(time.perf_counter() - t0)
+    ts.sort()
+    ms = ts[len(ts) // 2] * 1e3
+    m = int(cols[0].shape[0])
+    return {"rows": n, "selected": m, "ms": ms, "rows_per_s": n / (ms * 1e-3),
+            "what": "engine scan -> filter -> order (price desc, id) -> projection, result on the host"}
+
+
+def config3_line():
+    """Config 3: star join of a 600M-row fact table with four dimensions
+    (2,556 / 2,000 / 30,000 / 200,000 rows; region = 0 and category = 0
+    filters), SUM(extendedprice * discount) GROUP BY year x brand (7,000
+    groups), one fused kernel; HBM roofline by the fact columns read."""
+    import torch
+
+    from yallasql_amd import ops
+
+    dev = torch.device("cuda:0")
+    n = 600_000_000
+    g = torch.Generator(device=dev)
+    g.manual_seed(2)
+    rows = [2556, 2000, 30000, 200000]
+    fks = [torch.randint(0, R, (n,), generator=g, device=dev, dtype=torch.int32) for R in rows]
+    m1 = torch.randint(90000, 10000001, (n,), generator=g, device=dev, dtype=torch.int32)
+    m2 = torch.randint(0, 11, (n,), generator=g, device=dev, dtype=torch.int32)
+    year = (1992 + (7 * torch.arange(2556, device=dev)) // 2556).to(torch.int32)
+    region = torch.randint(0, 7, (30000,), generator=g, device=dev, dtype=torch.int32)
+    cat = torch.randint(0, 25, (200000,), generator=g, device=dev, dtype=torch.int32)
+    brand = torch.randint(0, 1000, (200000,), generator=g, device=dev, dtype=torch.int32)
+    dims = [{"fk": fks[0], "rows": 2556, "group": year, "group_base": 1992, "group_card": 7},
+            {"fk": fks[1], "rows": 2000},
+            {"fk": fks[2], "rows": 30000, "pass": (region == 0).to(torch.uint8)},
+            {"fk": fks[3], "rows": 200000, "pass": (cat == 0).to(torch.uint8), "group": brand, "group_card": 1000}]
+    sums = torch.zeros(7000, dtype=torch.int64, device=dev)
+    counts = torch.zeros(7000, dtype=torch.int64, device=dev)
+
+    def run():
+        sums.zero_()
+        counts.zero_()
+        ops.star_join_group_sum(dims, m1, m2, 7000, sums=sums, counts=counts)
+
+    run()
+    ms = _timed_ms(run)
+    b = n * 24  # four int32 foreign keys + two int32 measures per fact row
+    out = {"fact_rows": n, "ms": ms, "rows_per_s": n / (ms * 1e-3), "qualifying_rows": int(counts.sum().item()),
+           "GB_s": b / (ms * 1e-3) / 1e9, "frac": b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+           "what": "fused 4-FK star join + GROUP BY year x brand SUM(int64(price)*discount)"}
+    del fks, m1, m2
+    torch.cuda.empty_cache()
+    return out
+
+
 def batched_lines():
     """4,096 ragged sort/filter/aggregate jobs (log-uniform 1,024..4,194,304
     rows): the per-job launch loop against the captured graph replay."""
     import numpy as np
     import torch
 
+    from yallasql_amd.autotune import load_sort_variants
     from yallasql_amd.batched import JobSet, config4_jobs
 
     dev = torch.device("cuda:0")
     sizes, kinds, key64, thr = config4_jobs()
-    js = JobSet(sizes, kinds, key64, thr, dev, seed=4).create(chains=32)
+    js = JobSet(sizes, kinds, key64, thr, dev, seed=4).create(chains=32, sort_variants=load_sort_variants())
     js.run_batched()
     js.run_plain()
     torch.cuda.synchronize()
@@ -296,7 +375,9 @@ def batched_lines():
            "speedup": t_plain / t_graph, "identical_outputs": bool(same),
            "batched_GB_s": alg / t_graph / 1e9, "batched_frac": alg / t_graph / 1e9 / HBM_PEAK_GBS,
            "chains": js.info()["chains"],
-           "what": "plain: per-job launches on one stream; batched: one hipGraph replay of 32 launch chains"}
+           "what": "plain: per-job launches on one stream; batched: one hipGraph replay (jobs <= 64K rows one per "
+                   "workgroup in segmented kernels, larger jobs in 32 launch chains); sort tiles from the "
+                   "autotuned table"}
     js.close()
     del js
     torch.cuda.empty_cache()
@@ -395,6 +476,8 @@ def main():
     del job
     del keys
     torch.cuda.empty_cache()
+    configs = None if args.no_configs else {"config1_filter_sort_project": config1_line(),
+                                            "config3_star_join_group_by": config3_line()}
     batched = None if args.no_batched else batched_lines()
     serving = None if args.no_serving else serving_lines(args.serving_seconds)
 
@@ -440,6 +523,7 @@ def main():
                             "what": "torch copy_ of 1.6 GB (read + write), best of 3: attainable HBM rate"},
             **(extra_ops or {}),
         },
+        "configs": configs,
         "batched": batched,
         "serving": serving,
         "cpu_baseline": cpu,

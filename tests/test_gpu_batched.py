@@ -98,6 +98,27 @@ def test_batch_info(jobset):
     assert info["jobs"] == len(jobset.sizes) and 1 <= info["chains"] <= 64
 
 
+@pytest.mark.parametrize("seg_max", [0, 65536, 1 << 22])
+def test_segmented_and_chained_paths_same_bits(dev, seg_max):
+    """Every job through launch chains only (0), the default split, and one
+    workgroup per job (4M): the per-job loop's bits each time."""
+    from yallasql_amd.batched import JobSet
+
+    sizes, kinds, key64, thr = make_jobs(seed=8, extra=30, hi=400_000)
+    js = JobSet(sizes, kinds, key64, thr, dev, seed=3).create(seg_max=seg_max)
+    try:
+        js.clear_outputs()
+        js.run_plain()
+        plain = js.snapshot()
+        js.clear_outputs()
+        js.run_batched()
+        for a, b in zip(plain, js.snapshot()):
+            assert np.array_equal(a, b)
+        check_jobs_against_oracle(js)
+    finally:
+        js.close()
+
+
 def test_single_chain_equals_plain(dev):
     from yallasql_amd.batched import JobSet
 

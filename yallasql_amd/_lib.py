@@ -148,7 +148,8 @@ FA_OVERWRITE = 1
 
 
 class BatchTuning(C.Structure):
-    _fields_ = [("chains", C.c_int), ("reserved", C.c_int * 3), ("sort_variant", (C.c_int8 * 32) * 2)]
+    _fields_ = [("chains", C.c_int), ("seg_sort_max", C.c_int), ("seg_filter_max", C.c_int), ("seg_agg_max", C.c_int),
+                ("sort_variant", (C.c_int8 * 32) * 2)]
 
 
 JOB_SORT, JOB_FILTER, JOB_AGG = 0, 1, 2

@@ -10,17 +10,116 @@
 // kernels overlap on the device, so small jobs no longer run one at a time
 // on an otherwise idle GPU.
 //
-// ysql_batch_run_plain issues the same per-call entry points one job after
+// Jobs up to a size limit per kind (the tuning's seg_*_max) do not get a
+// chain: each runs in ONE workgroup of a segmented kernel (one launch per
+// kind and key width for all of them), which walks the job's tiles in order
+// with running offsets in LDS and needs no communication between
+// workgroups.  Jobs are dealt to workgroups largest first.
+//
+// ysql_batch_run_plain issues the per-call entry points one job after
 // another on one stream (the operator-at-a-time pattern of the reference's
-// engine).  Both forms run the same kernels in the same order for every job,
-// so their outputs are bit-identical.
+// engine).  Every path produces the same bits: sorts and compactions are
+// stable (unique results), and the segmented aggregate folds the same
+// canonical tiles with the same device functions as the per-call kernel.
 #include <algorithm>
 #include <numeric>
 #include <vector>
 
-#include "common.hpp"
+#include "agg_dev.hpp"
+#include "radix_dev.hpp"
+#include "scan_dev.hpp"
 
 namespace ysql {
+
+// ---------------------------------------------------------------- segmented
+constexpr int kSegSortBlock = 256, kSegSortKpt = 16;  // 4096-key tiles
+constexpr int kSegFiltBlock = 256;                    // 4096-row compaction tiles
+
+struct SegSort {
+  const void *keys;
+  void *keys_out;
+  uint32_t *perm_out;
+  void *kalt;
+  uint32_t *valt;
+  uint32_t n, pad;
+};
+
+struct SegFilter {
+  const void *keys;
+  const float *vals;
+  void *keys_out;
+  float *vals_out;
+  uint64_t *count_out;
+  float thr;
+  uint32_t n, kb, pad;
+};
+
+struct SegAgg {
+  const void *keys;
+  const float *vals;
+  ysql_acc *accs_out;
+  ysql_acc *partials;
+  float thr;
+  uint32_t n;
+  int key64, pad;
+};
+
+template <typename K>
+__global__ __launch_bounds__(kSegSortBlock) void k_seg_sort(const SegSort *jobs) {
+  __shared__ SegSortSmem<K, kSegSortBlock, kSegSortKpt> sm;
+  const SegSort j = jobs[blockIdx.x];
+  seg_sort_block<K, kSegSortBlock, kSegSortKpt>(sm, (const K *)j.keys, 0u, (K *)j.keys_out, j.perm_out, (K *)j.kalt,
+                                               j.valt, j.n, 0, (int)sizeof(K), 0u);
+}
+
+__global__ __launch_bounds__(kSegFiltBlock) void k_seg_filter(const SegFilter *jobs) {
+  __shared__ CompactSmem<kSegFiltBlock> sm;
+  const SegFilter j = jobs[blockIdx.x];
+  if (j.n == 0) {
+    if (threadIdx.x == 0) *j.count_out = 0;
+    return;
+  }
+  const ysql_col cols[2] = {{j.keys, nullptr, j.keys_out, nullptr, j.kb, 0}, {j.vals, nullptr, j.vals_out, nullptr, 4, 0}};
+  constexpr uint32_t TILE = kSegFiltBlock * 16;
+  const uint32_t ntiles = (j.n + TILE - 1) / TILE;
+  uint64_t running = 0;
+  for (uint32_t tile = 0; tile < ntiles; tile++)
+    running += compact_tile<kSegFiltBlock, 1, float, YSQL_LT>(sm, tile, nullptr, j.vals, j.thr, 0.f, cols, 2, j.n,
+                                                               ~0ull, running, j.count_out, nullptr, ntiles);
+}
+
+// SUM(vals), MIN(keys), MAX(keys) WHERE vals < thr: the specs
+// ysql_filter_aggregate_ex builds for the same job.
+__device__ __forceinline__ void seg_agg_specs(const SegAgg &j, FaSpec (&sp)[3]) {
+  const int kdt = j.key64 ? YSQL_DATETIME : YSQL_INT;
+  sp[0] = FaSpec{FA_KD, 0, YSQL_FLOAT, FA_PRED, j.vals};
+  sp[1] = FaSpec{FA_KI, 1, kdt, FA_OWN, j.keys};
+  sp[2] = FaSpec{FA_KI, 2, kdt, FA_SPEC0 + 1, j.keys};
+}
+
+__global__ __launch_bounds__(kFaBlock) void k_seg_agg(const SegAgg *jobs) {
+  __shared__ ysql_acc s_w[kFaBlock / kWave];
+  const SegAgg j = jobs[blockIdx.x];
+  FaSpec sp[3];
+  seg_agg_specs(j, sp);
+  ysql_acc *const out[3] = {j.accs_out, j.accs_out + 1, j.accs_out + 2};
+  if (j.n == 0) {
+    if (threadIdx.x < 3) {
+      ysql_acc a;
+      fa_identity(a, sp[threadIdx.x]);
+      *out[threadIdx.x] = a;
+    }
+    return;
+  }
+  const uint64_t tile_rows = fa_tile_rows(j.n);
+  const uint32_t ntiles = (uint32_t)ceil_div(j.n, tile_rows);
+  const float sc[1] = {j.thr};
+  for (uint32_t tile = 0; tile < ntiles; tile++)
+    filter_agg_tile<float, YSQL_LT, kFaBlock, 1>(tile, tile_rows, j.vals, sc, 1, sp, 3, j.n, j.partials, s_w);
+  __threadfence();
+  __syncthreads();
+  filter_agg_finish<kFaBlock>(sp, 3, 1, ntiles, j.partials, out, true, s_w);
+}
 
 constexpr int kMaxChains = 64;
 
@@ -91,8 +190,14 @@ struct ysql_batch {
   std::vector<size_t> chain_ws_off, chain_ws_len;
   void *chain_ws = nullptr;
   size_t chain_ws_bytes = 0;
+  // segmented jobs: device tables (largest first) and their workspace
+  std::vector<SegSort> seg_sort32, seg_sort64;
+  std::vector<SegFilter> seg_filter;
+  std::vector<SegAgg> seg_agg;
+  void *seg_dev = nullptr;  // the four tables, then per-job workspace
+  size_t seg_bytes = 0;
   std::vector<hipStream_t> streams;  // streams[0] is the capture stream
-  std::vector<hipEvent_t> events;    // fork, then one join per chain
+  std::vector<hipEvent_t> events;    // fork, then one join per chain and per segmented launch
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
@@ -103,6 +208,7 @@ void ysql_batch_tuning_default(ysql_batch_tuning *t) {
   if (!t) return;
   memset(t, 0, sizeof(*t));
   t->chains = 32;
+  t->seg_sort_max = t->seg_filter_max = t->seg_agg_max = 1 << 19;
   memset(t->sort_variant, -1, sizeof(t->sort_variant));
 }
 
@@ -115,6 +221,7 @@ void ysql_batch_destroy(ysql_batch *b) {
   (void)hipFree(b->d_thr);
   (void)hipFree(b->plain_ws);
   (void)hipFree(b->chain_ws);
+  (void)hipFree(b->seg_dev);
   delete b;
 }
 
@@ -122,7 +229,7 @@ int ysql_batch_info(const ysql_batch *b, uint64_t *tasks, uint32_t *grid, uint64
   if (!b) return YSQL_EINVAL;
   if (tasks) *tasks = b->jobs.size();
   if (grid) *grid = (uint32_t)b->nchains;
-  if (ws_bytes) *ws_bytes = b->chain_ws_bytes + b->plain_ws_bytes;
+  if (ws_bytes) *ws_bytes = b->chain_ws_bytes + b->plain_ws_bytes + b->seg_bytes;
   return YSQL_OK;
 }
 
@@ -173,12 +280,83 @@ int ysql_batch_create(const ysql_job *jobs, int njobs, const ysql_batch_tuning *
   if ((e = hipMalloc(&b->d_thr, thr.size() * 4)) != hipSuccess) return fail(e);
   if ((e = hipMemcpy(b->d_thr, thr.data(), thr.size() * 4, hipMemcpyHostToDevice)) != hipSuccess) return fail(e);
 
+  // ---- segmented jobs (one workgroup each) and their workspace
+  std::vector<int> chained;
+  {
+    std::vector<int> segs;
+    for (int q = 0; q < njobs; q++) {
+      const ysql_job &x = b->jobs[q];
+      const int lim = x.kind == YSQL_JOB_SORT ? tune.seg_sort_max
+                                               : (x.kind == YSQL_JOB_FILTER ? tune.seg_filter_max : tune.seg_agg_max);
+      if (x.n <= (uint64_t)std::max(lim, 0) && lim > 0) segs.push_back(q);
+      else chained.push_back(q);
+    }
+    std::stable_sort(segs.begin(), segs.end(), [&](int a, int c) { return b->jobs[a].n > b->jobs[c].n; });
+    // workspace offsets (256-byte pieces), after the tables
+    size_t wso = 0;
+    std::vector<size_t> wkalt(njobs, 0), wvalt(njobs, 0), wpart(njobs, 0);
+    for (int q : segs) {
+      const ysql_job &x = b->jobs[q];
+      const size_t kb = x.key_dtype == YSQL_UKEY64 ? 8 : 4;
+      if (x.kind == YSQL_JOB_SORT) {
+        wkalt[q] = wso;
+        wso += align_up(std::max<size_t>(x.n * kb, 1), 256);
+        wvalt[q] = wso;
+        wso += align_up(std::max<size_t>(x.n * 4, 1), 256);
+      } else if (x.kind == YSQL_JOB_AGG) {
+        wpart[q] = wso;
+        wso += align_up((ceil_div(x.n, fa_tile_rows(x.n)) * 3 + 1) * sizeof(ysql_acc), 256);
+      }
+    }
+    for (int q : segs) {
+      const ysql_job &x = b->jobs[q];
+      if (x.kind == YSQL_JOB_SORT) {
+        SegSort j{x.keys, x.keys_out, x.perm_out, (void *)wkalt[q], (uint32_t *)wvalt[q], (uint32_t)x.n, 0};
+        (x.key_dtype == YSQL_UKEY64 ? b->seg_sort64 : b->seg_sort32).push_back(j);
+      } else if (x.kind == YSQL_JOB_FILTER) {
+        b->seg_filter.push_back(SegFilter{x.keys, x.vals, x.keys_out, x.vals_out, x.count_out, x.threshold,
+                                          (uint32_t)x.n, x.key_dtype == YSQL_UKEY64 ? 8u : 4u, 0});
+      } else {
+        b->seg_agg.push_back(SegAgg{x.keys, x.vals, x.accs_out, (ysql_acc *)wpart[q], x.threshold, (uint32_t)x.n,
+                                    x.key_dtype == YSQL_UKEY64 ? 1 : 0, 0});
+      }
+    }
+    const size_t t0 = align_up(b->seg_sort32.size() * sizeof(SegSort), 256);
+    const size_t t1 = align_up(b->seg_sort64.size() * sizeof(SegSort), 256);
+    const size_t t2 = align_up(b->seg_filter.size() * sizeof(SegFilter), 256);
+    const size_t t3 = align_up(b->seg_agg.size() * sizeof(SegAgg), 256);
+    const size_t tables = t0 + t1 + t2 + t3;
+    b->seg_bytes = tables + wso;
+    if ((e = hipMalloc(&b->seg_dev, std::max<size_t>(b->seg_bytes, 256))) != hipSuccess) return fail(e);
+    char *base = (char *)b->seg_dev;
+    char *wbase = base + tables;
+    for (auto &j : b->seg_sort32) {
+      j.kalt = wbase + (size_t)j.kalt;
+      j.valt = (uint32_t *)(wbase + (size_t)j.valt);
+    }
+    for (auto &j : b->seg_sort64) {
+      j.kalt = wbase + (size_t)j.kalt;
+      j.valt = (uint32_t *)(wbase + (size_t)j.valt);
+    }
+    for (auto &j : b->seg_agg) j.partials = (ysql_acc *)(wbase + (size_t)j.partials);
+    size_t off = 0;
+    auto put = [&](const void *src, size_t bytes, size_t span) {
+      if (bytes && (e = hipMemcpy(base + off, src, bytes, hipMemcpyHostToDevice)) != hipSuccess) return false;
+      off += span;
+      return true;
+    };
+    if (!put(b->seg_sort32.data(), b->seg_sort32.size() * sizeof(SegSort), t0) ||
+        !put(b->seg_sort64.data(), b->seg_sort64.size() * sizeof(SegSort), t1) ||
+        !put(b->seg_filter.data(), b->seg_filter.size() * sizeof(SegFilter), t2) ||
+        !put(b->seg_agg.data(), b->seg_agg.size() * sizeof(SegAgg), t3))
+      return fail(e);
+  }
+
   // ---- chains: largest job first onto the least-loaded chain
-  const int K = std::max(1, std::min(tune.chains, std::max(1, njobs)));
+  const int K = std::max(1, std::min(tune.chains, std::max(1, (int)chained.size())));
   b->nchains = K;
   b->chain_jobs.assign(K, {});
-  std::vector<int> order(njobs);
-  std::iota(order.begin(), order.end(), 0);
+  std::vector<int> order = chained;
   std::vector<double> cost(njobs);
   for (int q = 0; q < njobs; q++) cost[q] = job_cost(b->jobs[q]);
   std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return cost[a] > cost[c]; });
@@ -199,11 +377,12 @@ int ysql_batch_create(const ysql_job *jobs, int njobs, const ysql_batch_tuning *
   b->chain_ws_bytes = off;
   if ((e = hipMalloc(&b->chain_ws, std::max<size_t>(off, 256))) != hipSuccess) return fail(e);
 
-  // ---- capture: fork -> K chains -> join
-  b->streams.assign(K + 1, nullptr);
+  // ---- capture: fork -> K chains + the segmented launches -> join
+  constexpr int kSegLaunches = 4;
+  b->streams.assign(K + 1 + kSegLaunches, nullptr);
   for (auto &s : b->streams)
     if ((e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking)) != hipSuccess) return fail(e);
-  b->events.assign(K + 1, nullptr);
+  b->events.assign(K + 1 + kSegLaunches, nullptr);
   for (auto &ev : b->events)
     if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess) return fail(e);
   hipStream_t cap = b->streams[0];
@@ -220,6 +399,27 @@ int ysql_batch_create(const ysql_job *jobs, int njobs, const ysql_batch_tuning *
     }
     ok = ok && hipEventRecord(b->events[c + 1], s) == hipSuccess;
     ok = ok && hipStreamWaitEvent(cap, b->events[c + 1], 0) == hipSuccess;
+  }
+  {
+    const char *base = (const char *)b->seg_dev;
+    const size_t t0 = align_up(b->seg_sort32.size() * sizeof(SegSort), 256);
+    const size_t t1 = align_up(b->seg_sort64.size() * sizeof(SegSort), 256);
+    const size_t t2 = align_up(b->seg_filter.size() * sizeof(SegFilter), 256);
+    const size_t counts[kSegLaunches] = {b->seg_sort32.size(), b->seg_sort64.size(), b->seg_filter.size(),
+                                         b->seg_agg.size()};
+    for (int k = 0; k < kSegLaunches && ok && !rc; k++) {
+      if (!counts[k]) continue;
+      hipStream_t s = b->streams[K + 1 + k];
+      ok = hipStreamWaitEvent(s, b->events[0], 0) == hipSuccess;
+      const unsigned g = (unsigned)counts[k];
+      if (k == 0) k_seg_sort<uint32_t><<<g, kSegSortBlock, 0, s>>>((const SegSort *)base);
+      if (k == 1) k_seg_sort<uint64_t><<<g, kSegSortBlock, 0, s>>>((const SegSort *)(base + t0));
+      if (k == 2) k_seg_filter<<<g, kSegFiltBlock, 0, s>>>((const SegFilter *)(base + t0 + t1));
+      if (k == 3) k_seg_agg<<<g, kFaBlock, 0, s>>>((const SegAgg *)(base + t0 + t1 + t2));
+      ok = ok && hipGetLastError() == hipSuccess;
+      ok = ok && hipEventRecord(b->events[K + 1 + k], s) == hipSuccess;
+      ok = ok && hipStreamWaitEvent(cap, b->events[K + 1 + k], 0) == hipSuccess;
+    }
   }
   hipGraph_t g = nullptr;
   e = hipStreamEndCapture(cap, &g);

@@ -351,4 +351,161 @@ __device__ __forceinline__ void small_sort_block(OnesweepSmem<K, BLOCK, KPT> &sm
   }
 }
 
+// One workgroup sorts a whole job of n keys (segmented batches: one job per
+// workgroup, no communication between workgroups).  Up to one tile it is
+// small_sort_block; beyond, it builds every digit's histogram in LDS with one
+// read, then runs the digit passes over the job's tiles in order: a tile is
+// ranked and reordered in LDS like a onesweep tile, but its digit offsets
+// come from running per-digit counters in LDS instead of a look-back.
+// Buffers alternate kin -> alt -> ... so that the last pass lands in
+// kout/vout (kin must not alias kout).  Stable: the same permutation as the
+// multi-workgroup sort.  Every thread of the block calls it.
+template <typename K, int BLOCK, int KPT>
+struct SegSortSmem {
+  OnesweepSmem<K, BLOCK, KPT> os;
+  uint32_t h[8 * kBins];  // per-digit-position histograms, then running offsets
+};
+
+template <typename K, int BLOCK, int KPT>
+__device__ __forceinline__ void seg_sort_block(SegSortSmem<K, BLOCK, KPT> &sm, const K *kin, uint32_t val_base,
+                                               K *kout, uint32_t *vout, K *kalt, uint32_t *valt, uint32_t n,
+                                               int begin_bit, int passes, uint32_t desc) {
+  constexpr int W = BLOCK / kWave;
+  constexpr uint32_t TILE = BLOCK * KPT;
+  if (n <= TILE) {
+    small_sort_block<K, BLOCK, KPT>(sm.os, kin, nullptr, val_base, kout, vout, n, begin_bit, passes, desc);
+    return;
+  }
+  const int t = threadIdx.x;
+  const int lane = lane_id();
+  const int wave = t / kWave;
+  // ---- histograms of every digit position, one read of the keys
+  for (int i = t; i < passes * kBins; i += BLOCK) sm.h[i] = 0;
+  __syncthreads();
+  {
+    uint32_t zc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (uint32_t i = t; i < n; i += BLOCK) hist_key<K>(kin[i], begin_bit, passes, desc, sm.h, zc);
+#pragma unroll
+    for (int p = 0; p < 8; p++) {
+      if (p < passes) {
+        const uint32_t z = wave_sum(zc[p]);
+        if (lane == 0 && z) atomicAdd(&sm.h[p * kBins + desc], z);
+      }
+    }
+  }
+  __syncthreads();
+  // ---- running offsets: exclusive prefix of each pass's histogram
+  for (int p = 0; p < passes; p++) {
+    uint32_t tot;
+    const uint32_t c = t < kBins ? sm.h[p * kBins + t] : 0u;
+    const uint32_t ex = block_exclusive_sum<BLOCK, uint32_t>(c, sm.os.red, &tot);
+    if (t < kBins) sm.h[p * kBins + t] = ex;
+    __syncthreads();
+  }
+  const uint32_t ntiles = (n + TILE - 1) / TILE;
+  uint32_t *s_v = reinterpret_cast<uint32_t *>(sm.os.k);
+  for (int p = 0; p < passes; p++) {
+    const int shift = begin_bit + 8 * p;
+    const bool to_out = ((passes - 1 - p) & 1) == 0;
+    const K *src = p == 0 ? kin : (((passes - p) & 1) == 0 ? kout : kalt);
+    const uint32_t *vsrc = p == 0 ? nullptr : (((passes - p) & 1) == 0 ? vout : valt);
+    K *dst = to_out ? kout : kalt;
+    uint32_t *vdst = to_out ? vout : valt;
+    uint32_t *gofs = sm.h + p * kBins;
+    for (uint32_t tile = 0; tile < ntiles; tile++) {
+      for (int i = t; i < W * kBins; i += BLOCK) (&sm.os.wh[0][0])[i] = 0;
+      __syncthreads();
+      const uint32_t base = tile * TILE;
+      const uint32_t wbase = base + (uint32_t)wave * KPT * kWave;
+      K key[KPT];
+      uint32_t val[KPT], rank[KPT];
+#pragma unroll
+      for (int i = 0; i < KPT; i++) {
+        const uint32_t idx = wbase + i * kWave + lane;
+        const bool valid = idx < n;
+        key[i] = valid ? src[idx] : (K)0;
+        val[i] = vsrc ? (valid ? vsrc[idx] : 0u) : val_base + idx;
+      }
+#pragma unroll
+      for (int i = 0; i < KPT; i++) {
+        const uint32_t idx = wbase + i * kWave + lane;
+        const bool valid = idx < n;
+        const uint32_t d = ((uint32_t)(key[i] >> shift) & 0xFFu) ^ desc;
+        uint64_t m = ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+          const bool bit = (d >> b) & 1u;
+          const uint64_t bal = ballot(bit);
+          m &= bit ? bal : ~bal;
+        }
+        const uint32_t lower = popc_below(m);
+        uint32_t pre = 0;
+        if (valid) pre = sm.os.wh[wave][d];
+        rank[i] = pre + lower;
+        if (valid && lower == 0) sm.os.wh[wave][d] = pre + (uint32_t)__popcll(m);
+      }
+      __syncthreads();
+      uint32_t cnt = 0;
+      if (t < kBins) {
+        uint32_t run = 0;
+#pragma unroll
+        for (int w = 0; w < W; w++) {
+          const uint32_t c = sm.os.wh[w][t];
+          sm.os.wh[w][t] = run;
+          run += c;
+        }
+        cnt = run;
+      }
+      uint32_t tot;
+      const uint32_t dstart = block_exclusive_sum<BLOCK, uint32_t>(t < kBins ? cnt : 0u, sm.os.red, &tot);
+      if (t < kBins) {
+        sm.os.dstart[t] = dstart;
+        sm.os.glob[t] = gofs[t] - dstart;  // wraps like the onesweep form; + j brings it back
+      }
+      __syncthreads();
+      uint32_t pos[KPT];
+#pragma unroll
+      for (int i = 0; i < KPT; i++) {
+        const uint32_t idx = wbase + i * kWave + lane;
+        pos[i] = 0xFFFFFFFFu;
+        if (idx < n) {
+          const uint32_t d = ((uint32_t)(key[i] >> shift) & 0xFFu) ^ desc;
+          pos[i] = sm.os.dstart[d] + sm.os.wh[wave][d] + rank[i];
+          sm.os.k[pos[i]] = key[i];
+        }
+      }
+      __syncthreads();
+      const uint32_t valid_cnt = (n - base) < TILE ? (n - base) : TILE;
+      uint32_t opos[KPT];
+#pragma unroll
+      for (int q = 0; q < KPT; q++) {
+        const uint32_t j = t + q * BLOCK;
+        opos[q] = 0;
+        if (j < valid_cnt) {
+          const K kk = sm.os.k[j];
+          const uint32_t d = ((uint32_t)(kk >> shift) & 0xFFu) ^ desc;
+          opos[q] = sm.os.glob[d] + j;
+          dst[opos[q]] = kk;
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < KPT; i++)
+        if (pos[i] != 0xFFFFFFFFu) s_v[pos[i]] = val[i];
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < KPT; q++) {
+        const uint32_t j = t + q * BLOCK;
+        if (j < valid_cnt) vdst[opos[q]] = s_v[j];
+      }
+      if (t < kBins) gofs[t] += cnt;
+      __syncthreads();
+    }
+    // the next pass reads what this one wrote (other threads' stores, which
+    // may have been cached before they were rewritten)
+    __threadfence();
+    __syncthreads();
+  }
+}
+
 }  // namespace ysql

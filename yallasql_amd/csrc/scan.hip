@@ -188,7 +188,7 @@ __global__ __launch_bounds__(kScanBlock) void k_compact(const uint8_t *mask, con
   const uint32_t tile = take_tile(ctr, &s_tile);
   const PT s = MODE >= 1 ? scalar[0] : PT();
   const PT hi = MODE == 2 ? scalar[1] : PT();
-  compact_tile<kScanBlock, MODE, PT, CMP>(sm, tile, mask, pred, s, hi, cols.c, ncols, n, cap, count_out, status,
+  compact_tile<kScanBlock, MODE, PT, CMP>(sm, tile, mask, pred, s, hi, cols.c, ncols, n, cap, 0, count_out, status,
                                           ntiles);
 }
 

@@ -39,12 +39,14 @@ __device__ __forceinline__ void load16(const PT *p, uint64_t base, uint64_t rows
 
 // MODE 0: selection from mask bytes; MODE 1: pred[i] <CMP> scalar;
 // MODE 2: scalar <= pred[i] < hi.  Output rows at index >= cap are not
-// written; *count_out receives the full count.
+// written; *count_out receives the full count.  status == nullptr: tiles are
+// processed in order by one caller that passes the running prefix (no
+// look-back).  Returns the tile's selected-row count.
 // Every thread of the block calls it; contains barriers.
 template <int BLOCK, int MODE, typename PT, int CMP>
-__device__ __forceinline__ void compact_tile(CompactSmem<BLOCK> &sm, uint32_t tile, const uint8_t *mask,
+__device__ __forceinline__ uint64_t compact_tile(CompactSmem<BLOCK> &sm, uint32_t tile, const uint8_t *mask,
                                              const PT *pred, PT scalar, PT hi, const ysql_col *cols, int ncols,
-                                             uint64_t n, uint64_t cap,
+                                             uint64_t n, uint64_t cap, uint64_t running,
                                              uint64_t *count_out, uint64_t *status, uint32_t ntiles) {
   constexpr int TILE = BLOCK * 16;
   const uint64_t base = (uint64_t)tile * TILE;
@@ -74,7 +76,9 @@ __device__ __forceinline__ void compact_tile(CompactSmem<BLOCK> &sm, uint32_t ti
   uint64_t total;
   const uint32_t excl = (uint32_t)block_exclusive_sum<BLOCK, uint64_t>(cnt, sm.red, &total);
   if (t < kWave) {
-    const uint64_t p = lookback_exclusive(status, tile, total);
+    // with no status array the caller processes tiles in order and passes
+    // the running prefix
+    const uint64_t p = status ? lookback_exclusive(status, tile, total) : running;
     if (t == 0) sm.prefix = p;
   }
   {
@@ -134,6 +138,7 @@ __device__ __forceinline__ void compact_tile(CompactSmem<BLOCK> &sm, uint32_t ti
   }
   if (tile == ntiles - 1 && t == 0 && count_out) *count_out = prefix + total;
   __syncthreads();  // LDS is reused by the caller's next tile
+  return total;
 }
 
 }  // namespace ysql
