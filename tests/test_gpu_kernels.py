@@ -604,9 +604,10 @@ def test_pairs_intersect(ops, dev):
     assert np.array_equal(N(got).view(np.uint32), exp)
 
 
-def test_star_join_group_sum(ops, dev):
+@pytest.mark.parametrize("n,off", [(2_000_000, 0), (1_000_003, 1)])
+def test_star_join_group_sum(ops, dev, n, off):
+    """off = 1: every fact column is an unaligned device view with a ragged tail."""
     r = rng(14)
-    n = 2_000_000
     dims_np = []
     rows = [2556, 2000, 30000, 200000]
     fks = [r.integers(0, R, n).astype(np.int32) for R in rows]
@@ -622,11 +623,16 @@ def test_star_join_group_sum(ops, dev):
     ]
     m1 = r.integers(90000, 10000001, n).astype(np.int32)
     m2 = r.integers(0, 11, n).astype(np.int32)
+    def fact(v):  # device column, shifted by `off` elements when requested
+        if not off:
+            return T(v, dev)
+        return T(np.concatenate([np.zeros(off, v.dtype), v]), dev)[off:]
+
     dims_t = []
     for d in dims_np:
-        dt_ = {k: (T(v, dev) if isinstance(v, np.ndarray) else v) for k, v in d.items()}
+        dt_ = {k: ((fact(v) if k == "fk" else T(v, dev)) if isinstance(v, np.ndarray) else v) for k, v in d.items()}
         dims_t.append(dt_)
-    sums, counts = ops.star_join_group_sum(dims_t, T(m1, dev), T(m2, dev), 7000)
+    sums, counts = ops.star_join_group_sum(dims_t, fact(m1), fact(m2), 7000)
     es, ec = O.star_join_group_sum(dims_np, m1, m2, 7000)
     assert np.array_equal(N(sums), es)
     assert np.array_equal(N(counts).astype(np.uint64), ec)

@@ -457,6 +457,27 @@ struct StarArgs {
 
 constexpr uint32_t kStarLdsMax = 8192;
 
+// 4 consecutive int32 values from row r (rows >= n read 0): one 128-bit
+// load when whole and aligned.
+__device__ __forceinline__ void star_load4(const int32_t *p, uint64_t r, uint64_t n, int32_t (&v)[4]) {
+  if (r + 4 <= n && ((uintptr_t)(p + r) & 15) == 0) {
+    const int4 q = *reinterpret_cast<const int4 *>(p + r);
+    v[0] = q.x;
+    v[1] = q.y;
+    v[2] = q.z;
+    v[3] = q.w;
+  } else {
+#pragma unroll
+    for (int e = 0; e < 4; e++) v[e] = r + e < n ? p[r + e] : 0;
+  }
+}
+
+// Each thread takes 4 consecutive fact rows per step: every fact column is
+// read with one 128-bit load; then every dimension's filter byte is looked
+// up (the 4 rows' gathers of a dimension together, only for rows still
+// qualifying), and the group values only for rows that pass every filter.
+// Qualifying rows fold into per-block LDS group arrays (LDS) or straight
+// into the global ones.
 template <bool LDS>
 __global__ __launch_bounds__(1024) void k_star(StarArgs a, int nd, const int32_t *__restrict__ m1,
                                                const int32_t *__restrict__ m2, uint64_t n, int64_t *sums,
@@ -470,30 +491,61 @@ __global__ __launch_bounds__(1024) void k_star(StarArgs a, int nd, const int32_t
     }
     __syncthreads();
   }
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
-    uint32_t g = 0;
-    bool ok = true;
+  const uint64_t step = (uint64_t)gridDim.x * blockDim.x * 4;
+  for (uint64_t r = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4; r < n; r += step) {
+    int32_t f[YSQL_MAX_DIMS][4], x1[4], x2[4];
+#pragma unroll
+    for (int d = 0; d < YSQL_MAX_DIMS; d++)
+      if (d < nd) star_load4(a.d[d].fk, r, n, f[d]);
+    star_load4(m1, r, n, x1);
+    star_load4(m2, r, n, x2);
+    bool ok[4];
+#pragma unroll
+    for (int e = 0; e < 4; e++) ok[e] = r + e < n;
+    // in-range checks, then all gathers
+#pragma unroll
+    for (int d = 0; d < YSQL_MAX_DIMS; d++)
+      if (d < nd) {
+#pragma unroll
+        for (int e = 0; e < 4; e++) ok[e] = ok[e] && f[d][e] >= 0 && (uint32_t)f[d][e] < a.d[d].rows;
+      }
+    // every dimension's filter first, then the group lookups of the rows
+    // that pass them all; the 4 rows' gathers of one dimension go together
 #pragma unroll
     for (int d = 0; d < YSQL_MAX_DIMS; d++) {
-      if (d < nd && ok) {
-        const int32_t f = a.d[d].fk[i];
-        if (f < 0 || (uint32_t)f >= a.d[d].rows) ok = false;
-        else if (a.d[d].pass && !a.d[d].pass[f]) ok = false;
-        else if (a.d[d].group) {
-          const int64_t v = (int64_t)a.d[d].group[f] - a.d[d].group_base;
-          if (v < 0 || v >= (int64_t)a.d[d].group_card) ok = false;
-          else g += (uint32_t)v * a.stride[d];
-        }
+      if (d >= nd || !a.d[d].pass) continue;
+      uint8_t pv[4];
+#pragma unroll
+      for (int e = 0; e < 4; e++) pv[e] = ok[e] ? a.d[d].pass[f[d][e]] : (uint8_t)0;
+#pragma unroll
+      for (int e = 0; e < 4; e++) ok[e] = ok[e] && pv[e] != 0;
+    }
+    uint32_t g[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int d = 0; d < YSQL_MAX_DIMS; d++) {
+      if (d >= nd || !a.d[d].group) continue;
+      int32_t gv[4];
+#pragma unroll
+      for (int e = 0; e < 4; e++) gv[e] = ok[e] ? a.d[d].group[f[d][e]] : a.d[d].group_base;
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        const int64_t v = (int64_t)gv[e] - a.d[d].group_base;
+        ok[e] = ok[e] && v >= 0 && v < (int64_t)a.d[d].group_card;
+        g[e] += (uint32_t)v * a.stride[d];
       }
     }
-    if (!ok) continue;
-    const long long prod = (long long)m1[i] * (long long)m2[i];
-    if (LDS) {
-      atomicAdd(&s_sum[g], (unsigned long long)prod);
-      atomicAdd(&s_cnt[g], 1u);
-    } else {
-      atomicAdd((unsigned long long *)&sums[g], (unsigned long long)prod);
-      atomicAdd((unsigned long long *)&counts[g], 1ull);
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      const bool q = ok[e];
+      if (!q) continue;
+      const long long prod = (long long)x1[e] * (long long)x2[e];
+      if (LDS) {
+        atomicAdd(&s_sum[g[e]], (unsigned long long)prod);
+        atomicAdd(&s_cnt[g[e]], 1u);
+      } else {
+        atomicAdd((unsigned long long *)&sums[g[e]], (unsigned long long)prod);
+        atomicAdd((unsigned long long *)&counts[g[e]], 1ull);
+      }
     }
   }
   if (LDS) {
@@ -642,7 +694,7 @@ int ysql_star_join_group_sum(const ysql_star_dim *dims, int ndims, const int32_t
   if (n == 0) return YSQL_OK;
   hipStream_t s = as_stream(stream);
   uint64_t g = ceil_div(n, 1024 * 16);
-  const uint64_t cap = (uint64_t)grid_cap();
+  const uint64_t cap = (uint64_t)grid_cap() * (n_groups <= kStarLdsMax ? 1 : 2);
   if (g > cap) g = cap;
   if (g < 1) g = 1;
   if (n_groups <= kStarLdsMax)
