@@ -3,12 +3,13 @@
 //
 // SELECT agg_1(c_1), ..., agg_k(c_k) WHERE pred <cmp> scalar in one read of
 // the columns.  The input is cut into canonical tiles (fa_tile_rows(n) rows); a
-// tile's partial is folded in a fixed order (thread t folds its runs of
-// kFaRpt rows in row order, then a fixed wave64 butterfly, then the 4 waves
-// in order) and the partials are combined in a fixed order, so float sums are a
+// tile's partial is folded in a fixed order (each thread folds its rows in a
+// fixed order, then a fixed wave64 butterfly, then the 4 waves in order) and the partials are combined in a fixed order, so float sums are a
 // function of the input alone, whatever the grid and whichever caller
 // launches the tiles.
 #pragma once
+
+#include <type_traits>
 
 #include "common.hpp"
 
@@ -27,7 +28,8 @@ __host__ __device__ inline uint64_t fa_tile_rows(uint64_t n) {
 constexpr int kFaBlock = 256;
 constexpr int kFaMaxSpecs = 4;
 constexpr int kFaMaxQueries = 4;  // queries sharing one scan
-constexpr int kFaRpt = 8;  // rows per thread per step
+constexpr int kFaRpt = 8;   // rows per thread per step
+constexpr int kFaPair = 2;  // consecutive rows per lane per load (dense 8/16-byte loads)
 
 // Accumulator kinds: int64 in .i, float in .f (MIN/MAX), double in .d (SUM),
 // count only.
@@ -47,6 +49,8 @@ __host__ __device__ inline int fa_kind(int dtype, int agg) {
   return FA_KI;
 }
 __host__ __device__ inline int fa_op(int agg) { return agg == YSQL_AGG_MIN ? 1 : (agg == YSQL_AGG_MAX ? 2 : 0); }
+// Element width of a spec's column: 8 bytes for int64, else 4.
+__host__ __device__ inline bool fa_wide(const FaSpec &s) { return s.kind == FA_KI && s.dtype != YSQL_INT; }
 
 __device__ __forceinline__ void fa_identity(ysql_acc &a, const FaSpec &s) {
   a.i = 0;
@@ -75,11 +79,9 @@ __device__ __forceinline__ void fa_combine(ysql_acc &a, const ysql_acc &b, const
   }
 }
 
-// Fixed-shape block reduction (BLOCK threads); result in every thread.
+// Fixed wave64 butterfly over a spec's accumulators; result in every lane.
 // Only the count and the field of the spec's kind travel.
-template <int BLOCK>
-__device__ __forceinline__ ysql_acc fa_block_reduce(ysql_acc v, const FaSpec &s, ysql_acc *s_w) {
-  constexpr int W = BLOCK / kWave;
+__device__ __forceinline__ ysql_acc fa_wave_reduce(ysql_acc v, const FaSpec &s) {
 #pragma unroll
   for (int m = 32; m >= 1; m >>= 1) {
     ysql_acc o = v;
@@ -89,6 +91,15 @@ __device__ __forceinline__ ysql_acc fa_block_reduce(ysql_acc v, const FaSpec &s,
     else if (s.kind == FA_KF) o.f = shfl_xor_t(v.f, m);
     fa_combine(v, o, s);
   }
+  return v;
+}
+
+// Fixed-shape block reduction (BLOCK threads): the wave butterfly, then the
+// waves in order.  Result in every thread.
+template <int BLOCK>
+__device__ __forceinline__ ysql_acc fa_block_reduce(ysql_acc v, const FaSpec &s, ysql_acc *s_w) {
+  constexpr int W = BLOCK / kWave;
+  v = fa_wave_reduce(v, s);
   if (lane_id() == 0) s_w[threadIdx.x / kWave] = v;
   __syncthreads();
   ysql_acc r = s_w[0];
@@ -98,73 +109,75 @@ __device__ __forceinline__ ysql_acc fa_block_reduce(ysql_acc v, const FaSpec &s,
   return r;
 }
 
-// kFaRpt consecutive values from row r0 (rows >= end read as T()): 128-bit
-// loads when the run is whole and 16-byte aligned.
-template <typename T>
-__device__ __forceinline__ void fa_load_rows(const T *p, uint64_t r0, uint64_t end, T (&v)[kFaRpt]) {
-  if (r0 + kFaRpt <= end && ((uintptr_t)(p + r0) & 15) == 0) {
-    constexpr int NQ = kFaRpt * sizeof(T) / 16;
-    uint4 q[NQ];
-    const uint4 *src = reinterpret_cast<const uint4 *>(p + r0);
-#pragma unroll
-    for (int i = 0; i < NQ; i++) q[i] = src[i];
-    memcpy(&v[0], &q[0], sizeof(q));
-  } else {
-#pragma unroll
-    for (int e = 0; e < kFaRpt; e++) v[e] = (r0 + e < end) ? p[r0 + e] : T();
+// Rows of a tile.  The tile is walked in steps of BLOCK*kFaRpt rows; in
+// every step, virtual thread v < BLOCK owns rows step + u*BLOCK*2 + 2v +
+// {0, 1} (u < kFaRpt/2), so at every load the 64 lanes of a wave read one
+// contiguous range.  A virtual thread folds its rows in that order, steps
+// in order, into one running value per (query, spec); a tile's partial is
+// the wave64 butterfly over each group of 64 virtual threads followed by the
+// four groups in order.  A query's fold sequence does not depend on the
+// other queries or specs of the launch, so its result is a function of n and
+// its own inputs alone: the same alone or sharing a scan, whichever kernel
+// runs the tile.
+
+// Fold bodies, one per (accumulator kind, operator, column width).
+enum {
+  FA_OPC_NONE = 0,  // COUNT: the per-query row count carries it
+  FA_OPC_I32_SUM,
+  FA_OPC_I32_MIN,
+  FA_OPC_I32_MAX,
+  FA_OPC_I64_SUM,
+  FA_OPC_I64_MIN,
+  FA_OPC_I64_MAX,
+  FA_OPC_F_MIN,
+  FA_OPC_F_MAX,
+  FA_OPC_D_SUM
+};
+
+__host__ __device__ inline int fa_opcode(const FaSpec &s) {
+  if (s.kind == FA_KI) {
+    const int base = s.dtype == YSQL_INT ? FA_OPC_I32_SUM : FA_OPC_I64_SUM;
+    return base + s.op;
   }
+  if (s.kind == FA_KF) return s.op == 1 ? FA_OPC_F_MIN : FA_OPC_F_MAX;
+  if (s.kind == FA_KD) return FA_OPC_D_SUM;
+  return FA_OPC_NONE;
 }
+constexpr __host__ __device__ inline bool fa_opc_wide(int opc) { return opc >= FA_OPC_I64_SUM && opc <= FA_OPC_I64_MAX; }
 
-// Element width of a spec's column: 8 bytes for int64, else 4.
-__host__ __device__ inline bool fa_wide(const FaSpec &s) { return s.kind == FA_KI && s.dtype != YSQL_INT; }
+// Column element type of an opcode (4 bytes for COUNT, which loads none).
+template <int OPC>
+using fa_col_t = typename std::conditional<(OPC >= FA_OPC_I64_SUM && OPC <= FA_OPC_I64_MAX), uint64_t,
+                                           uint32_t>::type;
 
-// The kFaRpt values of a spec's column at rows r0.. as 32-bit words (element e
-// in w[e], or in w[2e], w[2e+1] when 8 bytes wide); rows >= end read 0.
-__device__ __forceinline__ void fa_load_words(const FaSpec &s, uint64_t r0, uint64_t end, uint32_t (&w)[2 * kFaRpt]) {
-  const bool full = r0 + kFaRpt <= end;
-  if (fa_wide(s)) {
-    const uint64_t *p = (const uint64_t *)s.col + r0;
-    if (full && ((uintptr_t)p & 15) == 0) {
-      const uint4 *q = reinterpret_cast<const uint4 *>(p);
-#pragma unroll
-      for (int i = 0; i < kFaRpt / 2; i++) {
-        const uint4 x = q[i];
-        w[4 * i] = x.x;
-        w[4 * i + 1] = x.y;
-        w[4 * i + 2] = x.z;
-        w[4 * i + 3] = x.w;
-      }
-    } else {
-#pragma unroll
-      for (int e = 0; e < kFaRpt; e++) {
-        const uint64_t x = r0 + e < end ? p[e] : 0ull;
-        w[2 * e] = (uint32_t)x;
-        w[2 * e + 1] = (uint32_t)(x >> 32);
-      }
-    }
-  } else {
-    const uint32_t *p = (const uint32_t *)s.col + r0;
-    if (full && ((uintptr_t)p & 15) == 0) {
-      const uint4 *q = reinterpret_cast<const uint4 *>(p);
-#pragma unroll
-      for (int i = 0; i < kFaRpt / 4; i++) {
-        const uint4 x = q[i];
-        w[4 * i] = x.x;
-        w[4 * i + 1] = x.y;
-        w[4 * i + 2] = x.z;
-        w[4 * i + 3] = x.w;
-      }
-    } else {
-#pragma unroll
-      for (int e = 0; e < kFaRpt; e++) w[e] = r0 + e < end ? p[e] : 0u;
-    }
+// Folds one value into a running value; sel false leaves it bit-for-bit
+// unchanged (a double sum that starts at +0.0 never becomes -0.0, so adding
+// +0.0 is exact).
+template <int OPC, typename VT>
+__device__ __forceinline__ void fa_fold1(uint64_t &val, VT raw, bool sel) {
+  if constexpr (OPC == FA_OPC_I32_SUM || OPC == FA_OPC_I64_SUM) {
+    const uint64_t x = OPC == FA_OPC_I64_SUM ? (uint64_t)raw : (uint64_t)(int64_t)(int32_t)(uint32_t)raw;
+    val += sel ? x : 0ull;
+  } else if constexpr (OPC == FA_OPC_I32_MIN || OPC == FA_OPC_I32_MAX || OPC == FA_OPC_I64_MIN ||
+                       OPC == FA_OPC_I64_MAX) {
+    constexpr bool wide = OPC == FA_OPC_I64_MIN || OPC == FA_OPC_I64_MAX;
+    constexpr bool mn = OPC == FA_OPC_I32_MIN || OPC == FA_OPC_I64_MIN;
+    const int64_t x = wide ? (int64_t)raw : (int64_t)(int32_t)(uint32_t)raw;
+    const int64_t m = (int64_t)val;
+    val = (sel && (mn ? x < m : x > m)) ? (uint64_t)x : val;
+  } else if constexpr (OPC == FA_OPC_F_MIN || OPC == FA_OPC_F_MAX) {
+    const float x = __uint_as_float((uint32_t)raw), m = __uint_as_float((uint32_t)val);
+    const float r = OPC == FA_OPC_F_MIN ? fminf(m, x) : fmaxf(m, x);
+    val = sel ? (uint64_t)__float_as_uint(r) : val;
+  } else if constexpr (OPC == FA_OPC_D_SUM) {
+    double d = __longlong_as_double((long long)val);
+    d += (double)(sel ? __uint_as_float((uint32_t)raw) : 0.f);
+    val = (uint64_t)__double_as_longlong(d);
   }
 }
 
 // Per-thread running value of one spec, one 64-bit word: the int64 (KI),
-// the double's bits (KD) or the float's bits (KF).  The row count is kept
-// once per query: without null masks every spec of a query folds the same
-// rows.
+// the double's bits (KD) or the float's bits (KF).
 __device__ __forceinline__ uint64_t fa_val_identity(const FaSpec &s) {
   ysql_acc a;
   fa_identity(a, s);
@@ -183,61 +196,148 @@ __device__ __forceinline__ ysql_acc fa_val_to_acc(uint64_t v, uint32_t count, co
   return a;
 }
 
-// Folds the selected rows (bit e of `flags` = row r0 + e) of one spec, in
-// row order, from its loaded words.  Unselected rows fold the operator's
-// identity (0 for SUM, the current value for MIN/MAX), which leaves the
-// value bit-for-bit as if they were skipped.  One short body per
-// accumulator kind keeps the kernel small enough for the instruction cache.
-__device__ __forceinline__ void fa_fold_words(uint64_t &val, const FaSpec &s, const uint32_t (&w)[2 * kFaRpt],
-                                              uint32_t flags) {
-  if (s.kind == FA_KI) {
-    const bool wide = s.dtype != YSQL_INT;
-    int64_t x[kFaRpt];
+// ---------------------------------------------------------------------------
+// The tile kernel: each thread folds every spec of every query (up to QMAX
+// sharing the scan) for its virtual thread, kFaRpt rows per step.
+// ---------------------------------------------------------------------------
+
+// The kFaRpt values of a column at virtual thread r0's rows: pair u at
+// r0 + u * stride.  FULL: every row < end and the column aligned to two
+// elements (one vector load per pair); otherwise guarded, rows >= end read 0.
+template <bool FULL, typename T>
+__device__ __forceinline__ void fa_load_rows(const T *p, uint64_t r0, uint64_t stride, uint64_t end,
+                                             T (&v)[kFaRpt]) {
 #pragma unroll
-    for (int e = 0; e < kFaRpt; e++)
-      x[e] = wide ? (int64_t)((uint64_t)w[2 * e] | ((uint64_t)w[2 * e + 1] << 32)) : (int64_t)(int32_t)w[e];
-    if (s.op == 0) {
-      uint64_t sum = val;
-#pragma unroll
-      for (int e = 0; e < kFaRpt; e++) sum += ((flags >> e) & 1u) ? (uint64_t)x[e] : 0ull;
-      val = sum;
-    } else {
-      const bool mn = s.op == 1;
-      int64_t m = (int64_t)val;
-#pragma unroll
-      for (int e = 0; e < kFaRpt; e++) {
-        const bool better = mn ? x[e] < m : x[e] > m;
-        m = (((flags >> e) & 1u) && better) ? x[e] : m;
+  for (int u = 0; u < kFaRpt / kFaPair; u++) {
+    const uint64_t r = r0 + u * stride;
+    if constexpr (FULL) {
+      if constexpr (sizeof(T) == 8) {
+        const uint4 q = *static_cast<const uint4 *>(__builtin_assume_aligned(p + r, 16));
+        const uint64_t x = (uint64_t)q.x | ((uint64_t)q.y << 32), y = (uint64_t)q.z | ((uint64_t)q.w << 32);
+        memcpy(&v[2 * u], &x, 8);
+        memcpy(&v[2 * u + 1], &y, 8);
+      } else {
+        const uint2 q = *static_cast<const uint2 *>(__builtin_assume_aligned(p + r, 8));
+        memcpy(&v[2 * u], &q.x, 4);
+        memcpy(&v[2 * u + 1], &q.y, 4);
       }
-      val = (uint64_t)m;
+    } else {
+      v[2 * u] = r < end ? p[r] : T();
+      v[2 * u + 1] = r + 1 < end ? p[r + 1] : T();
     }
-  } else if (s.kind == FA_KF) {
-    const bool mn = s.op == 1;
-    float m = __uint_as_float((uint32_t)val);
+  }
+}
+
+// A spec's kFaRpt values as words: row k in w[k], or w[2k], w[2k + 1] when wide.
+template <bool FULL>
+__device__ __forceinline__ void fa_load_words(const void *col, bool wide, uint64_t r0, uint64_t stride, uint64_t end,
+                                              uint32_t (&w)[2 * kFaRpt]) {
+  if (wide) {
+    uint64_t v[kFaRpt];
+    fa_load_rows<FULL, uint64_t>((const uint64_t *)col, r0, stride, end, v);
 #pragma unroll
-    for (int e = 0; e < kFaRpt; e++) {
-      const float x = __uint_as_float(w[e]);
-      const float r = mn ? fminf(m, x) : fmaxf(m, x);
-      m = ((flags >> e) & 1u) ? r : m;
+    for (int k = 0; k < kFaRpt; k++) {
+      w[2 * k] = (uint32_t)v[k];
+      w[2 * k + 1] = (uint32_t)(v[k] >> 32);
     }
-    val = (uint64_t)__float_as_uint(m);
-  } else if (s.kind == FA_KD) {
-    double d = __longlong_as_double((long long)val);
+  } else {
+    uint32_t v[kFaRpt];
+    fa_load_rows<FULL, uint32_t>((const uint32_t *)col, r0, stride, end, v);
 #pragma unroll
-    for (int e = 0; e < kFaRpt; e++) d += ((flags >> e) & 1u) ? (double)__uint_as_float(w[e]) : 0.0;
-    val = (uint64_t)__double_as_longlong(d);
+    for (int k = 0; k < kFaRpt; k++) w[k] = v[k];
+  }
+}
+
+template <int OPC>
+__device__ __forceinline__ void fa_fold_words_op(uint64_t &val, const uint32_t (&w)[2 * kFaRpt],
+                                                 const bool (&sel)[kFaRpt]) {
+#pragma unroll
+  for (int k = 0; k < kFaRpt; k++) {
+    if constexpr (fa_opc_wide(OPC))
+      fa_fold1<OPC>(val, (uint64_t)w[2 * k] | ((uint64_t)w[2 * k + 1] << 32), sel[k]);
+    else
+      fa_fold1<OPC>(val, w[k], sel[k]);
+  }
+}
+
+// One body per opcode, chosen by a wave-uniform switch: no body evaluates
+// another's operations.
+__device__ __forceinline__ void fa_fold_words(uint64_t &val, int opc, const uint32_t (&w)[2 * kFaRpt],
+                                              const bool (&sel)[kFaRpt]) {
+  switch (opc) {
+    case FA_OPC_I32_SUM: fa_fold_words_op<FA_OPC_I32_SUM>(val, w, sel); break;
+    case FA_OPC_I32_MIN: fa_fold_words_op<FA_OPC_I32_MIN>(val, w, sel); break;
+    case FA_OPC_I32_MAX: fa_fold_words_op<FA_OPC_I32_MAX>(val, w, sel); break;
+    case FA_OPC_I64_SUM: fa_fold_words_op<FA_OPC_I64_SUM>(val, w, sel); break;
+    case FA_OPC_I64_MIN: fa_fold_words_op<FA_OPC_I64_MIN>(val, w, sel); break;
+    case FA_OPC_I64_MAX: fa_fold_words_op<FA_OPC_I64_MAX>(val, w, sel); break;
+    case FA_OPC_F_MIN: fa_fold_words_op<FA_OPC_F_MIN>(val, w, sel); break;
+    case FA_OPC_F_MAX: fa_fold_words_op<FA_OPC_F_MAX>(val, w, sel); break;
+    case FA_OPC_D_SUM: fa_fold_words_op<FA_OPC_D_SUM>(val, w, sel); break;
+    default: break;
+  }
+}
+
+// One step of BLOCK*kFaRpt rows from s0: issues every load (the predicate
+// and each spec column that is not the predicate column or an earlier
+// spec's column), then evaluates each query's predicate and folds the
+// thread's rows in row order.
+template <bool FULL, typename PT, int CMP, int BLOCK, int QMAX>
+__device__ __forceinline__ void fa_step(uint64_t s0, uint64_t end, const PT *pred, const PT (&scalar)[QMAX], int nq,
+                                        const FaSpec (&sp)[kFaMaxSpecs], const int (&opc)[kFaMaxSpecs], int ns,
+                                        uint64_t (&val)[QMAX][kFaMaxSpecs], uint32_t (&cnt)[QMAX]) {
+  constexpr uint64_t stride = (uint64_t)BLOCK * kFaPair;
+  const uint64_t r0 = s0 + (uint64_t)threadIdx.x * kFaPair;
+  PT pv[kFaRpt];
+  fa_load_rows<FULL, PT>(pred, r0, stride, end, pv);
+  uint32_t w[kFaMaxSpecs][2 * kFaRpt];
+#pragma unroll
+  for (int s = 0; s < kFaMaxSpecs; s++)
+    if (s < ns && opc[s] != FA_OPC_NONE && sp[s].src == FA_OWN)
+      fa_load_words<FULL>(sp[s].col, fa_opc_wide(opc[s]), r0, stride, end, w[s]);
+  // specs over the predicate column or an earlier spec's column take a copy
+  // of those words (register moves, no second load)
+  {
+    uint32_t pw[2 * kFaRpt];
+    memcpy(pw, pv, sizeof(pv));
+#pragma unroll
+    for (int s = 0; s < kFaMaxSpecs; s++) {
+      if (s >= ns) continue;
+      const int src = sp[s].src;
+      if (src == FA_PRED) {
+#pragma unroll
+        for (int i = 0; i < 2 * kFaRpt; i++) w[s][i] = i < (int)(sizeof(pv) / 4) ? pw[i] : 0u;
+      }
+#pragma unroll
+      for (int t = 0; t < s; t++)
+        if (src == FA_SPEC0 + t) {
+#pragma unroll
+          for (int i = 0; i < 2 * kFaRpt; i++) w[s][i] = w[t][i];
+        }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < QMAX; q++) {
+    if (q >= nq) break;
+    bool sel[kFaRpt];
+#pragma unroll
+    for (int k = 0; k < kFaRpt; k++) {
+      sel[k] = cmp_static<CMP>(pv[k], scalar[q]);
+      if constexpr (!FULL) sel[k] = sel[k] && r0 + (uint64_t)(k >> 1) * stride + (k & 1) < end;
+      cnt[q] += sel[k] ? 1u : 0u;
+    }
+#pragma unroll
+    for (int s = 0; s < kFaMaxSpecs; s++)
+      if (s < ns) fa_fold_words(val[q][s], opc[s], w[s], sel);
   }
 }
 
 // Partials of canonical tile `tile` for nq <= QMAX queries that share the
 // predicate column, the comparison and the spec columns and differ in the
-// constant: partials[(tile * nq + q) * ns + s].  Thread t owns rows
-// base + t*kFaRpt .. of every step of BLOCK*kFaRpt rows.  Each step first
-// issues every load (the predicate and each spec column that is not the
-// predicate column, 128-bit loads), then evaluates each query's predicate
-// and folds its rows in row order.  A query's fold sequence does not depend
-// on the other queries of the launch, so its result is the same alone or
-// batched.
+// constant: partials[(tile * nq + q) * ns + s].  Whole steps over aligned
+// columns take unguarded vector loads; the rest (the last, partial step of
+// the input, or columns not aligned to two elements) the guarded path, with
+// the same rows per thread and the same fold order.
 template <typename PT, int CMP, int BLOCK, int QMAX>
 __device__ __forceinline__ void filter_agg_tile(uint32_t tile, uint64_t tile_rows, const PT *pred,
                                                 const PT (&scalar)[QMAX], int nq, const FaSpec *specs, int ns,
@@ -247,56 +347,28 @@ __device__ __forceinline__ void filter_agg_tile(uint32_t tile, uint64_t tile_row
   uint64_t val[QMAX][kFaMaxSpecs];
   uint32_t cnt[QMAX];
   FaSpec sp[kFaMaxSpecs];
+  int opc[kFaMaxSpecs];
+  bool aligned = ((uintptr_t)pred & (2 * sizeof(PT) - 1)) == 0;
 #pragma unroll
   for (int s = 0; s < kFaMaxSpecs; s++) {
+    opc[s] = FA_OPC_NONE;
     if (s < ns) {
       sp[s] = specs[s];
+      opc[s] = fa_opcode(sp[s]);
+      if (opc[s] != FA_OPC_NONE && sp[s].src == FA_OWN)
+        aligned = aligned && ((uintptr_t)sp[s].col & (fa_opc_wide(opc[s]) ? 15 : 7)) == 0;
 #pragma unroll
       for (int q = 0; q < QMAX; q++) val[q][s] = fa_val_identity(sp[s]);
     }
   }
 #pragma unroll
   for (int q = 0; q < QMAX; q++) cnt[q] = 0;
-  for (uint64_t r0 = base + (uint64_t)threadIdx.x * kFaRpt; r0 < end; r0 += (uint64_t)BLOCK * kFaRpt) {
-    PT pv[kFaRpt];
-    fa_load_rows<PT>(pred, r0, end, pv);
-    uint32_t w[kFaMaxSpecs][2 * kFaRpt];
-#pragma unroll
-    for (int s = 0; s < kFaMaxSpecs; s++)
-      if (s < ns && sp[s].kind != FA_KC && sp[s].src == FA_OWN) fa_load_words(sp[s], r0, end, w[s]);
-    // specs over the predicate column or an earlier spec's column take a
-    // copy of those words (register moves, no second load)
-    {
-      uint32_t pw[2 * kFaRpt];
-      memcpy(pw, pv, sizeof(pv));
-#pragma unroll
-      for (int s = 0; s < kFaMaxSpecs; s++) {
-        if (s >= ns) continue;
-        const int src = sp[s].src;
-        if (src == FA_PRED) {
-#pragma unroll
-          for (int i = 0; i < 2 * kFaRpt; i++) w[s][i] = i < (int)(sizeof(pv) / 4) ? pw[i] : 0u;
-        }
-#pragma unroll
-        for (int t = 0; t < s; t++)
-          if (src == FA_SPEC0 + t) {
-#pragma unroll
-            for (int i = 0; i < 2 * kFaRpt; i++) w[s][i] = w[t][i];
-          }
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < QMAX; q++) {
-      if (q >= nq) break;
-      uint32_t flags = 0;
-#pragma unroll
-      for (int e = 0; e < kFaRpt; e++) flags |= (uint32_t)(r0 + e < end && cmp_static<CMP>(pv[e], scalar[q])) << e;
-      cnt[q] += __builtin_popcount(flags);
-#pragma unroll
-      for (int s = 0; s < kFaMaxSpecs; s++)
-        if (s < ns) fa_fold_words(val[q][s], sp[s], w[s], flags);
-    }
-  }
+  constexpr uint64_t step = (uint64_t)BLOCK * kFaRpt;
+  uint64_t s0 = base;
+  if (aligned)
+    for (; s0 + step <= end; s0 += step)
+      fa_step<true, PT, CMP, BLOCK, QMAX>(s0, end, pred, scalar, nq, sp, opc, ns, val, cnt);
+  for (; s0 < end; s0 += step) fa_step<false, PT, CMP, BLOCK, QMAX>(s0, end, pred, scalar, nq, sp, opc, ns, val, cnt);
 #pragma unroll
   for (int q = 0; q < QMAX; q++) {
     if (q >= nq) break;
@@ -309,6 +381,12 @@ __device__ __forceinline__ void filter_agg_tile(uint32_t tile, uint64_t tile_row
     }
   }
 }
+
+// Kernel arguments of a filtered multi-aggregate launch.
+struct FaArgs {
+  FaSpec spec[kFaMaxSpecs];
+  ysql_acc *out[kFaMaxQueries * kFaMaxSpecs];  // [query][spec]
+};
 
 // Combines the ntiles partials of every (query, spec) in a fixed order; folds
 // the totals into out[q * ns + s] (accumulators that persist across calls),

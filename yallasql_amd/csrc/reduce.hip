@@ -714,11 +714,6 @@ int ysql_star_join_group_sum(const ysql_star_dim *dims, int ndims, const int32_t
 
 namespace ysql {
 
-struct FaArgs {
-  FaSpec spec[kFaMaxSpecs];
-  ysql_acc *out[kFaMaxQueries * kFaMaxSpecs];  // [query][spec]
-};
-
 // QMAX = 1 for single queries (fewer registers, higher occupancy), 4 for
 // shared scans; a query's fold sequence is the same in both.
 template <typename PT, int CMP, int QMAX>
