@@ -73,7 +73,7 @@ __global__ __launch_bounds__(kSegSortBlock) void k_seg_sort(const SegSort *jobs)
 }
 
 __global__ __launch_bounds__(kSegFiltBlock) void k_seg_filter(const SegFilter *jobs) {
-  __shared__ CompactSmem<kSegFiltBlock> sm;
+  __shared__ CompactSmem<kSegFiltBlock, 4> sm;
   const SegFilter j = jobs[blockIdx.x];
   if (j.n == 0) {
     if (threadIdx.x == 0) *j.count_out = 0;

@@ -7,18 +7,25 @@
 // prefix by decoupled look-back over 64-bit status words, the selected rows'
 // tile offsets are listed in LDS in order, and every column is copied with
 // one output row per thread per step (coalesced stores, loads inside the
-// tile's cache lines).  Stable: selected rows keep their order.
+// tile's cache lines).  An output column that is the predicate column is
+// copied from the predicate values kept in LDS (no second read of it).
+// Stable: selected rows keep their order.
 #pragma once
+
+#include <type_traits>
 
 #include "common.hpp"
 
 namespace ysql {
 
-template <int BLOCK>
+// VB: bytes of a predicate value kept in LDS (0: none, mask selection).
+template <int BLOCK, int VB = 0>
 struct CompactSmem {
+  using V = typename std::conditional<VB == 8, uint64_t, uint32_t>::type;
   uint16_t src[BLOCK * 16];
   uint64_t red[BLOCK / kWave + 1];
   uint64_t prefix;
+  alignas(16) V pv[VB ? BLOCK * 16 : 1];
 };
 
 template <typename PT>
@@ -43,8 +50,8 @@ __device__ __forceinline__ void load16(const PT *p, uint64_t base, uint64_t rows
 // processed in order by one caller that passes the running prefix (no
 // look-back).  Returns the tile's selected-row count.
 // Every thread of the block calls it; contains barriers.
-template <int BLOCK, int MODE, typename PT, int CMP>
-__device__ __forceinline__ uint64_t compact_tile(CompactSmem<BLOCK> &sm, uint32_t tile, const uint8_t *mask,
+template <int BLOCK, int MODE, typename PT, int CMP, int VB>
+__device__ __forceinline__ uint64_t compact_tile(CompactSmem<BLOCK, VB> &sm, uint32_t tile, const uint8_t *mask,
                                              const PT *pred, PT scalar, PT hi, const ysql_col *cols, int ncols,
                                              uint64_t n, uint64_t cap, uint64_t running,
                                              uint64_t *count_out, uint64_t *status, uint32_t ntiles) {
@@ -66,6 +73,15 @@ __device__ __forceinline__ uint64_t compact_tile(CompactSmem<BLOCK> &sm, uint32_
     for (int e = 0; e < 16; e++) {
       const bool sel = MODE == 1 ? cmp_static<CMP>(v[e], scalar) : (!(v[e] < scalar) && v[e] < hi);
       flags |= (uint32_t)sel << e;
+    }
+    if constexpr (VB == (int)sizeof(PT)) {
+      // 16-byte LDS stores of the thread's 16 values
+      constexpr int NQ = 16 * VB / 16;
+      uint4 q[NQ];
+      memcpy(&q[0], &v[0], sizeof(q));
+      uint4 *d = reinterpret_cast<uint4 *>(&sm.pv[t * 16]);
+#pragma unroll
+      for (int i = 0; i < NQ; i++) d[i] = q[i];
     }
   }
   {
@@ -96,6 +112,18 @@ __device__ __forceinline__ uint64_t compact_tile(CompactSmem<BLOCK> &sm, uint32_
   const uint32_t tot = prefix >= cap ? 0u : (uint32_t)(cap - prefix < total ? cap - prefix : total);
   for (int c = 0; c < ncols; c++) {
     const ysql_col col = cols[c];
+    if constexpr (MODE != 0 && VB == (int)sizeof(PT)) {
+      if (col.src == (const void *)pred && col.elem_bytes == VB) {
+        using V = typename CompactSmem<BLOCK, VB>::V;
+        V *dst = (V *)col.dst;
+        for (uint32_t r = t; r < tot; r += BLOCK) dst[prefix + r] = sm.pv[sm.src[r]];
+        if (col.dst_null) {
+          for (uint32_t r = t; r < tot; r += BLOCK)
+            col.dst_null[prefix + r] = col.src_null ? col.src_null[base + sm.src[r]] : 0;
+        }
+        continue;
+      }
+    }
     switch (col.elem_bytes) {
       case 4: {
         const uint32_t *src = (const uint32_t *)col.src;
