@@ -375,9 +375,9 @@ def batched_lines():
            "speedup": t_plain / t_graph, "identical_outputs": bool(same),
            "batched_GB_s": alg / t_graph / 1e9, "batched_frac": alg / t_graph / 1e9 / HBM_PEAK_GBS,
            "chains": js.info()["chains"],
-           "what": "plain: per-job launches on one stream; batched: one hipGraph replay (jobs <= 64K rows one per "
-                   "workgroup in segmented kernels, larger jobs in 32 launch chains); sort tiles from the "
-                   "autotuned table"}
+           "what": "plain: per-job launches on one stream; batched: one hipGraph replay (sorts <= 512K rows, "
+                   "filters <= 2M rows and aggregates <= 4M rows one per workgroup in segmented kernels, larger "
+                   "jobs in 32 launch chains); sort tiles from the autotuned table"}
     js.close()
     del js
     torch.cuda.empty_cache()

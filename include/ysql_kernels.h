@@ -451,9 +451,10 @@ typedef struct ysql_batch ysql_batch;
 /* Tuning of the batched executor (ysql_batch_tuning_default fills it). */
 typedef struct ysql_batch_tuning {
   int chains;         /* independent launch chains in the graph (1..64) */
-  int seg_sort_max;   /* sort jobs up to this many rows run one per workgroup */
-  int seg_filter_max; /* filter jobs likewise (0 = every job gets a chain) */
-  int seg_agg_max;    /* aggregate jobs likewise */
+  int seg_sort_max;   /* sort jobs up to this many rows run one per workgroup
+                         (default 2^19) */
+  int seg_filter_max; /* filter jobs likewise (default 2^21; 0 = every job gets a chain) */
+  int seg_agg_max;    /* aggregate jobs likewise (default 2^22) */
   /* digit-pass tile variant of a sort job (see ysql_radix_sort_variant) by
    * key width (0: UKEY32, 1: UKEY64) and size bucket floor(log2(n));
    * -1 = the library default.  Filled by an autotuner. */

@@ -81,7 +81,8 @@ class JobSet:
     def create(self, chains=None, sort_variants=None, seg_max=None):
         """sort_variants: {32: [variant per log2 bucket], 64: [...]} (an
         autotuner's table); seg_max: rows up to which a job runs in one
-        workgroup of a segmented kernel (0 = never); None keeps the library
+        workgroup of a segmented kernel (0 = never), one value for every kind
+        or a (sort, filter, aggregate) triple; None keeps the library
         defaults."""
         tuning = None
         if chains is not None or sort_variants is not None or seg_max is not None:
@@ -90,7 +91,8 @@ class JobSet:
             if chains is not None:
                 tuning.chains = int(chains)
             if seg_max is not None:
-                tuning.seg_sort_max = tuning.seg_filter_max = tuning.seg_agg_max = int(seg_max)
+                smax = tuple(seg_max) if isinstance(seg_max, (tuple, list)) else (seg_max,) * 3
+                tuning.seg_sort_max, tuning.seg_filter_max, tuning.seg_agg_max = (int(x) for x in smax)
             for w, table in (sort_variants or {}).items():
                 row = tuning.sort_variant[0 if int(w) == 32 else 1]
                 for b, v in enumerate(table[:32]):
