@@ -282,7 +282,7 @@ __device__ __forceinline__ void fa_fold_words(uint64_t &val, int opc, const uint
 // and each spec column that is not the predicate column or an earlier
 // spec's column), then evaluates each query's predicate and folds the
 // thread's rows in row order.
-template <bool FULL, typename PT, int CMP, int BLOCK, int QMAX>
+template <bool FULL, bool SPECS, typename PT, int CMP, int BLOCK, int QMAX>
 __device__ __forceinline__ void fa_step(uint64_t s0, uint64_t end, const PT *pred, const PT (&scalar)[QMAX], int nq,
                                         const FaSpec (&sp)[kFaMaxSpecs], const int (&opc)[kFaMaxSpecs], int ns,
                                         uint64_t (&val)[QMAX][kFaMaxSpecs], uint32_t (&cnt)[QMAX]) {
@@ -291,13 +291,15 @@ __device__ __forceinline__ void fa_step(uint64_t s0, uint64_t end, const PT *pre
   PT pv[kFaRpt];
   fa_load_rows<FULL, PT>(pred, r0, stride, end, pv);
   uint32_t w[kFaMaxSpecs][2 * kFaRpt];
+  if constexpr (SPECS) {
 #pragma unroll
-  for (int s = 0; s < kFaMaxSpecs; s++)
-    if (s < ns && opc[s] != FA_OPC_NONE && sp[s].src == FA_OWN)
-      fa_load_words<FULL>(sp[s].col, fa_opc_wide(opc[s]), r0, stride, end, w[s]);
+    for (int s = 0; s < kFaMaxSpecs; s++)
+      if (s < ns && opc[s] != FA_OPC_NONE && sp[s].src == FA_OWN)
+        fa_load_words<FULL>(sp[s].col, fa_opc_wide(opc[s]), r0, stride, end, w[s]);
+  }
   // specs over the predicate column or an earlier spec's column take a copy
   // of those words (register moves, no second load)
-  {
+  if constexpr (SPECS) {
     uint32_t pw[2 * kFaRpt];
     memcpy(pw, pv, sizeof(pv));
 #pragma unroll
@@ -326,9 +328,11 @@ __device__ __forceinline__ void fa_step(uint64_t s0, uint64_t end, const PT *pre
       if constexpr (!FULL) sel[k] = sel[k] && r0 + (uint64_t)(k >> 1) * stride + (k & 1) < end;
       cnt[q] += sel[k] ? 1u : 0u;
     }
+    if constexpr (SPECS) {
 #pragma unroll
-    for (int s = 0; s < kFaMaxSpecs; s++)
-      if (s < ns) fa_fold_words(val[q][s], opc[s], w[s], sel);
+      for (int s = 0; s < kFaMaxSpecs; s++)
+        if (s < ns) fa_fold_words(val[q][s], opc[s], w[s], sel);
+    }
   }
 }
 
@@ -349,12 +353,14 @@ __device__ __forceinline__ void filter_agg_tile(uint32_t tile, uint64_t tile_row
   FaSpec sp[kFaMaxSpecs];
   int opc[kFaMaxSpecs];
   bool aligned = ((uintptr_t)pred & (2 * sizeof(PT) - 1)) == 0;
+  bool any = false;  // any spec other than COUNT
 #pragma unroll
   for (int s = 0; s < kFaMaxSpecs; s++) {
     opc[s] = FA_OPC_NONE;
     if (s < ns) {
       sp[s] = specs[s];
       opc[s] = fa_opcode(sp[s]);
+      any = any || opc[s] != FA_OPC_NONE;
       if (opc[s] != FA_OPC_NONE && sp[s].src == FA_OWN)
         aligned = aligned && ((uintptr_t)sp[s].col & (fa_opc_wide(opc[s]) ? 15 : 7)) == 0;
 #pragma unroll
@@ -365,10 +371,15 @@ __device__ __forceinline__ void filter_agg_tile(uint32_t tile, uint64_t tile_row
   for (int q = 0; q < QMAX; q++) cnt[q] = 0;
   constexpr uint64_t step = (uint64_t)BLOCK * kFaRpt;
   uint64_t s0 = base;
-  if (aligned)
+  // COUNT-only queries: a predicate-only loop
+  if (aligned && any)
     for (; s0 + step <= end; s0 += step)
-      fa_step<true, PT, CMP, BLOCK, QMAX>(s0, end, pred, scalar, nq, sp, opc, ns, val, cnt);
-  for (; s0 < end; s0 += step) fa_step<false, PT, CMP, BLOCK, QMAX>(s0, end, pred, scalar, nq, sp, opc, ns, val, cnt);
+      fa_step<true, true, PT, CMP, BLOCK, QMAX>(s0, end, pred, scalar, nq, sp, opc, ns, val, cnt);
+  if (aligned && !any)
+    for (; s0 + step <= end; s0 += step)
+      fa_step<true, false, PT, CMP, BLOCK, QMAX>(s0, end, pred, scalar, nq, sp, opc, ns, val, cnt);
+  for (; s0 < end; s0 += step)
+    fa_step<false, true, PT, CMP, BLOCK, QMAX>(s0, end, pred, scalar, nq, sp, opc, ns, val, cnt);
 #pragma unroll
   for (int q = 0; q < QMAX; q++) {
     if (q >= nq) break;
