@@ -471,7 +471,9 @@ def test_filter_aggregate(ops, dev, n, pdt):
             # specs over the predicate column and two specs over one column
             # (the kernel loads such a column once)
             shared = [(pdc, 2, tp), (pdc, 1, tp), (O.DATETIME, 0, tj), (O.DATETIME, 1, tj)]
-            for group in (specs[:4], specs[4:], shared):
+            # COUNT specs only: the predicate-only loop
+            counts = [(O.INT, 5, None), (O.INT, 4, None)]
+            for group in (specs[:4], specs[4:], shared, counts):
                 accs = [ops.acc_new(dc, agg, device=dev) for dc, agg, _ in group]
                 ops.filter_aggregate(tp, cmp, c.item(), [(dc, agg, col, a) for (dc, agg, col), a in zip(group, accs)],
                                      overwrite=(off == 1))
