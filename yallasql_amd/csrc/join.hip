@@ -192,12 +192,20 @@ __global__ __launch_bounds__(kJoinBlock) void k_join_probe(const T *left, uint64
 }
 
 // Same probe with the hash table staged in LDS, for tables of up to
-// kLdsSlots slots (right sides of up to kLdsSlots / 2 distinct keys).  The
-// workgroups are persistent: each copies the table into LDS once (a slot
-// with count 0 is empty) and then takes tiles in order from the counter
-// until none are left, so every look-back still waits only on tiles taken
-// earlier by running workgroups.
+// kLdsSlots slots (right sides of up to kLdsSlots / 2 rows).  The
+// workgroups are persistent: each copies the table into LDS once and then
+// takes tiles in order from the counter until none are left, so every
+// look-back still waits only on tiles taken earlier by running workgroups.
+// A slot holds the key and start << 16 | count (0: empty; start and count
+// are below 2^11 here).  A 32K-bit membership bitmap over the hash's high
+// bits rejects most non-matching rows with one LDS read; the rest probe in
+// rounds, every pending row of a thread advancing one slot per round, so
+// their LDS reads overlap instead of forming one chain per row.  A tile is
+// kLdsSub consecutive blocks of kProbeTile rows with one look-back.  52 KB
+// of LDS: three workgroups per CU.
 constexpr uint32_t kLdsSlots = 4096;
+constexpr uint32_t kLdsBits = 32768;
+constexpr int kLdsSub = 4;  // sub-steps per tile: one look-back per 4 * kProbeTile rows
 
 template <typename T>
 __global__ __launch_bounds__(kJoinBlock) void k_join_probe_lds(const T *left, uint64_t L, const HashSlot *table,
@@ -206,72 +214,112 @@ __global__ __launch_bounds__(kJoinBlock) void k_join_probe_lds(const T *left, ui
                                                                uint64_t *count_out, uint32_t *ctr, uint64_t *status,
                                                                uint32_t ntiles) {
   __shared__ uint64_t s_key[kLdsSlots];
-  __shared__ uint32_t s_start[kLdsSlots];
-  __shared__ uint32_t s_cnt[kLdsSlots];
+  __shared__ uint32_t s_sc[kLdsSlots];
+  __shared__ uint32_t s_bits[kLdsBits / 32];
   __shared__ uint64_t s_red[kJoinBlock / kWave + 1];
   __shared__ uint32_t s_tile;
   __shared__ uint64_t s_prefix;
+  for (uint32_t i = threadIdx.x; i < kLdsBits / 32; i += kJoinBlock) s_bits[i] = 0;
+  __syncthreads();
   for (uint32_t i = threadIdx.x; i <= mask; i += kJoinBlock) {
     const bool u = used[i] != 0;
     const HashSlot e = u ? table[i] : HashSlot{0, 0, 0};
     s_key[i] = e.key;
-    s_start[i] = e.start;
-    s_cnt[i] = u ? e.count : 0u;
+    s_sc[i] = u ? (e.start << 16) | e.count : 0u;
+    if (u) {
+      const uint32_t bit = hash_u64(e.key) >> 17;
+      atomicOr(&s_bits[bit >> 5], 1u << (bit & 31));
+    }
   }
   while (true) {
     const uint32_t tile = take_tile_j(ctr, &s_tile);  // (its barrier also publishes the table)
     if (tile >= ntiles) return;
-    const uint64_t row0 = (uint64_t)tile * kProbeTile + (uint64_t)threadIdx.x * kProbeRpt;
-    T v[kProbeRpt];
-    if (row0 + kProbeRpt <= L && ((uintptr_t)(left + row0) & 15) == 0) {
-      constexpr int NQ = kProbeRpt * sizeof(T) / 16;
-      uint4 q[NQ];
-      const uint4 *src = reinterpret_cast<const uint4 *>(left + row0);
+    // kLdsSub sub-steps of kProbeTile rows; a row's match (start << 16 |
+    // count, 0: none) stays in registers until its position is known
+    uint32_t found[kLdsSub][kProbeRpt];
+    uint32_t mine[kLdsSub];
 #pragma unroll
-      for (int i = 0; i < NQ; i++) q[i] = src[i];
-      memcpy(&v[0], &q[0], sizeof(q));
-    } else {
+    for (int sub = 0; sub < kLdsSub; sub++) {
+      const uint64_t row0 = ((uint64_t)tile * kLdsSub + sub) * kProbeTile + (uint64_t)threadIdx.x * kProbeRpt;
+      T v[kProbeRpt];
+      if (row0 + kProbeRpt <= L && ((uintptr_t)(left + row0) & 15) == 0) {
+        constexpr int NQ = kProbeRpt * sizeof(T) / 16;
+        uint4 q[NQ];
+        const uint4 *src = reinterpret_cast<const uint4 *>(left + row0);
 #pragma unroll
-      for (int e = 0; e < kProbeRpt; e++) v[e] = row0 + e < L ? left[row0 + e] : T();
-    }
-    uint32_t st[kProbeRpt], cn[kProbeRpt];
-    uint64_t mine = 0;
+        for (int i = 0; i < NQ; i++) q[i] = src[i];
+        memcpy(&v[0], &q[0], sizeof(q));
+      } else {
 #pragma unroll
-    for (int e = 0; e < kProbeRpt; e++) {
-      st[e] = 0;
-      cn[e] = 0;
-      if (row0 + e < L && JoinKey<T>::ok(v[e])) {
-        const uint64_t k = (uint64_t)JoinKey<T>::bits(v[e]);
-        uint32_t h = hash_u64(k) & mask;
-        while (s_cnt[h]) {
-          if (s_key[h] == k) {
-            st[e] = s_start[h];
-            cn[e] = s_cnt[h];
-            break;
+        for (int e = 0; e < kProbeRpt; e++) v[e] = row0 + e < L ? left[row0 + e] : T();
+      }
+      uint32_t slot[kProbeRpt];
+      uint32_t pend = 0;  // bit e: row e still probing
+#pragma unroll
+      for (int e = 0; e < kProbeRpt; e++) {
+        found[sub][e] = 0;
+        slot[e] = 0;
+        if (row0 + e < L && JoinKey<T>::ok(v[e])) {
+          const uint32_t h = hash_u64((uint64_t)JoinKey<T>::bits(v[e]));
+          const uint32_t bit = h >> 17;
+          if ((s_bits[bit >> 5] >> (bit & 31)) & 1u) {
+            slot[e] = h & mask;
+            pend |= 1u << e;
           }
-          h = (h + 1) & mask;
         }
       }
-      mine += cn[e];
+      while (pend) {
+#pragma unroll
+        for (int e = 0; e < kProbeRpt; e++) {
+          if ((pend >> e) & 1u) {
+            const uint32_t sc = s_sc[slot[e]];
+            const uint64_t key = s_key[slot[e]];
+            if (sc == 0u) {
+              pend &= ~(1u << e);
+            } else if (key == (uint64_t)JoinKey<T>::bits(v[e])) {
+              found[sub][e] = sc;
+              pend &= ~(1u << e);
+            } else {
+              slot[e] = (slot[e] + 1) & mask;
+            }
+          }
+        }
+      }
+      mine[sub] = 0;
+#pragma unroll
+      for (int e = 0; e < kProbeRpt; e++) mine[sub] += found[sub][e] & 0xffffu;
     }
-    uint64_t total;
-    const uint64_t excl = block_exclusive_sum<kJoinBlock, uint64_t>(mine, s_red, &total);
+    // output order: sub-steps in order, threads in order within one
+    uint64_t excl[kLdsSub], total = 0;
+#pragma unroll
+    for (int sub = 0; sub < kLdsSub; sub++) {
+      uint64_t t_sub;
+      excl[sub] = total + block_exclusive_sum<kJoinBlock, uint64_t>(mine[sub], s_red, &t_sub);
+      total += t_sub;
+    }
     if (threadIdx.x < kWave) {
       const uint64_t p = lookback_exclusive(status, tile, total);
       if (threadIdx.x == 0) s_prefix = p;
     }
     __syncthreads();
-    uint64_t pos = s_prefix + excl;
+    const uint64_t prefix = s_prefix;
 #pragma unroll
-    for (int e = 0; e < kProbeRpt; e++) {
-      for (uint32_t q = 0; q < cn[e]; q++, pos++) {
-        if (pos < cap) {
-          pairs[2 * pos] = (uint32_t)(row0 + e);
-          pairs[2 * pos + 1] = perm[st[e] + q];
+    for (int sub = 0; sub < kLdsSub; sub++) {
+      if (!mine[sub]) continue;
+      const uint64_t row0 = ((uint64_t)tile * kLdsSub + sub) * kProbeTile + (uint64_t)threadIdx.x * kProbeRpt;
+      uint64_t pos = prefix + excl[sub];
+#pragma unroll
+      for (int e = 0; e < kProbeRpt; e++) {
+        const uint32_t st = found[sub][e] >> 16, cn = found[sub][e] & 0xffffu;
+        for (uint32_t q = 0; q < cn; q++, pos++) {
+          if (pos < cap) {
+            pairs[2 * pos] = (uint32_t)(row0 + e);
+            pairs[2 * pos + 1] = perm[st + q];
+          }
         }
       }
     }
-    if (tile == ntiles - 1 && threadIdx.x == 0) *count_out = s_prefix + total;
+    if (tile == ntiles - 1 && threadIdx.x == 0) *count_out = prefix + total;
     __syncthreads();  // s_prefix and s_red are reused by the next tile
   }
 }
@@ -507,9 +555,10 @@ static int join_hash(const T *left, uint64_t L, const T *right, uint64_t R, uint
         b = 1;
       return (uint32_t)b;
     }();
-    const uint32_t g = std::min<uint32_t>(ntiles, per_cu * (uint32_t)grid_cap());
+    const uint32_t nt = (uint32_t)ceil_div(L, (uint64_t)kLdsSub * kProbeTile);  // <= ntiles: fits the status area
+    const uint32_t g = std::min<uint32_t>(nt, per_cu * (uint32_t)grid_cap());
     k_join_probe_lds<T><<<std::max<uint32_t>(g, 1), kJoinBlock, 0, s>>>(left, L, w.table, w.used, tcap - 1, w.perm,
-                                                                        pairs, cap, count_out, w.ctr, w.status, ntiles);
+                                                                        pairs, cap, count_out, w.ctr, w.status, nt);
   } else {
     k_join_probe<T><<<ntiles, kJoinBlock, 0, s>>>(left, L, w.table, w.used, tcap - 1, w.perm, pairs, cap, count_out,
                                                   w.ctr, w.status, ntiles);
