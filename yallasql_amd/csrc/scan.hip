@@ -183,7 +183,7 @@ template <int MODE, typename PT, int CMP>
 __global__ __launch_bounds__(kScanBlock) void k_compact(const uint8_t *mask, const PT *pred, const PT *scalar, ColArgs cols,
                                                         int ncols, uint64_t n, uint64_t cap, uint64_t *count_out,
                                                         uint32_t *ctr, uint64_t *status, uint32_t ntiles) {
-  __shared__ CompactSmem<kScanBlock, MODE == 0 ? 0 : (int)sizeof(PT)> sm;
+  __shared__ CompactSmem<kScanBlock, (MODE == 0 || sizeof(PT) != 4) ? 0 : 4> sm;
   __shared__ uint32_t s_tile;
   const uint32_t tile = take_tile(ctr, &s_tile);
   const PT s = MODE >= 1 ? scalar[0] : PT();

@@ -18,7 +18,8 @@
 
 namespace ysql {
 
-// VB: bytes of a predicate value kept in LDS (0: none, mask selection).
+// VB: bytes of a predicate value kept in LDS (0: none: mask selection, and
+// 8-byte predicates, whose 32 KB would halve the workgroups per CU).
 template <int BLOCK, int VB = 0>
 struct CompactSmem {
   using V = typename std::conditional<VB == 8, uint64_t, uint32_t>::type;
@@ -75,13 +76,16 @@ __device__ __forceinline__ uint64_t compact_tile(CompactSmem<BLOCK, VB> &sm, uin
       flags |= (uint32_t)sel << e;
     }
     if constexpr (VB == (int)sizeof(PT)) {
-      // 16-byte LDS stores of the thread's 16 values
-      constexpr int NQ = 16 * VB / 16;
-      uint4 q[NQ];
-      memcpy(&q[0], &v[0], sizeof(q));
-      uint4 *d = reinterpret_cast<uint4 *>(&sm.pv[t * 16]);
+      // 16-byte LDS stores of the thread's 16 values, when it selected any
+      // (only selected rows are read back)
+      if (flags) {
+        constexpr int NQ = 16 * VB / 16;
+        uint4 q[NQ];
+        memcpy(&q[0], &v[0], sizeof(q));
+        uint4 *d = reinterpret_cast<uint4 *>(&sm.pv[t * 16]);
 #pragma unroll
-      for (int i = 0; i < NQ; i++) d[i] = q[i];
+        for (int i = 0; i < NQ; i++) d[i] = q[i];
+      }
     }
   }
   {
