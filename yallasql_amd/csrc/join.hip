@@ -46,13 +46,15 @@ struct JoinKey<int64_t> {
   static __device__ __forceinline__ U bits(int64_t v) { return (uint64_t)v; }
 };
 
+// 32-bit hash of a 64-bit key: the high word is folded into the low one
+// with a multiply, the result is spread by a golden-ratio multiply, and the
+// high bits are folded back into the low ones, so both the low bits (the
+// slot) and the high bits (the probe's membership bitmap) depend on every
+// key bit.  Two 32-bit multiplies; build and probe use the same function.
 __device__ __forceinline__ uint32_t hash_u64(uint64_t x) {
-  x ^= x >> 33;
-  x *= 0xff51afd7ed558ccdull;
-  x ^= x >> 33;
-  x *= 0xc4ceb9fe1a85ec53ull;
-  x ^= x >> 33;
-  return (uint32_t)x;
+  uint32_t h = ((uint32_t)x ^ ((uint32_t)(x >> 32) * 0x85ebca6bu)) * 0x9e3779b1u;
+  h ^= h >> 15;
+  return h;
 }
 
 struct HashSlot {
@@ -196,16 +198,34 @@ __global__ __launch_bounds__(kJoinBlock) void k_join_probe(const T *left, uint64
 // workgroups are persistent: each copies the table into LDS once and then
 // takes tiles in order from the counter until none are left, so every
 // look-back still waits only on tiles taken earlier by running workgroups.
-// A slot holds the key and start << 16 | count (0: empty; start and count
-// are below 2^11 here).  A 32K-bit membership bitmap over the hash's high
-// bits rejects most non-matching rows with one LDS read; the rest probe in
-// rounds, every pending row of a thread advancing one slot per round, so
-// their LDS reads overlap instead of forming one chain per row.  A tile is
-// kLdsSub consecutive blocks of kProbeTile rows with one look-back.  52 KB
-// of LDS: three workgroups per CU.
+// LDS holds each slot's start << 16 | count (0: empty; start and count are
+// below 2^11 here) and a 32K-bit membership bitmap over the hash's high
+// bits, which rejects most non-matching rows with one LDS read; the rest
+// probe in rounds, every pending row of a thread advancing one slot per
+// round so that their reads overlap, comparing the full key in the global
+// table (64 KB, cache-resident).  A tile is kLdsSub consecutive blocks of
+// kProbeTile rows with one look-back.  A thread keeps only its match count
+// and a hit mask per block; the few matching rows are probed again when
+// their pairs are written.  20 KB of LDS and few registers, so many waves
+// are in flight.
 constexpr uint32_t kLdsSlots = 4096;
 constexpr uint32_t kLdsBits = 32768;
 constexpr int kLdsSub = 4;  // sub-steps per tile: one look-back per 4 * kProbeTile rows
+
+template <typename T>
+__device__ __forceinline__ void probe_load(const T *left, uint64_t L, uint64_t row0, T (&v)[kProbeRpt]) {
+  if (row0 + kProbeRpt <= L && ((uintptr_t)(left + row0) & 15) == 0) {
+    constexpr int NQ = kProbeRpt * sizeof(T) / 16;
+    uint4 q[NQ];
+    const uint4 *src = reinterpret_cast<const uint4 *>(left + row0);
+#pragma unroll
+    for (int i = 0; i < NQ; i++) q[i] = src[i];
+    memcpy(&v[0], &q[0], sizeof(q));
+  } else {
+#pragma unroll
+    for (int e = 0; e < kProbeRpt; e++) v[e] = row0 + e < L ? left[row0 + e] : T();
+  }
+}
 
 template <typename T>
 __global__ __launch_bounds__(kJoinBlock) void k_join_probe_lds(const T *left, uint64_t L, const HashSlot *table,
@@ -213,7 +233,6 @@ __global__ __launch_bounds__(kJoinBlock) void k_join_probe_lds(const T *left, ui
                                                                const uint32_t *perm, uint32_t *pairs, uint64_t cap,
                                                                uint64_t *count_out, uint32_t *ctr, uint64_t *status,
                                                                uint32_t ntiles) {
-  __shared__ uint64_t s_key[kLdsSlots];
   __shared__ uint32_t s_sc[kLdsSlots];
   __shared__ uint32_t s_bits[kLdsBits / 32];
   __shared__ uint64_t s_red[kJoinBlock / kWave + 1];
@@ -224,7 +243,6 @@ __global__ __launch_bounds__(kJoinBlock) void k_join_probe_lds(const T *left, ui
   for (uint32_t i = threadIdx.x; i <= mask; i += kJoinBlock) {
     const bool u = used[i] != 0;
     const HashSlot e = u ? table[i] : HashSlot{0, 0, 0};
-    s_key[i] = e.key;
     s_sc[i] = u ? (e.start << 16) | e.count : 0u;
     if (u) {
       const uint32_t bit = hash_u64(e.key) >> 17;
@@ -234,30 +252,16 @@ __global__ __launch_bounds__(kJoinBlock) void k_join_probe_lds(const T *left, ui
   while (true) {
     const uint32_t tile = take_tile_j(ctr, &s_tile);  // (its barrier also publishes the table)
     if (tile >= ntiles) return;
-    // kLdsSub sub-steps of kProbeTile rows; a row's match (start << 16 |
-    // count, 0: none) stays in registers until its position is known
-    uint32_t found[kLdsSub][kProbeRpt];
-    uint32_t mine[kLdsSub];
+    uint32_t mine[kLdsSub], hit[kLdsSub];
 #pragma unroll
     for (int sub = 0; sub < kLdsSub; sub++) {
       const uint64_t row0 = ((uint64_t)tile * kLdsSub + sub) * kProbeTile + (uint64_t)threadIdx.x * kProbeRpt;
       T v[kProbeRpt];
-      if (row0 + kProbeRpt <= L && ((uintptr_t)(left + row0) & 15) == 0) {
-        constexpr int NQ = kProbeRpt * sizeof(T) / 16;
-        uint4 q[NQ];
-        const uint4 *src = reinterpret_cast<const uint4 *>(left + row0);
-#pragma unroll
-        for (int i = 0; i < NQ; i++) q[i] = src[i];
-        memcpy(&v[0], &q[0], sizeof(q));
-      } else {
-#pragma unroll
-        for (int e = 0; e < kProbeRpt; e++) v[e] = row0 + e < L ? left[row0 + e] : T();
-      }
+      probe_load<T>(left, L, row0, v);
       uint32_t slot[kProbeRpt];
       uint32_t pend = 0;  // bit e: row e still probing
 #pragma unroll
       for (int e = 0; e < kProbeRpt; e++) {
-        found[sub][e] = 0;
         slot[e] = 0;
         if (row0 + e < L && JoinKey<T>::ok(v[e])) {
           const uint32_t h = hash_u64((uint64_t)JoinKey<T>::bits(v[e]));
@@ -268,16 +272,18 @@ __global__ __launch_bounds__(kJoinBlock) void k_join_probe_lds(const T *left, ui
           }
         }
       }
+      uint32_t cnt = 0, hm = 0;
       while (pend) {
 #pragma unroll
         for (int e = 0; e < kProbeRpt; e++) {
           if ((pend >> e) & 1u) {
             const uint32_t sc = s_sc[slot[e]];
-            const uint64_t key = s_key[slot[e]];
+            const uint64_t key = sc ? table[slot[e]].key : 0ull;
             if (sc == 0u) {
               pend &= ~(1u << e);
             } else if (key == (uint64_t)JoinKey<T>::bits(v[e])) {
-              found[sub][e] = sc;
+              cnt += sc & 0xffffu;
+              hm |= 1u << e;
               pend &= ~(1u << e);
             } else {
               slot[e] = (slot[e] + 1) & mask;
@@ -285,9 +291,8 @@ __global__ __launch_bounds__(kJoinBlock) void k_join_probe_lds(const T *left, ui
           }
         }
       }
-      mine[sub] = 0;
-#pragma unroll
-      for (int e = 0; e < kProbeRpt; e++) mine[sub] += found[sub][e] & 0xffffu;
+      mine[sub] = cnt;
+      hit[sub] = hm;
     }
     // output order: sub-steps in order, threads in order within one
     uint64_t excl[kLdsSub], total = 0;
@@ -303,14 +308,16 @@ __global__ __launch_bounds__(kJoinBlock) void k_join_probe_lds(const T *left, ui
     }
     __syncthreads();
     const uint64_t prefix = s_prefix;
-#pragma unroll
     for (int sub = 0; sub < kLdsSub; sub++) {
-      if (!mine[sub]) continue;
+      if (!hit[sub]) continue;
       const uint64_t row0 = ((uint64_t)tile * kLdsSub + sub) * kProbeTile + (uint64_t)threadIdx.x * kProbeRpt;
       uint64_t pos = prefix + excl[sub];
-#pragma unroll
-      for (int e = 0; e < kProbeRpt; e++) {
-        const uint32_t st = found[sub][e] >> 16, cn = found[sub][e] & 0xffffu;
+      for (uint32_t hm = hit[sub]; hm; hm &= hm - 1) {
+        const int e = __builtin_ctz(hm);
+        const uint64_t key = (uint64_t)JoinKey<T>::bits(left[row0 + e]);
+        uint32_t slot = hash_u64(key) & mask, sc;
+        while ((sc = s_sc[slot]) != 0u && table[slot].key != key) slot = (slot + 1) & mask;
+        const uint32_t st = sc >> 16, cn = sc & 0xffffu;
         for (uint32_t q = 0; q < cn; q++, pos++) {
           if (pos < cap) {
             pairs[2 * pos] = (uint32_t)(row0 + e);
