@@ -199,17 +199,17 @@ __global__ __launch_bounds__(kJoinBlock) void k_join_probe(const T *left, uint64
 // takes tiles in order from the counter until none are left, so every
 // look-back still waits only on tiles taken earlier by running workgroups.
 // LDS holds each slot's start << 16 | count (0: empty; start and count are
-// below 2^11 here) and a 32K-bit membership bitmap over the hash's high
+// below 2^11 here) and a 64K-bit membership bitmap over the hash's high
 // bits, which rejects most non-matching rows with one LDS read; the rest
 // probe in rounds, every pending row of a thread advancing one slot per
 // round so that their reads overlap, comparing the full key in the global
 // table (64 KB, cache-resident).  A tile is kLdsSub consecutive blocks of
 // kProbeTile rows with one look-back.  A thread keeps only its match count
 // and a hit mask per block; the few matching rows are probed again when
-// their pairs are written.  20 KB of LDS and few registers, so many waves
+// their pairs are written.  24 KB of LDS and few registers, so many waves
 // are in flight.
 constexpr uint32_t kLdsSlots = 4096;
-constexpr uint32_t kLdsBits = 32768;
+constexpr uint32_t kLdsBits = 65536;
 constexpr int kLdsSub = 4;  // sub-steps per tile: one look-back per 4 * kProbeTile rows
 
 template <typename T>
@@ -245,7 +245,7 @@ __global__ __launch_bounds__(kJoinBlock) void k_join_probe_lds(const T *left, ui
     const HashSlot e = u ? table[i] : HashSlot{0, 0, 0};
     s_sc[i] = u ? (e.start << 16) | e.count : 0u;
     if (u) {
-      const uint32_t bit = hash_u64(e.key) >> 17;
+      const uint32_t bit = hash_u64(e.key) >> 16;
       atomicOr(&s_bits[bit >> 5], 1u << (bit & 31));
     }
   }
@@ -265,7 +265,7 @@ __global__ __launch_bounds__(kJoinBlock) void k_join_probe_lds(const T *left, ui
         slot[e] = 0;
         if (row0 + e < L && JoinKey<T>::ok(v[e])) {
           const uint32_t h = hash_u64((uint64_t)JoinKey<T>::bits(v[e]));
-          const uint32_t bit = h >> 17;
+          const uint32_t bit = h >> 16;
           if ((s_bits[bit >> 5] >> (bit & 31)) & 1u) {
             slot[e] = h & mask;
             pend |= 1u << e;
