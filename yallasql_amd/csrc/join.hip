@@ -199,8 +199,9 @@ __global__ __launch_bounds__(kJoinBlock) void k_join_probe(const T *left, uint64
 // takes tiles in order from the counter until none are left, so every
 // look-back still waits only on tiles taken earlier by running workgroups.
 // LDS holds each slot's start << 16 | count (0: empty; start and count are
-// below 2^11 here) and a 64K-bit membership bitmap over the hash's high
-// bits, which rejects most non-matching rows with one LDS read; the rest
+// below 2^11 here) and a 64K-bit two-probe membership filter (a Bloom
+// filter over two 16-bit functions of the hash), which rejects all but
+// about 0.4% of non-matching rows with two LDS reads; the rest
 // probe in rounds, every pending row of a thread advancing one slot per
 // round so that their reads overlap, comparing the full key in the global
 // table (64 KB, cache-resident).  A tile is kLdsSub consecutive blocks of
@@ -245,8 +246,10 @@ __global__ __launch_bounds__(kJoinBlock) void k_join_probe_lds(const T *left, ui
     const HashSlot e = u ? table[i] : HashSlot{0, 0, 0};
     s_sc[i] = u ? (e.start << 16) | e.count : 0u;
     if (u) {
-      const uint32_t bit = hash_u64(e.key) >> 16;
-      atomicOr(&s_bits[bit >> 5], 1u << (bit & 31));
+      const uint32_t h = hash_u64(e.key);
+      const uint32_t b1 = h >> 16, b2 = (h * 0x9e3779b1u) >> 16;
+      atomicOr(&s_bits[b1 >> 5], 1u << (b1 & 31));
+      atomicOr(&s_bits[b2 >> 5], 1u << (b2 & 31));
     }
   }
   while (true) {
@@ -265,8 +268,8 @@ __global__ __launch_bounds__(kJoinBlock) void k_join_probe_lds(const T *left, ui
         slot[e] = 0;
         if (row0 + e < L && JoinKey<T>::ok(v[e])) {
           const uint32_t h = hash_u64((uint64_t)JoinKey<T>::bits(v[e]));
-          const uint32_t bit = h >> 16;
-          if ((s_bits[bit >> 5] >> (bit & 31)) & 1u) {
+          const uint32_t b1 = h >> 16, b2 = (h * 0x9e3779b1u) >> 16;
+          if ((s_bits[b1 >> 5] >> (b1 & 31)) & (s_bits[b2 >> 5] >> (b2 & 31)) & 1u) {
             slot[e] = h & mask;
             pend |= 1u << e;
           }
