@@ -91,7 +91,8 @@ typedef struct ysql_acc {
   int64_t count;
 } ysql_acc;
 
-/* Column descriptor for the fused multi-column ops (host struct). */
+/* Column descriptor for the fused multi-column ops (host struct).  dst must
+ * not overlap src (nor another column's dst); src == dst is rejected. */
 typedef struct ysql_col {
   const void *src;
   const uint8_t *src_null; /* may be NULL: no NULLs (output nulls zeroed) */

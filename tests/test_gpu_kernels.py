@@ -234,6 +234,22 @@ def test_filter_compact(ops, dev, n, cmp):
     assert np.array_equal(N(outs[1][0]), O.compact(price, mask)[0])
 
 
+def test_compact_rejects_in_place(ops, dev):
+    """A column whose destination is its source is rejected (tiles would
+    overwrite rows other tiles still read)."""
+    from yallasql_amd import _lib as L
+    n = 1000
+    x = torch.arange(n, dtype=torch.int32, device=dev)
+    m = torch.ones(n, dtype=torch.uint8, device=dev)
+    cnt = torch.zeros(1, dtype=torch.int64, device=dev)
+    col = (L.Col * 1)()
+    col[0] = L.Col(x.data_ptr(), None, x.data_ptr(), None, 4, 0)
+    ws = ops.workspace(L.ws_bytes("ysql_compact_ws_bytes", n), dev)
+    rc = L.lib.ysql_compact_cols(ops._p(m), col, 1, n, ops._p(cnt), ops._p(ws), ws.numel(), ops._s())
+    assert rc == L.EINVAL
+    assert torch.equal(x, torch.arange(n, dtype=torch.int32, device=dev))
+
+
 def test_compact_strings(ops, dev):
     r = rng(9)
     n = 5000

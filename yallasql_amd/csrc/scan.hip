@@ -198,6 +198,7 @@ static bool cols_ok(const ysql_col *cols, int ncols, bool need_ptrs) {
     uint32_t eb = cols[i].elem_bytes;
     if (!(eb == 1 || eb == 2 || eb == 4 || eb == 8 || eb == YSQL_STRING_BYTES)) return false;
     if (need_ptrs && (!cols[i].src || !cols[i].dst)) return false;
+    if (cols[i].src && cols[i].src == cols[i].dst) return false;  // the copy reads and writes distinct columns
   }
   return true;
 }
