@@ -1,0 +1,66 @@
+/* ysql_csv.h — CSV file -> column batches (the feeder of the scan path).
+ *
+ * Replaces the reference's GetOperator (src/engine/operators/get_operator.cpp:16-148) and its
+ * batch sizing (include/config.hpp:28-38): a header row names the columns; each requested
+ * column is parsed with the reference's rules (get_operator.cpp:68-95,
+ * include/enums/data_type_utils.hpp:24-44):
+ *   INT      std::stoi of the field (leading blanks, optional sign, digits; trailing text
+ *            ignored; no digits or out of int32 range is an error)
+ *   FLOAT    std::stof of the field (likewise; ERANGE is an error)
+ *   DATETIME strptime "%Y-%m-%d %H:%M:%S" as UTC (timegm) -> int64 epoch seconds; a field
+ *            that does not parse gives 0
+ *   STRING   the first 255 bytes, NUL-padded to 256
+ * An empty field is NULL: null byte 1 and the value INT_MIN / FLT_MIN / 0 / all-zero string.
+ * Fields follow RFC 4180 quoting ("a,b", "say ""hi""", embedded newlines); \r\n and \n end
+ * records.
+ *
+ * Batches hold ysql_csv_batch_rows() rows (the last one fewer): floor(4 MiB / row bytes),
+ * rounded down to a multiple of 256, at least 256.
+ *
+ * Device path: ysql_csv_next parses the next batch into one of two pinned staging buffers
+ * and queues its host-to-device copies on `stream`; it returns once the copies are queued,
+ * so parsing the next batch overlaps the previous batch's copies.  A staging buffer is
+ * reused only after the copies queued from it have completed.  The device buffers of a
+ * batch are valid once `stream` reaches that call's copies.
+ */
+#ifndef YSQL_CSV_H
+#define YSQL_CSV_H
+
+#include <stdint.h>
+
+#include "ysql_kernels.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ysql_csv_reader ysql_csv_reader;
+
+/* Rows per batch for columns of these YSQL_* types (config.hpp:28-38). */
+int ysql_csv_batch_rows(int ncols, const int *types, uint32_t *rows);
+
+/* Opens `path` and reads its header row; names[c] is the CSV header name of requested
+ * column c, types[c] its YSQL_* type (get_operator.cpp:16-53).  YSQL_EINVAL for a missing
+ * file, an unknown name or a bad type. */
+int ysql_csv_open(const char *path, int ncols, const char *const *names, const int *types,
+                  ysql_csv_reader **out);
+
+/* Next batch into host memory: host_cols[c] holds batch_rows elements of column c,
+ * host_nulls[c] batch_rows null bytes (may be NULL: not written).  *rows = rows read
+ * (0: end of file).  YSQL_EINVAL for a field that does not convert or a short record. */
+int ysql_csv_next_host(ysql_csv_reader *r, void *const *host_cols, uint8_t *const *host_nulls,
+                       uint32_t *rows);
+
+/* Next batch into device memory (get_operator.cpp:56-108, storeBuffer :125-148), through
+ * pinned staging and asynchronous copies on `stream` (see above). */
+int ysql_csv_next(ysql_csv_reader *r, void *const *dev_cols, uint8_t *const *dev_nulls, uint32_t *rows,
+                  ysql_stream_t stream);
+
+/* Waits for the reader's queued copies, frees its staging and closes the file. */
+void ysql_csv_close(ysql_csv_reader *r);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* YSQL_CSV_H */

@@ -1,0 +1,179 @@
+"""CSV -> column batches (include/ysql_csv.h): batch sizing in closed form,
+parsed columns against values built by construction and against the
+oracle's restatement of the reference's rules, the error paths, and (GPU)
+the pinned, overlapped device path against the host path."""
+import calendar
+import os
+import time
+
+import numpy as np
+import pytest
+
+from oracle import csv_oracle as O
+
+INT, FLOAT, DATETIME, STRING = 0, 1, 2, 3
+
+
+@pytest.fixture(scope="module")
+def loader():
+    from yallasql_amd import loader as _loader
+
+    return _loader
+
+
+def _quote(s):
+    return '"' + s.replace('"', '""') + '"' if any(ch in s for ch in ',"\n\r') else s
+
+
+def _gen(path, n, seed, crlf=False):
+    """A CSV with INT, FLOAT, DATETIME, STRING columns (plus an ignored one)
+    and the values it holds, 5% of fields empty."""
+    r = np.random.default_rng(seed)
+    ints = r.integers(-2**31, 2**31, n, dtype=np.int64).astype(np.int32)
+    ints[:2] = [-2**31 + 1, 2**31 - 1]
+    flts = (r.standard_normal(n) * 10.0 ** r.integers(-8, 9, n)).astype(np.float32)
+    secs = r.integers(0, 4_000_000_000, n, dtype=np.int64)
+    alphabet = np.array(list('abcXYZ 019,"\n;-'))
+    strs = ["".join(r.choice(alphabet, int(r.integers(0, 300)))) for _ in range(n)]
+    empty = r.random((n, 4)) < 0.05
+    eol = "\r\n" if crlf else "\n"
+    with open(path, "w", newline="") as f:
+        f.write("extra,id,price,ts,name" + eol)
+        for i in range(n):
+            fi = "" if empty[i, 0] else str(int(ints[i]))
+            ff = "" if empty[i, 1] else "%.9g" % float(flts[i])
+            ft = "" if empty[i, 2] else time.strftime("%Y-%m-%d %H:%M:%S", time.gmtime(int(secs[i])))
+            fs = "" if empty[i, 3] or strs[i] == "" else _quote(strs[i])
+            f.write(",".join(["x%d" % i, fi, ff, ft, fs]) + eol)
+    nul = empty.copy()
+    nul[:, 3] |= np.array([s == "" for s in strs])
+    return ints, flts, secs, strs, nul
+
+
+def _read_all(rd):
+    cols, nulls = None, None
+    while True:
+        b = rd.next_host()
+        if b is None:
+            break
+        c, nl = b
+        cols = c if cols is None else [np.concatenate([a, x]) for a, x in zip(cols, c)]
+        nulls = nl if nulls is None else [np.concatenate([a, x]) for a, x in zip(nulls, nl)]
+    return cols, nulls
+
+
+def test_batch_rows_closed_form(loader):
+    for types, rows in (([INT, FLOAT], 524288), ([STRING], 16384), ([DATETIME, STRING, INT], 15616),
+                        ([STRING] * 8, 2048), ([STRING] * 20, 768), ([STRING] * 70, 256)):
+        assert loader.batch_rows(types) == rows == O.batch_rows(types)
+
+
+@pytest.mark.parametrize("crlf", [False, True])
+def test_values_by_construction(loader, tmp_path, crlf):
+    n = 40_000  # 3 batches of 15,360 rows for this schema (272 B per row)
+    p = tmp_path / "t.csv"
+    ints, flts, secs, strs, nul = _gen(p, n, 5 + crlf, crlf)
+    names, types = ["id", "price", "ts", "name"], [INT, FLOAT, DATETIME, STRING]
+    rd = loader.CsvReader(p, names, types)
+    assert rd.rows == 15360
+    cols, nulls = _read_all(rd)
+    rd.close()
+    for c in range(4):
+        assert np.array_equal(nulls[c], nul[:, c].astype(np.uint8)), c
+    e0 = nul[:, 0]
+    assert np.array_equal(cols[0][~e0], ints[~e0]) and np.all(cols[0][e0] == -2**31)
+    e1 = nul[:, 1]
+    assert np.array_equal(cols[1][~e1].view(np.uint32), flts[~e1].view(np.uint32))
+    assert np.all(cols[1][e1] == np.finfo(np.float32).tiny)
+    e2 = nul[:, 2]
+    assert np.array_equal(cols[2][~e2], secs[~e2]) and np.all(cols[2][e2] == 0)
+    for i in range(n):
+        b = b"" if nul[i, 3] else strs[i].encode()[:255]
+        assert bytes(cols[3][i][: len(b)]) == b and not cols[3][i][len(b):].any(), i
+    # the oracle's restatement agrees on the same file
+    oc, on = O.read(p, names, types)
+    for c in range(4):
+        assert np.array_equal(cols[c], oc[c]) and np.array_equal(nulls[c], on[c]), c
+
+
+def test_reference_rules_edge_cases(loader, tmp_path):
+    p = tmp_path / "e.csv"
+    rows = [
+        ["a", "b", "c", "d"],
+        ["12abc", "3.5e2xyz", "2001-02-03 04:05:06", '"quoted, with ""quotes"""'],
+        ["  +7", "  -0.25", "not a date", ""],
+        ["-0", ".5", "1970-01-01 00:00:00", "x" * 400],
+        ["", "", "", '""'],
+        ["2147483647", "inf", " 2020-12-31 23:59:59", '"line\nbreak"'],
+    ]
+    with open(p, "w", newline="") as f:
+        for r in rows:
+            f.write(",".join(r) + "\r\n")
+    names, types = ["a", "b", "c", "d"], [INT, FLOAT, DATETIME, STRING]
+    rd = loader.CsvReader(p, names, types)
+    cols, nulls = _read_all(rd)
+    rd.close()
+    assert list(cols[0]) == [12, 7, 0, -2**31, 2147483647]
+    assert list(cols[1][:3]) == [np.float32(350.0), np.float32(-0.25), np.float32(0.5)]
+    assert cols[1][3] == np.finfo(np.float32).tiny and np.isinf(cols[1][4])
+    assert list(cols[2]) == [calendar.timegm((2001, 2, 3, 4, 5, 6)), 0, 0, 0, calendar.timegm((2020, 12, 31, 23, 59, 59))]
+    assert list(nulls[2]) == [0, 0, 0, 1, 0]  # "not a date" is 0 but not NULL
+    assert bytes(cols[3][0]).rstrip(b"\0") == b'quoted, with "quotes"'
+    assert bytes(cols[3][2]) == b"x" * 255 + b"\0"
+    assert bytes(cols[3][4]).rstrip(b"\0") == b"line\nbreak"
+    assert list(nulls[3]) == [0, 1, 0, 1, 0]
+    oc, on = O.read(p, names, types)
+    for c in range(4):
+        assert np.array_equal(cols[c], oc[c]) and np.array_equal(nulls[c], on[c]), c
+
+
+def test_errors(loader, tmp_path):
+    from yallasql_amd import _lib as L
+
+    p = tmp_path / "bad.csv"
+    p.write_text("a,b\nabc,1\n")
+    rd = loader.CsvReader(p, ["a"], [INT])
+    with pytest.raises(L.YsqlError):
+        rd.next_host()  # no digits: std::stoi throws
+    rd.close()
+    p.write_text("a,b\n99999999999,1\n")
+    rd = loader.CsvReader(p, ["a"], [INT])
+    with pytest.raises(L.YsqlError):
+        rd.next_host()  # out of int32 range
+    rd.close()
+    p.write_text("a,b\n1\n")
+    rd = loader.CsvReader(p, ["b"], [INT])
+    with pytest.raises(L.YsqlError):
+        rd.next_host()  # record shorter than the header
+    rd.close()
+    with pytest.raises(L.YsqlError):
+        loader.CsvReader(p, ["zz"], [INT])  # no such column
+    with pytest.raises(L.YsqlError):
+        loader.CsvReader(os.path.join(str(tmp_path), "missing.csv"), ["a"], [INT])
+
+
+@pytest.mark.gpu
+def test_device_path_matches_host(loader, tmp_path, dev):
+    import torch
+
+    n = 100_000
+    p = tmp_path / "g.csv"
+    _gen(p, n, 11)
+    names, types = ["id", "price", "ts", "name"], [INT, FLOAT, DATETIME, STRING]
+    rd = loader.CsvReader(p, names, types)
+    hc, hn = _read_all(rd)
+    rd.close()
+    rd = loader.CsvReader(p, names, types)
+    s = torch.cuda.Stream(device=dev)
+    got = []
+    while True:  # several batches: each parse overlaps the previous batch's copies
+        b = rd.next_device(dev, stream=s)
+        if b is None:
+            break
+        got.append(b)
+    s.synchronize()
+    rd.close()
+    for c in range(4):
+        dc = torch.cat([b[0][c] for b in got]).cpu().numpy()
+        dn = torch.cat([b[1][c] for b in got]).cpu().numpy()
+        assert np.array_equal(dc, hc[c]) and np.array_equal(dn, hn[c]), c

@@ -15,6 +15,11 @@ if not os.path.exists(LIB_PATH):
         "(or __graft_entry__.build()) first"
     )
 
+# torch (ROCm build) brings its own HIP runtime; load it first so that the
+# library binds to that same runtime instance instead of a second one (the
+# Python layers hand torch's device memory and streams to the library).
+import torch  # noqa: E402,F401
+
 lib = C.CDLL(LIB_PATH)
 
 # ---- constants (mirror include/ysql_kernels.h) ---------------------------
@@ -78,6 +83,10 @@ _SIGS = {
     "ysql_compact": [U32, P, P, P, P, U64, P, P, P],
     "ysql_compact_ws_bytes": [U64, SZP],
     "ysql_compact_cols": [P, C.POINTER(Col), I32, U64, P, P, C.c_size_t, P],
+    "ysql_csv_batch_rows": [I32, C.POINTER(I32), C.POINTER(U32)],
+    "ysql_csv_open": [C.c_char_p, I32, C.POINTER(C.c_char_p), C.POINTER(I32), C.POINTER(P)],
+    "ysql_csv_next_host": [P, C.POINTER(P), C.POINTER(P), C.POINTER(U32)],
+    "ysql_csv_next": [P, C.POINTER(P), C.POINTER(P), C.POINTER(U32), P],
     "ysql_filter_compact": [I32, I32, P, P, C.POINTER(Col), I32, U64, P, P, C.c_size_t, P],
     "ysql_filter_compact_range": [I32, P, P, C.POINTER(Col), I32, U64, U64, P, P, C.c_size_t, P],
     "ysql_compare": [I32, I32, P, I32, P, I32, I32, P, U64, P],
@@ -106,6 +115,8 @@ lib.ysql_version.restype = C.c_char_p
 lib.ysql_version.argtypes = []
 lib.ysql_last_error.restype = C.c_char_p
 lib.ysql_last_error.argtypes = []
+lib.ysql_csv_close.restype = None
+lib.ysql_csv_close.argtypes = [P]
 
 
 class YsqlError(RuntimeError):

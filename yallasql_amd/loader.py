@@ -1,0 +1,83 @@
+"""CSV file -> column batches: the scan path's feeder over the C ABI
+(include/ysql_csv.h), mirroring the reference's GetOperator
+(src/engine/operators/get_operator.cpp:16-148): open with the header names
+and types of the requested columns, then next() until it returns no rows."""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+
+_NP = {L.INT: np.int32, L.FLOAT: np.float32, L.DATETIME: np.int64}
+
+
+def batch_rows(types):
+    """Rows per batch for these column types (config.hpp:28-38)."""
+    t = (C.c_int * len(types))(*types)
+    out = C.c_uint32()
+    L.call("ysql_csv_batch_rows", len(types), t, C.byref(out))
+    return out.value
+
+
+class CsvReader:
+    def __init__(self, path, names, types):
+        self.types = list(types)
+        self.rows = batch_rows(self.types)
+        n = len(names)
+        cn = (C.c_char_p * n)(*[x.encode() for x in names])
+        ct = (C.c_int * n)(*self.types)
+        h = C.c_void_p()
+        L.call("ysql_csv_open", str(path).encode(), n, cn, ct, C.byref(h))
+        self.handle = h
+
+    def _host_bufs(self):
+        cols = []
+        for t in self.types:
+            if t == L.STRING:
+                cols.append(np.zeros((self.rows, L.STRING_BYTES), np.uint8))
+            else:
+                cols.append(np.zeros(self.rows, _NP[t]))
+        nulls = [np.zeros(self.rows, np.uint8) for _ in self.types]
+        return cols, nulls
+
+    def next_host(self):
+        """The next batch as numpy arrays: ([column], [null bytes]); None at the end."""
+        cols, nulls = self._host_bufs()
+        n = len(cols)
+        pc = (C.c_void_p * n)(*[a.ctypes.data for a in cols])
+        pn = (C.c_void_p * n)(*[a.ctypes.data for a in nulls])
+        rows = C.c_uint32()
+        L.call("ysql_csv_next_host", self.handle, pc, pn, C.byref(rows))
+        if rows.value == 0:
+            return None
+        return [a[: rows.value] for a in cols], [a[: rows.value] for a in nulls]
+
+    def next_device(self, device, stream=None):
+        """The next batch as device tensors ([column], [null bytes]), copied
+        asynchronously on `stream` (default: the current stream); None at the end."""
+        import torch
+
+        tdt = {L.INT: torch.int32, L.FLOAT: torch.float32, L.DATETIME: torch.int64}
+        cols = [torch.empty((self.rows, L.STRING_BYTES), dtype=torch.uint8, device=device) if t == L.STRING
+                else torch.empty(self.rows, dtype=tdt[t], device=device) for t in self.types]
+        nulls = [torch.empty(self.rows, dtype=torch.uint8, device=device) for _ in self.types]
+        n = len(cols)
+        pc = (C.c_void_p * n)(*[a.data_ptr() for a in cols])
+        pn = (C.c_void_p * n)(*[a.data_ptr() for a in nulls])
+        rows = C.c_uint32()
+        s = stream if stream is not None else torch.cuda.current_stream(device)
+        L.call("ysql_csv_next", self.handle, pc, pn, C.byref(rows), C.c_void_p(s.cuda_stream))
+        if rows.value == 0:
+            return None
+        return [a[: rows.value] for a in cols], [a[: rows.value] for a in nulls]
+
+    def close(self):
+        if getattr(self, "handle", None) is not None and L.lib is not None:
+            L.lib.ysql_csv_close(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
