@@ -9,7 +9,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cerrno>
+#include <charconv>
 #include <climits>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -60,70 +62,83 @@ struct ysql_csv_reader {
   uint32_t batch_rows = 0;
   Stage st[2];
   int cur = 0;
-  std::vector<std::string> rec;  // the current record's fields
+  std::vector<std::string> rec;  // the current record's fields: rec[0 .. nrec)
+  size_t nrec = 0;
 
+  // refills the buffer; false at the end of the file
+  bool refill() {
+    if (eof) return false;
+    len = fread(buf.data(), 1, buf.size(), f);
+    pos = 0;
+    if (len == 0) eof = true;
+    return len > 0;
+  }
   // next byte of the file, -1 at the end
   int get() {
-    if (pos == len) {
-      if (eof) return -1;
-      len = fread(buf.data(), 1, buf.size(), f);
-      pos = 0;
-      if (len == 0) {
-        eof = true;
-        return -1;
-      }
-    }
+    if (pos == len && !refill()) return -1;
     return (unsigned char)buf[pos++];
   }
   int peek() {
-    const int c = get();
-    if (c >= 0) pos--;
-    return c;
+    if (pos == len && !refill()) return -1;
+    return (unsigned char)buf[pos];
   }
 
   // Reads one record into rec; false at the end of the file.  RFC 4180:
   // fields separated by ',', records by "\n" or "\r\n", a field in quotes may
-  // hold separators and newlines, "" inside quotes is one quote.
+  // hold separators and newlines, "" inside quotes is one quote; a quote
+  // that does not open a field is an ordinary character.  Runs of ordinary
+  // characters are copied in bulk.
   bool read_record() {
-    rec.clear();
-    int c = get();
-    if (c < 0) return false;
-    std::string fld;
-    bool quoted = false, any = false;
+    nrec = 0;
+    if (pos == len && !refill()) return false;
+    std::string *fld = next_field();
+    bool quoted = false;
     while (true) {
-      if (quoted) {
-        if (c < 0) break;  // unterminated quote: the field ends at the end of the file
-        if (c == '"') {
-          if (peek() == '"') {
-            get();
-            fld.push_back('"');
-          } else {
-            quoted = false;
-          }
-        } else {
-          fld.push_back((char)c);
-        }
-      } else {
-        if (c < 0 || c == '\n') break;
-        if (c == '\r' && peek() == '\n') {
-          get();
-          break;
-        }
+      if (pos == len && !refill()) break;  // end of file ends the record
+      const char *p = buf.data() + pos, *e = buf.data() + len, *q = p;
+      if (!quoted) {
+        while (q < e && *q != ',' && *q != '\n' && *q != '\r' && *q != '"') q++;
+        fld->append(p, (size_t)(q - p));
+        pos += (size_t)(q - p);
+        if (pos == len) continue;
+        const char c = buf[pos++];
         if (c == ',') {
-          rec.push_back(fld);
-          fld.clear();
-        } else if (c == '"' && fld.empty()) {
+          fld = next_field();
+        } else if (c == '\n') {
+          break;
+        } else if (c == '\r') {
+          if (peek() == '\n') {
+            pos++;
+            break;
+          }
+          fld->push_back('\r');
+        } else if (fld->empty()) {  // '"' opening a field
           quoted = true;
         } else {
-          fld.push_back((char)c);
+          fld->push_back('"');
+        }
+      } else {
+        while (q < e && *q != '"') q++;
+        fld->append(p, (size_t)(q - p));
+        pos += (size_t)(q - p);
+        if (pos == len) continue;
+        pos++;  // the quote
+        if (peek() == '"') {
+          pos++;
+          fld->push_back('"');
+        } else {
+          quoted = false;
         }
       }
-      any = true;
-      c = get();
     }
-    rec.push_back(fld);
-    (void)any;
     return true;
+  }
+  // the next field of the current record (strings are reused across records)
+  std::string *next_field() {
+    if (nrec == rec.size()) rec.emplace_back();
+    std::string *f = &rec[nrec++];
+    f->clear();
+    return f;
   }
 };
 
@@ -138,10 +153,18 @@ bool convert(int type, const std::string &s, void *col, uint8_t *nulls, uint32_t
     case YSQL_INT: {
       int32_t v = std::numeric_limits<int32_t>::min();
       if (!empty) {
-        errno = 0;
-        char *end = nullptr;
-        const long x = strtol(s.c_str(), &end, 10);
-        if (end == s.c_str() || errno == ERANGE || x < INT_MIN || x > INT_MAX) return false;
+        // plain [-]digits prefix: from_chars gives what strtol gives; any
+        // other form (blanks, '+') takes strtol itself
+        long x = 0;
+        const auto fc = std::from_chars(s.data(), s.data() + s.size(), x, 10);
+        if (fc.ec == std::errc::result_out_of_range) return false;
+        if (fc.ec != std::errc()) {
+          errno = 0;
+          char *end = nullptr;
+          x = strtol(s.c_str(), &end, 10);
+          if (end == s.c_str() || errno == ERANGE) return false;
+        }
+        if (x < INT_MIN || x > INT_MAX) return false;
         v = (int32_t)x;
       }
       memcpy((char *)col + (size_t)row * 4, &v, 4);
@@ -150,10 +173,18 @@ bool convert(int type, const std::string &s, void *col, uint8_t *nulls, uint32_t
     case YSQL_FLOAT: {
       float v = std::numeric_limits<float>::min();
       if (!empty) {
-        errno = 0;
-        char *end = nullptr;
-        v = strtof(s.c_str(), &end);
-        if (end == s.c_str() || errno == ERANGE) return false;
+        // plain decimal prefix: from_chars rounds correctly, as strtof does;
+        // hex forms, blanks, '+', range errors and zero or subnormal results
+        // take strtof itself (its errno rules decide those)
+        const char *b = s.data() + (s[0] == '-');
+        const bool hex = s.size() > (size_t)(b - s.data()) + 1 && b[0] == '0' && (b[1] == 'x' || b[1] == 'X');
+        const auto fc = std::from_chars(s.data(), s.data() + s.size(), v);
+        if (hex || fc.ec != std::errc() || !(std::fabs(v) >= std::numeric_limits<float>::min())) {
+          errno = 0;
+          char *end = nullptr;
+          v = strtof(s.c_str(), &end);
+          if (end == s.c_str() || errno == ERANGE) return false;
+        }
       }
       memcpy((char *)col + (size_t)row * 4, &v, 4);
       return true;
@@ -184,8 +215,8 @@ int parse_batch(ysql_csv_reader *r, void *const *cols, uint8_t *const *nulls, ui
   uint32_t n = 0;
   const int nc = (int)r->types.size();
   while (n < r->batch_rows && r->read_record()) {
-    if (r->rec.size() == 1 && r->rec[0].empty() && r->nfields > 1) continue;  // blank line
-    if ((int)r->rec.size() < r->nfields) {
+    if (r->nrec == 1 && r->rec[0].empty() && r->nfields > 1) continue;  // blank line
+    if ((int)r->nrec < r->nfields) {
       ysql::set_last_error("csv: record with fewer fields than the header");
       return YSQL_EINVAL;
     }
@@ -252,7 +283,7 @@ int ysql_csv_open(const char *path, int ncols, const char *const *names, const i
     delete r;
     return YSQL_EINVAL;
   }
-  const std::vector<std::string> header = r->rec;
+  const std::vector<std::string> header(r->rec.begin(), r->rec.begin() + r->nrec);
   r->nfields = (int)header.size();
   for (int c = 0; c < ncols; c++) {
     if (!names[c]) rc = YSQL_EINVAL;
