@@ -606,6 +606,26 @@ def test_join_eq_hash_probe(ops, dev, L_, dtype, R_):
     assert np.array_equal(N(pairs).view(np.uint32), exp)
 
 
+@pytest.mark.parametrize("dtype", [np.int32, np.float32, np.int64])
+def test_join_eq_large_filter(ops, dev, dtype):
+    """EQ hash join whose right side (20,000 rows) takes the largest
+    membership filter: left keys hit, miss and land in long probe chains."""
+    r = rng(77)
+    L_, R_ = 32_769, 20_000
+    right = r.integers(0, 60_000, R_).astype(dtype)
+    right[:500] = right[500]  # one key with 501 right rows
+    left = r.integers(0, 120_000, L_ + 1).astype(dtype)
+    left[::1000] = right[500]
+    if dtype == np.float32:
+        left[::31] = np.nan
+        right[::17] = np.nan
+    lt = T(left, dev)[1:]
+    pairs, c = ops.join_pairs(4, lt, T(right, dev))
+    exp = O.join_pairs(4, left[1:], right)
+    assert c == exp.shape[0] > 10_000
+    assert np.array_equal(N(pairs).view(np.uint32), exp)
+
+
 def test_join_overflow_reports_count(ops, dev):
     left = np.zeros(100, np.int32)
     right = np.zeros(50, np.int32)

@@ -87,15 +87,24 @@ __device__ __forceinline__ uint64_t upper_bound_u(const U *a, uint64_t n, U v, u
 
 // One thread per run start of the sorted keys inserts (key, start, count).
 // Keys whose rows are NaN (counted in nan_count, sorted last) are skipped.
+// When bloom is given, each key also sets its two bits in a 2^lgb-bit
+// membership filter (bits h >> (32 - lgb) and (h * 0x9e3779b1) >> (32 - lgb)
+// of its hash h), which the filtered probe stages in LDS.
 template <typename U>
 __global__ void k_join_build(const U *skeys, uint64_t R, const uint32_t *nan_count, HashSlot *table,
-                             uint32_t *used, uint32_t mask) {
+                             uint32_t *used, uint32_t mask, uint32_t *bloom, int lgb) {
   const uint64_t valid = R - *nan_count;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < valid; i += (uint64_t)gridDim.x * blockDim.x) {
     const U k = skeys[i];
     if (i > 0 && skeys[i - 1] == k) continue;
     const uint64_t end = upper_bound_u<U>(skeys, valid, k, i + 1);
-    uint32_t h = hash_u64((uint64_t)k) & mask;
+    const uint32_t hk = hash_u64((uint64_t)k);
+    if (bloom) {
+      const uint32_t b1 = hk >> (32 - lgb), b2 = (hk * 0x9e3779b1u) >> (32 - lgb);
+      atomicOr(&bloom[b1 >> 5], 1u << (b1 & 31));
+      atomicOr(&bloom[b2 >> 5], 1u << (b2 & 31));
+    }
+    uint32_t h = hk & mask;
     while (atomicCAS(&used[h], 0u, 1u) != 0u) h = (h + 1) & mask;
     table[h].key = (uint64_t)k;
     table[h].start = (uint32_t)i;
@@ -109,89 +118,10 @@ __device__ __forceinline__ uint32_t take_tile_j(uint32_t *ctr, uint32_t *s_tile)
   return *s_tile;
 }
 
-// Probe: each thread owns kProbeRpt consecutive left rows of a tile.  The
-// keys come in with 128-bit loads; every row's first slot (its used flag
-// and its 16-byte entry) is loaded before any is examined, so the lookups
-// of a thread overlap; rows whose first slot is taken by another key walk
-// the probe sequence one slot at a time.
+// Probes: each thread owns kProbeRpt consecutive left rows of a block of
+// kProbeTile rows; the keys come in with 128-bit loads.
 constexpr int kProbeRpt = 16;
 constexpr uint64_t kProbeTile = (uint64_t)kJoinBlock * kProbeRpt;
-
-template <typename T>
-__global__ __launch_bounds__(kJoinBlock) void k_join_probe(const T *left, uint64_t L, const HashSlot *table,
-                                                           const uint32_t *used, uint32_t mask, const uint32_t *perm,
-                                                           uint32_t *pairs, uint64_t cap, uint64_t *count_out,
-                                                           uint32_t *ctr, uint64_t *status, uint32_t ntiles) {
-  __shared__ uint64_t s_red[kJoinBlock / kWave + 1];
-  __shared__ uint32_t s_tile;
-  __shared__ uint64_t s_prefix;
-  const uint32_t tile = take_tile_j(ctr, &s_tile);
-  const uint64_t row0 = (uint64_t)tile * kProbeTile + (uint64_t)threadIdx.x * kProbeRpt;
-  T v[kProbeRpt];
-  if (row0 + kProbeRpt <= L && ((uintptr_t)(left + row0) & 15) == 0) {
-    constexpr int NQ = kProbeRpt * sizeof(T) / 16;
-    uint4 q[NQ];
-    const uint4 *src = reinterpret_cast<const uint4 *>(left + row0);
-#pragma unroll
-    for (int i = 0; i < NQ; i++) q[i] = src[i];
-    memcpy(&v[0], &q[0], sizeof(q));
-  } else {
-#pragma unroll
-    for (int e = 0; e < kProbeRpt; e++) v[e] = row0 + e < L ? left[row0 + e] : T();
-  }
-  uint32_t h[kProbeRpt], u[kProbeRpt];
-  HashSlot slot[kProbeRpt];
-#pragma unroll
-  for (int e = 0; e < kProbeRpt; e++) {
-    const bool live = row0 + e < L && JoinKey<T>::ok(v[e]);
-    h[e] = hash_u64((uint64_t)JoinKey<T>::bits(v[e])) & mask;
-    u[e] = live ? used[h[e]] : 0u;
-    slot[e] = live ? table[h[e]] : HashSlot{0, 0, 0};
-  }
-  uint32_t st[kProbeRpt], cn[kProbeRpt];
-  uint64_t mine = 0;
-#pragma unroll
-  for (int e = 0; e < kProbeRpt; e++) {
-    st[e] = 0;
-    cn[e] = 0;
-    const uint64_t k = (uint64_t)JoinKey<T>::bits(v[e]);
-    if (u[e]) {
-      if (slot[e].key == k) {
-        st[e] = slot[e].start;
-        cn[e] = slot[e].count;
-      } else {
-        uint32_t hh = (h[e] + 1) & mask;
-        while (used[hh]) {
-          if (table[hh].key == k) {
-            st[e] = table[hh].start;
-            cn[e] = table[hh].count;
-            break;
-          }
-          hh = (hh + 1) & mask;
-        }
-      }
-    }
-    mine += cn[e];
-  }
-  uint64_t total;
-  const uint64_t excl = block_exclusive_sum<kJoinBlock, uint64_t>(mine, s_red, &total);
-  if (threadIdx.x < kWave) {
-    const uint64_t p = lookback_exclusive(status, tile, total);
-    if (threadIdx.x == 0) s_prefix = p;
-  }
-  __syncthreads();
-  uint64_t pos = s_prefix + excl;
-#pragma unroll
-  for (int e = 0; e < kProbeRpt; e++) {
-    for (uint32_t q = 0; q < cn[e]; q++, pos++) {
-      if (pos < cap) {
-        pairs[2 * pos] = (uint32_t)(row0 + e);
-        pairs[2 * pos + 1] = perm[st[e] + q];
-      }
-    }
-  }
-  if (tile == ntiles - 1 && threadIdx.x == 0) *count_out = s_prefix + total;
-}
 
 // Same probe with the hash table staged in LDS, for tables of up to
 // kLdsSlots slots (right sides of up to kLdsSlots / 2 rows).  The
@@ -321,6 +251,107 @@ __global__ __launch_bounds__(kJoinBlock) void k_join_probe_lds(const T *left, ui
         uint32_t slot = hash_u64(key) & mask, sc;
         while ((sc = s_sc[slot]) != 0u && table[slot].key != key) slot = (slot + 1) & mask;
         const uint32_t st = sc >> 16, cn = sc & 0xffffu;
+        for (uint32_t q = 0; q < cn; q++, pos++) {
+          if (pos < cap) {
+            pairs[2 * pos] = (uint32_t)(row0 + e);
+            pairs[2 * pos + 1] = perm[st + q];
+          }
+        }
+      }
+    }
+    if (tile == ntiles - 1 && threadIdx.x == 0) *count_out = prefix + total;
+    __syncthreads();  // s_prefix and s_red are reused by the next tile
+  }
+}
+
+// Same probe for hash tables too large for LDS.  The build kernel sets each
+// key's two bits in a 2^LGB-bit membership filter in global memory; the
+// persistent workgroups copy it into LDS once, so most non-matching rows
+// are rejected with two LDS reads instead of reading the table.  The rows
+// that pass probe the global table in rounds (every pending row of a
+// thread advances one slot per round, so their reads overlap), and the
+// tile structure (kLdsSub blocks, one look-back, hit masks re-probed at
+// write time) is that of the LDS-table probe.
+template <typename T, int LGB>
+__global__ __launch_bounds__(kJoinBlock) void k_join_probe_bloom(const T *left, uint64_t L, const HashSlot *table,
+                                                                 const uint32_t *used, uint32_t mask,
+                                                                 const uint32_t *bloom, const uint32_t *perm,
+                                                                 uint32_t *pairs, uint64_t cap, uint64_t *count_out,
+                                                                 uint32_t *ctr, uint64_t *status, uint32_t ntiles) {
+  constexpr uint32_t kWords = 1u << (LGB - 5);
+  __shared__ uint32_t s_bits[kWords];
+  __shared__ uint64_t s_red[kJoinBlock / kWave + 1];
+  __shared__ uint32_t s_tile;
+  __shared__ uint64_t s_prefix;
+  for (uint32_t i = threadIdx.x; i < kWords; i += kJoinBlock) s_bits[i] = bloom[i];
+  while (true) {
+    const uint32_t tile = take_tile_j(ctr, &s_tile);  // (its barrier also publishes the filter)
+    if (tile >= ntiles) return;
+    uint32_t mine[kLdsSub], hit[kLdsSub];
+#pragma unroll
+    for (int sub = 0; sub < kLdsSub; sub++) {
+      const uint64_t row0 = ((uint64_t)tile * kLdsSub + sub) * kProbeTile + (uint64_t)threadIdx.x * kProbeRpt;
+      T v[kProbeRpt];
+      probe_load<T>(left, L, row0, v);
+      uint32_t slot[kProbeRpt];
+      uint32_t pend = 0;  // bit e: row e still probing
+#pragma unroll
+      for (int e = 0; e < kProbeRpt; e++) {
+        slot[e] = 0;
+        if (row0 + e < L && JoinKey<T>::ok(v[e])) {
+          const uint32_t h = hash_u64((uint64_t)JoinKey<T>::bits(v[e]));
+          const uint32_t b1 = h >> (32 - LGB), b2 = (h * 0x9e3779b1u) >> (32 - LGB);
+          if ((s_bits[b1 >> 5] >> (b1 & 31)) & (s_bits[b2 >> 5] >> (b2 & 31)) & 1u) {
+            slot[e] = h & mask;
+            pend |= 1u << e;
+          }
+        }
+      }
+      uint32_t cnt = 0, hm = 0;
+      while (pend) {
+#pragma unroll
+        for (int e = 0; e < kProbeRpt; e++) {
+          if ((pend >> e) & 1u) {
+            const uint32_t u = used[slot[e]];
+            const HashSlot hs = table[slot[e]];
+            if (u == 0u) {
+              pend &= ~(1u << e);
+            } else if (hs.key == (uint64_t)JoinKey<T>::bits(v[e])) {
+              cnt += hs.count;
+              hm |= 1u << e;
+              pend &= ~(1u << e);
+            } else {
+              slot[e] = (slot[e] + 1) & mask;
+            }
+          }
+        }
+      }
+      mine[sub] = cnt;
+      hit[sub] = hm;
+    }
+    uint64_t excl[kLdsSub], total = 0;
+#pragma unroll
+    for (int sub = 0; sub < kLdsSub; sub++) {
+      uint64_t t_sub;
+      excl[sub] = total + block_exclusive_sum<kJoinBlock, uint64_t>(mine[sub], s_red, &t_sub);
+      total += t_sub;
+    }
+    if (threadIdx.x < kWave) {
+      const uint64_t p = lookback_exclusive(status, tile, total);
+      if (threadIdx.x == 0) s_prefix = p;
+    }
+    __syncthreads();
+    const uint64_t prefix = s_prefix;
+    for (int sub = 0; sub < kLdsSub; sub++) {
+      if (!hit[sub]) continue;
+      const uint64_t row0 = ((uint64_t)tile * kLdsSub + sub) * kProbeTile + (uint64_t)threadIdx.x * kProbeRpt;
+      uint64_t pos = prefix + excl[sub];
+      for (uint32_t hm = hit[sub]; hm; hm &= hm - 1) {
+        const int e = __builtin_ctz(hm);
+        const uint64_t key = (uint64_t)JoinKey<T>::bits(left[row0 + e]);
+        uint32_t slot = hash_u64(key) & mask;
+        while (used[slot] && table[slot].key != key) slot = (slot + 1) & mask;
+        const uint32_t st = table[slot].start, cn = table[slot].count;
         for (uint32_t q = 0; q < cn; q++, pos++) {
           if (pos < cap) {
             pairs[2 * pos] = (uint32_t)(row0 + e);
@@ -468,12 +499,14 @@ __global__ __launch_bounds__(kJoinBlock) void k_pairs_probe(const uint32_t *p1, 
   if (tile == ntiles - 1 && threadIdx.x == 0) *count_out = s_prefix + total;
 }
 
-__global__ void k_join_reset(uint32_t *ctr, uint64_t *status, uint32_t ntiles, uint32_t *used, uint32_t nused) {
+__global__ void k_join_reset(uint32_t *ctr, uint64_t *status, uint32_t ntiles, uint32_t *used, uint32_t nused,
+                             uint32_t *bloom = nullptr, uint32_t nbloom = 0) {
   const uint32_t i0 = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t stride = gridDim.x * blockDim.x;
   if (i0 < 64) ctr[i0] = 0;
   for (uint32_t i = i0; i < ntiles; i += stride) status[i] = 0;
   for (uint32_t i = i0; i < nused; i += stride) used[i] = 0;
+  for (uint32_t i = i0; i < nbloom; i += stride) bloom[i] = 0;
 }
 
 __global__ void k_store_zero_u64(uint64_t *p) { *p = 0; }
@@ -486,9 +519,15 @@ struct JoinWs {
   uint32_t *perm;     // R
   HashSlot *table;    // cap
   uint32_t *used;     // cap
+  uint32_t *bloom;    // kBloomMaxWords (membership filter of the global-table probe)
   void *sort_ws;
   size_t sort_ws_bytes;
 };
+
+// Filter of the global-table probe: 2^17 bits while that gives the right
+// side's keys 16 bits each, else 2^19 bits (64 KB of LDS).
+constexpr uint32_t kBloomMaxWords = (1u << 19) / 32;
+static int bloom_lgb(uint64_t R) { return 16 * R <= (1u << 17) ? 17 : 19; }
 
 static uint32_t hash_cap(uint64_t R) {
   uint64_t c = 64;
@@ -518,6 +557,7 @@ static size_t join_ws_layout(int dtype, int cmp, uint64_t L, uint64_t R, void *w
       c.take<uint32_t>(R + 1);
       c.take<HashSlot>(cap);
       c.take<uint32_t>(cap);
+      c.take<uint32_t>(kBloomMaxWords);
       c.take<char>(sort_bytes);
     }
     return c.off;
@@ -531,11 +571,27 @@ static size_t join_ws_layout(int dtype, int cmp, uint64_t L, uint64_t R, void *w
     out->perm = c.take<uint32_t>(R + 1);
     out->table = c.take<HashSlot>(cap);
     out->used = c.take<uint32_t>(cap);
+    out->bloom = c.take<uint32_t>(kBloomMaxWords);
     out->sort_ws = c.take<char>(sort_bytes);
     out->sort_ws_bytes = sort_bytes;
-    ok = ok && out->keys && out->perm && out->table && out->used && out->sort_ws;
+    ok = ok && out->keys && out->perm && out->table && out->used && out->bloom && out->sort_ws;
   }
   return ok ? c.off : 0;
+}
+
+template <typename T, int LGB>
+static void launch_probe_bloom(const T *left, uint64_t L, const JoinWs &w, uint32_t tcap, uint32_t *pairs,
+                               uint64_t cap, uint64_t *count_out, uint32_t nt, hipStream_t s) {
+  static const uint32_t per_cu = [] {
+    int b = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_join_probe_bloom<T, LGB>, kJoinBlock, 0) != hipSuccess ||
+        b < 1)
+      b = 1;
+    return (uint32_t)b;
+  }();
+  const uint32_t g = std::min<uint32_t>(nt, per_cu * (uint32_t)grid_cap());
+  k_join_probe_bloom<T, LGB><<<std::max<uint32_t>(g, 1), kJoinBlock, 0, s>>>(
+      left, L, w.table, w.used, tcap - 1, w.bloom, w.perm, pairs, cap, count_out, w.ctr, w.status, nt);
 }
 
 template <typename T>
@@ -544,7 +600,10 @@ static int join_hash(const T *left, uint64_t L, const T *right, uint64_t R, uint
   typedef typename JoinKey<T>::U U;
   const uint32_t tcap = hash_cap(R);
   const uint32_t ntiles = (uint32_t)ceil_div(L, kProbeTile);
-  k_join_reset<<<256, 256, 0, s>>>(w.ctr, w.status, ntiles + 1, w.used, tcap);
+  const bool filtered = tcap > kLdsSlots;
+  const int lgb = bloom_lgb(R);
+  k_join_reset<<<256, 256, 0, s>>>(w.ctr, w.status, ntiles + 1, w.used, tcap, w.bloom,
+                                   filtered ? (1u << lgb) / 32 : 0u);
   YSQL_LAUNCH_CHECK();
   uint64_t g = ceil_div(R, 256);
   if (g > 4096) g = 4096;
@@ -555,7 +614,8 @@ static int join_hash(const T *left, uint64_t L, const T *right, uint64_t R, uint
     const int kdt = sizeof(U) == 8 ? YSQL_UKEY64 : YSQL_UKEY32;
     int rc = ysql_radix_argsort(kdt, 0, w.keys, w.perm, R, 0, 8 * sizeof(U), w.sort_ws, w.sort_ws_bytes, s);
     if (rc) return rc;
-    k_join_build<U><<<(unsigned)g, 256, 0, s>>>((const U *)w.keys, R, w.ctr + 1, w.table, w.used, tcap - 1);
+    k_join_build<U><<<(unsigned)g, 256, 0, s>>>((const U *)w.keys, R, w.ctr + 1, w.table, w.used, tcap - 1,
+                                                filtered ? w.bloom : nullptr, lgb);
     YSQL_LAUNCH_CHECK();
   }
   if (tcap <= kLdsSlots) {
@@ -570,8 +630,11 @@ static int join_hash(const T *left, uint64_t L, const T *right, uint64_t R, uint
     k_join_probe_lds<T><<<std::max<uint32_t>(g, 1), kJoinBlock, 0, s>>>(left, L, w.table, w.used, tcap - 1, w.perm,
                                                                         pairs, cap, count_out, w.ctr, w.status, nt);
   } else {
-    k_join_probe<T><<<ntiles, kJoinBlock, 0, s>>>(left, L, w.table, w.used, tcap - 1, w.perm, pairs, cap, count_out,
-                                                  w.ctr, w.status, ntiles);
+    const uint32_t nt = (uint32_t)ceil_div(L, (uint64_t)kLdsSub * kProbeTile);
+    if (lgb == 17)
+      launch_probe_bloom<T, 17>(left, L, w, tcap, pairs, cap, count_out, nt, s);
+    else
+      launch_probe_bloom<T, 19>(left, L, w, tcap, pairs, cap, count_out, nt, s);
   }
   YSQL_LAUNCH_CHECK();
   return YSQL_OK;
