@@ -608,8 +608,9 @@ def test_join_eq_hash_probe(ops, dev, L_, dtype, R_):
 
 @pytest.mark.parametrize("dtype", [np.int32, np.float32, np.int64])
 def test_join_eq_large_filter(ops, dev, dtype):
-    """EQ hash join whose right side (20,000 rows) takes the largest
-    membership filter: left keys hit, miss and land in long probe chains."""
+    """EQ hash join whose right side (20,000 rows) takes the smallest
+    blocked filter in global memory (2^20 bits): left keys hit, miss and
+    land in long probe chains."""
     r = rng(77)
     L_, R_ = 32_769, 20_000
     right = r.integers(0, 60_000, R_).astype(dtype)
@@ -619,6 +620,26 @@ def test_join_eq_large_filter(ops, dev, dtype):
     if dtype == np.float32:
         left[::31] = np.nan
         right[::17] = np.nan
+    lt = T(left, dev)[1:]
+    pairs, c = ops.join_pairs(4, lt, T(right, dev))
+    exp = O.join_pairs(4, left[1:], right)
+    assert c == exp.shape[0] > 10_000
+    assert np.array_equal(N(pairs).view(np.uint32), exp)
+
+
+@pytest.mark.parametrize("dtype", [np.int32, np.float32])
+def test_join_eq_blocked_filter(ops, dev, dtype):
+    """EQ hash join with 70,000 right rows: a blocked filter in global
+    memory of 16 bits per right row (2^21 bits)."""
+    r = rng(78)
+    L_, R_ = 8_193, 70_000
+    right = r.integers(0, 200_000, R_).astype(dtype)
+    right[:300] = right[300]
+    left = r.integers(0, 400_000, L_ + 1).astype(dtype)
+    left[::100] = right[300]
+    if dtype == np.float32:
+        left[::37] = np.nan
+        right[::19] = np.nan
     lt = T(left, dev)[1:]
     pairs, c = ops.join_pairs(4, lt, T(right, dev))
     exp = O.join_pairs(4, left[1:], right)

@@ -87,19 +87,28 @@ __device__ __forceinline__ uint64_t upper_bound_u(const U *a, uint64_t n, U v, u
 
 // One thread per run start of the sorted keys inserts (key, start, count).
 // Keys whose rows are NaN (counted in nan_count, sorted last) are skipped.
-// When bloom is given, each key also sets its two bits in a 2^lgb-bit
-// membership filter (bits h >> (32 - lgb) and (h * 0x9e3779b1) >> (32 - lgb)
-// of its hash h), which the filtered probe stages in LDS.
+// When bloom is given, each key also sets two bits of a 2^lgb-bit membership
+// filter.  Split filter (staged in LDS by the probe): bits h >> (32 - lgb)
+// and (h * 0x9e3779b1) >> (32 - lgb) of its hash h.  Blocked filter (read
+// from global memory, one word per row): both bits in word h >> (37 - lgb),
+// at bit positions g >> 27 and (g >> 22) & 31 of g = h * 0x9e3779b1.
+__device__ __forceinline__ uint32_t bloom_word_bits(uint32_t h) {
+  const uint32_t g = h * 0x9e3779b1u;
+  return (1u << (g >> 27)) | (1u << ((g >> 22) & 31u));
+}
+
 template <typename U>
 __global__ void k_join_build(const U *skeys, uint64_t R, const uint32_t *nan_count, HashSlot *table,
-                             uint32_t *used, uint32_t mask, uint32_t *bloom, int lgb) {
+                             uint32_t *used, uint32_t mask, uint32_t *bloom, int lgb, bool blocked) {
   const uint64_t valid = R - *nan_count;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < valid; i += (uint64_t)gridDim.x * blockDim.x) {
     const U k = skeys[i];
     if (i > 0 && skeys[i - 1] == k) continue;
     const uint64_t end = upper_bound_u<U>(skeys, valid, k, i + 1);
     const uint32_t hk = hash_u64((uint64_t)k);
-    if (bloom) {
+    if (bloom && blocked) {
+      atomicOr(&bloom[hk >> (37 - lgb)], bloom_word_bits(hk));
+    } else if (bloom) {
       const uint32_t b1 = hk >> (32 - lgb), b2 = (hk * 0x9e3779b1u) >> (32 - lgb);
       atomicOr(&bloom[b1 >> 5], 1u << (b1 & 31));
       atomicOr(&bloom[b2 >> 5], 1u << (b2 & 31));
@@ -264,26 +273,29 @@ __global__ __launch_bounds__(kJoinBlock) void k_join_probe_lds(const T *left, ui
   }
 }
 
-// Same probe for hash tables too large for LDS.  The build kernel sets each
-// key's two bits in a 2^LGB-bit membership filter in global memory; the
-// persistent workgroups copy it into LDS once, so most non-matching rows
-// are rejected with two LDS reads instead of reading the table.  The rows
-// that pass probe the global table in rounds (every pending row of a
+// Same probe for hash tables too large for LDS.  The build kernel also
+// fills a membership filter, so most non-matching rows are rejected without
+// reading the table: LGB > 0, a split filter of 2^LGB bits that the
+// persistent workgroups copy into LDS once (two LDS reads per row); LGB ==
+// 0, a blocked filter of 2^lgb bits in global memory (L2-resident, one word
+// read per row, the words of a thread's 16 rows in flight together).  The
+// rows that pass probe the global table in rounds (every pending row of a
 // thread advances one slot per round, so their reads overlap), and the
 // tile structure (kLdsSub blocks, one look-back, hit masks re-probed at
 // write time) is that of the LDS-table probe.
 template <typename T, int LGB>
 __global__ __launch_bounds__(kJoinBlock) void k_join_probe_bloom(const T *left, uint64_t L, const HashSlot *table,
                                                                  const uint32_t *used, uint32_t mask,
-                                                                 const uint32_t *bloom, const uint32_t *perm,
+                                                                 const uint32_t *bloom, int lgb, const uint32_t *perm,
                                                                  uint32_t *pairs, uint64_t cap, uint64_t *count_out,
                                                                  uint32_t *ctr, uint64_t *status, uint32_t ntiles) {
-  constexpr uint32_t kWords = 1u << (LGB - 5);
+  constexpr uint32_t kWords = LGB ? 1u << (LGB - 5) : 1u;
   __shared__ uint32_t s_bits[kWords];
   __shared__ uint64_t s_red[kJoinBlock / kWave + 1];
   __shared__ uint32_t s_tile;
   __shared__ uint64_t s_prefix;
-  for (uint32_t i = threadIdx.x; i < kWords; i += kJoinBlock) s_bits[i] = bloom[i];
+  if constexpr (LGB != 0)
+    for (uint32_t i = threadIdx.x; i < kWords; i += kJoinBlock) s_bits[i] = bloom[i];
   while (true) {
     const uint32_t tile = take_tile_j(ctr, &s_tile);  // (its barrier also publishes the filter)
     if (tile >= ntiles) return;
@@ -295,16 +307,31 @@ __global__ __launch_bounds__(kJoinBlock) void k_join_probe_bloom(const T *left, 
       probe_load<T>(left, L, row0, v);
       uint32_t slot[kProbeRpt];
       uint32_t pend = 0;  // bit e: row e still probing
+      if constexpr (LGB != 0) {
 #pragma unroll
-      for (int e = 0; e < kProbeRpt; e++) {
-        slot[e] = 0;
-        if (row0 + e < L && JoinKey<T>::ok(v[e])) {
-          const uint32_t h = hash_u64((uint64_t)JoinKey<T>::bits(v[e]));
-          const uint32_t b1 = h >> (32 - LGB), b2 = (h * 0x9e3779b1u) >> (32 - LGB);
-          if ((s_bits[b1 >> 5] >> (b1 & 31)) & (s_bits[b2 >> 5] >> (b2 & 31)) & 1u) {
-            slot[e] = h & mask;
-            pend |= 1u << e;
+        for (int e = 0; e < kProbeRpt; e++) {
+          slot[e] = 0;
+          if (row0 + e < L && JoinKey<T>::ok(v[e])) {
+            const uint32_t h = hash_u64((uint64_t)JoinKey<T>::bits(v[e]));
+            const uint32_t b1 = h >> (32 - LGB), b2 = (h * 0x9e3779b1u) >> (32 - LGB);
+            if ((s_bits[b1 >> 5] >> (b1 & 31)) & (s_bits[b2 >> 5] >> (b2 & 31)) & 1u) {
+              slot[e] = h & mask;
+              pend |= 1u << e;
+            }
           }
+        }
+      } else {
+        uint32_t wd[kProbeRpt];
+#pragma unroll
+        for (int e = 0; e < kProbeRpt; e++) {  // every row's filter word is in flight before any test
+          slot[e] = hash_u64((uint64_t)JoinKey<T>::bits(v[e]));
+          wd[e] = (row0 + e < L && JoinKey<T>::ok(v[e])) ? bloom[slot[e] >> (37 - lgb)] : 0u;
+        }
+#pragma unroll
+        for (int e = 0; e < kProbeRpt; e++) {
+          const uint32_t m = bloom_word_bits(slot[e]);
+          if ((wd[e] & m) == m) pend |= 1u << e;
+          slot[e] &= mask;
         }
       }
       uint32_t cnt = 0, hm = 0;
@@ -524,10 +551,20 @@ struct JoinWs {
   size_t sort_ws_bytes;
 };
 
-// Filter of the global-table probe: 2^17 bits while that gives the right
-// side's keys 16 bits each, else 2^19 bits (64 KB of LDS).
-constexpr uint32_t kBloomMaxWords = (1u << 19) / 32;
-static int bloom_lgb(uint64_t R) { return 16 * R <= (1u << 17) ? 17 : 19; }
+// Filter of the global-table probe: a split filter of 2^17 bits in LDS up
+// to 8,192 right rows (16 bits per row), else a blocked filter in global
+// memory of 2^20..2^24 bits, 16 bits per row up to the cap.  (A 2^19-bit LDS
+// filter, 64 KB, measured slower than the blocked filter from 8,192 rows:
+// 1.39 / 2.61 ms against 1.33 / 1.71 ms at 20,000 / 50,000 rows.)
+constexpr int kBloomLdsLgb = 17;
+constexpr int kBloomMaxLgb = 24;
+constexpr uint32_t kBloomMaxWords = (1u << kBloomMaxLgb) / 32;
+static int bloom_lgb(uint64_t R) {
+  if (16 * R <= (1u << kBloomLdsLgb)) return kBloomLdsLgb;
+  int lg = 20;
+  while (lg < kBloomMaxLgb && (1ull << lg) < 16 * R) lg++;
+  return lg;
+}
 
 static uint32_t hash_cap(uint64_t R) {
   uint64_t c = 64;
@@ -580,7 +617,7 @@ static size_t join_ws_layout(int dtype, int cmp, uint64_t L, uint64_t R, void *w
 }
 
 template <typename T, int LGB>
-static void launch_probe_bloom(const T *left, uint64_t L, const JoinWs &w, uint32_t tcap, uint32_t *pairs,
+static void launch_probe_bloom(const T *left, uint64_t L, const JoinWs &w, uint32_t tcap, int lgb, uint32_t *pairs,
                                uint64_t cap, uint64_t *count_out, uint32_t nt, hipStream_t s) {
   static const uint32_t per_cu = [] {
     int b = 0;
@@ -591,7 +628,7 @@ static void launch_probe_bloom(const T *left, uint64_t L, const JoinWs &w, uint3
   }();
   const uint32_t g = std::min<uint32_t>(nt, per_cu * (uint32_t)grid_cap());
   k_join_probe_bloom<T, LGB><<<std::max<uint32_t>(g, 1), kJoinBlock, 0, s>>>(
-      left, L, w.table, w.used, tcap - 1, w.bloom, w.perm, pairs, cap, count_out, w.ctr, w.status, nt);
+      left, L, w.table, w.used, tcap - 1, w.bloom, lgb, w.perm, pairs, cap, count_out, w.ctr, w.status, nt);
 }
 
 template <typename T>
@@ -615,7 +652,7 @@ static int join_hash(const T *left, uint64_t L, const T *right, uint64_t R, uint
     int rc = ysql_radix_argsort(kdt, 0, w.keys, w.perm, R, 0, 8 * sizeof(U), w.sort_ws, w.sort_ws_bytes, s);
     if (rc) return rc;
     k_join_build<U><<<(unsigned)g, 256, 0, s>>>((const U *)w.keys, R, w.ctr + 1, w.table, w.used, tcap - 1,
-                                                filtered ? w.bloom : nullptr, lgb);
+                                                filtered ? w.bloom : nullptr, lgb, lgb > kBloomLdsLgb);
     YSQL_LAUNCH_CHECK();
   }
   if (tcap <= kLdsSlots) {
@@ -631,10 +668,10 @@ static int join_hash(const T *left, uint64_t L, const T *right, uint64_t R, uint
                                                                         pairs, cap, count_out, w.ctr, w.status, nt);
   } else {
     const uint32_t nt = (uint32_t)ceil_div(L, (uint64_t)kLdsSub * kProbeTile);
-    if (lgb == 17)
-      launch_probe_bloom<T, 17>(left, L, w, tcap, pairs, cap, count_out, nt, s);
+    if (lgb == kBloomLdsLgb)
+      launch_probe_bloom<T, kBloomLdsLgb>(left, L, w, tcap, lgb, pairs, cap, count_out, nt, s);
     else
-      launch_probe_bloom<T, 19>(left, L, w, tcap, pairs, cap, count_out, nt, s);
+      launch_probe_bloom<T, 0>(left, L, w, tcap, lgb, pairs, cap, count_out, nt, s);
   }
   YSQL_LAUNCH_CHECK();
   return YSQL_OK;
