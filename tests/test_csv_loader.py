@@ -177,3 +177,44 @@ def test_device_path_matches_host(loader, tmp_path, dev):
         dc = torch.cat([b[0][c] for b in got]).cpu().numpy()
         dn = torch.cat([b[1][c] for b in got]).cpu().numpy()
         assert np.array_equal(dc, hc[c]) and np.array_equal(dn, hn[c]), c
+
+
+def test_parallel_parse_quotes_and_blank_lines(loader, tmp_path):
+    """A batch of 200,000 rows (two 4-byte columns: 524,288 rows per batch)
+    is split over several parsing threads; quoted fields holding commas,
+    quotes and newlines, blank and "" lines and mixed line ends must not
+    move a record boundary."""
+    n = 200_000
+    r = np.random.default_rng(21)
+    ids = r.integers(-10**9, 10**9, n).astype(np.int32)
+    flts = r.standard_normal(n).astype(np.float32)
+    notes = ['"a,b"', '"x\ny"', '"say ""hi"""', "plain", '""', '"\r\n"', 'q"mid']
+    pick = r.integers(0, len(notes), n)
+    eols = r.integers(0, 2, n)
+    blanks = set(r.choice(n, 300, replace=False).tolist())
+    p = tmp_path / "par.csv"
+    with open(p, "w", newline="") as f:
+        f.write("note,id,price\n")
+        for i in range(n):
+            if i in blanks:
+                f.write("\n" if i % 2 else '""\r\n')
+            f.write("%s,%d,%.9g%s" % (notes[pick[i]], ids[i], flts[i], "\r\n" if eols[i] else "\n"))
+    rd = loader.CsvReader(p, ["id", "price"], [INT, FLOAT])
+    assert rd.rows == 524288
+    cols, nulls = _read_all(rd)
+    rd.close()
+    assert np.array_equal(cols[0], ids)
+    assert np.array_equal(cols[1].view(np.uint32), flts.view(np.uint32))
+    assert not nulls[0].any() and not nulls[1].any()
+    oc, on = O.read(p, ["id", "price"], [INT, FLOAT])
+    for c in range(2):
+        assert np.array_equal(cols[c], oc[c]) and np.array_equal(nulls[c], on[c]), c
+    # a bad field in a late row range is still reported
+    with open(p, "a", newline="") as f:
+        f.write("z,notanumber,1\n")
+    rd = loader.CsvReader(p, ["id", "price"], [INT, FLOAT])
+    from yallasql_amd import _lib as L
+
+    with pytest.raises(L.YsqlError):
+        _read_all(rd)
+    rd.close()

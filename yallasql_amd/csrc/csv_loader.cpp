@@ -2,12 +2,19 @@
 // the scan path, replacing the reference's GetOperator
 // (src/engine/operators/get_operator.cpp:16-148).
 //
-// The file is read in 4 MiB chunks and tokenised by hand (RFC 4180 quotes);
-// each requested column is converted with the reference's rules into one of
-// two pinned staging buffers, whose host-to-device copies are queued on the
-// caller's stream while the next batch is parsed.
+// The file is mapped; one pass over a batch's bytes finds its record starts
+// (RFC 4180 quotes), then up to 16 threads tokenise and convert disjoint row
+// ranges with the reference's rules into one of two pinned staging buffers,
+// whose host-to-device copies are queued on the caller's stream while the
+// next batch is parsed.
 #include <hip/hip_runtime.h>
 
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include <algorithm>
 #include <cerrno>
 #include <charconv>
 #include <climits>
@@ -18,6 +25,7 @@
 #include <ctime>
 #include <limits>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "common.hpp"
@@ -27,7 +35,6 @@ namespace {
 
 constexpr size_t kMaxBytesPerBatch = 4u << 20;  // config.hpp: MAX_BYTES_PER_BATCH
 constexpr uint32_t kAlign = 256;                // config.hpp: GPU_ALIGNMENT = BLOCK_DIM
-constexpr size_t kChunk = 4u << 20;             // file read size
 
 uint32_t type_bytes(int t) {
   switch (t) {
@@ -50,95 +57,120 @@ struct Stage {
 
 }  // namespace
 
+// Reads one record starting at p (RFC 4180: fields separated by ',', records
+// by "\n" or "\r\n", a field in quotes may hold separators and newlines, ""
+// inside quotes is one quote; a quote that does not open a field is an
+// ordinary character) into rec[0 .. nrec); returns the start of the next
+// record.  Runs of ordinary characters are copied in bulk.
+static const char *parse_record(const char *p, const char *e, std::vector<std::string> &rec, size_t &nrec) {
+  nrec = 0;
+  auto next_field = [&]() {
+    if (nrec == rec.size()) rec.emplace_back();
+    std::string *f = &rec[nrec++];
+    f->clear();
+    return f;
+  };
+  std::string *fld = next_field();
+  bool quoted = false;
+  while (p < e) {  // the end of the file ends the record
+    const char *q = p;
+    if (!quoted) {
+      while (q < e && *q != ',' && *q != '\n' && *q != '\r' && *q != '"') q++;
+      fld->append(p, (size_t)(q - p));
+      p = q;
+      if (p == e) break;
+      const char c = *p++;
+      if (c == ',') {
+        fld = next_field();
+      } else if (c == '\n') {
+        break;
+      } else if (c == '\r') {
+        if (p < e && *p == '\n') {
+          p++;
+          break;
+        }
+        fld->push_back('\r');
+      } else if (fld->empty()) {  // '"' opening a field
+        quoted = true;
+      } else {
+        fld->push_back('"');
+      }
+    } else {
+      while (q < e && *q != '"') q++;
+      fld->append(p, (size_t)(q - p));
+      p = q;
+      if (p == e) break;
+      p++;  // the quote
+      if (p < e && *p == '"') {
+        p++;
+        fld->push_back('"');
+      } else {
+        quoted = false;
+      }
+    }
+  }
+  return p;
+}
+
 struct ysql_csv_reader {
-  FILE *f = nullptr;
-  std::vector<char> buf;
-  size_t pos = 0, len = 0;
-  bool eof = false;
+  int fd = -1;
+  const char *data = nullptr;  // the whole file, mapped read-only
+  size_t size = 0;
+  const char *pos = nullptr, *end = nullptr;
+  const char *next_quote = nullptr;  // first '"' at or after the scan position (end: none)
   std::vector<int> types;
   std::vector<uint32_t> bytes;
   std::vector<int> field;  // CSV field index of each requested column
   int nfields = 0;
   uint32_t batch_rows = 0;
+  int threads = 1;
+  std::vector<const char *> starts;  // record starts of the batch being parsed
   Stage st[2];
   int cur = 0;
-  std::vector<std::string> rec;  // the current record's fields: rec[0 .. nrec)
-  size_t nrec = 0;
 
-  // refills the buffer; false at the end of the file
-  bool refill() {
-    if (eof) return false;
-    len = fread(buf.data(), 1, buf.size(), f);
-    pos = 0;
-    if (len == 0) eof = true;
-    return len > 0;
-  }
-  // next byte of the file, -1 at the end
-  int get() {
-    if (pos == len && !refill()) return -1;
-    return (unsigned char)buf[pos++];
-  }
-  int peek() {
-    if (pos == len && !refill()) return -1;
-    return (unsigned char)buf[pos];
+  const char *quote_at_or_after(const char *p) {
+    if (next_quote < p) {
+      const void *q = memchr(p, '"', (size_t)(end - p));
+      next_quote = q ? (const char *)q : end;
+    }
+    return next_quote;
   }
 
-  // Reads one record into rec; false at the end of the file.  RFC 4180:
-  // fields separated by ',', records by "\n" or "\r\n", a field in quotes may
-  // hold separators and newlines, "" inside quotes is one quote; a quote
-  // that does not open a field is an ordinary character.  Runs of ordinary
-  // characters are copied in bulk.
-  bool read_record() {
-    nrec = 0;
-    if (pos == len && !refill()) return false;
-    std::string *fld = next_field();
-    bool quoted = false;
-    while (true) {
-      if (pos == len && !refill()) break;  // end of file ends the record
-      const char *p = buf.data() + pos, *e = buf.data() + len, *q = p;
-      if (!quoted) {
-        while (q < e && *q != ',' && *q != '\n' && *q != '\r' && *q != '"') q++;
-        fld->append(p, (size_t)(q - p));
-        pos += (size_t)(q - p);
-        if (pos == len) continue;
-        const char c = buf[pos++];
-        if (c == ',') {
-          fld = next_field();
-        } else if (c == '\n') {
+  // Finds the end of the record starting at p (the start of the next one),
+  // following parse_record's quote rules, and whether the record is blank
+  // (no bytes, or a lone empty quoted field, before its terminator).
+  const char *scan_record(const char *p, bool *blank) {
+    const char *rs = p;
+    const char *content_end = nullptr;
+    while (p < end) {
+      const void *nlv = memchr(p, '\n', (size_t)(end - p));
+      const char *nl = nlv ? (const char *)nlv : end;
+      const char *q = quote_at_or_after(p);
+      while (q < nl && !(q == rs || q[-1] == ',')) q = quote_at_or_after(q + 1);  // ordinary quotes
+      if (q >= nl) {  // no field opens with a quote before the newline
+        content_end = (nl > rs && nl < end && nl[-1] == '\r') ? nl - 1 : nl;
+        p = nl < end ? nl + 1 : end;
+        break;
+      }
+      p = q + 1;  // inside a quoted field: find its closing quote
+      while (true) {
+        const char *c = quote_at_or_after(p);
+        if (c == end) {
+          p = end;
           break;
-        } else if (c == '\r') {
-          if (peek() == '\n') {
-            pos++;
-            break;
-          }
-          fld->push_back('\r');
-        } else if (fld->empty()) {  // '"' opening a field
-          quoted = true;
-        } else {
-          fld->push_back('"');
         }
-      } else {
-        while (q < e && *q != '"') q++;
-        fld->append(p, (size_t)(q - p));
-        pos += (size_t)(q - p);
-        if (pos == len) continue;
-        pos++;  // the quote
-        if (peek() == '"') {
-          pos++;
-          fld->push_back('"');
-        } else {
-          quoted = false;
+        if (c + 1 < end && c[1] == '"') {
+          p = c + 2;
+          continue;
         }
+        p = c + 1;
+        break;
       }
     }
-    return true;
-  }
-  // the next field of the current record (strings are reused across records)
-  std::string *next_field() {
-    if (nrec == rec.size()) rec.emplace_back();
-    std::string *f = &rec[nrec++];
-    f->clear();
-    return f;
+    if (!content_end) content_end = p;
+    const size_t n = (size_t)(content_end - rs);
+    *blank = n == 0 || (n == 2 && rs[0] == '"' && rs[1] == '"');
+    return p;
   }
 };
 
@@ -210,24 +242,64 @@ bool convert(int type, const std::string &s, void *col, uint8_t *nulls, uint32_t
   }
 }
 
-// Parses up to batch_rows records into the given column buffers.
-int parse_batch(ysql_csv_reader *r, void *const *cols, uint8_t *const *nulls, uint32_t *rows) {
-  uint32_t n = 0;
+// Converts records [r0, r1) of the batch (starts[]) into rows r0.. of the
+// column buffers; returns the first failing row (r1: none) and its error.
+uint32_t parse_rows(const ysql_csv_reader *r, uint32_t r0, uint32_t r1, void *const *cols, uint8_t *const *nulls,
+                    const char **err) {
+  std::vector<std::string> rec;
+  size_t nrec = 0;
   const int nc = (int)r->types.size();
-  while (n < r->batch_rows && r->read_record()) {
-    if (r->nrec == 1 && r->rec[0].empty() && r->nfields > 1) continue;  // blank line
-    if ((int)r->nrec < r->nfields) {
-      ysql::set_last_error("csv: record with fewer fields than the header");
-      return YSQL_EINVAL;
+  for (uint32_t i = r0; i < r1; i++) {
+    parse_record(r->starts[i], r->end, rec, nrec);
+    if ((int)nrec < r->nfields) {
+      *err = "csv: record with fewer fields than the header";
+      return i;
     }
     for (int c = 0; c < nc; c++) {
-      if (!convert(r->types[c], r->rec[r->field[c]], cols[c], nulls ? nulls[c] : nullptr, n)) {
-        ysql::set_last_error("csv: field does not convert to the column type");
-        return YSQL_EINVAL;
+      if (!convert(r->types[c], rec[r->field[c]], cols[c], nulls ? nulls[c] : nullptr, i)) {
+        *err = "csv: field does not convert to the column type";
+        return i;
       }
     }
-    n++;
   }
+  return r1;
+}
+
+// Parses up to batch_rows records into the given column buffers.  One pass
+// over the bytes finds the record starts (skipping blank lines); then up to
+// r->threads threads convert disjoint row ranges.
+int parse_batch(ysql_csv_reader *r, void *const *cols, uint8_t *const *nulls, uint32_t *rows) {
+  r->starts.clear();
+  while (r->starts.size() < r->batch_rows && r->pos < r->end) {
+    bool blank = false;
+    const char *rs = r->pos;
+    r->pos = r->scan_record(rs, &blank);
+    if (!(blank && r->nfields > 1)) r->starts.push_back(rs);
+  }
+  const uint32_t n = (uint32_t)r->starts.size();
+  constexpr uint32_t kMinRowsPerThread = 16384;
+  const uint32_t nt = std::max<uint32_t>(1, std::min<uint32_t>((uint32_t)r->threads, n / kMinRowsPerThread));
+  std::vector<uint32_t> bad(nt, 0);
+  std::vector<const char *> err(nt, nullptr);
+  const uint32_t per = (n + nt - 1) / nt;
+  auto work = [&](uint32_t t) {
+    const uint32_t r0 = std::min(n, t * per), r1 = std::min(n, r0 + per);
+    bad[t] = parse_rows(r, r0, r1, cols, nulls, &err[t]) < r1 ? 1 : 0;
+  };
+  if (nt == 1) {
+    work(0);
+  } else {
+    std::vector<std::thread> pool;
+    pool.reserve(nt - 1);
+    for (uint32_t t = 1; t < nt; t++) pool.emplace_back(work, t);
+    work(0);
+    for (auto &th : pool) th.join();
+  }
+  for (uint32_t t = 0; t < nt; t++)
+    if (bad[t]) {  // the error of the earliest failing row range
+      ysql::set_last_error(err[t]);
+      return YSQL_EINVAL;
+    }
   *rows = n;
   return YSQL_OK;
 }
@@ -270,20 +342,37 @@ int ysql_csv_open(const char *path, int ncols, const char *const *names, const i
     delete r;
     return rc;
   }
-  r->f = fopen(path, "rb");
-  if (!r->f) {
+  r->fd = open(path, O_RDONLY);
+  struct stat sb;
+  if (r->fd < 0 || fstat(r->fd, &sb) != 0) {
     ysql::set_last_error("csv: cannot open the file");
-    delete r;
+    ysql_csv_close(r);
     return YSQL_EINVAL;
   }
-  r->buf.resize(kChunk);
-  if (!r->read_record()) {
+  r->size = (size_t)sb.st_size;
+  if (r->size == 0) {
     ysql::set_last_error("csv: no header row");
-    fclose(r->f);
-    delete r;
+    ysql_csv_close(r);
     return YSQL_EINVAL;
   }
-  const std::vector<std::string> header(r->rec.begin(), r->rec.begin() + r->nrec);
+  void *m = mmap(nullptr, r->size, PROT_READ, MAP_PRIVATE, r->fd, 0);
+  if (m == MAP_FAILED) {
+    ysql::set_last_error("csv: cannot map the file");
+    ysql_csv_close(r);
+    return YSQL_EINVAL;
+  }
+  (void)madvise(m, r->size, MADV_SEQUENTIAL);
+  r->data = (const char *)m;
+  r->end = r->data + r->size;
+  r->next_quote = nullptr;
+  const unsigned hw = std::thread::hardware_concurrency();
+  r->threads = (int)std::max(1u, std::min(hw, 16u));
+  std::vector<std::string> header;
+  size_t nh = 0;
+  bool blank = false;
+  r->pos = r->scan_record(r->data, &blank);
+  parse_record(r->data, r->end, header, nh);
+  header.resize(nh);
   r->nfields = (int)header.size();
   for (int c = 0; c < ncols; c++) {
     if (!names[c]) rc = YSQL_EINVAL;
@@ -300,8 +389,7 @@ int ysql_csv_open(const char *path, int ncols, const char *const *names, const i
   }
   if (rc) {
     ysql::set_last_error("csv: a requested column is not in the header");
-    fclose(r->f);
-    delete r;
+    ysql_csv_close(r);
     return rc;
   }
   *out = r;
@@ -362,7 +450,8 @@ void ysql_csv_close(ysql_csv_reader *r) {
   if (!r) return;
   free_stage(r->st[0]);
   free_stage(r->st[1]);
-  if (r->f) fclose(r->f);
+  if (r->data) munmap((void *)r->data, r->size);
+  if (r->fd >= 0) close(r->fd);
   delete r;
 }
 
