@@ -17,6 +17,11 @@
  * Batches hold ysql_csv_batch_rows() rows (the last one fewer): floor(4 MiB / row bytes),
  * rounded down to a multiple of 256, at least 256.
  *
+ * The file is mapped read-only while the reader is open: it must not be truncated or
+ * rewritten meanwhile.  A batch of 32,768 rows or more is converted on up to 16 host
+ * threads (row ranges; results identical to one thread).  One reader serves one caller
+ * thread at a time.
+ *
  * Device path: ysql_csv_next parses the next batch into one of two pinned staging buffers
  * and queues its host-to-device copies on `stream`; it returns once the copies are queued,
  * so parsing the next batch overlaps the previous batch's copies.  A staging buffer is
