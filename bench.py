@@ -254,6 +254,26 @@ def op_lines(keys):
                              "GB_s": bf / (msf * 1e-3) / 1e9, "frac": bf / (msf * 1e-3) / 1e9 / HBM_PEAK_GBS,
                              "what": "WHERE f32 < 0.5 compacting (int64 key, f32) of 200M rows"}
     del vals, kout, vout, ws
+    # inner EQ join of the 200M int64 keys with a dimension of R int64 keys in
+    # [1000, 1e7): hash build of the right side + probe, (l, r) pairs written
+    for R in (2_000, 500_000):
+        right = torch.randint(1000, 10_000_000, (R,), generator=g, device=dev, dtype=torch.int64)
+        jws = ops.workspace(L.ws_bytes("ysql_join_ws_bytes", L.DATETIME, L.EQ, n, R), dev)
+        jcap = 1 << 24
+        pairs = torch.empty((jcap, 2), dtype=torch.int32, device=dev)
+        jcnt = torch.zeros(1, dtype=torch.int64, device=dev)
+        runj = lambda: L.call("ysql_join_pairs", L.DATETIME, L.EQ, ops._p(keys), n, ops._p(right), R,  # noqa: E731
+                              ops._p(pairs), jcap, ops._p(jcnt), ops._p(jws), jws.numel(), ops._s())
+        runj()
+        msj = _timed_ms(runj)
+        npairs = int(jcnt.item())
+        assert npairs <= jcap
+        bj = n * 8 + R * 8 + npairs * 8
+        out[f"join_eq_R{R}"] = {"rows_per_s": n / (msj * 1e-3), "ms": msj, "pairs": npairs,
+                                "GB_s": bj / (msj * 1e-3) / 1e9, "frac": bj / (msj * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                "what": f"inner EQ join, 200M int64 left keys x {R:,} right keys: build + "
+                                        "filtered probe + pairs"}
+        del right, jws, pairs
     return out
 
 

@@ -127,57 +127,184 @@ __device__ __forceinline__ uint32_t take_tile_j(uint32_t *ctr, uint32_t *s_tile)
   return *s_tile;
 }
 
-// Probes: each thread owns kProbeRpt consecutive left rows of a block of
-// kProbeTile rows; the keys come in with 128-bit loads.
+// Probes: blocks of kProbeTile left rows, kProbeRpt rows per thread (the
+// layout is described at probe_row below).
 constexpr int kProbeRpt = 16;
 constexpr uint64_t kProbeTile = (uint64_t)kJoinBlock * kProbeRpt;
 
-// Same probe with the hash table staged in LDS, for tables of up to
-// kLdsSlots slots (right sides of up to kLdsSlots / 2 rows).  The
-// workgroups are persistent: each copies the table into LDS once and then
-// takes tiles in order from the counter until none are left, so every
-// look-back still waits only on tiles taken earlier by running workgroups.
-// LDS holds each slot's start << 16 | count (0: empty; start and count are
-// below 2^11 here) and a 64K-bit two-probe membership filter (a Bloom
-// filter over two 16-bit functions of the hash), which rejects all but
-// about 0.4% of non-matching rows with two LDS reads; the rest
-// probe in rounds, every pending row of a thread advancing one slot per
-// round so that their reads overlap, comparing the full key in the global
-// table (64 KB, cache-resident).  A tile is kLdsSub consecutive blocks of
-// kProbeTile rows with one look-back.  A thread keeps only its match count
-// and a hit mask per block; the few matching rows are probed again when
-// their pairs are written.  24 KB of LDS and few registers, so many waves
-// are in flight.
+// EQ probe in three launches, so that no workgroup ever waits on another
+// (a decoupled look-back per tile measured slow here: each tile walked back
+// over the ~1,000 tiles in flight, one cross-die status read per 64):
+//   1. count pass (k_join_probe_lds / k_join_probe_bloom): persistent
+//      workgroups stream steps of kProbeTile rows and store, per thread, a
+//      hit mask of its 16 rows and, per wave, the pairs its rows produce;
+//   2. k_probe_block_totals, k_probe_offsets: prefix of those counts in row
+//      order, and the total;
+//   3. k_join_write: only waves with hits look their rows up again and
+//      write their pairs, ordered by (l, r).
+// k_join_probe_lds serves hash tables of up to kLdsSlots slots (right sides
+// of up to kLdsSlots / 2 rows): each workgroup stages the table in LDS
+// once.  LDS holds each slot's start << 16 | count (0: empty; start and
+// count are below 2^11 here) and a 64K-bit two-probe membership filter (a
+// Bloom filter over two 16-bit functions of the hash), which rejects all
+// but about 0.4% of non-matching rows with two LDS reads; the rest probe in
+// rounds, every pending row of a thread advancing one slot per round so
+// that their reads overlap, comparing the full key in the global table
+// (64 KB, cache-resident).
 constexpr uint32_t kLdsSlots = 4096;
 constexpr uint32_t kLdsBits = 65536;
-constexpr int kLdsSub = 4;  // sub-steps per tile: one look-back per 4 * kProbeTile rows
+
+// Probe layout: in each block of kProbeTile rows, wave w owns the
+// kWaveProbeRows rows from w * kWaveProbeRows; a lane holds kProbeRpt of
+// them as groups of R = 16 / key bytes consecutive rows, group i of lane l
+// at i * 64 * R + l * R, so that each 16-byte load instruction of a wave
+// reads 1 KB contiguous.
+constexpr uint64_t kWaveProbeRows = (uint64_t)kWave * kProbeRpt;
 
 template <typename T>
-__device__ __forceinline__ void probe_load(const T *left, uint64_t L, uint64_t row0, T (&v)[kProbeRpt]) {
-  if (row0 + kProbeRpt <= L && ((uintptr_t)(left + row0) & 15) == 0) {
-    constexpr int NQ = kProbeRpt * sizeof(T) / 16;
-    uint4 q[NQ];
-    const uint4 *src = reinterpret_cast<const uint4 *>(left + row0);
+__device__ __forceinline__ uint64_t probe_row(uint64_t wbase, int lane, int e) {
+  constexpr int R = 16 / sizeof(T);
+  return wbase + (uint64_t)(e / R) * (kWave * R) + (uint64_t)lane * R + (uint64_t)(e % R);
+}
+
+template <typename T>
+__device__ __forceinline__ void probe_load(const T *left, uint64_t L, uint64_t wbase, int lane, T (&v)[kProbeRpt]) {
+  constexpr int R = 16 / sizeof(T), G = kProbeRpt / R;
+  if (wbase + kWaveProbeRows <= L && ((uintptr_t)(left + wbase) & 15) == 0) {
+    uint4 q[G];
 #pragma unroll
-    for (int i = 0; i < NQ; i++) q[i] = src[i];
+    for (int i = 0; i < G; i++)
+      q[i] = *reinterpret_cast<const uint4 *>(left + wbase + (uint64_t)i * kWave * R + (uint64_t)lane * R);
     memcpy(&v[0], &q[0], sizeof(q));
   } else {
 #pragma unroll
-    for (int e = 0; e < kProbeRpt; e++) v[e] = row0 + e < L ? left[row0 + e] : T();
+    for (int e = 0; e < kProbeRpt; e++) {
+      const uint64_t r = probe_row<T>(wbase, lane, e);
+      v[e] = r < L ? left[r] : T();
+    }
   }
+}
+
+// Writes the pairs of one wave's hit rows (bit e of hm: row probe_row(e)),
+// the first at position pos, looking each up again in the global table.
+// Pairs come out in row order: groups in order and, within a group, lanes
+// in order (a wave scan of the group's counts).  Every lane of the wave
+// calls it.
+template <typename T>
+__device__ __forceinline__ void probe_write(const T *left, uint64_t wbase, int lane, uint32_t hm, uint64_t pos,
+                                            const HashSlot *table, const uint32_t *used, uint32_t mask,
+                                            const uint32_t *perm, uint32_t *pairs, uint64_t cap) {
+  constexpr int R = 16 / sizeof(T), G = kProbeRpt / R;
+#pragma unroll 1
+  for (int i = 0; i < G; i++) {
+    const uint32_t gm = (hm >> (i * R)) & ((1u << R) - 1u);
+    if (ballot(gm != 0u) == 0) continue;  // wave-uniform
+    uint32_t st[R], cn[R];
+    uint64_t ci = 0;
+#pragma unroll
+    for (int j = 0; j < R; j++) {
+      st[j] = 0;
+      cn[j] = 0;
+      if ((gm >> j) & 1u) {
+        const uint64_t key = (uint64_t)JoinKey<T>::bits(left[probe_row<T>(wbase, lane, i * R + j)]);
+        uint32_t slot = hash_u64(key) & mask;
+        while (used[slot] && table[slot].key != key) slot = (slot + 1) & mask;
+        st[j] = table[slot].start;
+        cn[j] = table[slot].count;
+        ci += cn[j];
+      }
+    }
+    const uint64_t inc = wave_inclusive_sum<uint64_t>(ci);
+    uint64_t p = pos + inc - ci;
+#pragma unroll
+    for (int j = 0; j < R; j++) {
+      const uint32_t row = (uint32_t)probe_row<T>(wbase, lane, i * R + j);
+      for (uint32_t q = 0; q < cn[j]; q++, p++) {
+        if (p < cap) {
+          pairs[2 * p] = row;
+          pairs[2 * p + 1] = perm[st[j] + q];
+        }
+      }
+    }
+    pos += shfl_t<uint64_t>(inc, kWave - 1);
+  }
+}
+
+// Count pass output of one step (kProbeTile rows): hits[step * kJoinBlock +
+// thread] is the thread's hit mask (bit e: row probe_row(e) has matches),
+// wcnt[step * kProbeWaves + wave] the number of pairs of the wave's rows.
+constexpr int kProbeWaves = kJoinBlock / kWave;
+
+__device__ __forceinline__ void probe_publish(uint16_t *hits, uint64_t *wcnt, uint64_t step, int wave, int lane,
+                                              uint32_t hm, uint64_t cnt) {
+  hits[step * kJoinBlock + (uint64_t)wave * kWave + lane] = (uint16_t)hm;
+  const uint64_t ws = wave_sum<uint64_t>(cnt);
+  if (lane == 0) wcnt[step * kProbeWaves + wave] = ws;
+}
+
+// Prefix of the steps' pair counts in row order, in two launches of one
+// workgroup per kProbeScanBlock steps: block totals, then each block's
+// exclusive scan plus the totals of the blocks before it (at most
+// kProbeScanBlock blocks: left sides below 2^32 rows).
+constexpr int kProbeScanBlock = 1024;
+
+__device__ __forceinline__ uint64_t step_pairs(const uint64_t *wcnt, uint64_t nsteps, uint64_t st) {
+  uint64_t v = 0;
+  if (st < nsteps) {
+#pragma unroll
+    for (int w = 0; w < kProbeWaves; w++) v += wcnt[st * kProbeWaves + w];
+  }
+  return v;
+}
+
+__global__ __launch_bounds__(kProbeScanBlock) void k_probe_block_totals(const uint64_t *wcnt, uint64_t nsteps,
+                                                                        uint64_t *btot) {
+  __shared__ uint64_t s_red[kProbeScanBlock / kWave + 1];
+  const uint64_t v = step_pairs(wcnt, nsteps, (uint64_t)blockIdx.x * kProbeScanBlock + threadIdx.x);
+  uint64_t total;
+  (void)block_exclusive_sum<kProbeScanBlock, uint64_t>(v, s_red, &total);
+  if (threadIdx.x == 0) btot[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kProbeScanBlock) void k_probe_offsets(const uint64_t *wcnt, uint64_t nsteps,
+                                                                   const uint64_t *btot, uint64_t *step_off,
+                                                                   uint64_t *count_out) {
+  __shared__ uint64_t s_red[kProbeScanBlock / kWave + 1];
+  uint64_t prefix;
+  (void)block_exclusive_sum<kProbeScanBlock, uint64_t>(threadIdx.x < blockIdx.x ? btot[threadIdx.x] : 0ull, s_red,
+                                                       &prefix);
+  const uint64_t st = (uint64_t)blockIdx.x * kProbeScanBlock + threadIdx.x;
+  uint64_t tot;
+  const uint64_t ex = block_exclusive_sum<kProbeScanBlock, uint64_t>(step_pairs(wcnt, nsteps, st), s_red, &tot);
+  if (st < nsteps) step_off[st] = prefix + ex;
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *count_out = prefix + tot;
+}
+
+// Write pass, one workgroup per step: every wave whose rows have hits looks
+// them up again in the global table and writes their pairs from its offset
+// (the step's prefix plus the earlier waves of the step).  No barriers.
+template <typename T>
+__global__ __launch_bounds__(kJoinBlock) void k_join_write(const T *left, const HashSlot *table, const uint32_t *used,
+                                                           uint32_t mask, const uint32_t *perm, const uint16_t *hits,
+                                                           const uint64_t *wcnt, const uint64_t *step_off,
+                                                           uint32_t *pairs, uint64_t cap) {
+  const int lane = lane_id(), wave = threadIdx.x / kWave;
+  const uint64_t step = blockIdx.x;
+  const uint32_t hm = hits[step * kJoinBlock + threadIdx.x];
+  if (ballot(hm != 0u) == 0) return;  // wave-uniform
+  uint64_t pos = step_off[step];
+  for (int w = 0; w < wave; w++) pos += wcnt[step * kProbeWaves + w];
+  probe_write<T>(left, step * kProbeTile + (uint64_t)wave * kWaveProbeRows, lane, hm, pos, table, used, mask, perm,
+                 pairs, cap);
 }
 
 template <typename T>
 __global__ __launch_bounds__(kJoinBlock) void k_join_probe_lds(const T *left, uint64_t L, const HashSlot *table,
-                                                               const uint32_t *used, uint32_t mask,
-                                                               const uint32_t *perm, uint32_t *pairs, uint64_t cap,
-                                                               uint64_t *count_out, uint32_t *ctr, uint64_t *status,
-                                                               uint32_t ntiles) {
+                                                               const uint32_t *used, uint32_t mask, uint16_t *hits,
+                                                               uint64_t *wcnt, uint64_t nsteps) {
   __shared__ uint32_t s_sc[kLdsSlots];
   __shared__ uint32_t s_bits[kLdsBits / 32];
-  __shared__ uint64_t s_red[kJoinBlock / kWave + 1];
-  __shared__ uint32_t s_tile;
-  __shared__ uint64_t s_prefix;
+  const int lane = lane_id(), wave = threadIdx.x / kWave;
   for (uint32_t i = threadIdx.x; i < kLdsBits / 32; i += kJoinBlock) s_bits[i] = 0;
   __syncthreads();
   for (uint32_t i = threadIdx.x; i <= mask; i += kJoinBlock) {
@@ -191,21 +318,18 @@ __global__ __launch_bounds__(kJoinBlock) void k_join_probe_lds(const T *left, ui
       atomicOr(&s_bits[b2 >> 5], 1u << (b2 & 31));
     }
   }
-  while (true) {
-    const uint32_t tile = take_tile_j(ctr, &s_tile);  // (its barrier also publishes the table)
-    if (tile >= ntiles) return;
-    uint32_t mine[kLdsSub], hit[kLdsSub];
-#pragma unroll
-    for (int sub = 0; sub < kLdsSub; sub++) {
-      const uint64_t row0 = ((uint64_t)tile * kLdsSub + sub) * kProbeTile + (uint64_t)threadIdx.x * kProbeRpt;
+  __syncthreads();  // LDS setup complete
+  for (uint64_t step = blockIdx.x; step < nsteps; step += gridDim.x) {
+    {
+      const uint64_t wbase = step * kProbeTile + (uint64_t)wave * kWaveProbeRows;
       T v[kProbeRpt];
-      probe_load<T>(left, L, row0, v);
+      probe_load<T>(left, L, wbase, lane, v);
       uint32_t slot[kProbeRpt];
       uint32_t pend = 0;  // bit e: row e still probing
 #pragma unroll
       for (int e = 0; e < kProbeRpt; e++) {
         slot[e] = 0;
-        if (row0 + e < L && JoinKey<T>::ok(v[e])) {
+        if (probe_row<T>(wbase, lane, e) < L && JoinKey<T>::ok(v[e])) {
           const uint32_t h = hash_u64((uint64_t)JoinKey<T>::bits(v[e]));
           const uint32_t b1 = h >> 16, b2 = (h * 0x9e3779b1u) >> 16;
           if ((s_bits[b1 >> 5] >> (b1 & 31)) & (s_bits[b2 >> 5] >> (b2 & 31)) & 1u) {
@@ -233,43 +357,8 @@ __global__ __launch_bounds__(kJoinBlock) void k_join_probe_lds(const T *left, ui
           }
         }
       }
-      mine[sub] = cnt;
-      hit[sub] = hm;
+      probe_publish(hits, wcnt, step, wave, lane, hm, cnt);
     }
-    // output order: sub-steps in order, threads in order within one
-    uint64_t excl[kLdsSub], total = 0;
-#pragma unroll
-    for (int sub = 0; sub < kLdsSub; sub++) {
-      uint64_t t_sub;
-      excl[sub] = total + block_exclusive_sum<kJoinBlock, uint64_t>(mine[sub], s_red, &t_sub);
-      total += t_sub;
-    }
-    if (threadIdx.x < kWave) {
-      const uint64_t p = lookback_exclusive(status, tile, total);
-      if (threadIdx.x == 0) s_prefix = p;
-    }
-    __syncthreads();
-    const uint64_t prefix = s_prefix;
-    for (int sub = 0; sub < kLdsSub; sub++) {
-      if (!hit[sub]) continue;
-      const uint64_t row0 = ((uint64_t)tile * kLdsSub + sub) * kProbeTile + (uint64_t)threadIdx.x * kProbeRpt;
-      uint64_t pos = prefix + excl[sub];
-      for (uint32_t hm = hit[sub]; hm; hm &= hm - 1) {
-        const int e = __builtin_ctz(hm);
-        const uint64_t key = (uint64_t)JoinKey<T>::bits(left[row0 + e]);
-        uint32_t slot = hash_u64(key) & mask, sc;
-        while ((sc = s_sc[slot]) != 0u && table[slot].key != key) slot = (slot + 1) & mask;
-        const uint32_t st = sc >> 16, cn = sc & 0xffffu;
-        for (uint32_t q = 0; q < cn; q++, pos++) {
-          if (pos < cap) {
-            pairs[2 * pos] = (uint32_t)(row0 + e);
-            pairs[2 * pos + 1] = perm[st + q];
-          }
-        }
-      }
-    }
-    if (tile == ntiles - 1 && threadIdx.x == 0) *count_out = prefix + total;
-    __syncthreads();  // s_prefix and s_red are reused by the next tile
   }
 }
 
@@ -280,38 +369,31 @@ __global__ __launch_bounds__(kJoinBlock) void k_join_probe_lds(const T *left, ui
 // 0, a blocked filter of 2^lgb bits in global memory (L2-resident, one word
 // read per row, the words of a thread's 16 rows in flight together).  The
 // rows that pass probe the global table in rounds (every pending row of a
-// thread advances one slot per round, so their reads overlap), and the
-// tile structure (kLdsSub blocks, one look-back, hit masks re-probed at
-// write time) is that of the LDS-table probe.
+// thread advances one slot per round, so their reads overlap).  Count pass
+// of the three-launch probe, like k_join_probe_lds.
 template <typename T, int LGB>
 __global__ __launch_bounds__(kJoinBlock) void k_join_probe_bloom(const T *left, uint64_t L, const HashSlot *table,
                                                                  const uint32_t *used, uint32_t mask,
-                                                                 const uint32_t *bloom, int lgb, const uint32_t *perm,
-                                                                 uint32_t *pairs, uint64_t cap, uint64_t *count_out,
-                                                                 uint32_t *ctr, uint64_t *status, uint32_t ntiles) {
+                                                                 const uint32_t *bloom, int lgb, uint16_t *hits,
+                                                                 uint64_t *wcnt, uint64_t nsteps) {
   constexpr uint32_t kWords = LGB ? 1u << (LGB - 5) : 1u;
   __shared__ uint32_t s_bits[kWords];
-  __shared__ uint64_t s_red[kJoinBlock / kWave + 1];
-  __shared__ uint32_t s_tile;
-  __shared__ uint64_t s_prefix;
+  const int lane = lane_id(), wave = threadIdx.x / kWave;
   if constexpr (LGB != 0)
     for (uint32_t i = threadIdx.x; i < kWords; i += kJoinBlock) s_bits[i] = bloom[i];
-  while (true) {
-    const uint32_t tile = take_tile_j(ctr, &s_tile);  // (its barrier also publishes the filter)
-    if (tile >= ntiles) return;
-    uint32_t mine[kLdsSub], hit[kLdsSub];
-#pragma unroll
-    for (int sub = 0; sub < kLdsSub; sub++) {
-      const uint64_t row0 = ((uint64_t)tile * kLdsSub + sub) * kProbeTile + (uint64_t)threadIdx.x * kProbeRpt;
+  __syncthreads();  // LDS setup complete
+  for (uint64_t step = blockIdx.x; step < nsteps; step += gridDim.x) {
+    {
+      const uint64_t wbase = step * kProbeTile + (uint64_t)wave * kWaveProbeRows;
       T v[kProbeRpt];
-      probe_load<T>(left, L, row0, v);
+      probe_load<T>(left, L, wbase, lane, v);
       uint32_t slot[kProbeRpt];
       uint32_t pend = 0;  // bit e: row e still probing
       if constexpr (LGB != 0) {
 #pragma unroll
         for (int e = 0; e < kProbeRpt; e++) {
           slot[e] = 0;
-          if (row0 + e < L && JoinKey<T>::ok(v[e])) {
+          if (probe_row<T>(wbase, lane, e) < L && JoinKey<T>::ok(v[e])) {
             const uint32_t h = hash_u64((uint64_t)JoinKey<T>::bits(v[e]));
             const uint32_t b1 = h >> (32 - LGB), b2 = (h * 0x9e3779b1u) >> (32 - LGB);
             if ((s_bits[b1 >> 5] >> (b1 & 31)) & (s_bits[b2 >> 5] >> (b2 & 31)) & 1u) {
@@ -325,7 +407,7 @@ __global__ __launch_bounds__(kJoinBlock) void k_join_probe_bloom(const T *left, 
 #pragma unroll
         for (int e = 0; e < kProbeRpt; e++) {  // every row's filter word is in flight before any test
           slot[e] = hash_u64((uint64_t)JoinKey<T>::bits(v[e]));
-          wd[e] = (row0 + e < L && JoinKey<T>::ok(v[e])) ? bloom[slot[e] >> (37 - lgb)] : 0u;
+          wd[e] = (probe_row<T>(wbase, lane, e) < L && JoinKey<T>::ok(v[e])) ? bloom[slot[e] >> (37 - lgb)] : 0u;
         }
 #pragma unroll
         for (int e = 0; e < kProbeRpt; e++) {
@@ -334,7 +416,8 @@ __global__ __launch_bounds__(kJoinBlock) void k_join_probe_bloom(const T *left, 
           slot[e] &= mask;
         }
       }
-      uint32_t cnt = 0, hm = 0;
+      uint64_t cnt = 0;
+      uint32_t hm = 0;
       while (pend) {
 #pragma unroll
         for (int e = 0; e < kProbeRpt; e++) {
@@ -353,42 +436,8 @@ __global__ __launch_bounds__(kJoinBlock) void k_join_probe_bloom(const T *left, 
           }
         }
       }
-      mine[sub] = cnt;
-      hit[sub] = hm;
+      probe_publish(hits, wcnt, step, wave, lane, hm, cnt);
     }
-    uint64_t excl[kLdsSub], total = 0;
-#pragma unroll
-    for (int sub = 0; sub < kLdsSub; sub++) {
-      uint64_t t_sub;
-      excl[sub] = total + block_exclusive_sum<kJoinBlock, uint64_t>(mine[sub], s_red, &t_sub);
-      total += t_sub;
-    }
-    if (threadIdx.x < kWave) {
-      const uint64_t p = lookback_exclusive(status, tile, total);
-      if (threadIdx.x == 0) s_prefix = p;
-    }
-    __syncthreads();
-    const uint64_t prefix = s_prefix;
-    for (int sub = 0; sub < kLdsSub; sub++) {
-      if (!hit[sub]) continue;
-      const uint64_t row0 = ((uint64_t)tile * kLdsSub + sub) * kProbeTile + (uint64_t)threadIdx.x * kProbeRpt;
-      uint64_t pos = prefix + excl[sub];
-      for (uint32_t hm = hit[sub]; hm; hm &= hm - 1) {
-        const int e = __builtin_ctz(hm);
-        const uint64_t key = (uint64_t)JoinKey<T>::bits(left[row0 + e]);
-        uint32_t slot = hash_u64(key) & mask;
-        while (used[slot] && table[slot].key != key) slot = (slot + 1) & mask;
-        const uint32_t st = table[slot].start, cn = table[slot].count;
-        for (uint32_t q = 0; q < cn; q++, pos++) {
-          if (pos < cap) {
-            pairs[2 * pos] = (uint32_t)(row0 + e);
-            pairs[2 * pos + 1] = perm[st + q];
-          }
-        }
-      }
-    }
-    if (tile == ntiles - 1 && threadIdx.x == 0) *count_out = prefix + total;
-    __syncthreads();  // s_prefix and s_red are reused by the next tile
   }
 }
 
@@ -547,6 +596,9 @@ struct JoinWs {
   HashSlot *table;    // cap
   uint32_t *used;     // cap
   uint32_t *bloom;    // kBloomMaxWords (membership filter of the global-table probe)
+  uint16_t *hits;     // probe steps * kJoinBlock hit masks
+  uint64_t *wcnt;     // probe steps * kProbeWaves pair counts
+  uint64_t *btot;     // kProbeScanBlock block totals of the step scan
   void *sort_ws;
   size_t sort_ws_bytes;
 };
@@ -595,6 +647,9 @@ static size_t join_ws_layout(int dtype, int cmp, uint64_t L, uint64_t R, void *w
       c.take<HashSlot>(cap);
       c.take<uint32_t>(cap);
       c.take<uint32_t>(kBloomMaxWords);
+      c.take<uint16_t>((ntiles - 1) * kJoinBlock);
+      c.take<uint64_t>((ntiles - 1) * kProbeWaves);
+      c.take<uint64_t>(kProbeScanBlock);
       c.take<char>(sort_bytes);
     }
     return c.off;
@@ -609,26 +664,24 @@ static size_t join_ws_layout(int dtype, int cmp, uint64_t L, uint64_t R, void *w
     out->table = c.take<HashSlot>(cap);
     out->used = c.take<uint32_t>(cap);
     out->bloom = c.take<uint32_t>(kBloomMaxWords);
+    out->hits = c.take<uint16_t>((ntiles - 1) * kJoinBlock);
+    out->wcnt = c.take<uint64_t>((ntiles - 1) * kProbeWaves);
+    out->btot = c.take<uint64_t>(kProbeScanBlock);
     out->sort_ws = c.take<char>(sort_bytes);
     out->sort_ws_bytes = sort_bytes;
-    ok = ok && out->keys && out->perm && out->table && out->used && out->bloom && out->sort_ws;
+    ok = ok && out->keys && out->perm && out->table && out->used && out->bloom && out->hits && out->wcnt && out->btot &&
+         out->sort_ws;
   }
   return ok ? c.off : 0;
 }
 
-template <typename T, int LGB>
-static void launch_probe_bloom(const T *left, uint64_t L, const JoinWs &w, uint32_t tcap, int lgb, uint32_t *pairs,
-                               uint64_t cap, uint64_t *count_out, uint32_t nt, hipStream_t s) {
-  static const uint32_t per_cu = [] {
-    int b = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_join_probe_bloom<T, LGB>, kJoinBlock, 0) != hipSuccess ||
-        b < 1)
-      b = 1;
-    return (uint32_t)b;
-  }();
-  const uint32_t g = std::min<uint32_t>(nt, per_cu * (uint32_t)grid_cap());
-  k_join_probe_bloom<T, LGB><<<std::max<uint32_t>(g, 1), kJoinBlock, 0, s>>>(
-      left, L, w.table, w.used, tcap - 1, w.bloom, lgb, w.perm, pairs, cap, count_out, w.ctr, w.status, nt);
+// Persistent grid of a count-pass kernel: enough workgroups to fill the
+// chip at its occupancy, at most one per step.
+template <typename K>
+static uint32_t probe_grid(K kernel, uint64_t nsteps) {
+  int b = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, kernel, kJoinBlock, 0) != hipSuccess || b < 1) b = 1;
+  return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(nsteps, (uint64_t)b * grid_cap()));
 }
 
 template <typename T>
@@ -636,11 +689,10 @@ static int join_hash(const T *left, uint64_t L, const T *right, uint64_t R, uint
                      uint64_t *count_out, const JoinWs &w, hipStream_t s) {
   typedef typename JoinKey<T>::U U;
   const uint32_t tcap = hash_cap(R);
-  const uint32_t ntiles = (uint32_t)ceil_div(L, kProbeTile);
+  const uint64_t nsteps = ceil_div(L, kProbeTile);
   const bool filtered = tcap > kLdsSlots;
   const int lgb = bloom_lgb(R);
-  k_join_reset<<<256, 256, 0, s>>>(w.ctr, w.status, ntiles + 1, w.used, tcap, w.bloom,
-                                   filtered ? (1u << lgb) / 32 : 0u);
+  k_join_reset<<<256, 256, 0, s>>>(w.ctr, w.status, 0, w.used, tcap, w.bloom, filtered ? (1u << lgb) / 32 : 0u);
   YSQL_LAUNCH_CHECK();
   uint64_t g = ceil_div(R, 256);
   if (g > 4096) g = 4096;
@@ -655,24 +707,29 @@ static int join_hash(const T *left, uint64_t L, const T *right, uint64_t R, uint
                                                 filtered ? w.bloom : nullptr, lgb, lgb > kBloomLdsLgb);
     YSQL_LAUNCH_CHECK();
   }
-  if (tcap <= kLdsSlots) {
-    static const uint32_t per_cu = [] {
-      int b = 0;
-      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_join_probe_lds<T>, kJoinBlock, 0) != hipSuccess || b < 1)
-        b = 1;
-      return (uint32_t)b;
-    }();
-    const uint32_t nt = (uint32_t)ceil_div(L, (uint64_t)kLdsSub * kProbeTile);  // <= ntiles: fits the status area
-    const uint32_t g = std::min<uint32_t>(nt, per_cu * (uint32_t)grid_cap());
-    k_join_probe_lds<T><<<std::max<uint32_t>(g, 1), kJoinBlock, 0, s>>>(left, L, w.table, w.used, tcap - 1, w.perm,
-                                                                        pairs, cap, count_out, w.ctr, w.status, nt);
+  // 1. count pass: hit masks and per-wave pair counts of every step
+  if (!filtered) {
+    static const uint32_t gl = probe_grid(k_join_probe_lds<T>, ~0ull);
+    k_join_probe_lds<T><<<(uint32_t)std::min<uint64_t>(gl, nsteps), kJoinBlock, 0, s>>>(
+        left, L, w.table, w.used, tcap - 1, w.hits, w.wcnt, nsteps);
+  } else if (lgb == kBloomLdsLgb) {
+    static const uint32_t gb = probe_grid(k_join_probe_bloom<T, kBloomLdsLgb>, ~0ull);
+    k_join_probe_bloom<T, kBloomLdsLgb><<<(uint32_t)std::min<uint64_t>(gb, nsteps), kJoinBlock, 0, s>>>(
+        left, L, w.table, w.used, tcap - 1, w.bloom, lgb, w.hits, w.wcnt, nsteps);
   } else {
-    const uint32_t nt = (uint32_t)ceil_div(L, (uint64_t)kLdsSub * kProbeTile);
-    if (lgb == kBloomLdsLgb)
-      launch_probe_bloom<T, kBloomLdsLgb>(left, L, w, tcap, lgb, pairs, cap, count_out, nt, s);
-    else
-      launch_probe_bloom<T, 0>(left, L, w, tcap, lgb, pairs, cap, count_out, nt, s);
+    static const uint32_t gg = probe_grid(k_join_probe_bloom<T, 0>, ~0ull);
+    k_join_probe_bloom<T, 0><<<(uint32_t)std::min<uint64_t>(gg, nsteps), kJoinBlock, 0, s>>>(
+        left, L, w.table, w.used, tcap - 1, w.bloom, lgb, w.hits, w.wcnt, nsteps);
   }
+  YSQL_LAUNCH_CHECK();
+  // 2. step offsets and the total; 3. pairs of the steps with hits
+  const uint32_t nblk = (uint32_t)ceil_div(nsteps, (uint64_t)kProbeScanBlock);  // <= kProbeScanBlock
+  k_probe_block_totals<<<nblk, kProbeScanBlock, 0, s>>>(w.wcnt, nsteps, w.btot);
+  YSQL_LAUNCH_CHECK();
+  k_probe_offsets<<<nblk, kProbeScanBlock, 0, s>>>(w.wcnt, nsteps, w.btot, w.status, count_out);
+  YSQL_LAUNCH_CHECK();
+  k_join_write<T><<<(uint32_t)nsteps, kJoinBlock, 0, s>>>(left, w.table, w.used, tcap - 1, w.perm, w.hits, w.wcnt,
+                                                          w.status, pairs, cap);
   YSQL_LAUNCH_CHECK();
   return YSQL_OK;
 }
