@@ -24,6 +24,7 @@ constexpr int kBins = 256;
 constexpr uint32_t kRFlagAgg = 1u << 30;
 constexpr uint32_t kRFlagInc = 2u << 30;
 constexpr uint32_t kRValMask = (1u << 30) - 1;
+constexpr int kLookbackBatch = 8;  // predecessor tiles read per look-back step
 
 __device__ __forceinline__ uint32_t rstatus_load(const uint32_t *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -174,20 +175,36 @@ __device__ __forceinline__ void onesweep_tile(OnesweepSmem<K, BLOCK, KPT> &sm, u
     cnt = run;
     rstatus_store(&status[tile * kBins + t], (tile == 0 ? kRFlagInc : kRFlagAgg) | cnt);
   }
-  // Decoupled look-back, one thread per digit.  EARLY walks back right
-  // after publishing the aggregate; the default walks back after the LDS
-  // reorder (measured faster on MI355X).
+  // Decoupled look-back, one thread per digit, kLookbackBatch predecessors
+  // per step (their status loads are in flight together).  A step counts
+  // only when every predecessor up to the first inclusive one in it has
+  // published; otherwise it is read again.  EARLY walks back right after
+  // publishing the aggregate; the default walks back after the LDS reorder
+  // (measured faster on MI355X).
   uint32_t excl = 0;
   auto lookback = [&]() {
     if (t < kBins && tile > 0) {
       int64_t k = (int64_t)tile - 1;
       while (true) {
-        const uint32_t s = rstatus_load(&status[(uint32_t)k * kBins + t]);
-        const uint32_t f = s & ~kRValMask;
-        if (f == 0) continue;
-        excl += s & kRValMask;
-        if (f == kRFlagInc) break;
-        k--;
+        uint32_t s[kLookbackBatch];
+#pragma unroll
+        for (int b = 0; b < kLookbackBatch; b++)  // tile 0 is inclusive, so k - b < 0 is never needed
+          s[b] = k - b >= 0 ? rstatus_load(&status[(uint32_t)(k - b) * kBins + t]) : kRFlagInc;
+        bool ready = true, inc = false;
+        uint32_t add = 0;
+#pragma unroll
+        for (int b = 0; b < kLookbackBatch; b++) {
+          if (!inc) {
+            const uint32_t f = s[b] & ~kRValMask;
+            ready = ready && f != 0;
+            add += s[b] & kRValMask;
+            inc = f == kRFlagInc;
+          }
+        }
+        if (!ready) continue;
+        excl += add;
+        if (inc) break;
+        k -= kLookbackBatch;
       }
       rstatus_store(&status[tile * kBins + t], kRFlagInc | (excl + cnt));
     }
