@@ -68,7 +68,23 @@ __device__ __forceinline__ void hist_block(const K *keys, uint64_t begin, uint64
   const uint64_t vstart = i;
   const uint64_t nv = (end - vstart) / PER;
   const uint4 *kv = reinterpret_cast<const uint4 *>(keys + vstart);
-  for (uint64_t v = threadIdx.x; v < nv; v += blockDim.x) {
+  // kHistUnroll loads of a thread are issued before any key is counted, so
+  // that enough bytes are in flight per wave
+  constexpr int kHistUnroll = 4;
+  uint64_t v = threadIdx.x;
+  for (; v + (kHistUnroll - 1) * (uint64_t)blockDim.x < nv; v += kHistUnroll * (uint64_t)blockDim.x) {
+    uint4 q[kHistUnroll];
+#pragma unroll
+    for (int u = 0; u < kHistUnroll; u++) q[u] = kv[v + u * (uint64_t)blockDim.x];
+#pragma unroll
+    for (int u = 0; u < kHistUnroll; u++) {
+      K ks[PER];
+      memcpy(ks, &q[u], 16);
+#pragma unroll
+      for (int e = 0; e < PER; e++) hist_key<K>(ks[e], begin_bit, passes, desc, s_h, zc);
+    }
+  }
+  for (; v < nv; v += blockDim.x) {
     const uint4 q = kv[v];
     K ks[PER];
     memcpy(ks, &q, 16);
