@@ -647,6 +647,31 @@ def test_join_eq_blocked_filter(ops, dev, dtype):
         assert np.array_equal(N(pairs).view(np.uint32), exp)
 
 
+@pytest.mark.parametrize("R_", [2_000, 20_000])
+def test_join_eq_many_steps(ops, dev, R_):
+    """EQ join of 5,000,001 left rows (1,221 probe steps: the step-offset
+    scan spans two blocks, the last step is partial).  Expected pairs from a
+    stable sort of the right keys and a binary search per left key: an
+    independent computation whose order is (l, r) by construction."""
+    r = rng(R_ + 9)
+    L_ = 5_000_001
+    left = r.integers(0, 100_000, L_).astype(np.int64)
+    right = r.integers(0, 100_000, R_).astype(np.int64)
+    right[:40] = right[40]
+    order = np.argsort(right, kind="stable")
+    rs = right[order]
+    lo = np.searchsorted(rs, left, "left")
+    cnt = np.searchsorted(rs, left, "right") - lo
+    total = int(cnt.sum())
+    first = np.cumsum(cnt) - cnt  # output position of each left row's first pair
+    exp = np.empty((total, 2), np.uint32)
+    exp[:, 0] = np.repeat(np.arange(L_, dtype=np.uint32), cnt)
+    exp[:, 1] = order[np.repeat(lo - first, cnt) + np.arange(total)]
+    pairs, c = ops.join_pairs(4, T(left, dev), T(right, dev))
+    assert c == total > 50_000
+    assert np.array_equal(N(pairs).view(np.uint32), exp)
+
+
 def test_join_overflow_reports_count(ops, dev):
     left = np.zeros(100, np.int32)
     right = np.zeros(50, np.int32)
