@@ -358,7 +358,7 @@ def csv_load_line(n=2_000_000):
         return {"rows": n, "csv_MB": size / 1e6, "s": t_dev, "rows_per_s": n / t_dev, "MB_s": size / t_dev / 1e6,
                 "parse_only_s": t_host,
                 "what": "CSV (id INT, price FLOAT, qty INT) -> device batches of 349,440 rows: file mapped, "
-                        "record starts found in one pass, up to 16 host threads converting row ranges, pinned "
+                        "record starts found in one SSE2 pass, up to 16 host threads converting row ranges, pinned "
                         "double-buffered staging, H2D copies on a side stream"}
     finally:
         os.unlink(path)

@@ -218,3 +218,26 @@ def test_parallel_parse_quotes_and_blank_lines(loader, tmp_path):
     with pytest.raises(L.YsqlError):
         _read_all(rd)
     rd.close()
+
+
+def test_quote_free_fast_path(loader, tmp_path):
+    """A file with no quotes (newlines alone end records): blank "\\n" and
+    "\\r\\n" lines, mixed line ends, one quoted record with an embedded
+    newline late in the file, and no final newline."""
+    n = 70_000
+    r = np.random.default_rng(33)
+    ids = r.integers(-10**6, 10**6, n).astype(np.int32)
+    p = tmp_path / "qf.csv"
+    with open(p, "w", newline="") as f:
+        f.write("id,tag\n")
+        for i in range(n):
+            if i % 997 == 5:
+                f.write("\n" if i % 2 else "\r\n")
+            tag = "a b" if i % 13 == 0 else ('"q,\n"' if i == n - 100 else "t")
+            f.write("%d,%s%s" % (ids[i], tag, "" if i == n - 1 else ("\r\n" if i % 3 == 0 else "\n")))
+    rd = loader.CsvReader(p, ["id"], [INT])
+    cols, nulls = _read_all(rd)
+    rd.close()
+    assert np.array_equal(cols[0], ids) and not nulls[0].any()
+    oc, on = O.read(p, ["id"], [INT])
+    assert np.array_equal(cols[0], oc[0]) and np.array_equal(nulls[0], on[0])

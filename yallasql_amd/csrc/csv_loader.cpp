@@ -9,6 +9,7 @@
 // next batch is parsed.
 #include <hip/hip_runtime.h>
 
+#include <emmintrin.h>
 #include <fcntl.h>
 #include <sys/mman.h>
 #include <sys/stat.h>
@@ -242,6 +243,37 @@ bool convert(int type, const std::string &s, void *col, uint8_t *nulls, uint32_t
   }
 }
 
+// Record starts in quote-free text [p, lim), where every '\n' ends a record:
+// appends the starts of the complete records found (blank ones, "\n" or
+// "\r\n", skipped when `skip_blank`) until `want` starts are held, and
+// returns the start of the first record not consumed.  Newlines are found
+// 16 bytes at a time.
+const char *quote_free_starts(const char *p, const char *lim, size_t want, bool skip_blank,
+                              std::vector<const char *> &starts) {
+  const char *rs = p, *c = p;
+  auto record_end = [&](const char *nl) {
+    const bool blank = nl == rs || (nl == rs + 1 && *rs == '\r');
+    if (!(blank && skip_blank)) starts.push_back(rs);
+    rs = nl + 1;
+  };
+  const __m128i nlv = _mm_set1_epi8('\n');
+  for (; c + 16 <= lim; c += 16) {
+    unsigned m = (unsigned)_mm_movemask_epi8(_mm_cmpeq_epi8(_mm_loadu_si128((const __m128i *)c), nlv));
+    while (m) {
+      record_end(c + __builtin_ctz(m));
+      m &= m - 1;
+      if (starts.size() >= want) return rs;
+    }
+  }
+  for (; c < lim; c++) {
+    if (*c == '\n') {
+      record_end(c);
+      if (starts.size() >= want) return rs;
+    }
+  }
+  return rs;
+}
+
 // Converts records [r0, r1) of the batch (starts[]) into rows r0.. of the
 // column buffers; returns the first failing row (r1: none) and its error.
 uint32_t parse_rows(const ysql_csv_reader *r, uint32_t r0, uint32_t r1, void *const *cols, uint8_t *const *nulls,
@@ -271,6 +303,12 @@ uint32_t parse_rows(const ysql_csv_reader *r, uint32_t r0, uint32_t r1, void *co
 int parse_batch(ysql_csv_reader *r, void *const *cols, uint8_t *const *nulls, uint32_t *rows) {
   r->starts.clear();
   while (r->starts.size() < r->batch_rows && r->pos < r->end) {
+    const char *q = r->quote_at_or_after(r->pos);
+    if (q > r->pos) {  // no quote before q: newlines alone end records
+      r->pos = quote_free_starts(r->pos, q, r->batch_rows, r->nfields > 1, r->starts);
+      if (r->starts.size() >= r->batch_rows || r->pos >= r->end) break;
+    }
+    // the record at r->pos holds the quote at q, or ends the file unterminated
     bool blank = false;
     const char *rs = r->pos;
     r->pos = r->scan_record(rs, &blank);
