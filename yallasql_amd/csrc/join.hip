@@ -589,13 +589,13 @@ __global__ void k_store_zero_u64(uint64_t *p) { *p = 0; }
 
 // ---- workspace layout
 struct JoinWs {
-  uint32_t *ctr;      // [0] tile counter, [1] NaN count
-  uint64_t *status;   // ntiles + 1
+  uint32_t *ctr;      // [0] tile counter (nested loop), [1] NaN count of the right keys (hash)
+  uint64_t *status;   // ntiles + 1: look-back status (nested loop) or step offsets (hash probe)
   void *keys;         // R keys (u32 or u64)
   uint32_t *perm;     // R
   HashSlot *table;    // cap
   uint32_t *used;     // cap
-  uint32_t *bloom;    // kBloomMaxWords (membership filter of the global-table probe)
+  uint32_t *bloom;    // kBloomMaxWords (membership filter of the global-table count pass)
   uint16_t *hits;     // probe steps * kJoinBlock hit masks
   uint64_t *wcnt;     // probe steps * kProbeWaves pair counts
   uint64_t *btot;     // kProbeScanBlock block totals of the step scan
