@@ -667,9 +667,23 @@ def test_join_eq_many_steps(ops, dev, R_):
     exp = np.empty((total, 2), np.uint32)
     exp[:, 0] = np.repeat(np.arange(L_, dtype=np.uint32), cnt)
     exp[:, 1] = order[np.repeat(lo - first, cnt) + np.arange(total)]
-    pairs, c = ops.join_pairs(4, T(left, dev), T(right, dev))
+    lt, rt = T(left, dev), T(right, dev)
+    pairs, c = ops.join_pairs(4, lt, rt)
     assert c == total > 50_000
     assert np.array_equal(N(pairs).view(np.uint32), exp)
+    # capacity of half the pairs: the exact count, and exactly the first half
+    from yallasql_amd import _lib as L
+
+    cap = total // 2
+    ws = ops.workspace(L.ws_bytes("ysql_join_ws_bytes", L.DATETIME, L.EQ, L_, R_), dev)
+    out = torch.full((cap + 64, 2), -1, dtype=torch.int32, device=dev)
+    cnt = torch.zeros(1, dtype=torch.int64, device=dev)
+    L.call("ysql_join_pairs", L.DATETIME, L.EQ, ops._p(lt), L_, ops._p(rt), R_, ops._p(out), cap, ops._p(cnt),
+           ops._p(ws), ws.numel(), ops._s())
+    assert int(cnt.item()) == total
+    got = N(out).view(np.uint32)
+    assert np.array_equal(got[:cap], exp[:cap])
+    assert (N(out)[cap:] == -1).all()  # nothing written past the capacity
 
 
 def test_join_overflow_reports_count(ops, dev):
