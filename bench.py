@@ -56,116 +56,6 @@ def gen_keys(n, dev):
     return multirun_keys(n, dev, seed=SEED)
 
 
-class MultiRunSort:
-    """Runs of RUN_ROWS rows radix-sorted stepwise, then merged pairwise."""
-
-    def __init__(self, keys_in, run_rows):
-        import torch
-
-        from yallasql_amd import _lib as L
-        from yallasql_amd import ops
-
-        self.L, self.ops, self.torch = L, ops, torch
-        dev = keys_in.device
-        self.n = n = keys_in.numel()
-        self.keys_in = keys_in
-        self.runs = [(o, min(run_rows, n - o)) for o in range(0, n, run_rows)]
-        self.bk = [torch.empty(n, dtype=torch.int64, device=dev) for _ in range(3)]
-        self.bv = [torch.empty(n, dtype=torch.int32, device=dev) for _ in range(3)]
-        self.alt_k = torch.empty(run_rows, dtype=torch.int64, device=dev)
-        self.alt_v = torch.empty(run_rows, dtype=torch.int32, device=dev)
-        self.sort_ws = ops.workspace(L.ws_bytes("ysql_radix_sort_ws_bytes", L.UKEY64, run_rows), dev)
-        self.merge_ws = ops.workspace(L.ws_bytes("ysql_merge_ws_bytes", n, n, n), dev)
-        self.passes = (KEY_BITS + 7) // 8
-        self.pass_events = []   # (start, end, rows) per onesweep launch
-        self.merge_events = []  # (start, end, rows) per merge
-        self.record = False
-        self.final = None
-
-    def _ev(self):
-        e = self.torch.cuda.Event(enable_timing=True)
-        e.record()
-        return e
-
-    def sort_run(self, off, rows):
-        L, ops, p = self.L, self.ops, self.ops._p
-        s = ops._s()
-        kin = self.keys_in[off:off + rows]
-        ko, vo = self.bk[0][off:off + rows], self.bv[0][off:off + rows]
-        ka, va = self.alt_k[:rows], self.alt_v[:rows]
-        ws = self.sort_ws
-        L.call("ysql_radix_begin", L.UKEY64, 0, p(kin), rows, 0, KEY_BITS, p(ws), ws.numel(), s)
-        src_k, src_v = kin, None
-        P = self.passes
-        for q in range(P):
-            to_out = ((P - 1 - q) & 1) == 0
-            dk, dv = (ko, vo) if to_out else (ka, va)
-            e0 = self._ev() if self.record else None
-            L.call("ysql_radix_pass", L.UKEY64, 0, q, p(src_k), p(src_v), off, p(dk), p(dv), rows, 0, KEY_BITS,
-                   p(ws), ws.numel(), s)
-            if self.record:
-                self.pass_events.append((e0, self._ev(), rows))
-            src_k, src_v = dk, dv
-        return (0, off, rows)
-
-    def merge(self, a, b):
-        (ba, oa, na), (bb, ob, nb) = a, b
-        out = min({0, 1, 2} - {ba, bb})
-        off = min(oa, ob)
-        ok, ov = self.bk[out][off:off + na + nb], self.bv[out][off:off + na + nb]
-        e0 = self._ev() if self.record else None
-        self.ops.merge_pairs(self.bk[ba][oa:oa + na], self.bv[ba][oa:oa + na], self.bk[bb][ob:ob + nb],
-                             self.bv[bb][ob:ob + nb], out_keys=ok, out_vals=ov, ws=self.merge_ws)
-        if self.record:
-            self.merge_events.append((e0, self._ev(), na + nb))
-        return (out, off, na + nb)
-
-    def step(self):
-        runs = [self.sort_run(o, r) for o, r in self.runs]
-        while len(runs) > 1:
-            nxt = [self.merge(runs[i], runs[i + 1]) for i in range(0, len(runs) - 1, 2)]
-            if len(runs) % 2:
-                nxt.append(runs[-1])
-            runs = nxt
-        self.final = runs[0]
-
-    def result(self):
-        b, o, n = self.final
-        return self.bk[b][o:o + n], self.bv[b][o:o + n]
-
-    def algorithmic_bytes(self):
-        """Sort: P * 2 * n * (8 + 4) per run; merge: 2 * (m + n) * 12 per merge."""
-        b = sum(self.passes * 2 * r * 12 for _, r in self.runs)
-        sizes = [r for _, r in self.runs]
-        while len(sizes) > 1:
-            nxt = []
-            for i in range(0, len(sizes) - 1, 2):
-                b += 2 * (sizes[i] + sizes[i + 1]) * 12
-                nxt.append(sizes[i] + sizes[i + 1])
-            if len(sizes) % 2:
-                nxt.append(sizes[-1])
-            sizes = nxt
-        return b
-
-
-def verify(keys_in, keys_out, vals_out):
-    """Sorted, stable, a permutation of the rows, and keys match their rows:
-    together these equal the oracle's global stable sort exactly."""
-    import torch
-
-    n = keys_in.numel()
-    assert keys_out.numel() == n
-    d = keys_out[1:] - keys_out[:-1]
-    assert bool((d >= 0).all()), "keys not sorted"
-    ties = d == 0
-    assert bool((vals_out[1:][ties] > vals_out[:-1][ties]).all()), "ties out of input order"
-    cnt = torch.zeros(n, dtype=torch.int32, device=keys_in.device)
-    cnt.index_add_(0, vals_out.long(), torch.ones_like(vals_out))
-    assert bool((cnt == 1).all()), "payload is not a permutation"
-    del cnt
-    assert bool((keys_in[vals_out.long()] == keys_out).all()), "keys do not match their rows"
-
-
 def cpu_baseline(keys_host, run_rows):
     """Oracle multi-run sort (same pipeline) on the first two runs, all host threads."""
     from oracle import pyoracle as O
@@ -372,23 +262,11 @@ def config3_line():
     import torch
 
     from yallasql_amd import ops
+    from yallasql_amd.datagen import star_schema
 
     dev = torch.device("cuda:0")
     n = 600_000_000
-    g = torch.Generator(device=dev)
-    g.manual_seed(2)
-    rows = [2556, 2000, 30000, 200000]
-    fks = [torch.randint(0, R, (n,), generator=g, device=dev, dtype=torch.int32) for R in rows]
-    m1 = torch.randint(90000, 10000001, (n,), generator=g, device=dev, dtype=torch.int32)
-    m2 = torch.randint(0, 11, (n,), generator=g, device=dev, dtype=torch.int32)
-    year = (1992 + (7 * torch.arange(2556, device=dev)) // 2556).to(torch.int32)
-    region = torch.randint(0, 7, (30000,), generator=g, device=dev, dtype=torch.int32)
-    cat = torch.randint(0, 25, (200000,), generator=g, device=dev, dtype=torch.int32)
-    brand = torch.randint(0, 1000, (200000,), generator=g, device=dev, dtype=torch.int32)
-    dims = [{"fk": fks[0], "rows": 2556, "group": year, "group_base": 1992, "group_card": 7},
-            {"fk": fks[1], "rows": 2000},
-            {"fk": fks[2], "rows": 30000, "pass": (region == 0).to(torch.uint8)},
-            {"fk": fks[3], "rows": 200000, "pass": (cat == 0).to(torch.uint8), "group": brand, "group_card": 1000}]
+    dims, m1, m2 = star_schema(n, dev, seed=2)
     sums = torch.zeros(7000, dtype=torch.int64, device=dev)
     counts = torch.zeros(7000, dtype=torch.int64, device=dev)
 
@@ -403,7 +281,7 @@ def config3_line():
     out = {"fact_rows": n, "ms": ms, "rows_per_s": n / (ms * 1e-3), "qualifying_rows": int(counts.sum().item()),
            "GB_s": b / (ms * 1e-3) / 1e9, "frac": b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
            "what": "fused 4-FK star join + GROUP BY year x brand SUM(int64(price)*discount)"}
-    del fks, m1, m2
+    del dims, m1, m2
     torch.cuda.empty_cache()
     return out
 
@@ -497,7 +375,9 @@ def main():
     if args.calib_copy:
         torch.empty_like(keys).copy_(keys)
     torch.cuda.synchronize()
-    job = MultiRunSort(keys, RUN_ROWS)
+    from yallasql_amd.multirun import MultiRunSort, verify_global_sort
+
+    job = MultiRunSort(keys, RUN_ROWS, key_bits=KEY_BITS)
     for _ in range(args.warmup):
         job.step()
     torch.cuda.synchronize()
@@ -521,7 +401,7 @@ def main():
 
     if not args.no_verify:
         ko, vo = job.result()
-        verify(keys, ko, vo)
+        verify_global_sort(keys, ko, vo)
 
     # attainable HBM bandwidth: a plain 1.6 GB device copy (read + write)
     dst = torch.empty_like(keys)

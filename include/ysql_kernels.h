@@ -172,7 +172,9 @@ int ysql_radix_sort_variant(int key_dtype, int variant, int descending, const vo
  * histograms of keys_in; then ysql_radix_pass for pass = 0 .. P-1,
  * P = ceil((end_bit - begin_bit) / 8), each reading the previous pass's
  * output (pass 0: keys_in, vals_in or val_base + row) into a distinct
- * buffer.  All calls of one sort share `ws` and the stream.               */
+ * buffer: keys_out == keys_in, or a non-null vals_in == vals_out, returns
+ * YSQL_EINVAL (tiles scatter while other tiles still read their input).
+ * All calls of one sort share `ws` and the stream.                        */
 int ysql_radix_begin(int key_dtype, int descending, const void *keys_in, uint64_t n, int begin_bit,
                      int end_bit, void *ws, size_t ws_bytes, ysql_stream_t stream);
 int ysql_radix_pass(int key_dtype, int descending, int pass, const void *keys_in,

@@ -65,6 +65,13 @@ def test_invalid_arguments_rejected_before_launch():
     assert L.lib.ysql_compact_cols(C.c_void_p(8), cols, 1, 10, C.c_void_p(8), C.c_void_p(8), 1 << 20,
                                    None) == L.EINVAL
     assert L.lib.ysql_star_join_group_sum(None, 0, None, None, 10, None, None, 10, None) == L.EINVAL
+    # a digit pass scatters into buffers other tiles may still read: keys or
+    # payloads in place are refused
+    k, v, k2 = C.c_void_p(256), C.c_void_p(512), C.c_void_p(768)
+    assert L.lib.ysql_radix_pass(L.UKEY64, 0, 0, k, None, 0, k, v, 10, 0, 64, C.c_void_p(8), 1 << 20,
+                                 None) == L.EINVAL
+    assert L.lib.ysql_radix_pass(L.UKEY64, 0, 1, k, v, 0, k2, v, 10, 0, 64, C.c_void_p(8), 1 << 20,
+                                 None) == L.EINVAL
 
 
 def test_reference_shaped_launchers_compile(tmp_path):

@@ -54,10 +54,13 @@ def test_graph_batch_equals_plain_bitwise(jobset):
             assert np.array_equal(a, b)
 
 
-def check_jobs_against_oracle(jobset):
-    from yallasql_amd.batched import AGG, FILTER, SORT
+def check_jobs_against_oracle(jobset, jobs=None):
+    """Each job (all, or the indices in `jobs`) against the oracle: sorts by
+    the stable argsort, filters by boolean indexing, aggregates exactly
+    (float SUM within 1e-6 relative of math.fsum)."""
+    from yallasql_amd.batched import AGG, FILTER, SORT  # noqa: F401
 
-    for j in range(len(jobset.sizes)):
+    for j in (range(len(jobset.sizes)) if jobs is None else jobs):
         keys, vals = jobset.job_inputs(j)
         kind, thr = jobset.kinds[j], np.float32(jobset.thr[j])
         ukeys = keys.view(np.uint64 if keys.itemsize == 8 else np.uint32)

@@ -326,7 +326,8 @@ int ysql_radix_pass(int key_dtype, int descending, int pass, const void *keys_in
                     void *ws, size_t ws_bytes, ysql_stream_t stream) {
   if (n == 0) return YSQL_OK;
   hipStream_t s = as_stream(stream);
-  if (!keys_out || !vals_out || keys_in == keys_out) return YSQL_EINVAL;
+  // each pass scatters into buffers distinct from what other tiles still read
+  if (!keys_out || !vals_out || keys_in == keys_out || (vals_in && vals_in == vals_out)) return YSQL_EINVAL;
   const int P = radix_passes(begin_bit, end_bit);
   if (pass < 0 || pass >= P) return YSQL_EINVAL;
   RadixWs w;

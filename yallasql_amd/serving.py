@@ -27,6 +27,7 @@ pipeline) reserves an estimate of its device memory from the HBM budget and
 waits (in arrival order) while the budget or every slot is taken.
 """
 import collections
+import contextlib
 import math
 import threading
 import time
@@ -194,10 +195,14 @@ class Server:
     """Dispatcher (the calling thread) and a pool of `slots` workers, each
     with its own stream.  Call close() when done."""
 
-    def __init__(self, table, slots=8, hbm_budget=200 << 30, shared_scan=True):
+    def __init__(self, table, slots=8, hbm_budget=200 << 30, shared_scan=True, runners=None):
         import queue
 
         self.table = table
+        # unit runners by query kind: fn(table, queries) -> table rows read
+        self.runners = runners or {FILTER_AGG: run_filter_agg, TOPK: lambda t, qs: run_topk(t, qs[0]),
+                                   JOIN_GROUP: lambda t, qs: run_join_group(t, qs[0])}
+        self.error = None  # the first exception a unit raised; serve() re-raises it
         self.slots = slots
         self.budget = hbm_budget
         self.shared_scan = shared_scan
@@ -219,9 +224,14 @@ class Server:
             th.join()
 
     def _worker(self, k):
-        torch.cuda.set_device(self.table.device)
-        stream = torch.cuda.Stream(device=self.table.device)
-        with torch.cuda.stream(stream):
+        dev = self.table.device
+        if dev.type == "cuda":
+            torch.cuda.set_device(dev)
+            stream = torch.cuda.Stream(device=dev)
+            ctx = torch.cuda.stream(stream)
+        else:  # host-only runners (tests of the dispatcher itself)
+            stream, ctx = None, contextlib.nullcontext()
+        with ctx:
             while True:
                 item = self.inboxes[k].get()
                 if item is None:
@@ -230,23 +240,26 @@ class Server:
                 t0 = time.perf_counter()
                 for q in qs:
                     q.start = t0
-                if kind == FILTER_AGG:
-                    scanned = run_filter_agg(self.table, qs)
-                elif kind == TOPK:
-                    scanned = run_topk(self.table, qs[0])
-                else:
-                    scanned = run_join_group(self.table, qs[0])
-                stream.synchronize()
-                t1 = time.perf_counter()
-                for q in qs:
-                    q.done = t1
-                with self.cv:
-                    self.free_bytes += nbytes
-                    self.outstanding -= 1
-                    self.rows_scanned += scanned
-                    self.rows_queried += self.table.n * len(qs)
-                    self.idle.append(k)
-                    self.cv.notify_all()
+                scanned = 0
+                try:
+                    scanned = self.runners[kind](self.table, qs)
+                    if stream is not None:
+                        stream.synchronize()
+                except BaseException as e:  # kept for serve(); the slot and bytes are still returned
+                    with self.cv:
+                        if self.error is None:
+                            self.error = e
+                finally:
+                    t1 = time.perf_counter()
+                    for q in qs:
+                        q.done = t1
+                    with self.cv:
+                        self.free_bytes += nbytes
+                        self.outstanding -= 1
+                        self.rows_scanned += scanned
+                        self.rows_queried += self.table.n * len(qs)
+                        self.idle.append(k)
+                        self.cv.notify_all()
 
     def _next_unit(self, pending):
         head = pending[0]
@@ -271,6 +284,13 @@ class Server:
                     q.arrival = now
                 pending.append(q)
             with self.cv:
+                if self.error is not None:
+                    # a unit failed: admit nothing more, let the units in
+                    # flight finish, then raise the failure
+                    while self.outstanding:
+                        self.cv.wait()
+                    err, self.error = self.error, None
+                    raise RuntimeError("a serving unit failed") from err
                 while pending and self.idle:
                     kind, group = self._next_unit(pending)
                     nbytes = unit_bytes(self.table, kind, len(group))
