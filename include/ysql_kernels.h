@@ -23,6 +23,8 @@
  *     `utils/macros.hpp:6-22`).
  *   - Entry points keep no global mutable state: they are reentrant across
  *     streams.  Calls that share one workspace on two streams at once race.
+ *     The one exception is ysql_filter_compact_noalloc, whose scratch is a
+ *     static device array shared by every call on a GPU (see there).
  */
 #ifndef YSQL_KERNELS_H
 #define YSQL_KERNELS_H
@@ -255,6 +257,17 @@ int ysql_compact_cols(const uint8_t *mask, const ysql_col *cols, int ncols, uint
 int ysql_filter_compact(int dtype, int cmp, const void *pred_col, const void *scalar,
                         const ysql_col *cols, int ncols, uint64_t n, uint64_t *count_out,
                         void *ws, size_t ws_bytes, ysql_stream_t stream);
+/* ysql_filter_compact without a workspace argument, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid allocation).  No counterpart in the reference.  Its
+ * look-back scratch is a static device array of the library (about 8 MiB,
+ * held by each GPU it runs on until the process ends), so it allocates
+ * nothing; n above 2^32 rows returns YSQL_EWORKSPACE.  Every call on one GPU
+ * shares that array: calls that overlap in time on one GPU (other streams,
+ * other threads) race and must be serialised by the caller.               */
+int ysql_filter_compact_noalloc(int dtype, int cmp, const void *pred_col, const void *scalar,
+                                const ysql_col *cols, int ncols, uint64_t n,
+                                uint64_t *count_out, ysql_stream_t stream);
 /* Same with a range predicate bounds[0] <= pred_col[i] < bounds[1] (bounds:
  * 2-element device array; the reference expresses it as two comparisons
  * ANDed, conjunction.hpp:76-113, and compacts from the mask) and an output

@@ -88,6 +88,7 @@ _SIGS = {
     "ysql_csv_next_host": [P, C.POINTER(P), C.POINTER(P), C.POINTER(U32)],
     "ysql_csv_next": [P, C.POINTER(P), C.POINTER(P), C.POINTER(U32), P],
     "ysql_filter_compact": [I32, I32, P, P, C.POINTER(Col), I32, U64, P, P, C.c_size_t, P],
+    "ysql_filter_compact_noalloc": [I32, I32, P, P, C.POINTER(Col), I32, U64, P, P],
     "ysql_filter_compact_range": [I32, P, P, C.POINTER(Col), I32, U64, U64, P, P, C.c_size_t, P],
     "ysql_compare": [I32, I32, P, I32, P, I32, I32, P, U64, P],
     "ysql_mask_logic": [I32, P, I32, P, I32, P, U64, P],

@@ -8,6 +8,9 @@
 // Brent-Kung scan with a serial carry (prefix_sum_kernel.cu:7-103), its
 // separate mask, map and per-column scatter passes (filter_operator.hpp:31-98)
 // become one pass.
+#include <atomic>
+#include <mutex>
+
 #include "scan_dev.hpp"
 
 namespace ysql {
@@ -268,6 +271,45 @@ static int compact_impl(int mode, int dtype, int cmp, const uint8_t *mask, const
   return YSQL_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Scratch for ysql_filter_compact_noalloc: the tile counter and look-back
+// status words of a compaction of up to kNoAllocMaxRows rows, in a static
+// device array (each GPU the library runs on holds its own copy until the
+// process ends).  Host code never passes `scratch` itself to a kernel: its
+// device address comes from hipGetSymbolAddress, resolved once per device on
+// that device's first call and kept in g_scratch_addr.
+// ---------------------------------------------------------------------------
+constexpr uint64_t kNoAllocMaxRows = 1ull << 32;
+// the counter words, then one 8-byte status word per 4096-row tile (+1), each piece 256-byte aligned
+constexpr size_t kNoAllocScratchBytes = 256 + (8 * (kNoAllocMaxRows / kScanTile + 1) + 255) / 256 * 256;  // ~8 MiB
+constexpr int kNoAllocMaxDevices = 64;
+
+__device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+static std::atomic<void *> g_scratch_addr[kNoAllocMaxDevices];
+static std::mutex g_scratch_mu;
+
+// The current device's address of `scratch`, or nullptr (error text set).
+static void *noalloc_scratch() {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kNoAllocMaxDevices) {
+    set_last_error("ysql_filter_compact_noalloc: no usable current device");
+    return nullptr;
+  }
+  void *p = g_scratch_addr[dev].load(std::memory_order_acquire);
+  if (p) return p;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  p = g_scratch_addr[dev].load(std::memory_order_relaxed);
+  if (p) return p;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    set_last_error(e != hipSuccess ? hipGetErrorString(e) : "hipGetSymbolAddress returned no address");
+    return nullptr;
+  }
+  g_scratch_addr[dev].store(p, std::memory_order_release);
+  return p;
+}
+
 }  // namespace ysql
 
 using namespace ysql;
@@ -325,6 +367,21 @@ int ysql_filter_compact(int dtype, int cmp, const void *pred_col, const void *sc
   if (cmp < YSQL_LT || cmp > YSQL_NE) return YSQL_EINVAL;
   return compact_impl(1, dtype, cmp, nullptr, pred_col, scalar, cols, ncols, n, ~0ull, count_out, ws, ws_bytes,
                       as_stream(stream));
+}
+
+int ysql_filter_compact_noalloc(int dtype, int cmp, const void *pred_col, const void *scalar, const ysql_col *cols,
+                                int ncols, uint64_t n, uint64_t *count_out, ysql_stream_t stream) {
+  if (cmp < YSQL_LT || cmp > YSQL_NE) return YSQL_EINVAL;
+  if (n > kNoAllocMaxRows) {
+    set_last_error("ysql_filter_compact_noalloc: more rows than the static scratch holds (2^32)");
+    return YSQL_EWORKSPACE;
+  }
+  void *ws = nullptr;
+  if (n > 0 && !(ws = noalloc_scratch())) return YSQL_EHIP;
+  // the tile counter and status words are cleared by a kernel inside
+  // compact_impl (scan_reset) before the compaction reads them
+  return compact_impl(1, dtype, cmp, nullptr, pred_col, scalar, cols, ncols, n, ~0ull, count_out, ws,
+                      kNoAllocScratchBytes, as_stream(stream));
 }
 
 int ysql_filter_compact_range(int dtype, const void *pred_col, const void *bounds, const ysql_col *cols, int ncols,
