@@ -288,8 +288,9 @@ def config3_line():
 
 def batched_lines():
     """4,096 ragged sort/filter/aggregate jobs (log-uniform 1,024..4,194,304
-    rows): the per-job launch loop against the captured graph replay."""
-    import numpy as np
+    rows): the per-job launch loop against the persistent launch (one kernel,
+    device work queue, captured with its reset in a hipGraph) and against the
+    launch-chain graph.  Outputs compared on the device, buffer by buffer."""
     import torch
 
     from yallasql_amd.autotune import load_sort_variants
@@ -299,6 +300,7 @@ def batched_lines():
     sizes, kinds, key64, thr = config4_jobs()
     js = JobSet(sizes, kinds, key64, thr, dev, seed=4).create(chains=32, sort_variants=load_sort_variants())
     js.run_batched()
+    js.run_graph()
     js.run_plain()
     torch.cuda.synchronize()
 
@@ -314,22 +316,31 @@ def batched_lines():
         return best
 
     t_plain = wall(js.run_plain)
-    t_graph = wall(js.run_batched)
+    t_graph = wall(js.run_graph)
+    t_pers = wall(js.run_batched)
+    outs = (js.o32, js.o64, js.ov, js.perm, js.counts, js.accs)
     js.clear_outputs()
     js.run_plain()
-    a = js.snapshot()
+    torch.cuda.synchronize()
+    ref = [t.clone() for t in outs]
     js.clear_outputs()
     js.run_batched()
-    b = js.snapshot()
-    same = all(np.array_equal(x, y) for x, y in zip(a, b))
+    done, total = js.check()
+    same = done == total and all(torch.equal(a, b) for a, b in zip(ref, outs))
+    js.clear_outputs()
+    js.run_graph()
+    torch.cuda.synchronize()
+    same_graph = all(torch.equal(a, b) for a, b in zip(ref, outs))
+    del ref
     alg = js.algorithmic_bytes()
-    out = {"jobs": len(sizes), "rows": js.total_rows, "plain_s": t_plain, "batched_s": t_graph,
-           "speedup": t_plain / t_graph, "identical_outputs": bool(same),
-           "batched_GB_s": alg / t_graph / 1e9, "batched_frac": alg / t_graph / 1e9 / HBM_PEAK_GBS,
-           "chains": js.info()["chains"],
-           "what": "plain: per-job launches on one stream; batched: one hipGraph replay (sorts <= 512K rows, "
-                   "filters <= 2M rows and aggregates <= 4M rows one per workgroup in segmented kernels, larger "
-                   "jobs in 32 launch chains); sort tiles from the autotuned table"}
+    out = {"jobs": len(sizes), "rows": js.total_rows, "plain_s": t_plain, "batched_s": t_pers,
+           "speedup": t_plain / t_pers, "identical_outputs": bool(same),
+           "batched_GB_s": alg / t_pers / 1e9, "batched_frac": alg / t_pers / 1e9 / HBM_PEAK_GBS,
+           "workers": js.workers(), "jobs_completed": done,
+           "graph_s": t_graph, "graph_speedup": t_plain / t_graph, "graph_identical_outputs": bool(same_graph),
+           "what": "plain: per-job launches on one stream; batched: ONE persistent launch (workgroups take whole "
+                   "jobs and split-sort chunk tasks from device job lists and a phase queue), replayed as a "
+                   "hipGraph with its reset kernel; graph: launch chains beside segmented kernels"}
     js.close()
     del js
     torch.cuda.empty_cache()

@@ -430,14 +430,21 @@ int ysql_star_join_group_sum(const ysql_star_dim *dims, int ndims, const int32_t
 
 /* ======================================================================
  * Batched ragged jobs (no reference counterpart: the reference launched
- * every operator of every batch separately, one after another).  Thousands
- * of independent sort / filter / aggregate jobs are split over K chains
- * (largest job first onto the least-loaded chain), each chain with its own
- * stream and workspace, and every chain's launches are captured once into
- * one hipGraph; ysql_batch_run replays it with one host call and the chains
- * overlap on the device.  ysql_batch_run_plain issues the same per-call
- * entry points job after job on one stream.  Both run the same kernels in
- * the same order per job, so they produce the same bits.
+ * every operator of every batch separately, one after another).
+ * ysql_batch_run is ONE persistent launch (captured with its reset kernel
+ * in a hipGraph): a fixed grid of workgroups takes tasks from device-side
+ * job lists and a phase queue until every job is done.  Filters,
+ * aggregates and sorts up to persist_split_min rows run whole in one
+ * workgroup; larger sorts run as digit-count and scatter phases over chunks
+ * of persist_chunk rows, each phase pushed by the workgroup that completes
+ * the previous one, so no workgroup ever waits for another.
+ * ysql_batch_run_graph replays another form: the jobs split over K launch
+ * chains (largest first onto the least-loaded chain) beside segmented
+ * one-workgroup-per-job kernels, captured as one hipGraph.
+ * ysql_batch_run_plain issues the per-call entry points job after job on one
+ * stream.  Sorts and compactions are stable (unique results) and every
+ * aggregate folds the same canonical tiles in the same order, so all three
+ * produce the same bits.
  *   SORT:   stable argsort of keys (UKEY32/UKEY64): keys_out = sorted keys,
  *           perm_out[pos] = row.
  *   FILTER: rows with vals[i] < threshold; keys_out / vals_out = selected
@@ -445,9 +452,10 @@ int ysql_star_join_group_sum(const ysql_star_dim *dims, int ndims, const int32_t
  *   AGG:    rows with vals[i] < threshold; accs_out[0] = SUM(vals) (its
  *           count is COUNT, d / count is AVG), accs_out[1] = MIN(keys),
  *           accs_out[2] = MAX(keys) (keys read as signed INT / DATETIME).
- * The batch object owns the queue, job table and every workspace (device
+ * The batch object owns the queues, job tables and every workspace (device
  * memory is allocated by ysql_batch_create only); inputs and outputs stay
- * the caller's and must outlive the batch.                                 */
+ * the caller's and must outlive the batch.  Runs of one batch must not
+ * overlap in time.                                                        */
 enum { YSQL_JOB_SORT = 0, YSQL_JOB_FILTER = 1, YSQL_JOB_AGG = 2 };
 typedef struct ysql_job {
   int kind;
@@ -475,14 +483,29 @@ typedef struct ysql_batch_tuning {
    * key width (0: UKEY32, 1: UKEY64) and size bucket floor(log2(n));
    * -1 = the library default.  Filled by an autotuner. */
   signed char sort_variant[2][32];
+  /* persistent launch (ysql_batch_run) */
+  int persist_split_min;        /* sorts above this many rows run as chunk phases (default 2^18) */
+  int persist_chunk;            /* rows per chunk, rounded up to 4096 (default 2^16) */
+  int persist_jobs_first_every; /* every k-th worker takes whole jobs before phase tasks
+                                   (default 4; 0 = phase tasks first everywhere) */
+  int persist_idle_limit;       /* empty rounds (~1 us each) before an idle worker leaves */
+  int persist_workers_per_cu;   /* 0 = as many as fit (occupancy) */
 } ysql_batch_tuning;
 void ysql_batch_tuning_default(ysql_batch_tuning *t);
 /* Validates the jobs, allocates every workspace and captures the graph.
  * Inputs and outputs must not move while the batch exists. */
 int ysql_batch_create(const ysql_job *jobs, int njobs, const ysql_batch_tuning *tuning,
                       ysql_batch **out);
-/* Replays the captured graph on `stream` (one host call). */
+/* The persistent launch on `stream` (one host call, one graph replay). */
 int ysql_batch_run(ysql_batch *b, ysql_stream_t stream);
+/* The launch-chain graph on `stream`. */
+int ysql_batch_run_graph(ysql_batch *b, ysql_stream_t stream);
+/* After a ysql_batch_run on `stream`: synchronises it and reports how many
+ * jobs with rows the persistent launch completed, of how many
+ * (*jobs_done == *jobs_total when it finished them all). */
+int ysql_batch_check(ysql_batch *b, ysql_stream_t stream, uint32_t *jobs_done, uint32_t *jobs_total);
+/* Workgroups of the persistent launch. */
+int ysql_batch_workers(const ysql_batch *b, uint32_t *workers);
 /* The same jobs as a per-job launch loop on `stream`. */
 int ysql_batch_run_plain(ysql_batch *b, ysql_stream_t stream);
 /* Job count, chain count and workspace bytes of a batch. */

@@ -1,6 +1,7 @@
-"""Batched executor: the same ragged jobs replayed as one captured graph of
-parallel launch chains and run as the per-job launch loop give identical
-bits, every job matches the oracle, and graph replays repeat the bits."""
+"""Batched executor, launch-chain graph form: the same ragged jobs replayed
+as one captured graph of parallel launch chains and run as the per-job
+launch loop give identical bits, every job matches the oracle, and graph
+replays repeat the bits."""
 import math
 
 import numpy as np
@@ -37,7 +38,7 @@ def test_graph_batch_equals_plain_bitwise(jobset):
     jobset.run_plain()
     plain = jobset.snapshot()
     jobset.clear_outputs()
-    jobset.run_batched()
+    jobset.run_graph()
     batched = jobset.snapshot()
     for a, b in zip(plain, batched):
         assert np.array_equal(a, b)
@@ -48,7 +49,7 @@ def test_graph_batch_equals_plain_bitwise(jobset):
         jobset.run_plain()
         jobset.snapshot()
         jobset.clear_outputs()
-        jobset.run_batched()
+        jobset.run_graph()
         again = jobset.snapshot()
         for a, b in zip(batched, again):
             assert np.array_equal(a, b)
@@ -86,7 +87,7 @@ def check_jobs_against_oracle(jobset, jobs=None):
 
 def test_graph_batch_jobs_match_oracle(jobset):
     jobset.clear_outputs()
-    jobset.run_batched()
+    jobset.run_graph()
     check_jobs_against_oracle(jobset)
 
 
@@ -114,7 +115,7 @@ def test_segmented_and_chained_paths_same_bits(dev, seg_max):
         js.run_plain()
         plain = js.snapshot()
         js.clear_outputs()
-        js.run_batched()
+        js.run_graph()
         for a, b in zip(plain, js.snapshot()):
             assert np.array_equal(a, b)
         check_jobs_against_oracle(js)
@@ -132,7 +133,7 @@ def test_single_chain_equals_plain(dev):
         js.run_plain()
         plain = js.snapshot()
         js.clear_outputs()
-        js.run_batched()
+        js.run_graph()
         for a, b in zip(plain, js.snapshot()):
             assert np.array_equal(a, b)
     finally:

@@ -9,8 +9,8 @@
   equal one global stable sort.
 * Star join + GROUP BY over the 600M-row fact table against the oracle's
   restatement, every group's sum and count.
-* The 4,096 ragged jobs: the batched run equals the per-job loop on every
-  output buffer, and a seeded sample of jobs from every kind and size bucket
+* The 4,096 ragged jobs: the persistent launch and the launch-chain graph
+  each equal the per-job loop on every output buffer, and a seeded sample of jobs from every kind and size bucket
   (the largest job of each kind included) equals the oracle.
 """
 import numpy as np
@@ -131,6 +131,12 @@ def test_batched_4096_jobs(dev):
         plain = [t.clone() for t in outs]
         js.clear_outputs()
         js.run_batched()
+        done, total = js.check()
+        assert done == total == int(np.count_nonzero(sizes))
+        for a, b in zip(plain, outs):
+            assert torch.equal(a, b)
+        js.clear_outputs()
+        js.run_graph()
         torch.cuda.synchronize()
         for a, b in zip(plain, outs):
             assert torch.equal(a, b)

@@ -161,7 +161,9 @@ FA_OVERWRITE = 1
 
 class BatchTuning(C.Structure):
     _fields_ = [("chains", C.c_int), ("seg_sort_max", C.c_int), ("seg_filter_max", C.c_int), ("seg_agg_max", C.c_int),
-                ("sort_variant", (C.c_int8 * 32) * 2)]
+                ("sort_variant", (C.c_int8 * 32) * 2), ("persist_split_min", C.c_int), ("persist_chunk", C.c_int),
+                ("persist_jobs_first_every", C.c_int), ("persist_idle_limit", C.c_int),
+                ("persist_workers_per_cu", C.c_int)]
 
 
 JOB_SORT, JOB_FILTER, JOB_AGG = 0, 1, 2
@@ -178,6 +180,9 @@ for _name, _args, _res in [
     ("ysql_batch_create", [C.POINTER(Job), I32, C.POINTER(BatchTuning), C.POINTER(P)], C.c_int),
     ("ysql_batch_run", [P, P], C.c_int),
     ("ysql_batch_run_plain", [P, P], C.c_int),
+    ("ysql_batch_run_graph", [P, P], C.c_int),
+    ("ysql_batch_check", [P, P, C.POINTER(U32), C.POINTER(U32)], C.c_int),
+    ("ysql_batch_workers", [P, C.POINTER(U32)], C.c_int),
     ("ysql_batch_info", [P, C.POINTER(U64), C.POINTER(U32), C.POINTER(U64)], C.c_int),
     ("ysql_batch_destroy", [P], None),
 ]:

@@ -2,9 +2,10 @@
 
 A JobSet holds the inputs and outputs of many independent sort / filter /
 aggregate jobs in a few large device buffers (one slice per job) and the
-ysql_job table that points into them.  It runs the same jobs either as the
-captured graph of parallel launch chains (run_batched) or as the per-job
-launch loop on one stream (run_plain).
+ysql_job table that points into them.  It runs the same jobs as one persistent
+launch with a device work queue (run_batched), as the captured graph of
+parallel launch chains (run_graph), or as the per-job launch loop on one
+stream (run_plain).
 """
 import ctypes as C
 
@@ -78,16 +79,20 @@ class JobSet:
         self.handle = None
         self.total_rows = sum(self.sizes)
 
-    def create(self, chains=None, sort_variants=None, seg_max=None):
+    def create(self, chains=None, sort_variants=None, seg_max=None, **persist):
         """sort_variants: {32: [variant per log2 bucket], 64: [...]} (an
         autotuner's table); seg_max: rows up to which a job runs in one
-        workgroup of a segmented kernel (0 = never), one value for every kind
-        or a (sort, filter, aggregate) triple; None keeps the library
-        defaults."""
+        workgroup of a segmented kernel of the launch-chain graph (0 =
+        never), one value for every kind or a (sort, filter, aggregate)
+        triple; persist: fields of the persistent launch's tuning
+        (split_min, chunk, jobs_first_every, idle_limit, workers_per_cu).
+        None keeps the library defaults."""
         tuning = None
-        if chains is not None or sort_variants is not None or seg_max is not None:
+        if chains is not None or sort_variants is not None or seg_max is not None or persist:
             tuning = L.BatchTuning()
             L.lib.ysql_batch_tuning_default(C.byref(tuning))
+            for k, v in persist.items():
+                setattr(tuning, "persist_" + k, int(v))
             if chains is not None:
                 tuning.chains = int(chains)
             if seg_max is not None:
@@ -109,7 +114,25 @@ class JobSet:
         return {"jobs": t.value, "chains": gsz.value, "ws_bytes": ws.value}
 
     def run_batched(self):
+        """The persistent launch (one kernel, device work queue)."""
         L.call("ysql_batch_run", self.handle, C.c_void_p(torch.cuda.current_stream().cuda_stream))
+
+    def run_graph(self):
+        """The launch-chain graph beside the segmented kernels."""
+        L.call("ysql_batch_run_graph", self.handle, C.c_void_p(torch.cuda.current_stream().cuda_stream))
+
+    def check(self):
+        """(jobs completed, jobs with rows) of the last persistent launch
+        (synchronises the current stream)."""
+        d, t = C.c_uint32(), C.c_uint32()
+        L.call("ysql_batch_check", self.handle, C.c_void_p(torch.cuda.current_stream().cuda_stream), C.byref(d),
+               C.byref(t))
+        return d.value, t.value
+
+    def workers(self):
+        w = C.c_uint32()
+        L.call("ysql_batch_workers", self.handle, C.byref(w))
+        return w.value
 
     def run_plain(self):
         L.call("ysql_batch_run_plain", self.handle, C.c_void_p(torch.cuda.current_stream().cuda_stream))

@@ -72,9 +72,9 @@ __global__ __launch_bounds__(kSegSortBlock) void k_seg_sort(const SegSort *jobs)
                                                j.valt, j.n, 0, (int)sizeof(K), 0u);
 }
 
-__global__ __launch_bounds__(kSegFiltBlock) void k_seg_filter(const SegFilter *jobs) {
-  __shared__ CompactSmem<kSegFiltBlock, 4> sm;
-  const SegFilter j = jobs[blockIdx.x];
+// One filter job by one workgroup: its tiles compacted in order with a
+// running prefix (no look-back).  Every thread of the block calls it.
+__device__ __forceinline__ void seg_filter_job(CompactSmem<kSegFiltBlock, 4> &sm, const SegFilter &j) {
   if (j.n == 0) {
     if (threadIdx.x == 0) *j.count_out = 0;
     return;
@@ -88,6 +88,11 @@ __global__ __launch_bounds__(kSegFiltBlock) void k_seg_filter(const SegFilter *j
                                                                ~0ull, running, j.count_out, nullptr, ntiles);
 }
 
+__global__ __launch_bounds__(kSegFiltBlock) void k_seg_filter(const SegFilter *jobs) {
+  __shared__ CompactSmem<kSegFiltBlock, 4> sm;
+  seg_filter_job(sm, jobs[blockIdx.x]);
+}
+
 // SUM(vals), MIN(keys), MAX(keys) WHERE vals < thr: the specs
 // ysql_filter_aggregate_ex builds for the same job.
 __device__ __forceinline__ void seg_agg_specs(const SegAgg &j, FaSpec (&sp)[3]) {
@@ -97,9 +102,9 @@ __device__ __forceinline__ void seg_agg_specs(const SegAgg &j, FaSpec (&sp)[3]) 
   sp[2] = FaSpec{FA_KI, 2, kdt, FA_SPEC0 + 1, j.keys};
 }
 
-__global__ __launch_bounds__(kFaBlock) void k_seg_agg(const SegAgg *jobs) {
-  __shared__ ysql_acc s_w[kFaBlock / kWave];
-  const SegAgg j = jobs[blockIdx.x];
+// One aggregate job by one workgroup: its canonical tiles folded in order,
+// then combined like the per-call kernel's last block.
+__device__ __forceinline__ void seg_agg_job(ysql_acc *s_w, const SegAgg &j) {
   FaSpec sp[3];
   seg_agg_specs(j, sp);
   ysql_acc *const out[3] = {j.accs_out, j.accs_out + 1, j.accs_out + 2};
@@ -119,6 +124,300 @@ __global__ __launch_bounds__(kFaBlock) void k_seg_agg(const SegAgg *jobs) {
   __threadfence();
   __syncthreads();
   filter_agg_finish<kFaBlock>(sp, 3, 1, ntiles, j.partials, out, true, s_w);
+}
+
+__global__ __launch_bounds__(kFaBlock) void k_seg_agg(const SegAgg *jobs) {
+  __shared__ ysql_acc s_w[kFaBlock / kWave];
+  seg_agg_job(s_w, jobs[blockIdx.x]);
+}
+
+// ------------------------------------------------------------- persistent
+// One persistent launch for the whole job set.  Workgroups loop taking
+// tasks from two device-side sources until every job is done:
+//   * two job lists (cursors advanced with atomics): split sorts, then the
+//     whole jobs largest first;
+//   * a phase queue: entries (job, phase, ntasks) with an atomic task
+//     counter; a workgroup takes one task (chunk) of the oldest entry that
+//     still has untaken tasks.
+// A whole job (filter, aggregate, or a sort up to the split size) is run by
+// the workgroup that takes it, with the segmented bodies above.  A split
+// sort runs as phases over chunks of `chunk` rows: U_p (each chunk's counts
+// of digit p) then D_p (each chunk's tiles scattered in order from running
+// per-digit offsets); the workgroup that completes the last task of a phase
+// (the tasks' done counter) computes what the next phase needs (after U_p:
+// every chunk's digit offsets, an exclusive scan over chunks) and pushes the
+// next phase's entry.  No workgroup ever waits for another: a task reads
+// only data whose producers completed before its entry was pushed.  A worker
+// that finds nothing to take re-checks for a bounded number of rounds (a
+// running task may push new work) and leaves; leaving is always safe,
+// because the worker that completes a phase pushes its successor and then
+// keeps taking tasks itself.  Release fences before every completion count
+// and an acquire fence at the start of every task carry the data between
+// workgroups.
+constexpr int kPB = 256;            // threads per worker (4 waves)
+constexpr uint32_t kPTile = kPB * 16;  // sort tile of the split phases (as the segmented sort)
+
+enum { PJ_SORT = 0, PJ_SORT_SPLIT = 1, PJ_FILTER = 2, PJ_AGG = 3 };
+enum { PT_NONE = 0, PT_JOB = 1, PT_PHASE = 2, PT_EXIT = 3 };
+// control words, 64 words (256 bytes) apart
+enum { PC_CUR_SPLIT = 0, PC_CUR_WHOLE = 64, PC_HEAD = 128, PC_TAIL = 192, PC_DONE = 256, PC_ERR = 320, PC_WORDS = 384 };
+
+struct PJob {
+  int type, key64;
+  uint32_t n, nchunks;
+  const void *keys;
+  void *keys_out;
+  uint32_t *perm_out;
+  void *kalt;
+  uint32_t *valt;
+  uint32_t *counts;  // split sort: nchunks * 256 words (counts, then offsets)
+  SegFilter f;
+  SegAgg a;
+};
+
+struct PEntry {
+  uint64_t desc;  // job << 32 | phase << 24 | ntasks; 0 while not yet pushed
+  uint32_t taken, done;
+};
+
+struct PArgs {
+  const PJob *jobs;
+  const uint32_t *split_order, *whole_order;
+  uint32_t nsplit, nwhole, total;  // total: jobs with rows
+  PEntry *entries;
+  uint32_t cap_entries;
+  uint32_t *ctl;
+  uint32_t chunk;          // rows per chunk, a multiple of kPTile
+  uint32_t jobs_first_every;  // every k-th worker takes whole jobs before phases (0: none)
+  uint32_t idle_limit;     // empty rounds before a worker leaves
+};
+
+union PSmem {
+  SegSortSmem<uint64_t, kPB, 16> s64;
+  SegSortSmem<uint32_t, kPB, 16> s32;
+  CompactSmem<kSegFiltBlock, 4> cs;
+  ysql_acc fa[kFaBlock / kWave];
+};
+
+__device__ __forceinline__ uint32_t pld(const uint32_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t pld64(const uint64_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Thread 0: the next index of a job list, or false.
+__device__ __forceinline__ bool ptake_job(uint32_t *cur, uint32_t count, const uint32_t *order, uint32_t *job) {
+  if (pld(cur) >= count) return false;
+  const uint32_t i = atomicAdd(cur, 1u);
+  if (i >= count) return false;
+  *job = order[i];
+  return true;
+}
+
+// Thread 0: one task of the oldest phase entry with untaken tasks, or false.
+__device__ __forceinline__ bool ptake_phase(const PArgs &a, uint32_t *entry, uint32_t *task) {
+  uint32_t tail = pld(&a.ctl[PC_TAIL]);
+  if (tail > a.cap_entries) tail = a.cap_entries;
+  uint32_t h = pld(&a.ctl[PC_HEAD]);
+  for (uint32_t e = h; e < tail; e++) {
+    PEntry *E = &a.entries[e];
+    const uint64_t d = pld64(&E->desc);
+    if (d == 0) continue;  // reserved, not yet written
+    const uint32_t nt = (uint32_t)d & 0xFFFFFFu;
+    if (pld(&E->taken) < nt) {
+      const uint32_t t = atomicAdd(&E->taken, 1u);
+      if (t < nt) {
+        *entry = e;
+        *task = t;
+        return true;
+      }
+    }
+    if (e == h) {  // every task of the head entry is taken: move the head on
+      atomicCAS(&a.ctl[PC_HEAD], h, h + 1);
+      h++;
+    }
+  }
+  return false;
+}
+
+// Thread 0 (after the workgroup's writes are fenced): appends a phase entry.
+__device__ __forceinline__ void ppush(const PArgs &a, uint32_t job, uint32_t phase, uint32_t ntasks) {
+  const uint32_t e = atomicAdd(&a.ctl[PC_TAIL], 1u);
+  if (e >= a.cap_entries) {  // cannot happen with the capacity ysql_batch_create sizes
+    atomicOr(&a.ctl[PC_ERR], 1u);
+    return;
+  }
+  const uint64_t d = ((uint64_t)job << 32) | ((uint64_t)phase << 24) | ntasks;
+  __hip_atomic_store(&a.entries[e].desc, d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void pjob_done(const PArgs &a) {
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(&a.ctl[PC_DONE], 1u);
+}
+
+// After U_p: counts[c][d] -> the global start of digit d's keys of chunk c
+// (digit totals' exclusive scan plus the counts of earlier chunks).  Thread
+// d walks the chunks 16 at a time with all 16 loads issued together.
+__device__ __forceinline__ void psplit_offsets(uint32_t *counts, uint32_t nch, uint32_t *red) {
+  constexpr uint32_t B = 16;
+  const int t = threadIdx.x;
+  uint32_t tot = 0;
+  if (t < kBins) {
+    for (uint32_t c = 0; c < nch; c += B) {
+      uint32_t v[B];
+#pragma unroll
+      for (uint32_t u = 0; u < B; u++) v[u] = c + u < nch ? counts[(c + u) * kBins + t] : 0u;
+#pragma unroll
+      for (uint32_t u = 0; u < B; u++) tot += v[u];
+    }
+  }
+  uint32_t all;
+  uint32_t run = block_exclusive_sum<kPB, uint32_t>(t < kBins ? tot : 0u, red, &all);
+  if (t < kBins) {
+    for (uint32_t c = 0; c < nch; c += B) {
+      uint32_t v[B];
+#pragma unroll
+      for (uint32_t u = 0; u < B; u++) v[u] = c + u < nch ? counts[(c + u) * kBins + t] : 0u;
+#pragma unroll
+      for (uint32_t u = 0; u < B; u++) {
+        if (c + u < nch) counts[(c + u) * kBins + t] = run;
+        run += v[u];
+      }
+    }
+  }
+}
+
+template <typename K>
+__device__ __noinline__ void psplit_task(const PArgs &a, SegSortSmem<K, kPB, 16> &sm, const PJob &j,
+                                            uint32_t job, uint32_t phase, uint32_t entry, uint32_t c,
+                                            uint32_t *s_flag) {
+  const int passes = (int)sizeof(K);
+  const int p = (int)(phase >> 1);
+  const bool scatter = phase & 1u;
+  const PassBufs<K> b = pass_bufs<K>(p, passes, (const K *)j.keys, (K *)j.keys_out, j.perm_out, (K *)j.kalt, j.valt);
+  const uint64_t r0 = (uint64_t)c * a.chunk;
+  const uint64_t r1 = r0 + a.chunk < j.n ? r0 + a.chunk : j.n;
+  uint32_t *cnt = j.counts + (uint64_t)c * kBins;
+  if (!scatter) {
+    digit_counts_range<K>(b.src, r0, r1, 8 * p, 0u, sm.h, cnt);
+  } else {
+    if (threadIdx.x < kBins) sm.h[threadIdx.x] = cnt[threadIdx.x];
+    __syncthreads();
+    sort_pass_range<K, kPB, 16>(sm.os, sm.h, b.src, b.vsrc, 0u, b.dst, b.vdst, j.n, (uint32_t)(r0 / kPTile),
+                                (uint32_t)ceil_div(r1, (uint64_t)kPTile), 8 * p, 0u);
+  }
+  // completion: the last task of the phase starts the next one
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) *s_flag = atomicAdd(&a.entries[entry].done, 1u) == j.nchunks - 1 ? 1u : 0u;
+  __syncthreads();
+  if (!*s_flag) return;
+  __threadfence();
+  if (!scatter) {
+    psplit_offsets(j.counts, j.nchunks, sm.os.red);
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) ppush(a, job, phase + 1, j.nchunks);
+  } else if (p + 1 < passes) {
+    if (threadIdx.x == 0) ppush(a, job, phase + 1, j.nchunks);
+  } else {
+    pjob_done(a);
+  }
+}
+
+// The task bodies stay separate functions: each keeps its own register
+// allocation, so the kernel needs the largest body's registers, not more.
+template <typename K>
+__device__ __noinline__ void pwhole_sort(SegSortSmem<K, kPB, 16> &sm, const PJob &j) {
+  seg_sort_block<K, kPB, 16>(sm, (const K *)j.keys, 0u, (K *)j.keys_out, j.perm_out, (K *)j.kalt, j.valt, j.n, 0,
+                             (int)sizeof(K), 0u);
+}
+__device__ __noinline__ void pwhole_filter(CompactSmem<kSegFiltBlock, 4> &sm, const SegFilter &f) {
+  seg_filter_job(sm, f);
+}
+__device__ __noinline__ void pwhole_agg(ysql_acc *s_w, const SegAgg &g) { seg_agg_job(s_w, g); }
+
+__global__ __launch_bounds__(kPB, 3) void k_persist(const PArgs *__restrict__ pa) {
+  const PArgs &a = *pa;
+  __shared__ PSmem sm;
+  __shared__ uint32_t s_kind, s_x, s_y, s_flag;
+  const bool jobs_first = a.jobs_first_every && blockIdx.x % a.jobs_first_every == 0;
+  uint32_t idle = 0;
+  while (true) {
+    if (threadIdx.x == 0) {
+      uint32_t x = 0, y = 0, k = PT_NONE;
+      if (jobs_first && ptake_job(&a.ctl[PC_CUR_WHOLE], a.nwhole, a.whole_order, &x)) k = PT_JOB;
+      else if (ptake_phase(a, &x, &y)) k = PT_PHASE;
+      else if (ptake_job(&a.ctl[PC_CUR_SPLIT], a.nsplit, a.split_order, &x)) k = PT_JOB;
+      else if (ptake_job(&a.ctl[PC_CUR_WHOLE], a.nwhole, a.whole_order, &x)) k = PT_JOB;
+      else if (pld(&a.ctl[PC_DONE]) >= a.total) k = PT_EXIT;
+      s_kind = k;
+      s_x = x;
+      s_y = y;
+    }
+    __syncthreads();
+    const uint32_t kind = s_kind, x = s_x, y = s_y;
+    __syncthreads();
+    if (kind == PT_EXIT) return;
+    if (kind == PT_NONE) {
+      if (++idle > a.idle_limit) return;
+      __builtin_amdgcn_s_sleep(32);
+      continue;
+    }
+    idle = 0;
+    __threadfence();  // see what the producers of this task's inputs wrote
+    if (kind == PT_JOB) {
+      const PJob &j = a.jobs[x];
+      if (j.type == PJ_SORT_SPLIT) {
+        if (threadIdx.x == 0) ppush(a, x, 0u, j.nchunks);
+        __syncthreads();
+        continue;
+      }
+      if (j.type == PJ_SORT) {
+        if (j.key64) pwhole_sort<uint64_t>(sm.s64, j);
+        else pwhole_sort<uint32_t>(sm.s32, j);
+      } else if (j.type == PJ_FILTER) {
+        pwhole_filter(sm.cs, j.f);
+      } else {
+        pwhole_agg(sm.fa, j.a);
+      }
+      pjob_done(a);
+    } else {
+      const uint64_t d = pld64(&a.entries[x].desc);
+      const uint32_t job = (uint32_t)(d >> 32), phase = (uint32_t)(d >> 24) & 0xFFu;
+      const PJob &j = a.jobs[job];
+      if (j.key64) psplit_task<uint64_t>(a, sm.s64, j, job, phase, x, y, &s_flag);
+      else psplit_task<uint32_t>(a, sm.s32, j, job, phase, x, y, &s_flag);
+    }
+    __syncthreads();
+  }
+}
+
+// Per run, before k_persist: control words and phase entries zeroed, and
+// the outputs of jobs without rows set (filter count 0, aggregate identities).
+__global__ void k_persist_reset(uint32_t *ctl, PEntry *entries, uint32_t cap, const PJob *jobs, uint32_t njobs) {
+  const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = tid; i < PC_WORDS; i += stride) ctl[i] = 0;
+  for (uint64_t i = tid; i < cap; i += stride) entries[i] = PEntry{0, 0, 0};
+  for (uint64_t q = tid; q < njobs; q += stride) {
+    const PJob &j = jobs[q];
+    if (j.n) continue;
+    if (j.type == PJ_FILTER) {
+      *j.f.count_out = 0;
+    } else if (j.type == PJ_AGG) {
+      FaSpec sp[3];
+      seg_agg_specs(j.a, sp);
+      for (int s = 0; s < 3; s++) {
+        ysql_acc acc;
+        fa_identity(acc, sp[s]);
+        j.a.accs_out[s] = acc;
+      }
+    }
+  }
 }
 
 constexpr int kMaxChains = 64;
@@ -200,7 +499,157 @@ struct ysql_batch {
   std::vector<hipEvent_t> events;    // fork, then one join per chain and per segmented launch
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // persistent executor: one buffer holding the job table, the two job
+  // lists, the control words, the phase entries and every job's workspace
+  void *p_dev = nullptr;
+  size_t p_bytes = 0;
+  PArgs p_args{};
+  const PArgs *p_args_dev = nullptr;  // the same, in device memory (the kernel's argument)
+  uint32_t p_workers = 0;
+  hipStream_t p_stream = nullptr;
+  hipGraph_t p_graph = nullptr;
+  hipGraphExec_t p_exec = nullptr;
 };
+
+// Builds the persistent executor's tables and workspace (one allocation)
+// and captures reset + persistent launch into b->p_graph.
+static int persist_setup(ysql_batch *b) {
+  const ysql_batch_tuning &t = b->tune;
+  const int njobs = (int)b->jobs.size();
+  const uint32_t chunk = (uint32_t)t.persist_chunk;
+  const uint64_t split_min = (uint64_t)std::max(t.persist_split_min, 0);
+  std::vector<PJob> pj(njobs);
+  std::vector<uint32_t> split_order, whole_order;
+  std::vector<size_t> o_kalt(njobs, 0), o_valt(njobs, 0), o_cnt(njobs, 0), o_part(njobs, 0);
+  size_t wso = 0;  // workspace offsets, 256-byte pieces
+  auto take = [&](size_t bytes) {
+    const size_t o = wso;
+    wso += align_up(std::max<size_t>(bytes, 1), 256);
+    return o;
+  };
+  uint64_t cap = 0;
+  uint32_t total = 0;
+  for (int q = 0; q < njobs; q++) {
+    const ysql_job &x = b->jobs[q];
+    PJob &j = pj[q];
+    memset(&j, 0, sizeof(j));
+    j.key64 = x.key_dtype == YSQL_UKEY64;
+    j.n = (uint32_t)x.n;
+    j.keys = x.keys;
+    const size_t kb = j.key64 ? 8 : 4;
+    if (x.kind == YSQL_JOB_SORT) {
+      j.type = x.n > split_min ? PJ_SORT_SPLIT : PJ_SORT;
+      j.keys_out = x.keys_out;
+      j.perm_out = x.perm_out;
+      if (x.n) {
+        o_kalt[q] = take(x.n * kb);
+        o_valt[q] = take(x.n * 4);
+      }
+      if (j.type == PJ_SORT_SPLIT) {
+        j.nchunks = (uint32_t)ceil_div(x.n, (uint64_t)chunk);
+        o_cnt[q] = take((size_t)j.nchunks * kBins * 4);
+        cap += 2 * kb;  // U_p and D_p for every digit pass
+      }
+    } else if (x.kind == YSQL_JOB_FILTER) {
+      j.type = PJ_FILTER;
+      j.f = SegFilter{x.keys, x.vals, x.keys_out, x.vals_out, x.count_out, x.threshold, (uint32_t)x.n, (uint32_t)kb,
+                      0};
+    } else {
+      j.type = PJ_AGG;
+      if (x.n) o_part[q] = take((ceil_div(x.n, fa_tile_rows(x.n)) * 3 + 1) * sizeof(ysql_acc));
+      j.a = SegAgg{x.keys, x.vals, x.accs_out, nullptr, x.threshold, (uint32_t)x.n, j.key64 ? 1 : 0, 0};
+    }
+    if (x.n) {
+      (j.type == PJ_SORT_SPLIT ? split_order : whole_order).push_back((uint32_t)q);
+      total++;
+    }
+  }
+  std::stable_sort(split_order.begin(), split_order.end(),
+                   [&](uint32_t a, uint32_t c) { return b->jobs[a].n > b->jobs[c].n; });
+  std::stable_sort(whole_order.begin(), whole_order.end(),
+                   [&](uint32_t a, uint32_t c) { return job_cost(b->jobs[a]) > job_cost(b->jobs[c]); });
+  cap = std::max<uint64_t>(cap, 1);
+  // layout: jobs | split list | whole list | control words | entries | workspace
+  const size_t sz_jobs = align_up(std::max<size_t>(pj.size(), 1) * sizeof(PJob), 256);
+  const size_t sz_split = align_up(std::max<size_t>(split_order.size(), 1) * 4, 256);
+  const size_t sz_whole = align_up(std::max<size_t>(whole_order.size(), 1) * 4, 256);
+  const size_t sz_ctl = align_up(PC_WORDS * 4, 256);
+  const size_t sz_ent = align_up(cap * sizeof(PEntry), 256);
+  const size_t sz_args = align_up(sizeof(PArgs), 256);
+  const size_t head = sz_jobs + sz_split + sz_whole + sz_ctl + sz_ent + sz_args;
+  b->p_bytes = head + wso;
+  hipError_t e;
+  if ((e = hipMalloc(&b->p_dev, b->p_bytes)) != hipSuccess) {
+    set_last_error(hipGetErrorString(e));
+    return YSQL_EHIP;
+  }
+  char *base = (char *)b->p_dev;
+  char *ws = base + head;
+  for (int q = 0; q < njobs; q++) {
+    PJob &j = pj[q];
+    if (!j.n) continue;
+    if (j.type == PJ_SORT || j.type == PJ_SORT_SPLIT) {
+      j.kalt = ws + o_kalt[q];
+      j.valt = (uint32_t *)(ws + o_valt[q]);
+      if (j.type == PJ_SORT_SPLIT) j.counts = (uint32_t *)(ws + o_cnt[q]);
+    } else if (j.type == PJ_AGG) {
+      j.a.partials = (ysql_acc *)(ws + o_part[q]);
+    }
+  }
+  PArgs &a = b->p_args;
+  a.jobs = (const PJob *)base;
+  a.split_order = (const uint32_t *)(base + sz_jobs);
+  a.whole_order = (const uint32_t *)(base + sz_jobs + sz_split);
+  a.nsplit = (uint32_t)split_order.size();
+  a.nwhole = (uint32_t)whole_order.size();
+  a.total = total;
+  a.ctl = (uint32_t *)(base + sz_jobs + sz_split + sz_whole);
+  a.entries = (PEntry *)(base + sz_jobs + sz_split + sz_whole + sz_ctl);
+  a.cap_entries = (uint32_t)cap;
+  a.chunk = chunk;
+  a.jobs_first_every = (uint32_t)std::max(t.persist_jobs_first_every, 0);
+  a.idle_limit = (uint32_t)std::max(t.persist_idle_limit, 1);
+  b->p_args_dev = (const PArgs *)(base + sz_jobs + sz_split + sz_whole + sz_ctl + sz_ent);
+  if ((e = hipMemcpy((void *)b->p_args_dev, &a, sizeof(PArgs), hipMemcpyHostToDevice)) != hipSuccess ||
+      (njobs && (e = hipMemcpy(base, pj.data(), pj.size() * sizeof(PJob), hipMemcpyHostToDevice)) != hipSuccess) ||
+      (!split_order.empty() && (e = hipMemcpy(base + sz_jobs, split_order.data(), split_order.size() * 4,
+                                              hipMemcpyHostToDevice)) != hipSuccess) ||
+      (!whole_order.empty() && (e = hipMemcpy(base + sz_jobs + sz_split, whole_order.data(), whole_order.size() * 4,
+                                              hipMemcpyHostToDevice)) != hipSuccess)) {
+    set_last_error(hipGetErrorString(e));
+    return YSQL_EHIP;
+  }
+  // workers: as many as can be resident at once (none of them ever waits
+  // for another, so more would only queue behind the resident ones)
+  int per_cu = t.persist_workers_per_cu;
+  if (per_cu <= 0) {
+    if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_persist, kPB, 0)) != hipSuccess) {
+      set_last_error(hipGetErrorString(e));
+      return YSQL_EHIP;
+    }
+  }
+  per_cu = std::max(1, per_cu);
+  b->p_workers = (uint32_t)(per_cu * grid_cap());
+  if ((e = hipStreamCreateWithFlags(&b->p_stream, hipStreamNonBlocking)) != hipSuccess ||
+      (e = hipStreamBeginCapture(b->p_stream, hipStreamCaptureModeThreadLocal)) != hipSuccess) {
+    set_last_error(hipGetErrorString(e));
+    return YSQL_EHIP;
+  }
+  {
+    const uint64_t words = std::max<uint64_t>(std::max<uint64_t>(cap, PC_WORDS), (uint64_t)njobs);
+    const unsigned g = (unsigned)std::min<uint64_t>(std::max<uint64_t>(ceil_div(words, 256), 1), 1024);
+    k_persist_reset<<<g, 256, 0, b->p_stream>>>(a.ctl, a.entries, a.cap_entries, a.jobs, (uint32_t)njobs);
+    if (total) k_persist<<<b->p_workers, kPB, 0, b->p_stream>>>(b->p_args_dev);
+  }
+  hipGraph_t g = nullptr;
+  e = hipStreamEndCapture(b->p_stream, &g);
+  b->p_graph = g;
+  if (e != hipSuccess || (e = hipGraphInstantiate(&b->p_exec, b->p_graph, nullptr, nullptr, 0)) != hipSuccess) {
+    set_last_error(hipGetErrorString(e));
+    return YSQL_EHIP;
+  }
+  return YSQL_OK;
+}
 
 extern "C" {
 
@@ -215,6 +664,11 @@ void ysql_batch_tuning_default(ysql_batch_tuning *t) {
   t->seg_filter_max = 1 << 21;
   t->seg_agg_max = 1 << 22;
   memset(t->sort_variant, -1, sizeof(t->sort_variant));
+  t->persist_split_min = 1 << 18;
+  t->persist_chunk = 1 << 16;
+  t->persist_jobs_first_every = 4;
+  t->persist_idle_limit = 1 << 15;
+  t->persist_workers_per_cu = 0;
 }
 
 void ysql_batch_destroy(ysql_batch *b) {
@@ -227,6 +681,10 @@ void ysql_batch_destroy(ysql_batch *b) {
   (void)hipFree(b->plain_ws);
   (void)hipFree(b->chain_ws);
   (void)hipFree(b->seg_dev);
+  if (b->p_exec) (void)hipGraphExecDestroy(b->p_exec);
+  if (b->p_graph) (void)hipGraphDestroy(b->p_graph);
+  if (b->p_stream) (void)hipStreamDestroy(b->p_stream);
+  (void)hipFree(b->p_dev);
   delete b;
 }
 
@@ -234,7 +692,7 @@ int ysql_batch_info(const ysql_batch *b, uint64_t *tasks, uint32_t *grid, uint64
   if (!b) return YSQL_EINVAL;
   if (tasks) *tasks = b->jobs.size();
   if (grid) *grid = (uint32_t)b->nchains;
-  if (ws_bytes) *ws_bytes = b->chain_ws_bytes + b->plain_ws_bytes + b->seg_bytes;
+  if (ws_bytes) *ws_bytes = b->chain_ws_bytes + b->plain_ws_bytes + b->seg_bytes + b->p_bytes;
   return YSQL_OK;
 }
 
@@ -245,6 +703,9 @@ int ysql_batch_create(const ysql_job *jobs, int njobs, const ysql_batch_tuning *
   ysql_batch_tuning_default(&tune);
   if (tuning) tune = *tuning;
   tune.chains = std::max(1, std::min(tune.chains, kMaxChains));
+  // split-sort chunks are whole tiles of the segmented sort
+  tune.persist_chunk = (int)std::min<uint64_t>(
+      align_up((uint64_t)std::max(tune.persist_chunk, (int)kPTile), (uint64_t)kPTile), 1ull << 28);
   for (auto &row : tune.sort_variant)
     for (auto &v : row)
       if (v >= ysql_radix_variant_count()) return YSQL_EINVAL;
@@ -435,13 +896,45 @@ int ysql_batch_create(const ysql_job *jobs, int njobs, const ysql_batch_tuning *
   }
   if (!ok || e != hipSuccess) return fail(e != hipSuccess ? e : hipErrorUnknown);
   if ((e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0)) != hipSuccess) return fail(e);
+  if ((rc = persist_setup(b)) != YSQL_OK) {
+    ysql_batch_destroy(b);
+    return rc;
+  }
   *out = b;
   return YSQL_OK;
 }
 
 int ysql_batch_run(ysql_batch *b, ysql_stream_t stream) {
   if (!b) return YSQL_EINVAL;
+  YSQL_HIP_TRY(hipGraphLaunch(b->p_exec, as_stream(stream)));
+  return YSQL_OK;
+}
+
+int ysql_batch_run_graph(ysql_batch *b, ysql_stream_t stream) {
+  if (!b) return YSQL_EINVAL;
   YSQL_HIP_TRY(hipGraphLaunch(b->exec, as_stream(stream)));
+  return YSQL_OK;
+}
+
+int ysql_batch_check(ysql_batch *b, ysql_stream_t stream, uint32_t *jobs_done, uint32_t *jobs_total) {
+  if (!b || !jobs_done || !jobs_total) return YSQL_EINVAL;
+  uint32_t w[2] = {0, 0};
+  hipStream_t s = as_stream(stream);
+  YSQL_HIP_TRY(hipMemcpyAsync(&w[0], b->p_args.ctl + PC_DONE, 4, hipMemcpyDeviceToHost, s));
+  YSQL_HIP_TRY(hipMemcpyAsync(&w[1], b->p_args.ctl + PC_ERR, 4, hipMemcpyDeviceToHost, s));
+  YSQL_HIP_TRY(hipStreamSynchronize(s));
+  *jobs_done = w[0];
+  *jobs_total = b->p_args.total;
+  if (w[1]) {
+    set_last_error("ysql_batch_check: the phase queue overflowed");
+    return YSQL_EOVERFLOW;
+  }
+  return YSQL_OK;
+}
+
+int ysql_batch_workers(const ysql_batch *b, uint32_t *workers) {
+  if (!b || !workers) return YSQL_EINVAL;
+  *workers = b->p_workers;
   return YSQL_OK;
 }
 

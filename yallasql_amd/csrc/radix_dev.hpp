@@ -50,13 +50,12 @@ __device__ __forceinline__ void hist_key(K k, int begin_bit, int passes, uint32_
   }
 }
 
-// All threads of the block: histograms of keys[begin, end) (s_h: 8 * 256
-// words of LDS) added into hist (passes * 256 words).  Contains barriers.
+// All threads of the block: the histograms of `passes` digit positions of
+// keys[begin, end) added into s_h (passes * 256 words of LDS, zeroed by the
+// caller).  Ends with a barrier.
 template <typename K>
-__device__ __forceinline__ void hist_block(const K *keys, uint64_t begin, uint64_t end, int begin_bit, int passes,
-                                           uint32_t desc, uint32_t *s_h, uint32_t *hist) {
-  for (int i = threadIdx.x; i < passes * kBins; i += blockDim.x) s_h[i] = 0;
-  __syncthreads();
+__device__ __forceinline__ void hist_range_lds(const K *keys, uint64_t begin, uint64_t end, int begin_bit, int passes,
+                                               uint32_t desc, uint32_t *s_h) {
   uint32_t zc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   constexpr int PER = 16 / sizeof(K);  // keys per 16-byte load
   uint64_t i = begin;
@@ -101,6 +100,16 @@ __device__ __forceinline__ void hist_block(const K *keys, uint64_t begin, uint64
     }
   }
   __syncthreads();
+}
+
+// All threads of the block: histograms of keys[begin, end) (s_h: 8 * 256
+// words of LDS) added into hist (passes * 256 words).  Contains barriers.
+template <typename K>
+__device__ __forceinline__ void hist_block(const K *keys, uint64_t begin, uint64_t end, int begin_bit, int passes,
+                                           uint32_t desc, uint32_t *s_h, uint32_t *hist) {
+  for (int i = threadIdx.x; i < passes * kBins; i += blockDim.x) s_h[i] = 0;
+  __syncthreads();
+  hist_range_lds<K>(keys, begin, end, begin_bit, passes, desc, s_h);
   for (int b = threadIdx.x; b < passes * kBins; b += blockDim.x) {
     const uint32_t v = s_h[b];
     if (v) atomicAdd(&hist[b], v);
@@ -384,12 +393,133 @@ __device__ __forceinline__ void small_sort_block(OnesweepSmem<K, BLOCK, KPT> &sm
   }
 }
 
+// One digit pass over tiles [tile_begin, tile_end) of a sequence of n keys,
+// in tile order, by one workgroup: each tile is ranked and reordered in LDS
+// like a onesweep tile, but its digit offsets come from running per-digit
+// counters `gofs` (256 words of LDS, the global start of each digit's next
+// key, advanced tile by tile) instead of a look-back.  src/vsrc -> dst/vdst;
+// vsrc == nullptr means the payload is val_base + row.  Every thread of the
+// block calls it; contains barriers.
+template <typename K, int BLOCK, int KPT>
+__device__ __forceinline__ void sort_pass_range(OnesweepSmem<K, BLOCK, KPT> &os, uint32_t *gofs, const K *src,
+                                                const uint32_t *vsrc, uint32_t val_base, K *dst, uint32_t *vdst,
+                                                uint32_t n, uint32_t tile_begin, uint32_t tile_end, int shift,
+                                                uint32_t desc) {
+  constexpr int W = BLOCK / kWave;
+  constexpr uint32_t TILE = BLOCK * KPT;
+  const int t = threadIdx.x;
+  const int lane = lane_id();
+  const int wave = t / kWave;
+  uint32_t *s_v = reinterpret_cast<uint32_t *>(os.k);
+  for (uint32_t tile = tile_begin; tile < tile_end; tile++) {
+    for (int i = t; i < W * kBins; i += BLOCK) (&os.wh[0][0])[i] = 0;
+    __syncthreads();
+    const uint32_t base = tile * TILE;
+    const uint32_t wbase = base + (uint32_t)wave * KPT * kWave;
+    K key[KPT];
+    uint32_t val[KPT], rank[KPT];
+#pragma unroll
+    for (int i = 0; i < KPT; i++) {
+      const uint32_t idx = wbase + i * kWave + lane;
+      const bool valid = idx < n;
+      key[i] = valid ? src[idx] : (K)0;
+      val[i] = vsrc ? (valid ? vsrc[idx] : 0u) : val_base + idx;
+    }
+#pragma unroll
+    for (int i = 0; i < KPT; i++) {
+      const uint32_t idx = wbase + i * kWave + lane;
+      const bool valid = idx < n;
+      const uint32_t d = ((uint32_t)(key[i] >> shift) & 0xFFu) ^ desc;
+      uint64_t m = ballot(valid);
+#pragma unroll
+      for (int b = 0; b < 8; b++) {
+        const bool bit = (d >> b) & 1u;
+        const uint64_t bal = ballot(bit);
+        m &= bit ? bal : ~bal;
+      }
+      const uint32_t lower = popc_below(m);
+      uint32_t pre = 0;
+      if (valid) pre = os.wh[wave][d];
+      rank[i] = pre + lower;
+      if (valid && lower == 0) os.wh[wave][d] = pre + (uint32_t)__popcll(m);
+    }
+    __syncthreads();
+    uint32_t cnt = 0;
+    if (t < kBins) {
+      uint32_t run = 0;
+#pragma unroll
+      for (int w = 0; w < W; w++) {
+        const uint32_t c = os.wh[w][t];
+        os.wh[w][t] = run;
+        run += c;
+      }
+      cnt = run;
+    }
+    uint32_t tot;
+    const uint32_t dstart = block_exclusive_sum<BLOCK, uint32_t>(t < kBins ? cnt : 0u, os.red, &tot);
+    if (t < kBins) {
+      os.dstart[t] = dstart;
+      os.glob[t] = gofs[t] - dstart;  // wraps like the onesweep form; + j brings it back
+    }
+    __syncthreads();
+    uint32_t pos[KPT];
+#pragma unroll
+    for (int i = 0; i < KPT; i++) {
+      const uint32_t idx = wbase + i * kWave + lane;
+      pos[i] = 0xFFFFFFFFu;
+      if (idx < n) {
+        const uint32_t d = ((uint32_t)(key[i] >> shift) & 0xFFu) ^ desc;
+        pos[i] = os.dstart[d] + os.wh[wave][d] + rank[i];
+        os.k[pos[i]] = key[i];
+      }
+    }
+    __syncthreads();
+    const uint32_t valid_cnt = (n - base) < TILE ? (n - base) : TILE;
+    uint32_t opos[KPT];
+#pragma unroll
+    for (int q = 0; q < KPT; q++) {
+      const uint32_t j = t + q * BLOCK;
+      opos[q] = 0;
+      if (j < valid_cnt) {
+        const K kk = os.k[j];
+        const uint32_t d = ((uint32_t)(kk >> shift) & 0xFFu) ^ desc;
+        opos[q] = os.glob[d] + j;
+        dst[opos[q]] = kk;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < KPT; i++)
+      if (pos[i] != 0xFFFFFFFFu) s_v[pos[i]] = val[i];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < KPT; q++) {
+      const uint32_t j = t + q * BLOCK;
+      if (j < valid_cnt) vdst[opos[q]] = s_v[j];
+    }
+    if (t < kBins) gofs[t] += cnt;
+    __syncthreads();
+  }
+}
+
+// The 256 counts of digit (key >> shift) & 0xFF (^ desc) over keys[begin,
+// end), written (not added) to out[0..255].  s_h: 256 words of LDS.  Every
+// thread of the block calls it; contains barriers.
+template <typename K>
+__device__ __forceinline__ void digit_counts_range(const K *keys, uint64_t begin, uint64_t end, int shift,
+                                                   uint32_t desc, uint32_t *s_h, uint32_t *out) {
+  for (int i = threadIdx.x; i < kBins; i += blockDim.x) s_h[i] = 0;
+  __syncthreads();
+  hist_range_lds<K>(keys, begin, end, shift, 1, desc, s_h);
+  for (int b = threadIdx.x; b < kBins; b += blockDim.x) out[b] = s_h[b];
+  __syncthreads();
+}
+
 // One workgroup sorts a whole job of n keys (segmented batches: one job per
 // workgroup, no communication between workgroups).  Up to one tile it is
 // small_sort_block; beyond, it builds every digit's histogram in LDS with one
-// read, then runs the digit passes over the job's tiles in order: a tile is
-// ranked and reordered in LDS like a onesweep tile, but its digit offsets
-// come from running per-digit counters in LDS instead of a look-back.
+// read, then runs the digit passes over the job's tiles in order
+// (sort_pass_range, offsets from running per-digit counters in LDS).
 // Buffers alternate kin -> alt -> ... so that the last pass lands in
 // kout/vout (kin must not alias kout).  Stable: the same permutation as the
 // multi-workgroup sort.  Every thread of the block calls it.
@@ -399,11 +529,31 @@ struct SegSortSmem {
   uint32_t h[8 * kBins];  // per-digit-position histograms, then running offsets
 };
 
+// Source and destination of digit pass p of `passes` that alternate
+// kin -> alt -> ... and end in out (pass 0 reads kin with row payloads).
+template <typename K>
+struct PassBufs {
+  const K *src;
+  const uint32_t *vsrc;
+  K *dst;
+  uint32_t *vdst;
+};
+template <typename K>
+__device__ __forceinline__ PassBufs<K> pass_bufs(int p, int passes, const K *kin, K *kout, uint32_t *vout, K *kalt,
+                                                 uint32_t *valt) {
+  const bool to_out = ((passes - 1 - p) & 1) == 0;
+  PassBufs<K> b;
+  b.src = p == 0 ? kin : (((passes - p) & 1) == 0 ? kout : kalt);
+  b.vsrc = p == 0 ? nullptr : (((passes - p) & 1) == 0 ? vout : valt);
+  b.dst = to_out ? kout : kalt;
+  b.vdst = to_out ? vout : valt;
+  return b;
+}
+
 template <typename K, int BLOCK, int KPT>
 __device__ __forceinline__ void seg_sort_block(SegSortSmem<K, BLOCK, KPT> &sm, const K *kin, uint32_t val_base,
                                                K *kout, uint32_t *vout, K *kalt, uint32_t *valt, uint32_t n,
                                                int begin_bit, int passes, uint32_t desc) {
-  constexpr int W = BLOCK / kWave;
   constexpr uint32_t TILE = BLOCK * KPT;
   if (n <= TILE) {
     small_sort_block<K, BLOCK, KPT>(sm.os, kin, nullptr, val_base, kout, vout, n, begin_bit, passes, desc);
@@ -411,7 +561,6 @@ __device__ __forceinline__ void seg_sort_block(SegSortSmem<K, BLOCK, KPT> &sm, c
   }
   const int t = threadIdx.x;
   const int lane = lane_id();
-  const int wave = t / kWave;
   // ---- histograms of every digit position, one read of the keys
   for (int i = t; i < passes * kBins; i += BLOCK) sm.h[i] = 0;
   __syncthreads();
@@ -436,104 +585,10 @@ __device__ __forceinline__ void seg_sort_block(SegSortSmem<K, BLOCK, KPT> &sm, c
     __syncthreads();
   }
   const uint32_t ntiles = (n + TILE - 1) / TILE;
-  uint32_t *s_v = reinterpret_cast<uint32_t *>(sm.os.k);
   for (int p = 0; p < passes; p++) {
-    const int shift = begin_bit + 8 * p;
-    const bool to_out = ((passes - 1 - p) & 1) == 0;
-    const K *src = p == 0 ? kin : (((passes - p) & 1) == 0 ? kout : kalt);
-    const uint32_t *vsrc = p == 0 ? nullptr : (((passes - p) & 1) == 0 ? vout : valt);
-    K *dst = to_out ? kout : kalt;
-    uint32_t *vdst = to_out ? vout : valt;
-    uint32_t *gofs = sm.h + p * kBins;
-    for (uint32_t tile = 0; tile < ntiles; tile++) {
-      for (int i = t; i < W * kBins; i += BLOCK) (&sm.os.wh[0][0])[i] = 0;
-      __syncthreads();
-      const uint32_t base = tile * TILE;
-      const uint32_t wbase = base + (uint32_t)wave * KPT * kWave;
-      K key[KPT];
-      uint32_t val[KPT], rank[KPT];
-#pragma unroll
-      for (int i = 0; i < KPT; i++) {
-        const uint32_t idx = wbase + i * kWave + lane;
-        const bool valid = idx < n;
-        key[i] = valid ? src[idx] : (K)0;
-        val[i] = vsrc ? (valid ? vsrc[idx] : 0u) : val_base + idx;
-      }
-#pragma unroll
-      for (int i = 0; i < KPT; i++) {
-        const uint32_t idx = wbase + i * kWave + lane;
-        const bool valid = idx < n;
-        const uint32_t d = ((uint32_t)(key[i] >> shift) & 0xFFu) ^ desc;
-        uint64_t m = ballot(valid);
-#pragma unroll
-        for (int b = 0; b < 8; b++) {
-          const bool bit = (d >> b) & 1u;
-          const uint64_t bal = ballot(bit);
-          m &= bit ? bal : ~bal;
-        }
-        const uint32_t lower = popc_below(m);
-        uint32_t pre = 0;
-        if (valid) pre = sm.os.wh[wave][d];
-        rank[i] = pre + lower;
-        if (valid && lower == 0) sm.os.wh[wave][d] = pre + (uint32_t)__popcll(m);
-      }
-      __syncthreads();
-      uint32_t cnt = 0;
-      if (t < kBins) {
-        uint32_t run = 0;
-#pragma unroll
-        for (int w = 0; w < W; w++) {
-          const uint32_t c = sm.os.wh[w][t];
-          sm.os.wh[w][t] = run;
-          run += c;
-        }
-        cnt = run;
-      }
-      uint32_t tot;
-      const uint32_t dstart = block_exclusive_sum<BLOCK, uint32_t>(t < kBins ? cnt : 0u, sm.os.red, &tot);
-      if (t < kBins) {
-        sm.os.dstart[t] = dstart;
-        sm.os.glob[t] = gofs[t] - dstart;  // wraps like the onesweep form; + j brings it back
-      }
-      __syncthreads();
-      uint32_t pos[KPT];
-#pragma unroll
-      for (int i = 0; i < KPT; i++) {
-        const uint32_t idx = wbase + i * kWave + lane;
-        pos[i] = 0xFFFFFFFFu;
-        if (idx < n) {
-          const uint32_t d = ((uint32_t)(key[i] >> shift) & 0xFFu) ^ desc;
-          pos[i] = sm.os.dstart[d] + sm.os.wh[wave][d] + rank[i];
-          sm.os.k[pos[i]] = key[i];
-        }
-      }
-      __syncthreads();
-      const uint32_t valid_cnt = (n - base) < TILE ? (n - base) : TILE;
-      uint32_t opos[KPT];
-#pragma unroll
-      for (int q = 0; q < KPT; q++) {
-        const uint32_t j = t + q * BLOCK;
-        opos[q] = 0;
-        if (j < valid_cnt) {
-          const K kk = sm.os.k[j];
-          const uint32_t d = ((uint32_t)(kk >> shift) & 0xFFu) ^ desc;
-          opos[q] = sm.os.glob[d] + j;
-          dst[opos[q]] = kk;
-        }
-      }
-      __syncthreads();
-#pragma unroll
-      for (int i = 0; i < KPT; i++)
-        if (pos[i] != 0xFFFFFFFFu) s_v[pos[i]] = val[i];
-      __syncthreads();
-#pragma unroll
-      for (int q = 0; q < KPT; q++) {
-        const uint32_t j = t + q * BLOCK;
-        if (j < valid_cnt) vdst[opos[q]] = s_v[j];
-      }
-      if (t < kBins) gofs[t] += cnt;
-      __syncthreads();
-    }
+    const PassBufs<K> b = pass_bufs<K>(p, passes, kin, kout, vout, kalt, valt);
+    sort_pass_range<K, BLOCK, KPT>(sm.os, sm.h + p * kBins, b.src, b.vsrc, val_base, b.dst, b.vdst, n, 0, ntiles,
+                                   begin_bit + 8 * p, desc);
     // the next pass reads what this one wrote (other threads' stores, which
     // may have been cached before they were rewritten)
     __threadfence();
