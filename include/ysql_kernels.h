@@ -434,10 +434,12 @@ int ysql_star_join_group_sum(const ysql_star_dim *dims, int ndims, const int32_t
  * ysql_batch_run is ONE persistent launch (captured with its reset kernel
  * in a hipGraph): a fixed grid of workgroups takes tasks from device-side
  * job lists and a phase queue until every job is done.  Filters,
- * aggregates and sorts up to persist_split_min rows run whole in one
- * workgroup; larger sorts run as digit-count and scatter phases over chunks
- * of persist_chunk rows, each phase pushed by the workgroup that completes
- * the previous one, so no workgroup ever waits for another.
+ * aggregates and sorts up to their split sizes run whole in one workgroup;
+ * larger sorts run as digit-count and scatter phases over chunks of
+ * persist_chunk rows, larger filters as count and compaction phases, larger
+ * aggregates as chunk tasks folded by the last one; each phase is pushed by
+ * the workgroup that completes the previous one, so no workgroup ever waits
+ * for another.
  * ysql_batch_run_graph replays another form: the jobs split over K launch
  * chains (largest first onto the least-loaded chain) beside segmented
  * one-workgroup-per-job kernels, captured as one hipGraph.
@@ -490,6 +492,9 @@ typedef struct ysql_batch_tuning {
                                    (default 4; 0 = phase tasks first everywhere) */
   int persist_idle_limit;       /* empty rounds (~1 us each) before an idle worker leaves */
   int persist_workers_per_cu;   /* 0 = as many as fit (occupancy) */
+  int persist_split_filter_min; /* filters above this many rows run as count + compact chunk
+                                   phases (default 2^20) */
+  int persist_split_agg_min;    /* aggregates above this many rows run as chunk tasks (default 2^20) */
 } ysql_batch_tuning;
 void ysql_batch_tuning_default(ysql_batch_tuning *t);
 /* Validates the jobs, allocates every workspace and captures the graph.

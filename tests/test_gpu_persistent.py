@@ -1,6 +1,6 @@
 """Batched executor, persistent form: one launch whose workgroups take whole
 jobs from device-side job lists and chunk tasks of split sorts from a phase
-queue.  Every job completes (ysql_batch_check), the outputs equal the
+queue (split sorts, filters and aggregates).  Every job completes (ysql_batch_check), the outputs equal the
 per-job launch loop bit for bit and the oracle, across graph replays and
 across the splitting choices (every sort split, none split, one-tile
 chunks, workers that take whole jobs first or phase tasks first)."""
@@ -30,6 +30,9 @@ def _run_and_compare(js, replays=1):
 @pytest.mark.parametrize("tuning", [
     {},
     {"split_min": 0, "chunk": 4096},
+    {"split_min": 0, "split_filter_min": 0, "split_agg_min": 0, "chunk": 4096},
+    {"split_filter_min": 4096, "split_agg_min": 4096, "chunk": 8192, "jobs_first_every": 2},
+    {"split_filter_min": 100_000, "split_agg_min": 50_000, "chunk": 65536},
     {"split_min": 4096, "chunk": 8192, "jobs_first_every": 1},
     {"split_min": 1 << 30, "jobs_first_every": 0},
     {"split_min": 65536, "chunk": 65536, "jobs_first_every": 0, "workers_per_cu": 1},
@@ -82,7 +85,8 @@ def test_persistent_empty_and_tiny_jobs(dev):
     key64 = [False, True, False, True, False, True, False, True, True, False]
     thr = np.linspace(0.1, 0.9, len(sizes)).astype(np.float32)
     print("persistent batch: empty and tiny jobs", flush=True)
-    js = JobSet(sizes, kinds, key64, thr, dev, seed=5).create(split_min=1, chunk=4096)
+    js = JobSet(sizes, kinds, key64, thr, dev, seed=5).create(split_min=1, split_filter_min=1, split_agg_min=1,
+                                                              chunk=4096)
     try:
         _run_and_compare(js, replays=2)
         check_jobs_against_oracle(js)

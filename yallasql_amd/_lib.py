@@ -163,7 +163,8 @@ class BatchTuning(C.Structure):
     _fields_ = [("chains", C.c_int), ("seg_sort_max", C.c_int), ("seg_filter_max", C.c_int), ("seg_agg_max", C.c_int),
                 ("sort_variant", (C.c_int8 * 32) * 2), ("persist_split_min", C.c_int), ("persist_chunk", C.c_int),
                 ("persist_jobs_first_every", C.c_int), ("persist_idle_limit", C.c_int),
-                ("persist_workers_per_cu", C.c_int)]
+                ("persist_workers_per_cu", C.c_int), ("persist_split_filter_min", C.c_int),
+                ("persist_split_agg_min", C.c_int)]
 
 
 JOB_SORT, JOB_FILTER, JOB_AGG = 0, 1, 2

@@ -157,7 +157,7 @@ __global__ __launch_bounds__(kFaBlock) void k_seg_agg(const SegAgg *jobs) {
 constexpr int kPB = 256;            // threads per worker (4 waves)
 constexpr uint32_t kPTile = kPB * 16;  // sort tile of the split phases (as the segmented sort)
 
-enum { PJ_SORT = 0, PJ_SORT_SPLIT = 1, PJ_FILTER = 2, PJ_AGG = 3 };
+enum { PJ_SORT = 0, PJ_SORT_SPLIT = 1, PJ_FILTER = 2, PJ_AGG = 3, PJ_FILTER_SPLIT = 4, PJ_AGG_SPLIT = 5 };
 enum { PT_NONE = 0, PT_JOB = 1, PT_PHASE = 2, PT_EXIT = 3 };
 // control words, 64 words (256 bytes) apart
 enum { PC_CUR_SPLIT = 0, PC_CUR_WHOLE = 64, PC_HEAD = 128, PC_TAIL = 192, PC_DONE = 256, PC_ERR = 320, PC_WORDS = 384 };
@@ -170,7 +170,10 @@ struct PJob {
   uint32_t *perm_out;
   void *kalt;
   uint32_t *valt;
-  uint32_t *counts;  // split sort: nchunks * 256 words (counts, then offsets)
+  uint32_t *counts;   // split sort: nchunks * 256 words (counts, then offsets)
+  uint64_t *fcounts;  // split filter: nchunks selected-row counts, then offsets
+  uint32_t agg_tiles; // split aggregate: canonical tiles per task
+  uint32_t pad;
   SegFilter f;
   SegAgg a;
 };
@@ -204,6 +207,20 @@ __device__ __forceinline__ uint32_t pld(const uint32_t *p) {
 }
 __device__ __forceinline__ uint64_t pld64(const uint64_t *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Cross-workgroup hand-offs.  The workgroup barrier first makes every
+// thread's stores complete, then one thread's agent-scope fence writes the
+// workgroup's data back (release) or drops stale cached lines (acquire):
+// the cache operations act for the whole compute unit and L2, so one wave
+// issuing them is enough.
+__device__ __forceinline__ void wg_release() {
+  __syncthreads();
+  if (threadIdx.x == 0) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+}
+__device__ __forceinline__ void wg_acquire() {
+  if (threadIdx.x == 0) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  __syncthreads();
 }
 
 // Thread 0: the next index of a job list, or false.
@@ -253,8 +270,7 @@ __device__ __forceinline__ void ppush(const PArgs &a, uint32_t job, uint32_t pha
 }
 
 __device__ __forceinline__ void pjob_done(const PArgs &a) {
-  __threadfence();
-  __syncthreads();
+  wg_release();
   if (threadIdx.x == 0) atomicAdd(&a.ctl[PC_DONE], 1u);
 }
 
@@ -310,22 +326,99 @@ __device__ __noinline__ void psplit_task(const PArgs &a, SegSortSmem<K, kPB, 16>
                                 (uint32_t)ceil_div(r1, (uint64_t)kPTile), 8 * p, 0u);
   }
   // completion: the last task of the phase starts the next one
-  __threadfence();
-  __syncthreads();
+  wg_release();
   if (threadIdx.x == 0) *s_flag = atomicAdd(&a.entries[entry].done, 1u) == j.nchunks - 1 ? 1u : 0u;
   __syncthreads();
   if (!*s_flag) return;
-  __threadfence();
+  wg_acquire();  // every chunk's output of this phase
   if (!scatter) {
     psplit_offsets(j.counts, j.nchunks, sm.os.red);
-    __threadfence();
-    __syncthreads();
+    wg_release();
     if (threadIdx.x == 0) ppush(a, job, phase + 1, j.nchunks);
   } else if (p + 1 < passes) {
     if (threadIdx.x == 0) ppush(a, job, phase + 1, j.nchunks);
   } else {
     pjob_done(a);
   }
+}
+
+// Split filter: phase 0 counts each chunk's selected rows (the compaction's
+// own loads and comparison); its last finisher turns the counts into
+// offsets (exclusive scan over chunks); phase 1 compacts each chunk's tiles
+// in order from its offset (the last tile writes *count_out).
+__device__ __noinline__ void pfilter_task(const PArgs &a, CompactSmem<kSegFiltBlock, 4> &sm, const PJob &j,
+                                          uint32_t job, uint32_t phase, uint32_t entry, uint32_t c,
+                                          uint32_t *s_flag) {
+  const SegFilter &f = j.f;
+  constexpr uint32_t TILE = kSegFiltBlock * 16;
+  const uint32_t ntiles = (f.n + TILE - 1) / TILE;
+  const uint32_t tpc = a.chunk / TILE;
+  const uint32_t t0 = c * tpc, t1 = t0 + tpc < ntiles ? t0 + tpc : ntiles;
+  if (phase == 0) {
+    uint32_t cnt = 0;
+    for (uint32_t tile = t0; tile < t1; tile++) {
+      const uint64_t base = (uint64_t)tile * TILE;
+      const uint64_t rows = (f.n - base) < (uint64_t)TILE ? (f.n - base) : (uint64_t)TILE;
+      float v[16];
+      load16<float>(f.vals, base, rows, threadIdx.x, v);
+#pragma unroll
+      for (int e = 0; e < 16; e++) cnt += ((uint64_t)threadIdx.x * 16 + e < rows && cmp_static<YSQL_LT>(v[e], f.thr));
+    }
+    uint64_t tot;
+    block_exclusive_sum<kSegFiltBlock, uint64_t>(cnt, sm.red, &tot);
+    if (threadIdx.x == 0) j.fcounts[c] = tot;
+  } else {
+    const ysql_col cols[2] = {{f.keys, nullptr, f.keys_out, nullptr, f.kb, 0},
+                              {f.vals, nullptr, f.vals_out, nullptr, 4, 0}};
+    uint64_t running = j.fcounts[c];
+    for (uint32_t tile = t0; tile < t1; tile++)
+      running += compact_tile<kSegFiltBlock, 1, float, YSQL_LT>(sm, tile, nullptr, f.vals, f.thr, 0.f, cols, 2, f.n,
+                                                                 ~0ull, running, f.count_out, nullptr, ntiles);
+  }
+  wg_release();
+  if (threadIdx.x == 0) *s_flag = atomicAdd(&a.entries[entry].done, 1u) == j.nchunks - 1 ? 1u : 0u;
+  __syncthreads();
+  if (!*s_flag) return;
+  wg_acquire();
+  if (phase == 0) {
+    uint64_t carry = 0;
+    for (uint32_t b = 0; b < j.nchunks; b += kSegFiltBlock) {
+      const uint32_t i = b + threadIdx.x;
+      const uint64_t v = i < j.nchunks ? j.fcounts[i] : 0;
+      uint64_t tot;
+      const uint64_t ex = block_exclusive_sum<kSegFiltBlock, uint64_t>(v, sm.red, &tot);
+      if (i < j.nchunks) j.fcounts[i] = carry + ex;
+      carry += tot;
+    }
+    wg_release();
+    if (threadIdx.x == 0) ppush(a, job, 1u, j.nchunks);
+  } else {
+    pjob_done(a);
+  }
+}
+
+// Split aggregate: each task folds its run of canonical tiles into the
+// per-tile partials; the last finisher combines every tile's partials in
+// tile order, exactly as the one-workgroup and per-call forms do.
+__device__ __noinline__ void pagg_task(const PArgs &a, ysql_acc *s_w, const PJob &j, uint32_t entry, uint32_t c,
+                                       uint32_t *s_flag) {
+  const SegAgg &g = j.a;
+  FaSpec sp[3];
+  seg_agg_specs(g, sp);
+  const uint64_t tile_rows = fa_tile_rows(g.n);
+  const uint32_t ntiles = (uint32_t)ceil_div((uint64_t)g.n, tile_rows);
+  const uint32_t t0 = c * j.agg_tiles, t1 = t0 + j.agg_tiles < ntiles ? t0 + j.agg_tiles : ntiles;
+  const float sc[1] = {g.thr};
+  for (uint32_t tile = t0; tile < t1; tile++)
+    filter_agg_tile<float, YSQL_LT, kFaBlock, 1>(tile, tile_rows, g.vals, sc, 1, sp, 3, g.n, g.partials, s_w);
+  wg_release();
+  if (threadIdx.x == 0) *s_flag = atomicAdd(&a.entries[entry].done, 1u) == j.nchunks - 1 ? 1u : 0u;
+  __syncthreads();
+  if (!*s_flag) return;
+  wg_acquire();
+  ysql_acc *const out[3] = {g.accs_out, g.accs_out + 1, g.accs_out + 2};
+  filter_agg_finish<kFaBlock>(sp, 3, 1, ntiles, g.partials, out, true, s_w);
+  pjob_done(a);
 }
 
 // The task bodies stay separate functions: each keeps its own register
@@ -368,10 +461,10 @@ __global__ __launch_bounds__(kPB, 3) void k_persist(const PArgs *__restrict__ pa
       continue;
     }
     idle = 0;
-    __threadfence();  // see what the producers of this task's inputs wrote
+    wg_acquire();  // see what the producers of this task's inputs wrote
     if (kind == PT_JOB) {
       const PJob &j = a.jobs[x];
-      if (j.type == PJ_SORT_SPLIT) {
+      if (j.type == PJ_SORT_SPLIT || j.type == PJ_FILTER_SPLIT || j.type == PJ_AGG_SPLIT) {
         if (threadIdx.x == 0) ppush(a, x, 0u, j.nchunks);
         __syncthreads();
         continue;
@@ -389,7 +482,9 @@ __global__ __launch_bounds__(kPB, 3) void k_persist(const PArgs *__restrict__ pa
       const uint64_t d = pld64(&a.entries[x].desc);
       const uint32_t job = (uint32_t)(d >> 32), phase = (uint32_t)(d >> 24) & 0xFFu;
       const PJob &j = a.jobs[job];
-      if (j.key64) psplit_task<uint64_t>(a, sm.s64, j, job, phase, x, y, &s_flag);
+      if (j.type == PJ_FILTER_SPLIT) pfilter_task(a, sm.cs, j, job, phase, x, y, &s_flag);
+      else if (j.type == PJ_AGG_SPLIT) pagg_task(a, sm.fa, j, x, y, &s_flag);
+      else if (j.key64) psplit_task<uint64_t>(a, sm.s64, j, job, phase, x, y, &s_flag);
       else psplit_task<uint32_t>(a, sm.s32, j, job, phase, x, y, &s_flag);
     }
     __syncthreads();
@@ -406,9 +501,9 @@ __global__ void k_persist_reset(uint32_t *ctl, PEntry *entries, uint32_t cap, co
   for (uint64_t q = tid; q < njobs; q += stride) {
     const PJob &j = jobs[q];
     if (j.n) continue;
-    if (j.type == PJ_FILTER) {
+    if (j.type == PJ_FILTER || j.type == PJ_FILTER_SPLIT) {
       *j.f.count_out = 0;
-    } else if (j.type == PJ_AGG) {
+    } else if (j.type == PJ_AGG || j.type == PJ_AGG_SPLIT) {
       FaSpec sp[3];
       seg_agg_specs(j.a, sp);
       for (int s = 0; s < 3; s++) {
@@ -551,21 +646,33 @@ static int persist_setup(ysql_batch *b) {
         cap += 2 * kb;  // U_p and D_p for every digit pass
       }
     } else if (x.kind == YSQL_JOB_FILTER) {
-      j.type = PJ_FILTER;
+      j.type = x.n > (uint64_t)std::max(t.persist_split_filter_min, 0) ? PJ_FILTER_SPLIT : PJ_FILTER;
       j.f = SegFilter{x.keys, x.vals, x.keys_out, x.vals_out, x.count_out, x.threshold, (uint32_t)x.n, (uint32_t)kb,
                       0};
+      if (j.type == PJ_FILTER_SPLIT) {
+        j.nchunks = (uint32_t)ceil_div(x.n, (uint64_t)chunk);
+        o_cnt[q] = take((size_t)j.nchunks * 8);
+        cap += 2;
+      }
     } else {
-      j.type = PJ_AGG;
+      j.type = x.n > (uint64_t)std::max(t.persist_split_agg_min, 0) ? PJ_AGG_SPLIT : PJ_AGG;
       if (x.n) o_part[q] = take((ceil_div(x.n, fa_tile_rows(x.n)) * 3 + 1) * sizeof(ysql_acc));
       j.a = SegAgg{x.keys, x.vals, x.accs_out, nullptr, x.threshold, (uint32_t)x.n, j.key64 ? 1 : 0, 0};
+      if (j.type == PJ_AGG_SPLIT) {
+        const uint64_t tr = fa_tile_rows(x.n);
+        j.agg_tiles = (uint32_t)std::max<uint64_t>(1, chunk / tr);
+        j.nchunks = (uint32_t)ceil_div(ceil_div(x.n, tr), (uint64_t)j.agg_tiles);
+        cap += 1;
+      }
     }
     if (x.n) {
-      (j.type == PJ_SORT_SPLIT ? split_order : whole_order).push_back((uint32_t)q);
+      const bool split = j.type == PJ_SORT_SPLIT || j.type == PJ_FILTER_SPLIT || j.type == PJ_AGG_SPLIT;
+      (split ? split_order : whole_order).push_back((uint32_t)q);
       total++;
     }
   }
   std::stable_sort(split_order.begin(), split_order.end(),
-                   [&](uint32_t a, uint32_t c) { return b->jobs[a].n > b->jobs[c].n; });
+                   [&](uint32_t a, uint32_t c) { return job_cost(b->jobs[a]) > job_cost(b->jobs[c]); });
   std::stable_sort(whole_order.begin(), whole_order.end(),
                    [&](uint32_t a, uint32_t c) { return job_cost(b->jobs[a]) > job_cost(b->jobs[c]); });
   cap = std::max<uint64_t>(cap, 1);
@@ -592,8 +699,10 @@ static int persist_setup(ysql_batch *b) {
       j.kalt = ws + o_kalt[q];
       j.valt = (uint32_t *)(ws + o_valt[q]);
       if (j.type == PJ_SORT_SPLIT) j.counts = (uint32_t *)(ws + o_cnt[q]);
-    } else if (j.type == PJ_AGG) {
+    } else if (j.type == PJ_AGG || j.type == PJ_AGG_SPLIT) {
       j.a.partials = (ysql_acc *)(ws + o_part[q]);
+    } else if (j.type == PJ_FILTER_SPLIT) {
+      j.fcounts = (uint64_t *)(ws + o_cnt[q]);
     }
   }
   PArgs &a = b->p_args;
@@ -664,10 +773,12 @@ void ysql_batch_tuning_default(ysql_batch_tuning *t) {
   t->seg_filter_max = 1 << 21;
   t->seg_agg_max = 1 << 22;
   memset(t->sort_variant, -1, sizeof(t->sort_variant));
-  t->persist_split_min = 1 << 18;
-  t->persist_chunk = 1 << 16;
+  t->persist_split_min = 1 << 19;
+  t->persist_chunk = 1 << 18;
   t->persist_jobs_first_every = 4;
   t->persist_idle_limit = 1 << 15;
+  t->persist_split_filter_min = 1 << 20;
+  t->persist_split_agg_min = 1 << 20;
   t->persist_workers_per_cu = 0;
 }
 
