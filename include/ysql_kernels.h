@@ -2,8 +2,8 @@
  * ysql_kernels.h — C ABI of the YallaSQL operator hot path for AMD MI355X (gfx950).
  *
  * Every entry point replaces one host launcher of the reference's
- * `YallaSQL::Kernel` namespace (reference `include/kernels/*.hpp`), which the
- * reference's operators (reference `include/engine/operators/*.hpp`) call with
+ * `YallaSQL::Kernel` namespace (the reference's headers under include/kernels/),
+ * which the reference's operators (under include/engine/operators/) call with
  * device pointers and a stream.  The reference launchers are C++ templates over
  * element type and functor; here they are type-tagged C functions.
  *
@@ -319,6 +319,25 @@ int ysql_reduce(int dtype, int agg, const void *in, const uint8_t *null_opt, uin
  * COUNT / COUNT_STAR -> int64.  Replaces div_avg_kernel and
  * convert_double_to_float_kernel (reduction_kernel.cu:5-25).             */
 int ysql_agg_finalize(int dtype, int agg, const ysql_acc *acc, void *out, ysql_stream_t stream);
+
+/* The reference's own accumulator form, for callers that keep it: one plain
+ * typed value per aggregate (aggregate.hpp:201-205: INT -> int32, FLOAT
+ * MIN/MAX -> float, FLOAT SUM/AVG -> double, DATETIME -> int64, COUNT and
+ * AVG's counter -> int32).  ysql_acc_from_value loads such a value into a
+ * ysql_acc; ysql_acc_to_value stores it back (an INT sum keeps its low 32
+ * bits, wrapping as the reference's int32 accumulator does).  With
+ * ysql_reduce between them they replace launch_reduction_operators and
+ * launch_reduction_count_notnull (reduction_kernel.hpp:7-15 ->
+ * reduction_kernel.cu:124-205), called from AggregateExpression::evaluate
+ * (aggregate.hpp:79-137).                                                 */
+int ysql_acc_from_value(int dtype, int agg, const void *value, ysql_acc *acc, ysql_stream_t stream);
+int ysql_acc_to_value(int dtype, int agg, const ysql_acc *acc, void *value, ysql_stream_t stream);
+/* launch_div_avg<T> (reduction_kernel.cu:5-15): *avg = (float)((double)*sum /
+ * (double)*counter), sum int32 (INT), double (FLOAT) or int64 (DATETIME);
+ * launch_convert_double_to_float_kernel (:17-25).  Called from
+ * AggregateExpression::getAggregate (aggregate.hpp:149-192).             */
+int ysql_div_avg(int dtype, const int32_t *counter, const void *sum, float *avg, ysql_stream_t stream);
+int ysql_double_to_float(const double *in, float *out, ysql_stream_t stream);
 
 /* Filtered multi-aggregate in one pass (FilterOperator feeding
  * AggregateOperator, filter_operator.hpp:31-98 + aggregate.hpp:55-146,

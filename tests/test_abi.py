@@ -74,37 +74,15 @@ def test_invalid_arguments_rejected_before_launch():
                                  None) == L.EINVAL
 
 
-def test_reference_shaped_launchers_compile(tmp_path):
-    """include/ysql_kernels.hpp instantiates for the reference's types."""
+def test_reference_shaped_launchers_compile():
+    """include/ysql_kernels.hpp instantiates for the reference's types: the
+    translation unit that the GPU test runs (every launcher, the batch-level
+    helpers over a Batch of the reference's shape) compiles here."""
     import shutil
     import subprocess
 
     hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
-    src = tmp_path / "compat.cpp"
-    src.write_text(r'''
-#include "ysql_kernels.hpp"
-using namespace YallaSQL::Kernel;
-void use(hipStream_t s) {
-  int *i = nullptr; float *f = nullptr; int64_t *l = nullptr; bool *m = nullptr; uint32_t *u = nullptr;
-  uint64_t *u64 = nullptr; char *nb = nullptr; int *cnt = nullptr;
-  launch_int_to_uint32(i, u, 10); launch_float_to_uint32(f, u, 10);
-  launch_radix_sort<DescOp<uint32_t>, uint32_t, 256>(u, u, 10, s);
-  launch_radix_sort<AscOp<uint64_t>, uint64_t, 256>(u64, u, 10, s);
-  launch_prefix_sum_mask(m, u, 10, s); launch_prefix_sum<uint32_t>(u, u, 10, s);
-  launch_merge_sorted_array_kernel<float, DescSortMergeOp<float>>(f, f, f, u, m, cnt, 10, 5, 5, s);
-  launch_move_one_col<int>(i, i, nb, nb, u, 10, s);
-  launch_move_one_col_merge<int64_t>(l, l, l, nb, nb, nb, u, m, true, 10, s);
-  launch_move_rows_filter_kernel<float>(f, f, u, m, nb, 10, s);
-  lanch_move_rows_join_kernel<int>(i, i, u, nb, nb, 10, true, s);
-  launch_conditional_operators<float, GEOperator<float>>(f, f, OperandType::SCALAR, OperandType::VECTOR, m, 10, s, false);
-  launch_conditional_operators<bool, ANDOperator>(m, m, OperandType::VECTOR, OperandType::VECTOR, m, 10, s);
-  launch_binary_operators<int, MulOperator<int>>(i, i, OperandType::SCALAR, OperandType::VECTOR, i, 10, s);
-  launch_numerical_cast<int, float>(i, f, 10, s);
-  launch_outer_join_operators<int64_t, EQOperator<int64_t>>(l, l, u, cnt, 10, 10, s);
-  launch_and_join_operators(u, u, u, u, cnt, 10, 10, s);
-  launch_cross_product_col<int>(i, i, nb, nb, 3, 4, s, true);
-}
-''')
-    r = subprocess.run([hipcc, "-fsyntax-only", "-std=c++17", "-I", f"{ROOT}/include", str(src)],
+    src = os.path.join(ROOT, "tests", "compat", "compat_launchers.cpp")
+    r = subprocess.run([hipcc, "-fsyntax-only", "-std=c++17", "-Wall", "-Werror", "-Wno-unused-command-line-argument", "-I", f"{ROOT}/include", src],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr[-3000:]

@@ -98,6 +98,10 @@ _SIGS = {
     "ysql_reduce_ws_bytes": [U64, SZP],
     "ysql_reduce": [I32, I32, P, P, U64, P, P, C.c_size_t, P],
     "ysql_agg_finalize": [I32, I32, P, P, P],
+    "ysql_acc_from_value": [I32, I32, P, P, P],
+    "ysql_acc_to_value": [I32, I32, P, P, P],
+    "ysql_div_avg": [I32, P, P, P, P],
+    "ysql_double_to_float": [P, P, P],
     "ysql_group_reduce": [I32, I32, P, U32, P, P, U64, P, P],
     "ysql_join_ws_bytes": [I32, I32, U64, U64, SZP],
     "ysql_join_pairs": [I32, I32, P, U64, P, U64, P, U64, P, P, C.c_size_t, P],

@@ -56,6 +56,29 @@ def build_library(verbose=True):
     return LIB
 
 
+COMPAT_SRC = os.path.join(ROOT, "tests", "compat", "compat_launchers.cpp")
+COMPAT_LIB = os.path.join(ROOT, "tests", "compat", "libcompat_launchers.so")
+
+
+def build_compat(verbose=True):
+    """The test library that drives include/ysql_kernels.hpp's launchers
+    (tests/compat/compat_launchers.cpp), linked against the kernel library."""
+    if not os.path.exists(COMPAT_SRC):
+        return None
+    tmp = COMPAT_LIB + ".tmp"
+    cmd = [HIPCC, "-O2", "-std=c++17", "-fPIC", "-shared", f"--offload-arch={ARCH}", "-Wno-unused-result",
+           "-I", os.path.join(ROOT, "include"), COMPAT_SRC, "-o", tmp, "-L", PKG, "-lyallasql_amd",
+           "-Wl,-rpath,$ORIGIN/../../yallasql_amd"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, COMPAT_LIB)
+    if verbose:
+        print(f"built {COMPAT_LIB}")
+    return COMPAT_LIB
+
+
 if __name__ == "__main__":
     build_library()
+    build_compat()
     sys.exit(0)

@@ -246,6 +246,44 @@ __global__ void k_finalize(int dtype, int agg, const ysql_acc *acc, void *out) {
   }
 }
 
+// The reference keeps one plain typed accumulator per aggregate
+// (aggregate.hpp:201-205): INT -> int32, FLOAT MIN/MAX -> float, FLOAT
+// SUM/AVG -> double, DATETIME -> int64, COUNT and AVG's counter -> int32.
+__global__ void k_acc_from_value(int dtype, int agg, const void *v, ysql_acc *acc) {
+  ysql_acc a;
+  a.i = 0;
+  a.d = 0.0;
+  a.f = 0.f;
+  a.pad = 0;
+  a.count = 0;
+  if (agg == YSQL_AGG_COUNT || agg == YSQL_AGG_COUNT_STAR) a.count = *(const int32_t *)v;
+  else if (dtype == YSQL_INT) a.i = *(const int32_t *)v;
+  else if (dtype == YSQL_DATETIME) a.i = *(const int64_t *)v;
+  else if (agg == YSQL_AGG_SUM || agg == YSQL_AGG_AVG) a.d = *(const double *)v;
+  else a.f = *(const float *)v;
+  *acc = a;
+}
+
+// Inverse of k_acc_from_value.  An INT sum keeps its low 32 bits: the
+// reference's int32 accumulator wraps the same way (aggregate.hpp:122-124).
+__global__ void k_acc_to_value(int dtype, int agg, const ysql_acc *acc, void *v) {
+  const ysql_acc a = *acc;
+  if (agg == YSQL_AGG_COUNT || agg == YSQL_AGG_COUNT_STAR) *(int32_t *)v = (int32_t)(uint32_t)(uint64_t)a.count;
+  else if (dtype == YSQL_INT) *(int32_t *)v = (int32_t)(uint32_t)(uint64_t)a.i;
+  else if (dtype == YSQL_DATETIME) *(int64_t *)v = a.i;
+  else if (agg == YSQL_AGG_SUM || agg == YSQL_AGG_AVG) *(double *)v = a.d;
+  else *(float *)v = a.f;
+}
+
+// div_avg_kernel (reduction_kernel.cu:5-10): avg = (double)sum / (double)count.
+__global__ void k_div_avg(int dtype, const int32_t *counter, const void *sum, float *avg) {
+  const double s = dtype == YSQL_INT ? (double)*(const int32_t *)sum
+                                     : (dtype == YSQL_DATETIME ? (double)*(const int64_t *)sum : *(const double *)sum);
+  *avg = (float)(s / (double)*counter);
+}
+
+__global__ void k_double_to_float(const double *in, float *out) { *out = (float)*in; }
+
 static size_t reduce_ws_size(uint64_t n) {
   WsCounter c;
   c.take<uint32_t>(64);
@@ -655,6 +693,39 @@ int ysql_reduce(int dtype, int agg, const void *in, const uint8_t *null_opt, uin
 int ysql_agg_finalize(int dtype, int agg, const ysql_acc *acc, void *out, ysql_stream_t stream) {
   if (!acc || !out || agg < YSQL_AGG_MAX || agg > YSQL_AGG_COUNT_STAR) return YSQL_EINVAL;
   k_finalize<<<1, 1, 0, as_stream(stream)>>>(dtype, agg, acc, out);
+  YSQL_LAUNCH_CHECK();
+  return YSQL_OK;
+}
+
+static bool acc_value_ok(int dtype, int agg) {
+  return agg >= YSQL_AGG_MAX && agg <= YSQL_AGG_COUNT_STAR &&
+         (dtype == YSQL_INT || dtype == YSQL_FLOAT || dtype == YSQL_DATETIME);
+}
+
+int ysql_acc_from_value(int dtype, int agg, const void *value, ysql_acc *acc, ysql_stream_t stream) {
+  if (!value || !acc || !acc_value_ok(dtype, agg)) return YSQL_EINVAL;
+  k_acc_from_value<<<1, 1, 0, as_stream(stream)>>>(dtype, agg, value, acc);
+  YSQL_LAUNCH_CHECK();
+  return YSQL_OK;
+}
+
+int ysql_acc_to_value(int dtype, int agg, const ysql_acc *acc, void *value, ysql_stream_t stream) {
+  if (!value || !acc || !acc_value_ok(dtype, agg)) return YSQL_EINVAL;
+  k_acc_to_value<<<1, 1, 0, as_stream(stream)>>>(dtype, agg, acc, value);
+  YSQL_LAUNCH_CHECK();
+  return YSQL_OK;
+}
+
+int ysql_div_avg(int dtype, const int32_t *counter, const void *sum, float *avg, ysql_stream_t stream) {
+  if (!counter || !sum || !avg || !acc_value_ok(dtype, YSQL_AGG_AVG)) return YSQL_EINVAL;
+  k_div_avg<<<1, 1, 0, as_stream(stream)>>>(dtype, counter, sum, avg);
+  YSQL_LAUNCH_CHECK();
+  return YSQL_OK;
+}
+
+int ysql_double_to_float(const double *in, float *out, ysql_stream_t stream) {
+  if (!in || !out) return YSQL_EINVAL;
+  k_double_to_float<<<1, 1, 0, as_stream(stream)>>>(in, out);
   YSQL_LAUNCH_CHECK();
   return YSQL_OK;
 }
