@@ -73,7 +73,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const K *__restrict__ kin, c
 // (8192-key tiles, look-back after the LDS reorder) measured best on 2^25
 // keys; the smaller tiles give mid-size sorts more workgroups.  A sort uses
 // one variant for all its passes (the status areas are laid out per tile).
-constexpr int kRadixVariants = 3;
+constexpr int kRadixVariants = 5;
 template <int V>
 struct RadixTile;
 template <>
@@ -88,6 +88,14 @@ template <>
 struct RadixTile<2> {
   static constexpr int BLOCK = 256, KPT = 8;
 };
+template <>
+struct RadixTile<3> {
+  static constexpr int BLOCK = 512, KPT = 8;
+};
+template <>
+struct RadixTile<4> {
+  static constexpr int BLOCK = 256, KPT = 12;
+};
 constexpr int kSmallSortBlock = 512, kSmallSortKpt = 16;  // one-workgroup sorts up to 8192 keys
 constexpr uint64_t kMinTile = 256 * 8;                     // smallest variant tile: sizes the status areas
 
@@ -97,7 +105,13 @@ constexpr uint64_t kMinTile = 256 * 8;                     // smallest variant t
 static inline int radix_default_variant(uint64_t n) { return n < (1ull << 19) ? 2 : 0; }
 
 static inline uint64_t variant_tile(int v) {
-  return v == 0 ? 512 * 16 : (v == 1 ? 256 * 16 : 256 * 8);
+  switch (v) {
+    case 0: return RadixTile<0>::BLOCK * RadixTile<0>::KPT;
+    case 1: return RadixTile<1>::BLOCK * RadixTile<1>::KPT;
+    case 2: return RadixTile<2>::BLOCK * RadixTile<2>::KPT;
+    case 3: return RadixTile<3>::BLOCK * RadixTile<3>::KPT;
+    default: return RadixTile<4>::BLOCK * RadixTile<4>::KPT;
+  }
 }
 
 struct RadixWs {
@@ -186,6 +200,8 @@ static int radix_pass_t(int desc, int p, int passes, const K *kin, const uint32_
     case 0: radix_pass_v<K, 0>(desc, p, passes, kin, vin, val_base, kout, vout, n, begin_bit, w, s); break;
     case 1: radix_pass_v<K, 1>(desc, p, passes, kin, vin, val_base, kout, vout, n, begin_bit, w, s); break;
     case 2: radix_pass_v<K, 2>(desc, p, passes, kin, vin, val_base, kout, vout, n, begin_bit, w, s); break;
+    case 3: radix_pass_v<K, 3>(desc, p, passes, kin, vin, val_base, kout, vout, n, begin_bit, w, s); break;
+    case 4: radix_pass_v<K, 4>(desc, p, passes, kin, vin, val_base, kout, vout, n, begin_bit, w, s); break;
     default: return YSQL_EINVAL;
   }
   YSQL_LAUNCH_CHECK();

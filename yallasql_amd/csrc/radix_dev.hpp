@@ -33,6 +33,25 @@ __device__ __forceinline__ void rstatus_store(uint32_t *p, uint32_t v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Lanes of the wave whose 8-bit digit equals this lane's, among the lanes
+// with `valid` set: the AND over the digit's 8 bits of the ballot of lanes
+// sharing that bit.  Each bit's lane value is a sign-extended 0 / ~0, so a
+// half of the mask is and-not(ballot ^ s); the compiler folds the eight
+// steps into or3 / bitop3 chains (about 45 VALU per key instead of ~80 for
+// a select per bit).
+__device__ __forceinline__ uint64_t match_digit8(uint32_t d, bool valid) {
+  const uint64_t v = __builtin_amdgcn_ballot_w64(valid);
+  uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+#pragma unroll
+  for (int b = 0; b < 8; b++) {
+    const uint32_t s = (uint32_t)(((int32_t)(d << (31 - b))) >> 31);
+    const uint64_t bal = __builtin_amdgcn_ballot_w64(s != 0u);
+    lo &= ~((uint32_t)bal ^ s);
+    hi &= ~((uint32_t)(bal >> 32) ^ s);
+  }
+  return ((uint64_t)hi << 32) | lo;
+}
+
 // Keys narrower than their type (small integers, Zipf ranks) put most of
 // their high digits in one bin; 64 lanes adding to one LDS word serialise.
 // Each thread therefore counts the bin of digit value `desc` (the encoded
@@ -117,16 +136,58 @@ __device__ __forceinline__ void hist_block(const K *keys, uint64_t begin, uint64
   __syncthreads();
 }
 
+// A tile's keys and payloads are reordered through one 32-bit LDS stage:
+// 64-bit keys go through it as two 32-bit halves (low, then high), so the
+// stage is TILE words for every key width.  Half the LDS of a 64-bit stage
+// lets more workgroups share a CU, and 32-bit scattered LDS stores conflict
+// less than 64-bit ones: on MI355X a 2,048-key tile pass over 2^25 u64 keys
+// runs in about half the time of the same tile with a 64-bit stage.
 template <typename K, int BLOCK, int KPT>
 struct OnesweepSmem {
   static constexpr int W = BLOCK / kWave;
   static constexpr int TILE = BLOCK * KPT;
-  K k[TILE];  // keys, then payloads, of the tile in digit order
+  uint32_t k[TILE];  // key halves, then payloads, of the tile in digit order
   uint32_t wh[W][kBins];
   uint32_t dstart[kBins];
   uint32_t glob[kBins];
   uint32_t red[W + 1];
 };
+
+// Moves key[i] to LDS position pos[i] (0xFFFFFFFF: no key) and returns in
+// out[q] the key at LDS position rd(q) (0 where rd(q) >= lim).  `between`
+// runs after the first stores, before the first barrier (work that overlaps
+// the LDS traffic).  Every thread of the block calls it; the caller
+// synchronises before it reuses the stage.
+template <typename K, int KPT, typename RdIdx, typename Between>
+__device__ __forceinline__ void lds_exchange(uint32_t *sk, const K (&key)[KPT], const uint32_t (&pos)[KPT],
+                                             K (&out)[KPT], RdIdx rd, uint32_t lim, Between between) {
+#pragma unroll
+  for (int i = 0; i < KPT; i++)
+    if (pos[i] != 0xFFFFFFFFu) sk[pos[i]] = (uint32_t)key[i];
+  between();
+  __syncthreads();
+  uint32_t lo[KPT];
+#pragma unroll
+  for (int q = 0; q < KPT; q++) {
+    const uint32_t j = rd(q);
+    lo[q] = j < lim ? sk[j] : 0u;
+  }
+  if constexpr (sizeof(K) == 8) {
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < KPT; i++)
+      if (pos[i] != 0xFFFFFFFFu) sk[pos[i]] = (uint32_t)((uint64_t)key[i] >> 32);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < KPT; q++) {
+      const uint32_t j = rd(q);
+      out[q] = j < lim ? (K)(((uint64_t)sk[j] << 32) | lo[q]) : (K)0;
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < KPT; q++) out[q] = (K)lo[q];
+  }
+}
 
 // One digit pass over tile `tile` of a sequence of n keys.
 //   kin/vin -> kout/vout; vin == nullptr means the payload is val_base + row.
@@ -172,13 +233,7 @@ __device__ __forceinline__ void onesweep_tile(OnesweepSmem<K, BLOCK, KPT> &sm, u
     const uint32_t idx = wbase + i * kWave + lane;
     const bool valid = idx < n;
     const uint32_t d = ((uint32_t)(key[i] >> shift) & 0xFFu) ^ desc;
-    uint64_t m = ballot(valid);
-#pragma unroll
-    for (int b = 0; b < 8; b++) {
-      const bool bit = (d >> b) & 1u;
-      const uint64_t bal = ballot(bit);
-      m &= bit ? bal : ~bal;
-    }
+    const uint64_t m = match_digit8(d, valid);
     const uint32_t lower = popc_below(m);
     uint32_t pre = 0;
     if (valid) pre = sm.wh[wave][d];
@@ -242,8 +297,9 @@ __device__ __forceinline__ void onesweep_tile(OnesweepSmem<K, BLOCK, KPT> &sm, u
   if (t < kBins) sm.dstart[t] = dstart;
   __syncthreads();
 
-  // ---- reorder the tile's keys in LDS by digit (payloads follow later
-  // through the same LDS area)
+  // ---- reorder the tile's keys in LDS by digit (the look-back runs while
+  // the first LDS stores drain), then write each digit's run of keys to its
+  // global position
   uint32_t pos[KPT];
 #pragma unroll
   for (int i = 0; i < KPT; i++) {
@@ -252,38 +308,37 @@ __device__ __forceinline__ void onesweep_tile(OnesweepSmem<K, BLOCK, KPT> &sm, u
     if (idx < n) {
       const uint32_t d = ((uint32_t)(key[i] >> shift) & 0xFFu) ^ desc;
       pos[i] = sm.dstart[d] + sm.wh[wave][d] + rank[i];
-      sm.k[pos[i]] = key[i];
     }
   }
-  if constexpr (!EARLY) lookback();
-  if (t < kBins) sm.glob[t] = gstart + excl - dstart;
-  __syncthreads();
-
-  // ---- write each digit's run of keys to its global position
   const uint32_t valid_cnt = (n - base) < (uint32_t)TILE ? (n - base) : (uint32_t)TILE;
+  K okey[KPT];
+  lds_exchange<K, KPT>(
+      sm.k, key, pos, okey, [&](int q) { return (uint32_t)(t + q * BLOCK); }, valid_cnt,
+      [&]() {
+        if constexpr (!EARLY) lookback();
+        if (t < kBins) sm.glob[t] = gstart + excl - dstart;
+      });
   uint32_t opos[KPT];
 #pragma unroll
   for (int q = 0; q < KPT; q++) {
     const uint32_t j = t + q * BLOCK;
     opos[q] = 0;
     if (j < valid_cnt) {
-      const K kk = sm.k[j];
-      const uint32_t d = ((uint32_t)(kk >> shift) & 0xFFu) ^ desc;
+      const uint32_t d = ((uint32_t)(okey[q] >> shift) & 0xFFu) ^ desc;
       opos[q] = sm.glob[d] + j;
-      kout[opos[q]] = kk;
+      kout[opos[q]] = okey[q];
     }
   }
   __syncthreads();
   // ---- then the payloads, through the same LDS area
-  uint32_t *s_v = reinterpret_cast<uint32_t *>(sm.k);
 #pragma unroll
   for (int i = 0; i < KPT; i++)
-    if (pos[i] != 0xFFFFFFFFu) s_v[pos[i]] = val[i];
+    if (pos[i] != 0xFFFFFFFFu) sm.k[pos[i]] = val[i];
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < KPT; q++) {
     const uint32_t j = t + q * BLOCK;
-    if (j < valid_cnt) vout[opos[q]] = s_v[j];
+    if (j < valid_cnt) vout[opos[q]] = sm.k[j];
   }
   __syncthreads();  // LDS is reused by the caller's next tile
 }
@@ -325,13 +380,7 @@ __device__ __forceinline__ void small_sort_block(OnesweepSmem<K, BLOCK, KPT> &sm
       const uint32_t idx = wbase + i * kWave + lane;
       const bool valid = idx < n;
       const uint32_t d = ((uint32_t)(key[i] >> shift) & 0xFFu) ^ desc;
-      uint64_t m = ballot(valid);
-#pragma unroll
-      for (int b = 0; b < 8; b++) {
-        const bool bit = (d >> b) & 1u;
-        const uint64_t bal = ballot(bit);
-        m &= bit ? bal : ~bal;
-      }
+      const uint64_t m = match_digit8(d, valid);
       const uint32_t lower = popc_below(m);
       uint32_t pre = 0;
       if (valid) pre = sm.wh[wave][d];
@@ -362,14 +411,16 @@ __device__ __forceinline__ void small_sort_block(OnesweepSmem<K, BLOCK, KPT> &sm
       if (idx < n) {
         const uint32_t d = ((uint32_t)(key[i] >> shift) & 0xFFu) ^ desc;
         pos[i] = sm.dstart[d] + sm.wh[wave][d] + rank[i];
-        sm.k[pos[i]] = key[i];
       }
     }
-    __syncthreads();
+    {
+      K nk[KPT];
+      // rows are read back in the same (wave, item, lane) order they were
+      // loaded in: the keys of the next pass, in this pass's order
+      lds_exchange<K, KPT>(
+          sm.k, key, pos, nk, [&](int i) { return wbase + (uint32_t)(i * kWave + lane); }, n, []() {});
 #pragma unroll
-    for (int i = 0; i < KPT; i++) {
-      const uint32_t idx = wbase + i * kWave + lane;
-      if (idx < n) key[i] = sm.k[idx];
+      for (int i = 0; i < KPT; i++) key[i] = nk[i];
     }
     __syncthreads();
 #pragma unroll
@@ -430,13 +481,7 @@ __device__ __forceinline__ void sort_pass_range(OnesweepSmem<K, BLOCK, KPT> &os,
       const uint32_t idx = wbase + i * kWave + lane;
       const bool valid = idx < n;
       const uint32_t d = ((uint32_t)(key[i] >> shift) & 0xFFu) ^ desc;
-      uint64_t m = ballot(valid);
-#pragma unroll
-      for (int b = 0; b < 8; b++) {
-        const bool bit = (d >> b) & 1u;
-        const uint64_t bal = ballot(bit);
-        m &= bit ? bal : ~bal;
-      }
+      const uint64_t m = match_digit8(d, valid);
       const uint32_t lower = popc_below(m);
       uint32_t pre = 0;
       if (valid) pre = os.wh[wave][d];
@@ -470,21 +515,21 @@ __device__ __forceinline__ void sort_pass_range(OnesweepSmem<K, BLOCK, KPT> &os,
       if (idx < n) {
         const uint32_t d = ((uint32_t)(key[i] >> shift) & 0xFFu) ^ desc;
         pos[i] = os.dstart[d] + os.wh[wave][d] + rank[i];
-        os.k[pos[i]] = key[i];
       }
     }
-    __syncthreads();
     const uint32_t valid_cnt = (n - base) < TILE ? (n - base) : TILE;
+    K okey[KPT];
+    lds_exchange<K, KPT>(
+        os.k, key, pos, okey, [&](int q) { return (uint32_t)(t + q * BLOCK); }, valid_cnt, []() {});
     uint32_t opos[KPT];
 #pragma unroll
     for (int q = 0; q < KPT; q++) {
       const uint32_t j = t + q * BLOCK;
       opos[q] = 0;
       if (j < valid_cnt) {
-        const K kk = os.k[j];
-        const uint32_t d = ((uint32_t)(kk >> shift) & 0xFFu) ^ desc;
+        const uint32_t d = ((uint32_t)(okey[q] >> shift) & 0xFFu) ^ desc;
         opos[q] = os.glob[d] + j;
-        dst[opos[q]] = kk;
+        dst[opos[q]] = okey[q];
       }
     }
     __syncthreads();
