@@ -4,9 +4,11 @@ Workload (one step): 200,000,000 rows; key int64 drawn 50% uniform over
 [0, 2^63) and 50% Zipf(s=1.1) over 10^7 distinct values; payload = uint32
 row index.  The table is cut into 256 MB key batches of 2^25 rows (6 runs:
 5 x 33,554,432 + 32,227,840); each run is radix-sorted with 8-bit digits
-(8 passes over the 63 significant bits), then the runs are merged pairwise
-with merge path, level by level, as the reference's OrderOperator does
-(order_operator.hpp:160-193).  Inputs are resident in HBM before timing.
+(8 passes over the 63 significant bits), then the runs are merged in one
+k-way merge (ysql_merge_runs: every row moves once; the reference's
+OrderOperator merges pairwise, level by level, order_operator.hpp:160-193,
+which bench.py --pairwise-merge measures).  Inputs are resident in HBM
+before timing.
 
 Prints one JSON line (rows/s of the whole job) with the roofline of the
 dominant kernel (the onesweep digit pass), measured with HIP events around
@@ -46,6 +48,8 @@ def parse():
     ap.add_argument("--no-serving", action="store_true", help="skip the serving loop")
     ap.add_argument("--no-configs", action="store_true", help="skip the config-1 and config-3 lines")
     ap.add_argument("--serving-seconds", type=float, default=2.5, help="arrival window per rate")
+    ap.add_argument("--pairwise-merge", action="store_true",
+                    help="merge the runs pairwise, level by level, instead of one k-way merge")
     return ap.parse_args()
 
 
@@ -388,7 +392,7 @@ def main():
     torch.cuda.synchronize()
     from yallasql_amd.multirun import MultiRunSort, verify_global_sort
 
-    job = MultiRunSort(keys, RUN_ROWS, key_bits=KEY_BITS)
+    job = MultiRunSort(keys, RUN_ROWS, key_bits=KEY_BITS, merge="pairwise" if args.pairwise_merge else "kway")
     for _ in range(args.warmup):
         job.step()
     torch.cuda.synchronize()
@@ -462,7 +466,8 @@ def main():
         "data": "synthetic (seeded torch generator on the GPU: 50% uniform [0,2^63), 50% Zipf(1.1) over 1e7)",
         "config": {
             "workload": "out-of-core multi-run sort: 200M int64 keys + uint32 row index, 2^25-row runs "
-                        "radix-sorted with 8-bit digits then merge-path merged pairwise",
+                        "radix-sorted with 8-bit digits, then merged ("
+                        + ("pairwise merge path, level by level" if args.pairwise_merge else "one k-way merge") + ")",
             "rows": args.rows,
             "run_rows": RUN_ROWS,
             "runs": nruns,
@@ -481,8 +486,10 @@ def main():
         "ops": {
             "onesweep_pass": {"GB_s": pass_gbs, "frac": pass_gbs / HBM_PEAK_GBS,
                               "avg_ms": sum(pass_ms) / len(pass_ms), "launches": len(pass_ms)},
-            "merge_pairs": None if merge_gbs is None else {"GB_s": merge_gbs, "frac": merge_gbs / HBM_PEAK_GBS,
-                                                           "avg_ms": sum(merge_ms) / len(merge_ms)},
+            "merge": None if merge_gbs is None else {"GB_s": merge_gbs, "frac": merge_gbs / HBM_PEAK_GBS,
+                                                     "avg_ms": sum(merge_ms) / len(merge_ms),
+                                                     "calls_per_step": len(merge_ms) // max(1, args.steps),
+                                                     "kind": "pairwise" if args.pairwise_merge else "k-way"},
             "whole_job": {"algorithmic_GB": alg / 1e9, "GB_s": alg / (ms * 1e-3) / 1e9,
                           "frac": alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
             "device_copy": {"GB_s": copy_gbs, "frac": copy_gbs / HBM_PEAK_GBS,

@@ -184,6 +184,22 @@ int ysql_radix_pass(int key_dtype, int descending, int pass, const void *keys_in
                     uint64_t n, int begin_bit, int end_bit, void *ws, size_t ws_bytes,
                     ysql_stream_t stream);
 
+/* k-way merge of nruns <= 16 sorted runs in one pass (the reference merges
+ * runs pairwise, level by level, OrderOperator::merge_all_sorted,
+ * order_operator.hpp:160-193, order_operator.cpp:116-220, so each row moves
+ * ceil(log2 nruns) times; here once).  Run r is keys[run_offsets[r],
+ * run_offsets[r+1]) with its payloads at the same positions of vals
+ * (run_offsets: host array of nruns + 1 ascending offsets).  Keys ascending
+ * (YSQL_UKEY32 / YSQL_UKEY64, encoded order-preserving keys: a descending
+ * order is an ascending one of complemented keys).  Output: keys_out /
+ * vals_out[0, total) in merged order, ties by run, then by position: for
+ * runs cut in row order and sorted stably, one global stable sort.  Output
+ * buffers must differ from the inputs.  total < 2^32.                   */
+int ysql_merge_runs_ws_bytes(int key_dtype, const uint64_t *run_offsets, int nruns, size_t *bytes);
+int ysql_merge_runs(int key_dtype, const void *keys, const uint32_t *vals, const uint64_t *run_offsets,
+                    int nruns, void *keys_out, uint32_t *vals_out, void *ws, size_t ws_bytes,
+                    ysql_stream_t stream);
+
 /* ======================================================================
  * Inclusive scans (A4, A13)
  * Replace launch_prefix_sum<uint32_t> (prefix_sum.hpp:28-29 -> .cu:158-178)

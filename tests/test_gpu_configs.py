@@ -3,7 +3,8 @@
 * Out-of-core multi-run sort of int64 keys (50% uniform over [0, 2^63), 50%
   Zipf(1.1) ranks over 10^7 values): the run sort (ysql_radix_begin +
   ysql_radix_pass with the run offset as payload base, runs written into
-  slices of shared buffers) and the pairwise merge levels (ysql_merge_pairs)
+  slices of shared buffers), then either the pairwise merge levels
+  (ysql_merge_pairs) or the k-way merge of all runs (ysql_merge_runs),
   against the oracle's stable argsort at 4 runs of 2^22 rows plus a partial
   one, and at 200M rows in 2^25-row runs by the properties that together
   equal one global stable sort.
@@ -26,8 +27,9 @@ def _say(msg):
     print(msg, flush=True)
 
 
+@pytest.mark.parametrize("merge", ["kway", "pairwise"])
 @pytest.mark.parametrize("runs,partial", [(4, 123_457), (2, 0)])
-def test_multirun_sort_matches_oracle(dev, runs, partial):
+def test_multirun_sort_matches_oracle(dev, runs, partial, merge):
     import torch
 
     from yallasql_amd.datagen import multirun_keys
@@ -35,9 +37,9 @@ def test_multirun_sort_matches_oracle(dev, runs, partial):
 
     run_rows = 1 << 22
     n = runs * run_rows + partial
-    _say(f"multirun sort vs oracle: {n} rows, runs of {run_rows}")
+    _say(f"multirun sort vs oracle: {n} rows, runs of {run_rows}, {merge} merge")
     keys = multirun_keys(n, dev, seed=1)
-    job = MultiRunSort(keys, run_rows, key_bits=63)
+    job = MultiRunSort(keys, run_rows, key_bits=63, merge=merge)
     job.step()
     ko, vo = job.result()
     torch.cuda.synchronize()
@@ -55,16 +57,17 @@ def test_multirun_sort_matches_oracle(dev, runs, partial):
 
 
 @pytest.mark.timeout(600)
-def test_multirun_sort_full_size(dev):
+@pytest.mark.parametrize("merge", ["kway", "pairwise"])
+def test_multirun_sort_full_size(dev, merge):
     import torch
 
     from yallasql_amd.datagen import multirun_keys
     from yallasql_amd.multirun import MultiRunSort, verify_global_sort
 
     n, run_rows = 200_000_000, 1 << 25
-    _say(f"multirun sort full size: {n} rows, runs of {run_rows}")
+    _say(f"multirun sort full size: {n} rows, runs of {run_rows}, {merge} merge")
     keys = multirun_keys(n, dev, seed=1)
-    job = MultiRunSort(keys, run_rows, key_bits=63)
+    job = MultiRunSort(keys, run_rows, key_bits=63, merge=merge)
     assert len(job.runs) == 6 and job.runs[-1][1] == 32_227_840
     job.step()
     ko, vo = job.result()

@@ -76,6 +76,8 @@ _SIGS = {
     "ysql_scan_mask_inclusive": [P, P, U64, P, C.c_size_t, P],
     "ysql_merge_ws_bytes": [U64, U64, U64, SZP],
     "ysql_merge_path": [I32, I32, P, U64, P, U64, U64, P, P, P, P, P, C.c_size_t, P],
+    "ysql_merge_runs_ws_bytes": [I32, C.POINTER(U64), I32, SZP],
+    "ysql_merge_runs": [I32, P, P, C.POINTER(U64), I32, P, P, P, C.c_size_t, P],
     "ysql_merge_pairs": [I32, I32, P, P, U64, P, P, U64, P, P, P, C.c_size_t, P],
     "ysql_gather": [U32, P, P, P, U64, P, P, P],
     "ysql_gather2": [U32, P, P, P, P, P, P, U64, I32, P, P, P],

@@ -1,9 +1,10 @@
 """Out-of-core multi-run sort (A11): the table is cut into runs, each run is
-radix-sorted with 8-bit digits, then the sorted runs are merged pairwise
-with merge path, level by level (the reference's OrderOperator:
-order_operator.hpp:160-193 sorts each child batch into a run, then
+radix-sorted with 8-bit digits, then the sorted runs are merged: either
+pairwise with merge path, level by level, as the reference's OrderOperator
+does (order_operator.hpp:160-193 sorts each child batch into a run, then
 merge_all_sorted merges runs (i, i+1) per level, carrying an odd run;
-order_operator.cpp:116-220).
+order_operator.cpp:116-220), or all at once by the k-way merge
+(ysql_merge_runs), which moves every row once instead of once per level.
 
 Keys are int64 (non-negative here, so their raw bits order like the
 values); the payload is the uint32 row index.  Runs are written into
@@ -22,7 +23,9 @@ class MultiRunSort:
     """keys_in: int64 device tensor; run_rows: rows per run; key_bits: the
     significant low bits of the keys (8-bit digits cover them)."""
 
-    def __init__(self, keys_in, run_rows, key_bits=63):
+    def __init__(self, keys_in, run_rows, key_bits=63, merge="kway"):
+        assert merge in ("kway", "pairwise")
+        self.merge_mode = merge
         dev = keys_in.device
         self.n = n = keys_in.numel()
         self.keys_in = keys_in
@@ -34,6 +37,14 @@ class MultiRunSort:
         self.alt_v = torch.empty(run_rows, dtype=torch.int32, device=dev)
         self.sort_ws = ops.workspace(L.ws_bytes("ysql_radix_sort_ws_bytes", L.UKEY64, run_rows), dev)
         self.merge_ws = ops.workspace(L.ws_bytes("ysql_merge_ws_bytes", n, n, n), dev)
+        offs = [o for o, _ in self.runs] + [n]
+        if merge == "kway" and len(self.runs) > 1:
+            import ctypes as C
+
+            b = C.c_size_t()
+            L.call("ysql_merge_runs_ws_bytes", L.UKEY64, (C.c_uint64 * len(offs))(*offs), len(self.runs), C.byref(b))
+            self.merge_ws = ops.workspace(b.value, dev)
+        self.run_offsets = offs
         self.passes = (key_bits + 7) // 8
         self.pass_events = []   # (start, end, rows) per digit-pass launch
         self.merge_events = []  # (start, end, rows) per merge
@@ -84,6 +95,15 @@ class MultiRunSort:
 
     def step(self):
         runs = [self.sort_run(o, r) for o, r in self.runs]
+        if self.merge_mode == "kway" and len(runs) > 1:
+            # every run sits at its own offset of buffer 0: one merge into buffer 1
+            e0 = self._ev() if self.record else None
+            ops.merge_runs(self.bk[0], self.bv[0], self.run_offsets, out_keys=self.bk[1], out_vals=self.bv[1],
+                           ws=self.merge_ws)
+            if self.record:
+                self.merge_events.append((e0, self._ev(), self.n))
+            self.final = (1, 0, self.n)
+            return
         while len(runs) > 1:
             nxt = [self.merge(runs[i], runs[i + 1]) for i in range(0, len(runs) - 1, 2)]
             if len(runs) % 2:
@@ -96,8 +116,11 @@ class MultiRunSort:
         return self.bk[b][o:o + n], self.bv[b][o:o + n]
 
     def algorithmic_bytes(self):
-        """Sort: P * 2 * n * (8 + 4) per run; merge: 2 * (m + n) * 12 per merge."""
+        """Sort: P * 2 * n * (8 + 4) per run; pairwise merge: 2 * (m + n) * 12
+        per merge; k-way merge: 2 * n * 12 once."""
         b = sum(self.passes * 2 * r * 12 for _, r in self.runs)
+        if self.merge_mode == "kway":
+            return b + (2 * self.n * 12 if len(self.runs) > 1 else 0)
         sizes = [r for _, r in self.runs]
         while len(sizes) > 1:
             nxt = []

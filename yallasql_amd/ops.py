@@ -144,6 +144,29 @@ def merge_pairs(a_keys, a_vals, b_keys, b_vals, descending=False, out_keys=None,
     return out_keys, out_vals
 
 
+def merge_runs(keys, vals, run_offsets, out_keys=None, out_vals=None, ws=None):
+    """k-way merge of the sorted runs keys[run_offsets[r]:run_offsets[r+1]]
+    (ascending uint32/uint64 keys as int32/int64 tensors, uint32 payloads):
+    ties by run, then position."""
+    import ctypes as C
+
+    nr = len(run_offsets) - 1
+    offs = (C.c_uint64 * (nr + 1))(*[int(o) for o in run_offsets])
+    n = int(run_offsets[-1]) - int(run_offsets[0])
+    dev = keys.device
+    if out_keys is None:
+        out_keys = torch.empty(n, dtype=keys.dtype, device=dev)
+    if out_vals is None:
+        out_vals = torch.empty(n, dtype=torch.int32, device=dev)
+    kdt = _kdt(keys)
+    if ws is None:
+        b = C.c_size_t()
+        call("ysql_merge_runs_ws_bytes", kdt, offs, nr, C.byref(b))
+        ws = workspace(b.value, dev)
+    call("ysql_merge_runs", kdt, _p(keys), _p(vals), offs, nr, _p(out_keys), _p(out_vals), _p(ws), ws.numel(), _s())
+    return out_keys, out_vals
+
+
 # ---- row movement --------------------------------------------------------
 def _null_out(src_null, n, dev):
     return torch.empty(n, dtype=torch.uint8, device=dev)
