@@ -6,13 +6,22 @@
  * include/enums/data_type_utils.hpp:24-44):
  *   INT      std::stoi of the field (leading blanks, optional sign, digits; trailing text
  *            ignored; no digits or out of int32 range is an error)
- *   FLOAT    std::stof of the field (likewise; ERANGE is an error)
+ *   FLOAT    std::stof of the field (likewise, hexadecimal forms included; a result that
+ *            overflows or underflows to a subnormal or zero is an error)
  *   DATETIME strptime "%Y-%m-%d %H:%M:%S" as UTC (timegm) -> int64 epoch seconds; a field
  *            that does not parse gives 0
  *   STRING   the first 255 bytes, NUL-padded to 256
  * An empty field is NULL: null byte 1 and the value INT_MIN / FLT_MIN / 0 / all-zero string.
- * Fields follow RFC 4180 quoting ("a,b", "say ""hi""", embedded newlines); \r\n and \n end
- * records.
+ *
+ * Records are split as the reference's csv::CSVReader (vendored csv-parser v2.3.0,
+ * vendor/csv-parser/csv.hpp) splits them with its default format: the delimiter (one of
+ * ',', '|', '\t', ';', '^') and the header row are guessed from the first 500,000 bytes; any
+ * run of '\r' and '\n' ends a record (blank lines vanish); a '"' that starts a field quotes
+ * it until a '"' followed by the delimiter, a newline or the end of the file; "" inside quotes
+ * is a quote; records whose field count differs from the header's are skipped; a UTF-8 BOM
+ * is skipped; a repeated header name means its last column.  The library's parse chunks of
+ * 10,000,000 bytes are followed too: a newline run crossing a chunk end adds an empty
+ * one-field record there (a NULL row in a one-column table).
  *
  * Batches hold ysql_csv_batch_rows() rows (the last one fewer): floor(4 MiB / row bytes),
  * rounded down to a multiple of 256, at least 256.
@@ -44,15 +53,15 @@ typedef struct ysql_csv_reader ysql_csv_reader;
 /* Rows per batch for columns of these YSQL_* types (config.hpp:28-38). */
 int ysql_csv_batch_rows(int ncols, const int *types, uint32_t *rows);
 
-/* Opens `path` and reads its header row; names[c] is the CSV header name of requested
- * column c, types[c] its YSQL_* type (get_operator.cpp:16-53).  YSQL_EINVAL for a missing
- * file, an unknown name or a bad type. */
+/* Opens `path`, guesses its delimiter and header row and reads the header; names[c] is the
+ * CSV header name of requested column c, types[c] its YSQL_* type (get_operator.cpp:16-53).
+ * YSQL_EINVAL for a missing or empty file, an unknown name or a bad type. */
 int ysql_csv_open(const char *path, int ncols, const char *const *names, const int *types,
                   ysql_csv_reader **out);
 
 /* Next batch into host memory: host_cols[c] holds batch_rows elements of column c,
  * host_nulls[c] batch_rows null bytes (may be NULL: not written).  *rows = rows read
- * (0: end of file).  YSQL_EINVAL for a field that does not convert or a short record. */
+ * (0: end of file).  YSQL_EINVAL for a field that does not convert. */
 int ysql_csv_next_host(ysql_csv_reader *r, void *const *host_cols, uint8_t *const *host_nulls,
                        uint32_t *rows);
 

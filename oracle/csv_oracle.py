@@ -11,12 +11,19 @@ and String::set (include/kernels/string_kernel.hpp:16-30):
   * FLOAT: std::stof (the longest decimal / inf / nan prefix, rounded to float32);
   * DATETIME: strptime "%Y-%m-%d %H:%M:%S" as UTC -> epoch seconds, 0 if it does not parse;
   * STRING: first 255 bytes (up to a NUL), NUL-padded to 256.
-Tokenising uses Python's csv module (RFC 4180 quoting).  The numeric prefix
-rules are restated with regular expressions; the tests check them against
-values built by construction.
+Tokenising restates the reference's vendored csv-parser v2.3.0
+(vendor/csv-parser/csv.hpp) as csv::CSVReader(path) runs it with its default
+format: the delimiter and header row guessed from the first 500,000 bytes
+(:7368-7443), the file parsed in 10,000,000-byte chunks (MmapParser::next,
+:7174-7201) by the parser's state machine (IBasicCSVParser::parse and
+end_feed, :6990-7141), "" pairs undone as CSVRow::get_field does
+(:7764-7791), rows of another length than the header skipped (IGNORE_ROW,
+CSVReader::read_row :7599-7630).  The numeric prefix rules are restated with
+regular expressions; the tests check them against values built by
+construction, and the whole reader against the reference's own library
+built from its sources (oracle/_ref) and the fixtures it wrote.
 """
 import calendar
-import csv
 import re
 
 import numpy as np
@@ -30,6 +37,8 @@ _WS = "[ \t\n\v\f\r]*"
 _INT_RE = re.compile(_WS + r"([+-]?[0-9]+)")
 _FLT_RE = re.compile(_WS + r"([+-]?(?:(?:[0-9]+\.?[0-9]*|\.[0-9]+)(?:[eE][+-]?[0-9]+)?|inf(?:inity)?|nan))",
                      re.IGNORECASE)
+# strtof's hexadecimal form: 0x, hex digits with an optional point, optional p exponent
+_HEX_RE = re.compile(_WS + r"([+-]?0[xX](?:[0-9a-fA-F]+\.?[0-9a-fA-F]*|\.[0-9a-fA-F]+)(?:[pP][+-]?[0-9]+)?)")
 _DT_RE = re.compile(r"\s*([0-9]{1,4})-([0-9]{1,2})-([0-9]{1,2})\s*([0-9]{1,2}):([0-9]{1,2}):([0-9]{1,2})")
 
 
@@ -55,12 +64,21 @@ def stoi(s):
 
 
 def stof(s):
-    m = _FLT_RE.match(s)
-    if not m:
-        raise ConversionError(s)
-    v = np.float32(float(m.group(1)))
+    h = _HEX_RE.match(s)
+    if h:
+        x = float.fromhex(h.group(1))
+        m = h
+    else:
+        m = _FLT_RE.match(s)
+        if not m:
+            raise ConversionError(s)
+        x = float(m.group(1))
+    with np.errstate(over="ignore"):
+        v = np.float32(x)
     if np.isinf(v) and "inf" not in m.group(1).lower():
         raise ConversionError(s)  # overflow: std::stof throws out_of_range
+    if x != 0.0 and abs(v) < np.finfo(np.float32).tiny:  # (nan compares false)
+        raise ConversionError(s)  # underflow to a subnormal or zero: strtof sets ERANGE, std::stof throws
     return v
 
 
@@ -75,7 +93,7 @@ def datetime_seconds(s):
 
 
 def string_bytes(s):
-    b = s.encode()
+    b = s if isinstance(s, bytes) else s.encode()
     cut = b.find(b"\0")
     if cut >= 0:
         b = b[:cut]
@@ -83,30 +101,239 @@ def string_bytes(s):
     return b + b"\0" * (256 - len(b))
 
 
+HEAD_BYTES = 500_000
+CHUNK_BYTES = 10_000_000
+DELIMS = b",|\t;^"
+_BOM = b"\xef\xbb\xbf"
+
+
+class _Chunk:
+    """One run of the parser's state machine over `data` (one chunk) with
+    delimiter `d`: rows of raw fields, and the offset of the first row left
+    incomplete (the remainder the next chunk starts from)."""
+
+    def __init__(self, data, d, skip_bom):
+        self.data, self.d = data, d
+        self.rows = []
+        self.cur = []             # fields of the row being read
+        self.row_start = 0
+        self.fstart = None        # field start relative to row_start (None: not set)
+        self.flen = 0
+        self.pairs = False        # the field held a "" pair
+        self.quoted = False
+        self.pos = 3 if skip_bom and len(data) >= 3 and data[:3] == _BOM else 0
+        self.unq = re.compile(b"[" + re.escape(bytes([d])) + b'\r\n"]')
+        self.q = re.compile(b'"')
+
+    def kind(self, i):
+        c = self.data[i]
+        if c == self.d:
+            return "D"
+        if c in (10, 13):
+            return "N"
+        if c == 34:
+            return "Q"
+        return "C"
+
+    def push_field(self):
+        s = (self.fstart or 0) + self.row_start
+        raw = self.data[s:s + self.flen]
+        if self.pairs:  # every second quote character of the field dropped
+            out, prev = bytearray(), False
+            for ch in raw:
+                if ch == 34:
+                    if prev:
+                        prev = False
+                        continue
+                    prev = True
+                out.append(ch)
+            raw = bytes(out)
+        self.cur.append(bytes(raw))
+        self.fstart, self.flen, self.pairs = None, 0, False
+
+    def push_row(self):
+        self.rows.append(self.cur)
+        self.cur = []
+
+    def parse(self):
+        data, n = self.data, len(self.data)
+        while self.pos < n:
+            k = self.kind(self.pos)
+            if self.quoted and k in "DN":
+                k = "C"
+            if k == "D":
+                self.push_field()
+                self.pos += 1
+            elif k == "N":
+                self.pos += 1
+                while self.pos < n and data[self.pos] in (10, 13):
+                    self.pos += 1
+                self.push_field()
+                self.push_row()
+                self.row_start = self.pos
+            elif k == "C":
+                if self.fstart is None:
+                    self.fstart = self.pos - self.row_start
+                m = (self.q if self.quoted else self.unq).search(data, self.pos)
+                self.pos = m.start() if m else n
+                self.flen = self.pos - (self.fstart + self.row_start)
+            elif self.quoted:  # a quote inside a quoted field
+                if self.pos + 1 == n:
+                    return self.row_start
+                nk = self.kind(self.pos + 1)
+                if nk in "DN":
+                    self.quoted = False
+                    self.pos += 1
+                elif nk == "Q":
+                    self.pos += 2
+                    self.flen += 2
+                    self.pairs = True
+                else:
+                    self.flen += 1
+                    self.pos += 1
+            elif self.flen == 0:  # opens a quoted field
+                self.quoted = True
+                self.pos += 1
+                if self.fstart is None and self.pos < n:
+                    self.fstart = self.pos - self.row_start
+            else:
+                self.flen += 1
+                self.pos += 1
+        return self.row_start
+
+    def end_feed(self):
+        data = self.data
+        last_empty = len(data) > 0 and data[-1] in (self.d, 34)
+        if self.flen > 0 or last_empty:
+            self.push_field()
+        if self.cur:
+            self.push_row()
+
+
+def parse_rows(buf, d, chunk=CHUNK_BYTES):
+    """Every row csv-parser reads from the bytes `buf` with delimiter `d`,
+    chunk by chunk as MmapParser does (a row crossing a chunk end is read
+    again from its start in the next chunk)."""
+    rows, pos, first, size = [], 0, True, len(buf)
+    while True:
+        length = min(size - pos, chunk)
+        ch = _Chunk(buf[pos:pos + length], d, first)
+        rem = ch.parse()
+        last = pos + length == size or size < chunk
+        if last:
+            ch.end_feed()
+        rows.extend(ch.rows)
+        if last:
+            return rows
+        if rem == 0:
+            raise ConversionError("a record longer than one parse chunk")
+        pos += rem
+        first = False
+
+
+class _TallyOrder:
+    """The order in which csv-parser's row-length tally, a libstdc++
+    std::unordered_map<size_t, size_t> built from {{0, 0}}, iterates its
+    lengths (csv.hpp:7370-7414): the nodes form one list with each bucket's
+    nodes together; a new node goes to the front of its bucket's nodes, or
+    to the front of the list when its bucket is empty (hashtable.h
+    _M_insert_bucket_begin); growing re-inserts the nodes in list order by
+    the same rule (_M_rehash_aux).  The map starts with 13 buckets and grows
+    through the counts below when an insert would exceed the count (counts
+    read off such a map)."""
+
+    GROW = (29, 59, 127, 257, 541, 1109, 2357, 5087, 10273, 20753)
+
+    def __init__(self):
+        self.nb, self.order, self.front = 13, [], {}
+        self._put(0)
+
+    def _put(self, k):
+        b = k % self.nb
+        at = self.order.index(self.front[b]) if b in self.front else 0
+        self.order.insert(at, k)
+        self.front[b] = k
+
+    def add(self, k):
+        if len(self.order) + 1 > self.nb:
+            self.nb = next(c for c in self.GROW if c > self.nb)
+            old, self.order, self.front = self.order, [], {}
+            for x in old:
+                self._put(x)
+        self._put(k)
+
+
+def guess_format(buf):
+    """(delimiter, header row) as csv::CSVReader guesses them from the first
+    500,000 bytes up to a NUL: per candidate delimiter, the score is the
+    largest (row length x rows of that length), equal scores going to the
+    length the tally iterates first; the delimiter with the strictly highest
+    score wins (first in ',|\\t;^' on ties) and the header is the first row
+    of its length."""
+    head = buf[:HEAD_BYTES]
+    cut = head.find(b"\0")
+    if cut >= 0:
+        head = head[:cut]
+    best, delim, header = 0, DELIMS[0], 0
+    for d in DELIMS:
+        ch = _Chunk(head, d, True)
+        ch.parse()
+        ch.end_feed()
+        count, first, tally = {0: 0}, {}, _TallyOrder()
+        for i, r in enumerate(ch.rows):
+            if len(r) not in first:
+                first[len(r)] = i
+                count[len(r)] = 0
+                tally.add(len(r))
+            count[len(r)] += 1
+        score, hdr = 0, 0
+        for length in tally.order:
+            if length * count[length] > score:
+                score, hdr = length * count[length], first[length]
+        if score > best:
+            best, delim, header = score, d, hdr
+    return delim, header
+
+
+def read_fields(path):
+    """(header names, data rows of raw fields) as the reference's reader
+    yields them."""
+    with open(path, "rb") as f:
+        buf = f.read()
+    if not buf:
+        raise ConversionError("no header row")
+    d, header = guess_format(buf)
+    rows = parse_rows(buf, d)
+    if header >= len(rows):
+        raise ConversionError("no header row")
+    names = rows[header]
+    return names, [r for r in rows[header + 1:] if len(r) == len(names)]
+
+
 def read(path, names, types):
     """All rows of the requested columns: ([column arrays], [null byte arrays])."""
-    with open(path, newline="") as f:
-        rd = csv.reader(f)
-        header = next(rd)
-        idx = [header.index(n) for n in names]
-        cols = [[] for _ in names]
-        nulls = [[] for _ in names]
-        for rec in rd:
-            if not rec or (len(rec) == 1 and rec[0] == "" and len(header) > 1):
-                continue
-            if len(rec) < len(header):
-                raise ConversionError("short record")
-            for c, (i, t) in enumerate(zip(idx, types)):
-                s = rec[i]
-                nulls[c].append(1 if s == "" else 0)
-                if t == INT:
-                    cols[c].append(-2**31 if s == "" else stoi(s))
-                elif t == FLOAT:
-                    cols[c].append(np.finfo(np.float32).tiny if s == "" else stof(s))
-                elif t == DATETIME:
-                    cols[c].append(0 if s == "" else datetime_seconds(s))
-                else:
-                    cols[c].append(np.frombuffer(string_bytes(s), np.uint8))
+    header, recs = read_fields(path)
+    idx = []
+    for n in names:
+        hits = [i for i, h in enumerate(header) if h == n.encode()]
+        if not hits:
+            raise ConversionError("no column named %r" % n)
+        idx.append(hits[-1])  # a repeated name means its last occurrence
+    cols = [[] for _ in names]
+    nulls = [[] for _ in names]
+    for rec in recs:
+        for c, (i, t) in enumerate(zip(idx, types)):
+            raw = rec[i]
+            s = raw.decode("latin-1")
+            nulls[c].append(1 if raw == b"" else 0)
+            if t == INT:
+                cols[c].append(-2**31 if raw == b"" else stoi(s.split("\0")[0]))
+            elif t == FLOAT:
+                cols[c].append(np.finfo(np.float32).tiny if raw == b"" else stof(s.split("\0")[0]))
+            elif t == DATETIME:
+                cols[c].append(0 if raw == b"" else datetime_seconds(s.split("\0")[0]))
+            else:
+                cols[c].append(np.frombuffer(string_bytes(raw), np.uint8))
     out = []
     for c, t in enumerate(types):
         if t == STRING:

@@ -1,8 +1,12 @@
 """CSV -> column batches (include/ysql_csv.h): batch sizing in closed form,
-parsed columns against values built by construction and against the
-oracle's restatement of the reference's rules, the error paths, and (GPU)
-the pinned, overlapped device path against the host path."""
+parsed columns against values built by construction, against the fixtures
+the reference's own csv-parser wrote (tests/golden/csv), against the
+oracle's restatement of that parser and, where oracle/_ref is built, against
+the reference's parser itself on seeded random files; the 10,000,000-byte
+parse-chunk records; the error paths; and (GPU) the pinned, overlapped
+device path against the reference's parser."""
 import calendar
+import importlib.util
 import os
 import time
 
@@ -10,6 +14,13 @@ import numpy as np
 import pytest
 
 from oracle import csv_oracle as O
+from oracle import csv_ref as R
+
+_GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "csv")
+_spec = importlib.util.spec_from_file_location("make_csv_golden", os.path.join(_GOLDEN, "make_csv_golden.py"))
+G = importlib.util.module_from_spec(_spec)
+_spec.loader.exec_module(G)
+CASES = G.load()
 
 INT, FLOAT, DATETIME, STRING = 0, 1, 2, 3
 
@@ -33,8 +44,12 @@ def _gen(path, n, seed, crlf=False):
     ints[:2] = [-2**31 + 1, 2**31 - 1]
     flts = (r.standard_normal(n) * 10.0 ** r.integers(-8, 9, n)).astype(np.float32)
     secs = r.integers(0, 4_000_000_000, n, dtype=np.int64)
-    alphabet = np.array(list('abcXYZ 019,"\n;-'))
+    alphabet = np.array(list('abcXYZ 019,";-'))
     strs = ["".join(r.choice(alphabet, int(r.integers(0, 300)))) for _ in range(n)]
+    # a quoted newline in one string of 50: the reference's parser guesses
+    # the delimiter from line counts, and a file whose quoted fields hold
+    # more newlines than records would read as one '|'-separated column
+    strs = [s[:len(s) // 2] + "\n" + s[len(s) // 2:] if i % 50 == 7 else s for i, s in enumerate(strs)]
     empty = r.random((n, 4)) < 0.05
     eol = "\r\n" if crlf else "\n"
     with open(path, "w", newline="") as f:
@@ -143,8 +158,7 @@ def test_errors(loader, tmp_path):
     rd.close()
     p.write_text("a,b\n1\n")
     rd = loader.CsvReader(p, ["b"], [INT])
-    with pytest.raises(L.YsqlError):
-        rd.next_host()  # record shorter than the header
+    assert rd.next_host() is None  # a record shorter than the header is skipped
     rd.close()
     with pytest.raises(L.YsqlError):
         loader.CsvReader(p, ["zz"], [INT])  # no such column
@@ -153,16 +167,20 @@ def test_errors(loader, tmp_path):
 
 
 @pytest.mark.gpu
-def test_device_path_matches_host(loader, tmp_path, dev):
+def test_device_path_matches_reference(loader, tmp_path, dev):
+    """Device columns (pinned staging, copies overlapping the next parse)
+    equal what the reference's own csv-parser reads (oracle/_ref), or the
+    oracle's restatement of it where that library was not built."""
     import torch
 
     n = 100_000
     p = tmp_path / "g.csv"
     _gen(p, n, 11)
+    with open(p, "a", newline="") as f:  # records the parser skips or splits
+        f.write('short\r\r\n\nx1,5,2.5,2001-02-03 04:05:06,"p"q,r"\rx2,6,,,z\n')
     names, types = ["id", "price", "ts", "name"], [INT, FLOAT, DATETIME, STRING]
-    rd = loader.CsvReader(p, names, types)
-    hc, hn = _read_all(rd)
-    rd.close()
+    want_c, want_n = R.read(p, names, types) if R.available() else O.read(p, names, types)
+    assert len(want_c[0]) == n + 2
     rd = loader.CsvReader(p, names, types)
     s = torch.cuda.Stream(device=dev)
     got = []
@@ -173,10 +191,11 @@ def test_device_path_matches_host(loader, tmp_path, dev):
         got.append(b)
     s.synchronize()
     rd.close()
+    assert len(got) > 1
     for c in range(4):
         dc = torch.cat([b[0][c] for b in got]).cpu().numpy()
         dn = torch.cat([b[1][c] for b in got]).cpu().numpy()
-        assert np.array_equal(dc, hc[c]) and np.array_equal(dn, hn[c]), c
+        assert np.array_equal(dc, want_c[c]) and np.array_equal(dn, want_n[c]), c
 
 
 def test_parallel_parse_quotes_and_blank_lines(loader, tmp_path):
@@ -241,3 +260,150 @@ def test_quote_free_fast_path(loader, tmp_path):
     assert np.array_equal(cols[0], ids) and not nulls[0].any()
     oc, on = O.read(p, ["id"], [INT])
     assert np.array_equal(cols[0], oc[0]) and np.array_equal(nulls[0], on[0])
+
+
+def _read_file(loader, path, names, types):
+    rd = loader.CsvReader(path, names, types)
+    try:
+        cols, nulls = _read_all(rd)
+    finally:
+        rd.close()
+    if cols is None:  # no rows
+        cols = [np.zeros((0, 256), np.uint8) if t == STRING else
+                np.zeros(0, {INT: np.int32, FLOAT: np.float32, DATETIME: np.int64}[t]) for t in types]
+        nulls = [np.zeros(0, np.uint8) for _ in types]
+    return cols, nulls
+
+
+def _same(a, b):
+    ca, na = a
+    cb, nb = b
+    return all(np.array_equal(x.view(np.uint8), y.view(np.uint8)) for x, y in zip(ca, cb)) and \
+        all(np.array_equal(x, y) for x, y in zip(na, nb))
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_reference_parser_fixture(loader, tmp_path, case):
+    """Record ends, quotes, skipped rows, delimiter and header guessing, BOM,
+    repeated names and conversions, as the reference's csv-parser read them
+    (fixture); the oracle's restatement agrees too."""
+    from yallasql_amd import _lib as L
+
+    data, names, types, cols, nulls = CASES[case]
+    p = tmp_path / "c.csv"
+    p.write_bytes(data)
+    if cols is None:  # the reference raised (std::stoi / std::stof)
+        with pytest.raises(L.YsqlError):
+            _read_file(loader, p, names, types)
+        with pytest.raises(O.ConversionError):
+            O.read(p, names, types)
+        return
+    assert _same(_read_file(loader, p, names, types), (cols, nulls))
+    assert _same(O.read(p, names, types), (cols, nulls))
+
+
+@pytest.mark.skipif(not R.available(), reason="oracle/_ref not built (needs the reference tree)")
+def test_fixture_matches_reference_build(tmp_path):
+    """The fixtures are what the reference's parser, built from its
+    sources, reads today."""
+    for case, (data, names, types, cols, nulls) in CASES.items():
+        p = tmp_path / (case + ".csv")
+        p.write_bytes(data)
+        if cols is None:
+            with pytest.raises(R.RefCsvError):
+                R.read(p, names, types)
+        else:
+            assert _same(R.read(p, names, types), (cols, nulls)), case
+
+
+def test_random_files_three_way(loader, tmp_path):
+    """600 seeded random files over delimiters, quotes, "" pairs and every
+    newline form: the loader equals the oracle's restatement and, where
+    oracle/_ref is built, the reference's parser, column by column (every
+    header name read as a string)."""
+    r = np.random.default_rng(1234)
+    alpha = [b"a", b"b", b",", b";", b'"', b"\r", b"\n", b"|", b"\t", b"^", b"xy", b'""', b"\r\n"]
+    p = tmp_path / "f.csv"
+    checked = 0
+    for t in range(600):
+        n = int(r.integers(1, 80 if t < 500 else 4000))
+        data = b"".join(alpha[i] for i in r.integers(0, len(alpha), n))
+        p.write_bytes(data)
+        try:
+            header, _ = O.read_fields(p)
+        except O.ConversionError:
+            continue
+        names = []
+        for h in header:
+            if h.isascii() and b"\0" not in h and h.decode() not in names:
+                names.append(h.decode())
+        if not names:
+            continue
+        types = [STRING] * len(names)
+        ours = _read_file(loader, p, names, types)
+        assert _same(ours, O.read(p, names, types)), data
+        if R.available():
+            assert _same(ours, R.read(p, names, types)), data
+        checked += 1
+    assert checked > 400
+
+
+def _chunk_file(suffix=b""):
+    """A CRLF file whose "\r\n" crosses the first parse-chunk end, one
+    record crosses the second, and a blank line's run crosses the third
+    chunk's end, which lies 10,000,000 bytes past that record's start.
+    Returns (bytes, data lines, header bytes)."""
+    C = O.CHUNK_BYTES
+    head = b"v,w\r\n" if suffix else b"v\r\n"
+    out = bytearray(head)
+    n = 0
+
+    def line(k):
+        nonlocal n
+        body = ((b"%d-" % n) + b"x" * 80)[:k - len(suffix)] + suffix
+        assert len(body) == k
+        out.extend(body + b"\r\n")
+        n += 1
+
+    while len(out) + 62 < C - 1:
+        line(58)
+    line(C - 1 - len(out))  # '\r' at C - 1, '\n' at C
+    while len(out) + 62 < 2 * C - 30:
+        line(58)
+    rs = len(out)
+    line(58)  # crosses 2C
+    assert rs < 2 * C < len(out) - 2
+    while len(out) + 62 < rs + C - 1:
+        line(58)
+    line(rs + C - 1 - len(out))
+    out.extend(b"\r\n")  # a blank line: the run crosses rs + C
+    for _ in range(1000):
+        line(40)
+    return bytes(out), n, head
+
+
+def test_chunk_boundary_records(loader, tmp_path):
+    """The reference's parser reads the file in 10,000,000-byte chunks; a
+    newline run crossing a chunk end yields an empty record there, which a
+    one-column table keeps as a NULL row (two here) and a two-column table
+    skips."""
+    data, n, head = _chunk_file()
+    p = tmp_path / "big.csv"
+    p.write_bytes(data)
+    cols, nulls = _read_file(loader, p, ["v"], [STRING])
+    assert len(cols[0]) == n + 2
+    # data rows before the first empty record: the lines ended before C - 1
+    # (the header's not counted) plus the line whose "\r\n" crosses C
+    before = data.count(b"\r\n", len(head), O.CHUNK_BYTES - 1) + 1
+    empty = list(np.nonzero(nulls[0])[0])
+    assert len(empty) == 2 and empty[0] == before
+    assert _same((cols, nulls), O.read(p, ["v"], [STRING]))
+    if R.available():
+        assert _same((cols, nulls), R.read(p, ["v"], [STRING]))
+    data2, n2, _ = _chunk_file(b",7")
+    q = tmp_path / "big2.csv"
+    q.write_bytes(data2)
+    c2, nl2 = _read_file(loader, q, ["w"], [INT])
+    assert len(c2[0]) == n2 and not nl2[0].any() and (c2[0] == 7).all()
+    if R.available():
+        assert _same((c2, nl2), R.read(q, ["w"], [INT]))
