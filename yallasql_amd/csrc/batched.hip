@@ -1,26 +1,24 @@
-// batched.hip — thousands of ragged sort / filter / aggregate jobs replayed
-// as one captured hipGraph.
+// batched.hip — thousands of ragged sort / filter / aggregate jobs in one
+// call.  Three forms, all giving the same bits:
 //
-// ysql_batch_create assigns the jobs to K independent chains (largest job
-// first onto the least-loaded chain; a job's cost is the HBM bytes it moves
-// plus a fixed per-launch cost), gives every chain its own workspace and
-// stream, and captures each chain's per-call launches between one fork event
-// and one join event into a single hipGraph.  ysql_batch_run replays it: one
-// host call for the whole job set, no per-launch host cost, and the chains'
-// kernels overlap on the device, so small jobs no longer run one at a time
-// on an otherwise idle GPU.
+// ysql_batch_run: ONE persistent kernel launch (k_persist, below) whose
+// workgroups take whole jobs and chunk tasks from device-side job lists and
+// a phase queue; no workgroup ever waits for another.
 //
-// Jobs up to a size limit per kind (the tuning's seg_*_max) do not get a
-// chain: each runs in ONE workgroup of a segmented kernel (one launch per
-// kind and key width for all of them), which walks the job's tiles in order
-// with running offsets in LDS and needs no communication between
-// workgroups.  Jobs are dealt to workgroups largest first.
+// ysql_batch_run_graph: a captured hipGraph.  Jobs up to a size limit per
+// kind (the tuning's seg_*_max) each run in ONE workgroup of a segmented
+// kernel (one launch per kind and key width for all of them), which walks
+// the job's tiles in order with running offsets in LDS.  Larger jobs are
+// assigned to K chains (largest first onto the least-loaded chain; a job's
+// cost is the HBM bytes it moves plus a fixed per-launch cost), each with
+// its own workspace and stream, and their per-call launches are captured
+// between one fork event and one join event.
 //
 // ysql_batch_run_plain issues the per-call entry points one job after
 // another on one stream (the operator-at-a-time pattern of the reference's
-// engine).  Every path produces the same bits: sorts and compactions are
-// stable (unique results), and the segmented aggregate folds the same
-// canonical tiles with the same device functions as the per-call kernel.
+// engine).  Sorts and compactions are stable (unique results), and every
+// aggregate path folds the same canonical tiles with the same device
+// functions as the per-call kernel, so all three forms agree bit for bit.
 #include <algorithm>
 #include <numeric>
 #include <vector>
