@@ -73,6 +73,36 @@ int ysql_csv_next(ysql_csv_reader *r, void *const *dev_cols, uint8_t *const *dev
 /* Waits for the reader's queued copies, frees its staging and closes the file. */
 void ysql_csv_close(ysql_csv_reader *r);
 
+/* ----------------------------------------------------------------------
+ * Result writer: replaces the reference's CsvWriter
+ * (include/engine/operators/csv_writer.hpp:12-160), fed by ExecutorEngine
+ * with every result batch (src/engine/executor_engine.cpp:19-58).  The
+ * first write puts the column names joined by ',' on the first line; each
+ * row is its fields joined by ',' and ended by '\n':
+ *   NULL      ""                          STRING    "text up to the NUL" (not escaped)
+ *   INT       decimal                     FLOAT     fixed, 4 decimals (std::to_chars)
+ *   DATETIME  "YYYY-MM-DD HH:MM:SS" (UTC, strftime of gmtime_r)
+ * Unlike the reference, floats whose fixed form is longer than 32 characters
+ * (|x| >= about 1e27) are written correctly, not as undefined buffer
+ * contents, and a string with no NUL in its 256 bytes is cut at 256.  Rows
+ * are formatted on up to 16 host threads and written in order. */
+typedef struct ysql_csv_writer ysql_csv_writer;
+/* Creates (truncates) `path`; names[c] / types[c] (YSQL_INT, YSQL_FLOAT,
+ * YSQL_DATETIME, YSQL_STRING) describe the result columns. */
+int ysql_csv_writer_open(const char *path, int ncols, const char *const *names, const int *types,
+                         ysql_csv_writer **out);
+/* Appends `rows` rows from host buffers; host_nulls (or any entry) may be
+ * NULL: no NULLs in that column. */
+int ysql_csv_writer_write_host(ysql_csv_writer *w, const void *const *host_cols,
+                               const uint8_t *const *host_nulls, uint64_t rows);
+/* Appends `rows` rows from device buffers: copied on `stream` into the
+ * writer's pinned staging (grown as needed), then formatted.  Returns after
+ * the rows are written (synchronises `stream`). */
+int ysql_csv_writer_write(ysql_csv_writer *w, const void *const *dev_cols, const uint8_t *const *dev_nulls,
+                          uint64_t rows, ysql_stream_t stream);
+/* Flushes and closes the file, frees the writer; YSQL_EINVAL if a write failed. */
+int ysql_csv_writer_close(ysql_csv_writer *w);
+
 #ifdef __cplusplus
 }
 #endif

@@ -86,6 +86,11 @@ void ysql_plan_destroy(ysql_plan *root);
 /* Pulls the root's next() until it returns 0 and copies every batch to host
  * memory.  `out` receives a result to free with ysql_result_destroy.      */
 int ysql_plan_execute(ysql_plan *root, ysql_result **out);
+/* Pulls the root's next() until it returns 0 and writes every batch to the
+ * CSV file `path` as it comes (ExecutorEngine::execute with its CsvWriter,
+ * src/engine/executor_engine.cpp:17-58; the format is ysql_csv_writer's,
+ * include/ysql_csv.h).  *rows_out (may be NULL) = rows written. */
+int ysql_plan_execute_csv(ysql_plan *root, const char *path, uint64_t *rows_out);
 int ysql_result_ncols(const ysql_result *r);
 uint64_t ysql_result_rows(const ysql_result *r);
 uint64_t ysql_result_batches(const ysql_result *r);

@@ -201,6 +201,49 @@ int ysql_merge_runs(int key_dtype, const void *keys, const uint32_t *vals, const
                     ysql_stream_t stream);
 
 /* ======================================================================
+ * STRING sort keys and merges
+ * Replace launch_radix_sort<AscOpString / DescOpString, String, ...>
+ * (radix_sort_kernel.hpp:45-73, radix_sort_kernel.cu:174,178), called from
+ * OrderOperator::sortLocallyBatch (order_operator.hpp:220,250), and
+ * launch_merge_sorted_array_kernel<String, AscSortStrOp / DescSortStrOp>
+ * (merge_batchs.hpp:36-51).  Order: strcmp_device's (string_kernel.hpp:46-56):
+ * unsigned bytes up to the first NUL, at most 256; stable (ties keep their
+ * order; DESC reverses the order, not the ties).  String columns are
+ * YSQL_STRING_BYTES-byte rows, 8-byte aligned.
+ *
+ * ysql_string_sort: perm_inout holds row indices of strs (each < n, e.g. a
+ * permutation); on return it is stably reordered by strs[perm_inout[i]].
+ * Sorting an earlier sort's permutation by another column this way is one
+ * LSD step of a multi-column ORDER BY.  The call reads the longest string's
+ * length back to the host once (it sets the number of 8-byte chunk passes),
+ * so it synchronises `stream`.  n < 2^30.
+ * ysql_string_argsort: perm_out = the stable sorting permutation of strs.  */
+int ysql_string_sort_ws_bytes(uint64_t n, size_t *bytes);
+int ysql_string_sort(int descending, const void *strs, uint32_t *perm_inout, uint64_t n, void *ws,
+                     size_t ws_bytes, ysql_stream_t stream);
+int ysql_string_argsort(int descending, const void *strs, uint32_t *perm_out, uint64_t n, void *ws,
+                        size_t ws_bytes, ysql_stream_t stream);
+/* One ORDER BY key column of two sorted runs: a (m rows) and b (n rows) of
+ * type YSQL_INT, YSQL_FLOAT, YSQL_DATETIME or YSQL_STRING.  Numeric columns
+ * compare by ysql_sortkey_encode's order (the order the sorts use). */
+typedef struct ysql_sortcol {
+  int dtype;
+  int descending;
+  const void *a;
+  const void *b;
+} ysql_sortcol;
+/* Stable merge of two runs sorted by nkeys (<= YSQL_MAX_COLS) key columns,
+ * compared lexicographically, ties from A first: the first k_max outputs as
+ * src_is_a[k] / src_idx[k] (ysql_merge_path's outputs, for ysql_gather2);
+ * *consumed_a_out (device, may be NULL) = A rows among them.  keys_out
+ * (may be NULL; one STRING key only, 16-byte aligned) receives the merged
+ * strings.  No workspace.  ysql_merge_path with dtype YSQL_STRING is the
+ * one-column form.                                                         */
+int ysql_merge_rows(const ysql_sortcol *keys, int nkeys, uint64_t m, uint64_t n, uint64_t k_max,
+                    uint32_t *src_idx, uint8_t *src_is_a, void *keys_out, int64_t *consumed_a_out,
+                    ysql_stream_t stream);
+
+/* ======================================================================
  * Inclusive scans (A4, A13)
  * Replace launch_prefix_sum<uint32_t> (prefix_sum.hpp:28-29 -> .cu:158-178)
  * and launch_prefix_sum_mask (prefix_sum.hpp:26 -> .cu:140-156): out[i] =
@@ -223,8 +266,10 @@ int ysql_scan_mask_inclusive(const uint8_t *mask, uint32_t *out, uint64_t n, voi
  * merge_batchs.cu:22, which duplicates and drops rows on ties — fixed here).
  * *consumed_a_out (device int64) = number of A elements among the outputs.
  * dtype: INT/FLOAT/DATETIME compare typed values (FLOAT with IEEE '<', as the
- * reference), UKEY32/UKEY64 compare encoded keys.  `descending` merges
- * descending sequences (DescSortMergeOp).                                 */
+ * reference), UKEY32/UKEY64 compare encoded keys, STRING compares with
+ * strcmp_device's order (C_opt then receives the 256-byte strings; A, B and
+ * C_opt 16-byte aligned).  `descending` merges descending sequences
+ * (DescSortMergeOp / DescSortStrOp).                                      */
 int ysql_merge_ws_bytes(uint64_t m, uint64_t n, uint64_t k_max, size_t *bytes);
 int ysql_merge_path(int dtype, int descending, const void *A, uint64_t m, const void *B,
                     uint64_t n, uint64_t k_max, void *C_opt, uint32_t *src_idx,

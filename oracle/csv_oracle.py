@@ -341,3 +341,41 @@ def read(path, names, types):
         else:
             out.append(np.array(cols[c], dtype={INT: np.int32, FLOAT: np.float32, DATETIME: np.int64}[t]))
     return out, [np.array(x, np.uint8) for x in nulls]
+
+
+def write_rows(names, types, cols, nulls=None, header=True):
+    """The bytes the reference's CsvWriter writes for one batch
+    (include/engine/operators/csv_writer.hpp:24-77): names joined by ',' on
+    the first line, then per row the fields joined by ',' and '\\n': NULL ->
+    "", STRING -> "text up to the NUL" (not escaped), INT -> decimal, FLOAT ->
+    fixed with 4 decimals, DATETIME -> "%Y-%m-%d %H:%M:%S" of the UTC time
+    (getDateTimeStr, include/enums/data_type_utils.hpp:46-58).  Formatted
+    with Python's own routines: '%.4f' of the float's exact value (correctly
+    rounded, as std::to_chars is), time.strftime of time.gmtime."""
+    import time
+
+    out = []
+    if header:
+        out.append(",".join(names).encode() + b"\n")
+    n = cols[0].shape[0] if cols else 0
+    for r in range(n):
+        fields = []
+        for c, t in enumerate(types):
+            if nulls is not None and nulls[c] is not None and nulls[c][r]:
+                fields.append(b'""')
+            elif t == STRING:
+                raw = bytes(cols[c][r])
+                cut = raw.find(b"\0")
+                fields.append(b'"' + (raw if cut < 0 else raw[:cut]) + b'"')
+            elif t == INT:
+                fields.append(str(int(cols[c][r])).encode())
+            elif t == FLOAT:
+                v = np.float32(cols[c][r])
+                if np.isnan(v):
+                    fields.append(b"-nan" if np.signbit(v) else b"nan")
+                else:
+                    fields.append(("%.4f" % float(v)).encode())
+            else:
+                fields.append(b'"' + time.strftime("%Y-%m-%d %H:%M:%S", time.gmtime(int(cols[c][r]))).encode() + b'"')
+        out.append(b",".join(fields) + b"\n")
+    return b"".join(out)

@@ -60,6 +60,7 @@ _SIG = {
     "orc_cross_expand": ([C.c_uint32, P, P, U64, U64, I, P, P], I),
     "orc_star_join_group_sum": ([C.POINTER(StarDim), I, P, P, U64, P, P, C.c_uint32], I),
     "orc_cpu_multirun_sort_u64": ([P, U64, U64, I, P, P], I),
+    "orc_sort_rows": ([I, C.POINTER(I), C.POINTER(I), C.POINTER(P), U64, P], I),
 }
 for _n, (_a, _r) in _SIG.items():
     f = getattr(lib, _n)
@@ -106,9 +107,9 @@ def stable_argsort(keys, desc=False):
 def merge(dtype_c, desc, A, B, k_max=None, want_keys=True):
     A = np.ascontiguousarray(A)
     B = np.ascontiguousarray(B)
-    m, n = A.size, B.size
+    m, n = A.shape[0], B.shape[0]
     kmax = m + n if k_max is None else min(k_max, m + n)
-    Cc = np.empty(kmax, dtype=A.dtype) if want_keys else None
+    Cc = np.empty((kmax,) + A.shape[1:], dtype=A.dtype) if want_keys else None
     idx = np.empty(kmax, dtype=np.uint32)
     isa = np.empty(kmax, dtype=np.uint8)
     cons = np.zeros(1, dtype=np.int64)
@@ -282,3 +283,15 @@ def cpu_multirun_sort_u64(keys, run_rows, threads=1):
     return ok, ov
 
 LT, LE, GT, GE, EQ, NE = 0, 1, 2, 3, 4, 5
+
+
+def sort_rows(cols, desc):
+    """Stable ORDER BY permutation over key columns (int32, float32, int64
+    arrays or (n, 256) uint8 string arrays), desc[c] per column."""
+    cols = [np.ascontiguousarray(c) for c in cols]
+    n = cols[0].shape[0]
+    k = len(cols)
+    perm = np.empty(n, dtype=np.uint32)
+    ptrs = (P * k)(*[c.ctypes.data_as(P) for c in cols])
+    _chk(lib.orc_sort_rows(k, (I * k)(*[dt(c) for c in cols]), (I * k)(*[int(d) for d in desc]), ptrs, n, _p(perm)))
+    return perm

@@ -101,6 +101,8 @@ extern "C" int orc_stable_argsort(int key_dtype, int desc, const void *keys, uin
 
 /* Typed "a <= b" of the merge ops (merge_batchs.hpp:20-34): AscSortMergeOp
  * is_less_or_equal = a <= b, DescSortMergeOp = a >= b. */
+static int str_cmp(const char *a, const char *b);
+
 static inline bool merge_le(int dtype, int desc, const void *A, uint64_t i, const void *B, uint64_t j) {
   switch (dtype) {
     case O_INT: {
@@ -122,6 +124,10 @@ static inline bool merge_le(int dtype, int desc, const void *A, uint64_t i, cons
     case O_UKEY64: {
       uint64_t a = ((const uint64_t *)A)[i], b = ((const uint64_t *)B)[j];
       return desc ? a >= b : a <= b;
+    }
+    case O_STRING: { /* AscSortStrOp / DescSortStrOp, merge_batchs.hpp:36-48 */
+      int c = str_cmp((const char *)A + i * STR_BYTES, (const char *)B + j * STR_BYTES);
+      return desc ? c >= 0 : c <= 0;
     }
   }
   return true;
@@ -162,6 +168,45 @@ extern "C" int orc_merge(int dtype, int desc, const void *A, uint64_t m, const v
     }
   }
   *consumed_a = (int64_t)i;
+  return 0;
+}
+
+/* ORDER BY over key columns, strings included (OrderOperator,
+ * order_operator.hpp:196-265; AscOpString / DescOpString,
+ * radix_sort_kernel.hpp:45-73, ordering strings as strcmp_device does,
+ * string_kernel.hpp:46-56; the reference sorted by orders[0] only,
+ * order_operator.hpp:277, and the multi-column rule is the intended one):
+ * a stable sort of row indices comparing the columns in order, numeric
+ * columns by their sort-key encoding (orc_sortkey_encode without the DESC
+ * complement), strings by str_cmp, a descending column's result negated. */
+extern "C" int orc_sort_rows(int nkeys, const int *dtypes, const int *desc, const void *const *cols, uint64_t n,
+                             uint32_t *perm) {
+  std::vector<std::vector<uint64_t>> enc((size_t)nkeys);
+  for (int c = 0; c < nkeys; c++) {
+    if (dtypes[c] == O_STRING) continue;
+    if (dtypes[c] != O_INT && dtypes[c] != O_FLOAT && dtypes[c] != O_DATETIME) return -1;
+    enc[c].resize(n);
+    if (dtypes[c] == O_DATETIME) {
+      orc_sortkey_encode(dtypes[c], 0, cols[c], enc[c].data(), n);
+    } else {
+      std::vector<uint32_t> k(n);
+      orc_sortkey_encode(dtypes[c], 0, cols[c], k.data(), n);
+      for (uint64_t i = 0; i < n; i++) enc[c][i] = k[i];
+    }
+  }
+  for (uint64_t i = 0; i < n; i++) perm[i] = (uint32_t)i;
+  std::stable_sort(perm, perm + n, [&](uint32_t x, uint32_t y) {
+    for (int c = 0; c < nkeys; c++) {
+      int r;
+      if (dtypes[c] == O_STRING) {
+        r = str_cmp((const char *)cols[c] + (uint64_t)x * STR_BYTES, (const char *)cols[c] + (uint64_t)y * STR_BYTES);
+      } else {
+        r = enc[c][x] < enc[c][y] ? -1 : (enc[c][x] > enc[c][y] ? 1 : 0);
+      }
+      if (r) return desc[c] ? r > 0 : r < 0;
+    }
+    return false;
+  });
   return 0;
 }
 

@@ -86,3 +86,19 @@ def test_reference_shaped_launchers_compile():
     r = subprocess.run([hipcc, "-fsyntax-only", "-std=c++17", "-Wall", "-Werror", "-Wno-unused-command-line-argument", "-I", f"{ROOT}/include", src],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr[-3000:]
+
+
+def test_string_entry_points_reject_bad_arguments():
+    from yallasql_amd import _lib as L
+
+    ws = C.c_void_p(1 << 20)
+    # strings must be 8-byte aligned; a NULL permutation is refused
+    assert L.lib.ysql_string_argsort(0, C.c_void_p(12), C.c_void_p(256), 10, ws, 1 << 30, None) == L.EINVAL
+    assert L.lib.ysql_string_argsort(0, C.c_void_p(256), None, 10, ws, 1 << 30, None) == L.EINVAL
+    assert L.lib.ysql_string_sort(0, C.c_void_p(256), C.c_void_p(512), 10, ws, 16, None) == L.EWORKSPACE
+    assert L.lib.ysql_string_sort_ws_bytes(1 << 30, C.byref(C.c_size_t())) == L.EINVAL
+    keys = (L.SortCol * 1)(L.SortCol(L.UKEY32, 0, 256, 512))  # encoded keys are not a row key type
+    assert L.lib.ysql_merge_rows(keys, 1, 10, 10, 20, C.c_void_p(8), C.c_void_p(8), None, None, None) == L.EINVAL
+    keys = (L.SortCol * 1)(L.SortCol(L.STRING, 0, 260, 512))  # misaligned column
+    assert L.lib.ysql_merge_rows(keys, 1, 10, 10, 20, C.c_void_p(8), C.c_void_p(8), None, None, None) == L.EINVAL
+    assert L.lib.ysql_merge_rows(keys, 0, 10, 10, 20, C.c_void_p(8), C.c_void_p(8), None, None, None) == L.EINVAL

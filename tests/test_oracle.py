@@ -287,3 +287,73 @@ def test_cpu_multirun_sort_is_global_stable_sort():
         ok, ov = O.cpu_multirun_sort_u64(keys, 4096, threads)
         order = np.argsort(keys, kind="stable")
         assert np.array_equal(ov, order) and np.array_equal(ok, keys[order])
+
+
+def _strings(r, n, alphabet=b"ab\x80\xffz", maxlen=20):
+    """(n, 256) NUL-padded strings over a small alphabet (shared prefixes,
+    duplicates, bytes above 0x7f), and their texts."""
+    s = np.zeros((n, 256), np.uint8)
+    texts = []
+    for i in range(n):
+        k = int(r.integers(0, maxlen + 1))
+        t = bytes(r.choice(list(alphabet), k).tolist()) if k else b""
+        s[i, :k] = list(t)
+        texts.append(t)
+    return s, texts
+
+
+def test_sort_rows_strings_match_python_sorted():
+    """orc_sort_rows on one STRING column equals Python's stable sorted()
+    over the bytes up to the first NUL (unsigned lexicographic = strcmp),
+    ascending and descending (reverse=True keeps ties in input order)."""
+    from oracle import pyoracle as P
+
+    r = np.random.default_rng(71)
+    s, texts = _strings(r, 3000)
+    s[5, 3:] = 0
+    s[5, :3] = list(b"ab\0")[:3]
+    s[6, :] = 0x61  # no NUL in 256 bytes: strcmp compares all of them
+    s[7, :4] = list(b"ab\0z")  # a byte after the NUL is ignored by strcmp
+    texts = [bytes(row).split(b"\0")[0] for row in s]
+    for desc in (False, True):
+        want = sorted(range(len(texts)), key=lambda i: texts[i], reverse=desc)
+        assert list(P.sort_rows([s], [desc])) == want
+
+
+def test_sort_rows_mixed_keys_match_tuple_sort():
+    """Several key columns: string ASC, int DESC, float ASC, datetime DESC
+    against Python's stable sorted() on key tuples (a wrong column order,
+    direction or tie rule fails)."""
+    from oracle import pyoracle as P
+
+    r = np.random.default_rng(72)
+    n = 4000
+    s, texts = _strings(r, n, alphabet=b"ab", maxlen=3)
+    iv = r.integers(-3, 3, n).astype(np.int32)
+    fv = r.choice(np.array([-1.5, -0.25, 0.5, 2.0], np.float32), n)
+    dv = r.integers(-2, 2, n).astype(np.int64)
+    perm = P.sort_rows([s, iv, fv, dv], [0, 1, 0, 1])
+    want = sorted(range(n), key=lambda i: (texts[i], -int(iv[i]), float(fv[i]), -int(dv[i])))
+    assert list(perm) == want
+
+
+def test_merge_strings_is_stable_sort_of_concatenation():
+    """orc_merge on STRING runs equals the stable sort of A then B (ties
+    from A first), checked against Python's sorted()."""
+    from oracle import pyoracle as P
+
+    r = np.random.default_rng(73)
+    for desc in (False, True):
+        a, ta = _strings(r, 700, maxlen=4)
+        b, tb = _strings(r, 500, maxlen=4)
+        oa = sorted(range(700), key=lambda i: ta[i], reverse=desc)
+        ob = sorted(range(500), key=lambda i: tb[i], reverse=desc)
+        a, ta = a[oa], [ta[i] for i in oa]
+        b, tb = b[ob], [tb[i] for i in ob]
+        C_, idx, isa, cons = P.merge(P.STRING, desc, a, b)
+        got = [ta[i] if x else tb[i] for i, x in zip(idx, isa)]
+        tags = [(0, i) if x else (1, i) for i, x in zip(idx, isa)]
+        allt = [(t, (0, i)) for i, t in enumerate(ta)] + [(t, (1, i)) for i, t in enumerate(tb)]
+        want = sorted(allt, key=lambda x: x[0], reverse=desc)
+        assert got == [w[0] for w in want] and tags == [w[1] for w in want]
+        assert cons == 700

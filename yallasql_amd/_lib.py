@@ -46,6 +46,10 @@ class Col(C.Structure):
     ]
 
 
+class SortCol(C.Structure):
+    _fields_ = [("dtype", C.c_int32), ("descending", C.c_int32), ("a", C.c_void_p), ("b", C.c_void_p)]
+
+
 class StarDim(C.Structure):
     _fields_ = [
         ("fk", C.c_void_p), ("rows", C.c_uint32), ("group_card", C.c_uint32), ("pass_", C.c_void_p),
@@ -78,6 +82,14 @@ _SIGS = {
     "ysql_merge_path": [I32, I32, P, U64, P, U64, U64, P, P, P, P, P, C.c_size_t, P],
     "ysql_merge_runs_ws_bytes": [I32, C.POINTER(U64), I32, SZP],
     "ysql_merge_runs": [I32, P, P, C.POINTER(U64), I32, P, P, P, C.c_size_t, P],
+    "ysql_csv_writer_open": [C.c_char_p, I32, C.POINTER(C.c_char_p), C.POINTER(C.c_int), C.POINTER(C.c_void_p)],
+    "ysql_csv_writer_write_host": [P, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), U64],
+    "ysql_csv_writer_write": [P, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), U64, P],
+    "ysql_csv_writer_close": [P],
+    "ysql_string_sort_ws_bytes": [U64, SZP],
+    "ysql_string_sort": [I32, P, P, U64, P, C.c_size_t, P],
+    "ysql_string_argsort": [I32, P, P, U64, P, C.c_size_t, P],
+    "ysql_merge_rows": [C.POINTER(SortCol), I32, U64, U64, U64, P, P, P, P, P],
     "ysql_merge_pairs": [I32, I32, P, P, U64, P, P, U64, P, P, P, C.c_size_t, P],
     "ysql_gather": [U32, P, P, P, U64, P, P, P],
     "ysql_gather2": [U32, P, P, P, P, P, P, U64, I32, P, P, P],

@@ -371,12 +371,49 @@ OrderOperator::OrderOperator(ExecContext &ctx, OpPtr child, std::vector<OrderKey
   for (auto &k : keys_) {
     if (k.col < 0 || (size_t)k.col >= columns.size()) throw EngineError("ORDER BY column out of range");
     const DataType t = columns[k.col]->type;
-    if (t == DataType::STRING) throw EngineError("ORDER BY on strings is not supported");
+    if (t == DataType::STRING) by_rows_ = true;
     key_bits_ += t == DataType::DATETIME ? 64 : 32;
   }
-  if (key_bits_ > 64) throw EngineError("ORDER BY keys wider than 64 bits");
+  if (keys_.size() > YSQL_MAX_COLS) throw EngineError("at most 8 ORDER BY columns");
+  // string keys, or numeric keys wider than one 64-bit composite: sorted
+  // column by column (last key first, each sort stable) and merged by
+  // comparing the key rows themselves
+  if (key_bits_ > 64) by_rows_ = true;
   wide_ = key_bits_ > 32;
   children.push_back(std::move(child));
+}
+
+// One stable sort step of the column-by-column form: perm (row indices of
+// b) reordered by column `key` of the rows it names.  `first`: perm is not
+// yet set (the step writes the sorting permutation of the column itself).
+void OrderOperator::sortStep(const Batch &b, const OrderKey &key, BufPtr &perm, bool first, void *ws, size_t wsb) {
+  const size_t n = b.batchSize;
+  hipStream_t s = ctx.stream();
+  const DataType t = columns[key.col]->type;
+  if (t == DataType::STRING) {
+    if (first)
+      abiCheck(ysql_string_argsort(key.desc, b.col(key.col), (uint32_t *)perm->ptr, n, ws, wsb, s),
+               "ysql_string_argsort");
+    else
+      abiCheck(ysql_string_sort(key.desc, b.col(key.col), (uint32_t *)perm->ptr, n, ws, wsb, s), "ysql_string_sort");
+    return;
+  }
+  const uint32_t eb = b.columns[key.col]->bytes;
+  const int kdt = eb == 8 ? YSQL_UKEY64 : YSQL_UKEY32;
+  const void *src = b.col(key.col);
+  BufPtr tmp;
+  if (!first) {  // the column in the current order
+    tmp = newBuf(n * eb, s);
+    abiCheck(ysql_gather(eb, src, nullptr, (const uint32_t *)perm->ptr, n, tmp->ptr, nullptr, s), "ysql_gather");
+    src = tmp->ptr;
+  }
+  auto k = newBuf(n * eb, s), k2 = newBuf(n * eb, s);
+  abiCheck(ysql_sortkey_encode(abiType(t), key.desc, src, k->ptr, n, s), "ysql_sortkey_encode");
+  auto out = newBuf(n * 4, s);
+  abiCheck(ysql_radix_sort_ex(kdt, 0, k->ptr, first ? nullptr : (const uint32_t *)perm->ptr, 0, k2->ptr,
+                              (uint32_t *)out->ptr, n, 0, (int)eb * 8, ws, wsb, s),
+           "ysql_radix_sort_ex");
+  perm = out;
 }
 
 BufPtr OrderOperator::compositeKeys(const Batch &b) {
@@ -400,14 +437,27 @@ OrderOperator::Run OrderOperator::sortBatch(const Batch &b) {
   const size_t n = b.batchSize;
   hipStream_t s = ctx.stream();
   Run r;
-  r.keys = compositeKeys(b);
   auto perm = newBuf(n * 4, s);
-  const int kdt = wide_ ? YSQL_UKEY64 : YSQL_UKEY32;
-  size_t wsb = 0;
-  abiCheck(ysql_radix_sort_ws_bytes(kdt, n, &wsb), "ysql_radix_sort_ws_bytes");
-  // the packed key occupies its low key_bits_ bits
-  abiCheck(ysql_radix_argsort(kdt, 0, r.keys->ptr, (uint32_t *)perm->ptr, n, 0, key_bits_, ctx.workspace(wsb), wsb, s),
-           "ysql_radix_argsort");
+  if (by_rows_) {
+    size_t wsb = 0, w = 0;
+    abiCheck(ysql_radix_sort_ws_bytes(YSQL_UKEY64, n, &w), "ysql_radix_sort_ws_bytes");
+    wsb = w;
+    if (n > 1) {
+      abiCheck(ysql_string_sort_ws_bytes(n, &w), "ysql_string_sort_ws_bytes");
+      wsb = std::max(wsb, w);
+    }
+    void *ws = ctx.workspace(wsb);
+    for (size_t q = keys_.size(); q-- > 0;) sortStep(b, keys_[q], perm, q + 1 == keys_.size(), ws, wsb);
+  } else {
+    r.keys = compositeKeys(b);
+    const int kdt = wide_ ? YSQL_UKEY64 : YSQL_UKEY32;
+    size_t wsb = 0;
+    abiCheck(ysql_radix_sort_ws_bytes(kdt, n, &wsb), "ysql_radix_sort_ws_bytes");
+    // the packed key occupies its low key_bits_ bits
+    abiCheck(ysql_radix_argsort(kdt, 0, r.keys->ptr, (uint32_t *)perm->ptr, n, 0, key_bits_, ctx.workspace(wsb), wsb,
+                                s),
+             "ysql_radix_argsort");
+  }
   auto out = std::make_shared<Batch>();
   out->columns = b.columns;
   out->stream = s;
@@ -427,15 +477,25 @@ OrderOperator::Run OrderOperator::sortBatch(const Batch &b) {
 OrderOperator::Run OrderOperator::mergeRuns(const Run &a, const Run &b) {
   hipStream_t s = ctx.stream();
   const size_t m = a.batch->batchSize, n = b.batch->batchSize, k = m + n;
-  const int kdt = wide_ ? YSQL_UKEY64 : YSQL_UKEY32;
   Run r;
-  r.keys = newBuf(k * (wide_ ? 8 : 4), s);
   auto idx = newBuf(k * 4, s), isa = newBuf(k, s);
-  size_t wsb = 0;
-  abiCheck(ysql_merge_ws_bytes(m, n, k, &wsb), "ysql_merge_ws_bytes");
-  abiCheck(ysql_merge_path(kdt, 0, a.keys->ptr, m, b.keys->ptr, n, k, r.keys->ptr, (uint32_t *)idx->ptr,
-                           (uint8_t *)isa->ptr, (int64_t *)ctx.deviceCounter(), ctx.workspace(wsb), wsb, s),
-           "ysql_merge_path");
+  if (by_rows_) {  // the runs' key columns compared row against row
+    ysql_sortcol sc[YSQL_MAX_COLS];
+    for (size_t q = 0; q < keys_.size(); q++)
+      sc[q] = ysql_sortcol{abiType(columns[keys_[q].col]->type), keys_[q].desc ? 1 : 0, a.batch->col(keys_[q].col),
+                           b.batch->col(keys_[q].col)};
+    abiCheck(ysql_merge_rows(sc, (int)keys_.size(), m, n, k, (uint32_t *)idx->ptr, (uint8_t *)isa->ptr, nullptr,
+                             nullptr, s),
+             "ysql_merge_rows");
+  } else {
+    const int kdt = wide_ ? YSQL_UKEY64 : YSQL_UKEY32;
+    r.keys = newBuf(k * (wide_ ? 8 : 4), s);
+    size_t wsb = 0;
+    abiCheck(ysql_merge_ws_bytes(m, n, k, &wsb), "ysql_merge_ws_bytes");
+    abiCheck(ysql_merge_path(kdt, 0, a.keys->ptr, m, b.keys->ptr, n, k, r.keys->ptr, (uint32_t *)idx->ptr,
+                             (uint8_t *)isa->ptr, (int64_t *)ctx.deviceCounter(), ctx.workspace(wsb), wsb, s),
+             "ysql_merge_path");
+  }
   auto out = std::make_shared<Batch>();
   out->columns = a.batch->columns;
   out->stream = s;

@@ -293,15 +293,17 @@ class OrderOperator : public Operator {
  private:
   struct Run {
     std::shared_ptr<Batch> batch;  // sorted rows
-    BufPtr keys;                   // sorted composite keys (u32 or u64)
+    BufPtr keys;                   // sorted composite keys (u32 or u64); none column by column
   };
   void firstPass(CacheManager &cache);
   Run sortBatch(const Batch &b);
   Run mergeRuns(const Run &a, const Run &b);
   BufPtr compositeKeys(const Batch &b);
+  void sortStep(const Batch &b, const OrderKey &key, BufPtr &perm, bool first, void *ws, size_t wsb);
   std::vector<OrderKey> keys_;
   int key_bits_ = 0;
-  bool wide_ = false;  // composite key is 64 bits
+  bool wide_ = false;     // composite key is 64 bits
+  bool by_rows_ = false;  // string keys or keys wider than 64 bits: column by column
   bool done_first_ = false;
   std::vector<Run> runs_;
   size_t max_batch_ = 0, emit_pos_ = 0;

@@ -2,6 +2,7 @@
 #include <cstring>
 
 #include "engine.hpp"
+#include "ysql_csv.h"
 #include "ysql_engine.h"
 
 using namespace YallaSQL;
@@ -248,6 +249,62 @@ int ysql_plan_execute(ysql_plan *root, ysql_result **out) {
     return e.code ? e.code : YSQL_EINVAL;
   } catch (const std::exception &e) {
     g_err = e.what();
+    return YSQL_EINVAL;
+  }
+}
+
+int ysql_plan_execute_csv(ysql_plan *root, const char *path, uint64_t *rows_out) {
+  if (!root || !path) return YSQL_EINVAL;
+  ysql_csv_writer *w = nullptr;
+  try {
+    Operator &op = *root->op;
+    ExecContext &ctx = root->owner->ctx;
+    const size_t nc = op.columns.size();
+    std::vector<std::string> names(nc);
+    std::vector<const char *> np(nc);
+    std::vector<int> types(nc);
+    for (size_t c = 0; c < nc; c++) {
+      names[c] = op.columns[c]->name;
+      np[c] = names[c].c_str();
+      types[c] = (int)op.columns[c]->type;
+    }
+    int rc = ysql_csv_writer_open(path, (int)nc, np.data(), types.data(), &w);
+    if (rc) {
+      g_err = ysql_last_error();
+      return rc;
+    }
+    op.init();
+    CacheManager &cache = ctx.cache();
+    uint64_t rows = 0;
+    std::vector<const void *> cols(nc);
+    std::vector<const uint8_t *> nulls(nc);
+    while (true) {
+      const BatchID id = op.next(cache);
+      if (!id) break;
+      auto b = cache.getBatch(id);
+      for (size_t c = 0; c < nc; c++) {
+        cols[c] = b->col(c);
+        nulls[c] = b->nul(c);
+      }
+      rc = ysql_csv_writer_write(w, cols.data(), nulls.data(), b->batchSize, ctx.stream());
+      if (rc) {
+        g_err = ysql_last_error();
+        (void)ysql_csv_writer_close(w);
+        return rc;
+      }
+      rows += b->batchSize;
+    }
+    rc = ysql_csv_writer_close(w);
+    w = nullptr;
+    if (rows_out) *rows_out = rows;
+    return rc;
+  } catch (const EngineError &e) {
+    g_err = e.what();
+    if (w) (void)ysql_csv_writer_close(w);
+    return e.code ? e.code : YSQL_EINVAL;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    if (w) (void)ysql_csv_writer_close(w);
     return YSQL_EINVAL;
   }
 }

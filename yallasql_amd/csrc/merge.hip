@@ -170,6 +170,10 @@ static int merge_dir(int desc, const T *A, const uint32_t *av, uint64_t m, const
 
 __global__ void k_store_i64(int64_t *p, int64_t v) { *p = v; }
 
+// STRING merge path (strings.hip).
+int merge_path_strings(int descending, const void *A, uint64_t m, const void *B, uint64_t n, uint64_t kmax, void *C_opt,
+                       uint32_t *src_idx, uint8_t *src_is_a, int64_t *consumed_a_out, hipStream_t s);
+
 // ---------------------------------------------------------------------------
 // k-way merge of R <= 16 sorted runs in one pass over the data (the
 // reference merges runs pairwise, level by level: every row moves
@@ -510,6 +514,8 @@ int ysql_merge_path(int dtype, int descending, const void *A, uint64_t m, const 
     case YSQL_UKEY64:
       return merge_dir<uint64_t, 1>(descending, (const uint64_t *)A, nullptr, m, (const uint64_t *)B, nullptr, n, kmax,
                                     (uint64_t *)C_opt, src_idx, src_is_a, consumed_a_out, ws, ws_bytes, s);
+    case YSQL_STRING:
+      return merge_path_strings(descending, A, m, B, n, kmax, C_opt, src_idx, src_is_a, consumed_a_out, s);
   }
   return YSQL_EINVAL;
 }

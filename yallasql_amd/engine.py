@@ -39,6 +39,7 @@ _sig = {
     "ysql_plan_filter": ([P, P], P),
     "ysql_plan_project": ([P, C.POINTER(P), C.c_int], P),
     "ysql_plan_order": ([P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int], P),
+    "ysql_plan_execute_csv": ([P, C.c_char_p, C.POINTER(C.c_uint64)], C.c_int),
     "ysql_plan_aggregate": ([P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int),
                              C.POINTER(C.c_int32), C.POINTER(C.c_uint32), C.c_int], P),
     "ysql_plan_join": ([P, P, C.c_int, C.c_int, C.c_int], P),
@@ -154,6 +155,16 @@ class Plan:
 
     def cross(self, right):
         return Plan(self.ctx, lib.ysql_plan_cross(self._take(), right._take()))
+
+    def execute_csv(self, path):
+        """Runs the plan to exhaustion, writing each result batch to the CSV
+        file `path` as it comes (the reference's CsvWriter format); returns
+        the number of rows written."""
+        rows = C.c_uint64()
+        rc = lib.ysql_plan_execute_csv(self.h, str(path).encode(), C.byref(rows))
+        if rc != 0:
+            raise EngineError(f"execute_csv returned {rc}: {lib.ysql_engine_last_error().decode()}")
+        return rows.value
 
     def execute(self):
         """Runs the plan to exhaustion; returns (columns, nulls, batches) as numpy arrays."""

@@ -81,3 +81,48 @@ class CsvReader:
             self.close()
         except Exception:
             pass
+
+
+class CsvWriter:
+    """Result batches -> CSV file in the reference's CsvWriter format
+    (include/engine/operators/csv_writer.hpp:24-77) over ysql_csv_writer_*."""
+
+    def __init__(self, path, names, types):
+        self.types = list(types)
+        n = len(names)
+        h = C.c_void_p()
+        L.call("ysql_csv_writer_open", str(path).encode(), n, (C.c_char_p * n)(*[x.encode() for x in names]),
+               (C.c_int * n)(*self.types), C.byref(h))
+        self.handle = h
+
+    def write_host(self, cols, nulls=None, rows=None):
+        """cols: numpy arrays ((rows, 256) uint8 for strings); nulls: uint8 arrays or None."""
+        n = len(cols)
+        cols = [np.ascontiguousarray(c) for c in cols]
+        nulls = None if nulls is None else [None if x is None else np.ascontiguousarray(x, np.uint8) for x in nulls]
+        rows = cols[0].shape[0] if rows is None else rows
+        pc = (C.c_void_p * n)(*[c.ctypes.data for c in cols])
+        pn = None if nulls is None else (C.c_void_p * n)(*[None if x is None else x.ctypes.data for x in nulls])
+        L.call("ysql_csv_writer_write_host", self.handle, pc, pn, rows)
+
+    def write_device(self, cols, nulls=None, stream=None):
+        """cols: device tensors; nulls: device uint8 tensors or None."""
+        import torch
+
+        n = len(cols)
+        rows = cols[0].shape[0]
+        pc = (C.c_void_p * n)(*[c.data_ptr() for c in cols])
+        pn = None if nulls is None else (C.c_void_p * n)(*[None if x is None else x.data_ptr() for x in nulls])
+        s = stream if stream is not None else torch.cuda.current_stream(cols[0].device)
+        L.call("ysql_csv_writer_write", self.handle, pc, pn, rows, C.c_void_p(s.cuda_stream))
+
+    def close(self):
+        if getattr(self, "handle", None) is not None:
+            h, self.handle = self.handle, None
+            L.call("ysql_csv_writer_close", h)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

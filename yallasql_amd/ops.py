@@ -117,10 +117,13 @@ def scan_mask_inclusive(mask):
 
 # ---- A8-A10 -------------------------------------------------------------
 def merge_path(dtype, descending, A, B, k_max=None, want_keys=True):
-    m, n = A.numel(), B.numel()
+    """Merge path of sorted A and B; STRING runs are (rows, 256) uint8."""
+    m, n = _rows(A), _rows(B)
     kmax = m + n if k_max is None else min(k_max, m + n)
     dev = A.device
-    C_ = torch.empty(kmax, dtype=A.dtype, device=dev) if want_keys else None
+    C_ = None
+    if want_keys:
+        C_ = torch.empty((kmax,) + tuple(A.shape[1:]), dtype=A.dtype, device=dev)
     idx = torch.empty(kmax, dtype=torch.int32, device=dev)
     isa = torch.empty(kmax, dtype=torch.uint8, device=dev)
     cons = torch.zeros(1, dtype=torch.int64, device=dev)
@@ -128,6 +131,39 @@ def merge_path(dtype, descending, A, B, k_max=None, want_keys=True):
     call("ysql_merge_path", dtype, int(descending), _p(A), m, _p(B), n, kmax, _p(C_), _p(idx), _p(isa), _p(cons),
          _p(ws), ws.numel(), _s())
     return C_, idx, isa, cons
+
+
+def string_argsort(strs, descending=False):
+    """Stable strcmp-order permutation of a (rows, 256) uint8 string column."""
+    n = strs.shape[0]
+    perm = torch.empty(n, dtype=torch.int32, device=strs.device)
+    ws = workspace(ws_bytes("ysql_string_sort_ws_bytes", n) if n > 1 else 0, strs.device)
+    call("ysql_string_argsort", int(descending), _p(strs), _p(perm), n, _p(ws), ws.numel(), _s())
+    return perm
+
+
+def string_sort(strs, perm, descending=False):
+    """perm (int32 row indices of strs) reordered stably by strs[perm], in place."""
+    n = strs.shape[0]
+    assert perm.numel() == n
+    ws = workspace(ws_bytes("ysql_string_sort_ws_bytes", n) if n > 1 else 0, strs.device)
+    call("ysql_string_sort", int(descending), _p(strs), _p(perm), n, _p(ws), ws.numel(), _s())
+    return perm
+
+
+def merge_rows(keys, m, n, k_max=None, want_keys=False):
+    """Stable merge of two runs sorted by key columns: keys = [(dtype code,
+    descending, a column, b column)].  Returns (src_idx, src_is_a, consumed_a,
+    merged strings or None)."""
+    kmax = m + n if k_max is None else min(k_max, m + n)
+    dev = keys[0][2].device
+    arr = (L.SortCol * len(keys))(*[L.SortCol(t, int(d), a.data_ptr(), b.data_ptr()) for t, d, a, b in keys])
+    idx = torch.empty(kmax, dtype=torch.int32, device=dev)
+    isa = torch.empty(kmax, dtype=torch.uint8, device=dev)
+    cons = torch.zeros(1, dtype=torch.int64, device=dev)
+    out = torch.empty((kmax, L.STRING_BYTES), dtype=torch.uint8, device=dev) if want_keys else None
+    call("ysql_merge_rows", arr, len(keys), m, n, kmax, _p(idx), _p(isa), _p(out), _p(cons), _s())
+    return idx, isa, cons, out
 
 
 def merge_pairs(a_keys, a_vals, b_keys, b_vals, descending=False, out_keys=None, out_vals=None, ws=None):
