@@ -47,7 +47,8 @@ def parse():
     ap.add_argument("--no-batched", action="store_true", help="skip the 4,096-job batched vs plain comparison")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving loop")
     ap.add_argument("--no-configs", action="store_true", help="skip the config-1 and config-3 lines")
-    ap.add_argument("--serving-seconds", type=float, default=2.5, help="arrival window per rate")
+    ap.add_argument("--serving-queries", type=int, default=10_000, help="queries per rate, at most")
+    ap.add_argument("--serving-seconds", type=float, default=60.0, help="arrival window per rate, at most")
     ap.add_argument("--pairwise-merge", action="store_true",
                     help="merge the runs pairwise, level by level, instead of one k-way merge")
     return ap.parse_args()
@@ -60,25 +61,105 @@ def gen_keys(n, dev):
     return multirun_keys(n, dev, seed=SEED)
 
 
-def cpu_baseline(keys_host, run_rows):
-    """Oracle multi-run sort (same pipeline) on the first two runs, all host threads."""
+CPU_SAMPLE_ROWS = 50_000_000  # rows of the 200M-row table the CPU op baselines run on
+CPU_JOIN_ROWS = 20_000_000
+CPU_STAR_ROWS = 60_000_000
+
+
+def _cpu_threads():
+    """(std::thread::hardware_concurrency(), threads used): all the host's
+    hardware threads, capped by OMP_NUM_THREADS where it is set (the CPU share
+    a GPU gets on a shared box)."""
     from oracle import pyoracle as O
 
-    threads = min(16, os.cpu_count() or 1)
-    sample = keys_host[: 2 * run_rows]
-    best = None
+    hc = O.cpb_hardware_concurrency()
+    cap = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or hc
+    return hc, max(1, min(hc, cap))
+
+
+def _cpu_model():
+    import subprocess
+
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                return line.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def _cpu_median(fn):
+    """2 untimed warm-up runs, then the median of 5 timed runs (seconds)."""
     for _ in range(2):
+        fn()
+    ts = []
+    for _ in range(5):
         t0 = time.perf_counter()
-        O.cpu_multirun_sort_u64(sample, run_rows, threads)
-        dt = time.perf_counter() - t0
-        best = dt if best is None else min(best, dt)
+        fn()
+        ts.append(time.perf_counter() - t0)
+    ts.sort()
+    return ts[2]
+
+
+def cpu_baseline(inp):
+    """The oracle's multi-threaded CPU versions (oracle/cpu_baseline.cpp,
+    -O3 -march=native) on bounded samples of the same inputs, at 1 thread
+    and at all permitted threads; 2 warm-ups + median of 5 each."""
+    from oracle import pyoracle as O
+
+    hc, T = _cpu_threads()
+    ops = {}
+
+    def both(name, rows, fn, sample, rows_1=None, fn_1=None):
+        t1 = _cpu_median(lambda: (fn_1 or fn)(1))
+        tT = _cpu_median(lambda: fn(T))
+        ops[name] = {"rows_per_s_1_thread": (rows_1 or rows) / t1, "rows_per_s": rows / tT, "threads": T,
+                     "ms_1_thread": t1 * 1e3, "ms": tT * 1e3, "sample": sample}
+
+    k2 = inp["config2_keys"]  # 2 runs of RUN_ROWS
+    k1 = k2[:RUN_ROWS // 2]  # 2 runs of RUN_ROWS / 4 for the 1-thread figure
+    both("config2_multirun_sort", k2.size, lambda t: O.cpb_multirun_sort(k2, RUN_ROWS, t),
+         f"all threads: first {k2.size:,} keys of the config-2 stream as 2 runs of {RUN_ROWS:,} + 1 merge; "
+         f"1 thread: first {k1.size:,} keys as 2 runs of {RUN_ROWS // 4:,} + 1 merge",
+         rows_1=k1.size, fn_1=lambda t: O.cpb_multirun_sort(k1, RUN_ROWS // 4, t))
+    if "vals" in inp:
+        v, k = inp["vals"], inp["keys"]
+        both("filter_aggregate", v.size, lambda t: O.cpb_filter_aggregate(v, k, 0.5, t),
+             f"first {v.size:,} rows of the op table: WHERE f32 < 0.5 -> COUNT, SUM, MIN/MAX(int64)")
+        both("filter_compact", v.size, lambda t: O.cpb_filter_compact(v, k, 0.5, t),
+             f"first {v.size:,} rows: WHERE f32 < 0.5 compacting (int64 key, f32)")
+        for R, right in inp["join_right"].items():
+            left = inp["join_left"]
+            both(f"join_eq_R{R}", left.size, lambda t: O.cpb_join_eq(left, right, t, 1 << 24),
+                 f"first {left.size:,} left int64 keys x {R:,} right keys: hash build + parallel probe + pairs")
+    if "config1" in inp:
+        ids, price = inp["config1"]
+        both("config1_filter_sort_project", ids.size, lambda t: O.cpb_config1(ids, price, t),
+             f"all {ids.size:,} rows: WHERE price > 500, composite-key radix sort (price DESC, id), 2*id")
+    if "star" in inp:
+        dims, m1, m2 = inp["star"]
+        both("config3_star_join_group_by", m1.size, lambda t: O.cpb_star_join_group_sum(dims, m1, m2, 7000, t),
+             f"first {m1.size:,} of the 600M fact rows: direct-index join, per-thread private group arrays")
+    head = ops["config2_multirun_sort"]
     return {
-        "value": sample.size / best,
+        "value": head["rows_per_s"],
         "unit": "rows/s",
-        "cores": threads,
+        "cores": T,
         "kind": "port",
-        "sample": f"first {sample.size} rows (2 runs of {run_rows} + 1 merge) of the same key stream, "
-                  f"oracle LSD radix + stable merge, best of 2",
+        "sample": head["sample"],
+        "value_1_thread": head["rows_per_s_1_thread"],
+        "hardware_concurrency": hc,
+        "cpu_model": _cpu_model(),
+        "runs": "2 warm-up + median of 5",
+        "ops": ops,
     }
 
 
@@ -98,9 +179,10 @@ def _timed_ms(fn, reps=10):
     return ts[len(ts) // 2]
 
 
-def op_lines(keys):
+def op_lines(keys, cpu_inputs=None):
     """Filter (compaction) and fused filter+aggregate over the 200M-row table:
-    rows/s and fraction of HBM peak by their algorithmic bytes."""
+    rows/s and fraction of HBM peak by their algorithmic bytes.  With
+    `cpu_inputs`, host copies of the samples the CPU baseline runs on."""
     import torch
 
     from yallasql_amd import _lib as L
@@ -112,6 +194,12 @@ def op_lines(keys):
     g.manual_seed(SEED + 7)
     vals = torch.rand(n, generator=g, device=dev)
     out = {}
+    if cpu_inputs is not None:
+        S = min(n, CPU_SAMPLE_ROWS)
+        cpu_inputs["vals"] = vals[:S].cpu().numpy()
+        cpu_inputs["keys"] = keys[:S].cpu().numpy()
+        cpu_inputs["join_left"] = keys[:min(n, CPU_JOIN_ROWS)].cpu().numpy()
+        cpu_inputs["join_right"] = {}
     # WHERE vals < 0.5: COUNT, SUM(vals) (double), MIN(key), MAX(key) -- reads 12 B/row
     cols = [(L.INT, L.AGG_COUNT_STAR, None), (L.FLOAT, L.AGG_SUM, vals), (L.DATETIME, L.AGG_MIN, keys),
             (L.DATETIME, L.AGG_MAX, keys)]
@@ -152,6 +240,8 @@ def op_lines(keys):
     # [1000, 1e7): hash build of the right side + probe, (l, r) pairs written
     for R in (2_000, 500_000):
         right = torch.randint(1000, 10_000_000, (R,), generator=g, device=dev, dtype=torch.int64)
+        if cpu_inputs is not None:
+            cpu_inputs["join_right"][R] = right.cpu().numpy()
         jws = ops.workspace(L.ws_bytes("ysql_join_ws_bytes", L.DATETIME, L.EQ, n, R), dev)
         jcap = 1 << 24
         pairs = torch.empty((jcap, 2), dtype=torch.int32, device=dev)
@@ -171,7 +261,7 @@ def op_lines(keys):
     return out
 
 
-def config1_line():
+def config1_line(cpu_inputs=None):
     """Config 1: WHERE price > 500 ORDER BY price DESC, id ASC; SELECT id, price,
     2*id over 1,048,576 rows in one batch, through the engine (result copied
     to the host), median wall time of 10 queries."""
@@ -186,6 +276,8 @@ def config1_line():
     cat = r.integers(0, 64, n).astype(np.int32)
     ctx = E.Context()
     t = ctx.table("t", [("id", ids), ("price", price), ("category", cat)])
+    if cpu_inputs is not None:
+        cpu_inputs["config1"] = (ids, price)
 
     def query():
         p = ctx.scan(t, n).filter(E.cmp(2, E.ref(1), E.f32(500.0))).order([1, 0], [1, 0])
@@ -258,7 +350,7 @@ def csv_load_line(n=2_000_000):
         os.unlink(path)
 
 
-def config3_line():
+def config3_line(cpu_inputs=None):
     """Config 3: star join of a 600M-row fact table with four dimensions
     (2,556 / 2,000 / 30,000 / 200,000 rows; region = 0 and category = 0
     filters), SUM(extendedprice * discount) GROUP BY year x brand (7,000
@@ -281,6 +373,14 @@ def config3_line():
 
     run()
     ms = _timed_ms(run)
+    if cpu_inputs is not None:
+        S = min(n, CPU_STAR_ROWS)
+        hd = []
+        for d in dims:
+            h = {k: (v.cpu().numpy() if hasattr(v, "cpu") else v) for k, v in d.items()}
+            h["fk"] = h["fk"][:S]
+            hd.append(h)
+        cpu_inputs["star"] = (hd, m1[:S].cpu().numpy(), m2[:S].cpu().numpy())
     b = n * 24  # four int32 foreign keys + two int32 measures per fact row
     out = {"fact_rows": n, "ms": ms, "rows_per_s": n / (ms * 1e-3), "qualifying_rows": int(counts.sum().item()),
            "GB_s": b / (ms * 1e-3) / 1e9, "frac": b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
@@ -351,27 +451,31 @@ def batched_lines():
     return out
 
 
-def serving_lines(seconds):
-    """Serving loop over the 200M-row table at 50/200/800 queries/s."""
+def serving_lines(max_queries, max_seconds):
+    """Serving loop over the 200M-row table and the 600M-row star schema at
+    50/200/800 queries/s: each rate runs min(max_queries, rate *
+    max_seconds) Poisson arrivals."""
     import torch
 
     from yallasql_amd.serving import Server, Table, latency_report, make_queries
 
     dev = torch.device("cuda:0")
-    table = Table(N_ROWS, dev, seed=SEED)
+    table = Table(N_ROWS, dev, seed=SEED, star_rows=600_000_000)
     srv = Server(table, slots=8, hbm_budget=200 << 30)
     out = {}
     try:
         srv.serve(make_queries(table, 24, rate_qps=None, seed=1), realtime=False)  # warm-up
         for rate in (50, 200, 800):
-            qs = make_queries(table, int(rate * seconds), rate_qps=rate, seed=rate)
+            qs = make_queries(table, int(min(max_queries, rate * max_seconds)), rate_qps=rate, seed=4 + rate)
             wall = srv.serve(qs)
             out[f"{rate}qps"] = latency_report(qs, wall, srv)
+            print(f"serving {rate} qps: {len(qs)} queries done", file=sys.stderr, flush=True)
     finally:
         srv.close()
-    out["what"] = ("Poisson arrivals; 50% filter+aggregate, 30% ORDER BY key LIMIT 1000 above a random key, "
-                   "20% join with a 2,000-row dimension + GROUP BY (64 groups); 8 slots, 200 GB HBM budget; "
-                   "latency = arrival to result on the host")
+    out["what"] = ("Poisson arrivals; 50% WHERE key < c -> COUNT/SUM/MIN/MAX over 200M rows (up to 4 share a scan), "
+                   "30% ORDER BY key LIMIT 1000 above a random key, 20% the star join + GROUP BY year, brand over "
+                   "600M fact rows with random region/category constants; 8 slots, 200 GB HBM budget reserved by "
+                   "each unit's own allocation sizes; latency = arrival to result on the host")
     del table
     torch.cuda.empty_cache()
     return out
@@ -436,20 +540,27 @@ def main():
     if os.path.exists(tf):
         traffic = json.load(open(tf)).get("onesweep_u64_bytes_per_launch")
 
-    cpu = None
+    cpu_inputs = None
     if not args.no_cpu:
-        cpu = cpu_baseline(keys[: 2 * RUN_ROWS].cpu().numpy().view("uint64"), RUN_ROWS)
+        cpu_inputs = {"config2_keys": keys[: 2 * RUN_ROWS].cpu().numpy().view("uint64")}
 
-    extra_ops = None if args.no_ops else op_lines(keys)
+    print(f"sort: {ms:.2f} ms per step", file=sys.stderr, flush=True)
+    extra_ops = None if args.no_ops else op_lines(keys, cpu_inputs)
+    print("op lines done", file=sys.stderr, flush=True)
     nruns = len(job.runs)
     del job
     del keys
     torch.cuda.empty_cache()
-    configs = None if args.no_configs else {"config1_filter_sort_project": config1_line(),
-                                            "config3_star_join_group_by": config3_line(),
+    configs = None if args.no_configs else {"config1_filter_sort_project": config1_line(cpu_inputs),
+                                            "config3_star_join_group_by": config3_line(cpu_inputs),
                                             "a23_csv_load": csv_load_line()}
+    print("config lines done", file=sys.stderr, flush=True)
+    cpu = None if cpu_inputs is None else cpu_baseline(cpu_inputs)
+    del cpu_inputs
+    print("cpu baseline done", file=sys.stderr, flush=True)
     batched = None if args.no_batched else batched_lines()
-    serving = None if args.no_serving else serving_lines(args.serving_seconds)
+    print("batched lines done", file=sys.stderr, flush=True)
+    serving = None if args.no_serving else serving_lines(args.serving_queries, args.serving_seconds)
 
     out = {
         "metric": "rows/s & %HBM roofline per op; batched vs plain speedup; p99 query latency",
@@ -479,6 +590,10 @@ def main():
             "unit": "GB/s",
             "frac": pass_gbs / HBM_PEAK_GBS,
             "traffic": traffic,
+            "traffic_source": None if traffic is None else
+            "profiles/onesweep_u64_hbm_traffic.md: rocprofv3 FETCH_SIZE + WRITE_SIZE of one digit pass of a "
+            "2^25-row run, each scaled by the counter's reading for a device copy of known size (not measured "
+            "in this run)",
             "kernel": "k_onesweep<uint64,512,16> (one 8-bit digit pass of a 2^25-row run)",
             "bytes_per_launch": 2 * RUN_ROWS * 12,
             "avg_launch_ms": sum(pass_ms) / len(pass_ms),

@@ -36,8 +36,21 @@ typedef struct ysql_result ysql_result;
  * budget for cached batches (bytes; 0 = unlimited).                       */
 ysql_ctx *ysql_ctx_create(uint64_t hbm_budget_bytes, void *stream);
 void ysql_ctx_destroy(ysql_ctx *ctx);
-/* Peak bytes of batches held in the context's CacheManager so far. */
+/* Peak bytes of batches held in HBM by the context's CacheManager so far. */
 uint64_t ysql_ctx_peak_bytes(const ysql_ctx *ctx);
+/* Residency tiers below HBM (the reference's CPU and FS caches,
+ * cache_manager.hpp:67-88): when the HBM budget would be exceeded, least
+ * recently used cached batches (ORDER BY's sorted runs among them) move to
+ * pinned host memory, copied on a side stream; past `host_budget_bytes`
+ * (0 = unlimited) the least recently used host batches go to files in
+ * `spill_dir` (NULL: no file tier).  Batches come back into HBM when an
+ * operator takes them.                                                   */
+int ysql_ctx_set_spill(ysql_ctx *ctx, uint64_t host_budget_bytes, const char *spill_dir);
+typedef struct ysql_cache_stats {
+  uint64_t hbm_bytes, host_bytes, file_bytes, peak_hbm_bytes;
+  uint64_t evictions, spills, restores_from_host, restores_from_file;
+} ysql_cache_stats;
+int ysql_ctx_cache_stats(const ysql_ctx *ctx, ysql_cache_stats *out);
 
 /* Tables.  add_column copies `rows` host values (and optional host null
  * bytes) to the device; add_device_column borrows device buffers that must

@@ -507,6 +507,17 @@ typedef struct ysql_star_dim {
 int ysql_star_join_group_sum(const ysql_star_dim *dims, int ndims, const int32_t *m1,
                              const int32_t *m2, uint64_t n, int64_t *sums, uint64_t *counts,
                              uint32_t n_groups, ysql_stream_t stream);
+/* nq (1..8) star-join queries over the same facts and dimensions that differ
+ * only in their dimension filters, in one read of the facts: bit q of
+ * dims[d].pass[row] says whether query q keeps that dimension row (a NULL
+ * pass keeps every row for every query).  Query q folds into
+ * sums[q * n_groups + g] and counts[q * n_groups + g]; each query's result
+ * equals its own ysql_star_join_group_sum.  No counterpart in the reference
+ * (it runs one query at a time); the serving loop batches concurrent star
+ * joins with it.                                                          */
+int ysql_star_join_group_sum_multi(const ysql_star_dim *dims, int ndims, int nq, const int32_t *m1,
+                                   const int32_t *m2, uint64_t n, int64_t *sums, uint64_t *counts,
+                                   uint32_t n_groups, ysql_stream_t stream);
 
 /* ======================================================================
  * Batched ragged jobs (no reference counterpart: the reference launched

@@ -84,10 +84,16 @@ template <typename T> struct RemOperator { static constexpr int code = YSQL_REM;
 template <typename T> struct MaxOperator { static constexpr int code = YSQL_AGG_MAX; };
 template <typename T> struct MinOperator { static constexpr int code = YSQL_AGG_MIN; };
 template <typename T> struct SumOperator { static constexpr int code = YSQL_AGG_SUM; };
-template <typename T> struct AscOp { static constexpr bool desc = false; };
-template <typename T> struct DescOp { static constexpr bool desc = true; };
-template <typename T> struct AscSortMergeOp { static constexpr bool desc = false; };
-template <typename T> struct DescSortMergeOp { static constexpr bool desc = true; };
+template <typename T> struct AscOp { static constexpr bool desc = false, is_string = false; };
+template <typename T> struct DescOp { static constexpr bool desc = true, is_string = false; };
+template <typename T> struct AscSortMergeOp { static constexpr bool desc = false, is_string = false; };
+template <typename T> struct DescSortMergeOp { static constexpr bool desc = true, is_string = false; };
+// STRING orders (radix_sort_kernel.hpp:45-73, merge_batchs.hpp:36-48): the
+// element type is the reference's 256-byte String.
+struct AscOpString { static constexpr bool desc = false, is_string = true; };
+struct DescOpString { static constexpr bool desc = true, is_string = true; };
+struct AscSortStrOp { static constexpr bool desc = false, is_string = true; };
+struct DescSortStrOp { static constexpr bool desc = true, is_string = true; };
 
 template <typename T> struct TypeCode;
 template <> struct TypeCode<int> { static constexpr int v = YSQL_INT; };
@@ -126,15 +132,32 @@ inline void launch_uint32_to_float(uint32_t *arr, float *res, int N, hipStream_t
 
 // ---- A2-A6: radix_sort_kernel.cu:139-178 ---------------------------------
 // T = uint32_t or uint64_t keys (already normalised); OP = AscOp / DescOp.
+// T = the 256-byte String with OP = AscOpString / DescOpString sorts the
+// strings in place (strcmp order, stable) through ysql_string_argsort and a
+// gather.
 template <typename OP, typename T, size_t BlockDim>
 void launch_radix_sort(T *d_input, uint32_t *&d_idxs_in, const uint32_t N, hipStream_t stream) {
-  size_t ws_bytes = 0;
-  check("ysql_radix_sort_ws_bytes", ysql_radix_sort_ws_bytes(TypeCode<T>::v, N, &ws_bytes));
-  TempBuf ws(ws_bytes, stream);
-  if (hipMallocAsync(&d_idxs_in, (size_t)N * sizeof(uint32_t) + 4, stream) != hipSuccess)
-    throw KernelError("hipMallocAsync", YSQL_EHIP);
-  check("ysql_radix_argsort", ysql_radix_argsort(TypeCode<T>::v, OP::desc, d_input, d_idxs_in, N, 0, 8 * sizeof(T),
-                                                 ws.p, ws_bytes, stream));
+  if constexpr (OP::is_string) {
+    static_assert(sizeof(T) == YSQL_STRING_BYTES, "String is 256 bytes");
+    size_t ws_bytes = 0;
+    if (N > 1) check("ysql_string_sort_ws_bytes", ysql_string_sort_ws_bytes(N, &ws_bytes));
+    TempBuf ws(ws_bytes, stream), sorted((size_t)N * YSQL_STRING_BYTES, stream);
+    if (hipMallocAsync(&d_idxs_in, (size_t)N * sizeof(uint32_t) + 4, stream) != hipSuccess)
+      throw KernelError("hipMallocAsync", YSQL_EHIP);
+    check("ysql_string_argsort", ysql_string_argsort(OP::desc, d_input, d_idxs_in, N, ws.p, ws_bytes, stream));
+    check("ysql_gather", ysql_gather(YSQL_STRING_BYTES, d_input, nullptr, d_idxs_in, N, sorted.p, nullptr, stream));
+    if (N && hipMemcpyAsync(d_input, sorted.p, (size_t)N * YSQL_STRING_BYTES, hipMemcpyDeviceToDevice, stream) !=
+                 hipSuccess)
+      throw KernelError("hipMemcpyAsync", YSQL_EHIP);
+  } else {
+    size_t ws_bytes = 0;
+    check("ysql_radix_sort_ws_bytes", ysql_radix_sort_ws_bytes(TypeCode<T>::v, N, &ws_bytes));
+    TempBuf ws(ws_bytes, stream);
+    if (hipMallocAsync(&d_idxs_in, (size_t)N * sizeof(uint32_t) + 4, stream) != hipSuccess)
+      throw KernelError("hipMallocAsync", YSQL_EHIP);
+    check("ysql_radix_argsort", ysql_radix_argsort(TypeCode<T>::v, OP::desc, d_input, d_idxs_in, N, 0, 8 * sizeof(T),
+                                                   ws.p, ws_bytes, stream));
+  }
 }
 
 // ---- A4 / A13: prefix_sum_kernel.cu:140-178 --------------------------------
@@ -162,7 +185,14 @@ void launch_merge_sorted_array_kernel(T *A, T *B, T *C, uint32_t *new_idx, bool 
   size_t b = 0;
   check("ysql_merge_ws_bytes", ysql_merge_ws_bytes(m, n, k_mx, &b));
   TempBuf ws(b, stream), cons(8, stream);
-  check("ysql_merge_path", ysql_merge_path(TypeCode<T>::v, OP::desc, A, m, B, n, k_mx, C, new_idx,
+  int dtype;
+  if constexpr (OP::is_string) {
+    static_assert(sizeof(T) == YSQL_STRING_BYTES, "String is 256 bytes");
+    dtype = YSQL_STRING;
+  } else {
+    dtype = TypeCode<T>::v;
+  }
+  check("ysql_merge_path", ysql_merge_path(dtype, OP::desc, A, m, B, n, k_mx, C, new_idx,
                                            reinterpret_cast<uint8_t *>(table_idx), static_cast<int64_t *>(cons.p),
                                            ws.p, b, stream));
   if (lasti) {  // consumed_a <= m fits the int the reference reads back (order_operator.cpp:175)

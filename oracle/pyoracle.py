@@ -61,6 +61,13 @@ _SIG = {
     "orc_star_join_group_sum": ([C.POINTER(StarDim), I, P, P, U64, P, P, C.c_uint32], I),
     "orc_cpu_multirun_sort_u64": ([P, U64, U64, I, P, P], I),
     "orc_sort_rows": ([I, C.POINTER(I), C.POINTER(I), C.POINTER(P), U64, P], I),
+    "cpb_hardware_concurrency": ([], C.c_uint),
+    "cpb_multirun_sort_u64": ([P, U64, U64, I, P, P], I),
+    "cpb_filter_aggregate": ([P, P, U64, C.c_float, I, P, P, P, P], I),
+    "cpb_filter_compact": ([P, P, U64, C.c_float, I, P, P], U64),
+    "cpb_join_eq": ([P, U64, P, U64, I, P, U64], U64),
+    "cpb_star_join_group_sum": ([C.POINTER(StarDim), I, P, P, U64, I, P, P, C.c_uint32], I),
+    "cpb_config1": ([P, P, U64, I, P, P, P], U64),
 }
 for _n, (_a, _r) in _SIG.items():
     f = getattr(lib, _n)
@@ -295,3 +302,59 @@ def sort_rows(cols, desc):
     ptrs = (P * k)(*[c.ctypes.data_as(P) for c in cols])
     _chk(lib.orc_sort_rows(k, (I * k)(*[dt(c) for c in cols]), (I * k)(*[int(d) for d in desc]), ptrs, n, _p(perm)))
     return perm
+
+
+# ---- multi-threaded CPU baselines (oracle/cpu_baseline.cpp) ---------------
+def cpb_hardware_concurrency():
+    return int(lib.cpb_hardware_concurrency())
+
+
+def cpb_multirun_sort(keys, run_rows, threads):
+    keys = np.ascontiguousarray(keys, dtype=np.uint64)
+    ok = np.empty(keys.size, np.uint64)
+    ov = np.empty(keys.size, np.uint32)
+    _chk(lib.cpb_multirun_sort_u64(_p(keys), keys.size, run_rows, threads, _p(ok), _p(ov)))
+    return ok, ov
+
+
+def cpb_filter_aggregate(vals, keys, thr, threads):
+    c, s_, lo, hi = np.zeros(1, np.uint64), np.zeros(1, np.float64), np.zeros(1, np.int64), np.zeros(1, np.int64)
+    _chk(lib.cpb_filter_aggregate(_p(vals), _p(keys), vals.size, thr, threads, _p(c), _p(s_), _p(lo), _p(hi)))
+    return int(c[0]), float(s_[0]), int(lo[0]), int(hi[0])
+
+
+def cpb_filter_compact(vals, keys, thr, threads):
+    ok = np.empty(vals.size, np.int64)
+    ov = np.empty(vals.size, np.float32)
+    m = lib.cpb_filter_compact(_p(vals), _p(keys), vals.size, thr, threads, _p(ok), _p(ov))
+    return ok[:m], ov[:m]
+
+
+def cpb_join_eq(left, right, threads, cap):
+    pairs = np.empty((cap, 2), np.uint32)
+    m = lib.cpb_join_eq(_p(left), left.size, _p(right), right.size, threads, _p(pairs), cap)
+    return pairs[:min(m, cap)], int(m)
+
+
+def cpb_star_join_group_sum(dims, m1, m2, n_groups, threads):
+    arr = (StarDim * len(dims))()
+    keep = []
+    for i, d in enumerate(dims):
+        fk = np.ascontiguousarray(d["fk"], dtype=np.int32)
+        ps = None if d.get("pass") is None else np.ascontiguousarray(d["pass"], dtype=np.uint8)
+        gr = None if d.get("group") is None else np.ascontiguousarray(d["group"], dtype=np.int32)
+        keep += [fk, ps, gr]
+        arr[i] = StarDim(fk.ctypes.data, d["rows"], d.get("group_card", 0), None if ps is None else ps.ctypes.data,
+                         None if gr is None else gr.ctypes.data, d.get("group_base", 0), 0)
+    sums = np.zeros(n_groups, dtype=np.int64)
+    counts = np.zeros(n_groups, dtype=np.uint64)
+    _chk(lib.cpb_star_join_group_sum(arr, len(dims), _p(m1), _p(m2), m1.size, threads, _p(sums), _p(counts),
+                                     n_groups))
+    return sums, counts
+
+
+def cpb_config1(ids, price, threads):
+    n = ids.size
+    oi, op, o2 = np.empty(n, np.int32), np.empty(n, np.float32), np.empty(n, np.int32)
+    m = lib.cpb_config1(_p(ids), _p(price), n, threads, _p(oi), _p(op), _p(o2))
+    return oi[:m], op[:m], o2[:m]

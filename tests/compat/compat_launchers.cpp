@@ -31,6 +31,11 @@ struct NullBitSet {
   bool owned = true;
 };
 
+// The reference's fixed-width string (include/kernels/string_kernel.hpp:6-8).
+struct String {
+  char data[256];
+};
+
 struct Column {
   DataType type;
   unsigned int bytes;
@@ -90,6 +95,30 @@ int compat_radix_sort(int key64, int desc, void *keys, int n, uint32_t *idx_out,
     }
     if (n) (void)hipMemcpyAsync(idx_out, idx, (size_t)n * 4, hipMemcpyDeviceToDevice, s);
     (void)hipFreeAsync(idx, s);
+  });
+}
+
+// STRING keys: the reference's 256-byte String sorted in place, and merged
+int compat_sort_strings(int desc, void *strs, int n, uint32_t *idx_out, void *stream) {
+  return guarded([&] {
+    hipStream_t s = S(stream);
+    uint32_t *idx = nullptr;
+    if (desc) launch_radix_sort<DescOpString, String, 256>((String *)strs, idx, n, s);
+    else launch_radix_sort<AscOpString, String, 256>((String *)strs, idx, n, s);
+    if (n) (void)hipMemcpyAsync(idx_out, idx, (size_t)n * 4, hipMemcpyDeviceToDevice, s);
+    (void)hipFreeAsync(idx, s);
+  });
+}
+int compat_merge_strings(int desc, void *A, void *B, void *C, uint32_t *new_idx, bool *table_idx, int *lasti,
+                         uint32_t k, uint32_t m, uint32_t n, void *stream) {
+  return guarded([&] {
+    hipStream_t s = S(stream);
+    if (desc)
+      launch_merge_sorted_array_kernel<String, DescSortStrOp>((String *)A, (String *)B, (String *)C, new_idx, table_idx,
+                                                              lasti, k, m, n, s);
+    else
+      launch_merge_sorted_array_kernel<String, AscSortStrOp>((String *)A, (String *)B, (String *)C, new_idx, table_idx,
+                                                             lasti, k, m, n, s);
   });
 }
 

@@ -27,7 +27,7 @@ def cl(dev):
     lib = C.CDLL(LIB)
     for f in ("compat_sortkey", "compat_radix_sort", "compat_prefix_sum", "compat_merge", "compat_move",
               "compat_conditional", "compat_binary", "compat_outer_join", "compat_and_join", "compat_cross",
-              "compat_reduce", "compat_finalize", "compat_move_batch"):
+              "compat_reduce", "compat_finalize", "compat_move_batch", "compat_sort_strings", "compat_merge_strings"):
         getattr(lib, f).restype = C.c_int
     lib.compat_outer_join.argtypes = [C.c_int, P, P, P, P, C.c_uint32, C.c_uint32, C.c_uint64, P]
     return lib
@@ -338,3 +338,39 @@ def test_batch_level_gathers(cl, dev, merge):
             else:
                 e, en = O.gather(a_cols[c], idx, a_cols[3][c])
             assert np.array_equal(_h(outs[c]), e) and np.array_equal(_h(onul[c]), en), (merge, asc, c)
+
+
+@pytest.mark.parametrize("desc", [0, 1])
+def test_string_sort_and_merge(cl, dev, desc):
+    """launch_radix_sort<AscOpString / DescOpString, String, ...> sorts the
+    strings in place and returns the permutation; launch_merge_sorted_array_kernel
+    <String, AscSortStrOp / DescSortStrOp> merges two sorted string runs."""
+    import torch
+
+    r = np.random.default_rng(40 + desc)
+
+    def strs(n):
+        s = np.zeros((n, 256), np.uint8)
+        for i in range(n):
+            k = int(r.integers(0, 6))
+            s[i, :k] = r.choice(np.frombuffer(b"ab\x90", np.uint8), k)
+        return s
+
+    s = strs(5000)
+    t = _t(s, dev)
+    idx = torch.empty(5000, dtype=torch.int32, device=dev)
+    assert cl.compat_sort_strings(desc, _p(t), 5000, _p(idx), _s()) == 0
+    want = O.sort_rows([s], [desc])
+    assert np.array_equal(_h(idx).view(np.uint32), want) and np.array_equal(_h(t), s[want])
+    A, B = s[want][:3000].copy(), strs(2000)
+    B = B[O.sort_rows([B], [desc])]
+    k = 4500
+    Cd = torch.empty((k, 256), dtype=torch.uint8, device=dev)
+    ix = torch.empty(k, dtype=torch.int32, device=dev)
+    isa = torch.empty(k, dtype=torch.uint8, device=dev)
+    lasti = torch.zeros(1, dtype=torch.int32, device=dev)
+    assert cl.compat_merge_strings(desc, _p(_t(A, dev)), _p(_t(B, dev)), _p(Cd), _p(ix), _p(isa), _p(lasti), k,
+                                   3000, 2000, _s()) == 0
+    ec, ei, ea, cons = O.merge(O.STRING, desc, A, B, k)
+    assert np.array_equal(_h(Cd), ec) and np.array_equal(_h(ix).view(np.uint32), ei)
+    assert np.array_equal(_h(isa), ea) and int(_h(lasti)[0]) == cons

@@ -157,9 +157,52 @@ def test_filter_keeps_null_bytes_and_strings(E):
     assert np.array_equal(nulls[0], nul[m])
 
 
-def test_budget_is_enforced(E):
+def test_budget_moves_batches_out_of_hbm(E):
+    """A 64 KB HBM budget: ORDER BY's sorted runs (40 KB each) leave HBM for
+    host memory as new ones arrive; a merged run larger than the budget stays
+    in HBM once everything else has moved out; the result is unchanged."""
     n = 100_000
+    r = np.random.default_rng(1)
+    a = r.integers(-1000, 1000, n).astype(np.int32)
     ctx = E.Context(hbm_budget=1 << 16)
-    t = ctx.table("t", [("a", np.arange(n, dtype=np.int32))])
-    with pytest.raises(E.EngineError):
-        ctx.scan(t, n).execute()
+    t = ctx.table("t", [("a", a)])
+    cols, _, _ = ctx.scan(t, 10_000).order([0], [0]).execute()
+    assert np.array_equal(cols[0], a[np.argsort(a, kind="stable")])
+    st = ctx.cache_stats()
+    assert st["evictions"] > 0 and st["restores_from_host"] > 0 and st["spills"] == 0, st
+
+
+@pytest.mark.parametrize("strings", [False, True])
+def test_order_runs_spill_to_host_and_files(E, tmp_path, strings):
+    """An ORDER BY over 16 batches under a 3 MB HBM budget and a 2 MB host
+    budget: its sorted runs leave HBM for pinned host memory and then for
+    spill files, and come back from the files for their merges; the result
+    equals the unlimited run and the oracle."""
+    n = 1 << 20
+    ids, price, cat = config1_table(n, seed=5)
+    cols = [("id", ids), ("price", price), ("category", cat)]
+    if strings:
+        names = np.zeros((n, 256), np.uint8)
+        names[:, 0] = 97 + (cat % 3)
+        cols = [("id", ids), ("price", price), ("name", names)]
+
+    def run(ctx):
+        t = ctx.table("t", cols)
+        keys = ([2, 1, 0], [1, 1, 0]) if strings else ([1, 0], [1, 0])
+        return ctx.scan(t, 1 << 16).order(*keys).execute()
+
+    big = E.Context()
+    want, _, _ = run(big)
+    assert big.cache_stats()["evictions"] == 0
+    small = E.Context(hbm_budget=3 << 20, host_budget=2 << 20, spill_dir=tmp_path)
+    got, _, _ = run(small)
+    for a, b in zip(got, want):
+        assert np.array_equal(a, b)
+    st = small.cache_stats()
+    assert st["evictions"] > 0 and st["spills"] > 0 and st["restores_from_file"] > 0, st
+    if strings:
+        p = O.sort_rows([cols[2][1], price, ids], [1, 1, 0])
+    else:
+        p = O.sort_rows([price, ids], [1, 0])
+    assert np.array_equal(got[0], ids[p]) and np.array_equal(got[1], price[p])
+    assert not list(tmp_path.iterdir())  # spill files are removed once read back

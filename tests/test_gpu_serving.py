@@ -1,7 +1,8 @@
 """Serving loop: every query's result is the same alone and in a mix of
 concurrent queries (shared scans, overlapping pipelines, admission waits
-under a small memory budget), and matches numpy on the host copy of the
-table."""
+under a small memory budget), and matches numpy (filter+aggregate, top-k)
+or the oracle (the star join + GROUP BY) on host copies of the tables; a
+unit's device allocations stay within what admission reserved for it."""
 import numpy as np
 import pytest
 
@@ -18,9 +19,10 @@ def table(dev):
 
 
 def expected(table_host, q):
-    from yallasql_amd.serving import FILTER_AGG, TOPK, TOPK_K
+    from oracle import pyoracle as O
+    from yallasql_amd.serving import FILTER_AGG, STAR_GROUPS, TOPK, TOPK_K
 
-    key, pay, dkey, dgrp = table_host
+    key, pay, star = table_host
     if q.kind == FILTER_AGG:
         m = key < q.param
         c = int(m.sum())
@@ -30,18 +32,17 @@ def expected(table_host, q):
         idx = np.nonzero(key >= q.param)[0]
         order = idx[np.argsort(key[idx], kind="stable")][:TOPK_K]
         return key[order], pay[order]
-    pos = {int(k): i for i, k in enumerate(dkey)}
-    sel = np.nonzero(np.isin(key, dkey) & (pay < q.param))[0]
-    g = np.array([dgrp[pos[int(k)]] for k in key[sel]], dtype=np.int64)
-    G = 64
-    sums = np.zeros(G, np.int64)
-    np.add.at(sums, g, pay[sel].astype(np.int64))
-    return sums, np.bincount(g, minlength=G).astype(np.int64)
+    dims, m1, m2, attrs = star
+    d = [dict(x) for x in dims]
+    d[2]["pass"] = (attrs["region"] == q.param[0]).astype(np.uint8)
+    d[3]["pass"] = (attrs["cat"] == q.param[1]).astype(np.uint8)
+    sums, counts = O.star_join_group_sum(d, m1, m2, STAR_GROUPS)
+    return sums, counts.astype(np.int64)
 
 
 def same(a, b):
     if isinstance(a, tuple) and a and isinstance(a[0], np.ndarray):
-        return all(np.array_equal(x, y) for x, y in zip(a, b))
+        return all(np.array_equal(np.asarray(x).astype(np.int64), np.asarray(y).astype(np.int64)) for x, y in zip(a, b))
     return a == b
 
 
@@ -97,3 +98,62 @@ def test_latency_report(table):
     rep = latency_report(qs, wall, srv)
     assert rep["queries"] == 30 and 0 < rep["p50_ms"] <= rep["p99_ms"]
     assert rep["rows_per_s"] > 0
+
+
+def test_topk_on_a_hot_key(table):
+    """ORDER BY key LIMIT 1000 starting at the hottest key (hundreds of
+    thousands of rows share it): the window is bisected instead of compacted
+    whole, and ties come in row order."""
+    from yallasql_amd.serving import TOPK, Query, run_topk
+
+    host = table.host()
+    vals, cnts = np.unique(host[0], return_counts=True)
+    hot = int(vals[np.argmax(cnts)])
+    assert cnts.max() > 16_000 * 8
+    for lo in (hot, hot - 1 if hot > 0 else hot):
+        q = Query(0, TOPK, lo)
+        run_topk(table, q)
+        assert same(q.result, expected(host, q))
+
+
+def test_units_allocate_within_their_reservation(table, dev):
+    """Each kind of unit, run by itself, allocates at most the bytes
+    admission reserves for it (the device allocator's peak)."""
+    import torch
+
+    from yallasql_amd.serving import FILTER_AGG, JOIN_GROUP, TOPK, Query, run_filter_agg, run_join_group, run_topk
+    from yallasql_amd.serving import unit_bytes
+
+    vals, cnts = np.unique(table.host()[0], return_counts=True)
+    hot = int(vals[np.argmax(cnts)])
+    cases = [(FILTER_AGG, lambda: run_filter_agg(table, [Query(i, FILTER_AGG, int(table.sample[1000 * i]))
+                                                         for i in range(4)]), 4),
+             (FILTER_AGG, lambda: run_filter_agg(table, [Query(0, FILTER_AGG, int(table.sample[500]))]), 1),
+             (TOPK, lambda: run_topk(table, Query(0, TOPK, int(table.sample[100]))), 1),
+             (TOPK, lambda: run_topk(table, Query(0, TOPK, hot)), 1),
+             (TOPK, lambda: run_topk(table, Query(0, TOPK, int(table.sample[-3]))), 1),
+             (JOIN_GROUP, lambda: run_join_group(table, [Query(0, JOIN_GROUP, (3, 11))]), 1),
+             (JOIN_GROUP, lambda: run_join_group(table, [Query(i, JOIN_GROUP, (i % 7, i)) for i in range(8)]), 8)]
+    for kind, fn, nq in cases:
+        torch.cuda.synchronize()
+        base = torch.cuda.memory_allocated(dev)
+        torch.cuda.reset_peak_memory_stats(dev)
+        fn()
+        torch.cuda.synchronize()
+        peak = torch.cuda.max_memory_allocated(dev) - base
+        assert 0 < peak <= unit_bytes(table, kind, nq), (kind, peak, unit_bytes(table, kind, nq))
+
+
+def test_shared_star_join_equals_lone_queries(table):
+    """Eight star-join queries with different region/category constants in
+    one read of the facts give each query's lone result, bit for bit."""
+    from yallasql_amd.serving import JOIN_GROUP, Query, run_join_group
+
+    params = [(0, 0), (6, 24), (3, 3), (3, 3), (1, 12), (5, 0), (2, 7), (4, 19)]
+    shared = [Query(i, JOIN_GROUP, p) for i, p in enumerate(params)]
+    run_join_group(table, shared)
+    host = table.host()
+    for q in shared:
+        lone = Query(q.qid, JOIN_GROUP, q.param)
+        run_join_group(table, [lone])
+        assert same(q.result, lone.result) and same(q.result, expected(host, q))

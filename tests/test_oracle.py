@@ -357,3 +357,47 @@ def test_merge_strings_is_stable_sort_of_concatenation():
         want = sorted(allt, key=lambda x: x[0], reverse=desc)
         assert got == [w[0] for w in want] and tags == [w[1] for w in want]
         assert cons == 700
+
+
+def test_cpu_baselines_compute_the_oracle_results():
+    """The multi-threaded CPU baselines that bench.py times give the oracle's
+    results at 1 and 5 threads (so the CPU figures time correct work)."""
+    from oracle import pyoracle as P
+
+    r = np.random.default_rng(81)
+    n = 300_001
+    keys = np.where(r.random(n) < 0.5, r.integers(0, 2**62, n), r.zipf(1.1, n) % 10**7).astype(np.uint64)
+    want_k, want_v = P.cpu_multirun_sort_u64(keys, 70_000, 1)
+    vals = r.random(n).astype(np.float32)
+    ik = keys.astype(np.int64)
+    m = vals < np.float32(0.5)
+    right = r.integers(0, 5000, 3000).astype(np.int64)
+    left = r.integers(0, 5000, 20_000).astype(np.int64)
+    ids = r.permutation(50_000).astype(np.int32)
+    price = (1 + 999 * r.random(50_000)).astype(np.float32)
+    price[::7] = np.float32(750.5)
+    for t in (1, 5):
+        k, v = P.cpb_multirun_sort(keys, 70_000, t)
+        assert np.array_equal(k, want_k) and np.array_equal(v, want_v)
+        c, s_, lo, hi = P.cpb_filter_aggregate(vals, ik, 0.5, t)
+        assert c == m.sum() and lo == ik[m].min() and hi == ik[m].max()
+        assert abs(s_ - vals[m].astype(np.float64).sum()) <= 1e-9 * s_
+        ck, cv = P.cpb_filter_compact(vals, ik, 0.5, t)
+        assert np.array_equal(ck, ik[m]) and np.array_equal(cv, vals[m])
+        pairs, cnt = P.cpb_join_eq(left, right, t, 1 << 20)
+        want = sorted((int(i), int(j)) for i in range(left.size) for j in np.nonzero(right == left[i])[0])
+        assert cnt == len(want) and [tuple(p) for p in pairs.tolist()] == want
+        oi, op, o2 = P.cpb_config1(ids, price, t)
+        sel = price > np.float32(500)
+        perm = P.sort_rows([price[sel], ids[sel]], [1, 0])
+        assert np.array_equal(oi, ids[sel][perm]) and np.array_equal(op, price[sel][perm])
+        assert np.array_equal(o2, 2 * ids[sel][perm])
+    dims = [{"fk": r.integers(0, 100, n).astype(np.int32), "rows": 100, "group": r.integers(0, 7, 100).astype(np.int32),
+             "group_base": 0, "group_card": 7},
+            {"fk": r.integers(0, 50, n).astype(np.int32), "rows": 50, "pass": (r.random(50) < 0.5).astype(np.uint8)}]
+    m1 = r.integers(0, 1000, n).astype(np.int32)
+    m2 = r.integers(0, 10, n).astype(np.int32)
+    ws, wc = P.star_join_group_sum(dims, m1, m2, 7)
+    for t in (1, 5):
+        s2, c2 = P.cpb_star_join_group_sum(dims, m1, m2, 7, t)
+        assert np.array_equal(s2, ws) and np.array_equal(c2, wc)

@@ -123,6 +123,7 @@ _SIGS = {
     "ysql_pairs_intersect": [P, U64, P, U64, P, U64, P, P, C.c_size_t, P],
     "ysql_cross_expand": [U32, P, P, U64, U64, I32, P, P, P],
     "ysql_star_join_group_sum": [C.POINTER(StarDim), I32, P, P, U64, P, P, U32, P],
+    "ysql_star_join_group_sum_multi": [C.POINTER(StarDim), I32, I32, P, P, U64, P, P, U32, P],
 }
 
 for _name, _args in _SIGS.items():

@@ -2,6 +2,7 @@
 #include "engine.hpp"
 
 #include <algorithm>
+#include <cstdio>
 #include <cstring>
 
 namespace YallaSQL {
@@ -36,30 +37,262 @@ std::unique_ptr<Batch> Batch::slice(size_t start, size_t rows) const {
 }
 
 // ---------------------------------------------------------------- cache
+CacheManager::CacheManager(size_t hbm_budget, size_t host_budget, std::string spill_dir)
+    : budget_(hbm_budget), host_budget_(host_budget), spill_dir_(std::move(spill_dir)) {}
+
+CacheManager::~CacheManager() {
+  clear();
+  if (copy_) (void)hipStreamDestroy(copy_);
+}
+
+void CacheManager::clear() {
+  if (copy_) (void)hipStreamSynchronize(copy_);
+  for (auto &kv : cache_) {
+    Entry &e = kv.second;
+    if (e.copied) (void)hipEventSynchronize(e.copied);
+    if (e.tier == HOST) {  // (in the file tier the null entries are only markers)
+      for (void *p : e.host_cols)
+        if (p) (void)hipHostFree(p);
+      for (void *p : e.host_nulls)
+        if (p) (void)hipHostFree(p);
+    }
+    if (e.copied) (void)hipEventDestroy(e.copied);
+    if (!e.file.empty()) std::remove(e.file.c_str());
+  }
+  for (auto &r : host_retired_) {
+    (void)hipEventSynchronize(r.first);
+    for (void *p : r.second) (void)hipHostFree(p);
+    (void)hipEventDestroy(r.first);
+  }
+  cache_.clear();
+  host_retired_.clear();
+  dev_bytes_ = host_bytes_ = file_bytes_ = 0;
+}
+
+void CacheManager::configure(size_t host_budget, std::string spill_dir) {
+  host_budget_ = host_budget;
+  spill_dir_ = std::move(spill_dir);
+}
+
+CacheManager::Stats CacheManager::stats() const {
+  return Stats{dev_bytes_, host_bytes_, file_bytes_, peak_, evictions_, spills_, from_host_, from_file_};
+}
+
+CacheManager::Entry &CacheManager::find(BatchID id) {
+  auto it = cache_.find(id);
+  if (it == cache_.end()) throw EngineError("unknown batch id " + std::to_string(id));
+  return it->second;
+}
+
+// Host buffers whose host-to-device copies have completed are freed.
+void CacheManager::reapHost() {
+  size_t k = 0;
+  for (auto &r : host_retired_) {
+    if (hipEventQuery(r.first) == hipSuccess) {
+      for (void *p : r.second) (void)hipHostFree(p);
+      (void)hipEventDestroy(r.first);
+    } else {
+      host_retired_[k++] = std::move(r);
+    }
+  }
+  host_retired_.resize(k);
+}
+
+// HBM -> pinned host on the copy stream, ordered after the compute stream's
+// work so far; device buffers held only by this batch are freed on the copy
+// stream after the copies (views shared with other batches stay alive).
+void CacheManager::evictToHost(Entry &e) {
+  Batch &b = *e.batch;
+  if (!copy_) hipCheck(hipStreamCreateWithFlags(&copy_, hipStreamNonBlocking), "hipStreamCreate");
+  hipEvent_t produced;
+  hipCheck(hipEventCreateWithFlags(&produced, hipEventDisableTiming), "hipEventCreate");
+  hipCheck(hipEventRecord(produced, compute_), "hipEventRecord");
+  hipCheck(hipStreamWaitEvent(copy_, produced, 0), "hipStreamWaitEvent");
+  (void)hipEventDestroy(produced);
+  const size_t nc = b.numCols();
+  e.host_cols.assign(nc, nullptr);
+  e.host_nulls.assign(nc, nullptr);
+  for (size_t c = 0; c < nc; c++) {
+    const size_t bytes = b.batchSize * b.columns[c]->bytes;
+    hipCheck(hipHostMalloc(&e.host_cols[c], bytes ? bytes : 1, hipHostMallocDefault), "hipHostMalloc");
+    if (bytes) hipCheck(hipMemcpyAsync(e.host_cols[c], b.col(c), bytes, hipMemcpyDeviceToHost, copy_), "hipMemcpyAsync");
+    if (b.nulls[c]) {
+      hipCheck(hipHostMalloc(&e.host_nulls[c], b.batchSize ? b.batchSize : 1, hipHostMallocDefault), "hipHostMalloc");
+      if (b.batchSize)
+        hipCheck(hipMemcpyAsync(e.host_nulls[c], b.nul(c), b.batchSize, hipMemcpyDeviceToHost, copy_),
+                 "hipMemcpyAsync");
+    }
+  }
+  hipCheck(hipEventCreateWithFlags(&e.copied, hipEventDisableTiming), "hipEventCreate");
+  hipCheck(hipEventRecord(e.copied, copy_), "hipEventRecord");
+  for (auto *v : {&b.data, &b.nulls})
+    for (auto &buf : *v)
+      if (buf && buf.use_count() == 1 && buf->owned) buf->stream = copy_;  // freed after the copies
+  b.data.assign(nc, nullptr);
+  b.nulls.assign(nc, nullptr);
+  b.offset.assign(nc, 0);
+  e.tier = HOST;
+  dev_bytes_ -= e.bytes;
+  host_bytes_ += e.bytes;
+  evictions_++;
+}
+
+// Pinned host -> one file (columns, then null columns, in order).
+void CacheManager::spillToFile(Entry &e, BatchID id) {
+  const Batch &b = *e.batch;
+  hipCheck(hipEventSynchronize(e.copied), "hipEventSynchronize");
+  const std::string path = spill_dir_ + "/ysql_batch_" + std::to_string(reinterpret_cast<uintptr_t>(this)) + "_" +
+                           std::to_string(id) + ".bin";
+  FILE *f = std::fopen(path.c_str(), "wb");
+  if (!f) throw EngineError("cannot create spill file " + path);
+  bool ok = true;
+  for (size_t c = 0; c < b.numCols(); c++) {
+    const size_t bytes = b.batchSize * b.columns[c]->bytes;
+    ok = ok && std::fwrite(e.host_cols[c], 1, bytes, f) == bytes;
+    if (e.host_nulls[c]) ok = ok && std::fwrite(e.host_nulls[c], 1, b.batchSize, f) == b.batchSize;
+  }
+  ok = (std::fclose(f) == 0) && ok;
+  if (!ok) {
+    std::remove(path.c_str());
+    throw EngineError("cannot write spill file " + path);
+  }
+  for (auto &p : e.host_cols) {
+    (void)hipHostFree(p);
+    p = nullptr;
+  }
+  for (auto &p : e.host_nulls)
+    if (p) {
+      (void)hipHostFree(p);
+      p = reinterpret_cast<void *>(1);  // marks "this column has null bytes" for the reload
+    }
+  (void)hipEventDestroy(e.copied);
+  e.copied = nullptr;
+  e.file = path;
+  e.tier = FILE_TIER;
+  host_bytes_ -= e.bytes;
+  file_bytes_ += e.bytes;
+  spills_++;
+}
+
+// Back into HBM on the compute stream (from a file through pinned memory).
+void CacheManager::restore(Entry &e) {
+  Batch &b = *e.batch;
+  const size_t nc = b.numCols();
+  if (e.tier == FILE_TIER) {
+    FILE *f = std::fopen(e.file.c_str(), "rb");
+    if (!f) throw EngineError("cannot open spill file " + e.file);
+    bool ok = true;
+    for (size_t c = 0; c < nc; c++) {
+      const size_t bytes = b.batchSize * b.columns[c]->bytes;
+      hipCheck(hipHostMalloc(&e.host_cols[c], bytes ? bytes : 1, hipHostMallocDefault), "hipHostMalloc");
+      ok = ok && std::fread(e.host_cols[c], 1, bytes, f) == bytes;
+      if (e.host_nulls[c]) {
+        hipCheck(hipHostMalloc(&e.host_nulls[c], b.batchSize ? b.batchSize : 1, hipHostMallocDefault),
+                 "hipHostMalloc");
+        ok = ok && std::fread(e.host_nulls[c], 1, b.batchSize, f) == b.batchSize;
+      }
+    }
+    std::fclose(f);
+    if (!ok) throw EngineError("short spill file " + e.file);
+    std::remove(e.file.c_str());
+    e.file.clear();
+    file_bytes_ -= e.bytes;
+    host_bytes_ += e.bytes;
+    from_file_++;
+  } else {
+    from_host_++;
+    hipCheck(hipStreamWaitEvent(compute_, e.copied, 0), "hipStreamWaitEvent");
+  }
+  std::vector<void *> release;
+  for (size_t c = 0; c < nc; c++) {
+    const size_t bytes = b.batchSize * b.columns[c]->bytes;
+    b.data[c] = std::make_shared<DeviceBuffer>(bytes, compute_);
+    if (bytes)
+      hipCheck(hipMemcpyAsync(b.data[c]->ptr, e.host_cols[c], bytes, hipMemcpyHostToDevice, compute_),
+               "hipMemcpyAsync");
+    release.push_back(e.host_cols[c]);
+    if (e.host_nulls[c]) {
+      b.nulls[c] = std::make_shared<DeviceBuffer>(b.batchSize, compute_);
+      if (b.batchSize)
+        hipCheck(hipMemcpyAsync(b.nulls[c]->ptr, e.host_nulls[c], b.batchSize, hipMemcpyHostToDevice, compute_),
+                 "hipMemcpyAsync");
+      release.push_back(e.host_nulls[c]);
+    }
+    b.offset[c] = 0;
+  }
+  hipEvent_t done;
+  hipCheck(hipEventCreateWithFlags(&done, hipEventDisableTiming), "hipEventCreate");
+  hipCheck(hipEventRecord(done, compute_), "hipEventRecord");
+  host_retired_.push_back({done, std::move(release)});
+  if (e.copied) (void)hipEventDestroy(e.copied);
+  e.copied = nullptr;
+  e.host_cols.clear();
+  e.host_nulls.clear();
+  e.tier = HBM;
+  host_bytes_ -= e.bytes;
+  dev_bytes_ += e.bytes;
+  peak_ = std::max(peak_, dev_bytes_);
+}
+
+// Evicts least recently used HBM batches (not `keep`) until `bytes` more fit.
+void CacheManager::makeRoom(size_t bytes, BatchID keep) {
+  reapHost();
+  while (budget_ && dev_bytes_ + bytes > budget_) {
+    Entry *victim = nullptr;
+    for (auto &kv : cache_)
+      if (kv.first != keep && kv.second.tier == HBM && kv.second.bytes &&
+          (!victim || kv.second.last_use < victim->last_use))
+        victim = &kv.second;
+    if (!victim) break;  // nothing left to move out: this batch stays in HBM over the budget
+    evictToHost(*victim);
+  }
+  while (!spill_dir_.empty() && host_budget_ && host_bytes_ > host_budget_) {
+    BatchID vid = 0;
+    Entry *victim = nullptr;
+    for (auto &kv : cache_)
+      if (kv.second.tier == HOST && (!victim || kv.second.last_use < victim->last_use)) {
+        victim = &kv.second;
+        vid = kv.first;
+      }
+    if (!victim) break;
+    spillToFile(*victim, vid);
+  }
+}
+
 BatchID CacheManager::putBatch(std::unique_ptr<Batch> b) {
   if (!b) throw EngineError("Invalid batch provided.");
   const size_t sz = b->totalBytes();
-  if (budget_ && bytes_ + sz > budget_)
-    throw EngineError("HBM budget exceeded: " + std::to_string(bytes_ + sz) + " > " + std::to_string(budget_),
-                      YSQL_EOVERFLOW);
-  bytes_ += sz;
-  peak_ = std::max(peak_, bytes_);
+  makeRoom(sz, 0);
   const BatchID id = next_++;
-  cache_[id] = std::move(b);
+  Entry &e = cache_[id];
+  e.batch = std::move(b);
+  e.bytes = sz;
+  e.last_use = ++tick_;
+  dev_bytes_ += sz;
+  peak_ = std::max(peak_, dev_bytes_);
   return id;
 }
+
 std::unique_ptr<Batch> CacheManager::getBatch(BatchID id) {
-  auto it = cache_.find(id);
-  if (it == cache_.end()) throw EngineError("unknown batch id " + std::to_string(id));
-  auto b = std::move(it->second);
-  cache_.erase(it);
-  bytes_ -= b->totalBytes();
+  Entry &e = find(id);
+  if (e.tier != HBM) {
+    makeRoom(e.bytes, id);
+    restore(e);
+  }
+  auto b = std::move(e.batch);
+  dev_bytes_ -= e.bytes;
+  cache_.erase(id);
   return b;
 }
+
 Batch &CacheManager::refBatch(BatchID id) {
-  auto it = cache_.find(id);
-  if (it == cache_.end()) throw EngineError("unknown batch id " + std::to_string(id));
-  return *it->second;
+  Entry &e = find(id);
+  if (e.tier != HBM) {
+    makeRoom(e.bytes, id);
+    restore(e);
+  }
+  e.last_use = ++tick_;
+  return *e.batch;
 }
 
 // ---------------------------------------------------------------- context
@@ -70,10 +303,13 @@ ExecContext::ExecContext(size_t budget, hipStream_t stream) : cache_(budget) {
     hipCheck(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
     own_stream_ = true;
   }
+  cache_.setComputeStream(stream_);
   hipCheck(hipMalloc(&dcount_, 64), "hipMalloc");
   hipCheck(hipHostMalloc(&hcount_, 64, hipHostMallocDefault), "hipHostMalloc");
 }
 ExecContext::~ExecContext() {
+  (void)hipStreamSynchronize(stream_);
+  cache_.clear();  // cached batches free their buffers on this stream
   (void)hipStreamSynchronize(stream_);
   if (ws_) (void)hipFreeAsync(ws_, stream_);
   (void)hipStreamSynchronize(stream_);
@@ -433,10 +669,14 @@ BufPtr OrderOperator::compositeKeys(const Batch &b) {
   return k;
 }
 
-OrderOperator::Run OrderOperator::sortBatch(const Batch &b) {
+// A child batch sorted into a run.  Unless sorting column by column, the
+// sorted composite keys ride along as a last, hidden column (UKEY32 or
+// UKEY64 bits, typed INT / DATETIME for their width), so the run keeps
+// everything its merges need while the cache may move it out of HBM.
+std::unique_ptr<Batch> OrderOperator::sortBatch(const Batch &b) {
   const size_t n = b.batchSize;
   hipStream_t s = ctx.stream();
-  Run r;
+  BufPtr keys;
   auto perm = newBuf(n * 4, s);
   if (by_rows_) {
     size_t wsb = 0, w = 0;
@@ -449,16 +689,16 @@ OrderOperator::Run OrderOperator::sortBatch(const Batch &b) {
     void *ws = ctx.workspace(wsb);
     for (size_t q = keys_.size(); q-- > 0;) sortStep(b, keys_[q], perm, q + 1 == keys_.size(), ws, wsb);
   } else {
-    r.keys = compositeKeys(b);
+    keys = compositeKeys(b);
     const int kdt = wide_ ? YSQL_UKEY64 : YSQL_UKEY32;
     size_t wsb = 0;
     abiCheck(ysql_radix_sort_ws_bytes(kdt, n, &wsb), "ysql_radix_sort_ws_bytes");
     // the packed key occupies its low key_bits_ bits
-    abiCheck(ysql_radix_argsort(kdt, 0, r.keys->ptr, (uint32_t *)perm->ptr, n, 0, key_bits_, ctx.workspace(wsb), wsb,
+    abiCheck(ysql_radix_argsort(kdt, 0, keys->ptr, (uint32_t *)perm->ptr, n, 0, key_bits_, ctx.workspace(wsb), wsb,
                                 s),
              "ysql_radix_argsort");
   }
-  auto out = std::make_shared<Batch>();
+  auto out = std::make_unique<Batch>();
   out->columns = b.columns;
   out->stream = s;
   out->batchSize = n;
@@ -470,81 +710,106 @@ OrderOperator::Run OrderOperator::sortBatch(const Batch &b) {
                          out->nulls[c] ? (uint8_t *)out->nulls[c]->ptr : nullptr, s),
              "ysql_gather");
   }
-  r.batch = out;
-  return r;
+  if (keys) {
+    out->columns.push_back(std::make_shared<Column>("", wide_ ? DataType::DATETIME : DataType::INT));
+    out->data.push_back(keys);
+    out->nulls.push_back(nullptr);
+    out->offset.push_back(0);
+  }
+  return out;
 }
 
-OrderOperator::Run OrderOperator::mergeRuns(const Run &a, const Run &b) {
+// Two runs taken from the cache, merged (stable, the first run first on
+// ties), the result put back.
+BatchID OrderOperator::mergeRuns(BatchID ia, BatchID ib, CacheManager &cache) {
   hipStream_t s = ctx.stream();
-  const size_t m = a.batch->batchSize, n = b.batch->batchSize, k = m + n;
-  Run r;
+  auto a = cache.getBatch(ia);
+  auto b = cache.getBatch(ib);
+  const size_t m = a->batchSize, n = b->batchSize, k = m + n;
+  const size_t nc = columns.size();  // user columns; a composite key column follows them
   auto idx = newBuf(k * 4, s), isa = newBuf(k, s);
+  BufPtr keys;
   if (by_rows_) {  // the runs' key columns compared row against row
     ysql_sortcol sc[YSQL_MAX_COLS];
     for (size_t q = 0; q < keys_.size(); q++)
-      sc[q] = ysql_sortcol{abiType(columns[keys_[q].col]->type), keys_[q].desc ? 1 : 0, a.batch->col(keys_[q].col),
-                           b.batch->col(keys_[q].col)};
+      sc[q] = ysql_sortcol{abiType(columns[keys_[q].col]->type), keys_[q].desc ? 1 : 0, a->col(keys_[q].col),
+                           b->col(keys_[q].col)};
     abiCheck(ysql_merge_rows(sc, (int)keys_.size(), m, n, k, (uint32_t *)idx->ptr, (uint8_t *)isa->ptr, nullptr,
                              nullptr, s),
              "ysql_merge_rows");
   } else {
     const int kdt = wide_ ? YSQL_UKEY64 : YSQL_UKEY32;
-    r.keys = newBuf(k * (wide_ ? 8 : 4), s);
+    keys = newBuf(k * (wide_ ? 8 : 4), s);
     size_t wsb = 0;
     abiCheck(ysql_merge_ws_bytes(m, n, k, &wsb), "ysql_merge_ws_bytes");
-    abiCheck(ysql_merge_path(kdt, 0, a.keys->ptr, m, b.keys->ptr, n, k, r.keys->ptr, (uint32_t *)idx->ptr,
+    abiCheck(ysql_merge_path(kdt, 0, a->col(nc), m, b->col(nc), n, k, keys->ptr, (uint32_t *)idx->ptr,
                              (uint8_t *)isa->ptr, (int64_t *)ctx.deviceCounter(), ctx.workspace(wsb), wsb, s),
              "ysql_merge_path");
   }
-  auto out = std::make_shared<Batch>();
-  out->columns = a.batch->columns;
+  auto out = std::make_unique<Batch>();
+  out->columns = a->columns;
   out->stream = s;
   out->batchSize = k;
-  for (size_t c = 0; c < out->columns.size(); c++) {
-    const bool hasn = a.batch->nulls[c] || b.batch->nulls[c];
+  for (size_t c = 0; c < nc; c++) {
+    const bool hasn = a->nulls[c] || b->nulls[c];
     out->data.push_back(newBuf(k * out->columns[c]->bytes, s));
     out->nulls.push_back(hasn ? newBuf(k, s) : nullptr);
     out->offset.push_back(0);
-    abiCheck(ysql_gather2(out->columns[c]->bytes, a.batch->col(c), a.batch->nul(c), b.batch->col(c), b.batch->nul(c),
+    abiCheck(ysql_gather2(out->columns[c]->bytes, a->col(c), a->nul(c), b->col(c), b->nul(c),
                           (const uint32_t *)idx->ptr, (const uint8_t *)isa->ptr, k, 0, out->data[c]->ptr,
                           hasn ? (uint8_t *)out->nulls[c]->ptr : nullptr, s),
              "ysql_gather2");
   }
-  r.batch = out;
-  return r;
+  if (keys) {
+    out->data.push_back(keys);
+    out->nulls.push_back(nullptr);
+    out->offset.push_back(0);
+  }
+  a.reset();
+  b.reset();
+  return cache.putBatch(std::move(out));
 }
 
 void OrderOperator::firstPass(CacheManager &cache) {
-  // pass 1: every child batch becomes one sorted run (order_operator.hpp:89-121)
+  // pass 1: every child batch becomes one sorted run, kept in the cache
+  // (order_operator.hpp:89-121)
   while (true) {
     const BatchID id = children[0]->next(cache);
     if (!id) break;
     auto b = cache.getBatch(id);
     if (b->batchSize == 0) continue;
     max_batch_ = std::max(max_batch_, b->batchSize);
-    runs_.push_back(sortBatch(*b));
+    runs_.push_back(cache.putBatch(sortBatch(*b)));
   }
   // pass 2: pairwise merges, level by level, carrying an odd run
   // (order_operator.hpp:160-193)
   while (runs_.size() > 1) {
-    std::vector<Run> nxt;
-    for (size_t i = 0; i + 1 < runs_.size(); i += 2) nxt.push_back(mergeRuns(runs_[i], runs_[i + 1]));
+    std::vector<BatchID> nxt;
+    for (size_t i = 0; i + 1 < runs_.size(); i += 2) nxt.push_back(mergeRuns(runs_[i], runs_[i + 1], cache));
     if (runs_.size() % 2) nxt.push_back(runs_.back());
     runs_ = std::move(nxt);
   }
+  if (!runs_.empty()) final_ = std::shared_ptr<Batch>(cache.getBatch(runs_[0]));
+  runs_.clear();
   done_first_ = true;
 }
 
 BatchID OrderOperator::next(CacheManager &cache) {
   if (isFinished) return 0;
   if (!done_first_) firstPass(cache);
-  if (runs_.empty() || emit_pos_ >= runs_[0].batch->batchSize) {
+  if (!final_ || emit_pos_ >= final_->batchSize) {
     isFinished = true;
-    runs_.clear();
+    final_.reset();
     return 0;
   }
-  const size_t n = std::min(max_batch_, runs_[0].batch->batchSize - emit_pos_);
-  auto v = runs_[0].batch->slice(emit_pos_, n);
+  const size_t n = std::min(max_batch_, final_->batchSize - emit_pos_);
+  auto v = final_->slice(emit_pos_, n);
+  while (v->columns.size() > columns.size()) {  // the composite key column stays behind
+    v->columns.pop_back();
+    v->data.pop_back();
+    v->nulls.pop_back();
+    v->offset.pop_back();
+  }
   emit_pos_ += n;
   return cache.putBatch(std::move(v));
 }

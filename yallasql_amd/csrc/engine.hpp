@@ -97,24 +97,67 @@ class Batch {
 
 using BatchID = unsigned;
 
-// BatchID -> Batch, with byte accounting against an HBM budget.  Exceeding
-// the budget raises EngineError (the serving loop admits queries so that it
-// does not happen; the reference spilled to host memory and files instead,
-// cache_manager.hpp:67-88).
+// BatchID -> Batch, and the batches' residency: HBM, pinned host memory, or
+// a spill file (the reference's GPU -> CPU -> FS tiers, cache_manager.hpp:
+// 11-89, batch.cu:6-108, with budgets sized for 288 GB of HBM instead of its
+// 1 GiB / 2 GiB).  When a batch enters and the HBM budget would be exceeded,
+// the least recently used batches are copied to pinned host memory on the
+// manager's own copy stream (after the compute stream's work on them, so the
+// copies overlap later kernels) and their device buffers are released once
+// the copies are done; when the host budget is exceeded the least recently
+// used host batches go to files in the spill directory.  Taking a batch
+// (getBatch / refBatch) brings it back into HBM on the compute stream first.
+// Budgets of 0 are unlimited; with no spill directory the host tier is
+// unlimited.  A batch larger than what the budget leaves once every other
+// batch has moved out stays in HBM over the budget (it is already there).
 class CacheManager {
  public:
-  explicit CacheManager(size_t budget_bytes) : budget_(budget_bytes) {}
+  explicit CacheManager(size_t hbm_budget, size_t host_budget = 0, std::string spill_dir = std::string());
+  ~CacheManager();
+  CacheManager(const CacheManager &) = delete;
+  CacheManager &operator=(const CacheManager &) = delete;
+  void setComputeStream(hipStream_t s) { compute_ = s; }
+  void configure(size_t host_budget, std::string spill_dir);
   BatchID putBatch(std::unique_ptr<Batch> b);
   std::unique_ptr<Batch> getBatch(BatchID id);
   Batch &refBatch(BatchID id);
-  size_t bytesInUse() const { return bytes_; }
+  size_t bytesInUse() const { return dev_bytes_; }  // in HBM
   size_t peakBytes() const { return peak_; }
   size_t budget() const { return budget_; }
+  struct Stats {
+    size_t hbm_bytes, host_bytes, file_bytes, peak_hbm_bytes;
+    uint64_t evictions, spills, restores_from_host, restores_from_file;
+  };
+  Stats stats() const;
+  void clear();  // drops every batch (device buffers freed on their streams)
 
  private:
-  std::unordered_map<BatchID, std::unique_ptr<Batch>> cache_;
+  enum Tier { HBM, HOST, FILE_TIER };
+  struct Entry {
+    std::unique_ptr<Batch> batch;
+    Tier tier = HBM;
+    uint64_t last_use = 0;
+    size_t bytes = 0;
+    std::vector<void *> host_cols, host_nulls;  // HOST tier (nullptr: no null column)
+    hipEvent_t copied = nullptr;                // the copies into the host buffers are done
+    std::string file;                           // FILE tier
+  };
+  void makeRoom(size_t bytes, BatchID keep);
+  void evictToHost(Entry &e);
+  void spillToFile(Entry &e, BatchID id);
+  void restore(Entry &e);
+  void reapHost();
+  Entry &find(BatchID id);
+  std::unordered_map<BatchID, Entry> cache_;
   BatchID next_ = 1;
-  size_t bytes_ = 0, peak_ = 0, budget_ = 0;
+  uint64_t tick_ = 0;
+  size_t dev_bytes_ = 0, host_bytes_ = 0, file_bytes_ = 0, peak_ = 0;
+  size_t budget_ = 0, host_budget_ = 0;
+  std::string spill_dir_;
+  hipStream_t compute_ = nullptr, copy_ = nullptr;
+  uint64_t evictions_ = 0, spills_ = 0, from_host_ = 0, from_file_ = 0;
+  // pinned host buffers released once the copies reading them are done
+  std::vector<std::pair<hipEvent_t, std::vector<void *>>> host_retired_;
 };
 
 // Per-context execution state: the stream, a growable scratch workspace for
@@ -291,13 +334,9 @@ class OrderOperator : public Operator {
   BatchID next(CacheManager &cache) override;
 
  private:
-  struct Run {
-    std::shared_ptr<Batch> batch;  // sorted rows
-    BufPtr keys;                   // sorted composite keys (u32 or u64); none column by column
-  };
   void firstPass(CacheManager &cache);
-  Run sortBatch(const Batch &b);
-  Run mergeRuns(const Run &a, const Run &b);
+  std::unique_ptr<Batch> sortBatch(const Batch &b);
+  BatchID mergeRuns(BatchID a, BatchID b, CacheManager &cache);
   BufPtr compositeKeys(const Batch &b);
   void sortStep(const Batch &b, const OrderKey &key, BufPtr &perm, bool first, void *ws, size_t wsb);
   std::vector<OrderKey> keys_;
@@ -305,7 +344,8 @@ class OrderOperator : public Operator {
   bool wide_ = false;     // composite key is 64 bits
   bool by_rows_ = false;  // string keys or keys wider than 64 bits: column by column
   bool done_first_ = false;
-  std::vector<Run> runs_;
+  std::vector<BatchID> runs_;     // sorted runs, held by the cache (may leave HBM)
+  std::shared_ptr<Batch> final_;  // the merged result being emitted
   size_t max_batch_ = 0, emit_pos_ = 0;
 };
 

@@ -49,6 +49,18 @@ ysql_ctx *ysql_ctx_create(uint64_t budget, void *stream) {
 }
 void ysql_ctx_destroy(ysql_ctx *c) { delete c; }
 uint64_t ysql_ctx_peak_bytes(const ysql_ctx *c) { return c ? const_cast<ysql_ctx *>(c)->ctx.cache().peakBytes() : 0; }
+int ysql_ctx_set_spill(ysql_ctx *c, uint64_t host_budget, const char *dir) {
+  if (!c) return YSQL_EINVAL;
+  c->ctx.cache().configure(host_budget, dir ? std::string(dir) : std::string());
+  return YSQL_OK;
+}
+int ysql_ctx_cache_stats(const ysql_ctx *c, ysql_cache_stats *out) {
+  if (!c || !out) return YSQL_EINVAL;
+  const auto st = const_cast<ysql_ctx *>(c)->ctx.cache().stats();
+  *out = ysql_cache_stats{st.hbm_bytes, st.host_bytes, st.file_bytes, st.peak_hbm_bytes,
+                          st.evictions, st.spills, st.restores_from_host, st.restores_from_file};
+  return YSQL_OK;
+}
 
 ysql_table *ysql_table_create(ysql_ctx *ctx, const char *name) {
   if (!ctx) {

@@ -597,6 +597,71 @@ __global__ __launch_bounds__(1024) void k_star(StarArgs a, int nd, const int32_t
   }
 }
 
+// Several star-join queries sharing one read of the facts (the serving
+// loop's batching): they differ only in their dimension filters, so bit q
+// of a dimension's pass byte is query q's filter.  A row's group value is
+// the same for every query; the row folds into the group arrays of each
+// query whose bit survives (sums[q * ng + g], counts[q * ng + g]).
+// Integer folds commute, so every query's result equals its lone run.
+__global__ __launch_bounds__(1024) void k_star_multi(StarArgs a, int nd, uint32_t qmask,
+                                                     const int32_t *__restrict__ m1, const int32_t *__restrict__ m2,
+                                                     uint64_t n, int64_t *sums, uint64_t *counts, uint32_t ng) {
+  const uint64_t step = (uint64_t)gridDim.x * blockDim.x * 4;
+  for (uint64_t r = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4; r < n; r += step) {
+    int32_t f[YSQL_MAX_DIMS][4], x1[4], x2[4];
+#pragma unroll
+    for (int d = 0; d < YSQL_MAX_DIMS; d++)
+      if (d < nd) star_load4(a.d[d].fk, r, n, f[d]);
+    star_load4(m1, r, n, x1);
+    star_load4(m2, r, n, x2);
+    uint32_t mk[4];
+#pragma unroll
+    for (int e = 0; e < 4; e++) mk[e] = r + e < n ? qmask : 0u;
+#pragma unroll
+    for (int d = 0; d < YSQL_MAX_DIMS; d++)
+      if (d < nd) {
+#pragma unroll
+        for (int e = 0; e < 4; e++)
+          if (!(f[d][e] >= 0 && (uint32_t)f[d][e] < a.d[d].rows)) mk[e] = 0;
+      }
+#pragma unroll
+    for (int d = 0; d < YSQL_MAX_DIMS; d++) {
+      if (d >= nd || !a.d[d].pass) continue;
+      uint32_t pv[4];
+#pragma unroll
+      for (int e = 0; e < 4; e++) pv[e] = mk[e] ? (uint32_t)a.d[d].pass[f[d][e]] : 0u;
+#pragma unroll
+      for (int e = 0; e < 4; e++) mk[e] &= pv[e];
+    }
+    uint32_t g[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int d = 0; d < YSQL_MAX_DIMS; d++) {
+      if (d >= nd || !a.d[d].group) continue;
+      int32_t gv[4];
+#pragma unroll
+      for (int e = 0; e < 4; e++) gv[e] = mk[e] ? a.d[d].group[f[d][e]] : a.d[d].group_base;
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        const int64_t v = (int64_t)gv[e] - a.d[d].group_base;
+        if (!(v >= 0 && v < (int64_t)a.d[d].group_card)) mk[e] = 0;
+        g[e] += (uint32_t)v * a.stride[d];
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      uint32_t m = mk[e];
+      if (!m) continue;
+      const long long prod = (long long)x1[e] * (long long)x2[e];
+      while (m) {
+        const uint32_t q = (uint32_t)__builtin_ctz(m);
+        m &= m - 1;
+        atomicAdd((unsigned long long *)&sums[(uint64_t)q * ng + g[e]], (unsigned long long)prod);
+        atomicAdd((unsigned long long *)&counts[(uint64_t)q * ng + g[e]], 1ull);
+      }
+    }
+  }
+}
+
 struct GidArgs {
   const int32_t *col[4];
   int32_t base[4];
@@ -745,11 +810,8 @@ int ysql_group_reduce(int dtype, int agg, const uint32_t *group_id, uint32_t n_g
   return YSQL_EINVAL;
 }
 
-int ysql_star_join_group_sum(const ysql_star_dim *dims, int ndims, const int32_t *m1, const int32_t *m2, uint64_t n,
-                             int64_t *sums, uint64_t *counts, uint32_t n_groups, ysql_stream_t stream) {
-  if (ndims < 1 || ndims > YSQL_MAX_DIMS || !dims || !m1 || !m2 || !sums || !counts) return YSQL_EINVAL;
-  if (n_groups == 0 || n_groups > YSQL_MAX_GROUPS) return YSQL_EINVAL;
-  StarArgs a;
+static int star_args(const ysql_star_dim *dims, int ndims, uint32_t n_groups, StarArgs *out) {
+  StarArgs &a = *out;
   memset(&a, 0, sizeof(a));
   uint64_t stride = 1;
   for (int d = ndims - 1; d >= 0; d--) {
@@ -761,7 +823,33 @@ int ysql_star_join_group_sum(const ysql_star_dim *dims, int ndims, const int32_t
       stride *= dims[d].group_card;
     }
   }
-  if (stride > n_groups) return YSQL_EINVAL;
+  return stride > n_groups ? YSQL_EINVAL : YSQL_OK;
+}
+
+int ysql_star_join_group_sum_multi(const ysql_star_dim *dims, int ndims, int nq, const int32_t *m1, const int32_t *m2,
+                                   uint64_t n, int64_t *sums, uint64_t *counts, uint32_t n_groups,
+                                   ysql_stream_t stream) {
+  if (ndims < 1 || ndims > YSQL_MAX_DIMS || !dims || !m1 || !m2 || !sums || !counts) return YSQL_EINVAL;
+  if (n_groups == 0 || n_groups > YSQL_MAX_GROUPS || nq < 1 || nq > 8) return YSQL_EINVAL;
+  StarArgs a;
+  if (int rc = star_args(dims, ndims, n_groups, &a)) return rc;
+  if (n == 0) return YSQL_OK;
+  hipStream_t s = as_stream(stream);
+  uint64_t g = ceil_div(n, 1024 * 16);
+  const uint64_t cap = (uint64_t)grid_cap() * 2;
+  if (g > cap) g = cap;
+  if (g < 1) g = 1;
+  k_star_multi<<<(unsigned)g, 1024, 0, s>>>(a, ndims, (1u << nq) - 1u, m1, m2, n, sums, counts, n_groups);
+  YSQL_LAUNCH_CHECK();
+  return YSQL_OK;
+}
+
+int ysql_star_join_group_sum(const ysql_star_dim *dims, int ndims, const int32_t *m1, const int32_t *m2, uint64_t n,
+                             int64_t *sums, uint64_t *counts, uint32_t n_groups, ysql_stream_t stream) {
+  if (ndims < 1 || ndims > YSQL_MAX_DIMS || !dims || !m1 || !m2 || !sums || !counts) return YSQL_EINVAL;
+  if (n_groups == 0 || n_groups > YSQL_MAX_GROUPS) return YSQL_EINVAL;
+  StarArgs a;
+  if (int rc = star_args(dims, ndims, n_groups, &a)) return rc;
   if (n == 0) return YSQL_OK;
   hipStream_t s = as_stream(stream);
   uint64_t g = ceil_div(n, 1024 * 16);

@@ -32,14 +32,15 @@ def multirun_keys(n, device, seed=1):
 STAR_DIM_ROWS = (2556, 2000, 30000, 200000)  # date, supplier, customer, part
 
 
-def star_schema(n, device, seed=2):
+def star_schema(n, device, seed=2, attrs=False):
     """Star-join fact table and dimensions: four int32 foreign keys uniform
     over the dimension rows, extendedprice int32 in [90000, 10^7] cents,
     discount int32 in [0, 10]; date.year = 1992 + floor(7 i / 2556),
     customer.region in [0, 7) filtered = 0, part.category in [0, 25)
     filtered = 0, part.brand in [0, 1000).  Returns the dimension list of
     ops.star_join_group_sum (GROUP BY year x brand, 7,000 groups) and the two
-    measures."""
+    measures; with `attrs`, also {"region": customer.region, "cat":
+    part.category} for filters with other constants."""
     g = torch.Generator(device=device)
     g.manual_seed(seed)
     fks = [torch.randint(0, R, (n,), generator=g, device=device, dtype=torch.int32) for R in STAR_DIM_ROWS]
@@ -53,4 +54,6 @@ def star_schema(n, device, seed=2):
             {"fk": fks[1], "rows": 2000},
             {"fk": fks[2], "rows": 30000, "pass": (region == 0).to(torch.uint8)},
             {"fk": fks[3], "rows": 200000, "pass": (cat == 0).to(torch.uint8), "group": brand, "group_card": 1000}]
+    if attrs:
+        return dims, m1, m2, {"region": region, "cat": cat}
     return dims, m1, m2

@@ -21,6 +21,8 @@ _sig = {
     "ysql_ctx_create": ([C.c_uint64, P], P),
     "ysql_ctx_destroy": ([P], None),
     "ysql_ctx_peak_bytes": ([P], C.c_uint64),
+    "ysql_ctx_set_spill": ([P, C.c_uint64, C.c_char_p], C.c_int),
+    "ysql_ctx_cache_stats": ([P, P], C.c_int),
     "ysql_table_create": ([P, C.c_char_p], P),
     "ysql_table_add_column": ([P, C.c_char_p, C.c_int, P, P, C.c_uint64], C.c_int),
     "ysql_table_add_device_column": ([P, C.c_char_p, C.c_int, P, P, C.c_uint64], C.c_int),
@@ -214,10 +216,25 @@ class Table:
             self.h = None
 
 
+class CacheStats(C.Structure):
+    _fields_ = [(f, C.c_uint64) for f in ("hbm_bytes", "host_bytes", "file_bytes", "peak_hbm_bytes", "evictions",
+                                           "spills", "restores_from_host", "restores_from_file")]
+
+
 class Context:
-    def __init__(self, hbm_budget=0, stream=None):
+    def __init__(self, hbm_budget=0, stream=None, host_budget=0, spill_dir=None):
+        """hbm_budget: bytes of cached batches kept in HBM (0 = unlimited);
+        past it the least recently used go to pinned host memory, and past
+        host_budget to files in spill_dir (None: no file tier)."""
         self.h = _chk_ptr(lib.ysql_ctx_create(hbm_budget, stream), "context")
         self._tables = []
+        if host_budget or spill_dir:
+            lib.ysql_ctx_set_spill(self.h, host_budget, None if spill_dir is None else str(spill_dir).encode())
+
+    def cache_stats(self):
+        st = CacheStats()
+        lib.ysql_ctx_cache_stats(self.h, C.byref(st))
+        return {f: getattr(st, f) for f, _ in CacheStats._fields_}
 
     def table(self, name, columns, nulls=None):
         """columns: list of (name, numpy array | (n, 256) uint8 strings)."""

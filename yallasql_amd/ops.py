@@ -274,10 +274,11 @@ def filter_compact(cmp, pred, scalar, srcs, nulls=None):
     return [(d[:c], dn[:c]) for d, dn in outs], c
 
 
-def filter_compact_range(pred, lo, hi, srcs, cap=None):
+def filter_compact_range(pred, lo, hi, srcs, cap=None, retry=True):
     """Rows with lo <= pred < hi, every column of srcs compacted (no nulls).
     Output buffers hold `cap` rows (default n); on overflow the call is
-    repeated with the exact count.  Returns ([dst], count)."""
+    repeated with the exact count, or with retry=False the first `cap` rows
+    in row order are returned.  Returns ([dst], count)."""
     n = pred.numel()
     dev = pred.device
     cap = n if cap is None else max(int(cap), 1)
@@ -291,9 +292,9 @@ def filter_compact_range(pred, lo, hi, srcs, cap=None):
     call("ysql_filter_compact_range", dtype_code(pred), _p(pred), _p(bounds), cols, len(srcs), n, cap, _p(cnt),
          _p(ws), ws.numel(), _s())
     c = int(cnt.item())
-    if c > cap:
+    if c > cap and retry:
         return filter_compact_range(pred, lo, hi, srcs, cap=c)
-    return [d[:c] for d in outs], c
+    return [d[:min(c, cap)] for d in outs], c
 
 
 # ---- A12 / A15 ---------------------------------------------------------
@@ -448,4 +449,23 @@ def star_join_group_sum(dims, m1, m2, n_groups, sums=None, counts=None):
     if counts is None:
         counts = torch.zeros(n_groups, dtype=torch.int64, device=dev)
     call("ysql_star_join_group_sum", arr, len(dims), _p(m1), _p(m2), m1.numel(), _p(sums), _p(counts), n_groups, _s())
+    return sums, counts
+
+
+def star_join_group_sum_multi(dims, m1, m2, n_groups, nq, sums=None, counts=None):
+    """nq star-join queries in one read of the facts: bit q of each dims[d]
+    "pass" byte is query q's filter.  sums / counts: (nq, n_groups)."""
+    dev = m1.device
+    arr = (L.StarDim * len(dims))()
+    for i, d in enumerate(dims):
+        arr[i] = L.StarDim(d["fk"].data_ptr(), d["rows"], d.get("group_card", 0),
+                           d["pass"].data_ptr() if d.get("pass") is not None else None,
+                           d["group"].data_ptr() if d.get("group") is not None else None,
+                           d.get("group_base", 0), 0)
+    if sums is None:
+        sums = torch.zeros((nq, n_groups), dtype=torch.int64, device=dev)
+    if counts is None:
+        counts = torch.zeros((nq, n_groups), dtype=torch.int64, device=dev)
+    call("ysql_star_join_group_sum_multi", arr, len(dims), nq, _p(m1), _p(m2), m1.numel(), _p(sums), _p(counts),
+         n_groups, _s())
     return sums, counts

@@ -6,25 +6,34 @@ dispatcher admits them as execution slots and device memory free up,
 batches operators across the queries in flight, and keeps every
 intermediate on the device; only final results come back to the host.
 
-Query kinds over the table T(key int64, payload int32 = row id) and a
-dimension D(dkey int64, grp int32):
+Query kinds over the table T(key int64, payload int32 = row id) and the
+star schema of the star-join workload (a fact table F with four foreign keys
+and two measures; dimensions date, supplier, customer, part):
   FILTER_AGG  SELECT COUNT(*), SUM(payload), MIN(key), MAX(key)
               FROM T WHERE key < c
   TOPK        SELECT key, payload FROM T WHERE key >= lo
               ORDER BY key LIMIT 1000        (ties in row order)
-  JOIN_GROUP  SELECT grp, SUM(payload), COUNT(*) FROM T JOIN D ON key = dkey
-              WHERE payload < c GROUP BY grp
+  JOIN_GROUP  SELECT year, brand, SUM(extendedprice * discount), COUNT(*)
+              FROM F JOIN date JOIN supplier JOIN customer JOIN part
+              WHERE customer.region = r AND part.category = c
+              GROUP BY year, brand           (7,000 groups)
 
 Operator batching across queries: up to 4 FILTER_AGG queries in flight
 share one scan of the table (ysql_filter_aggregate_multi, one read of the
-columns for all of them); TOPK and JOIN_GROUP pipelines run concurrently on
+columns for all of them), and up to 8 JOIN_GROUP queries share one read of
+the fact table (ysql_star_join_group_sum_multi, each query's filters as one
+bit of the dimension filter bytes); TOPK pipelines run concurrently on
 their own streams.  A query's result does not depend on what it was batched
 with: the shared scan folds each query in the same order as a lone scan,
 and every other operator runs per query.
 
 Admission: each unit of work (a shared scan, or one TOPK / JOIN_GROUP
-pipeline) reserves an estimate of its device memory from the HBM budget and
-waits (in arrival order) while the budget or every slot is taken.
+pipeline) reserves from the HBM budget the device memory it allocates,
+computed from the same sizes and workspace queries its operators use
+(each allocation rounded up to the device allocator's 512-byte blocks), and
+waits (in arrival order) while the budget or every slot is taken.  A TOPK
+whose key window needs a third widening (64 times the estimate) allocates
+past its reservation.
 """
 import collections
 import contextlib
@@ -37,20 +46,24 @@ import torch
 
 from . import _lib as L
 from . import ops
-from .datagen import multirun_keys
+from .datagen import multirun_keys, star_schema
 
 FILTER_AGG, TOPK, JOIN_GROUP = 0, 1, 2
 KIND_NAMES = {FILTER_AGG: "filter_agg", TOPK: "topk", JOIN_GROUP: "join_group"}
 TOPK_K = 1000
+TOPK_CAPS = (16 * TOPK_K, 128 * TOPK_K)  # compaction capacity of the first two window widths
+STAR_GROUPS = 7000  # 7 years x 1,000 brands
+STAR_REGIONS, STAR_CATEGORIES = 7, 25
 SHARED_SCAN_MAX = 4
+SHARED_STAR_MAX = 8
 I64_MAX = 2**63 - 1
 
 
 class Table:
-    """T(key, payload) resident in HBM, and the join dimension D."""
+    """T(key, payload) and the star schema, resident in HBM."""
 
-    def __init__(self, n, device, seed=1, dim_rows=2000, groups=64, sample=1 << 20):
-        self.n, self.device, self.groups = n, device, groups
+    def __init__(self, n, device, seed=1, star_rows=None, sample=1 << 20):
+        self.n, self.device = n, device
         self.key = multirun_keys(n, device, seed=seed)
         self.payload = torch.arange(n, dtype=torch.int32, device=device)
         g = torch.Generator(device=device)
@@ -58,17 +71,15 @@ class Table:
         # sorted key sample: statistics for choosing TOPK windows
         idx = torch.randint(0, n, (min(sample, n),), generator=g, device=device)
         self.sample = np.sort(self.key[idx].cpu().numpy())
-        # D: distinct ranks of the Zipf half away from the head (each matches
-        # a few hundred rows or fewer), random groups
-        r = np.random.default_rng(seed)
-        dk = r.choice(np.arange(1000, 10_000_000, dtype=np.int64), size=dim_rows, replace=False)
-        self.dkey = torch.from_numpy(dk).to(device)
-        self.dgrp = torch.from_numpy(r.integers(0, groups, dim_rows).astype(np.int32)).to(device)
+        self.star_rows = n if star_rows is None else star_rows
+        self.star = star_schema(self.star_rows, device, seed=seed + 7, attrs=True)
 
     def host(self):
-        """Host copies (tests compare against numpy)."""
-        return (self.key.cpu().numpy(), self.payload.cpu().numpy(), self.dkey.cpu().numpy(),
-                self.dgrp.cpu().numpy())
+        """Host copies (tests compare against numpy and the oracle)."""
+        dims, m1, m2, attrs = self.star
+        hd = [{k: (v.cpu().numpy() if hasattr(v, "cpu") else v) for k, v in d.items()} for d in dims]
+        star = (hd, m1.cpu().numpy(), m2.cpu().numpy(), {k: v.cpu().numpy() for k, v in attrs.items()})
+        return self.key.cpu().numpy(), self.payload.cpu().numpy(), star
 
     def topk_window(self, lo, widen=1):
         """[lo, hi) expected to hold about 4 * TOPK_K * widen rows, by the
@@ -110,7 +121,7 @@ def make_queries(table, count, rate_qps=None, seed=0, mix=(0.5, 0.3, 0.2)):
         elif k == TOPK:
             param = int(s[r.integers(0, len(s))])  # key >= lo
         else:
-            param = int(r.integers(0, table.n + 1))  # payload < c
+            param = (int(r.integers(0, STAR_REGIONS)), int(r.integers(0, STAR_CATEGORIES)))  # region, category
         out.append(Query(q, k, param, float(t[q])))
     return out
 
@@ -138,54 +149,143 @@ def run_filter_agg(table, queries):
     return table.n
 
 
+def _range(table, lo, hi, cap):
+    """The first min(c, cap) rows in row order with lo <= key < hi (keys,
+    payloads), and c."""
+    (k, p), c = ops.filter_compact_range(table.key, lo, hi, [table.key, table.payload], cap=cap, retry=False)
+    return k, p, c
+
+
+def _first_sorted(k, p, m):
+    """The first m of (k, p) stably sorted by key (non-negative int64 keys:
+    their raw bits order like the values)."""
+    perm = ops.radix_argsort(k.clone().view(torch.int64))
+    take = perm[:m]
+    return ops.gather(k, take)[0], ops.gather(p, take)[0]
+
+
 def run_topk(table, q):
-    lo = q.param
-    widen, scans = 1, 0
-    while True:
+    """ORDER BY key LIMIT 1000 over key >= lo.  The key sample picks a
+    window [lo, hi) of about 4,000 rows; a window with too few rows is
+    widened 8 times, then to the top of the key range.  A window holding
+    more rows than the compaction capacity (a hot key) is not compacted
+    whole: count-only scans bisect for the key s at which the count from lo
+    first reaches 1,000; every row below s is taken and sorted, then the
+    first rows of key s in row order."""
+    lo, K = q.param, TOPK_K
+    scans = 0
+    hi = I64_MAX
+    dense = False
+    for cap, widen in ((TOPK_CAPS[0], 1), (TOPK_CAPS[1], 8), (TOPK_CAPS[1], None)):
+        lo_, hi = table.topk_window(lo, widen) if widen else (lo, I64_MAX)
+        k, p, c = _range(table, lo_, hi, cap)
         scans += 1
-        lo_, hi = table.topk_window(lo, widen)
-        (k, p), c = ops.filter_compact_range(table.key, lo_, hi, [table.key, table.payload], cap=16 * TOPK_K * widen)
-        if c >= TOPK_K or hi == I64_MAX:
+        if c > cap:
+            dense = True
             break
-        widen *= 8
-    if c:
-        kk = k.clone()  # non-negative int64: the raw bits order like the values
-        perm = ops.radix_argsort(kk.view(torch.int64))
-        take = perm[:TOPK_K]
-        keys, _ = ops.gather(k, take)
-        pays, _ = ops.gather(p, take)
-        q.result = (keys.cpu().numpy(), pays.cpu().numpy())
-    else:
-        q.result = (np.zeros(0, np.int64), np.zeros(0, np.int32))
+        if c >= K or hi == I64_MAX:
+            if c:
+                keys, pays = _first_sorted(k, p, K)
+                q.result = (keys.cpu().numpy(), pays.cpu().numpy())
+            else:
+                q.result = (np.zeros(0, np.int64), np.zeros(0, np.int32))
+            return table.n * scans
+    assert dense
+    del k, p
+    L_, H = lo, hi  # count(lo, L_) < K <= count(lo, H)
+    while H - L_ > 1:
+        mid = L_ + (H - L_) // 2
+        _, _, cm = _range(table, lo, mid, 1)
+        scans += 1
+        if cm >= K:
+            H = mid
+        else:
+            L_ = mid
+    s_ = L_
+    k1, p1, c1 = _range(table, lo, s_, K)  # c1 < K: every row below s
+    k2, p2, c2 = _range(table, s_, s_ + 1, K - c1)  # the first rows of key s
+    scans += 2
+    if c1:
+        k1, p1 = _first_sorted(k1, p1, c1)
+    q.result = (torch.cat([k1, k2]).cpu().numpy(), torch.cat([p1, p2]).cpu().numpy())
     return table.n * scans
 
 
-def run_join_group(table, q):
+def join_group_dims(table, params):
+    """The star join's dimension list for queries with these (region,
+    category) constants: one query's 0/1 filters, or for several queries
+    filter bytes whose bit q is query q's."""
+    dims, _, _, attrs = table.star
+    d = [dict(x) for x in dims]
+    if len(params) == 1:
+        d[2]["pass"] = (attrs["region"] == params[0][0]).to(torch.uint8)
+        d[3]["pass"] = (attrs["cat"] == params[0][1]).to(torch.uint8)
+        return d
     dev = table.device
-    pairs, c = ops.join_pairs(L.EQ, table.key, table.dkey, cap=1 << 16)
-    G = table.groups
-    sums = ops.acc_new(L.INT, L.AGG_SUM, nacc=G, device=dev)
-    cnts = ops.acc_new(L.INT, L.AGG_COUNT, nacc=G, device=dev)
-    if c:
-        pay, _ = ops.gather(table.payload, pairs[:, 0].contiguous())
-        grp, _ = ops.gather(table.dgrp, pairs[:, 1].contiguous())
-        cst = torch.tensor([q.param], dtype=torch.int32, device=dev)
-        skip = ops.compare(L.GE, pay, cst, rhs_scalar=True)  # payload >= c: not selected
-        ops.group_reduce(L.INT, L.AGG_SUM, grp, G, pay, sums, null=skip)
-        ops.group_reduce(L.INT, L.AGG_COUNT, grp, G, pay, cnts, null=skip)
-    s = ops.acc_read(sums)
-    n = ops.acc_read(cnts)
-    q.result = (np.array([a["i"] for a in s], np.int64), np.array([a["count"] for a in n], np.int64))
-    return table.n
+    weights = torch.tensor([1 << q for q in range(len(params))], dtype=torch.uint8, device=dev)
+    for j, (name, k) in ((2, ("region", 0)), (3, ("cat", 1))):
+        consts = torch.tensor([p[k] for p in params], dtype=torch.int32, device=dev)
+        hit = attrs[name].unsqueeze(1) == consts.unsqueeze(0)
+        d[j]["pass"] = (hit.to(torch.uint8) * weights).sum(dim=1, dtype=torch.uint8)
+    return d
+
+
+def run_join_group(table, queries):
+    """Up to SHARED_STAR_MAX star-join queries in one read of the facts."""
+    _, m1, m2, _ = table.star
+    dims = join_group_dims(table, [q.param for q in queries])
+    if len(queries) == 1:
+        sums, counts = ops.star_join_group_sum(dims, m1, m2, STAR_GROUPS)
+        sums, counts = sums.view(1, -1), counts.view(1, -1)
+    else:
+        sums, counts = ops.star_join_group_sum_multi(dims, m1, m2, STAR_GROUPS, len(queries))
+    hs, hc = sums.cpu().numpy(), counts.cpu().numpy()
+    for q, a, b in zip(queries, hs, hc):
+        q.result = (a, b)
+    return table.star_rows
+
+
+def _blk(nbytes):
+    """A device allocation as the caching allocator accounts it (512-byte blocks)."""
+    return max(512, (int(nbytes) + 511) // 512 * 512)
 
 
 def unit_bytes(table, kind, nq=1):
-    """Device memory a unit reserves from the budget (an upper estimate)."""
+    """Device memory a unit allocates, reserved from the budget at admission:
+    the sum of its operators' buffers and workspaces."""
+    n = table.n
     if kind == FILTER_AGG:
-        return (table.n // 32768 + 2) * 32 * 4 * nq + (1 << 20)
+        ws = L.ws_bytes("ysql_filter_aggregate_multi_ws_bytes", n, nq, 4)
+        # constants, workspace, 4 accumulators per query, the per-query
+        # concatenation and the stacked copy read back to the host
+        return _blk(8 * nq) + _blk(max(ws, 256)) + nq * 4 * _blk(32) + nq * _blk(128) + _blk(128 * nq)
     if kind == TOPK:
-        return 16 * TOPK_K * 8 * 12 * 3 + (8 << 20)
-    return (1 << 16) * 8 * 3 + (16 << 20)
+        cws = _blk(max(L.ws_bytes("ysql_compact_ws_bytes", n), 256))
+
+        def compaction(cap):  # bounds, two output columns, count, workspace
+            return _blk(16) + _blk(8 * cap) + _blk(4 * cap) + _blk(8) + cws
+
+        def first_sorted(c, m):  # key copy, permutation, sort workspace, two gathers
+            rws = L.ws_bytes("ysql_radix_sort_ws_bytes", L.UKEY64, c)
+            return _blk(8 * c) + _blk(4 * c) + _blk(max(rws, 256)) + _blk(8 * m) + _blk(m) + _blk(4 * m) + _blk(m)
+
+        K = TOPK_K
+        # two window widths alive at once (the wider window's rows sorted),
+        # or the hot-key path: count-only scans, then rows below s and of s,
+        # the sort of the former and the concatenation of the result
+        windows = 2 * compaction(TOPK_CAPS[1]) + first_sorted(TOPK_CAPS[1], K)
+        hot = compaction(1) + 2 * compaction(K) + first_sorted(K, K) + _blk(8 * K) + _blk(4 * K)
+        return max(windows, hot)
+    # JOIN_GROUP: the two dimension filters, sums and counts.  One query:
+    # a bool then a uint8 per filtered dimension row; nq queries: the bit
+    # weights, and per dimension the constants, the row x query comparison,
+    # its uint8 copy, the weighted copy and the summed filter byte.
+    rows = (30000, 200000)
+    if nq == 1:
+        filters = sum(2 * _blk(r) for r in rows)
+    else:
+        filters = _blk(nq) + sum(_blk(4 * nq) + 3 * _blk(r * nq) + _blk(r) for r in rows)
+    return filters + 2 * _blk(8 * STAR_GROUPS * nq)
 
 
 # ---------------------------------------------------------------------------
@@ -201,7 +301,7 @@ class Server:
         self.table = table
         # unit runners by query kind: fn(table, queries) -> table rows read
         self.runners = runners or {FILTER_AGG: run_filter_agg, TOPK: lambda t, qs: run_topk(t, qs[0]),
-                                   JOIN_GROUP: lambda t, qs: run_join_group(t, qs[0])}
+                                   JOIN_GROUP: run_join_group}
         self.error = None  # the first exception a unit raised; serve() re-raises it
         self.slots = slots
         self.budget = hbm_budget
@@ -265,6 +365,8 @@ class Server:
         head = pending[0]
         if head.kind == FILTER_AGG and self.shared_scan:
             return head.kind, [q for q in pending if q.kind == FILTER_AGG][:SHARED_SCAN_MAX]
+        if head.kind == JOIN_GROUP and self.shared_scan:
+            return head.kind, [q for q in pending if q.kind == JOIN_GROUP][:SHARED_STAR_MAX]
         return head.kind, [head]
 
     def serve(self, queries, realtime=True):
