@@ -38,6 +38,14 @@ ysql_ctx *ysql_ctx_create(uint64_t hbm_budget_bytes, void *stream);
 void ysql_ctx_destroy(ysql_ctx *ctx);
 /* Peak bytes of batches held in HBM by the context's CacheManager so far. */
 uint64_t ysql_ctx_peak_bytes(const ysql_ctx *ctx);
+/* Expression fusion (on by default): a filter predicate or projection
+ * tree runs as one fused program per batch (ysql_expr_filter_compact /
+ * ysql_expr_eval, include/ysql_kernels.h) instead of one kernel and one
+ * intermediate column per node (the reference's evaluate() chain,
+ * include/engine/operators/expressions/).  on = 0 runs the nodes one by
+ * one; both give the same columns.  A tree the fused program cannot express
+ * (strings, mixed operand types) always runs node by node.               */
+int ysql_ctx_set_fusion(ysql_ctx *ctx, int on);
 /* Residency tiers below HBM (the reference's CPU and FS caches,
  * cache_manager.hpp:67-88): when the HBM budget would be exceeded, least
  * recently used cached batches (ORDER BY's sorted runs among them) move to

@@ -363,6 +363,82 @@ int ysql_cast(int src_dtype, int dst_dtype, const void *in, void *out, uint64_t 
               ysql_stream_t stream);
 
 /* ======================================================================
+ * Fused expression programs (expression fusion)
+ * Replace the node-at-a-time evaluation of a predicate or projection tree:
+ * BoundFuncExpression::evaluate (bound_func.hpp:45-167), CastExpression
+ * (cast.hpp:22-90), ComparisonExpression (filters/comparison.hpp:51-215) and
+ * ConjuntionExpression (filters/conjunction.hpp:76-113), each of which
+ * launches one kernel into a freshly allocated column per tree node.  Here a
+ * tree is a short register program evaluated per row in one pass: input
+ * columns are read once, no intermediate column is written to HBM.
+ *
+ * Registers 0 .. ninputs-1 hold the input columns; the other registers up to
+ * YSQL_XMAX_REGS - 1 are temporaries.  A register holds one typed value per
+ * row (dtype INT, FLOAT, DATETIME or BOOL) and a NULL flag per row.
+ *   YSQL_XOP_CONST  r[dst] = imm (dtype `type`: INT/DATETIME take imm as an
+ *                   integer, FLOAT the float whose bits are imm's low 32 bits);
+ *                   NULL flag 0
+ *   YSQL_XOP_CMP    r[dst] = r[a] <sub> r[b], both of dtype `type` -> BOOL
+ *                   (ysql_compare's rule, sub = YSQL_LT..YSQL_NE)
+ *   YSQL_XOP_LOGIC  r[dst] = r[a] AND/OR r[b] (sub = YSQL_AND/YSQL_OR), BOOL
+ *   YSQL_XOP_ARITH  r[dst] = r[a] <sub> r[b] over dtype `type` (ysql_arith's
+ *                   rules: wrap, /0 -> 0, REM for integers only)
+ *   YSQL_XOP_CAST   r[dst] = r[a] converted from dtype `type` to dtype `sub`
+ *                   (ysql_cast's rules; BOOL sources to INT only)
+ *   YSQL_XOP_NOT    r[dst] = !r[a] (BOOL; the `isneg` of the reference's
+ *                   comparison and conjunction nodes)
+ * Every instruction's result takes the NULL flags of register `nsrc`
+ * (-1: none), the rule of the node it replaces (the non-constant operand's
+ * nullset, bound_func.hpp:165, filters/comparison.hpp:213; the child's, cast.hpp:87;
+ * the left operand's, filters/conjunction.hpp:111).  Operations run on the stored
+ * values of NULL rows, as the per-node kernels do.                          */
+enum {
+  YSQL_XOP_CONST = 1,
+  YSQL_XOP_CMP = 2,
+  YSQL_XOP_LOGIC = 3,
+  YSQL_XOP_ARITH = 4,
+  YSQL_XOP_CAST = 5,
+  YSQL_XOP_NOT = 6
+};
+#define YSQL_XMAX_REGS 16
+#define YSQL_XMAX_INSN 48
+#define YSQL_XMAX_INPUTS 8
+typedef struct ysql_xinsn {
+  int16_t op;   /* YSQL_XOP_* */
+  int16_t type; /* operand dtype (CONST: the constant's dtype) */
+  int16_t sub;  /* comparison / logic / arithmetic op, or CAST's target dtype */
+  int8_t dst, a, b;
+  int8_t nsrc;  /* register whose NULL flags the result takes, -1 = none */
+  int64_t imm;  /* CONST value */
+} ysql_xinsn;
+typedef struct ysql_xinput {
+  const void *data;     /* device column of `dtype` (INT, FLOAT, DATETIME) */
+  const uint8_t *nulls; /* device null bytes, or NULL: no NULLs */
+  int32_t dtype;
+  int32_t pad;
+} ysql_xinput;
+typedef struct ysql_xoutput {
+  void *data;     /* device column of `dtype`, n rows */
+  uint8_t *nulls; /* device null bytes written from the register's flags, or NULL */
+  int32_t reg;    /* register written; a BOOL register is written as INT 0/1 */
+  int32_t dtype;  /* the register's dtype (INT for a BOOL register) */
+} ysql_xoutput;
+/* Projection: evaluates the program over n rows and writes every output.
+ * The program is checked on the host (register bounds, operand dtypes,
+ * defined-before-use); a bad program returns YSQL_EINVAL.               */
+int ysql_expr_eval(const ysql_xinsn *prog, int ninsn, const ysql_xinput *inputs, int ninputs,
+                   const ysql_xoutput *outputs, int noutputs, uint64_t n, ysql_stream_t stream);
+/* Filter: the rows where BOOL register `pred_reg` is true are compacted
+ * (ysql_compact_cols' contract: stable, up to YSQL_MAX_COLS columns, output
+ * NULL bytes copied, *count_out = selected rows) in the same pass that
+ * evaluates the program; the workspace is ysql_compact_ws_bytes(n).  As in
+ * the per-node filter, the predicate's NULL flags do not affect selection.  */
+int ysql_expr_filter_compact(const ysql_xinsn *prog, int ninsn, int pred_reg,
+                             const ysql_xinput *inputs, int ninputs, const ysql_col *cols,
+                             int ncols, uint64_t n, uint64_t *count_out, void *ws,
+                             size_t ws_bytes, ysql_stream_t stream);
+
+/* ======================================================================
  * Reductions (A16, A17)
  * Replace launch_reduction_operators<T,Tres,Op> / launch_reduction_count_notnull
  * (reduction_kernel.hpp:7-15 -> reduction_kernel.cu:78-205): fold n values

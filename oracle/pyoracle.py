@@ -358,3 +358,72 @@ def cpb_config1(ids, price, threads):
     oi, op, o2 = np.empty(n, np.int32), np.empty(n, np.float32), np.empty(n, np.int32)
     m = lib.cpb_config1(_p(ids), _p(price), n, threads, _p(oi), _p(op), _p(o2))
     return oi[:m], op[:m], o2[:m]
+
+
+# ---------------------------------------------------------------------------
+# Expression trees, evaluated node by node as the reference's evaluate()
+# chain does (include/engine/operators/expressions/): every node is one call
+# of the oracle's per-node function above on its children's results.
+#   ("ref", i)                     bound_ref.hpp:30-37: column i (values, nulls)
+#   ("i32"|"f32"|"i64", v)         bound_value.hpp:21-48: a 1-element scalar
+#   ("cmp", op, l, r)              filters/comparison.hpp:51-215
+#   ("logic", op, l, r)            filters/conjunction.hpp:76-113
+#   ("arith", op, l, r)            bound_func.hpp:45-167
+#   ("cast", dtype, child)         cast.hpp:23-90
+# NULL rules: a comparison or arithmetic node takes the nullset of its
+# non-constant operand, children[0]'s when it is not a bound value
+# (comparison.hpp:213, bound_func.hpp:165); a conjunction its left
+# operand's (conjunction.hpp:111); a cast its child's (cast.hpp:87).  A
+# constant next to an operand of another numeric type is converted to that
+# type first (the reference left this to DuckDB's literal casts).
+# Returns (values, is_scalar, nulls or None); masks are uint8.
+# ---------------------------------------------------------------------------
+_NP_OF = {INT: np.int32, FLOAT: np.float32, DATETIME: np.int64}
+_CONST_DT = {"i32": np.int32, "f32": np.float32, "i64": np.int64}
+
+
+def _tree_type(node, cols):
+    k = node[0]
+    if k == "ref":
+        return np.dtype(cols[node[1]].dtype)
+    if k in _CONST_DT:
+        return np.dtype(_CONST_DT[k])
+    if k in ("cmp", "logic"):
+        return np.dtype(np.int32)  # a mask reports INT, as the engine's returnType does
+    if k == "arith":
+        return _tree_type(node[2], cols)
+    return np.dtype(_NP_OF[node[1]])
+
+
+def _coerce(node, want):
+    if node[0] not in _CONST_DT:
+        return node
+    v = node[1]
+    if want == np.dtype(np.float32):
+        return ("f32", float(np.float32(v)))
+    return ("i64" if want == np.dtype(np.int64) else "i32", int(np.int64(np.float32(v))) if node[0] == "f32" else int(v))
+
+
+def eval_tree(node, cols, nulls=None):
+    nulls = nulls or [None] * len(cols)
+    k = node[0]
+    if k == "ref":
+        return np.ascontiguousarray(cols[node[1]]), False, nulls[node[1]]
+    if k in _CONST_DT:
+        return np.array([node[1]], dtype=_CONST_DT[k]), True, None
+    if k == "cast":
+        v, sc, nl = eval_tree(node[2], cols, nulls)
+        return cast(v, _NP_OF[node[1]]), sc, nl
+    lnode, rnode = node[2], node[3]
+    if k in ("cmp", "arith"):
+        rnode = _coerce(rnode, _tree_type(lnode, cols))
+        lnode = _coerce(lnode, _tree_type(rnode, cols))
+    lv, ls, ln = eval_tree(lnode, cols, nulls)
+    rv, rs, rn = eval_tree(rnode, cols, nulls)
+    n = 1 if (ls and rs) else (rv.shape[0] if ls else lv.shape[0])
+    if k == "logic":
+        return mask_logic(node[1], lv, rv, ls, rs, n=n), ls and rs, ln
+    nl = rn if ls else ln
+    if k == "cmp":
+        return compare(node[1], lv, rv, ls, rs, n=n), ls and rs, nl
+    return arith(node[1], lv, rv, ls, rs, n=n), ls and rs, nl

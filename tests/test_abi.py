@@ -102,3 +102,47 @@ def test_string_entry_points_reject_bad_arguments():
     keys = (L.SortCol * 1)(L.SortCol(L.STRING, 0, 260, 512))  # misaligned column
     assert L.lib.ysql_merge_rows(keys, 1, 10, 10, 20, C.c_void_p(8), C.c_void_p(8), None, None, None) == L.EINVAL
     assert L.lib.ysql_merge_rows(keys, 0, 10, 10, 20, C.c_void_p(8), C.c_void_p(8), None, None, None) == L.EINVAL
+
+
+def test_expression_programs_checked_before_launch():
+    """Fused expression programs are checked on the host: undefined or
+    out-of-range registers, operand dtypes that do not match the
+    instruction, REM on floats, a non-BOOL predicate, too many inputs or
+    instructions.  A valid program over 0 rows launches nothing."""
+    from yallasql_amd import _lib as L
+
+    def prog(*ins):
+        return (L.XInsn * max(1, len(ins)))(*[L.XInsn(*i) for i in ins]), len(ins)
+
+    # (op, type, sub, dst, a, b, nsrc, imm)
+    inp = (L.XInput * 2)(L.XInput(8, None, L.INT, 0), L.XInput(8, None, L.FLOAT, 0))
+    out = (L.XOutput * 1)(L.XOutput(8, None, 2, L.INT))
+    ok = prog((L.XOP_CONST, L.INT, 0, 2, 0, 0, -1, 5), (L.XOP_ARITH, L.INT, L.ADD, 2, 0, 2, 0, 0))
+    assert L.lib.ysql_expr_eval(ok[0], ok[1], inp, 2, out, 1, 0, None) == 0
+    bad = [
+        prog((L.XOP_ARITH, L.INT, L.ADD, 2, 0, 5, -1, 0)),                     # r5 undefined
+        prog((L.XOP_ARITH, L.INT, L.ADD, 2, 0, 1, -1, 0)),                     # r1 is FLOAT
+        prog((L.XOP_ARITH, L.FLOAT, L.REM, 2, 1, 1, -1, 0)),                   # REM on floats
+        prog((L.XOP_CONST, L.INT, 0, 16, 0, 0, -1, 5)),                        # register 16
+        prog((L.XOP_CMP, L.INT, 9, 2, 0, 0, -1, 0)),                           # comparison op 9
+        prog((L.XOP_CAST, L.BOOL, L.INT, 2, 0, 0, -1, 0)),                     # r0 is not BOOL
+        prog((L.XOP_CONST, L.INT, 0, 2, 0, 0, 7, 5)),                          # NULL source r7 undefined
+        prog((9, L.INT, 0, 2, 0, 0, -1, 0)),                                   # opcode 9
+    ]
+    for p, n in bad:
+        rc = L.lib.ysql_expr_eval(p, n, inp, 2, out, 1, 0, None)
+        assert rc in (L.EINVAL, L.EUNSUPPORTED), rc
+    # the output's dtype must be its register's (INT for BOOL registers)
+    cmpp = prog((L.XOP_CONST, L.INT, 0, 2, 0, 0, -1, 5), (L.XOP_CMP, L.INT, L.LT, 3, 0, 2, -1, 0))
+    assert L.lib.ysql_expr_eval(cmpp[0], cmpp[1], inp, 2, (L.XOutput * 1)(L.XOutput(8, None, 3, L.INT)), 1, 0,
+                                None) == 0
+    assert L.lib.ysql_expr_eval(cmpp[0], cmpp[1], inp, 2, (L.XOutput * 1)(L.XOutput(8, None, 3, L.FLOAT)), 1, 0,
+                                None) == L.EINVAL
+    cols = (L.Col * 1)(L.Col(8, None, 16, None, 4, 0))
+    # the predicate register must be BOOL
+    assert L.lib.ysql_expr_filter_compact(ok[0], ok[1], 2, inp, 2, cols, 1, 0, None, None, 0, None) == L.EINVAL
+    assert L.lib.ysql_expr_filter_compact(cmpp[0], cmpp[1], 3, inp, 2, cols, 1, 0, None, None, 0, None) == 0
+    nine = (L.XInput * 9)(*[L.XInput(8, None, L.INT, 0) for _ in range(9)])
+    assert L.lib.ysql_expr_eval(ok[0], ok[1], nine, 9, out, 1, 0, None) == L.EINVAL
+    many = prog(*[(L.XOP_CONST, L.INT, 0, 2, 0, 0, -1, 1)] * (L.XMAX_INSN + 1))
+    assert L.lib.ysql_expr_eval(many[0], many[1], inp, 2, out, 1, 0, None) == L.EINVAL

@@ -401,3 +401,36 @@ def test_cpu_baselines_compute_the_oracle_results():
     for t in (1, 5):
         s2, c2 = P.cpb_star_join_group_sum(dims, m1, m2, 7, t)
         assert np.array_equal(s2, ws) and np.array_equal(c2, wc)
+
+
+def test_eval_tree_node_rules():
+    """The oracle's node-by-node tree evaluation against numpy written
+    directly, and its NULL rule: a comparison or arithmetic node takes the
+    NULLs of its non-constant operand (the left one between two columns), a
+    conjunction the left operand's, a cast the child's; a constant is
+    converted to the other operand's type."""
+    r = np.random.default_rng(0)
+    n = 1000
+    a = r.integers(-20, 20, n).astype(np.int32)
+    b = r.integers(-3, 4, n).astype(np.int32)
+    f = (r.standard_normal(n) * 10).astype(np.float32)
+    na = (r.random(n) < 0.3).astype(np.uint8)
+    nb = (r.random(n) < 0.3).astype(np.uint8)
+    cols, nulls = [a, b, f], [na, nb, None]
+    v, sc, nl = O.eval_tree(("arith", 0, ("i32", 3), ("ref", 0)), cols, nulls)
+    assert not sc and np.array_equal(v, a + 3) and np.array_equal(nl, na)
+    v, _, nl = O.eval_tree(("arith", 2, ("ref", 1), ("ref", 0)), cols, nulls)
+    assert np.array_equal(v, a * b) and np.array_equal(nl, nb)
+    v, _, nl = O.eval_tree(("cmp", 0, ("ref", 0), ("f32", 2.75)), cols, nulls)  # 2.75 -> INT 2
+    assert np.array_equal(v, (a < 2).astype(np.uint8)) and np.array_equal(nl, na)
+    v, _, nl = O.eval_tree(("logic", 1, ("cmp", 2, ("ref", 2), ("f32", 0.0)), ("cmp", 4, ("ref", 1), ("i32", 0))),
+                           cols, nulls)
+    assert np.array_equal(v, ((f > 0) | (b == 0)).astype(np.uint8)) and nl is None
+    v, _, nl = O.eval_tree(("logic", 0, ("cmp", 2, ("ref", 1), ("i32", 0)), ("cmp", 2, ("ref", 2), ("f32", 0.0))),
+                           cols, nulls)
+    assert np.array_equal(v, ((b > 0) & (f > 0)).astype(np.uint8)) and np.array_equal(nl, nb)
+    v, _, nl = O.eval_tree(("cast", 2, ("arith", 3, ("ref", 0), ("ref", 1))), cols, nulls)
+    q = np.where(b == 0, 0, np.where(b == -1, -a, np.trunc(a / np.where(b == 0, 1, b)))).astype(np.int64)
+    assert v.dtype == np.int64 and np.array_equal(v, q) and np.array_equal(nl, na)
+    v, sc, nl = O.eval_tree(("cmp", 0, ("i32", 1), ("i32", 2)), cols, nulls)
+    assert sc and v.tolist() == [1] and nl is None

@@ -57,7 +57,24 @@ class StarDim(C.Structure):
     ]
 
 
+class XInsn(C.Structure):
+    _fields_ = [("op", C.c_int16), ("type", C.c_int16), ("sub", C.c_int16), ("dst", C.c_int8), ("a", C.c_int8),
+                ("b", C.c_int8), ("nsrc", C.c_int8), ("imm", C.c_int64)]
+
+
+class XInput(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("nulls", C.c_void_p), ("dtype", C.c_int32), ("pad", C.c_int32)]
+
+
+class XOutput(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("nulls", C.c_void_p), ("reg", C.c_int32), ("dtype", C.c_int32)]
+
+
+XOP_CONST, XOP_CMP, XOP_LOGIC, XOP_ARITH, XOP_CAST, XOP_NOT = 1, 2, 3, 4, 5, 6
+XMAX_REGS, XMAX_INSN, XMAX_INPUTS = 16, 48, 8
+
 assert C.sizeof(Acc) == 32 and C.sizeof(Col) == 40 and C.sizeof(StarDim) == 40
+assert C.sizeof(XInsn) == 24 and C.sizeof(XInput) == 24 and C.sizeof(XOutput) == 24
 
 P = C.c_void_p
 U64 = C.c_uint64
@@ -104,6 +121,9 @@ _SIGS = {
     "ysql_filter_compact": [I32, I32, P, P, C.POINTER(Col), I32, U64, P, P, C.c_size_t, P],
     "ysql_filter_compact_noalloc": [I32, I32, P, P, C.POINTER(Col), I32, U64, P, P],
     "ysql_filter_compact_range": [I32, P, P, C.POINTER(Col), I32, U64, U64, P, P, C.c_size_t, P],
+    "ysql_expr_eval": [C.POINTER(XInsn), I32, C.POINTER(XInput), I32, C.POINTER(XOutput), I32, U64, P],
+    "ysql_expr_filter_compact": [C.POINTER(XInsn), I32, I32, C.POINTER(XInput), I32, C.POINTER(Col), I32, U64, P, P,
+                                 C.c_size_t, P],
     "ysql_compare": [I32, I32, P, I32, P, I32, I32, P, U64, P],
     "ysql_mask_logic": [I32, P, I32, P, I32, P, U64, P],
     "ysql_arith": [I32, I32, P, I32, P, I32, P, U64, P],

@@ -7,6 +7,7 @@
 #include <limits>
 
 #include "common.hpp"
+#include "ops_dev.hpp"
 
 namespace ysql {
 
@@ -140,30 +141,6 @@ __global__ void k_mask_logic(const uint8_t *a, int as, const uint8_t *b, int bs,
 // A15: arithmetic (binary_operators_kernel.hpp:27-51).  Integer add/sub/mul
 // wrap; DIV/REM by 0 give 0; INT_MIN / -1 wraps and INT_MIN % -1 is 0.
 // ---------------------------------------------------------------------------
-template <typename T>
-struct Unsigned { typedef T type; };
-template <> struct Unsigned<int32_t> { typedef uint32_t type; };
-template <> struct Unsigned<int64_t> { typedef uint64_t type; };
-
-template <typename T, int OP>
-__device__ __forceinline__ T arith_op(T a, T b) {
-  typedef typename Unsigned<T>::type U;
-  if constexpr (OP == YSQL_ADD) return (T)((U)a + (U)b);
-  else if constexpr (OP == YSQL_SUB) return (T)((U)a - (U)b);
-  else if constexpr (OP == YSQL_MUL) return (T)((U)a * (U)b);
-  else if constexpr (OP == YSQL_DIV) {
-    if (b == 0) return 0;
-    if constexpr (std::numeric_limits<T>::is_integer) {
-      if (b == (T)-1) return (T)((U)0 - (U)a);
-    }
-    return a / b;
-  } else {
-    if (b == 0) return 0;
-    if (b == (T)-1) return 0;
-    return a % b;
-  }
-}
-
 template <typename T, int OP>
 __global__ void k_arith(const T *lhs, int ls, const T *rhs, int rs, T *out, uint64_t n) {
   const T l0 = ls ? lhs[0] : T();
@@ -203,16 +180,6 @@ static int launch_arith_t(int op, const void *l, int ls, const void *r, int rs, 
 // A15: casts (cast_operators_kernel.cu:5-17).  float -> integer saturates
 // and maps NaN to 0 (what v_cvt_i32_f32 does; made explicit for int64).
 // ---------------------------------------------------------------------------
-template <typename D>
-__device__ __forceinline__ D sat_from_float(float v) {
-  if (isnan(v)) return 0;
-  const float lo = (float)std::numeric_limits<D>::min();
-  const float hi = (float)std::numeric_limits<D>::max();
-  if (v <= lo) return std::numeric_limits<D>::min();
-  if (v >= hi) return std::numeric_limits<D>::max();
-  return (D)v;
-}
-
 template <typename S, typename D>
 __global__ void k_cast(const S *in, D *out, uint64_t n) {
   GRID_STRIDE(i, n) {

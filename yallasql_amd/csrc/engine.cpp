@@ -418,6 +418,7 @@ ExprValue ConjunctionExpression::evaluate(const Batch &b, ExecContext &ctx) {
   out.mask = true;
   out.scalar = l.scalar && r.scalar;
   out.buf = newBuf(out.rows, ctx.stream());
+  out.nullbuf = copyNulls(l, ctx);  // the left operand's nullset (filters/conjunction.hpp:111)
   abiCheck(ysql_mask_logic(op, (const uint8_t *)l.ptr(), l.scalar, (const uint8_t *)r.ptr(), r.scalar,
                            (uint8_t *)out.buf->ptr, out.rows, ctx.stream()),
            "ysql_mask_logic");
@@ -455,6 +456,201 @@ ExprValue CastExpression::evaluate(const Batch &b, ExecContext &ctx) {
   return out;
 }
 
+// ---------------------------------------------------------------- expression fusion
+// Compiles predicate and projection trees into one register program per
+// batch (ysql_expr_filter_compact / ysql_expr_eval), replacing one kernel
+// and one intermediate column per node.  The node rules are those of the
+// evaluate() methods above, hence of the reference's expression nodes:
+// constants are coerced to the other operand's type; a comparison or
+// arithmetic node takes the NULL flags of its non-constant operand (the left
+// one when neither is constant), a cast its child's, a conjunction its left
+// operand's.  A tree the program cannot express (a string operand, a mask
+// used as a number, operands of different types, REM on floats, more than
+// 16 registers or 48 instructions) is evaluated node by node, so it gives
+// the same result or the same error as before.
+namespace {
+
+struct XVal {
+  int reg = -1;
+  int type = YSQL_INT;
+  bool scalar = false;
+  bool nullable = false;
+};
+
+static void collectCols(Expression *e, std::vector<int> &cols) {
+  if (auto *r = dynamic_cast<BoundRefExpression *>(e)) {
+    if (std::find(cols.begin(), cols.end(), r->idx) == cols.end()) cols.push_back(r->idx);
+  } else if (auto *c = dynamic_cast<ComparisonExpression *>(e)) {
+    collectCols(c->lhs.get(), cols);
+    collectCols(c->rhs.get(), cols);
+  } else if (auto *j = dynamic_cast<ConjunctionExpression *>(e)) {
+    collectCols(j->lhs.get(), cols);
+    collectCols(j->rhs.get(), cols);
+  } else if (auto *f = dynamic_cast<BoundFuncExpression *>(e)) {
+    collectCols(f->lhs.get(), cols);
+    collectCols(f->rhs.get(), cols);
+  } else if (auto *k = dynamic_cast<CastExpression *>(e)) {
+    collectCols(k->child.get(), cols);
+  }
+}
+
+class XProgram {
+ public:
+  // `cols`: the batch columns the trees read; they become registers 0..n-1.
+  XProgram(const Batch &b, const std::vector<int> &cols) : b_(b) {
+    for (int c : cols) {
+      if (c < 0 || (size_t)c >= b.numCols() || inputs.size() >= YSQL_XMAX_INPUTS) {
+        bad_ = true;
+        return;
+      }
+      const DataType t = b.columns[c]->type;
+      if (t == DataType::STRING) {
+        bad_ = true;
+        return;
+      }
+      colreg_.push_back({c, (int)inputs.size()});
+      inputs.push_back(ysql_xinput{b.col(c), b.nul(c), abiType(t), 0});
+      used_ |= 1u << (inputs.size() - 1);
+    }
+  }
+  std::vector<ysql_xinsn> insn;
+  std::vector<ysql_xinput> inputs;
+
+  // Compiles one tree; false (program unchanged) when it cannot be fused.
+  bool add(Expression *e, XVal *out) {
+    if (bad_) return false;
+    const size_t n0 = insn.size();
+    const uint32_t u0 = used_;
+    if (!compile(e, out) || out->scalar) {
+      insn.resize(n0);
+      used_ = u0;
+      return false;
+    }
+    return true;
+  }
+
+ private:
+  const Batch &b_;
+  std::vector<std::pair<int, int>> colreg_;
+  uint32_t used_ = 0;
+  bool bad_ = false;
+
+  int alloc() {
+    for (int r = 0; r < YSQL_XMAX_REGS; r++)
+      if (!(used_ & (1u << r))) {
+        used_ |= 1u << r;
+        return r;
+      }
+    return -1;
+  }
+  void release(const XVal &v) {
+    if (v.reg >= (int)inputs.size()) used_ &= ~(1u << v.reg);
+  }
+  bool emit(int op, int type, int sub, int dst, int a, int b, int nsrc, int64_t imm) {
+    if (insn.size() >= YSQL_XMAX_INSN || dst < 0) return false;
+    ysql_xinsn I;
+    memset(&I, 0, sizeof(I));
+    I.op = (int16_t)op;
+    I.type = (int16_t)type;
+    I.sub = (int16_t)sub;
+    I.dst = (int8_t)dst;
+    I.a = (int8_t)a;
+    I.b = (int8_t)b;
+    I.nsrc = (int8_t)nsrc;
+    I.imm = imm;
+    insn.push_back(I);
+    return true;
+  }
+  // the operand whose NULL flags a binary node takes
+  static const XVal &nullSide(const XVal &l, const XVal &r) { return l.scalar ? r : l; }
+
+  bool compile(Expression *e, XVal *out) {
+    const std::vector<ColumnPtr> &schema = b_.columns;
+    if (auto *ref = dynamic_cast<BoundRefExpression *>(e)) {
+      for (auto &cr : colreg_)
+        if (cr.first == ref->idx) {
+          out->reg = cr.second;
+          out->type = inputs[cr.second].dtype;
+          out->scalar = false;
+          out->nullable = inputs[cr.second].nulls != nullptr;
+          return true;
+        }
+      return false;
+    }
+    if (auto *val = dynamic_cast<BoundValueExpression *>(e)) {
+      if (val->type == DataType::STRING) return false;
+      int64_t imm = val->ival;
+      if (val->type == DataType::FLOAT) {
+        uint32_t bits;
+        memcpy(&bits, &val->fval, 4);
+        imm = bits;
+      } else if (val->type == DataType::INT) {
+        imm = (int32_t)val->ival;
+      }
+      const int r = alloc();
+      if (!emit(YSQL_XOP_CONST, abiType(val->type), 0, r, 0, 0, -1, imm)) return false;
+      *out = XVal{r, abiType(val->type), true, false};
+      return true;
+    }
+    if (auto *c = dynamic_cast<ComparisonExpression *>(e)) {
+      if (dynamic_cast<BoundValueExpression *>(c->rhs.get())) coerce(c->rhs.get(), c->lhs->returnType(schema));
+      if (dynamic_cast<BoundValueExpression *>(c->lhs.get())) coerce(c->lhs.get(), c->rhs->returnType(schema));
+      XVal l, r;
+      if (!compile(c->lhs.get(), &l) || !compile(c->rhs.get(), &r)) return false;
+      if (l.type == YSQL_BOOL || r.type == YSQL_BOOL || l.type != r.type) return false;
+      const XVal &ns = nullSide(l, r);
+      release(l);
+      release(r);
+      const int d = alloc();
+      if (!emit(YSQL_XOP_CMP, l.type, c->cmp, d, l.reg, r.reg, ns.nullable ? ns.reg : -1, 0)) return false;
+      *out = XVal{d, YSQL_BOOL, l.scalar && r.scalar, ns.nullable};
+      return true;
+    }
+    if (auto *j = dynamic_cast<ConjunctionExpression *>(e)) {
+      XVal l, r;
+      if (!compile(j->lhs.get(), &l) || !compile(j->rhs.get(), &r)) return false;
+      if (l.type != YSQL_BOOL || r.type != YSQL_BOOL) return false;
+      release(l);
+      release(r);
+      const int d = alloc();
+      if (!emit(YSQL_XOP_LOGIC, YSQL_BOOL, j->op, d, l.reg, r.reg, l.nullable ? l.reg : -1, 0)) return false;
+      *out = XVal{d, YSQL_BOOL, l.scalar && r.scalar, l.nullable};
+      return true;
+    }
+    if (auto *f = dynamic_cast<BoundFuncExpression *>(e)) {
+      if (dynamic_cast<BoundValueExpression *>(f->rhs.get())) coerce(f->rhs.get(), f->lhs->returnType(schema));
+      if (dynamic_cast<BoundValueExpression *>(f->lhs.get())) coerce(f->lhs.get(), f->rhs->returnType(schema));
+      XVal l, r;
+      if (!compile(f->lhs.get(), &l) || !compile(f->rhs.get(), &r)) return false;
+      if (l.type == YSQL_BOOL || r.type == YSQL_BOOL || l.type != r.type) return false;
+      if (f->op == YSQL_REM && l.type == YSQL_FLOAT) return false;
+      if (f->op < YSQL_ADD || f->op > YSQL_REM) return false;
+      const XVal &ns = nullSide(l, r);
+      release(l);
+      release(r);
+      const int d = alloc();
+      if (!emit(YSQL_XOP_ARITH, l.type, f->op, d, l.reg, r.reg, ns.nullable ? ns.reg : -1, 0)) return false;
+      *out = XVal{d, l.type, l.scalar && r.scalar, ns.nullable};
+      return true;
+    }
+    if (auto *k = dynamic_cast<CastExpression *>(e)) {
+      XVal c;
+      if (!compile(k->child.get(), &c)) return false;
+      const int to = abiType(k->type);
+      if (k->type == DataType::STRING) return false;
+      if (c.type == YSQL_BOOL && to != YSQL_INT) return false;
+      release(c);
+      const int d = alloc();
+      if (!emit(YSQL_XOP_CAST, c.type, to, d, c.reg, 0, c.nullable ? c.reg : -1, 0)) return false;
+      *out = XVal{d, to, c.scalar, c.nullable};
+      return true;
+    }
+    return false;
+  }
+};
+
+}  // namespace
+
 // ---------------------------------------------------------------- scan
 TableScanOperator::TableScanOperator(ExecContext &ctx, std::shared_ptr<Table> t, size_t batch_rows)
     : Operator(ctx), table_(std::move(t)), batch_rows_(batch_rows ? batch_rows : (size_t)1 << 22) {
@@ -491,6 +687,22 @@ static int flipCmp(int c) {
     case YSQL_GE: return YSQL_LE;
   }
   return c;
+}
+
+bool FilterOperator::fusedFilter(const Batch &in, const std::vector<ysql_col> &cols, size_t n, size_t wsb) {
+  std::vector<int> pcols;
+  collectCols(pred_.get(), pcols);
+  XProgram prog(in, pcols);
+  XVal pv;
+  if (!prog.add(pred_.get(), &pv) || pv.type != YSQL_BOOL) return false;
+  for (size_t c0 = 0; c0 < cols.size(); c0 += YSQL_MAX_COLS) {
+    const int nc = (int)std::min<size_t>(YSQL_MAX_COLS, cols.size() - c0);
+    abiCheck(ysql_expr_filter_compact(prog.insn.data(), (int)prog.insn.size(), pv.reg, prog.inputs.data(),
+                                      (int)prog.inputs.size(), cols.data() + c0, nc, n, ctx.deviceCounter(),
+                                      ctx.workspace(wsb), wsb, ctx.stream()),
+             "ysql_expr_filter_compact");
+  }
+  return true;
 }
 
 BatchID FilterOperator::next(CacheManager &cache) {
@@ -542,6 +754,8 @@ BatchID FilterOperator::next(CacheManager &cache) {
                                      cols.data() + c0, nc, n, ctx.deviceCounter(), ctx.workspace(wsb), wsb, s),
                  "ysql_filter_compact");
       }
+    } else if (ctx.fusion() && fusedFilter(*in, cols, n, wsb)) {
+      // the predicate tree ran as one program inside the compaction pass
     } else {
       ExprValue m = pred_->evaluate(*in, ctx);
       if (!m.mask) throw EngineError("filter predicate is not boolean");
@@ -569,6 +783,58 @@ ProjectionOperator::ProjectionOperator(ExecContext &ctx, OpPtr child, std::vecto
   }
   children.push_back(std::move(child));
 }
+// Runs the non-reference expressions as fused programs: consecutive trees
+// share one program while their columns fit the inputs (8) and their
+// registers the register file; a tree that cannot be fused is left to the
+// node-by-node path.
+void ProjectionOperator::fuseExprs(const Batch &in, std::vector<ExprValue> &vals, std::vector<char> &done) {
+  std::vector<size_t> cand;
+  for (size_t i = 0; i < exprs_.size(); i++)
+    if (!dynamic_cast<BoundRefExpression *>(exprs_[i].get())) cand.push_back(i);
+  const size_t n = in.batchSize;
+  size_t k = 0;
+  while (k < cand.size()) {
+    std::vector<size_t> group;
+    std::vector<int> cols;
+    for (size_t j = k; j < cand.size() && group.size() < YSQL_MAX_COLS; j++) {
+      std::vector<int> c2 = cols;
+      collectCols(exprs_[cand[j]].get(), c2);
+      if (c2.size() > YSQL_XMAX_INPUTS) break;
+      XProgram trial(in, c2);
+      XVal v;
+      bool ok = true;
+      for (size_t g : group) ok = ok && trial.add(exprs_[g].get(), &v);
+      ok = ok && trial.add(exprs_[cand[j]].get(), &v);
+      if (!ok) break;
+      group.push_back(cand[j]);
+      cols = c2;
+    }
+    if (group.empty()) {  // this tree alone cannot be fused
+      k++;
+      continue;
+    }
+    XProgram prog(in, cols);
+    std::vector<ysql_xoutput> outs;
+    for (size_t g : group) {
+      XVal v;
+      if (!prog.add(exprs_[g].get(), &v)) throw EngineError("expression fusion: a checked tree failed to compile");
+      const int dt = v.type == YSQL_BOOL ? YSQL_INT : v.type;
+      ExprValue ev;
+      ev.type = (DataType)dt;
+      ev.rows = n;
+      ev.buf = newBuf(n * getDataTypeNumBytes(ev.type), ctx.stream());
+      ev.nullbuf = v.nullable ? newBuf(n, ctx.stream()) : nullptr;
+      outs.push_back(ysql_xoutput{ev.buf->ptr, ev.nullbuf ? (uint8_t *)ev.nullbuf->ptr : nullptr, v.reg, dt});
+      vals[g] = ev;
+      done[g] = 1;
+    }
+    abiCheck(ysql_expr_eval(prog.insn.data(), (int)prog.insn.size(), prog.inputs.data(), (int)prog.inputs.size(),
+                            outs.data(), (int)outs.size(), n, ctx.stream()),
+             "ysql_expr_eval");
+    k += group.size();
+  }
+}
+
 BatchID ProjectionOperator::next(CacheManager &cache) {
   if (isFinished) return 0;
   const BatchID cid = children[0]->next(cache);
@@ -580,8 +846,11 @@ BatchID ProjectionOperator::next(CacheManager &cache) {
   auto out = std::make_unique<Batch>();
   out->stream = ctx.stream();
   out->batchSize = in->batchSize;
+  std::vector<ExprValue> fused(exprs_.size());
+  std::vector<char> is_fused(exprs_.size(), 0);
+  if (ctx.fusion() && in->batchSize > 0) fuseExprs(*in, fused, is_fused);
   for (size_t i = 0; i < exprs_.size(); i++) {
-    ExprValue v = exprs_[i]->evaluate(*in, ctx);
+    ExprValue v = is_fused[i] ? fused[i] : exprs_[i]->evaluate(*in, ctx);
     if (v.scalar && in->batchSize != 1) throw EngineError("constant projection is not supported");
     if (v.mask) {  // boolean -> INT column
       auto buf = newBuf(v.rows * 4, ctx.stream());

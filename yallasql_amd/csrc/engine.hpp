@@ -172,8 +172,13 @@ class ExecContext {
   uint64_t *deviceCounter();        // one device uint64
   uint64_t readCounter();           // synchronises the stream
   void sync();
+  // Expression fusion: predicate and projection trees run as one fused
+  // program per batch (on by default); off runs them node by node.
+  bool fusion() const { return fusion_; }
+  void setFusion(bool on) { fusion_ = on; }
 
  private:
+  bool fusion_ = true;
   hipStream_t stream_ = nullptr;
   bool own_stream_ = false;
   CacheManager cache_;
@@ -312,6 +317,8 @@ class FilterOperator : public Operator {
 
  private:
   ExprPtr pred_;
+  // the predicate as one fused program inside the compaction; false: not fusible
+  bool fusedFilter(const Batch &in, const std::vector<ysql_col> &cols, size_t n, size_t wsb);
 };
 
 class ProjectionOperator : public Operator {
@@ -321,6 +328,7 @@ class ProjectionOperator : public Operator {
 
  private:
   std::vector<ExprPtr> exprs_;
+  void fuseExprs(const Batch &in, std::vector<ExprValue> &vals, std::vector<char> &done);
 };
 
 struct OrderKey {

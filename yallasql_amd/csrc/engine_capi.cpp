@@ -49,6 +49,11 @@ ysql_ctx *ysql_ctx_create(uint64_t budget, void *stream) {
 }
 void ysql_ctx_destroy(ysql_ctx *c) { delete c; }
 uint64_t ysql_ctx_peak_bytes(const ysql_ctx *c) { return c ? const_cast<ysql_ctx *>(c)->ctx.cache().peakBytes() : 0; }
+int ysql_ctx_set_fusion(ysql_ctx *c, int on) {
+  if (!c) return YSQL_EINVAL;
+  c->ctx.setFusion(on != 0);
+  return YSQL_OK;
+}
 int ysql_ctx_set_spill(ysql_ctx *c, uint64_t host_budget, const char *dir) {
   if (!c) return YSQL_EINVAL;
   c->ctx.cache().configure(host_budget, dir ? std::string(dir) : std::string());

@@ -11,6 +11,7 @@
 #include <atomic>
 #include <mutex>
 
+#include "expr_dev.hpp"
 #include "scan_dev.hpp"
 
 namespace ysql {
@@ -193,6 +194,52 @@ __global__ __launch_bounds__(kScanBlock) void k_compact(const uint8_t *mask, con
   const PT hi = MODE == 2 ? scalar[1] : PT();
   compact_tile<kScanBlock, MODE, PT, CMP>(sm, tile, mask, pred, s, hi, cols.c, ncols, n, cap, 0, count_out, status,
                                           ntiles);
+}
+
+// ---------------------------------------------------------------------------
+// Filter by a fused expression program (expr_dev.hpp): the program runs over
+// the tile in striped steps (thread t: rows 256 R j + R t .. + R-1, so each
+// load instruction of a wave reads one contiguous range), its BOOL result
+// goes to LDS as one byte per row, and each thread then takes the flags of
+// its 16 consecutive rows for the stable compaction of compact_scatter.
+// ---------------------------------------------------------------------------
+template <typename S>
+__global__ __launch_bounds__(kScanBlock) void k_xfilter(XProg p, ColArgs cols, int ncols, uint64_t n,
+                                                        uint64_t *count_out, uint32_t *ctr, uint64_t *status,
+                                                        uint32_t ntiles) {
+  constexpr int R = sizeof(S) == 8 ? 1 : 4;  // rows per thread per step (64-bit slots: the file is twice as wide)
+  __shared__ CompactSmem<kScanBlock, 0> sm;
+  __shared__ uint32_t s_sel[kScanTile / 4];
+  __shared__ uint32_t s_tile;
+  const uint32_t tile = take_tile(ctr, &s_tile);
+  const uint64_t base = (uint64_t)tile * kScanTile;
+  const int t = threadIdx.x;
+  XRegs<S, R> reg;
+  reg.clear();
+  uint8_t *sel = reinterpret_cast<uint8_t *>(s_sel);
+#pragma unroll
+  for (int j = 0; j < kScanItems / R; j++) {
+    const uint32_t r0 = j * (kScanBlock * R) + t * R;
+    xload_inputs<S, R, false>(p, base + r0, n, reg);
+    xrun<S, R, false>(p, reg);
+    S pr[R];
+    xget(reg, p.pred, pr);
+    if constexpr (R == 4) {
+      s_sel[r0 / 4] = (uint32_t)(pr[0] != 0) | (uint32_t)(pr[1] != 0) << 8 | (uint32_t)(pr[2] != 0) << 16 |
+                      (uint32_t)(pr[3] != 0) << 24;
+    } else {
+#pragma unroll
+      for (int e = 0; e < R; e++) sel[r0 + e] = (uint8_t)(pr[e] != 0);
+    }
+  }
+  __syncthreads();
+  const uint4 q = reinterpret_cast<const uint4 *>(s_sel)[t];
+  const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+  uint32_t flags = 0;
+#pragma unroll
+  for (int e = 0; e < 16; e++) flags |= (uint32_t)(((w[e >> 2] >> (8 * (e & 3))) & 0xFF) != 0) << e;
+  compact_scatter<kScanBlock, 0, uint32_t>(sm, tile, flags, nullptr, cols.c, ncols, n, ~0ull, 0, count_out, status,
+                                           ntiles);
 }
 
 static bool cols_ok(const ysql_col *cols, int ncols, bool need_ptrs) {
@@ -389,6 +436,43 @@ int ysql_filter_compact_range(int dtype, const void *pred_col, const void *bound
                               ysql_stream_t stream) {
   return compact_impl(1, dtype, kCmpRange, nullptr, pred_col, bounds, cols, ncols, n, cap, count_out, ws, ws_bytes,
                       as_stream(stream));
+}
+
+int ysql_expr_filter_compact(const ysql_xinsn *prog, int ninsn, int pred_reg, const ysql_xinput *inputs, int ninputs,
+                             const ysql_col *cols, int ncols, uint64_t n, uint64_t *count_out, void *ws,
+                             size_t ws_bytes, ysql_stream_t stream) {
+  XProg p;
+  bool wide = false;
+  int8_t ty[kXRegs];
+  int rc = xprog_prepare(prog, ninsn, inputs, ninputs, n, &p, &wide, ty);
+  if (rc) return rc;
+  if (pred_reg < 0 || pred_reg >= kXRegs || ty[pred_reg] != YSQL_BOOL) return YSQL_EINVAL;
+  p.pred = pred_reg;
+  p.nout = 0;
+  if (!cols_ok(cols, ncols, n > 0)) return YSQL_EINVAL;
+  if (n >= (1ull << 40)) return YSQL_EINVAL;
+  hipStream_t s = as_stream(stream);
+  if (n == 0) {
+    if (count_out) {
+      k_store_zero<<<1, 1, 0, s>>>(count_out);
+      YSQL_LAUNCH_CHECK();
+    }
+    return YSQL_OK;
+  }
+  ScanWs w;
+  if (!ws || ws_bytes < scan_ws_size(n) || !scan_ws_carve(ws, ws_bytes, n, &w)) return YSQL_EWORKSPACE;
+  const uint32_t ntiles = (uint32_t)ceil_div(n, kScanTile);
+  rc = scan_reset(w, ntiles, s);
+  if (rc) return rc;
+  ColArgs ca;
+  memset(&ca, 0, sizeof(ca));
+  for (int i = 0; i < ncols; i++) ca.c[i] = cols[i];
+  if (wide)
+    k_xfilter<uint64_t><<<ntiles, kScanBlock, 0, s>>>(p, ca, ncols, n, count_out, w.ctr, w.status, ntiles);
+  else
+    k_xfilter<uint32_t><<<ntiles, kScanBlock, 0, s>>>(p, ca, ncols, n, count_out, w.ctr, w.status, ntiles);
+  YSQL_LAUNCH_CHECK();
+  return YSQL_OK;
 }
 
 }  // extern "C"

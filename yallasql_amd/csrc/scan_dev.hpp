@@ -45,49 +45,22 @@ __device__ __forceinline__ void load16(const PT *p, uint64_t base, uint64_t rows
   }
 }
 
-// MODE 0: selection from mask bytes; MODE 1: pred[i] <CMP> scalar;
-// MODE 2: scalar <= pred[i] < hi.  Output rows at index >= cap are not
-// written; *count_out receives the full count.  status == nullptr: tiles are
-// processed in order by one caller that passes the running prefix (no
-// look-back).  Returns the tile's selected-row count.
+// Second half of a compaction tile: given each thread's selection flags for
+// its 16 consecutive rows (bit e = row t*16 + e of the tile), scans the
+// counts, finds the tile's prefix (look-back over `status`, or `running`
+// when status is nullptr), lists the selected rows in LDS and copies every
+// column.  VB != 0: the predicate values `pred` of the thread's rows were
+// stored in sm.pv, and an output column reading `pred` is copied from there.
 // Every thread of the block calls it; contains barriers.
-template <int BLOCK, int MODE, typename PT, int CMP, int VB>
-__device__ __forceinline__ uint64_t compact_tile(CompactSmem<BLOCK, VB> &sm, uint32_t tile, const uint8_t *mask,
-                                             const PT *pred, PT scalar, PT hi, const ysql_col *cols, int ncols,
-                                             uint64_t n, uint64_t cap, uint64_t running,
-                                             uint64_t *count_out, uint64_t *status, uint32_t ntiles) {
+template <int BLOCK, int VB, typename PT>
+__device__ __forceinline__ uint64_t compact_scatter(CompactSmem<BLOCK, VB> &sm, uint32_t tile, uint32_t flags,
+                                                    const PT *pred, const ysql_col *cols, int ncols, uint64_t n,
+                                                    uint64_t cap, uint64_t running, uint64_t *count_out,
+                                                    uint64_t *status, uint32_t ntiles) {
   constexpr int TILE = BLOCK * 16;
   const uint64_t base = (uint64_t)tile * TILE;
   const int t = threadIdx.x;
   const uint64_t rows = (n - base) < (uint64_t)TILE ? (n - base) : (uint64_t)TILE;
-
-  uint32_t flags = 0;
-  if constexpr (MODE == 0) {
-    uint8_t m[16];
-    load16<uint8_t>(mask, base, rows, t, m);
-#pragma unroll
-    for (int e = 0; e < 16; e++) flags |= (uint32_t)(m[e] != 0) << e;
-  } else {
-    PT v[16];
-    load16<PT>(pred, base, rows, t, v);
-#pragma unroll
-    for (int e = 0; e < 16; e++) {
-      const bool sel = MODE == 1 ? cmp_static<CMP>(v[e], scalar) : (!(v[e] < scalar) && v[e] < hi);
-      flags |= (uint32_t)sel << e;
-    }
-    if constexpr (VB == (int)sizeof(PT)) {
-      // 16-byte LDS stores of the thread's 16 values, when it selected any
-      // (only selected rows are read back)
-      if (flags) {
-        constexpr int NQ = 16 * VB / 16;
-        uint4 q[NQ];
-        memcpy(&q[0], &v[0], sizeof(q));
-        uint4 *d = reinterpret_cast<uint4 *>(&sm.pv[t * 16]);
-#pragma unroll
-        for (int i = 0; i < NQ; i++) d[i] = q[i];
-      }
-    }
-  }
   {
     const uint64_t r0 = (uint64_t)t * 16;
     if (r0 + 16 > rows) flags &= (r0 >= rows) ? 0u : ((1u << (rows - r0)) - 1u);
@@ -116,7 +89,7 @@ __device__ __forceinline__ uint64_t compact_tile(CompactSmem<BLOCK, VB> &sm, uin
   const uint32_t tot = prefix >= cap ? 0u : (uint32_t)(cap - prefix < total ? cap - prefix : total);
   for (int c = 0; c < ncols; c++) {
     const ysql_col col = cols[c];
-    if constexpr (MODE != 0 && VB == (int)sizeof(PT)) {
+    if constexpr (VB != 0 && VB == (int)sizeof(PT)) {
       if (col.src == (const void *)pred && col.elem_bytes == VB) {
         using V = typename CompactSmem<BLOCK, VB>::V;
         V *dst = (V *)col.dst;
@@ -171,6 +144,54 @@ __device__ __forceinline__ uint64_t compact_tile(CompactSmem<BLOCK, VB> &sm, uin
   if (tile == ntiles - 1 && t == 0 && count_out) *count_out = prefix + total;
   __syncthreads();  // LDS is reused by the caller's next tile
   return total;
+}
+
+// MODE 0: selection from mask bytes; MODE 1: pred[i] <CMP> scalar;
+// MODE 2: scalar <= pred[i] < hi.  Output rows at index >= cap are not
+// written; *count_out receives the full count.  status == nullptr: tiles are
+// processed in order by one caller that passes the running prefix (no
+// look-back).  Returns the tile's selected-row count.
+// Every thread of the block calls it; contains barriers.
+template <int BLOCK, int MODE, typename PT, int CMP, int VB>
+__device__ __forceinline__ uint64_t compact_tile(CompactSmem<BLOCK, VB> &sm, uint32_t tile, const uint8_t *mask,
+                                             const PT *pred, PT scalar, PT hi, const ysql_col *cols, int ncols,
+                                             uint64_t n, uint64_t cap, uint64_t running,
+                                             uint64_t *count_out, uint64_t *status, uint32_t ntiles) {
+  constexpr int TILE = BLOCK * 16;
+  const uint64_t base = (uint64_t)tile * TILE;
+  const int t = threadIdx.x;
+  const uint64_t rows = (n - base) < (uint64_t)TILE ? (n - base) : (uint64_t)TILE;
+
+  uint32_t flags = 0;
+  if constexpr (MODE == 0) {
+    uint8_t m[16];
+    load16<uint8_t>(mask, base, rows, t, m);
+#pragma unroll
+    for (int e = 0; e < 16; e++) flags |= (uint32_t)(m[e] != 0) << e;
+  } else {
+    PT v[16];
+    load16<PT>(pred, base, rows, t, v);
+#pragma unroll
+    for (int e = 0; e < 16; e++) {
+      const bool sel = MODE == 1 ? cmp_static<CMP>(v[e], scalar) : (!(v[e] < scalar) && v[e] < hi);
+      flags |= (uint32_t)sel << e;
+    }
+    if constexpr (VB == (int)sizeof(PT)) {
+      // 16-byte LDS stores of the thread's 16 values, when it selected any
+      // (only selected rows are read back)
+      if (flags) {
+        constexpr int NQ = 16 * VB / 16;
+        uint4 q[NQ];
+        memcpy(&q[0], &v[0], sizeof(q));
+        uint4 *d = reinterpret_cast<uint4 *>(&sm.pv[t * 16]);
+#pragma unroll
+        for (int i = 0; i < NQ; i++) d[i] = q[i];
+      }
+    }
+  }
+  // with mask selection no column is copied from the LDS values
+  return compact_scatter<BLOCK, VB, PT>(sm, tile, flags, MODE != 0 ? pred : (const PT *)nullptr, cols, ncols, n, cap,
+                                        running, count_out, status, ntiles);
 }
 
 }  // namespace ysql

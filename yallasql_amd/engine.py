@@ -9,6 +9,7 @@
 Expression helpers build ysql_expr trees; plan methods consume their inputs.
 """
 import ctypes as C
+import weakref
 
 import numpy as np
 
@@ -22,6 +23,7 @@ _sig = {
     "ysql_ctx_destroy": ([P], None),
     "ysql_ctx_peak_bytes": ([P], C.c_uint64),
     "ysql_ctx_set_spill": ([P, C.c_uint64, C.c_char_p], C.c_int),
+    "ysql_ctx_set_fusion": ([P, C.c_int], C.c_int),
     "ysql_ctx_cache_stats": ([P, P], C.c_int),
     "ysql_table_create": ([P, C.c_char_p], P),
     "ysql_table_add_column": ([P, C.c_char_p, C.c_int, P, P, C.c_uint64], C.c_int),
@@ -123,6 +125,7 @@ class Plan:
     def __init__(self, ctx, h):
         self.ctx = ctx
         self.h = _chk_ptr(h, "plan")
+        ctx._plans.add(self)  # destroyed before the context they run in
 
     def _take(self):
         h, self.h = self.h, None
@@ -222,14 +225,19 @@ class CacheStats(C.Structure):
 
 
 class Context:
-    def __init__(self, hbm_budget=0, stream=None, host_budget=0, spill_dir=None):
+    def __init__(self, hbm_budget=0, stream=None, host_budget=0, spill_dir=None, fusion=True):
         """hbm_budget: bytes of cached batches kept in HBM (0 = unlimited);
         past it the least recently used go to pinned host memory, and past
-        host_budget to files in spill_dir (None: no file tier)."""
+        host_budget to files in spill_dir (None: no file tier).  fusion:
+        predicate and projection trees run as one fused program per batch
+        (False: node by node, one kernel per expression node)."""
         self.h = _chk_ptr(lib.ysql_ctx_create(hbm_budget, stream), "context")
         self._tables = []
+        self._plans = weakref.WeakSet()
         if host_budget or spill_dir:
             lib.ysql_ctx_set_spill(self.h, host_budget, None if spill_dir is None else str(spill_dir).encode())
+        if not fusion:
+            lib.ysql_ctx_set_fusion(self.h, 0)
 
     def cache_stats(self):
         st = CacheStats()
@@ -276,6 +284,8 @@ class Context:
         return lib.ysql_ctx_peak_bytes(self.h)
 
     def close(self):
+        for p in list(self._plans):  # a plan's operators hold the context's stream and cache
+            p.__del__()
         for t in self._tables:
             t.__del__()
         self._tables = []

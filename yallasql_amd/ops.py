@@ -297,6 +297,52 @@ def filter_compact_range(pred, lo, hi, srcs, cap=None, retry=True):
     return [d[:min(c, cap)] for d in outs], c
 
 
+# ---- fused expression programs -------------------------------------------
+def xinsn(op, dst, a=0, b=0, type=0, sub=0, nsrc=-1, imm=0):
+    """One program instruction (ysql_xinsn); FLOAT constants pass their bits in imm."""
+    return L.XInsn(op, type, sub, dst, a, b, nsrc, imm)
+
+
+def _xprog(insns, inputs, nulls):
+    prog = (L.XInsn * max(1, len(insns)))(*insns)
+    xin = (L.XInput * max(1, len(inputs)))()
+    for i, t in enumerate(inputs):
+        nl = nulls[i] if nulls else None
+        xin[i] = L.XInput(t.data_ptr(), nl.data_ptr() if nl is not None else None, dtype_code(t), 0)
+    return prog, xin
+
+
+def expr_eval(insns, inputs, outputs, nulls=None, n=None):
+    """Runs a program over n rows.  inputs: tensors (registers 0..); outputs:
+    list of (register, torch dtype, with_nulls).  Returns [(values, nulls or None)]."""
+    n = inputs[0].shape[0] if n is None else n
+    dev = inputs[0].device
+    prog, xin = _xprog(insns, inputs, nulls)
+    res, xout = [], (L.XOutput * max(1, len(outputs)))()
+    for i, (reg, dtype, with_nulls) in enumerate(outputs):
+        d = torch.empty(n, dtype=dtype, device=dev)
+        dn = torch.empty(n, dtype=torch.uint8, device=dev) if with_nulls else None
+        xout[i] = L.XOutput(d.data_ptr(), dn.data_ptr() if dn is not None else None, reg, _DT[dtype])
+        res.append((d, dn))
+    call("ysql_expr_eval", prog, len(insns), xin, len(inputs), xout, len(outputs), n, _s())
+    return res
+
+
+def expr_filter_compact(insns, pred_reg, inputs, srcs, nulls=None, src_nulls=None):
+    """Rows where BOOL register pred_reg holds, every column of srcs compacted.
+    Returns ([(dst, dst_null)], count)."""
+    n = inputs[0].shape[0]
+    dev = inputs[0].device
+    prog, xin = _xprog(insns, inputs, nulls)
+    cols, outs = _cols(srcs, src_nulls, n)
+    cnt = torch.zeros(1, dtype=torch.int64, device=dev)
+    ws = workspace(ws_bytes("ysql_compact_ws_bytes", n), dev)
+    call("ysql_expr_filter_compact", prog, len(insns), pred_reg, xin, len(inputs), cols, len(srcs), n, _p(cnt),
+         _p(ws), ws.numel(), _s())
+    c = int(cnt.item())
+    return [(d[:c], dn[:c]) for d, dn in outs], c
+
+
 # ---- A12 / A15 ---------------------------------------------------------
 def compare(cmp, lhs, rhs, lhs_scalar=False, rhs_scalar=False, negate=False, n=None):
     n = n if n is not None else (rhs.shape[0] if lhs_scalar else lhs.shape[0])
