@@ -47,6 +47,7 @@ def parse():
     ap.add_argument("--no-batched", action="store_true", help="skip the 4,096-job batched vs plain comparison")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving loop")
     ap.add_argument("--no-configs", action="store_true", help="skip the config-1 and config-3 lines")
+    ap.add_argument("--no-stream", action="store_true", help="skip config 2 streamed from pinned host memory")
     ap.add_argument("--serving-queries", type=int, default=10_000, help="queries per rate, at most")
     ap.add_argument("--serving-seconds", type=float, default=60.0, help="arrival window per rate, at most")
     ap.add_argument("--pairwise-merge", action="store_true",
@@ -235,6 +236,7 @@ def op_lines(keys, cpu_inputs=None):
     out["filter_compact"] = {"rows_per_s": n / (msf * 1e-3), "ms": msf, "selected": m,
                              "GB_s": bf / (msf * 1e-3) / 1e9, "frac": bf / (msf * 1e-3) / 1e9 / HBM_PEAK_GBS,
                              "what": "WHERE f32 < 0.5 compacting (int64 key, f32) of 200M rows"}
+    out.update(fusion_lines(keys, vals, g))
     del vals, kout, vout, ws
     # inner EQ join of the 200M int64 keys with a dimension of R int64 keys in
     # [1000, 1e7): hash build of the right side + probe, (l, r) pairs written
@@ -258,6 +260,104 @@ def op_lines(keys, cpu_inputs=None):
                                 "what": f"inner EQ join, 200M int64 left keys x {R:,} right keys: build + "
                                         "filtered probe + pairs"}
         del right, jws, pairs
+    return out
+
+
+def fusion_lines(keys, vals, g):
+    """Expression fusion over the 200M-row table: a two-comparison predicate
+    compacting (key, f32), and a three-column projection, each as one fused
+    program against the same tree run node by node (one kernel and one
+    intermediate column per node, as the reference evaluates it)."""
+    import struct
+
+    import torch
+
+    from yallasql_amd import _lib as L
+    from yallasql_amd import ops
+
+    n = keys.numel()
+    dev = keys.device
+    cat = torch.randint(0, 64, (n,), generator=g, device=dev, dtype=torch.int32)
+    X = ops.xinsn
+    fbits = lambda x: struct.unpack("<I", struct.pack("<f", x))[0]  # noqa: E731
+    out = {}
+    # WHERE f32 > 0.25 AND cat < 32, compacting (key, f32)
+    inp = (L.XInput * 2)(L.XInput(vals.data_ptr(), None, L.FLOAT, 0), L.XInput(cat.data_ptr(), None, L.INT, 0))
+    prog = [X(L.XOP_CONST, 2, type=L.FLOAT, imm=fbits(0.25)), X(L.XOP_CMP, 2, a=0, b=2, type=L.FLOAT, sub=L.GT),
+            X(L.XOP_CONST, 3, type=L.INT, imm=32), X(L.XOP_CMP, 3, a=1, b=3, type=L.INT, sub=L.LT),
+            X(L.XOP_LOGIC, 2, a=2, b=3, sub=L.AND)]
+    xp = (L.XInsn * len(prog))(*prog)
+    kout, vout = torch.empty_like(keys), torch.empty_like(vals)
+    col = (L.Col * 2)()
+    col[0] = L.Col(keys.data_ptr(), None, kout.data_ptr(), None, 8, 0)
+    col[1] = L.Col(vals.data_ptr(), None, vout.data_ptr(), None, 4, 0)
+    cnt = torch.zeros(1, dtype=torch.int64, device=dev)
+    ws = ops.workspace(L.ws_bytes("ysql_compact_ws_bytes", n), dev)
+    fused = lambda: L.call("ysql_expr_filter_compact", xp, len(prog), 2, inp, 2, col, 2, n, ops._p(cnt),  # noqa: E731
+                           ops._p(ws), ws.numel(), ops._s())
+    c025 = torch.tensor([0.25], device=dev)
+    c32 = torch.tensor([32], dtype=torch.int32, device=dev)
+    m1, m2, m3 = (torch.empty(n, dtype=torch.uint8, device=dev) for _ in range(3))
+
+    def nodes():
+        L.call("ysql_compare", L.FLOAT, L.GT, ops._p(vals), 0, ops._p(c025), 1, 0, ops._p(m1), n, ops._s())
+        L.call("ysql_compare", L.INT, L.LT, ops._p(cat), 0, ops._p(c32), 1, 0, ops._p(m2), n, ops._s())
+        L.call("ysql_mask_logic", L.AND, ops._p(m1), 0, ops._p(m2), 0, ops._p(m3), n, ops._s())
+        L.call("ysql_compact_cols", ops._p(m3), col, 2, n, ops._p(cnt), ops._p(ws), ws.numel(), ops._s())
+
+    fused()
+    mf = int(cnt.item())
+    ms_f = _timed_ms(fused)
+    nodes()
+    assert int(cnt.item()) == mf
+    ms_n = _timed_ms(nodes)
+    b = n * (4 + 4 + 8) + mf * 12
+    out["fused_filter_tree"] = {
+        "rows_per_s": n / (ms_f * 1e-3), "ms": ms_f, "node_by_node_ms": ms_n, "speedup": ms_n / ms_f,
+        "selected": mf, "GB_s": b / (ms_f * 1e-3) / 1e9, "frac": b / (ms_f * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "what": "WHERE f32 > 0.25 AND i32 < 32 compacting (int64 key, f32) of 200M rows: one fused program vs "
+                "2 compares + AND + mask compaction"}
+    del m1, m2, m3, kout, vout
+    # SELECT 2*cat + 1, f32*1.5 - 0.5, CAST(f32*100 AS INT) + cat
+    prog = [X(L.XOP_CONST, 2, type=L.INT, imm=2), X(L.XOP_ARITH, 2, a=2, b=1, type=L.INT, sub=L.MUL),
+            X(L.XOP_CONST, 3, type=L.INT, imm=1), X(L.XOP_ARITH, 2, a=2, b=3, type=L.INT, sub=L.ADD),
+            X(L.XOP_CONST, 3, type=L.FLOAT, imm=fbits(1.5)), X(L.XOP_ARITH, 3, a=0, b=3, type=L.FLOAT, sub=L.MUL),
+            X(L.XOP_CONST, 4, type=L.FLOAT, imm=fbits(0.5)), X(L.XOP_ARITH, 3, a=3, b=4, type=L.FLOAT, sub=L.SUB),
+            X(L.XOP_CONST, 4, type=L.FLOAT, imm=fbits(100.0)), X(L.XOP_ARITH, 4, a=0, b=4, type=L.FLOAT, sub=L.MUL),
+            X(L.XOP_CAST, 4, a=4, type=L.FLOAT, sub=L.INT), X(L.XOP_ARITH, 4, a=4, b=1, type=L.INT, sub=L.ADD)]
+    xp = (L.XInsn * len(prog))(*prog)
+    o1, o3 = torch.empty_like(cat), torch.empty_like(cat)
+    o2 = torch.empty_like(vals)
+    xo = (L.XOutput * 3)(L.XOutput(o1.data_ptr(), None, 2, L.INT), L.XOutput(o2.data_ptr(), None, 3, L.FLOAT),
+                         L.XOutput(o3.data_ptr(), None, 4, L.INT))
+    fusedp = lambda: L.call("ysql_expr_eval", xp, len(prog), inp, 2, xo, 3, n, ops._s())  # noqa: E731
+    t1, t2 = torch.empty_like(cat), torch.empty_like(vals)
+    k2, k1, k15, k05, k100 = (torch.tensor([v], dtype=dt, device=dev) for v, dt in
+                              ((2, torch.int32), (1, torch.int32), (1.5, torch.float32), (0.5, torch.float32),
+                               (100.0, torch.float32)))
+
+    def nodesp():
+        s = ops._s()
+        L.call("ysql_arith", L.INT, L.MUL, ops._p(k2), 1, ops._p(cat), 0, ops._p(t1), n, s)
+        L.call("ysql_arith", L.INT, L.ADD, ops._p(t1), 0, ops._p(k1), 1, ops._p(o1), n, s)
+        L.call("ysql_arith", L.FLOAT, L.MUL, ops._p(vals), 0, ops._p(k15), 1, ops._p(t2), n, s)
+        L.call("ysql_arith", L.FLOAT, L.SUB, ops._p(t2), 0, ops._p(k05), 1, ops._p(o2), n, s)
+        L.call("ysql_arith", L.FLOAT, L.MUL, ops._p(vals), 0, ops._p(k100), 1, ops._p(t2), n, s)
+        L.call("ysql_cast", L.FLOAT, L.INT, ops._p(t2), ops._p(t1), n, s)
+        L.call("ysql_arith", L.INT, L.ADD, ops._p(t1), 0, ops._p(cat), 0, ops._p(o3), n, s)
+
+    fusedp()
+    ref = [o1.clone(), o2.clone(), o3.clone()]
+    ms_pf = _timed_ms(fusedp)
+    nodesp()
+    assert all(torch.equal(a, b) for a, b in zip(ref, (o1, o2, o3)))
+    ms_pn = _timed_ms(nodesp)
+    bp = n * (4 + 4) + n * 12
+    out["fused_projection_tree"] = {
+        "rows_per_s": n / (ms_pf * 1e-3), "ms": ms_pf, "node_by_node_ms": ms_pn, "speedup": ms_pn / ms_pf,
+        "GB_s": bp / (ms_pf * 1e-3) / 1e9, "frac": bp / (ms_pf * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "what": "SELECT 2*i32+1, f32*1.5-0.5, CAST(f32*100 AS INT)+i32 over 200M rows: one fused program vs 7 "
+                "node kernels"}
     return out
 
 
@@ -348,6 +448,122 @@ def csv_load_line(n=2_000_000):
                         "double-buffered staging, H2D copies on a side stream"}
     finally:
         os.unlink(path)
+
+
+def csv_scale_line(target_bytes=2 << 30):
+    """CSV at scale: a file of about 2 GB (a 1M-row block of the config-1
+    shape written repeatedly) read into device batches two ways, every batch
+    consumed by a fused filter+aggregate kernel on the caller's stream:
+    the reference's sizing (4 MiB batches, copies on the caller's stream) and
+    scale mode (16M-row batches, 3 pinned staging areas, copies on the
+    reader's own stream overlapping the kernels).  Also the pinned
+    host-to-device copy rate of 1 GiB: the PCIe roofline of this path."""
+    import os
+    import tempfile
+
+    import numpy as np
+    import pandas as pd
+    import torch
+
+    from yallasql_amd import _lib as L
+    from yallasql_amd import loader, ops
+
+    dev = torch.device("cuda:0")
+    blk = 1 << 20
+    r = np.random.default_rng(SEED + 5)
+    body = pd.DataFrame({"id": np.arange(blk, dtype=np.int32),
+                         "price": np.round(1 + 999 * r.random(blk), 2).astype(np.float32),
+                         "qty": r.integers(0, 100, blk, dtype=np.int32)}).to_csv(index=False, header=False)
+    body = body.encode()
+    reps = max(1, target_bytes // len(body))
+    fd, path = tempfile.mkstemp(suffix=".csv")
+    os.close(fd)
+    try:
+        with open(path, "wb") as f:
+            f.write(b"id,price,qty\n")
+            for _ in range(reps):
+                f.write(body)
+        size = os.path.getsize(path)
+        n = reps * blk
+        names, types = ["id", "price", "qty"], [L.INT, L.FLOAT, L.INT]
+
+        def run(scale):
+            rd = (loader.CsvReader(path, names, types, batch_rows_=1 << 24, slots=3, async_copies=True) if scale
+                  else loader.CsvReader(path, names, types))
+            s = torch.cuda.Stream(device=dev)
+            accs = [[torch.empty(32, dtype=torch.uint8, device=dev) for _ in range(2)]]
+            rows, first = 0, True
+            t0 = time.perf_counter()
+            with torch.cuda.stream(s):
+                while True:
+                    b = rd.next_async(stream=s) if scale else rd.next_device(dev, stream=s)
+                    if b is None:
+                        break
+                    cols = [(L.INT, L.AGG_COUNT_STAR, None), (L.INT, L.AGG_SUM, b[0][2])]
+                    ops.filter_aggregate_multi(b[0][1], L.LT, [500.0], cols, accs, overwrite=first)
+                    first = False
+                    rows += int(b[0][0].shape[0])
+            s.synchronize()
+            dt = time.perf_counter() - t0
+            rd.close()
+            return rows, dt, [ops.acc_read(a) for a in accs[0]]
+
+        run(True)
+        rows_r, t_ref, acc_r = run(False)
+        rows_s, t_scale, acc_s = run(True)
+        assert rows_r == rows_s == n and acc_r == acc_s
+        # PCIe roofline of the path: pinned host -> device, 1 GiB, best of 3
+        hb = torch.empty(1 << 30, dtype=torch.uint8, pin_memory=True)
+        db = torch.empty(1 << 30, dtype=torch.uint8, device=dev)
+        h2d_ms = _timed_ms(lambda: db.copy_(hb, non_blocking=True), reps=3)
+        pcie = (1 << 30) / (h2d_ms * 1e-3) / 1e9
+        del hb, db
+        col_bytes = n * (4 + 4 + 4 + 3)  # three columns and their null bytes cross PCIe
+        return {"rows": n, "csv_GB": size / 1e9, "reference_sizing_s": t_ref, "scale_mode_s": t_scale,
+                "speedup": t_ref / t_scale, "rows_per_s": n / t_scale, "file_GB_s": size / t_scale / 1e9,
+                "h2d_GB": col_bytes / 1e9, "pcie_h2d_GB_s": pcie,
+                "pcie_frac": col_bytes / t_scale / 1e9 / pcie,
+                "what": "CSV (id INT, price FLOAT, qty INT) of ~2 GB -> device, each batch folded by a fused "
+                        "WHERE price < 500 -> COUNT, SUM(qty) kernel: 4 MiB batches with copies on the caller's "
+                        "stream vs 16M-row batches, 3 pinned staging areas, copies on the reader's own stream"}
+    finally:
+        os.unlink(path)
+
+
+def config2_streamed_line(job, keys, steps):
+    """Config 2 out of core: the 200M keys start in pinned host memory; each
+    2^25-row run is copied on a copy stream while the previous run sorts,
+    then the runs are merged.  Also the bare host-to-device copy of the keys
+    (the PCIe bound of this form)."""
+    import torch
+
+    from yallasql_amd.multirun import verify_global_sort
+
+    n = keys.numel()
+    hk = torch.empty(n, dtype=torch.int64, pin_memory=True)
+    hk.copy_(keys)
+    tmp = torch.empty_like(keys)
+    h2d_ms = _timed_ms(lambda: tmp.copy_(hk, non_blocking=True), reps=3)
+    del tmp
+    cs = torch.cuda.Stream()
+    job.step_streamed(hk, cs)
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        job.step_streamed(hk, cs)
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+    ko, vo = job.result()
+    verify_global_sort(keys, ko, vo)
+    ts.sort()
+    ms = ts[len(ts) // 2] * 1e3
+    del hk
+    return {"rows": n, "ms": ms, "rows_per_s": n / (ms * 1e-3), "h2d_ms": h2d_ms,
+            "h2d_GB_s": n * 8 / (h2d_ms * 1e-3) / 1e9, "pcie_frac": (n * 8 / (ms * 1e-3)) / (n * 8 / (h2d_ms * 1e-3)),
+            "verified": True,
+            "what": "keys in pinned host memory (1.6 GB; the row-index payload is made on the device): runs copied "
+                    "on a copy stream while the previous run sorts, then one k-way merge; median of the steps"}
 
 
 def config3_line(cpu_inputs=None):
@@ -544,6 +760,9 @@ def main():
     if not args.no_cpu:
         cpu_inputs = {"config2_keys": keys[: 2 * RUN_ROWS].cpu().numpy().view("uint64")}
 
+    streamed = None
+    if not args.no_stream and not args.pairwise_merge:
+        streamed = config2_streamed_line(job, keys, args.steps)
     print(f"sort: {ms:.2f} ms per step", file=sys.stderr, flush=True)
     extra_ops = None if args.no_ops else op_lines(keys, cpu_inputs)
     print("op lines done", file=sys.stderr, flush=True)
@@ -553,7 +772,10 @@ def main():
     torch.cuda.empty_cache()
     configs = None if args.no_configs else {"config1_filter_sort_project": config1_line(cpu_inputs),
                                             "config3_star_join_group_by": config3_line(cpu_inputs),
-                                            "a23_csv_load": csv_load_line()}
+                                            "a23_csv_load": csv_load_line(),
+                                            "a23_csv_scale": csv_scale_line()}
+    if configs is not None and streamed is not None:
+        configs["config2_streamed_from_host"] = streamed
     print("config lines done", file=sys.stderr, flush=True)
     cpu = None if cpu_inputs is None else cpu_baseline(cpu_inputs)
     del cpu_inputs

@@ -70,7 +70,37 @@ int ysql_csv_next_host(ysql_csv_reader *r, void *const *host_cols, uint8_t *cons
 int ysql_csv_next(ysql_csv_reader *r, void *const *dev_cols, uint8_t *const *dev_nulls, uint32_t *rows,
                   ysql_stream_t stream);
 
-/* Waits for the reader's queued copies, frees its staging and closes the file. */
+/* ----------------------------------------------------------------------
+ * Scale mode: batches sized for HBM rather than the reference's 4 MiB, a ring
+ * of pinned staging areas, and copies on a copy stream of the reader that
+ * overlap the caller's kernels (the reference copied each column
+ * synchronously, get_operator.cpp:125-148).                              */
+typedef struct ysql_csv_options {
+  uint64_t batch_rows;   /* rows per batch (< 2^31); 0 = ysql_csv_batch_rows' sizing */
+  uint32_t slots;        /* staging areas in the ring, 2..64; 0 = 2 */
+  uint32_t async_copies; /* nonzero: the reader owns device buffers per area and copies on its own
+                            stream (ysql_csv_next_async); zero: ysql_csv_next as above */
+} ysql_csv_options;
+/* ysql_csv_open with options (opt may be NULL: the defaults).  Pinned host
+ * memory = slots x batch_rows x (row bytes + 1 null byte per column), and as
+ * much device memory with async_copies.                                    */
+int ysql_csv_open_ex(const char *path, int ncols, const char *const *names, const int *types,
+                     const ysql_csv_options *opt, ysql_csv_reader **out);
+/* Next batch into the reader's device buffers (reader opened with
+ * async_copies): parses into the next staging area, queues its host-to-device
+ * copies on the reader's copy stream, makes `stream` wait for them, and
+ * returns the batch's device column and null pointers in dev_cols[c] /
+ * dev_nulls[c] (dev_nulls may be NULL).  Work queued on `stream` before the
+ * call runs while the copies do; work queued after it sees the batch.
+ * Contract: queue every piece of work that reads a batch on `stream` before
+ * the next call.  A batch's buffers are overwritten by the call `slots`
+ * calls later, after that work has completed (the copy stream waits for it
+ * on the device, not the host).                                            */
+int ysql_csv_next_async(ysql_csv_reader *r, void **dev_cols, uint8_t **dev_nulls, uint32_t *rows,
+                        ysql_stream_t stream);
+
+/* Waits for the reader's queued copies (and, for a reader opened with
+ * async_copies, for the device), frees its staging and closes the file. */
 void ysql_csv_close(ysql_csv_reader *r);
 
 /* ----------------------------------------------------------------------

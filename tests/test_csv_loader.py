@@ -198,6 +198,45 @@ def test_device_path_matches_reference(loader, tmp_path, dev):
         assert np.array_equal(dc, want_c[c]) and np.array_equal(dn, want_n[c]), c
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("batch,slots", [(7_001, 2), (40_000, 3), (1 << 20, 4)])
+def test_async_ring_matches_reference(loader, tmp_path, dev, batch, slots):
+    """Scale mode (ysql_csv_open_ex: batch size of our choosing, a ring of
+    pinned staging areas with device buffers of the reader, copies on its
+    own stream): each batch is consumed by work queued on the caller's
+    stream before the next call, while the next copies run; every column
+    equals what the reference's parser reads."""
+    import torch
+
+    n = 100_000
+    p = tmp_path / "g.csv"
+    _gen(p, n, 12)
+    names, types = ["id", "price", "ts", "name"], [INT, FLOAT, DATETIME, STRING]
+    want_c, want_n = R.read(p, names, types) if R.available() else O.read(p, names, types)
+    rd = loader.CsvReader(p, names, types, batch_rows_=batch, slots=slots, async_copies=True)
+    s = torch.cuda.Stream(device=dev)
+    outs = [torch.empty((n,) + ((256,) if t == STRING else ()), dtype=torch.uint8 if t == STRING else
+                        {INT: torch.int32, FLOAT: torch.float32, DATETIME: torch.int64}[t], device=dev) for t in types]
+    onul = [torch.empty(n, dtype=torch.uint8, device=dev) for _ in types]
+    pos, nb = 0, 0
+    with torch.cuda.stream(s):
+        while True:
+            b = rd.next_async(stream=s)
+            if b is None:
+                break
+            m = b[0][0].shape[0]
+            for c in range(4):  # the consumer: kernels on the caller's stream
+                outs[c][pos:pos + m].copy_(b[0][c])
+                onul[c][pos:pos + m].copy_(b[1][c])
+            pos, nb = pos + m, nb + 1
+    s.synchronize()
+    rd.close()
+    assert pos == n and nb == -(-n // batch)
+    for c in range(4):
+        assert np.array_equal(outs[c].cpu().numpy(), want_c[c]), c
+        assert np.array_equal(onul[c].cpu().numpy(), want_n[c]), c
+
+
 def test_parallel_parse_quotes_and_blank_lines(loader, tmp_path):
     """A batch of 200,000 rows (two 4-byte columns: 524,288 rows per batch)
     is split over several parsing threads; quoted fields holding commas,

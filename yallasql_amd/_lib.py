@@ -118,6 +118,8 @@ _SIGS = {
     "ysql_csv_open": [C.c_char_p, I32, C.POINTER(C.c_char_p), C.POINTER(I32), C.POINTER(P)],
     "ysql_csv_next_host": [P, C.POINTER(P), C.POINTER(P), C.POINTER(U32)],
     "ysql_csv_next": [P, C.POINTER(P), C.POINTER(P), C.POINTER(U32), P],
+    "ysql_csv_open_ex": [C.c_char_p, I32, C.POINTER(C.c_char_p), C.POINTER(I32), P, C.POINTER(P)],
+    "ysql_csv_next_async": [P, C.POINTER(P), C.POINTER(P), C.POINTER(U32), P],
     "ysql_filter_compact": [I32, I32, P, P, C.POINTER(Col), I32, U64, P, P, C.c_size_t, P],
     "ysql_filter_compact_noalloc": [I32, I32, P, P, C.POINTER(Col), I32, U64, P, P],
     "ysql_filter_compact_range": [I32, P, P, C.POINTER(Col), I32, U64, U64, P, P, C.c_size_t, P],

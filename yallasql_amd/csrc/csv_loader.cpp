@@ -114,12 +114,19 @@ void drop_pair_quotes(std::string &f) {
 }
 
 // One staging area: a pinned buffer per column and per null column, and the
-// event recorded after the copies queued from it.
+// event recorded after the copies queued from it.  In the asynchronous mode
+// it also owns the batch's device buffers, and `used` is recorded on the
+// caller's stream at the call after the one that handed the batch out (its
+// work on the batch is queued by then); a copy into the area waits on it.
 struct Stage {
   std::vector<void *> cols;
   std::vector<uint8_t *> nulls;
+  std::vector<void *> dcols;
+  std::vector<uint8_t *> dnulls;
   hipEvent_t done = nullptr;
+  hipEvent_t used = nullptr;
   bool pending = false;
+  bool used_pending = false;
 };
 
 }  // namespace
@@ -325,8 +332,10 @@ struct ysql_csv_reader {
   int threads = 1;
   std::vector<const char *> starts;  // record starts of the batch being parsed
   std::vector<uint8_t> keep;         // per record of the batch: its field count matches the header
-  Stage st[2];
+  std::vector<Stage> st = std::vector<Stage>(2);
   int cur = 0;
+  hipStream_t copy_stream = nullptr;  // asynchronous mode: the reader's copy stream
+  int handed = -1;                    // asynchronous mode: the area handed out by the last call
 
   const char *quote_at_or_after(const char *p) {
     if (next_quote < p) {
@@ -610,12 +619,44 @@ int parse_batch(ysql_csv_reader *r, void *const *cols, uint8_t *const *nulls, ui
 
 void free_stage(Stage &s) {
   if (s.pending && s.done) (void)hipEventSynchronize(s.done);
+  if (s.used_pending && s.used) (void)hipEventSynchronize(s.used);
   for (void *p : s.cols)
     if (p) (void)hipHostFree(p);
   for (uint8_t *p : s.nulls)
     if (p) (void)hipHostFree(p);
+  for (void *p : s.dcols)
+    if (p) (void)hipFree(p);
+  for (uint8_t *p : s.dnulls)
+    if (p) (void)hipFree(p);
   if (s.done) (void)hipEventDestroy(s.done);
+  if (s.used) (void)hipEventDestroy(s.used);
   s = Stage();
+}
+
+// Pinned staging (and, with `device`, device buffers) for one batch.
+int alloc_stage(ysql_csv_reader *r, Stage &st, bool device) {
+  const size_t nc = r->types.size();
+  st.cols.assign(nc, nullptr);
+  st.nulls.assign(nc, nullptr);
+  if (device) {
+    st.dcols.assign(nc, nullptr);
+    st.dnulls.assign(nc, nullptr);
+  }
+  for (size_t c = 0; c < nc; c++) {
+    const size_t b = (size_t)r->batch_rows * r->bytes[c];
+    hipError_t e = hipHostMalloc(&st.cols[c], b, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&st.nulls[c], r->batch_rows, hipHostMallocDefault);
+    if (e == hipSuccess && device) e = hipMalloc(&st.dcols[c], b);
+    if (e == hipSuccess && device) e = hipMalloc((void **)&st.dnulls[c], r->batch_rows);
+    if (e != hipSuccess) {
+      std::string msg = std::string("csv: staging allocation failed: ") + hipGetErrorString(e);
+      ysql::set_last_error(msg.c_str());
+      return YSQL_EHIP;
+    }
+  }
+  if (hipEventCreateWithFlags(&st.done, hipEventDisableTiming) != hipSuccess) return YSQL_EHIP;
+  if (device && hipEventCreateWithFlags(&st.used, hipEventDisableTiming) != hipSuccess) return YSQL_EHIP;
+  return YSQL_OK;
 }
 
 }  // namespace
@@ -730,19 +771,13 @@ int ysql_csv_next(ysql_csv_reader *r, void *const *dev_cols, uint8_t *const *dev
     if (!dev_cols[c]) return YSQL_EINVAL;
   hipStream_t s = (hipStream_t)stream;
   Stage &st = r->st[r->cur];
+  if (r->copy_stream) {
+    ysql::set_last_error("csv: a reader opened for ysql_csv_next_async");
+    return YSQL_EINVAL;
+  }
   if (st.cols.empty()) {  // first use: pinned staging for one batch
-    st.cols.assign(nc, nullptr);
-    st.nulls.assign(nc, nullptr);
-    for (size_t c = 0; c < nc; c++) {
-      hipError_t e = hipHostMalloc(&st.cols[c], (size_t)r->batch_rows * r->bytes[c], hipHostMallocDefault);
-      if (e == hipSuccess) e = hipHostMalloc((void **)&st.nulls[c], r->batch_rows, hipHostMallocDefault);
-      if (e != hipSuccess) {
-        std::string msg = std::string("csv: pinned staging allocation failed: ") + hipGetErrorString(e);
-        ysql::set_last_error(msg.c_str());
-        return YSQL_EHIP;
-      }
-    }
-    if (hipEventCreateWithFlags(&st.done, hipEventDisableTiming) != hipSuccess) return YSQL_EHIP;
+    int rc = alloc_stage(r, st, false);
+    if (rc) return rc;
   }
   if (st.pending) {  // the copies queued from this staging area must have landed
     if (hipEventSynchronize(st.done) != hipSuccess) return YSQL_EHIP;
@@ -761,14 +796,90 @@ int ysql_csv_next(ysql_csv_reader *r, void *const *dev_cols, uint8_t *const *dev
   }
   if (hipEventRecord(st.done, s) != hipSuccess) return YSQL_EHIP;
   st.pending = true;
-  r->cur ^= 1;
+  r->cur = (r->cur + 1) % (int)r->st.size();
+  return YSQL_OK;
+}
+
+int ysql_csv_open_ex(const char *path, int ncols, const char *const *names, const int *types,
+                     const ysql_csv_options *opt, ysql_csv_reader **out) {
+  if (!out) return YSQL_EINVAL;
+  *out = nullptr;
+  if (opt && opt->slots != 0 && (opt->slots < 2 || opt->slots > 64)) return YSQL_EINVAL;
+  if (opt && opt->batch_rows >= (1ull << 31)) return YSQL_EINVAL;
+  ysql_csv_reader *r = nullptr;
+  int rc = ysql_csv_open(path, ncols, names, types, &r);
+  if (rc) return rc;
+  if (opt && opt->batch_rows) r->batch_rows = (uint32_t)opt->batch_rows;
+  if (opt && opt->slots) r->st.resize(opt->slots);
+  if (opt && opt->async_copies) {
+    if (hipStreamCreateWithFlags(&r->copy_stream, hipStreamNonBlocking) != hipSuccess) {
+      ysql::set_last_error("csv: cannot create the copy stream");
+      ysql_csv_close(r);
+      return YSQL_EHIP;
+    }
+  }
+  *out = r;
+  return YSQL_OK;
+}
+
+int ysql_csv_next_async(ysql_csv_reader *r, void **dev_cols, uint8_t **dev_nulls, uint32_t *rows,
+                        ysql_stream_t stream) {
+  if (!r || !dev_cols || !rows) return YSQL_EINVAL;
+  if (!r->copy_stream) {
+    ysql::set_last_error("csv: ysql_csv_next_async needs a reader opened with async_copies");
+    return YSQL_EINVAL;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  const size_t nc = r->types.size();
+  // the caller's work on the batch handed out last is queued by now
+  if (r->handed >= 0) {
+    Stage &h = r->st[r->handed];
+    if (hipEventRecord(h.used, s) != hipSuccess) return YSQL_EHIP;
+    h.used_pending = true;
+    r->handed = -1;
+  }
+  Stage &st = r->st[r->cur];
+  if (st.cols.empty()) {
+    int rc = alloc_stage(r, st, true);
+    if (rc) return rc;
+  }
+  if (st.pending) {  // the staging area's previous copies have landed
+    if (hipEventSynchronize(st.done) != hipSuccess) return YSQL_EHIP;
+    st.pending = false;
+  }
+  int rc = parse_batch(r, st.cols.data(), st.nulls.data(), rows);
+  if (rc) return rc;
+  if (*rows == 0) return YSQL_OK;
+  // the copies wait, on the copy stream, for the caller's work on the
+  // batch that last used these device buffers; the caller's stream then
+  // waits for the copies, so its earlier work overlaps them
+  if (st.used_pending) {
+    if (hipStreamWaitEvent(r->copy_stream, st.used, 0) != hipSuccess) return YSQL_EHIP;
+    st.used_pending = false;
+  }
+  for (size_t c = 0; c < nc; c++) {
+    if (hipMemcpyAsync(st.dcols[c], st.cols[c], (size_t)*rows * r->bytes[c], hipMemcpyHostToDevice,
+                       r->copy_stream) != hipSuccess ||
+        hipMemcpyAsync(st.dnulls[c], st.nulls[c], *rows, hipMemcpyHostToDevice, r->copy_stream) != hipSuccess)
+      return YSQL_EHIP;
+    dev_cols[c] = st.dcols[c];
+    if (dev_nulls) dev_nulls[c] = st.dnulls[c];
+  }
+  if (hipEventRecord(st.done, r->copy_stream) != hipSuccess) return YSQL_EHIP;
+  if (hipStreamWaitEvent(s, st.done, 0) != hipSuccess) return YSQL_EHIP;
+  st.pending = true;
+  r->handed = r->cur;
+  r->cur = (r->cur + 1) % (int)r->st.size();
   return YSQL_OK;
 }
 
 void ysql_csv_close(ysql_csv_reader *r) {
   if (!r) return;
-  free_stage(r->st[0]);
-  free_stage(r->st[1]);
+  // the caller's work on the last batch handed out may still read the
+  // reader's device buffers: let the device finish before freeing them
+  if (r->copy_stream) (void)hipDeviceSynchronize();
+  for (Stage &st : r->st) free_stage(st);
+  if (r->copy_stream) (void)hipStreamDestroy(r->copy_stream);
   if (r->data) munmap((void *)r->data, r->size);
   if (r->fd >= 0) close(r->fd);
   delete r;

@@ -216,14 +216,22 @@ __global__ __launch_bounds__(kScanBlock) void k_xfilter(XProg p, ColArgs cols, i
   const int t = threadIdx.x;
   XRegs<S, R> reg;
   reg.clear();
+  S acc[R] = {};
+  uint32_t nacc = 0;
   uint8_t *sel = reinterpret_cast<uint8_t *>(s_sel);
 #pragma unroll
   for (int j = 0; j < kScanItems / R; j++) {
     const uint32_t r0 = j * (kScanBlock * R) + t * R;
-    xload_inputs<S, R, false>(p, base + r0, n, reg);
-    xrun<S, R, false>(p, reg);
+    const bool fast = p.aligned && base + (uint64_t)(j + 1) * (kScanBlock * R) <= n;  // wave-uniform
+    xload_inputs<S, R, false>(p, base + r0, n, fast, reg);
+    xrun<S, R, false>(p, reg, acc, nacc);
     S pr[R];
-    xget(reg, p.pred, pr);
+    if (p.pred == kXAcc) {
+#pragma unroll
+      for (int e = 0; e < R; e++) pr[e] = acc[e];
+    } else {
+      xget(reg, p.pred, pr);
+    }
     if constexpr (R == 4) {
       s_sel[r0 / 4] = (uint32_t)(pr[0] != 0) | (uint32_t)(pr[1] != 0) << 8 | (uint32_t)(pr[2] != 0) << 16 |
                       (uint32_t)(pr[3] != 0) << 24;
@@ -444,10 +452,9 @@ int ysql_expr_filter_compact(const ysql_xinsn *prog, int ninsn, int pred_reg, co
   XProg p;
   bool wide = false;
   int8_t ty[kXRegs];
-  int rc = xprog_prepare(prog, ninsn, inputs, ninputs, n, &p, &wide, ty);
+  if (pred_reg < 0 || pred_reg >= kXRegs) return YSQL_EINVAL;
+  int rc = xprog_prepare(prog, ninsn, inputs, ninputs, n, 0u, pred_reg, &p, &wide, ty);
   if (rc) return rc;
-  if (pred_reg < 0 || pred_reg >= kXRegs || ty[pred_reg] != YSQL_BOOL) return YSQL_EINVAL;
-  p.pred = pred_reg;
   p.nout = 0;
   if (!cols_ok(cols, ncols, n > 0)) return YSQL_EINVAL;
   if (n >= (1ull << 40)) return YSQL_EINVAL;

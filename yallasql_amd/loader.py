@@ -19,15 +19,35 @@ def batch_rows(types):
     return out.value
 
 
+_TYPESTR = {L.INT: "<i4", L.FLOAT: "<f4", L.DATETIME: "<i8", L.STRING: "|u1"}
+
+
+class _DeviceView:
+    """A device buffer of the reader, exposed to torch without a copy."""
+
+    def __init__(self, ptr, shape, typestr):
+        self.__cuda_array_interface__ = {"shape": shape, "typestr": typestr, "data": (ptr, False), "version": 2,
+                                         "strides": None}
+
+
+class CsvOptions(C.Structure):
+    _fields_ = [("batch_rows", C.c_uint64), ("slots", C.c_uint32), ("async_copies", C.c_uint32)]
+
+
 class CsvReader:
-    def __init__(self, path, names, types):
+    def __init__(self, path, names, types, batch_rows_=0, slots=0, async_copies=False):
+        """batch_rows_ / slots / async_copies: the scale mode of
+        ysql_csv_open_ex (0 / 0 / False: the reference's sizing, 2 staging
+        areas, copies on the caller's stream)."""
         self.types = list(types)
-        self.rows = batch_rows(self.types)
+        self.rows = int(batch_rows_) if batch_rows_ else batch_rows(self.types)
+        self.async_copies = bool(async_copies)
         n = len(names)
         cn = (C.c_char_p * n)(*[x.encode() for x in names])
         ct = (C.c_int * n)(*self.types)
         h = C.c_void_p()
-        L.call("ysql_csv_open", str(path).encode(), n, cn, ct, C.byref(h))
+        opt = CsvOptions(int(batch_rows_), int(slots), int(bool(async_copies)))
+        L.call("ysql_csv_open_ex", str(path).encode(), n, cn, ct, C.byref(opt), C.byref(h))
         self.handle = h
 
     def _host_bufs(self):
@@ -70,6 +90,28 @@ class CsvReader:
         if rows.value == 0:
             return None
         return [a[: rows.value] for a in cols], [a[: rows.value] for a in nulls]
+
+    def next_async(self, stream=None):
+        """The next batch in the reader's own device buffers (scale mode), as
+        torch tensors viewing them: ([column], [null bytes]).  They hold the
+        batch until the call `slots` calls later; queue all work reading them
+        on `stream` (default: the current stream) before the next call.
+        None at the end."""
+        import torch
+
+        n = len(self.types)
+        pc = (C.c_void_p * n)()
+        pn = (C.c_void_p * n)()
+        rows = C.c_uint32()
+        s = stream if stream is not None else torch.cuda.current_stream()
+        L.call("ysql_csv_next_async", self.handle, pc, pn, C.byref(rows), C.c_void_p(s.cuda_stream))
+        m = rows.value
+        if m == 0:
+            return None
+        cols = [torch.as_tensor(_DeviceView(pc[i], (m, L.STRING_BYTES) if t == L.STRING else (m,), _TYPESTR[t]),
+                                device=s.device) for i, t in enumerate(self.types)]
+        nulls = [torch.as_tensor(_DeviceView(pn[i], (m,), "|u1"), device=s.device) for i in range(n)]
+        return cols, nulls
 
     def close(self):
         if getattr(self, "handle", None) is not None and L.lib is not None:

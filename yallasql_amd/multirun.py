@@ -111,6 +111,33 @@ class MultiRunSort:
             runs = nxt
         self.final = runs[0]
 
+    def step_streamed(self, host_keys, copy_stream):
+        """step() with the keys arriving from pinned host memory, out of core:
+        every run is copied into its slice of keys_in on copy_stream, and the
+        current stream sorts a run as soon as its copy has landed, so run r
+        sorts while run r+1 crosses PCIe.  The copies first wait for the
+        current stream's earlier work (a previous step still reading
+        keys_in)."""
+        cur = torch.cuda.current_stream()
+        copy_stream.wait_stream(cur)
+        landed = []
+        with torch.cuda.stream(copy_stream):
+            for o, r in self.runs:
+                self.keys_in[o:o + r].copy_(host_keys[o:o + r], non_blocking=True)
+                e = torch.cuda.Event()
+                e.record(copy_stream)
+                landed.append(e)
+        runs = []
+        for (o, r), e in zip(self.runs, landed):
+            cur.wait_event(e)
+            runs.append(self.sort_run(o, r))
+        if len(runs) > 1:
+            ops.merge_runs(self.bk[0], self.bv[0], self.run_offsets, out_keys=self.bk[1], out_vals=self.bv[1],
+                           ws=self.merge_ws)
+            self.final = (1, 0, self.n)
+        else:
+            self.final = runs[0]
+
     def result(self):
         b, o, n = self.final
         return self.bk[b][o:o + n], self.bv[b][o:o + n]
