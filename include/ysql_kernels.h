@@ -501,6 +501,10 @@ int ysql_filter_aggregate(int pred_dtype, int cmp, const void *pred_col, const v
  * accumulators instead of folding it in (no ysql_acc_init needed; an empty
  * input stores the identities). */
 #define YSQL_FA_OVERWRITE 1
+/* YSQL_FA_GENERIC runs the general tile kernel even where the two-column
+ * kernel applies (every spec reading the predicate column, one other column
+ * or nothing); both give the same bits.  For tests and measurements.      */
+#define YSQL_FA_GENERIC 2
 int ysql_filter_aggregate_ex(int pred_dtype, int cmp, const void *pred_col, const void *scalar,
                              const ysql_agg_spec *specs, int nspecs, uint64_t n, int flags,
                              void *ws, size_t ws_bytes, ysql_stream_t stream);

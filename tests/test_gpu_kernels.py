@@ -535,6 +535,57 @@ def test_filter_aggregate_shared_scan_same_bits(ops, dev, n, pdt):
                     assert np.array_equal(N(a), N(b)), (cmp, nq, q)
 
 
+@pytest.mark.parametrize("pdt", [np.int32, np.float32, np.int64])
+def test_filter_aggregate_two_column_kernel_equals_generic(ops, dev, pdt):
+    """The two-column kernel (a comparison as a range test, static fold
+    bodies) returns the bytes of the general tile kernel for every operator
+    and the edge constants: NaN, +-inf, +-0, the smallest subnormals,
+    INT_MIN / INT_MAX; with predicate values holding the same edges; one
+    query and shared scans of four."""
+    r = rng(5)
+    n = 70_001
+    if pdt == np.float32:
+        pred = (r.standard_normal(n) * 3).astype(np.float32)
+        edges = np.array([np.nan, np.inf, -np.inf, 0.0, -0.0, 1.4e-45, -1.4e-45, 3.0, -3.0], np.float32)
+        pred[::97] = edges[r.integers(0, len(edges), len(pred[::97]))]
+        consts = [float(x) for x in edges] + [0.5]
+    else:
+        info = np.iinfo(pdt)
+        pred = r.integers(-5, 6, n).astype(pdt)
+        pred[::101] = info.min
+        pred[::103] = info.max
+        consts = [int(info.min), int(info.max), int(info.min) + 1, int(info.max) - 1, 0, 3, -2]
+    tp = T(pred, dev)
+    other = T(r.integers(-2**40, 2**40, n).astype(np.int64), dev)
+    i32 = T(r.integers(-1000, 1000, n).astype(np.int32), dev)
+    pdc = O._DT[np.dtype(pdt)]
+    shapes = [
+        [(O.INT, 5, None), (pdc, 2, tp), (O.DATETIME, 1, other), (O.DATETIME, 0, other)],
+        [(O.INT, 4, None), (O.INT, 2, i32), (pdc, 1, tp), (pdc, 0, tp)],
+        [(O.INT, 5, None)],
+    ]
+    for cmp in range(6):
+        for c in consts:
+            for cols in shapes:
+                got, ref = ([ops.acc_new(dc, agg, device=dev) for dc, agg, _ in cols] for _ in range(2))
+                ops.filter_aggregate(tp, cmp, c, [(dc, agg, col, a) for (dc, agg, col), a in zip(cols, got)],
+                                     overwrite=True)
+                ops.filter_aggregate(tp, cmp, c, [(dc, agg, col, a) for (dc, agg, col), a in zip(cols, ref)],
+                                     overwrite=True, generic=True)
+                for a, b in zip(got, ref):
+                    assert np.array_equal(N(a), N(b)), (cmp, c, cols)
+                m = O.compare(cmp, pred, np.array([c], dtype=pdt), rhs_scalar=True, n=n).astype(bool)
+                assert ops.acc_read(got[0])[0]["count"] == int(m.sum()), (cmp, c)
+        cols = shapes[0]
+        accs = [[ops.acc_new(dc, agg, device=dev) for dc, agg, _ in cols] for _ in range(4)]
+        gacc = [[ops.acc_new(dc, agg, device=dev) for dc, agg, _ in cols] for _ in range(4)]
+        ops.filter_aggregate_multi(tp, cmp, consts[:4], cols, accs)
+        ops.filter_aggregate_multi(tp, cmp, consts[:4], cols, gacc, generic=True)
+        for qa, qb in zip(accs, gacc):
+            for a, b in zip(qa, qb):
+                assert np.array_equal(N(a), N(b)), (cmp, "shared")
+
+
 def test_filter_aggregate_fold_vs_overwrite(ops, dev):
     r = rng(3)
     x = r.random(100_000).astype(np.float32)

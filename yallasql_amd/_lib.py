@@ -198,6 +198,7 @@ class AggSpec(C.Structure):
 
 
 FA_OVERWRITE = 1
+FA_GENERIC = 2
 
 
 class BatchTuning(C.Structure):

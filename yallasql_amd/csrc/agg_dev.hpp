@@ -9,6 +9,7 @@
 // launches the tiles.
 #pragma once
 
+#include <limits>
 #include <type_traits>
 
 #include "common.hpp"
@@ -196,6 +197,101 @@ __device__ __forceinline__ ysql_acc fa_val_to_acc(uint64_t v, uint32_t count, co
   return a;
 }
 
+// `x <cmp> c` as `(lo <= x && x <= hi) != neg`, so one kernel serves every
+// comparison: strict bounds become inclusive at the neighbouring value (the
+// next float toward the open side, c -/+ 1 for integers), NE is EQ negated,
+// and an impossible range (x < -inf, x > INT_MAX, any ordered comparison
+// with NaN) is lo > hi.  Exact for every x: NaN fails both tests, as it
+// fails every ordered comparison, and passes NE by the negation.
+template <typename PT>
+struct Fa2Range {
+  PT lo, hi;
+};
+template <typename PT>
+__device__ __forceinline__ void fa2_empty(Fa2Range<PT> &r) {
+  if constexpr (std::is_same<PT, float>::value) {
+    r.lo = __builtin_inff();
+    r.hi = -__builtin_inff();
+  } else {
+    r.lo = 1;
+    r.hi = 0;
+  }
+}
+// The neighbour of c toward -inf (down) or +inf; false when there is none.
+template <typename PT>
+__device__ __forceinline__ bool fa2_step_value(PT c, bool down, PT *out) {
+  if constexpr (std::is_same<PT, float>::value) {
+    if (isnan(c) || (down && c == -__builtin_inff()) || (!down && c == __builtin_inff())) return false;
+    const uint32_t b = __float_as_uint(c);
+    uint32_t nb;
+    if (c == 0.f) nb = down ? 0x80000001u : 0x00000001u;  // the smallest subnormals
+    else if ((b >> 31) == 0) nb = down ? b - 1 : b + 1;    // positive: bits grow with the value
+    else nb = down ? b + 1 : b - 1;                        // negative: bits grow with the magnitude
+    *out = __uint_as_float(nb);
+    return true;
+  } else {
+    if (down && c == std::numeric_limits<PT>::min()) return false;
+    if (!down && c == std::numeric_limits<PT>::max()) return false;
+    *out = down ? (PT)(c - 1) : (PT)(c + 1);
+    return true;
+  }
+}
+template <typename PT>
+__device__ __forceinline__ Fa2Range<PT> fa2_range(int cmp, PT c) {
+  Fa2Range<PT> r;
+  PT lowest, highest;
+  if constexpr (std::is_same<PT, float>::value) {
+    lowest = -__builtin_inff();
+    highest = __builtin_inff();
+    if (isnan(c)) {
+      fa2_empty(r);
+      return r;
+    }
+  } else {
+    lowest = std::numeric_limits<PT>::min();
+    highest = std::numeric_limits<PT>::max();
+  }
+  PT v;
+  switch (cmp) {
+    case YSQL_LT:
+      r.lo = lowest;
+      if (fa2_step_value(c, true, &v)) r.hi = v;
+      else fa2_empty(r);
+      break;
+    case YSQL_LE: r.lo = lowest; r.hi = c; break;
+    case YSQL_GT:
+      r.hi = highest;
+      if (fa2_step_value(c, false, &v)) r.lo = v;
+      else fa2_empty(r);
+      break;
+    case YSQL_GE: r.lo = c; r.hi = highest; break;
+    default: r.lo = c; r.hi = c; break;  // EQ, and NE negated
+  }
+  return r;
+}
+
+// Row selection of the filtered aggregates, by query: a comparison fixed at
+// compile time (FaSelCmp) or any comparison as a range test (FaSelRange).
+template <typename PT, int CMP, int QMAX>
+struct FaSelCmp {
+  PT c[QMAX];
+  __device__ __forceinline__ bool operator()(int q, PT x) const { return cmp_static<CMP>(x, c[q]); }
+};
+template <typename PT, int QMAX>
+struct FaSelRange {
+  Fa2Range<PT> rg[QMAX];
+  bool neg;
+  __device__ __forceinline__ bool operator()(int q, PT x) const { return (rg[q].lo <= x && x <= rg[q].hi) != neg; }
+};
+template <typename PT, int QMAX>
+__device__ __forceinline__ FaSelRange<PT, QMAX> fa_sel_range(int cmp, const PT *scalars, int nq) {
+  FaSelRange<PT, QMAX> r;
+#pragma unroll
+  for (int q = 0; q < QMAX; q++) r.rg[q] = fa2_range<PT>(cmp, q < nq ? scalars[q] : PT());
+  r.neg = cmp == YSQL_NE;
+  return r;
+}
+
 // ---------------------------------------------------------------------------
 // The tile kernel: each thread folds every spec of every query (up to QMAX
 // sharing the scan) for its virtual thread, kFaRpt rows per step.
@@ -282,8 +378,8 @@ __device__ __forceinline__ void fa_fold_words(uint64_t &val, int opc, const uint
 // and each spec column that is not the predicate column or an earlier
 // spec's column), then evaluates each query's predicate and folds the
 // thread's rows in row order.
-template <bool FULL, bool SPECS, typename PT, int CMP, int BLOCK, int QMAX>
-__device__ __forceinline__ void fa_step(uint64_t s0, uint64_t end, const PT *pred, const PT (&scalar)[QMAX], int nq,
+template <bool FULL, bool SPECS, typename PT, typename SEL, int BLOCK, int QMAX>
+__device__ __forceinline__ void fa_step(uint64_t s0, uint64_t end, const PT *pred, const SEL &selq, int nq,
                                         const FaSpec (&sp)[kFaMaxSpecs], const int (&opc)[kFaMaxSpecs], int ns,
                                         uint64_t (&val)[QMAX][kFaMaxSpecs], uint32_t (&cnt)[QMAX]) {
   constexpr uint64_t stride = (uint64_t)BLOCK * kFaPair;
@@ -324,7 +420,7 @@ __device__ __forceinline__ void fa_step(uint64_t s0, uint64_t end, const PT *pre
     bool sel[kFaRpt];
 #pragma unroll
     for (int k = 0; k < kFaRpt; k++) {
-      sel[k] = cmp_static<CMP>(pv[k], scalar[q]);
+      sel[k] = selq(q, pv[k]);
       if constexpr (!FULL) sel[k] = sel[k] && r0 + (uint64_t)(k >> 1) * stride + (k & 1) < end;
       cnt[q] += sel[k] ? 1u : 0u;
     }
@@ -342,9 +438,9 @@ __device__ __forceinline__ void fa_step(uint64_t s0, uint64_t end, const PT *pre
 // columns take unguarded vector loads; the rest (the last, partial step of
 // the input, or columns not aligned to two elements) the guarded path, with
 // the same rows per thread and the same fold order.
-template <typename PT, int CMP, int BLOCK, int QMAX>
+template <typename PT, typename SEL, int BLOCK, int QMAX>
 __device__ __forceinline__ void filter_agg_tile(uint32_t tile, uint64_t tile_rows, const PT *pred,
-                                                const PT (&scalar)[QMAX], int nq, const FaSpec *specs, int ns,
+                                                const SEL &selq, int nq, const FaSpec *specs, int ns,
                                                 uint64_t n, ysql_acc *partials, ysql_acc *s_w) {
   const uint64_t base = (uint64_t)tile * tile_rows;
   const uint64_t end = base + tile_rows < n ? base + tile_rows : n;
@@ -374,12 +470,12 @@ __device__ __forceinline__ void filter_agg_tile(uint32_t tile, uint64_t tile_row
   // COUNT-only queries: a predicate-only loop
   if (aligned && any)
     for (; s0 + step <= end; s0 += step)
-      fa_step<true, true, PT, CMP, BLOCK, QMAX>(s0, end, pred, scalar, nq, sp, opc, ns, val, cnt);
+      fa_step<true, true, PT, SEL, BLOCK, QMAX>(s0, end, pred, selq, nq, sp, opc, ns, val, cnt);
   if (aligned && !any)
     for (; s0 + step <= end; s0 += step)
-      fa_step<true, false, PT, CMP, BLOCK, QMAX>(s0, end, pred, scalar, nq, sp, opc, ns, val, cnt);
+      fa_step<true, false, PT, SEL, BLOCK, QMAX>(s0, end, pred, selq, nq, sp, opc, ns, val, cnt);
   for (; s0 < end; s0 += step)
-    fa_step<false, true, PT, CMP, BLOCK, QMAX>(s0, end, pred, scalar, nq, sp, opc, ns, val, cnt);
+    fa_step<false, true, PT, SEL, BLOCK, QMAX>(s0, end, pred, selq, nq, sp, opc, ns, val, cnt);
 #pragma unroll
   for (int q = 0; q < QMAX; q++) {
     if (q >= nq) break;
@@ -393,38 +489,259 @@ __device__ __forceinline__ void filter_agg_tile(uint32_t tile, uint64_t tile_row
   }
 }
 
+// ---------------------------------------------------------------------------
+// Two-column form: every spec reads the predicate column (slot 0), one other
+// column (slot 1, elements of W1 bytes; W1 = 0: none) or nothing (COUNT).
+// The step loads both columns once, evaluates the comparison as a range test
+// (fa2_range) and folds each spec with the static body of its (opcode, slot)
+// pair.  The kernel stays a few hundred instructions
+// long, where the generic tile above, which must move any spec's words into
+// place at run time, runs to thousands and misses the instruction cache.
+// Rows per thread, fold order and partial layout are those of
+// filter_agg_tile, so every result is bit-identical to it.
+// ---------------------------------------------------------------------------
+__host__ __device__ inline int fa2_code(int opc, int slot) { return opc * 2 + slot; }
+
+
+// One spec's fold over the step's rows from slot 0 (pw) or slot 1 (cw).
+template <int PB, int W1>
+__device__ __forceinline__ void fa2_fold(uint64_t &val, int code, const uint32_t (&pw)[2 * kFaRpt],
+                                         const uint32_t (&cw)[2 * kFaRpt], const bool (&sel)[kFaRpt]) {
+  switch (code) {
+#define YSQL_FA2_OPC(OPC)                                                                       \
+  case OPC * 2:                                                                                 \
+    if constexpr ((fa_opc_wide(OPC) ? 8 : 4) == PB) fa_fold_words_op<OPC>(val, pw, sel);        \
+    break;                                                                                      \
+  case OPC * 2 + 1:                                                                             \
+    if constexpr ((fa_opc_wide(OPC) ? 8 : 4) == W1) fa_fold_words_op<OPC>(val, cw, sel);        \
+    break;
+    YSQL_FA2_OPC(FA_OPC_I32_SUM)
+    YSQL_FA2_OPC(FA_OPC_I32_MIN)
+    YSQL_FA2_OPC(FA_OPC_I32_MAX)
+    YSQL_FA2_OPC(FA_OPC_I64_SUM)
+    YSQL_FA2_OPC(FA_OPC_I64_MIN)
+    YSQL_FA2_OPC(FA_OPC_I64_MAX)
+    YSQL_FA2_OPC(FA_OPC_F_MIN)
+    YSQL_FA2_OPC(FA_OPC_F_MAX)
+    YSQL_FA2_OPC(FA_OPC_D_SUM)
+#undef YSQL_FA2_OPC
+    default: break;  // COUNT: the per-query row count carries it
+  }
+}
+
+template <bool FULL, typename PT, typename SEL, int W1, int BLOCK, int QMAX>
+__device__ __forceinline__ void fa2_step(uint64_t s0, uint64_t end, const PT *pred, const void *col1,
+                                         const SEL &selq, int nq, const int (&code)[kFaMaxSpecs], int ns,
+                                         uint64_t (&val)[QMAX][kFaMaxSpecs], uint32_t (&cnt)[QMAX]) {
+  constexpr uint64_t stride = (uint64_t)BLOCK * kFaPair;
+  const uint64_t r0 = s0 + (uint64_t)threadIdx.x * kFaPair;
+  PT pv[kFaRpt];
+  fa_load_rows<FULL, PT>(pred, r0, stride, end, pv);
+  uint32_t cw[2 * kFaRpt] = {};
+  if constexpr (W1 == 8) {
+    uint64_t v[kFaRpt];
+    fa_load_rows<FULL, uint64_t>((const uint64_t *)col1, r0, stride, end, v);
+    memcpy(cw, v, sizeof(v));
+  } else if constexpr (W1 == 4) {
+    uint32_t v[kFaRpt];
+    fa_load_rows<FULL, uint32_t>((const uint32_t *)col1, r0, stride, end, v);
+    memcpy(cw, v, sizeof(v));
+  }
+  uint32_t pw[2 * kFaRpt] = {};
+  memcpy(pw, pv, sizeof(pv));
+#pragma unroll
+  for (int q = 0; q < QMAX; q++) {
+    if (q >= nq) break;
+    bool sel[kFaRpt];
+#pragma unroll
+    for (int k = 0; k < kFaRpt; k++) {
+      sel[k] = selq(q, pv[k]);
+      if constexpr (!FULL) sel[k] = sel[k] && r0 + (uint64_t)(k >> 1) * stride + (k & 1) < end;
+      cnt[q] += sel[k] ? 1u : 0u;
+    }
+#pragma unroll
+    for (int s = 0; s < kFaMaxSpecs; s++)
+      if (s < ns) fa2_fold<(int)sizeof(PT), W1>(val[q][s], code[s], pw, cw, sel);
+  }
+}
+
+// The same step with every spec's (opcode, slot) pair fixed at compile time
+// (code C < 0: no spec there): no dispatch is left in the loop.  For the
+// common aggregate shapes (reduce.hip lists them); same rows, order and bits.
+template <int PB, int W1, int C>
+__device__ __forceinline__ void fa3_fold(uint64_t &val, const uint32_t (&pw)[2 * kFaRpt],
+                                         const uint32_t (&cw)[2 * kFaRpt], const bool (&sel)[kFaRpt]) {
+  if constexpr (C >= 2) {  // C / 2 = opcode, C % 2 = slot; opcode 0 (COUNT) folds nothing
+    constexpr int OPC = C / 2;
+    if constexpr (C % 2 == 0) {
+      static_assert((fa_opc_wide(OPC) ? 8 : 4) == PB, "slot 0 width");
+      fa_fold_words_op<OPC>(val, pw, sel);
+    } else {
+      static_assert((fa_opc_wide(OPC) ? 8 : 4) == W1, "slot 1 width");
+      fa_fold_words_op<OPC>(val, cw, sel);
+    }
+  }
+}
+
+template <bool FULL, typename PT, typename SEL, int W1, int BLOCK, int QMAX, int C0, int C1, int C2, int C3>
+__device__ __forceinline__ void fa3_step(uint64_t s0, uint64_t end, const PT *pred, const void *col1,
+                                         const SEL &selq, int nq, uint64_t (&val)[QMAX][kFaMaxSpecs],
+                                         uint32_t (&cnt)[QMAX]) {
+  constexpr uint64_t stride = (uint64_t)BLOCK * kFaPair;
+  const uint64_t r0 = s0 + (uint64_t)threadIdx.x * kFaPair;
+  PT pv[kFaRpt];
+  fa_load_rows<FULL, PT>(pred, r0, stride, end, pv);
+  uint32_t cw[2 * kFaRpt] = {};
+  if constexpr (W1 == 8) {
+    uint64_t v[kFaRpt];
+    fa_load_rows<FULL, uint64_t>((const uint64_t *)col1, r0, stride, end, v);
+    memcpy(cw, v, sizeof(v));
+  } else if constexpr (W1 == 4) {
+    uint32_t v[kFaRpt];
+    fa_load_rows<FULL, uint32_t>((const uint32_t *)col1, r0, stride, end, v);
+    memcpy(cw, v, sizeof(v));
+  }
+  uint32_t pw[2 * kFaRpt] = {};
+  memcpy(pw, pv, sizeof(pv));
+#pragma unroll
+  for (int q = 0; q < QMAX; q++) {
+    if (q >= nq) break;
+    bool sel[kFaRpt];
+#pragma unroll
+    for (int k = 0; k < kFaRpt; k++) {
+      sel[k] = selq(q, pv[k]);
+      if constexpr (!FULL) sel[k] = sel[k] && r0 + (uint64_t)(k >> 1) * stride + (k & 1) < end;
+      cnt[q] += sel[k] ? 1u : 0u;
+    }
+    fa3_fold<(int)sizeof(PT), W1, C0>(val[q][0], pw, cw, sel);
+    fa3_fold<(int)sizeof(PT), W1, C1>(val[q][1], pw, cw, sel);
+    fa3_fold<(int)sizeof(PT), W1, C2>(val[q][2], pw, cw, sel);
+    fa3_fold<(int)sizeof(PT), W1, C3>(val[q][3], pw, cw, sel);
+  }
+}
+
+template <typename PT, typename SEL, int W1, int BLOCK, int QMAX, int C0, int C1, int C2, int C3>
+__device__ __forceinline__ void filter_agg3_tile(uint32_t tile, uint64_t tile_rows, const PT *pred,
+                                                 const void *col1, const SEL &selq, int nq, const FaSpec *specs,
+                                                 int ns, uint64_t n, bool aligned, ysql_acc *partials,
+                                                 ysql_acc *s_w) {
+  const uint64_t base = (uint64_t)tile * tile_rows;
+  const uint64_t end = base + tile_rows < n ? base + tile_rows : n;
+  uint64_t val[QMAX][kFaMaxSpecs];
+  uint32_t cnt[QMAX];
+#pragma unroll
+  for (int s = 0; s < kFaMaxSpecs; s++) {
+    const uint64_t id = s < ns ? fa_val_identity(specs[s]) : 0ull;
+#pragma unroll
+    for (int q = 0; q < QMAX; q++) val[q][s] = id;
+  }
+#pragma unroll
+  for (int q = 0; q < QMAX; q++) cnt[q] = 0;
+  constexpr uint64_t step = (uint64_t)BLOCK * kFaRpt;
+  uint64_t s0 = base;
+  if (aligned)
+    for (; s0 + step <= end; s0 += step)
+      fa3_step<true, PT, SEL, W1, BLOCK, QMAX, C0, C1, C2, C3>(s0, end, pred, col1, selq, nq, val, cnt);
+  for (; s0 < end; s0 += step)
+    fa3_step<false, PT, SEL, W1, BLOCK, QMAX, C0, C1, C2, C3>(s0, end, pred, col1, selq, nq, val, cnt);
+#pragma unroll
+  for (int q = 0; q < QMAX; q++) {
+    if (q >= nq) break;
+#pragma unroll
+    for (int s = 0; s < kFaMaxSpecs; s++) {
+      if (s < ns) {
+        const ysql_acc tot = fa_block_reduce<BLOCK>(fa_val_to_acc(val[q][s], cnt[q], specs[s]), specs[s], s_w);
+        if (threadIdx.x == 0) partials[((uint64_t)tile * nq + q) * ns + s] = tot;
+      }
+    }
+  }
+}
+
+// filter_agg_tile for the two-column form; `code` from fa2_code per spec.
+template <typename PT, typename SEL, int W1, int BLOCK, int QMAX>
+__device__ __forceinline__ void filter_agg2_tile(uint32_t tile, uint64_t tile_rows, const PT *pred,
+                                                 const void *col1, const SEL &selq, int nq,
+                                                 const FaSpec *specs, const int (&code)[kFaMaxSpecs], int ns,
+                                                 uint64_t n, bool aligned, ysql_acc *partials, ysql_acc *s_w) {
+  const uint64_t base = (uint64_t)tile * tile_rows;
+  const uint64_t end = base + tile_rows < n ? base + tile_rows : n;
+  uint64_t val[QMAX][kFaMaxSpecs];
+  uint32_t cnt[QMAX];
+#pragma unroll
+  for (int s = 0; s < kFaMaxSpecs; s++) {
+    if (s < ns) {
+      const uint64_t id = fa_val_identity(specs[s]);
+#pragma unroll
+      for (int q = 0; q < QMAX; q++) val[q][s] = id;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < QMAX; q++) cnt[q] = 0;
+  constexpr uint64_t step = (uint64_t)BLOCK * kFaRpt;
+  uint64_t s0 = base;
+  if (aligned)
+    for (; s0 + step <= end; s0 += step)
+      fa2_step<true, PT, SEL, W1, BLOCK, QMAX>(s0, end, pred, col1, selq, nq, code, ns, val, cnt);
+  for (; s0 < end; s0 += step)
+    fa2_step<false, PT, SEL, W1, BLOCK, QMAX>(s0, end, pred, col1, selq, nq, code, ns, val, cnt);
+#pragma unroll
+  for (int q = 0; q < QMAX; q++) {
+    if (q >= nq) break;
+#pragma unroll
+    for (int s = 0; s < kFaMaxSpecs; s++) {
+      if (s < ns) {
+        const ysql_acc tot = fa_block_reduce<BLOCK>(fa_val_to_acc(val[q][s], cnt[q], specs[s]), specs[s], s_w);
+        if (threadIdx.x == 0) partials[((uint64_t)tile * nq + q) * ns + s] = tot;
+      }
+    }
+  }
+}
+
 // Kernel arguments of a filtered multi-aggregate launch.
 struct FaArgs {
   FaSpec spec[kFaMaxSpecs];
   ysql_acc *out[kFaMaxQueries * kFaMaxSpecs];  // [query][spec]
 };
 
-// Combines the ntiles partials of every (query, spec) in a fixed order; folds
-// the totals into out[q * ns + s] (accumulators that persist across calls),
-// or with `overwrite` stores them there.
+// Combines the ntiles partials of (query q, spec s) in a fixed order and
+// folds the total into out[q * ns + s] (accumulators that persist across
+// calls), or with `overwrite` stores it there.
+template <int BLOCK>
+__device__ __forceinline__ void filter_agg_finish_one(const FaSpec *specs, int ns, int nq, int q, int s,
+                                                      uint32_t ntiles, const ysql_acc *partials, ysql_acc *const *out,
+                                                      bool overwrite, ysql_acc *s_w) {
+  constexpr int B8 = 8;  // partials loaded together per thread (their combine order is unchanged)
+  const FaSpec sp = specs[s];
+  ysql_acc p;
+  fa_identity(p, sp);
+  for (uint32_t x0 = threadIdx.x; x0 < ntiles; x0 += BLOCK * B8) {
+    ysql_acc v[B8];
+#pragma unroll
+    for (int j = 0; j < B8; j++) {
+      const uint32_t x = x0 + (uint32_t)j * BLOCK;
+      if (x < ntiles) v[j] = partials[((uint64_t)x * nq + q) * ns + s];
+    }
+#pragma unroll
+    for (int j = 0; j < B8; j++)
+      if (x0 + (uint32_t)j * BLOCK < ntiles) fa_combine(p, v[j], sp);
+  }
+  const ysql_acc tot = fa_block_reduce<BLOCK>(p, sp, s_w);
+  if (threadIdx.x == 0) {
+    ysql_acc a;
+    if (overwrite) fa_identity(a, sp);
+    else a = *out[q * ns + s];
+    fa_combine(a, tot, sp);
+    *out[q * ns + s] = a;
+  }
+}
+
+// Every (query, spec) by one workgroup, one after another.
 template <int BLOCK>
 __device__ __forceinline__ void filter_agg_finish(const FaSpec *specs, int ns, int nq, uint32_t ntiles,
                                                   const ysql_acc *partials, ysql_acc *const *out, bool overwrite,
                                                   ysql_acc *s_w) {
-  for (int q = 0; q < nq; q++) {
-    for (int s = 0; s < ns; s++) {
-      const FaSpec sp = specs[s];
-      ysql_acc p;
-      fa_identity(p, sp);
-      for (uint32_t x = threadIdx.x; x < ntiles; x += BLOCK) {
-        const ysql_acc v = partials[((uint64_t)x * nq + q) * ns + s];
-        fa_combine(p, v, sp);
-      }
-      const ysql_acc tot = fa_block_reduce<BLOCK>(p, sp, s_w);
-      if (threadIdx.x == 0) {
-        ysql_acc a;
-        if (overwrite) fa_identity(a, sp);
-        else a = *out[q * ns + s];
-        fa_combine(a, tot, sp);
-        *out[q * ns + s] = a;
-      }
-    }
-  }
+  for (int q = 0; q < nq; q++)
+    for (int s = 0; s < ns; s++) filter_agg_finish_one<BLOCK>(specs, ns, nq, q, s, ntiles, partials, out, overwrite, s_w);
 }
 
 }  // namespace ysql

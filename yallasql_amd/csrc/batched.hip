@@ -116,9 +116,10 @@ __device__ __forceinline__ void seg_agg_job(ysql_acc *s_w, const SegAgg &j) {
   }
   const uint64_t tile_rows = fa_tile_rows(j.n);
   const uint32_t ntiles = (uint32_t)ceil_div(j.n, tile_rows);
-  const float sc[1] = {j.thr};
+  const FaSelCmp<float, YSQL_LT, 1> sel = {{j.thr}};
   for (uint32_t tile = 0; tile < ntiles; tile++)
-    filter_agg_tile<float, YSQL_LT, kFaBlock, 1>(tile, tile_rows, j.vals, sc, 1, sp, 3, j.n, j.partials, s_w);
+    filter_agg_tile<float, FaSelCmp<float, YSQL_LT, 1>, kFaBlock, 1>(tile, tile_rows, j.vals, sel, 1, sp, 3, j.n,
+                                                                     j.partials, s_w);
   __threadfence();
   __syncthreads();
   filter_agg_finish<kFaBlock>(sp, 3, 1, ntiles, j.partials, out, true, s_w);
@@ -406,9 +407,10 @@ __device__ __noinline__ void pagg_task(const PArgs &a, ysql_acc *s_w, const PJob
   const uint64_t tile_rows = fa_tile_rows(g.n);
   const uint32_t ntiles = (uint32_t)ceil_div((uint64_t)g.n, tile_rows);
   const uint32_t t0 = c * j.agg_tiles, t1 = t0 + j.agg_tiles < ntiles ? t0 + j.agg_tiles : ntiles;
-  const float sc[1] = {g.thr};
+  const FaSelCmp<float, YSQL_LT, 1> sel = {{g.thr}};
   for (uint32_t tile = t0; tile < t1; tile++)
-    filter_agg_tile<float, YSQL_LT, kFaBlock, 1>(tile, tile_rows, g.vals, sc, 1, sp, 3, g.n, g.partials, s_w);
+    filter_agg_tile<float, FaSelCmp<float, YSQL_LT, 1>, kFaBlock, 1>(tile, tile_rows, g.vals, sel, 1, sp, 3, g.n,
+                                                                     g.partials, s_w);
   wg_release();
   if (threadIdx.x == 0) *s_flag = atomicAdd(&a.entries[entry].done, 1u) == j.nchunks - 1 ? 1u : 0u;
   __syncthreads();

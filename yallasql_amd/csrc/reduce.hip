@@ -170,12 +170,15 @@ __device__ __forceinline__ void fold_tile(ysql_acc &p, const T *in, const uint8_
   }
 }
 
+// Tile partials of one aggregate; k_reduce_finish combines them.  The
+// combine runs as a second launch on the stream rather than in the last
+// workgroup to finish: a last-finisher hand-off needs an agent-scope release
+// per workgroup, which on this multi-XCD device writes back the L2 each time
+// (measured: 0.07 to 0.28 ms per 200M-row call, growing with the grid).
 template <typename T, int KIND, int OP, bool STAR>
 __global__ __launch_bounds__(kRedBlock) void k_reduce(const T *in, const uint8_t *nul, uint64_t n, int dtype,
-                                                      ysql_acc *acc, ysql_acc *partials, uint32_t *ticket,
-                                                      uint32_t ntiles) {
+                                                      ysql_acc *partials, uint32_t ntiles) {
   __shared__ ysql_acc s_w[kRedBlock / kWave];
-  __shared__ int s_last;
   for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     ysql_acc p;
     acc_identity<KIND, OP>(p, dtype);
@@ -185,11 +188,13 @@ __global__ __launch_bounds__(kRedBlock) void k_reduce(const T *in, const uint8_t
     ysql_acc tot = block_reduce_acc<KIND, OP>(p, s_w);
     if (threadIdx.x == 0) partials[tile] = tot;
   }
-  __threadfence();
-  if (threadIdx.x == 0) s_last = (atomicAdd(ticket, 1u) == gridDim.x - 1);
-  __syncthreads();
-  if (!s_last) return;
-  __threadfence();
+}
+
+// One workgroup: the tile partials in a fixed order, folded into *acc.
+template <int KIND, int OP>
+__global__ __launch_bounds__(kRedBlock) void k_reduce_finish(int dtype, ysql_acc *acc, const ysql_acc *partials,
+                                                             uint32_t ntiles) {
+  __shared__ ysql_acc s_w[kRedBlock / kWave];
   ysql_acc p;
   acc_identity<KIND, OP>(p, dtype);
   for (uint32_t x = threadIdx.x; x < ntiles; x += kRedBlock) {
@@ -201,11 +206,8 @@ __global__ __launch_bounds__(kRedBlock) void k_reduce(const T *in, const uint8_t
     ysql_acc a = *acc;
     acc_combine<KIND, OP>(a, tot);
     *acc = a;
-    *ticket = 0;
   }
 }
-
-__global__ void k_reduce_reset(uint32_t *ticket) { *ticket = 0; }
 
 __global__ void k_acc_init(ysql_acc *acc, int nacc, int dtype, int agg) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -286,39 +288,40 @@ __global__ void k_double_to_float(const double *in, float *out) { *out = (float)
 
 static size_t reduce_ws_size(uint64_t n) {
   WsCounter c;
-  c.take<uint32_t>(64);
   c.take<ysql_acc>(ceil_div(n, kRedTile) + 1);
   return c.off;
 }
 
 template <typename T, int KIND, int OP, bool STAR>
-static int reduce_launch(const void *in, const uint8_t *nul, uint64_t n, int dtype, ysql_acc *acc, uint32_t *ticket,
+static int reduce_launch(const void *in, const uint8_t *nul, uint64_t n, int dtype, ysql_acc *acc,
                          ysql_acc *partials, hipStream_t s) {
   const uint32_t ntiles = (uint32_t)ceil_div(n, kRedTile);
   uint32_t g = ntiles;
   const uint32_t cap = (uint32_t)grid_cap() * 4;
   if (g > cap) g = cap;
-  k_reduce<T, KIND, OP, STAR><<<g, kRedBlock, 0, s>>>((const T *)in, nul, n, dtype, acc, partials, ticket, ntiles);
+  k_reduce<T, KIND, OP, STAR><<<g, kRedBlock, 0, s>>>((const T *)in, nul, n, dtype, partials, ntiles);
+  YSQL_LAUNCH_CHECK();
+  k_reduce_finish<KIND, OP><<<1, kRedBlock, 0, s>>>(dtype, acc, partials, ntiles);
   YSQL_LAUNCH_CHECK();
   return YSQL_OK;
 }
 
 template <typename T>
 static int reduce_typed(int dtype, int agg, const void *in, const uint8_t *nul, uint64_t n, ysql_acc *acc,
-                        uint32_t *ticket, ysql_acc *partials, hipStream_t s) {
+                        ysql_acc *partials, hipStream_t s) {
   const int kind = acc_kind(dtype, agg), op = agg_op(agg);
   const bool star = agg == YSQL_AGG_COUNT_STAR;
-  if (star) return reduce_launch<T, KC, 0, true>(in, nullptr, n, dtype, acc, ticket, partials, s);
-  if (kind == KC) return reduce_launch<T, KC, 0, false>(in, nul, n, dtype, acc, ticket, partials, s);
+  if (star) return reduce_launch<T, KC, 0, true>(in, nullptr, n, dtype, acc, partials, s);
+  if (kind == KC) return reduce_launch<T, KC, 0, false>(in, nul, n, dtype, acc, partials, s);
   if (kind == KI) {
-    if (op == 0) return reduce_launch<T, KI, 0, false>(in, nul, n, dtype, acc, ticket, partials, s);
-    if (op == 1) return reduce_launch<T, KI, 1, false>(in, nul, n, dtype, acc, ticket, partials, s);
-    return reduce_launch<T, KI, 2, false>(in, nul, n, dtype, acc, ticket, partials, s);
+    if (op == 0) return reduce_launch<T, KI, 0, false>(in, nul, n, dtype, acc, partials, s);
+    if (op == 1) return reduce_launch<T, KI, 1, false>(in, nul, n, dtype, acc, partials, s);
+    return reduce_launch<T, KI, 2, false>(in, nul, n, dtype, acc, partials, s);
   }
   if constexpr (std::is_same<T, float>::value) {
-    if (kind == KD) return reduce_launch<T, KD, 0, false>(in, nul, n, dtype, acc, ticket, partials, s);
-    if (op == 1) return reduce_launch<T, KF, 1, false>(in, nul, n, dtype, acc, ticket, partials, s);
-    return reduce_launch<T, KF, 2, false>(in, nul, n, dtype, acc, ticket, partials, s);
+    if (kind == KD) return reduce_launch<T, KD, 0, false>(in, nul, n, dtype, acc, partials, s);
+    if (op == 1) return reduce_launch<T, KF, 1, false>(in, nul, n, dtype, acc, partials, s);
+    return reduce_launch<T, KF, 2, false>(in, nul, n, dtype, acc, partials, s);
   }
   return YSQL_EINVAL;
 }
@@ -737,20 +740,17 @@ int ysql_reduce(int dtype, int agg, const void *in, const uint8_t *null_opt, uin
   if (!in && agg != YSQL_AGG_COUNT_STAR && agg != YSQL_AGG_COUNT) return YSQL_EINVAL;
   if (n >= (1ull << 40)) return YSQL_EINVAL;
   WsCarver c(ws, ws_bytes);
-  uint32_t *ticket = c.take<uint32_t>(64);
   ysql_acc *partials = c.take<ysql_acc>(ceil_div(n, kRedTile) + 1);
-  if (!ws || !ticket || !partials) return YSQL_EWORKSPACE;
+  if (!ws || !partials) return YSQL_EWORKSPACE;
   hipStream_t s = as_stream(stream);
-  k_reduce_reset<<<1, 1, 0, s>>>(ticket);
-  YSQL_LAUNCH_CHECK();
   if (agg == YSQL_AGG_COUNT || agg == YSQL_AGG_COUNT_STAR) {
     // counts read only the null mask (reduction_count_notnull reads only isnull)
-    return reduce_typed<int32_t>(YSQL_INT, agg, nullptr, null_opt, n, acc_inout, ticket, partials, s);
+    return reduce_typed<int32_t>(YSQL_INT, agg, nullptr, null_opt, n, acc_inout, partials, s);
   }
   switch (dtype) {
-    case YSQL_INT: return reduce_typed<int32_t>(dtype, agg, in, null_opt, n, acc_inout, ticket, partials, s);
-    case YSQL_FLOAT: return reduce_typed<float>(dtype, agg, in, null_opt, n, acc_inout, ticket, partials, s);
-    case YSQL_DATETIME: return reduce_typed<int64_t>(dtype, agg, in, null_opt, n, acc_inout, ticket, partials, s);
+    case YSQL_INT: return reduce_typed<int32_t>(dtype, agg, in, null_opt, n, acc_inout, partials, s);
+    case YSQL_FLOAT: return reduce_typed<float>(dtype, agg, in, null_opt, n, acc_inout, partials, s);
+    case YSQL_DATETIME: return reduce_typed<int64_t>(dtype, agg, in, null_opt, n, acc_inout, partials, s);
   }
   return YSQL_EINVAL;
 }
@@ -874,25 +874,135 @@ int ysql_star_join_group_sum(const ysql_star_dim *dims, int ndims, const int32_t
 namespace ysql {
 
 // QMAX = 1 for single queries (fewer registers, higher occupancy), 4 for
-// shared scans; a query's fold sequence is the same in both.
-template <typename PT, int CMP, int QMAX>
-__global__ __launch_bounds__(kFaBlock) void k_filter_agg(const PT *pred, const PT *scalars, int nq, FaArgs a, int ns,
-                                                         uint64_t n, uint64_t tile_rows, ysql_acc *partials,
-                                                         uint32_t *ticket, uint32_t ntiles, int overwrite) {
+// shared scans; a query's fold sequence is the same in both.  The
+// comparison is the range test of fa2_range (exact for every operator).
+template <typename PT, int QMAX>
+__global__ __launch_bounds__(kFaBlock) void k_filter_agg(const PT *pred, const PT *scalars, int cmp, int nq, FaArgs a,
+                                                         int ns, uint64_t n, uint64_t tile_rows, ysql_acc *partials,
+                                                         uint32_t ntiles) {
   __shared__ ysql_acc s_w[kFaBlock / kWave];
-  __shared__ int s_last;
-  PT sc[QMAX];
-#pragma unroll
-  for (int q = 0; q < QMAX; q++) sc[q] = q < nq ? scalars[q] : PT();
+  const FaSelRange<PT, QMAX> sel = fa_sel_range<PT, QMAX>(cmp, scalars, nq);
   for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x)
-    filter_agg_tile<PT, CMP, kFaBlock, QMAX>(tile, tile_rows, pred, sc, nq, a.spec, ns, n, partials, s_w);
-  __threadfence();
-  if (threadIdx.x == 0) s_last = (atomicAdd(ticket, 1u) == gridDim.x - 1);
-  __syncthreads();
-  if (!s_last) return;
-  __threadfence();
-  filter_agg_finish<kFaBlock>(a.spec, ns, nq, ntiles, partials, a.out, overwrite != 0, s_w);
-  if (threadIdx.x == 0) *ticket = 0;
+    filter_agg_tile<PT, FaSelRange<PT, QMAX>, kFaBlock, QMAX>(tile, tile_rows, pred, sel, nq, a.spec, ns, n, partials,
+                                                             s_w);
+}
+
+template <typename K>
+static uint32_t fa_blocks_per_cu(K kernel) {
+  int b = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, kernel, kFaBlock, 0) != hipSuccess || b < 1) b = 2;
+  return (uint32_t)b;
+}
+
+// One workgroup per (query, spec): its tile partials combined in a fixed
+// order into its output (a second launch on the stream, for the reason
+// given at k_reduce).
+__global__ __launch_bounds__(kFaBlock) void k_fa_finish(FaArgs a, int ns, int nq, uint32_t ntiles,
+                                                        const ysql_acc *partials, int overwrite) {
+  __shared__ ysql_acc s_w[kFaBlock / kWave];
+  const int q = (int)blockIdx.x / ns, sp = (int)blockIdx.x % ns;
+  filter_agg_finish_one<kFaBlock>(a.spec, ns, nq, q, sp, ntiles, partials, a.out, overwrite != 0, s_w);
+}
+
+static int fa_finish_launch(const FaArgs &a, int ns, int nq, uint32_t ntiles, const ysql_acc *partials, int overwrite,
+                            hipStream_t s) {
+  k_fa_finish<<<(unsigned)(nq * ns), kFaBlock, 0, s>>>(a, ns, nq, ntiles, partials, overwrite);
+  YSQL_LAUNCH_CHECK();
+  return YSQL_OK;
+}
+
+// The two-column form (agg_dev.hpp, filter_agg2_tile): slot 1 holds W1-byte
+// elements (0: no column besides the predicate).
+struct Fa2Args {
+  int code[kFaMaxSpecs];
+  const void *col1;
+  int cmp;
+  int aligned;
+};
+
+template <typename PT, int W1, int QMAX>
+__global__ __launch_bounds__(kFaBlock) void k_filter_agg2(const PT *pred, const PT *scalars, int nq, FaArgs a,
+                                                          Fa2Args b, int ns, uint64_t n, uint64_t tile_rows,
+                                                          ysql_acc *partials, uint32_t ntiles) {
+  __shared__ ysql_acc s_w[kFaBlock / kWave];
+  const FaSelRange<PT, QMAX> sel = fa_sel_range<PT, QMAX>(b.cmp, scalars, nq);
+  for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x)
+    filter_agg2_tile<PT, FaSelRange<PT, QMAX>, W1, kFaBlock, QMAX>(tile, tile_rows, pred, b.col1, sel, nq, a.spec,
+                                                                  b.code, ns, n, b.aligned != 0, partials, s_w);
+}
+
+template <typename PT, int W1, int QMAX>
+static int filter_agg2_launch(const void *pred, const void *scalars, int nq, const FaArgs &a, const Fa2Args &b,
+                              int ns, uint64_t n, ysql_acc *partials, int overwrite,
+                              hipStream_t s) {
+  const uint64_t tile_rows = fa_tile_rows(n);
+  const uint32_t ntiles = (uint32_t)ceil_div(n, tile_rows);
+  static const uint32_t per_cu = fa_blocks_per_cu(k_filter_agg2<PT, W1, QMAX>);
+  const uint32_t g = std::min<uint32_t>(ntiles, per_cu * (uint32_t)grid_cap());
+  k_filter_agg2<PT, W1, QMAX><<<g, kFaBlock, 0, s>>>((const PT *)pred, (const PT *)scalars, nq, a, b, ns, n,
+                                                      tile_rows, partials, ntiles);
+  YSQL_LAUNCH_CHECK();
+  return fa_finish_launch(a, ns, nq, ntiles, partials, overwrite, s);
+}
+
+// Common shapes compiled whole (agg_dev.hpp, filter_agg3_tile): the spec
+// codes (fa2_code: opcode * 2 + slot) are template arguments.
+template <typename PT, int W1, int QMAX, int C0, int C1, int C2, int C3>
+__global__ __launch_bounds__(kFaBlock) void k_filter_agg3(const PT *pred, const PT *scalars, int nq, FaArgs a,
+                                                          Fa2Args b, int ns, uint64_t n, uint64_t tile_rows,
+                                                          ysql_acc *partials, uint32_t ntiles) {
+  __shared__ ysql_acc s_w[kFaBlock / kWave];
+  const FaSelRange<PT, QMAX> sel = fa_sel_range<PT, QMAX>(b.cmp, scalars, nq);
+  for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x)
+    filter_agg3_tile<PT, FaSelRange<PT, QMAX>, W1, kFaBlock, QMAX, C0, C1, C2, C3>(
+        tile, tile_rows, pred, b.col1, sel, nq, a.spec, ns, n, b.aligned != 0, partials, s_w);
+}
+
+template <typename PT, int W1, int QMAX, int C0, int C1, int C2, int C3>
+static int filter_agg3_launch(const void *pred, const void *scalars, int nq, const FaArgs &a, const Fa2Args &b,
+                              int ns, uint64_t n, ysql_acc *partials, int overwrite,
+                              hipStream_t s) {
+  const uint64_t tile_rows = fa_tile_rows(n);
+  const uint32_t ntiles = (uint32_t)ceil_div(n, tile_rows);
+  static const uint32_t per_cu = fa_blocks_per_cu(k_filter_agg3<PT, W1, QMAX, C0, C1, C2, C3>);
+  const uint32_t g = std::min<uint32_t>(ntiles, per_cu * (uint32_t)grid_cap());
+  k_filter_agg3<PT, W1, QMAX, C0, C1, C2, C3><<<g, kFaBlock, 0, s>>>(
+      (const PT *)pred, (const PT *)scalars, nq, a, b, ns, n, tile_rows, partials, ntiles);
+  YSQL_LAUNCH_CHECK();
+  return fa_finish_launch(a, ns, nq, ntiles, partials, overwrite, s);
+}
+
+// The shapes compiled whole: COUNT(*), SUM(x), MIN(y), MAX(y) with the
+// predicate a float column and y an int64 column (the summary of a filtered
+// range), and with the predicate an int64 key whose MIN/MAX are taken and x
+// an int32 column (the serving loop's filtered aggregate).  Returns
+// YSQL_EUNSUPPORTED for any other shape.
+static int filter_agg3_try(int pred_dtype, int w1, int nq, const void *pred, const void *scalars, const FaArgs &a,
+                           const Fa2Args &b, int ns, uint64_t n, ysql_acc *partials, int overwrite,
+                           hipStream_t s) {
+  if (ns != 4) return YSQL_EUNSUPPORTED;
+  const int *c = b.code;
+#define FA3_SHAPE(DT, PT, W, C0, C1, C2, C3)                                                                   \
+  if (pred_dtype == DT && w1 == W && c[0] == C0 && c[1] == C1 && c[2] == C2 && c[3] == C3)                   \
+    return nq == 1 ? filter_agg3_launch<PT, W, 1, C0, C1, C2, C3>(pred, scalars, nq, a, b, ns, n, partials, \
+                                                                 overwrite, s)                                     \
+                   : filter_agg3_launch<PT, W, kFaMaxQueries, C0, C1, C2, C3>(pred, scalars, nq, a, b, ns, n,        \
+                                                                             partials, overwrite, s);
+  // COUNT(*), SUM(float predicate), MIN(int64 y), MAX(int64 y)
+  FA3_SHAPE(YSQL_FLOAT, float, 8, 0, FA_OPC_D_SUM * 2, FA_OPC_I64_MIN * 2 + 1, FA_OPC_I64_MAX * 2 + 1)
+  // COUNT(*), SUM(int32 x), MIN(int64 predicate), MAX(int64 predicate)
+  FA3_SHAPE(YSQL_DATETIME, int64_t, 4, 0, FA_OPC_I32_SUM * 2 + 1, FA_OPC_I64_MIN * 2, FA_OPC_I64_MAX * 2)
+#undef FA3_SHAPE
+  return YSQL_EUNSUPPORTED;
+}
+
+template <typename PT, int W1>
+static int filter_agg2_dispatch(const void *pred, const void *scalars, int nq, const FaArgs &a, const Fa2Args &b,
+                                int ns, uint64_t n, ysql_acc *partials, int overwrite,
+                                hipStream_t s) {
+  return nq == 1 ? filter_agg2_launch<PT, W1, 1>(pred, scalars, nq, a, b, ns, n, partials, overwrite, s)
+                 : filter_agg2_launch<PT, W1, kFaMaxQueries>(pred, scalars, nq, a, b, ns, n, partials,
+                                                             overwrite, s);
 }
 
 // Identities into the outputs (an empty input with overwrite).
@@ -905,46 +1015,24 @@ __global__ void k_fa_identity(FaArgs a, int ns, int nq) {
   }
 }
 
-template <typename K>
-static uint32_t fa_blocks_per_cu(K kernel) {
-  int b = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, kernel, kFaBlock, 0) != hipSuccess || b < 1) b = 2;
-  return (uint32_t)b;
-}
 
 template <typename PT, int QMAX>
 static int filter_agg_launch(int cmp, const void *pred, const void *scalars, int nq, const FaArgs &a, int ns,
-                             uint64_t n, ysql_acc *partials, uint32_t *ticket, int overwrite, hipStream_t s) {
+                             uint64_t n, ysql_acc *partials, int overwrite, hipStream_t s) {
+  if (cmp < YSQL_LT || cmp > YSQL_NE) return YSQL_EINVAL;
   const uint64_t tile_rows = fa_tile_rows(n);
   const uint32_t ntiles = (uint32_t)ceil_div(n, tile_rows);
-  const PT *p = (const PT *)pred;
-  const PT *sc = (const PT *)scalars;
   // grid: the workgroups that fit at once (tiles are taken grid-stride)
-#define FA_CASE(C)                                                                                         \
-  case C: {                                                                                                \
-    static const uint32_t per_cu = fa_blocks_per_cu(k_filter_agg<PT, C, QMAX>);                           \
-    const uint32_t g = std::min<uint32_t>(ntiles, per_cu * (uint32_t)grid_cap());                          \
-    k_filter_agg<PT, C, QMAX><<<g, kFaBlock, 0, s>>>(p, sc, nq, a, ns, n, tile_rows, partials, ticket, ntiles,      \
-                                                     overwrite);                                               \
-    break;                                                                                                 \
-  }
-  switch (cmp) {
-    FA_CASE(YSQL_LT)
-    FA_CASE(YSQL_LE)
-    FA_CASE(YSQL_GT)
-    FA_CASE(YSQL_GE)
-    FA_CASE(YSQL_EQ)
-    FA_CASE(YSQL_NE)
-    default: return YSQL_EINVAL;
-  }
-#undef FA_CASE
+  static const uint32_t per_cu = fa_blocks_per_cu(k_filter_agg<PT, QMAX>);
+  const uint32_t g = std::min<uint32_t>(ntiles, per_cu * (uint32_t)grid_cap());
+  k_filter_agg<PT, QMAX><<<g, kFaBlock, 0, s>>>((const PT *)pred, (const PT *)scalars, cmp, nq, a, ns, n, tile_rows,
+                                                 partials, ntiles);
   YSQL_LAUNCH_CHECK();
-  return YSQL_OK;
+  return fa_finish_launch(a, ns, nq, ntiles, partials, overwrite, s);
 }
 
 static size_t filter_agg_ws_size(uint64_t n, int nq, int ns) {
   WsCounter c;
-  c.take<uint32_t>(64);
   c.take<ysql_acc>(ceil_div(n, fa_tile_rows(n)) * (uint64_t)nq * ns + 1);
   return c.off;
 }
@@ -957,7 +1045,7 @@ static int filter_agg_impl(int pred_dtype, int cmp, const void *pred_col, const 
   if (nspecs < 1 || nspecs > kFaMaxSpecs || nq < 1 || nq > kFaMaxQueries || !specs || cmp < YSQL_LT ||
       cmp > YSQL_NE)
     return YSQL_EINVAL;
-  if (flags & ~YSQL_FA_OVERWRITE) return YSQL_EINVAL;
+  if (flags & ~(YSQL_FA_OVERWRITE | YSQL_FA_GENERIC)) return YSQL_EINVAL;
   const int overwrite = (flags & YSQL_FA_OVERWRITE) ? 1 : 0;
   FaArgs a;
   memset(&a, 0, sizeof(a));
@@ -998,14 +1086,54 @@ static int filter_agg_impl(int pred_dtype, int cmp, const void *pred_col, const 
   if (!pred_col || !scalars) return YSQL_EINVAL;
   if (n >= (1ull << 40)) return YSQL_EINVAL;
   WsCarver c(ws, ws_bytes);
-  uint32_t *ticket = c.take<uint32_t>(64);
   ysql_acc *partials = c.take<ysql_acc>(ceil_div(n, fa_tile_rows(n)) * (uint64_t)nq * nspecs + 1);
-  if (!ws || !ticket || !partials) return YSQL_EWORKSPACE;
-  k_reduce_reset<<<1, 1, 0, s>>>(ticket);
-  YSQL_LAUNCH_CHECK();
+  if (!ws || !partials) return YSQL_EWORKSPACE;
+  // the two-column form, when every spec reads the predicate column, one
+  // other column or nothing
+  if (!(flags & YSQL_FA_GENERIC)) {
+    Fa2Args b;
+    memset(&b, 0, sizeof(b));
+    b.cmp = cmp;
+    int w1 = 0;
+    bool ok = true;
+    const int pb = dtype_bytes(pred_dtype);
+    for (int k = 0; k < nspecs && ok; k++) {
+      const FaSpec &x = a.spec[k];
+      const int opc = fa_opcode(x);
+      const int wb = fa_opc_wide(opc) ? 8 : 4;
+      if (opc == FA_OPC_NONE) {
+        b.code[k] = fa2_code(opc, 0);
+      } else if (x.src == FA_PRED && wb == pb) {
+        b.code[k] = fa2_code(opc, 0);
+      } else if (b.col1 == nullptr || (b.col1 == x.col && w1 == wb)) {
+        b.col1 = x.col;
+        w1 = wb;
+        b.code[k] = fa2_code(opc, 1);
+      } else {
+        ok = false;
+      }
+    }
+    if (ok) {
+      b.aligned = ((uintptr_t)pred_col & (2 * pb - 1)) == 0 && (w1 == 0 || ((uintptr_t)b.col1 & (2 * w1 - 1)) == 0);
+      const int rc3 = filter_agg3_try(pred_dtype, w1, nq, pred_col, scalars, a, b, nspecs, n, partials,
+                                      overwrite, s);
+      if (rc3 != YSQL_EUNSUPPORTED) return rc3;
+#define FA2_DT(PT)                                                                                                 \
+  return w1 == 8   ? filter_agg2_dispatch<PT, 8>(pred_col, scalars, nq, a, b, nspecs, n, partials, overwrite, s) \
+         : w1 == 4 ? filter_agg2_dispatch<PT, 4>(pred_col, scalars, nq, a, b, nspecs, n, partials, overwrite, s) \
+                   : filter_agg2_dispatch<PT, 0>(pred_col, scalars, nq, a, b, nspecs, n, partials, overwrite, s);
+      switch (pred_dtype) {
+        case YSQL_INT: FA2_DT(int32_t)
+        case YSQL_FLOAT: FA2_DT(float)
+        case YSQL_DATETIME: FA2_DT(int64_t)
+      }
+#undef FA2_DT
+      return YSQL_EINVAL;
+    }
+  }
 #define FA_DT(PT)                                                                                              \
-  return nq == 1 ? filter_agg_launch<PT, 1>(cmp, pred_col, scalars, nq, a, nspecs, n, partials, ticket, overwrite, s) \
-                 : filter_agg_launch<PT, kFaMaxQueries>(cmp, pred_col, scalars, nq, a, nspecs, n, partials, ticket,    \
+  return nq == 1 ? filter_agg_launch<PT, 1>(cmp, pred_col, scalars, nq, a, nspecs, n, partials, overwrite, s) \
+                 : filter_agg_launch<PT, kFaMaxQueries>(cmp, pred_col, scalars, nq, a, nspecs, n, partials,    \
                                                         overwrite, s);
   switch (pred_dtype) {
     case YSQL_INT: FA_DT(int32_t)

@@ -400,10 +400,11 @@ def reduce(dtype_c, agg, x, acc, null=None, n=None):
     return acc
 
 
-def filter_aggregate(pred, cmp, scalar, specs, overwrite=False, n=None):
+def filter_aggregate(pred, cmp, scalar, specs, overwrite=False, n=None, generic=False):
     """Fused WHERE pred <cmp> scalar + aggregates.  specs: list of
     (dtype_c, agg, column tensor or None, acc tensor).  With overwrite the
-    accumulators receive this call's result; otherwise it is folded in."""
+    accumulators receive this call's result; otherwise it is folded in.
+    generic: run the general tile kernel (same bits as the two-column one)."""
     n = pred.numel() if n is None else n
     dev = pred.device
     sc = torch.tensor([scalar], dtype=pred.dtype, device=dev)
@@ -411,12 +412,13 @@ def filter_aggregate(pred, cmp, scalar, specs, overwrite=False, n=None):
     for k, (dc, agg, col, acc) in enumerate(specs):
         arr[k] = L.AggSpec(dc, agg, _p(col) if col is not None else None, _p(acc))
     ws = workspace(ws_bytes("ysql_filter_aggregate_ws_bytes", n, len(specs)), dev)
-    call("ysql_filter_aggregate_ex", _DT[pred.dtype], cmp, _p(pred), _p(sc), arr, len(specs), n,
-         L.FA_OVERWRITE if overwrite else 0, _p(ws), ws.numel(), _s())
+    flags = (L.FA_OVERWRITE if overwrite else 0) | (L.FA_GENERIC if generic else 0)
+    call("ysql_filter_aggregate_ex", _DT[pred.dtype], cmp, _p(pred), _p(sc), arr, len(specs), n, flags, _p(ws),
+         ws.numel(), _s())
     return [acc for (_, _, _, acc) in specs]
 
 
-def filter_aggregate_multi(pred, cmp, scalars, columns, accs, overwrite=True, n=None):
+def filter_aggregate_multi(pred, cmp, scalars, columns, accs, overwrite=True, n=None, generic=False):
     """Shared scan: query q is WHERE pred <cmp> scalars[q] with aggregates
     columns[s] = (dtype_c, agg, column or None) into accs[q][s]."""
     n = pred.numel() if n is None else n
@@ -428,8 +430,9 @@ def filter_aggregate_multi(pred, cmp, scalars, columns, accs, overwrite=True, n=
         for k, (dc, agg, col) in enumerate(columns):
             arr[q * ns + k] = L.AggSpec(dc, agg, _p(col) if col is not None else None, _p(accs[q][k]))
     ws = workspace(ws_bytes("ysql_filter_aggregate_multi_ws_bytes", n, nq, ns), dev)
-    call("ysql_filter_aggregate_multi", _DT[pred.dtype], cmp, _p(pred), _p(sc), nq, arr, ns, n,
-         L.FA_OVERWRITE if overwrite else 0, _p(ws), ws.numel(), _s())
+    flags = (L.FA_OVERWRITE if overwrite else 0) | (L.FA_GENERIC if generic else 0)
+    call("ysql_filter_aggregate_multi", _DT[pred.dtype], cmp, _p(pred), _p(sc), nq, arr, ns, n, flags, _p(ws),
+         ws.numel(), _s())
     return accs
 
 
