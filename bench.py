@@ -50,6 +50,8 @@ def parse():
     ap.add_argument("--no-stream", action="store_true", help="skip config 2 streamed from pinned host memory")
     ap.add_argument("--serving-queries", type=int, default=10_000, help="queries per rate, at most")
     ap.add_argument("--serving-seconds", type=float, default=60.0, help="arrival window per rate, at most")
+    ap.add_argument("--config1-only", action="store_true",
+                    help="run only the config-1 engine query (for a kernel trace of that query alone)")
     ap.add_argument("--pairwise-merge", action="store_true",
                     help="merge the runs pairwise, level by level, instead of one k-way merge")
     return ap.parse_args()
@@ -705,6 +707,9 @@ def main():
         sys.exit(2)
     import torch
 
+    if args.config1_only:
+        print(json.dumps({"config1_filter_sort_project": config1_line()}), flush=True)
+        return
     dev = torch.device("cuda:0")
     keys = gen_keys(args.rows, dev)
     if args.calib_copy:

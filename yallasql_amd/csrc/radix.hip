@@ -50,8 +50,9 @@ __global__ void k_copy_pairs(const K *ks, const uint32_t *vs, K *kd, uint32_t *v
   }
 }
 
-// One digit pass; tiles are taken in launch order from *ctr.
-template <typename K, int BLOCK, int KPT, bool EARLY>
+// One digit pass; tiles are taken in launch order from *ctr.  The digit
+// counts are read and the wave counters cleared while the ticket is taken.
+template <typename K, int BLOCK, int KPT>
 __global__ __launch_bounds__(BLOCK) void k_onesweep(const K *__restrict__ kin, const uint32_t *__restrict__ vin,
                                                     uint32_t val_base, K *__restrict__ kout,
                                                     uint32_t *__restrict__ vout, uint32_t n, int shift, uint32_t desc,
@@ -59,10 +60,12 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const K *__restrict__ kin, c
                                                     uint32_t *status_next, uint32_t *ctr) {
   __shared__ OnesweepSmem<K, BLOCK, KPT> sm;
   __shared__ uint32_t s_tile;
-  if (threadIdx.x == 0) s_tile = atomicAdd(ctr, 1u);
+  const int t = threadIdx.x;
+  const uint32_t h = t < kBins ? hist[t] : 0u;
+  for (int i = t; i < (BLOCK / kWave) * kBins; i += BLOCK) (&sm.wh[0][0])[i] = 0;
+  if (t == 0) s_tile = atomicAdd(ctr, 1u);
   __syncthreads();
-  onesweep_tile<K, BLOCK, KPT, EARLY>(sm, s_tile, kin, vin, val_base, kout, vout, n, shift, desc, hist, status,
-                                      status_next);
+  onesweep_tile<K, BLOCK, KPT>(sm, s_tile, kin, vin, val_base, kout, vout, n, shift, desc, h, status, status_next);
 }
 
 // ---------------------------------------------------------------------------
@@ -70,7 +73,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const K *__restrict__ kin, c
 // Host driver
 // ---------------------------------------------------------------------------
 // Tile shapes of the digit pass (threads x keys per thread).  Variant 0
-// (8192-key tiles, look-back after the LDS reorder) measured best on 2^25
+// (8192-key tiles) measured best on 2^25
 // keys; the smaller tiles give mid-size sorts more workgroups.  A sort uses
 // one variant for all its passes (the status areas are laid out per tile).
 constexpr int kRadixVariants = 5;
@@ -188,7 +191,7 @@ static void radix_pass_v(int desc, int p, int passes, const K *kin, const uint32
   const uint32_t ntiles = (uint32_t)ceil_div(n, TILE);
   uint32_t *st = (p & 1) ? w.st1 : w.st0;
   uint32_t *snext = (p + 1 < passes) ? ((p & 1) ? w.st0 : w.st1) : nullptr;
-  k_onesweep<K, BLOCK, KPT, false><<<ntiles, BLOCK, 0, s>>>(kin, vin, val_base, kout, vout, (uint32_t)n,
+  k_onesweep<K, BLOCK, KPT><<<ntiles, BLOCK, 0, s>>>(kin, vin, val_base, kout, vout, (uint32_t)n,
                                                             begin_bit + 8 * p, desc ? 0xFFu : 0u, w.hist + p * kBins,
                                                             st, snext, w.ctr + p);
 }

@@ -189,55 +189,81 @@ __device__ __forceinline__ void lds_exchange(uint32_t *sk, const K (&key)[KPT], 
   }
 }
 
+// Makes v opaque to the optimiser at this point (no instruction): the value
+// stays in one register instead of being re-derived later from operands
+// that would then have to stay live.
+template <typename T>
+__device__ __forceinline__ void pin_vgpr(T &v) {
+  __asm__ volatile("" : "+v"(v));
+}
+
 // One digit pass over tile `tile` of a sequence of n keys.
 //   kin/vin -> kout/vout; vin == nullptr means the payload is val_base + row.
-//   hist: the 256 counts of this digit over all n keys.
+//   h: for thread t < 256, the count of digit t over all n keys (read by the
+//   caller before the tile is known).  The caller zeroes sm.wh and
+//   synchronises before the call.
 //   status: ntiles * 256 words of this pass (zeroed before the pass);
 //   status_next: the same area for the following pass, zeroed here tile by
 //   tile (it was last read two passes ago, which has completed).
+// Register budget: a thread holds its KPT keys (as 32-bit halves), their
+// packed digits and one rank/position word each; the payloads are loaded
+// only once the keys' low halves are staged, so they are in flight during
+// the look-back instead of occupying registers through the ranking.  Keys
+// are written as they are read back from the stage.
 // Every thread of the block calls it; contains barriers.
-template <typename K, int BLOCK, int KPT, bool EARLY>
+template <typename K, int BLOCK, int KPT>
 __device__ __forceinline__ void onesweep_tile(OnesweepSmem<K, BLOCK, KPT> &sm, uint32_t tile,
                                               const K *__restrict__ kin, const uint32_t *__restrict__ vin,
                                               uint32_t val_base, K *__restrict__ kout, uint32_t *__restrict__ vout,
-                                              uint32_t n, int shift, uint32_t desc,
-                                              const uint32_t *__restrict__ hist, uint32_t *status,
+                                              uint32_t n, int shift, uint32_t desc, uint32_t h, uint32_t *status,
                                               uint32_t *status_next) {
   static_assert(BLOCK >= kBins, "one thread per digit");
+  static_assert(KPT % 4 == 0, "digits are packed four per word");
+  constexpr bool WIDE = sizeof(K) == 8;
   constexpr int W = BLOCK / kWave;
   constexpr int TILE = BLOCK * KPT;
   const int t = threadIdx.x;
   const int lane = lane_id();
   const int wave = t / kWave;
-  for (int i = t; i < W * kBins; i += BLOCK) (&sm.wh[0][0])[i] = 0;
-  __syncthreads();
   const uint32_t base = tile * (uint32_t)TILE;
   if (status_next && t < kBins) status_next[tile * kBins + t] = 0;
 
   // ---- load: wave w owns rows [base + w*KPT*64, ...), item i lane l -> row
-  K key[KPT];
-  uint32_t val[KPT];
   const uint32_t wbase = base + (uint32_t)wave * KPT * kWave;
+  uint32_t klo[KPT], khi[KPT];
+  uint32_t dg[KPT / 4];  // the items' digits, four per word
+  {
+    K key[KPT];
 #pragma unroll
-  for (int i = 0; i < KPT; i++) {
-    const uint32_t idx = wbase + i * kWave + lane;
-    const bool valid = idx < n;
-    key[i] = valid ? kin[idx] : (K)0;
-    val[i] = vin ? (valid ? vin[idx] : 0u) : val_base + idx;
+    for (int i = 0; i < KPT; i++) {
+      const uint32_t idx = wbase + i * kWave + lane;
+      key[i] = idx < n ? kin[idx] : (K)0;
+    }
+#pragma unroll
+    for (int i = 0; i < KPT / 4; i++) dg[i] = 0;
+#pragma unroll
+    for (int i = 0; i < KPT; i++) {
+      dg[i / 4] |= ((((uint32_t)(key[i] >> shift)) & 0xFFu) ^ desc) << (8 * (i % 4));
+      klo[i] = (uint32_t)key[i];
+      khi[i] = WIDE ? (uint32_t)((uint64_t)key[i] >> 32) : 0u;
+      pin_vgpr(klo[i]);
+      if constexpr (WIDE) pin_vgpr(khi[i]);
+    }
   }
 
   // ---- rank inside the wave, in (item, lane) order
-  uint32_t rank[KPT];
+  uint32_t pos[KPT];  // the rank inside the wave, then the position in the tile
 #pragma unroll
   for (int i = 0; i < KPT; i++) {
     const uint32_t idx = wbase + i * kWave + lane;
     const bool valid = idx < n;
-    const uint32_t d = ((uint32_t)(key[i] >> shift) & 0xFFu) ^ desc;
+    const uint32_t d = (dg[i / 4] >> (8 * (i % 4))) & 0xFFu;
     const uint64_t m = match_digit8(d, valid);
     const uint32_t lower = popc_below(m);
     uint32_t pre = 0;
     if (valid) pre = sm.wh[wave][d];
-    rank[i] = pre + lower;
+    pos[i] = pre + lower;
+    pin_vgpr(pos[i]);
     if (valid && lower == 0) sm.wh[wave][d] = pre + (uint32_t)__popcll(m);
   }
   __syncthreads();
@@ -255,29 +281,59 @@ __device__ __forceinline__ void onesweep_tile(OnesweepSmem<K, BLOCK, KPT> &sm, u
     cnt = run;
     rstatus_store(&status[tile * kBins + t], (tile == 0 ? kRFlagInc : kRFlagAgg) | cnt);
   }
-  // Decoupled look-back, one thread per digit, kLookbackBatch predecessors
-  // per step (their status loads are in flight together).  A step counts
-  // only when every predecessor up to the first inclusive one in it has
-  // published; otherwise it is read again.  EARLY walks back right after
-  // publishing the aggregate; the default walks back after the LDS reorder
-  // (measured faster on MI355X).
-  uint32_t excl = 0;
-  auto lookback = [&]() {
-    if (t < kBins && tile > 0) {
+  uint32_t tot;
+  const uint32_t dstart = block_exclusive_sum<BLOCK, uint32_t>(t < kBins ? cnt : 0u, sm.red, &tot);
+  const uint32_t gstart = block_exclusive_sum<BLOCK, uint32_t>(t < kBins ? h : 0u, sm.red, &tot);
+  if (t < kBins) sm.dstart[t] = dstart;
+  __syncthreads();
+
+  // ---- positions in the tile's digit order; low halves into the stage
+#pragma unroll
+  for (int i = 0; i < KPT; i++) {
+    const uint32_t idx = wbase + i * kWave + lane;
+    if (idx < n) {
+      uint32_t w4 = dg[i / 4];
+      pin_vgpr(w4);  // re-derive the LDS addresses rather than keep them from the ranking
+      const uint32_t d = (w4 >> (8 * (i % 4))) & 0xFFu;
+      pos[i] += sm.dstart[d] + sm.wh[wave][d];
+    } else {
+      pos[i] = 0xFFFFFFFFu;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < KPT; i++)
+    if (pos[i] != 0xFFFFFFFFu) sm.k[pos[i]] = klo[i];
+  // payloads: in flight during the look-back
+  uint32_t val[KPT];
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int i = 0; i < KPT; i++) {
+    const uint32_t idx = wbase + i * kWave + lane;
+    val[i] = vin ? (idx < n ? vin[idx] : 0u) : val_base + idx;
+  }
+  __builtin_amdgcn_sched_barrier(0);
+
+  // ---- decoupled look-back, one thread per digit, kLookbackBatch
+  // predecessors per step (their status loads are in flight together).  A
+  // step counts only when every predecessor up to the first inclusive one in
+  // it has published; otherwise it is read again.
+  if (t < kBins) {
+    uint32_t excl = 0;
+    if (tile > 0) {
       int64_t k = (int64_t)tile - 1;
       while (true) {
-        uint32_t s[kLookbackBatch];
+        uint32_t st[kLookbackBatch];
 #pragma unroll
         for (int b = 0; b < kLookbackBatch; b++)  // tile 0 is inclusive, so k - b < 0 is never needed
-          s[b] = k - b >= 0 ? rstatus_load(&status[(uint32_t)(k - b) * kBins + t]) : kRFlagInc;
+          st[b] = k - b >= 0 ? rstatus_load(&status[(uint32_t)(k - b) * kBins + t]) : kRFlagInc;
         bool ready = true, inc = false;
         uint32_t add = 0;
 #pragma unroll
         for (int b = 0; b < kLookbackBatch; b++) {
           if (!inc) {
-            const uint32_t f = s[b] & ~kRValMask;
+            const uint32_t f = st[b] & ~kRValMask;
             ready = ready && f != 0;
-            add += s[b] & kRValMask;
+            add += st[b] & kRValMask;
             inc = f == kRFlagInc;
           }
         }
@@ -288,45 +344,48 @@ __device__ __forceinline__ void onesweep_tile(OnesweepSmem<K, BLOCK, KPT> &sm, u
       }
       rstatus_store(&status[tile * kBins + t], kRFlagInc | (excl + cnt));
     }
-  };
-  if constexpr (EARLY) lookback();
-  uint32_t tot;
-  const uint32_t dstart = block_exclusive_sum<BLOCK, uint32_t>(t < kBins ? cnt : 0u, sm.red, &tot);
-  const uint32_t h = t < kBins ? hist[t] : 0u;
-  const uint32_t gstart = block_exclusive_sum<BLOCK, uint32_t>(h, sm.red, &tot);
-  if (t < kBins) sm.dstart[t] = dstart;
+    sm.glob[t] = gstart + excl - dstart;
+  }
   __syncthreads();
 
-  // ---- reorder the tile's keys in LDS by digit (the look-back runs while
-  // the first LDS stores drain), then write each digit's run of keys to its
-  // global position
-  uint32_t pos[KPT];
-#pragma unroll
-  for (int i = 0; i < KPT; i++) {
-    const uint32_t idx = wbase + i * kWave + lane;
-    pos[i] = 0xFFFFFFFFu;
-    if (idx < n) {
-      const uint32_t d = ((uint32_t)(key[i] >> shift) & 0xFFu) ^ desc;
-      pos[i] = sm.dstart[d] + sm.wh[wave][d] + rank[i];
-    }
-  }
+  // ---- keys back from the stage in digit order, each written to its
+  // global position as soon as it is whole
   const uint32_t valid_cnt = (n - base) < (uint32_t)TILE ? (n - base) : (uint32_t)TILE;
-  K okey[KPT];
-  lds_exchange<K, KPT>(
-      sm.k, key, pos, okey, [&](int q) { return (uint32_t)(t + q * BLOCK); }, valid_cnt,
-      [&]() {
-        if constexpr (!EARLY) lookback();
-        if (t < kBins) sm.glob[t] = gstart + excl - dstart;
-      });
   uint32_t opos[KPT];
+  if constexpr (WIDE) {
+    uint32_t lo[KPT];
 #pragma unroll
-  for (int q = 0; q < KPT; q++) {
-    const uint32_t j = t + q * BLOCK;
-    opos[q] = 0;
-    if (j < valid_cnt) {
-      const uint32_t d = ((uint32_t)(okey[q] >> shift) & 0xFFu) ^ desc;
-      opos[q] = sm.glob[d] + j;
-      kout[opos[q]] = okey[q];
+    for (int q = 0; q < KPT; q++) {
+      const uint32_t j = t + q * BLOCK;
+      lo[q] = j < valid_cnt ? sm.k[j] : 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < KPT; i++)
+      if (pos[i] != 0xFFFFFFFFu) sm.k[pos[i]] = khi[i];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < KPT; q++) {
+      const uint32_t j = t + q * BLOCK;
+      opos[q] = 0;
+      if (j < valid_cnt) {
+        const K key = (K)(((uint64_t)sm.k[j] << 32) | lo[q]);
+        const uint32_t d = ((uint32_t)(key >> shift) & 0xFFu) ^ desc;
+        opos[q] = sm.glob[d] + j;
+        kout[opos[q]] = key;
+      }
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < KPT; q++) {
+      const uint32_t j = t + q * BLOCK;
+      opos[q] = 0;
+      if (j < valid_cnt) {
+        const K key = (K)sm.k[j];
+        const uint32_t d = ((uint32_t)(key >> shift) & 0xFFu) ^ desc;
+        opos[q] = sm.glob[d] + j;
+        kout[opos[q]] = key;
+      }
     }
   }
   __syncthreads();
@@ -340,7 +399,6 @@ __device__ __forceinline__ void onesweep_tile(OnesweepSmem<K, BLOCK, KPT> &sm, u
     const uint32_t j = t + q * BLOCK;
     if (j < valid_cnt) vout[opos[q]] = sm.k[j];
   }
-  __syncthreads();  // LDS is reused by the caller's next tile
 }
 
 // Whole sort of n <= BLOCK * KPT keys in one workgroup: every digit pass
