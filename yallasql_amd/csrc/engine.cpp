@@ -2,6 +2,9 @@
 #include "engine.hpp"
 
 #include <algorithm>
+#include <climits>
+#include <map>
+#include <mutex>
 #include <cstdio>
 #include <cstring>
 
@@ -14,8 +17,80 @@ void abiCheck(int rc, const char *what) {
   if (rc != YSQL_OK) throw EngineError(std::string(what) + " returned " + std::to_string(rc) + " (" + ysql_last_error() + ")", rc);
 }
 
+// ---------------------------------------------------------------- memory
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_pool_mu;
+hipMemPool_t g_pool[kMaxDevices] = {};
+int g_live_contexts = 0;
+
+std::mutex g_pin_mu;
+constexpr size_t kPinnedCacheBytes = size_t(512) << 20;
+constexpr size_t kPinnedGrain = size_t(1) << 20;
+// Free pinned blocks by size.  Never destroyed: blocks still cached when the
+// process ends are released with it (a static destructor could run after
+// the HIP runtime has shut down).
+std::multimap<size_t, void *> &pinFree() {
+  static auto *m = new std::multimap<size_t, void *>();
+  return *m;
+}
+size_t g_pin_free_bytes = 0;
+}  // namespace
+
+hipMemPool_t enginePool() {
+  int dev = 0;
+  hipCheck(hipGetDevice(&dev), "hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices) throw EngineError("device index out of range");
+  std::lock_guard<std::mutex> g(g_pool_mu);
+  if (!g_pool[dev]) {
+    hipMemPoolProps props = {};
+    props.allocType = hipMemAllocationTypePinned;
+    props.handleTypes = hipMemHandleTypeNone;
+    props.location.type = hipMemLocationTypeDevice;
+    props.location.id = dev;
+    hipMemPool_t pool = nullptr;
+    hipCheck(hipMemPoolCreate(&pool, &props), "hipMemPoolCreate");
+    uint64_t keep = UINT64_MAX;
+    hipCheck(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep), "hipMemPoolSetAttribute");
+    g_pool[dev] = pool;
+  }
+  return g_pool[dev];
+}
+
+PinnedBlock pinnedTake(size_t bytes) {
+  bytes = (bytes + kPinnedGrain - 1) / kPinnedGrain * kPinnedGrain;
+  {
+    std::lock_guard<std::mutex> g(g_pin_mu);
+    auto &m = pinFree();
+    auto it = m.lower_bound(bytes);
+    if (it != m.end() && it->first <= 2 * bytes) {
+      PinnedBlock b{it->second, it->first};
+      g_pin_free_bytes -= it->first;
+      m.erase(it);
+      return b;
+    }
+  }
+  PinnedBlock b;
+  hipCheck(hipHostMalloc(&b.ptr, bytes, hipHostMallocDefault), "hipHostMalloc");
+  b.bytes = bytes;
+  return b;
+}
+
+void pinnedGive(PinnedBlock b) {
+  if (!b.ptr) return;
+  {
+    std::lock_guard<std::mutex> g(g_pin_mu);
+    if (g_pin_free_bytes + b.bytes <= kPinnedCacheBytes) {
+      pinFree().emplace(b.bytes, b.ptr);
+      g_pin_free_bytes += b.bytes;
+      return;
+    }
+  }
+  (void)hipHostFree(b.ptr);
+}
+
 DeviceBuffer::DeviceBuffer(size_t n, hipStream_t s) : bytes(n), stream(s) {
-  hipCheck(hipMallocAsync(&ptr, n ? n : 1, s), "hipMallocAsync");
+  hipCheck(hipMallocFromPoolAsync(&ptr, n ? n : 1, enginePool(), s), "hipMallocFromPoolAsync");
 }
 DeviceBuffer::~DeviceBuffer() {
   if (owned && ptr) (void)hipFreeAsync(ptr, stream);
@@ -306,6 +381,9 @@ ExecContext::ExecContext(size_t budget, hipStream_t stream) : cache_(budget) {
   cache_.setComputeStream(stream_);
   hipCheck(hipMalloc(&dcount_, 64), "hipMalloc");
   hipCheck(hipHostMalloc(&hcount_, 64, hipHostMallocDefault), "hipHostMalloc");
+  (void)enginePool();
+  std::lock_guard<std::mutex> g(g_pool_mu);
+  g_live_contexts++;
 }
 ExecContext::~ExecContext() {
   (void)hipStreamSynchronize(stream_);
@@ -316,12 +394,16 @@ ExecContext::~ExecContext() {
   (void)hipFree(dcount_);
   (void)hipHostFree(hcount_);
   if (own_stream_) (void)hipStreamDestroy(stream_);
+  std::lock_guard<std::mutex> g(g_pool_mu);
+  if (--g_live_contexts == 0)  // the last context: hand the pools' cached memory back
+    for (hipMemPool_t p : g_pool)
+      if (p) (void)hipMemPoolTrimTo(p, 0);
 }
 void *ExecContext::workspace(size_t bytes) {
   if (bytes > ws_bytes_) {
     if (ws_) hipCheck(hipFreeAsync(ws_, stream_), "hipFreeAsync");
     ws_bytes_ = std::max(bytes, ws_bytes_ + ws_bytes_ / 2) + 256;
-    hipCheck(hipMallocAsync(&ws_, ws_bytes_, stream_), "hipMallocAsync");
+    hipCheck(hipMallocFromPoolAsync(&ws_, ws_bytes_, enginePool(), stream_), "hipMallocFromPoolAsync");
   }
   return ws_;
 }
@@ -1389,35 +1471,95 @@ BatchID CrossProductOperator::next(CacheManager &cache) {
 }
 
 // ---------------------------------------------------------------- execution
+// Layout of a result block of n rows: every column's data, then every
+// column's null bytes, each at a 256-byte boundary.  Returns the block size.
+static size_t resultLayout(const std::vector<ColumnPtr> &cols, size_t n, std::vector<size_t> &doff,
+                           std::vector<size_t> &noff) {
+  const size_t nc = cols.size();
+  doff.resize(nc);
+  noff.resize(nc);
+  size_t off = 0;
+  for (size_t c = 0; c < nc; c++) {
+    doff[c] = off;
+    off = (off + n * cols[c]->bytes + 255) / 256 * 256;
+  }
+  for (size_t c = 0; c < nc; c++) {
+    noff[c] = off;
+    off = (off + n + 255) / 256 * 256;
+  }
+  return off;
+}
+
 HostResult executeToHost(Operator &root, ExecContext &ctx) {
   HostResult r;
   r.columns = root.columns;
-  r.data.resize(r.columns.size());
-  r.nulls.resize(r.columns.size());
+  const size_t nc = r.columns.size();
   root.init();
   CacheManager &cache = ctx.cache();
-  while (true) {
-    const BatchID id = root.next(cache);
-    if (!id) break;
-    auto b = cache.getBatch(id);
-    const size_t n = b->batchSize;
-    for (size_t c = 0; c < r.columns.size(); c++) {
-      const size_t eb = r.columns[c]->bytes;
-      auto &d = r.data[c];
-      auto &nl = r.nulls[c];
-      d.resize((r.rows + n) * eb);
-      nl.resize(r.rows + n, 0);
-      if (n) {
-        hipCheck(hipMemcpyAsync(d.data() + r.rows * eb, b->col(c), n * eb, hipMemcpyDeviceToHost, ctx.stream()),
+  struct Part {
+    PinnedBlock blk;
+    size_t rows;
+  };
+  std::vector<Part> parts;
+  std::vector<size_t> doff, noff;
+  try {
+    while (true) {
+      const BatchID id = root.next(cache);
+      if (!id) break;
+      auto b = cache.getBatch(id);
+      const size_t n = b->batchSize;
+      r.batches++;
+      if (!n) continue;
+      // each batch's columns go straight into pinned memory, all copies queued
+      // before one wait at the end (a batch's device buffers are freed in
+      // stream order, after its copies)
+      Part p{pinnedTake(resultLayout(r.columns, n, doff, noff)), n};
+      parts.push_back(p);
+      char *base = static_cast<char *>(p.blk.ptr);
+      for (size_t c = 0; c < nc; c++) {
+        hipCheck(hipMemcpyAsync(base + doff[c], b->col(c), n * r.columns[c]->bytes, hipMemcpyDeviceToHost,
+                                ctx.stream()),
                  "hipMemcpyAsync");
         if (b->nul(c))
-          hipCheck(hipMemcpyAsync(nl.data() + r.rows, b->nul(c), n, hipMemcpyDeviceToHost, ctx.stream()),
+          hipCheck(hipMemcpyAsync(base + noff[c], b->nul(c), n, hipMemcpyDeviceToHost, ctx.stream()),
                    "hipMemcpyAsync");
+        else
+          std::memset(base + noff[c], 0, n);
       }
+      r.rows += n;
     }
     ctx.sync();
-    r.rows += n;
-    r.batches++;
+  } catch (...) {
+    (void)hipStreamSynchronize(ctx.stream());
+    for (auto &p : parts) pinnedGive(p.blk);
+    throw;
+  }
+  r.data.assign(nc, nullptr);
+  r.nulls.assign(nc, nullptr);
+  if (parts.empty()) return r;
+  if (parts.size() == 1) {
+    r.block = parts[0].blk;
+  } else {  // several batches: one block in row order
+    r.block = pinnedTake(resultLayout(r.columns, r.rows, doff, noff));
+    std::vector<size_t> pd, pn;
+    size_t at = 0;
+    for (auto &p : parts) {
+      resultLayout(r.columns, p.rows, pd, pn);
+      const char *src = static_cast<const char *>(p.blk.ptr);
+      char *dst = static_cast<char *>(r.block.ptr);
+      for (size_t c = 0; c < nc; c++) {
+        const size_t eb = r.columns[c]->bytes;
+        std::memcpy(dst + doff[c] + at * eb, src + pd[c], p.rows * eb);
+        std::memcpy(dst + noff[c] + at, src + pn[c], p.rows);
+      }
+      at += p.rows;
+      pinnedGive(p.blk);
+    }
+  }
+  resultLayout(r.columns, r.rows, doff, noff);
+  for (size_t c = 0; c < nc; c++) {
+    r.data[c] = static_cast<uint8_t *>(r.block.ptr) + doff[c];
+    r.nulls[c] = static_cast<uint8_t *>(r.block.ptr) + noff[c];
   }
   return r;
 }

@@ -57,6 +57,22 @@ using ColumnPtr = std::shared_ptr<Column>;
 
 // Device allocation on a stream (stream-ordered allocator), freed on the
 // same stream when the last owner drops it.
+// Device memory of the engine comes from one stream-ordered pool per device
+// that keeps freed memory for reuse (a plain hipMallocAsync from the default
+// pool maps new memory after each release, tens of microseconds per call).
+// The pool is trimmed when the last ExecContext of the process is destroyed.
+hipMemPool_t enginePool();
+
+// Pinned host memory for query results, reused across queries: taking a
+// block reuses a free one of at least the size (and at most twice it) or
+// pins a new one; up to 512 MB of free blocks stay pinned.  Thread-safe.
+struct PinnedBlock {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+};
+PinnedBlock pinnedTake(size_t bytes);
+void pinnedGive(PinnedBlock b);
+
 struct DeviceBuffer {
   void *ptr = nullptr;
   size_t bytes = 0;
@@ -406,10 +422,22 @@ class CrossProductOperator : public Operator {
 };
 
 // Copies every batch the root produces to host memory.
+// A query's result in host memory: every column's data and null bytes
+// (1 = NULL, zeros for a column without nulls) in one pinned block.
 struct HostResult {
   std::vector<ColumnPtr> columns;
-  std::vector<std::vector<uint8_t>> data, nulls;
+  std::vector<uint8_t *> data, nulls;  // into block; null when rows == 0
   size_t rows = 0, batches = 0;
+  PinnedBlock block;
+  HostResult() = default;
+  HostResult(const HostResult &) = delete;
+  HostResult &operator=(const HostResult &) = delete;
+  HostResult(HostResult &&o) noexcept
+      : columns(std::move(o.columns)), data(std::move(o.data)), nulls(std::move(o.nulls)), rows(o.rows),
+        batches(o.batches), block(o.block) {
+    o.block = PinnedBlock{};
+  }
+  ~HostResult() { pinnedGive(block); }
 };
 HostResult executeToHost(Operator &root, ExecContext &ctx);
 

@@ -333,10 +333,10 @@ int ysql_result_dtype(const ysql_result *r, int c) {
   return (r && c >= 0 && (size_t)c < r->r.columns.size()) ? (int)r->r.columns[c]->type : -1;
 }
 const void *ysql_result_data(const ysql_result *r, int c) {
-  return (r && c >= 0 && (size_t)c < r->r.data.size()) ? r->r.data[c].data() : nullptr;
+  return (r && c >= 0 && (size_t)c < r->r.data.size()) ? r->r.data[c] : nullptr;
 }
 const uint8_t *ysql_result_nulls(const ysql_result *r, int c) {
-  return (r && c >= 0 && (size_t)c < r->r.nulls.size()) ? r->r.nulls[c].data() : nullptr;
+  return (r && c >= 0 && (size_t)c < r->r.nulls.size()) ? r->r.nulls[c] : nullptr;
 }
 void ysql_result_destroy(ysql_result *r) { delete r; }
 

@@ -172,41 +172,51 @@ class Plan:
         return rows.value
 
     def execute(self):
-        """Runs the plan to exhaustion; returns (columns, nulls, batches) as numpy arrays."""
+        """Runs the plan to exhaustion; returns (columns, nulls, batches) as
+        numpy arrays.  The arrays are views of the result's pinned host
+        memory (no copy); it is released when the last of them is."""
         out = P()
         rc = lib.ysql_plan_execute(self.h, C.byref(out))
         if rc != 0:
             raise EngineError(f"execute returned {rc}: {lib.ysql_engine_last_error().decode()}")
-        try:
-            n = lib.ysql_result_rows(out)
-            cols, nulls = [], []
-            for c in range(lib.ysql_result_ncols(out)):
-                dt = lib.ysql_result_dtype(out, c)
-                if dt == L.STRING:
-                    nb = n * 256
-                    arr = np.ctypeslib.as_array((C.c_uint8 * max(nb, 1)).from_address(lib.ysql_result_data(out, c) or 0)) if n else np.zeros(0, np.uint8)
-                    cols.append(np.array(arr[:nb]).reshape(n, 256))
-                else:
-                    npdt = np.dtype(NP_DT[dt])
-                    if n:
-                        buf = (C.c_uint8 * (n * npdt.itemsize)).from_address(lib.ysql_result_data(out, c))
-                        cols.append(np.frombuffer(bytes(buf), dtype=npdt).copy())
-                    else:
-                        cols.append(np.zeros(0, npdt))
-                if n:
-                    nbuf = (C.c_uint8 * n).from_address(lib.ysql_result_nulls(out, c))
-                    nulls.append(np.frombuffer(bytes(nbuf), dtype=np.uint8).copy())
-                else:
-                    nulls.append(np.zeros(0, np.uint8))
-            batches = lib.ysql_result_batches(out)
-        finally:
-            lib.ysql_result_destroy(out)
-        return cols, nulls, batches
+        holder = _Result(out)
+        n = lib.ysql_result_rows(out)
+        cols, nulls = [], []
+        for c in range(lib.ysql_result_ncols(out)):
+            dt = lib.ysql_result_dtype(out, c)
+            if dt == L.STRING:
+                cols.append(_host_view(holder, lib.ysql_result_data(out, c), n * 256, np.uint8).reshape(n, 256))
+            else:
+                cols.append(_host_view(holder, lib.ysql_result_data(out, c), n * np.dtype(NP_DT[dt]).itemsize,
+                                       NP_DT[dt]))
+            nulls.append(_host_view(holder, lib.ysql_result_nulls(out, c), n, np.uint8))
+        return cols, nulls, lib.ysql_result_batches(out)
 
     def __del__(self):
         if getattr(self, "h", None):
             lib.ysql_plan_destroy(self.h)
             self.h = None
+
+
+class _Result:
+    """Owns a ysql_result; the numpy views of its memory keep it alive."""
+
+    def __init__(self, h):
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.ysql_result_destroy(self.h)
+            self.h = None
+
+
+def _host_view(holder, ptr, nbytes, dtype):
+    """A numpy array over nbytes of result memory at ptr, holding `holder`."""
+    if not nbytes:
+        return np.zeros(0, dtype)
+    buf = (C.c_uint8 * nbytes).from_address(ptr)
+    buf._owner = holder
+    return np.frombuffer(buf, dtype=dtype)
 
 
 class Table:
