@@ -947,8 +947,17 @@ static int filter_agg2_launch(const void *pred, const void *scalars, int nq, con
 
 // Common shapes compiled whole (agg_dev.hpp, filter_agg3_tile): the spec
 // codes (fa2_code: opcode * 2 + slot) are template arguments.
+// Waves per SIMD the compiled shapes are built for: the register budget at
+// which each compiles without spilling (measured per shape; one more wave
+// spills the four-query forms).
+template <typename PT, int QMAX>
+struct Fa3Waves {
+  static constexpr int value = QMAX == 1 ? 7 : (sizeof(PT) == 4 ? 5 : 4);
+};
+
 template <typename PT, int W1, int QMAX, int C0, int C1, int C2, int C3>
-__global__ __launch_bounds__(kFaBlock) void k_filter_agg3(const PT *pred, const PT *scalars, int nq, FaArgs a,
+__global__ __launch_bounds__(kFaBlock) __attribute__((amdgpu_waves_per_eu(Fa3Waves<PT, QMAX>::value)))
+void k_filter_agg3(const PT *pred, const PT *scalars, int nq, FaArgs a,
                                                           Fa2Args b, int ns, uint64_t n, uint64_t tile_rows,
                                                           ysql_acc *partials, uint32_t ntiles) {
   __shared__ ysql_acc s_w[kFaBlock / kWave];
