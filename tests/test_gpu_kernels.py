@@ -753,9 +753,12 @@ def test_pairs_intersect(ops, dev):
     assert np.array_equal(N(got).view(np.uint32), exp)
 
 
-@pytest.mark.parametrize("n,off", [(2_000_000, 0), (1_000_003, 1)])
-def test_star_join_group_sum(ops, dev, n, off):
-    """off = 1: every fact column is an unaligned device view with a ragged tail."""
+@pytest.mark.parametrize("n,off,filters", [(2_000_000, 0, True), (1_000_003, 1, True), (1_000_003, 1, False)])
+def test_star_join_group_sum(ops, dev, n, off, filters):
+    """off = 1: every fact column is an unaligned device view with a ragged tail.  With filters the
+    unfiltered dimensions' keys and the measures are read only for rows that pass them; some of those
+    keys are out of their dimension's range (the row joins nothing).  Without filters every column is
+    read up front."""
     r = rng(14)
     dims_np = []
     rows = [2556, 2000, 30000, 200000]
@@ -764,12 +767,17 @@ def test_star_join_group_sum(ops, dev, n, off):
     region = r.integers(0, 7, 30000).astype(np.int32)
     cat = r.integers(0, 25, 200000).astype(np.int32)
     brand = r.integers(0, 1000, 200000).astype(np.int32)
+    bad = r.random(n) < 0.01
+    fks[1][bad] = np.where(r.random(int(bad.sum())) < 0.5, -1, 2000 + 7).astype(np.int32)
     dims_np = [
         {"fk": fks[0], "rows": 2556, "group": year, "group_base": 1992, "group_card": 7},
         {"fk": fks[1], "rows": 2000},
         {"fk": fks[2], "rows": 30000, "pass": (region == 0).astype(np.uint8)},
         {"fk": fks[3], "rows": 200000, "pass": (cat == 0).astype(np.uint8), "group": brand, "group_card": 1000},
     ]
+    if not filters:
+        for d in dims_np:
+            d.pop("pass", None)
     m1 = r.integers(90000, 10000001, n).astype(np.int32)
     m2 = r.integers(0, 11, n).astype(np.int32)
     def fact(v):  # device column, shifted by `off` elements when requested

@@ -500,6 +500,18 @@ constexpr uint32_t kStarLdsMax = 8192;
 
 // 4 consecutive int32 values from row r (rows >= n read 0): one 128-bit
 // load when whole and aligned.
+__device__ __forceinline__ void star_load4p(const int32_t *p, uint64_t r, uint64_t n, int32_t *v) {
+  if (r + 4 <= n && ((uintptr_t)(p + r) & 15) == 0) {
+    const int4 q = *reinterpret_cast<const int4 *>(p + r);
+    v[0] = q.x;
+    v[1] = q.y;
+    v[2] = q.z;
+    v[3] = q.w;
+  } else {
+#pragma unroll
+    for (int e = 0; e < 4; e++) v[e] = r + e < n ? p[r + e] : 0;
+  }
+}
 __device__ __forceinline__ void star_load4(const int32_t *p, uint64_t r, uint64_t n, int32_t (&v)[4]) {
   if (r + 4 <= n && ((uintptr_t)(p + r) & 15) == 0) {
     const int4 q = *reinterpret_cast<const int4 *>(p + r);
@@ -513,16 +525,32 @@ __device__ __forceinline__ void star_load4(const int32_t *p, uint64_t r, uint64_
   }
 }
 
-// Each thread takes 4 consecutive fact rows per step: every fact column is
-// read with one 128-bit load; then every dimension's filter byte is looked
-// up (the 4 rows' gathers of a dimension together, only for rows still
-// qualifying), and the group values only for rows that pass every filter.
-// Qualifying rows fold into per-block LDS group arrays (LDS) or straight
-// into the global ones.
-template <bool LDS>
+// The values of rows r..r+3 whose flag is set (0 elsewhere): a load per
+// such row only, so a column is read only in the lines holding such rows.
+__device__ __forceinline__ void star_load_sel(const int32_t *p, uint64_t r, const bool *sel, int32_t *v) {
+#pragma unroll
+  for (int e = 0; e < 4; e++) v[e] = sel[e] ? p[r + e] : 0;
+}
+
+// Each thread takes G groups of 4 consecutive fact rows per step (group j at
+// row r + j * S, S = 4 * threads in the grid, so each group's 128-bit loads
+// are contiguous across the grid), so that a step's chain of dependent
+// latencies (column loads, filter lookups, group lookups) covers 4G rows.
+// With LATE (some dimension has a filter), only the filtered dimensions'
+// foreign keys are read up front; their filter bytes are looked up (the
+// rows' gathers of a dimension together), and only the rows that pass every
+// filter read the other foreign keys and the two measures: at the join's
+// selectivity most lines of those columns are never fetched.  Without
+// filters every column is read up front.  Then the group values of the
+// qualifying rows are looked up, and the rows fold into per-block LDS group
+// arrays (LDS) or straight into the global ones.  The rows that qualify, and
+// so every sum and count, are the same either way.
+template <bool LDS, bool LATE>
 __global__ __launch_bounds__(1024) void k_star(StarArgs a, int nd, const int32_t *__restrict__ m1,
                                                const int32_t *__restrict__ m2, uint64_t n, int64_t *sums,
                                                uint64_t *counts, uint32_t ng) {
+  constexpr int G = 2;  // 8 rows: their keys, measures and group values fit the registers of a 1,024-thread block
+  constexpr int RPT = 4 * G;
   __shared__ unsigned long long s_sum[LDS ? kStarLdsMax : 1];
   __shared__ uint32_t s_cnt[LDS ? kStarLdsMax : 1];
   if (LDS) {
@@ -532,51 +560,75 @@ __global__ __launch_bounds__(1024) void k_star(StarArgs a, int nd, const int32_t
     }
     __syncthreads();
   }
-  const uint64_t step = (uint64_t)gridDim.x * blockDim.x * 4;
-  for (uint64_t r = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4; r < n; r += step) {
-    int32_t f[YSQL_MAX_DIMS][4], x1[4], x2[4];
+  const uint64_t S = (uint64_t)gridDim.x * blockDim.x * 4;
+  for (uint64_t r = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4; r < n; r += S * G) {
+    int32_t f[YSQL_MAX_DIMS][RPT], x1[RPT], x2[RPT];
+    bool ok[RPT];
+#pragma unroll
+    for (int e = 0; e < RPT; e++) ok[e] = r + (uint64_t)(e / 4) * S + (e % 4) < n;
 #pragma unroll
     for (int d = 0; d < YSQL_MAX_DIMS; d++)
-      if (d < nd) star_load4(a.d[d].fk, r, n, f[d]);
-    star_load4(m1, r, n, x1);
-    star_load4(m2, r, n, x2);
-    bool ok[4];
+      if (d < nd && (!LATE || a.d[d].pass))
 #pragma unroll
-    for (int e = 0; e < 4; e++) ok[e] = r + e < n;
-    // in-range checks, then all gathers
+        for (int jg = 0; jg < G; jg++) star_load4p(a.d[d].fk, r + jg * S, n, &f[d][4 * jg]);
+    if constexpr (!LATE) {
+#pragma unroll
+      for (int jg = 0; jg < G; jg++) {
+        star_load4p(m1, r + jg * S, n, &x1[4 * jg]);
+        star_load4p(m2, r + jg * S, n, &x2[4 * jg]);
+      }
+    }
+    // in-range checks of the keys read so far, then all gathers
 #pragma unroll
     for (int d = 0; d < YSQL_MAX_DIMS; d++)
-      if (d < nd) {
+      if (d < nd && (!LATE || a.d[d].pass)) {
 #pragma unroll
-        for (int e = 0; e < 4; e++) ok[e] = ok[e] && f[d][e] >= 0 && (uint32_t)f[d][e] < a.d[d].rows;
+        for (int e = 0; e < RPT; e++) ok[e] = ok[e] && f[d][e] >= 0 && (uint32_t)f[d][e] < a.d[d].rows;
       }
     // every dimension's filter first, then the group lookups of the rows
-    // that pass them all; the 4 rows' gathers of one dimension go together
+    // that pass them all; the rows' gathers of one dimension go together
 #pragma unroll
     for (int d = 0; d < YSQL_MAX_DIMS; d++) {
       if (d >= nd || !a.d[d].pass) continue;
-      uint8_t pv[4];
+      uint8_t pv[RPT];
 #pragma unroll
-      for (int e = 0; e < 4; e++) pv[e] = ok[e] ? a.d[d].pass[f[d][e]] : (uint8_t)0;
+      for (int e = 0; e < RPT; e++) pv[e] = ok[e] ? a.d[d].pass[f[d][e]] : (uint8_t)0;
 #pragma unroll
-      for (int e = 0; e < 4; e++) ok[e] = ok[e] && pv[e] != 0;
+      for (int e = 0; e < RPT; e++) ok[e] = ok[e] && pv[e] != 0;
     }
-    uint32_t g[4] = {0, 0, 0, 0};
+    if constexpr (LATE) {  // the rest of the row, for the rows still qualifying
+#pragma unroll
+      for (int d = 0; d < YSQL_MAX_DIMS; d++)
+        if (d < nd && !a.d[d].pass) {
+#pragma unroll
+          for (int jg = 0; jg < G; jg++) star_load_sel(a.d[d].fk, r + jg * S, &ok[4 * jg], &f[d][4 * jg]);
+#pragma unroll
+          for (int e = 0; e < RPT; e++) ok[e] = ok[e] && f[d][e] >= 0 && (uint32_t)f[d][e] < a.d[d].rows;
+        }
+#pragma unroll
+      for (int jg = 0; jg < G; jg++) {
+        star_load_sel(m1, r + jg * S, &ok[4 * jg], &x1[4 * jg]);
+        star_load_sel(m2, r + jg * S, &ok[4 * jg], &x2[4 * jg]);
+      }
+    }
+    uint32_t g[RPT];
+#pragma unroll
+    for (int e = 0; e < RPT; e++) g[e] = 0;
 #pragma unroll
     for (int d = 0; d < YSQL_MAX_DIMS; d++) {
       if (d >= nd || !a.d[d].group) continue;
-      int32_t gv[4];
+      int32_t gv[RPT];
 #pragma unroll
-      for (int e = 0; e < 4; e++) gv[e] = ok[e] ? a.d[d].group[f[d][e]] : a.d[d].group_base;
+      for (int e = 0; e < RPT; e++) gv[e] = ok[e] ? a.d[d].group[f[d][e]] : a.d[d].group_base;
 #pragma unroll
-      for (int e = 0; e < 4; e++) {
+      for (int e = 0; e < RPT; e++) {
         const int64_t v = (int64_t)gv[e] - a.d[d].group_base;
         ok[e] = ok[e] && v >= 0 && v < (int64_t)a.d[d].group_card;
         g[e] += (uint32_t)v * a.stride[d];
       }
     }
 #pragma unroll
-    for (int e = 0; e < 4; e++) {
+    for (int e = 0; e < RPT; e++) {
       const bool q = ok[e];
       if (!q) continue;
       const long long prod = (long long)x1[e] * (long long)x2[e];
@@ -826,6 +878,14 @@ static int star_args(const ysql_star_dim *dims, int ndims, uint32_t n_groups, St
   return stride > n_groups ? YSQL_EINVAL : YSQL_OK;
 }
 
+// Whether some dimension filters the rows (then the other columns are read
+// late, for the rows that pass).
+static bool star_has_filter(const ysql_star_dim *dims, int ndims) {
+  for (int d = 0; d < ndims; d++)
+    if (dims[d].pass) return true;
+  return false;
+}
+
 int ysql_star_join_group_sum_multi(const ysql_star_dim *dims, int ndims, int nq, const int32_t *m1, const int32_t *m2,
                                    uint64_t n, int64_t *sums, uint64_t *counts, uint32_t n_groups,
                                    ysql_stream_t stream) {
@@ -856,10 +916,14 @@ int ysql_star_join_group_sum(const ysql_star_dim *dims, int ndims, const int32_t
   const uint64_t cap = (uint64_t)grid_cap() * (n_groups <= kStarLdsMax ? 1 : 2);
   if (g > cap) g = cap;
   if (g < 1) g = 1;
-  if (n_groups <= kStarLdsMax)
-    k_star<true><<<(unsigned)g, 1024, 0, s>>>(a, ndims, m1, m2, n, sums, counts, n_groups);
-  else
-    k_star<false><<<(unsigned)g, 1024, 0, s>>>(a, ndims, m1, m2, n, sums, counts, n_groups);
+  const bool late = star_has_filter(dims, ndims);
+  if (n_groups <= kStarLdsMax) {
+    if (late) k_star<true, true><<<(unsigned)g, 1024, 0, s>>>(a, ndims, m1, m2, n, sums, counts, n_groups);
+    else k_star<true, false><<<(unsigned)g, 1024, 0, s>>>(a, ndims, m1, m2, n, sums, counts, n_groups);
+  } else {
+    if (late) k_star<false, true><<<(unsigned)g, 1024, 0, s>>>(a, ndims, m1, m2, n, sums, counts, n_groups);
+    else k_star<false, false><<<(unsigned)g, 1024, 0, s>>>(a, ndims, m1, m2, n, sums, counts, n_groups);
+  }
   YSQL_LAUNCH_CHECK();
   return YSQL_OK;
 }
