@@ -202,13 +202,18 @@ __global__ __launch_bounds__(kScanBlock) void k_compact(const uint8_t *mask, con
 // load instruction of a wave reads one contiguous range), its BOOL result
 // goes to LDS as one byte per row, and each thread then takes the flags of
 // its 16 consecutive rows for the stable compaction of compact_scatter.
+// stage_in >= 0 (32-bit programs): that input column is also an output
+// column (stage_src); its values, loaded for the program anyway, are kept in
+// the LDS stage and copied out from there instead of read again after the
+// tile's look-back.
 // ---------------------------------------------------------------------------
 template <typename S>
 __global__ __launch_bounds__(kScanBlock) void k_xfilter(XProg p, ColArgs cols, int ncols, uint64_t n,
                                                         uint64_t *count_out, uint32_t *ctr, uint64_t *status,
-                                                        uint32_t ntiles) {
+                                                        uint32_t ntiles, int stage_in, const void *stage_src) {
   constexpr int R = sizeof(S) == 8 ? 1 : 4;  // rows per thread per step (64-bit slots: the file is twice as wide)
-  __shared__ CompactSmem<kScanBlock, 0> sm;
+  constexpr int VB = sizeof(S) == 4 ? 4 : 0;
+  __shared__ CompactSmem<kScanBlock, VB> sm;
   __shared__ uint32_t s_sel[kScanTile / 4];
   __shared__ uint32_t s_tile;
   const uint32_t tile = take_tile(ctr, &s_tile);
@@ -224,6 +229,14 @@ __global__ __launch_bounds__(kScanBlock) void k_xfilter(XProg p, ColArgs cols, i
     const uint32_t r0 = j * (kScanBlock * R) + t * R;
     const bool fast = p.aligned && base + (uint64_t)(j + 1) * (kScanBlock * R) <= n;  // wave-uniform
     xload_inputs<S, R, false>(p, base + r0, n, fast, reg);
+    if constexpr (VB == 4) {
+      if (stage_in >= 0) {  // wave-uniform
+        S sv[R];
+        xget(reg, stage_in, sv);
+#pragma unroll
+        for (int e = 0; e < R; e++) sm.pv[r0 + e] = (uint32_t)sv[e];
+      }
+    }
     xrun<S, R, false>(p, reg, acc, nacc);
     S pr[R];
     if (p.pred == kXAcc) {
@@ -246,8 +259,8 @@ __global__ __launch_bounds__(kScanBlock) void k_xfilter(XProg p, ColArgs cols, i
   uint32_t flags = 0;
 #pragma unroll
   for (int e = 0; e < 16; e++) flags |= (uint32_t)(((w[e >> 2] >> (8 * (e & 3))) & 0xFF) != 0) << e;
-  compact_scatter<kScanBlock, 0, uint32_t>(sm, tile, flags, nullptr, cols.c, ncols, n, ~0ull, 0, count_out, status,
-                                           ntiles);
+  compact_scatter<kScanBlock, VB, uint32_t>(sm, tile, flags, stage_in >= 0 ? (const uint32_t *)stage_src : nullptr,
+                                            cols.c, ncols, n, ~0ull, 0, count_out, status, ntiles);
 }
 
 static bool cols_ok(const ysql_col *cols, int ncols, bool need_ptrs) {
@@ -474,10 +487,20 @@ int ysql_expr_filter_compact(const ysql_xinsn *prog, int ninsn, int pred_reg, co
   ColArgs ca;
   memset(&ca, 0, sizeof(ca));
   for (int i = 0; i < ncols; i++) ca.c[i] = cols[i];
+  // a 4-byte input column that is also an output column: staged in LDS
+  int stage_in = -1;
+  for (int c = 0; c < p.nin && stage_in < 0 && !wide; c++) {
+    if (p.in[c].dtype != YSQL_INT && p.in[c].dtype != YSQL_FLOAT) continue;
+    for (int i = 0; i < ncols; i++)
+      if (cols[i].src == p.in[c].data && cols[i].elem_bytes == 4) stage_in = c;
+  }
+  const void *stage_src = stage_in >= 0 ? p.in[stage_in].data : nullptr;
   if (wide)
-    k_xfilter<uint64_t><<<ntiles, kScanBlock, 0, s>>>(p, ca, ncols, n, count_out, w.ctr, w.status, ntiles);
+    k_xfilter<uint64_t><<<ntiles, kScanBlock, 0, s>>>(p, ca, ncols, n, count_out, w.ctr, w.status, ntiles, -1,
+                                                      nullptr);
   else
-    k_xfilter<uint32_t><<<ntiles, kScanBlock, 0, s>>>(p, ca, ncols, n, count_out, w.ctr, w.status, ntiles);
+    k_xfilter<uint32_t><<<ntiles, kScanBlock, 0, s>>>(p, ca, ncols, n, count_out, w.ctr, w.status, ntiles, stage_in,
+                                                      stage_src);
   YSQL_LAUNCH_CHECK();
   return YSQL_OK;
 }
