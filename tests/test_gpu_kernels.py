@@ -793,3 +793,51 @@ def test_star_join_group_sum(ops, dev, n, off, filters):
     es, ec = O.star_join_group_sum(dims_np, m1, m2, 7000)
     assert np.array_equal(N(sums), es)
     assert np.array_equal(N(counts).astype(np.uint64), ec)
+
+
+@pytest.mark.parametrize("n,off", [(2_000_000, 0), (1_000_003, 1)])
+def test_star_join_group_sum_multi(ops, dev, n, off):
+    """Three star-join queries in one read of the facts (bit q of each pass byte is query q's filter):
+    each query's sums and counts equal the oracle's for that query's filters alone.  off = 1: unaligned
+    fact columns with a ragged tail; 1% of the supplier keys are out of range."""
+    r = rng(15)
+    rows = [2556, 2000, 30000, 200000]
+    fks = [r.integers(0, R, n).astype(np.int32) for R in rows]
+    year = (1992 + (7 * np.arange(2556)) // 2556).astype(np.int32)
+    region = r.integers(0, 7, 30000).astype(np.int32)
+    cat = r.integers(0, 25, 200000).astype(np.int32)
+    brand = r.integers(0, 1000, 200000).astype(np.int32)
+    bad = r.random(n) < 0.01
+    fks[1][bad] = np.where(r.random(int(bad.sum())) < 0.5, -1, 2000 + 7).astype(np.int32)
+    m1 = r.integers(90000, 10000001, n).astype(np.int32)
+    m2 = r.integers(0, 11, n).astype(np.int32)
+    params = [(0, 0), (3, 7), (6, 24)]
+    rp = np.zeros(30000, np.uint8)
+    cp = np.zeros(200000, np.uint8)
+    for q, (rv, cv) in enumerate(params):
+        rp |= ((region == rv).astype(np.uint8) << q).astype(np.uint8)
+        cp |= ((cat == cv).astype(np.uint8) << q).astype(np.uint8)
+
+    def fact(v):
+        if not off:
+            return T(v, dev)
+        return T(np.concatenate([np.zeros(off, v.dtype), v]), dev)[off:]
+
+    dims_t = [
+        {"fk": fact(fks[0]), "rows": 2556, "group": T(year, dev), "group_base": 1992, "group_card": 7},
+        {"fk": fact(fks[1]), "rows": 2000},
+        {"fk": fact(fks[2]), "rows": 30000, "pass": T(rp, dev)},
+        {"fk": fact(fks[3]), "rows": 200000, "pass": T(cp, dev), "group": T(brand, dev), "group_card": 1000},
+    ]
+    sums, counts = ops.star_join_group_sum_multi(dims_t, fact(m1), fact(m2), 7000, len(params))
+    for q, (rv, cv) in enumerate(params):
+        dims_np = [
+            {"fk": fks[0], "rows": 2556, "group": year, "group_base": 1992, "group_card": 7},
+            {"fk": fks[1], "rows": 2000},
+            {"fk": fks[2], "rows": 30000, "pass": (region == rv).astype(np.uint8)},
+            {"fk": fks[3], "rows": 200000, "pass": (cat == cv).astype(np.uint8), "group": brand,
+             "group_card": 1000},
+        ]
+        es, ec = O.star_join_group_sum(dims_np, m1, m2, 7000)
+        assert np.array_equal(N(sums[q]), es), q
+        assert np.array_equal(N(counts[q]).astype(np.uint64), ec), q

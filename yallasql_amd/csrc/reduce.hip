@@ -512,18 +512,6 @@ __device__ __forceinline__ void star_load4p(const int32_t *p, uint64_t r, uint64
     for (int e = 0; e < 4; e++) v[e] = r + e < n ? p[r + e] : 0;
   }
 }
-__device__ __forceinline__ void star_load4(const int32_t *p, uint64_t r, uint64_t n, int32_t (&v)[4]) {
-  if (r + 4 <= n && ((uintptr_t)(p + r) & 15) == 0) {
-    const int4 q = *reinterpret_cast<const int4 *>(p + r);
-    v[0] = q.x;
-    v[1] = q.y;
-    v[2] = q.z;
-    v[3] = q.w;
-  } else {
-#pragma unroll
-    for (int e = 0; e < 4; e++) v[e] = r + e < n ? p[r + e] : 0;
-  }
-}
 
 // The values of rows r..r+3 whose flag is set (0 elsewhere): a load per
 // such row only, so a column is read only in the lines holding such rows.
@@ -661,49 +649,58 @@ __global__ __launch_bounds__(1024) void k_star(StarArgs a, int nd, const int32_t
 __global__ __launch_bounds__(1024) void k_star_multi(StarArgs a, int nd, uint32_t qmask,
                                                      const int32_t *__restrict__ m1, const int32_t *__restrict__ m2,
                                                      uint64_t n, int64_t *sums, uint64_t *counts, uint32_t ng) {
-  const uint64_t step = (uint64_t)gridDim.x * blockDim.x * 4;
-  for (uint64_t r = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4; r < n; r += step) {
-    int32_t f[YSQL_MAX_DIMS][4], x1[4], x2[4];
+  constexpr int G = 2;  // groups of 4 rows per thread and step, as in k_star
+  constexpr int RPT = 4 * G;
+  const uint64_t S = (uint64_t)gridDim.x * blockDim.x * 4;
+  for (uint64_t r = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4; r < n; r += S * G) {
+    int32_t f[YSQL_MAX_DIMS][RPT], x1[RPT], x2[RPT];
 #pragma unroll
     for (int d = 0; d < YSQL_MAX_DIMS; d++)
-      if (d < nd) star_load4(a.d[d].fk, r, n, f[d]);
-    star_load4(m1, r, n, x1);
-    star_load4(m2, r, n, x2);
-    uint32_t mk[4];
+      if (d < nd)
 #pragma unroll
-    for (int e = 0; e < 4; e++) mk[e] = r + e < n ? qmask : 0u;
+        for (int jg = 0; jg < G; jg++) star_load4p(a.d[d].fk, r + jg * S, n, &f[d][4 * jg]);
+#pragma unroll
+    for (int jg = 0; jg < G; jg++) {
+      star_load4p(m1, r + jg * S, n, &x1[4 * jg]);
+      star_load4p(m2, r + jg * S, n, &x2[4 * jg]);
+    }
+    uint32_t mk[RPT];
+#pragma unroll
+    for (int e = 0; e < RPT; e++) mk[e] = r + (uint64_t)(e / 4) * S + (e % 4) < n ? qmask : 0u;
 #pragma unroll
     for (int d = 0; d < YSQL_MAX_DIMS; d++)
       if (d < nd) {
 #pragma unroll
-        for (int e = 0; e < 4; e++)
+        for (int e = 0; e < RPT; e++)
           if (!(f[d][e] >= 0 && (uint32_t)f[d][e] < a.d[d].rows)) mk[e] = 0;
       }
 #pragma unroll
     for (int d = 0; d < YSQL_MAX_DIMS; d++) {
       if (d >= nd || !a.d[d].pass) continue;
-      uint32_t pv[4];
+      uint32_t pv[RPT];
 #pragma unroll
-      for (int e = 0; e < 4; e++) pv[e] = mk[e] ? (uint32_t)a.d[d].pass[f[d][e]] : 0u;
+      for (int e = 0; e < RPT; e++) pv[e] = mk[e] ? (uint32_t)a.d[d].pass[f[d][e]] : 0u;
 #pragma unroll
-      for (int e = 0; e < 4; e++) mk[e] &= pv[e];
+      for (int e = 0; e < RPT; e++) mk[e] &= pv[e];
     }
-    uint32_t g[4] = {0, 0, 0, 0};
+    uint32_t g[RPT];
+#pragma unroll
+    for (int e = 0; e < RPT; e++) g[e] = 0;
 #pragma unroll
     for (int d = 0; d < YSQL_MAX_DIMS; d++) {
       if (d >= nd || !a.d[d].group) continue;
-      int32_t gv[4];
+      int32_t gv[RPT];
 #pragma unroll
-      for (int e = 0; e < 4; e++) gv[e] = mk[e] ? a.d[d].group[f[d][e]] : a.d[d].group_base;
+      for (int e = 0; e < RPT; e++) gv[e] = mk[e] ? a.d[d].group[f[d][e]] : a.d[d].group_base;
 #pragma unroll
-      for (int e = 0; e < 4; e++) {
+      for (int e = 0; e < RPT; e++) {
         const int64_t v = (int64_t)gv[e] - a.d[d].group_base;
         if (!(v >= 0 && v < (int64_t)a.d[d].group_card)) mk[e] = 0;
         g[e] += (uint32_t)v * a.stride[d];
       }
     }
 #pragma unroll
-    for (int e = 0; e < 4; e++) {
+    for (int e = 0; e < RPT; e++) {
       uint32_t m = mk[e];
       if (!m) continue;
       const long long prod = (long long)x1[e] * (long long)x2[e];
