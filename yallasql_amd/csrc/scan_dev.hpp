@@ -45,6 +45,29 @@ __device__ __forceinline__ void load16(const PT *p, uint64_t base, uint64_t rows
   }
 }
 
+// Copies the listed rows of one column: dst[prefix + r] = src[base + idx[r]]
+// for r < tot.  Each thread loads its next 8 rows (r = t, t + BLOCK, ...)
+// before it stores any, so 8 loads per thread are in flight at once instead
+// of one load's latency per row.
+template <int BLOCK, typename E>
+__device__ __forceinline__ void gather_listed(const E *src, E *dst, const uint16_t *idx, uint64_t base,
+                                              uint64_t prefix, uint32_t tot) {
+  constexpr int U = 8;
+  for (uint32_t r0 = threadIdx.x; r0 < tot; r0 += BLOCK * U) {
+    E v[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const uint32_t r = r0 + u * BLOCK;
+      v[u] = r < tot ? src[base + idx[r]] : E();
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const uint32_t r = r0 + u * BLOCK;
+      if (r < tot) dst[prefix + r] = v[u];
+    }
+  }
+}
+
 // Second half of a compaction tile: given each thread's selection flags for
 // its 16 consecutive rows (bit e = row t*16 + e of the tile), scans the
 // counts, finds the tile's prefix (look-back over `status`, or `running`
@@ -103,27 +126,19 @@ __device__ __forceinline__ uint64_t compact_scatter(CompactSmem<BLOCK, VB> &sm, 
     }
     switch (col.elem_bytes) {
       case 4: {
-        const uint32_t *src = (const uint32_t *)col.src;
-        uint32_t *dst = (uint32_t *)col.dst;
-        for (uint32_t r = t; r < tot; r += BLOCK) dst[prefix + r] = src[base + sm.src[r]];
+        gather_listed<BLOCK, uint32_t>((const uint32_t *)col.src, (uint32_t *)col.dst, sm.src, base, prefix, tot);
         break;
       }
       case 8: {
-        const uint64_t *src = (const uint64_t *)col.src;
-        uint64_t *dst = (uint64_t *)col.dst;
-        for (uint32_t r = t; r < tot; r += BLOCK) dst[prefix + r] = src[base + sm.src[r]];
+        gather_listed<BLOCK, uint64_t>((const uint64_t *)col.src, (uint64_t *)col.dst, sm.src, base, prefix, tot);
         break;
       }
       case 1: {
-        const uint8_t *src = (const uint8_t *)col.src;
-        uint8_t *dst = (uint8_t *)col.dst;
-        for (uint32_t r = t; r < tot; r += BLOCK) dst[prefix + r] = src[base + sm.src[r]];
+        gather_listed<BLOCK, uint8_t>((const uint8_t *)col.src, (uint8_t *)col.dst, sm.src, base, prefix, tot);
         break;
       }
       case 2: {
-        const uint16_t *src = (const uint16_t *)col.src;
-        uint16_t *dst = (uint16_t *)col.dst;
-        for (uint32_t r = t; r < tot; r += BLOCK) dst[prefix + r] = src[base + sm.src[r]];
+        gather_listed<BLOCK, uint16_t>((const uint16_t *)col.src, (uint16_t *)col.dst, sm.src, base, prefix, tot);
         break;
       }
       default: {  // 256-byte strings: 16 chunks of 16 bytes per row
@@ -137,8 +152,11 @@ __device__ __forceinline__ uint64_t compact_scatter(CompactSmem<BLOCK, VB> &sm, 
       }
     }
     if (col.dst_null) {
-      for (uint32_t r = t; r < tot; r += BLOCK)
-        col.dst_null[prefix + r] = col.src_null ? col.src_null[base + sm.src[r]] : 0;
+      if (col.src_null) {
+        gather_listed<BLOCK, uint8_t>(col.src_null, col.dst_null, sm.src, base, prefix, tot);
+      } else {
+        for (uint32_t r = t; r < tot; r += BLOCK) col.dst_null[prefix + r] = 0;
+      }
     }
   }
   if (tile == ntiles - 1 && t == 0 && count_out) *count_out = prefix + total;
