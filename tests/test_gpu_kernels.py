@@ -737,6 +737,38 @@ def test_join_eq_many_steps(ops, dev, R_):
     assert (N(out)[cap:] == -1).all()  # nothing written past the capacity
 
 
+@pytest.mark.parametrize("R_", [2_000, 20_000])
+def test_join_eq_dense_hits_int32(ops, dev, R_):
+    """EQ join where almost every left row matches many right rows, with
+    multiplicities that differ from row to row (1 to ~60 right rows per key,
+    one key with none), int32 keys: every lane of a wave writes several
+    pairs, so the write pass's lane order and output offsets are exercised
+    on every wave.  R_ = 2,000: table in LDS; 20,000: table and blocked
+    filter in global memory.  Expected pairs from a stable sort of the right
+    keys and a binary search per left key, in (l, r) order by construction."""
+    r = rng(R_ + 21)
+    L_ = 100_003 if R_ == 2_000 else 20_003  # about 3M and 6M pairs
+    keys = np.arange(64, dtype=np.int32) * 7919 - 250_000
+    w = np.arange(1, 65, dtype=np.float64)
+    right = r.choice(keys[1:], R_, p=w[1:] / w[1:].sum()).astype(np.int32)  # keys[0] has no right row
+    left = r.choice(keys, L_ + 1).astype(np.int32)
+    left[::101] = 12345  # some rows match nothing
+    order = np.argsort(right, kind="stable")
+    rs = right[order]
+    lk = left[1:]
+    lo = np.searchsorted(rs, lk, "left")
+    cnt = np.searchsorted(rs, lk, "right") - lo
+    total = int(cnt.sum())
+    first = np.cumsum(cnt) - cnt
+    exp = np.empty((total, 2), np.uint32)
+    exp[:, 0] = np.repeat(np.arange(L_, dtype=np.uint32), cnt)
+    exp[:, 1] = order[np.repeat(lo - first, cnt) + np.arange(total)]
+    for lt in (T(left, dev)[1:], T(lk, dev)):  # unaligned view, aligned copy
+        pairs, c = ops.join_pairs(4, lt, T(right, dev))
+        assert c == total > 1_000_000
+        assert np.array_equal(N(pairs).view(np.uint32), exp)
+
+
 def test_join_overflow_reports_count(ops, dev):
     left = np.zeros(100, np.int32)
     right = np.zeros(50, np.int32)
